@@ -1,0 +1,306 @@
+#!/usr/bin/env python3
+"""Occupancy-update benchmark on one MI355X.
+
+One step = one LiDAR scan through the drop-in OccupancyGrid: fused scan
+transform + ray cast + log-odds apply (update), then recenter on the robot
+base position — the per-tick pair the reference driver times (reference
+src/main.cpp:468-483). Inputs (sensor-frame SoA points) are resident in HBM
+before the timed region starts; each call blocks, as in the reference.
+
+Default workload: 701x701x151 voxels at 0.1 m, 65,536-point scans
+(64 beams x 1024 azimuths, 30 m), 400-box scene, 1 m/scan, 2 m recenter
+threshold. Prints ONE JSON line on stdout; details go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+README_3090_MS = 6.167  # update 1.344 + recenter 4.822 ms (reference README.md:35)
+METRIC = "ms/scan update+recenter @73.5M voxels; batched scans/s + p99 latency vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return 0.0
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def make_scans(w, count, scene):
+    scans = []
+    for k in range(count):
+        scans.append(scene.scan(k, w.beams, w.azimuths, w.el_lo, w.el_hi, w.max_range))
+    return scans
+
+
+def upload(cb, scans):
+    """All scans' SoA columns into three device buffers (HBM-resident)."""
+    n = scans[0].n
+    total = n * len(scans)
+    cols = [np.empty(total, np.float32) for _ in range(3)]
+    for i, s in enumerate(scans):
+        cols[0][i * n:(i + 1) * n] = s.xs
+        cols[1][i * n:(i + 1) * n] = s.ys
+        cols[2][i * n:(i + 1) * n] = s.zs
+    bufs = [cb.DeviceBuffer.from_array(c) for c in cols]
+    return bufs, n
+
+
+def new_grid(cb, w):
+    g = cb.OccupancyGrid(w.half, w.resolution, w.origin_at_center, w.threshold)
+    g.set_max_raycast_range(w.max_range)
+    g.set_log_odds_params(*cb.log_odds())
+    return g
+
+
+def run_gpu(cb, w, scans, bufs, n, warmup, steps):
+    g = new_grid(cb, w)
+    g.enable_kernel_timing(True)
+
+    def step(i):
+        off = 4 * n * i
+        s = scans[i]
+        g.update_sensor_frame_device(bufs[0].ptr + off, bufs[1].ptr + off,
+                                     bufs[2].ptr + off, n, s.T)
+        g.recenter(s.base)
+
+    for i in range(warmup):
+        step(i)
+    g.reset_kernel_times()
+    cb.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        a = time.perf_counter()
+        step(i)
+        per.append((time.perf_counter() - a) * 1e3)
+    cb.synchronize()
+    t1 = time.perf_counter()
+    kt = g.kernel_times()
+    return g, per, (t1 - t0) * 1e3, kt
+
+
+def census(cb, w, scans, bufs, n, warmup, steps):
+    """Replay of the same scans with a counter census after each ray cast
+    (untimed): voxel visits and unique voxels per scan."""
+    g = new_grid(cb, w)
+    out = []
+    for i in range(warmup + steps):
+        off = 4 * n * i
+        s = scans[i]
+        g.cast_rays_sensor_frame_device(bufs[0].ptr + off, bufs[1].ptr + off,
+                                        bufs[2].ptr + off, n, s.T)
+        c = g.counter_census()
+        g.apply_counters()
+        g.recenter(s.base)
+        if i >= warmup:
+            out.append(c)
+    g.close()
+    return out
+
+
+def run_pcie(cb, w, scans, warmup, steps):
+    """Same steps with host-resident points (H2D inside update)."""
+    g = new_grid(cb, w)
+    for i in range(warmup):
+        s = scans[i]
+        g.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+        g.recenter(s.base)
+    per = []
+    for i in range(warmup, warmup + steps):
+        s = scans[i]
+        a = time.perf_counter()
+        g.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+        g.recenter(s.base)
+        per.append((time.perf_counter() - a) * 1e3)
+    g.close()
+    return per
+
+
+def run_fetch(cb, w, g, scene, k, reps=5):
+    if not w.fetch_queries:
+        return None
+    qs = scene.queries(k, w.fetch_queries, w.fetch_spread)
+    r_vox = int(np.ceil(w.fetch_radius / w.resolution))
+    cap = (2 * r_vox + 1) ** 3
+    times = []
+    counts = None
+    for _ in range(reps):
+        a = time.perf_counter()
+        counts, _, _ = g.fetch_occupancy_batch(qs, w.fetch_radius, cb.OCCUPIED, cap)
+        times.append((time.perf_counter() - a) * 1e3)
+    return dict(queries=int(len(qs)), radius_m=w.fetch_radius, filter="occupied",
+                ms_per_batch=statistics.median(times),
+                samples=int(counts.sum()))
+
+
+def cpu_baseline(cb, w, scans, budget_s, threads):
+    """The oracle (CPU restatement, OpenMP) on the same scans."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    def timed(nthreads, max_scans, budget):
+        o = oracle.OracleGrid(w.half, w.resolution, w.origin_at_center, w.threshold,
+                              threads=nthreads)
+        o.set_max_raycast_range(w.max_range)
+        o.set_log_odds_params(*cb.log_odds())
+        per = []
+        start = time.perf_counter()
+        for s in scans[:max_scans]:
+            a = time.perf_counter()
+            o.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+            o.recenter(s.base)
+            per.append((time.perf_counter() - a) * 1e3)
+            if time.perf_counter() - start > budget:
+                break
+        return per
+
+    per = timed(threads, len(scans), budget_s)
+    per1 = timed(1, 3, budget_s / 3)
+    return dict(value=statistics.median(per), unit="ms/scan", cores=threads,
+                kind="port",
+                sample=f"first {len(per)} scans of the same sequence, update+recenter, "
+                       f"OpenMP {threads} threads (median)",
+                one_thread_ms=statistics.median(per1), one_thread_scans=len(per1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="readme")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip census, PCIe-inclusive and fetch passes")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(grids and sessions shard across GPUs with no exchange). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import cublox_amd as cb
+
+    if cb.device_count() < 1:
+        log("no HIP device visible")
+        sys.exit(1)
+    w = cb.WORKLOADS[args.workload]
+    log(f"device: {cb.device_name()}")
+    log(f"workload {w.name}: grid {w.map_size} ({w.voxels} voxels) @ {w.resolution} m, "
+        f"{w.points} points/scan")
+    total = args.warmup + args.steps
+    scene = w.scene()
+    t = time.perf_counter()
+    scans = make_scans(w, total, scene)
+    bufs, n = upload(cb, scans)
+    log(f"generated + uploaded {total} scans in {time.perf_counter() - t:.2f} s")
+
+    g, per, wall_ms, kt = run_gpu(cb, w, scans, bufs, n, args.warmup, args.steps)
+    ms_per_step = wall_ms / args.steps
+    log(f"plain per-scan: {ms_per_step:.4f} ms/scan (p50 {pct(per, 50):.4f}, "
+        f"p99 {pct(per, 99):.4f}) kernels {kt}")
+
+    extras = {}
+    roofline = None
+    rooflines = {}
+    if not args.no_extras:
+        cen = census(cb, w, scans, bufs, n, args.warmup, args.steps)
+        U = statistics.mean(c["touched"] for c in cen)
+        visits = statistics.mean(c["visits"] for c in cen)
+        Uh = statistics.mean(c["hit_voxels"] for c in cen)
+        V = w.voxels
+        rc_ms = kt["raycast_ms"] / max(kt["raycast_launches"], 1)
+        ap_ms = kt["apply_ms"] / max(kt["apply_launches"], 1)
+        re_n = max(kt["recenter_launches"], 1)
+        re_ms = kt["recenter_ms"] / re_n
+        re_vox = kt["recenter_zeroed_voxels"] / re_n
+        # Algorithmic bytes per launch.
+        rc_bytes = 12.0 * n + 8.0 * U + 8.0 * Uh
+        ap_bytes = 4.0 * V + 16.0 * U + 4.0 * Uh
+        re_bytes = 4.0 * re_vox
+
+        def rl(bytes_, ms):
+            gbs = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+            return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "traffic": None, "bytes_per_launch": int(bytes_),
+                    "avg_launch_ms": round(ms, 5)}
+
+        rooflines = {"ray_cast": rl(rc_bytes, rc_ms), "apply": rl(ap_bytes, ap_ms),
+                     "recenter": rl(re_bytes, re_ms)}
+        rooflines["ray_cast"]["visits_per_s"] = visits / (rc_ms * 1e-3) if rc_ms else 0
+        totals = {"ray_cast": kt["raycast_ms"], "apply": kt["apply_ms"],
+                  "recenter": kt["recenter_ms"]}
+        dom = max(totals, key=totals.get)
+        roofline = dict(rooflines[dom], kernel=dom)
+        extras["census"] = dict(visits_per_scan=visits, unique_voxels_per_scan=U,
+                                hit_voxels_per_scan=Uh,
+                                recenter_events=kt["recenter_launches"],
+                                zeroed_voxels_per_event=re_vox)
+        pcie = run_pcie(cb, w, scans, args.warmup, args.steps)
+        extras["pcie_inclusive_ms_per_scan"] = statistics.mean(pcie)
+        extras["fetch"] = run_fetch(cb, w, g, scene, total - 1)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(cb, w, scans, args.cpu_budget, threads)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 5),
+        "unit": "ms/scan",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(ms_per_step / README_3090_MS, 5),
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: {w.map_size[0]}x{w.map_size[1]}x{w.map_size[2]} "
+                        f"@ {w.resolution} m, {w.beams}x{w.azimuths} points/scan, "
+                        f"max range {w.max_range} m, {w.n_boxes} boxes, "
+                        f"{w.speed} m/scan, recenter threshold {w.threshold} m",
+            "voxels": w.voxels,
+            "points_per_scan": w.points,
+            "form": "plain per-scan, blocking drop-in calls, inputs HBM-resident",
+        },
+        "p50_ms": round(pct(per, 50), 5),
+        "p99_ms": round(pct(per, 99), 5),
+        "scans_per_s": round(1e3 / ms_per_step, 2),
+        "kernel_ms": {k: round(v / max(kt[k.replace('_ms', '_launches')], 1), 5)
+                      for k, v in kt.items() if k.endswith("_ms")},
+        "speedup_vs_3090_readme": round(README_3090_MS / ms_per_step, 2),
+        "roofline": roofline,
+        "rooflines": rooflines,
+        "cpu_baseline": cpu,
+    }
+    out.update(extras)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,261 @@
+"""ctypes bindings of the built libraries.
+
+libcublox_hip.so      HIP kernels + C ABI (include/cublox/capi.h)
+libcublox.so          drop-in C++ OccupancyGrid + flat C binding (og_capi.cpp)
+libcublox_scangen.so  synthetic LiDAR workload generator
+
+Nothing here falls back to a CPU path: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int)
+u32p = C.POINTER(C.c_uint)
+u8p = C.POINTER(C.c_ubyte)
+vp = C.c_void_p
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Int3(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int)]
+
+
+class RaycastCfg(C.Structure):
+    _fields_ = [
+        ("origin", Float3), ("origin_i", Int3), ("map_size_i", Int3),
+        ("half_map_size_i", Int3), ("resolution", C.c_float),
+        ("inv_resolution", C.c_float), ("max_range", C.c_float),
+        ("map_vox_num", C.c_int), ("origin_at_center", C.c_int),
+    ]
+
+
+class RecenterCfg(C.Structure):
+    _fields_ = [
+        ("pos", Float3), ("origin_f", Float3), ("origin_i", Int3),
+        ("map_size_i", Int3), ("half_map_size_i", Int3),
+        ("resolution", C.c_float), ("inv_resolution", C.c_float),
+        ("recenter_threshold_sq", C.c_float), ("origin_at_center", C.c_int),
+        ("map_vox_num", C.c_int),
+    ]
+
+
+class RecenterResult(C.Structure):
+    _fields_ = [
+        ("action", C.c_int), ("new_origin_i", Int3), ("n_slices", C.c_int * 3),
+        ("zeroed_voxels", C.c_longlong),
+    ]
+
+
+class FetchCfg(C.Structure):
+    _fields_ = [
+        ("center_g", Int3), ("origin_i", Int3), ("map_size_i", Int3),
+        ("half_map_size_i", Int3), ("gz_min", C.c_int), ("gz_max", C.c_int),
+        ("disk_radius_vox", C.c_int), ("center", Float3),
+        ("radius_sq", C.c_float), ("resolution", C.c_float),
+        ("l_free", C.c_float), ("l_occupied", C.c_float),
+        ("l_unknown", C.c_float), ("origin_at_center", C.c_int),
+        ("state_filter", C.c_int),
+    ]
+
+
+class SceneCfg(C.Structure):
+    _fields_ = [
+        ("n_boxes", C.c_int), ("box_extent", C.c_float),
+        ("box_min_side", C.c_float), ("box_max_side", C.c_float),
+        ("corridor_half", C.c_float), ("speed", C.c_float),
+        ("yaw_rate_deg", C.c_float), ("sensor_height", C.c_float),
+        ("noise_sigma", C.c_float), ("miss_extra", C.c_float),
+        ("seed", C.c_ulonglong),
+    ]
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is not built; run `python -c 'import __graft_entry__ as g; "
+            "g.build()'` (or `make -C cublox_amd/csrc`) first")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+def _sig(lib, name, restype, *argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+_hip = None
+_host = None
+_gen = None
+
+
+def hip() -> C.CDLL:
+    """The HIP kernel library with its C ABI signatures declared."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    L = _load("libcublox_hip.so")
+    S = C.c_size_t
+    i = C.c_int
+    _sig(L, "cublox_last_error", C.c_char_p)
+    _sig(L, "cublox_device_count", i, i32p)
+    _sig(L, "cublox_set_device", i, i)
+    _sig(L, "cublox_get_device", i, i32p)
+    _sig(L, "cublox_device_name", i, C.c_char_p, i)
+    _sig(L, "cublox_mem_info", i, C.POINTER(S), C.POINTER(S))
+    _sig(L, "cublox_malloc", i, C.POINTER(vp), S)
+    _sig(L, "cublox_free", i, vp)
+    _sig(L, "cublox_host_alloc", i, C.POINTER(vp), S)
+    _sig(L, "cublox_host_free", i, vp)
+    _sig(L, "cublox_memset_async", i, vp, i, S, vp)
+    _sig(L, "cublox_memcpy_h2d_async", i, vp, vp, S, vp)
+    _sig(L, "cublox_memcpy_d2h_async", i, vp, vp, S, vp)
+    _sig(L, "cublox_memcpy_d2d_async", i, vp, vp, S, vp)
+    _sig(L, "cublox_stream_create", i, C.POINTER(vp))
+    _sig(L, "cublox_stream_destroy", i, vp)
+    _sig(L, "cublox_stream_synchronize", i, vp)
+    _sig(L, "cublox_device_synchronize", i)
+    _sig(L, "cublox_event_create", i, C.POINTER(vp))
+    _sig(L, "cublox_event_destroy", i, vp)
+    _sig(L, "cublox_event_record", i, vp, vp)
+    _sig(L, "cublox_event_synchronize", i, vp)
+    _sig(L, "cublox_event_elapsed_ms", i, vp, vp, f32p)
+    _sig(L, "cublox_fill_zero_async", i, vp, S, vp)
+    _sig(L, "cublox_launch_ray_cast_update", i, C.POINTER(RaycastCfg), vp, vp,
+         vp, i, vp, vp, vp)
+    _sig(L, "cublox_launch_ray_cast_update_sensor", i, C.POINTER(RaycastCfg),
+         f32p, vp, vp, vp, i, vp, vp, vp)
+    _sig(L, "cublox_launch_apply_update", i, vp, vp, vp, i, C.c_float,
+         C.c_float, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp, C.c_uint)
+    _sig(L, "cublox_launch_clear_voxels_by_index", i, vp, vp, i, i, vp)
+    _sig(L, "cublox_plan_recenter", i, C.POINTER(RecenterCfg), i32p, i32p,
+         i32p, C.POINTER(RecenterResult))
+    _sig(L, "cublox_launch_recenter", i, vp, vp, vp, C.POINTER(RecenterCfg),
+         vp, C.POINTER(RecenterResult))
+    _sig(L, "cublox_launch_clear_recenter_slabs_for_axis", i, vp, vp, vp, Int3,
+         Int3, vp, i, i, vp)
+    _sig(L, "cublox_fetch_plan", i, Int3, Int3, Int3, C.c_float, C.c_float, i,
+         C.c_float, C.c_float, C.c_float, Float3, C.c_float, i, i,
+         C.POINTER(FetchCfg), C.POINTER(C.c_longlong), u32p)
+    _sig(L, "cublox_launch_fetch_occupancy_around", i, C.POINTER(FetchCfg), vp,
+         vp, vp, vp, C.c_uint, C.c_uint, vp)
+    _sig(L, "cublox_launch_fetch_occupancy_batch", i, vp, i, i, i, vp, vp, vp,
+         vp, C.c_uint, vp)
+    _sig(L, "cublox_counter_census", i, vp, vp, i, vp, vp)
+    _hip = L
+    return L
+
+
+def host() -> C.CDLL:
+    """The drop-in OccupancyGrid library (loads libcublox_hip.so first)."""
+    global _host
+    if _host is not None:
+        return _host
+    hip()
+    L = _load("libcublox.so")
+    i = C.c_int
+    F = C.c_float
+    _sig(L, "cublox_og_last_error", C.c_char_p)
+    _sig(L, "cublox_og_create", vp, i, i, i, F, i, i, C.c_double, f32p)
+    _sig(L, "cublox_og_destroy", None, vp)
+    _sig(L, "cublox_og_set_max_range", i, vp, F)
+    _sig(L, "cublox_og_set_log_odds", i, vp, F, F, F, F, F, F)
+    _sig(L, "cublox_og_update", i, vp, f32p, f32p, f32p, i, f32p)
+    _sig(L, "cublox_og_update_sensor", i, vp, f32p, f32p, f32p, i, f32p)
+    _sig(L, "cublox_og_update_sensor_device", i, vp, vp, vp, vp, i, f32p)
+    _sig(L, "cublox_og_update_device", i, vp, vp, vp, vp, i, f32p)
+    _sig(L, "cublox_og_cast_rays_device", i, vp, vp, vp, vp, i, f32p)
+    _sig(L, "cublox_og_apply_counters", i, vp)
+    _sig(L, "cublox_og_cast_rays_sensor_device", i, vp, vp, vp, vp, i, f32p)
+    _sig(L, "cublox_og_census", i, vp, C.POINTER(C.c_ulonglong))
+    _sig(L, "cublox_og_recenter", i, vp, f32p, i32p)
+    _sig(L, "cublox_og_fetch", i, vp, f32p, F, i, i, f32p, u8p, i, i32p)
+    _sig(L, "cublox_og_fetch_batch", i, vp, f32p, i, F, i, i, u32p, f32p, u8p)
+    _sig(L, "cublox_og_occ_to_host", i, vp, f32p)
+    _sig(L, "cublox_og_counters_to_host", i, vp, i32p, i32p)
+    _sig(L, "cublox_og_load_occ", i, vp, f32p)
+    _sig(L, "cublox_og_reset", i, vp)
+    _sig(L, "cublox_og_reset_voxel", i, vp, i)
+    _sig(L, "cublox_og_geometry", i, vp, i32p, i32p, i32p, f32p, i32p, i32p,
+         i32p, f32p, f32p)
+    _sig(L, "cublox_og_query", i, vp, i, i, f32p, i32p, i32p)
+    _sig(L, "cublox_og_max_horizontal_radius", i, vp, f32p, f32p)
+    _sig(L, "cublox_og_enable_timing", i, vp, i)
+    _sig(L, "cublox_og_kernel_times", i, vp, C.POINTER(C.c_double),
+         C.POINTER(C.c_longlong))
+    _sig(L, "cublox_og_reset_kernel_times", i, vp)
+    _sig(L, "cublox_og_device_occ", vp, vp)
+    _sig(L, "cublox_og_stream", vp, vp)
+    _sig(L, "cublox_utils_pos_to_global", i, F, F, i)
+    _sig(L, "cublox_utils_global_to_pos", F, i, F, i)
+    _sig(L, "cublox_utils_global_to_local", i, i, i, i)
+    _sig(L, "cublox_utils_local_to_global", i, i, i, i, i)
+    _sig(L, "cublox_utils_local_to_hash", i, i32p, i32p, i32p)
+    _sig(L, "cublox_utils_hash_to_local", None, i, i32p, i32p, i32p)
+    _sig(L, "cublox_utils_hash_to_pos", None, i, i32p, i32p, i32p, F, i, f32p)
+    _sig(L, "cublox_utils_pos_to_hash", i, f32p, i32p, i32p, F, i)
+    _sig(L, "cublox_logit", F, F)
+    _host = L
+    return L
+
+
+def gen() -> C.CDLL:
+    """The synthetic workload generator."""
+    global _gen
+    if _gen is not None:
+        return _gen
+    L = _load("libcublox_scangen.so")
+    i = C.c_int
+    F = C.c_float
+    _sig(L, "cublox_scene_create", vp, C.POINTER(SceneCfg))
+    _sig(L, "cublox_scene_destroy", None, vp)
+    _sig(L, "cublox_scene_box_count", i, vp)
+    _sig(L, "cublox_scene_boxes", None, vp, C.POINTER(C.c_double))
+    _sig(L, "cublox_scene_pose", None, vp, i, f32p, f32p)
+    _sig(L, "cublox_scene_scan", i, vp, i, i, i, F, F, F, f32p, f32p, f32p,
+         f32p, f32p, i32p, i)
+    _sig(L, "cublox_scene_queries", None, vp, i, i, F, f32p)
+    _sig(L, "cublox_rng_uniform", C.c_double, C.c_ulonglong, C.c_ulonglong,
+         C.c_ulonglong)
+    _gen = L
+    return L
+
+
+def check(status: int, lib=None) -> None:
+    """Raise RuntimeError with the library's message on a non-zero status."""
+    if status == 0:
+        return
+    msg = ""
+    try:
+        if lib is _host and _host is not None:
+            msg = _host.cublox_og_last_error().decode()
+        else:
+            msg = hip().cublox_last_error().decode()
+    except Exception:  # pragma: no cover - message lookup is best effort
+        pass
+    raise RuntimeError(f"cublox status {status}: {msg}")
+
+
+def fptr(a):
+    return a.ctypes.data_as(f32p)
+
+
+def iptr(a):
+    return a.ctypes.data_as(i32p)
+
+
+def uptr(a):
+    return a.ctypes.data_as(u32p)
+
+
+def bptr(a):
+    return a.ctypes.data_as(u8p)

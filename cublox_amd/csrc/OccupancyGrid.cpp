@@ -1,0 +1,661 @@
+// Host orchestration of the occupancy grid (reference
+// src/OccupancyGrid.cpp:54-521). Compiled by the host compiler; reaches the
+// device only through the C ABI of cublox/capi.h.
+#include <cublox/OccupancyGrid.hpp>
+#include <cublox/capi.h>
+#include <cublox/utils.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+namespace cublox {
+
+namespace {
+
+// Status -> exception, message shaped as the reference's cudaCheck
+// (OccupancyGrid.cpp:32-49).
+void statusCheck(int status, const char *expr, const char *file, int line) {
+  if (status == CUBLOX_OK) return;
+  std::string msg = "cublox::cudaCheck: CUDA error in `";
+  msg += expr;
+  msg += "`: ";
+  msg += cublox_last_error();
+  msg += " (";
+  msg += file;
+  msg += ":";
+  msg += std::to_string(line);
+  msg += ")";
+  throw std::runtime_error(msg);
+}
+
+#define CUBLOX_CHECK(call) ::cublox::statusCheck((call), #call, __FILE__, __LINE__)
+
+cublox_int3 i3(const Eigen::Vector3i &v) { return cublox_int3{v.x(), v.y(), v.z()}; }
+cublox_float3 f3(const Eigen::Vector3f &v) {
+  return cublox_float3{v.x(), v.y(), v.z()};
+}
+
+template <typename T> void freeDevice(T *&p) noexcept {
+  if (p) {
+    cublox_free(p);
+    p = nullptr;
+  }
+}
+
+} // namespace
+
+OccupancyGrid::OccupancyGrid(const Eigen::Vector3i &half_map_size_i,
+                             const float resolution,
+                             const bool origin_at_center,
+                             const std::optional<double> recenter_threshold,
+                             const Eigen::Vector3f &origin)
+    : Grid(half_map_size_i, resolution,
+           recenter_threshold ? std::optional<float>(
+                                    static_cast<float>(*recenter_threshold))
+                              : std::nullopt,
+           origin_at_center) {
+  Eigen::Vector3i origin_i;
+  posToGlobalIndex(origin, config_.resolution_inv, config_.origin_at_center,
+                   origin_i);
+  updateOriginAndBound(origin_i);
+  allocateVoxelBuffers();
+}
+
+OccupancyGrid::~OccupancyGrid() { releaseAll(); }
+
+OccupancyGrid::OccupancyGrid(OccupancyGrid &&o) noexcept : Grid(o) {
+  *this = std::move(o);
+}
+
+OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
+  if (this == &o) return *this;
+  releaseAll();
+  static_cast<Grid &>(*this) = static_cast<const Grid &>(o);
+  first_run_ = o.first_run_;
+  stream_ = std::exchange(o.stream_, nullptr);
+  d_op_cnt_ = std::exchange(o.d_op_cnt_, nullptr);
+  d_hit_cnt_ = std::exchange(o.d_hit_cnt_, nullptr);
+  d_occ_ = std::exchange(o.d_occ_, nullptr);
+  d_cloud_x_ = std::exchange(o.d_cloud_x_, nullptr);
+  d_cloud_y_ = std::exchange(o.d_cloud_y_, nullptr);
+  d_cloud_z_ = std::exchange(o.d_cloud_z_, nullptr);
+  cloud_capacity_ = std::exchange(o.cloud_capacity_, 0);
+  d_fetch_count_ = std::exchange(o.d_fetch_count_, nullptr);
+  d_fetch_pos_ = std::exchange(o.d_fetch_pos_, nullptr);
+  d_fetch_state_ = std::exchange(o.d_fetch_state_, nullptr);
+  d_fetch_cfgs_ = std::exchange(o.d_fetch_cfgs_, nullptr);
+  d_fetch_counts_ = std::exchange(o.d_fetch_counts_, nullptr);
+  fetch_capacity_ = std::exchange(o.fetch_capacity_, 0);
+  fetch_cfg_capacity_ = std::exchange(o.fetch_cfg_capacity_, 0);
+  timing_ = std::exchange(o.timing_, false);
+  for (int k = 0; k < 6; ++k) events_[k] = std::exchange(o.events_[k], nullptr);
+  for (int k = 0; k < 3; ++k) pending_[k] = std::exchange(o.pending_[k], false);
+  pending_zeroed_ = std::exchange(o.pending_zeroed_, 0);
+  times_ = o.times_;
+  max_raycast_range_ = o.max_raycast_range_;
+  l_hit_ = o.l_hit_;
+  l_miss_ = o.l_miss_;
+  l_min_ = o.l_min_;
+  l_max_ = o.l_max_;
+  l_free_ = o.l_free_;
+  l_occupied_ = o.l_occupied_;
+  l_unknown_ = o.l_unknown_;
+  return *this;
+}
+
+void OccupancyGrid::releaseAll() noexcept {
+  if (stream_) cublox_stream_synchronize(stream_);
+  freeDevice(d_fetch_state_);
+  freeDevice(d_fetch_pos_);
+  freeDevice(d_fetch_count_);
+  freeDevice(d_fetch_cfgs_);
+  freeDevice(d_fetch_counts_);
+  freeDevice(d_op_cnt_);
+  freeDevice(d_hit_cnt_);
+  freeDevice(d_occ_);
+  freeDevice(d_cloud_x_);
+  freeDevice(d_cloud_y_);
+  freeDevice(d_cloud_z_);
+  cloud_capacity_ = 0;
+  fetch_capacity_ = 0;
+  fetch_cfg_capacity_ = 0;
+  for (auto &e : events_) {
+    if (e) {
+      cublox_event_destroy(e);
+      e = nullptr;
+    }
+  }
+  timing_ = false;
+  if (stream_) {
+    cublox_stream_destroy(stream_);
+    stream_ = nullptr;
+  }
+}
+
+void OccupancyGrid::enableKernelTiming(bool on) {
+  if (on && !events_[0]) {
+    for (auto &e : events_) CUBLOX_CHECK(cublox_event_create(&e));
+  }
+  timing_ = on;
+}
+
+void OccupancyGrid::mark(int ev) const {
+  if (timing_) CUBLOX_CHECK(cublox_event_record(events_[ev], stream_));
+}
+
+// Called after a stream sync: fold the completed event pairs into times_.
+void OccupancyGrid::harvestTimes() {
+  if (!timing_) return;
+  double *acc[3] = {&times_.raycast_ms, &times_.apply_ms, &times_.recenter_ms};
+  long long *cnt[3] = {&times_.raycast_launches, &times_.apply_launches,
+                       &times_.recenter_launches};
+  for (int k = 0; k < 3; ++k) {
+    if (!pending_[k]) continue;
+    float ms = 0.0f;
+    CUBLOX_CHECK(cublox_event_elapsed_ms(events_[2 * k], events_[2 * k + 1], &ms));
+    *acc[k] += ms;
+    *cnt[k] += 1;
+    pending_[k] = false;
+  }
+  times_.recenter_zeroed_voxels += pending_zeroed_;
+  pending_zeroed_ = 0;
+}
+
+void OccupancyGrid::sync() const { CUBLOX_CHECK(cublox_stream_synchronize(stream_)); }
+
+// OccupancyGrid.cpp:431-480: three dense arrays of V, zeroed; partial
+// allocations are rolled back before throwing.
+void OccupancyGrid::allocateVoxelBuffers() {
+  const size_t n = static_cast<size_t>(config_.voxel_num);
+  if (n == 0) return;
+  try {
+    CUBLOX_CHECK(cublox_stream_create(&stream_));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_op_cnt_), n * sizeof(int)));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_hit_cnt_), n * sizeof(int)));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_occ_), n * sizeof(float)));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, n, stream_));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, n, stream_));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_occ_, n, stream_));
+    sync();
+  } catch (...) {
+    releaseAll();
+    throw std::runtime_error(
+        "cublox::OccupancyGrid: failed to allocate voxel buffers");
+  }
+}
+
+void OccupancyGrid::reset() {
+  if (d_occ_ == nullptr) return;
+  const size_t n = static_cast<size_t>(config_.voxel_num);
+  CUBLOX_CHECK(cublox_fill_zero_async(d_occ_, n, stream_));
+  CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, n, stream_));
+  CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, n, stream_));
+  sync();
+  first_run_ = true;
+}
+
+void OccupancyGrid::resetVoxel(const int hash_id) {
+  if (d_occ_ == nullptr || hash_id < 0 || hash_id >= config_.voxel_num) return;
+  CUBLOX_CHECK(cublox_memset_async(d_occ_ + hash_id, 0, sizeof(float), stream_));
+  CUBLOX_CHECK(cublox_memset_async(d_op_cnt_ + hash_id, 0, sizeof(int), stream_));
+  CUBLOX_CHECK(cublox_memset_async(d_hit_cnt_ + hash_id, 0, sizeof(int), stream_));
+  sync();
+}
+
+// OccupancyGrid.cpp:141-179. The plan runs on the host (integer math on the
+// config), so there is no device round trip. The counters are zero between
+// updates (apply zeroes every non-zero counter), so only occ is cleared.
+void OccupancyGrid::recenter(const Eigen::Vector3f &pos) {
+  if (!config_.recenter_threshold.has_value() || d_occ_ == nullptr) return;
+  if ((pos - origin_f_).squaredNorm() < config_.recenter_threshold_squared) return;
+
+  cublox_recenter_cfg cfg{};
+  cfg.pos = f3(pos);
+  cfg.origin_f = f3(origin_f_);
+  cfg.origin_i = i3(origin_i_);
+  cfg.map_size_i = i3(config_.map_size_i);
+  cfg.half_map_size_i = i3(config_.half_map_size_i);
+  cfg.resolution = config_.resolution;
+  cfg.inv_resolution = config_.resolution_inv;
+  cfg.recenter_threshold_sq = config_.recenter_threshold_squared;
+  cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
+  cfg.map_vox_num = config_.voxel_num;
+
+  cublox_recenter_result result{};
+  mark(4);
+  CUBLOX_CHECK(cublox_launch_recenter(d_occ_, nullptr, nullptr, &cfg, stream_, &result));
+  mark(5);
+  if (result.action == 0) return;
+  sync();
+  if (timing_) {
+    pending_[2] = true;
+    pending_zeroed_ += result.zeroed_voxels;
+    harvestTimes();
+  }
+  updateOriginAndBound(Eigen::Vector3i(result.new_origin_i.x,
+                                       result.new_origin_i.y,
+                                       result.new_origin_i.z));
+}
+
+// OccupancyGrid.cpp:192-198: the first update pins the window origin to the
+// sensor voxel.
+void OccupancyGrid::pinFirstOrigin(const Eigen::Vector3f &sensor_origin) {
+  if (!first_run_) return;
+  Eigen::Vector3i origin_i;
+  posToGlobalIndex(sensor_origin, config_.resolution_inv,
+                   config_.origin_at_center, origin_i);
+  updateOriginAndBound(origin_i);
+  first_run_ = false;
+}
+
+void OccupancyGrid::castRaysWorld(const float *d_x, const float *d_y,
+                                  const float *d_z, int n,
+                                  const Eigen::Vector3f &sensor_origin) {
+  cublox_raycast_cfg cfg{};
+  cfg.origin = f3(sensor_origin);
+  cfg.origin_i = i3(origin_i_);
+  cfg.map_size_i = i3(config_.map_size_i);
+  cfg.half_map_size_i = i3(config_.half_map_size_i);
+  cfg.resolution = config_.resolution;
+  cfg.inv_resolution = config_.resolution_inv;
+  cfg.max_range = max_raycast_range_;
+  cfg.map_vox_num = config_.voxel_num;
+  cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
+  mark(0);
+  CUBLOX_CHECK(cublox_launch_ray_cast_update(&cfg, d_x, d_y, d_z, n, stream_,
+                                             d_op_cnt_, d_hit_cnt_));
+  mark(1);
+  pending_[0] = timing_;
+}
+
+void OccupancyGrid::castRaysSensor(const float *d_xs, const float *d_ys,
+                                   const float *d_zs, int n,
+                                   const float T[12]) {
+  cublox_raycast_cfg cfg{};
+  cfg.origin = cublox_float3{T[3], T[7], T[11]};
+  cfg.origin_i = i3(origin_i_);
+  cfg.map_size_i = i3(config_.map_size_i);
+  cfg.half_map_size_i = i3(config_.half_map_size_i);
+  cfg.resolution = config_.resolution;
+  cfg.inv_resolution = config_.resolution_inv;
+  cfg.max_range = max_raycast_range_;
+  cfg.map_vox_num = config_.voxel_num;
+  cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
+  mark(0);
+  CUBLOX_CHECK(cublox_launch_ray_cast_update_sensor(&cfg, T, d_xs, d_ys, d_zs, n,
+                                                    stream_, d_op_cnt_,
+                                                    d_hit_cnt_));
+  mark(1);
+  pending_[0] = timing_;
+}
+
+void OccupancyGrid::launchApply() {
+  mark(2);
+  CUBLOX_CHECK(cublox_launch_apply_update(d_occ_, d_op_cnt_, d_hit_cnt_,
+                                          config_.voxel_num, l_hit_, l_miss_,
+                                          l_min_, l_max_, l_unknown_, stream_,
+                                          nullptr, nullptr, nullptr, 0u));
+  mark(3);
+  pending_[1] = timing_;
+}
+
+void OccupancyGrid::ensureCloudCapacity(const int n) {
+  if (n <= cloud_capacity_) return;
+  sync();
+  freeDevice(d_cloud_x_);
+  freeDevice(d_cloud_y_);
+  freeDevice(d_cloud_z_);
+  cloud_capacity_ = 0;
+  const size_t bytes = static_cast<size_t>(n) * sizeof(float);
+  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_x_), bytes));
+  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_y_), bytes));
+  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_z_), bytes));
+  cloud_capacity_ = n;
+}
+
+// OccupancyGrid.cpp:181-238: cast, then fold, then one sync (the reference
+// syncs after each pass; stream order makes the first unnecessary).
+void OccupancyGrid::update(const PointCloud &cloud,
+                           const Eigen::Vector3f &sensor_origin) {
+  // PointCloud is column-major: each column is a contiguous SoA array.
+  updateColumns(cloud.col(0).data(), cloud.col(1).data(), cloud.col(2).data(),
+                static_cast<int>(cloud.rows()), sensor_origin);
+}
+
+void OccupancyGrid::updateColumns(const float *x, const float *y,
+                                  const float *z, int n,
+                                  const Eigen::Vector3f &sensor_origin) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(sensor_origin);
+  ensureCloudCapacity(n);
+  const size_t bytes = static_cast<size_t>(n) * sizeof(float);
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_x_, x, bytes, stream_));
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_y_, y, bytes, stream_));
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_z_, z, bytes, stream_));
+  castRaysWorld(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, sensor_origin);
+  launchApply();
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::updateSensorFrame(const PointCloud &cloud,
+                                      const Eigen::Matrix3f &R,
+                                      const Eigen::Vector3f &t) {
+  float T[12];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) T[4 * r + c] = R(r, c);
+    T[4 * r + 3] = t(r);
+  }
+  updateSensorFrameColumns(cloud.col(0).data(), cloud.col(1).data(),
+                           cloud.col(2).data(), static_cast<int>(cloud.rows()),
+                           T);
+}
+
+void OccupancyGrid::updateSensorFrameColumns(const float *xs, const float *ys,
+                                             const float *zs, int n,
+                                             const float T[12]) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
+  ensureCloudCapacity(n);
+  const size_t bytes = static_cast<size_t>(n) * sizeof(float);
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_x_, xs, bytes, stream_));
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_y_, ys, bytes, stream_));
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_z_, zs, bytes, stream_));
+  castRaysSensor(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T);
+  launchApply();
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::updateSensorFrameDevice(const float *d_xs,
+                                            const float *d_ys,
+                                            const float *d_zs, int n,
+                                            const float T[12]) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
+  castRaysSensor(d_xs, d_ys, d_zs, n, T);
+  launchApply();
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::updateDevice(const float *d_x, const float *d_y,
+                                 const float *d_z, int n,
+                                 const Eigen::Vector3f &sensor_origin) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(sensor_origin);
+  castRaysWorld(d_x, d_y, d_z, n, sensor_origin);
+  launchApply();
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::castRaysDevice(const float *d_x, const float *d_y,
+                                   const float *d_z, int n,
+                                   const Eigen::Vector3f &sensor_origin) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(sensor_origin);
+  castRaysWorld(d_x, d_y, d_z, n, sensor_origin);
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::castRaysSensorFrameDevice(const float *d_xs,
+                                              const float *d_ys,
+                                              const float *d_zs, int n,
+                                              const float T[12]) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
+  castRaysSensor(d_xs, d_ys, d_zs, n, T);
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::counterCensus(unsigned long long out[4]) const {
+  for (int k = 0; k < 4; ++k) out[k] = 0;
+  if (!d_occ_) return;
+  unsigned long long *d = nullptr;
+  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d), 4 * sizeof(unsigned long long)));
+  int st = cublox_memset_async(d, 0, 4 * sizeof(unsigned long long), stream_);
+  if (st == CUBLOX_OK)
+    st = cublox_counter_census(d_op_cnt_, d_hit_cnt_, config_.voxel_num, d, stream_);
+  if (st == CUBLOX_OK)
+    st = cublox_memcpy_d2h_async(out, d, 4 * sizeof(unsigned long long), stream_);
+  if (st == CUBLOX_OK) st = cublox_stream_synchronize(stream_);
+  cublox_free(d);
+  CUBLOX_CHECK(st);
+}
+
+void OccupancyGrid::applyCounters() {
+  if (config_.voxel_num <= 0) return;
+  launchApply();
+  sync();
+  harvestTimes();
+}
+
+void OccupancyGrid::ensureFetchCapacity(const long long n) const {
+  if (n <= fetch_capacity_) return;
+  sync();
+  freeDevice(d_fetch_state_);
+  freeDevice(d_fetch_pos_);
+  fetch_capacity_ = 0;
+  if (d_fetch_count_ == nullptr) {
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_count_),
+                               sizeof(unsigned int)));
+  }
+  const size_t cap = static_cast<size_t>(n);
+  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_pos_),
+                             cap * 3 * sizeof(float)));
+  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_state_), cap));
+  fetch_capacity_ = n;
+}
+
+// OccupancyGrid.cpp:240-356
+std::vector<OccupancyGrid::OccupancySample> OccupancyGrid::fetchOccupancyAround(
+    const Eigen::Vector3f &center, const float radius,
+    const std::optional<VoxelState> state_filter, const int max_results) const {
+  std::vector<OccupancySample> out;
+  if (d_occ_ == nullptr || config_.voxel_num <= 0 || radius <= 0.0f) return out;
+
+  cublox_fetch_cfg cfg{};
+  long long est = 0;
+  unsigned int output_cap = 0;
+  const int filter = state_filter ? static_cast<int>(*state_filter) : -1;
+  if (!cublox_fetch_plan(i3(origin_i_), i3(config_.map_size_i),
+                         i3(config_.half_map_size_i), config_.resolution,
+                         config_.resolution_inv,
+                         config_.origin_at_center ? 1 : 0, l_free_,
+                         l_occupied_, l_unknown_, f3(center), radius, filter,
+                         max_results, &cfg, &est, &output_cap)) {
+    return out;
+  }
+  const long long buf = std::min<long long>(output_cap, est);
+  ensureFetchCapacity(buf);
+  CUBLOX_CHECK(cublox_memset_async(d_fetch_count_, 0, sizeof(unsigned int), stream_));
+  const unsigned int max_out =
+      max_results > 0 ? std::min(static_cast<unsigned int>(max_results), output_cap)
+                      : output_cap;
+  CUBLOX_CHECK(cublox_launch_fetch_occupancy_around(
+      &cfg, d_occ_, d_fetch_count_, d_fetch_pos_, d_fetch_state_,
+      static_cast<unsigned int>(buf), max_out, stream_));
+  unsigned int n = 0;
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(&n, d_fetch_count_, sizeof(n), stream_));
+  sync();
+  n = std::min(n, output_cap);
+  if (max_results > 0) n = std::min(n, static_cast<unsigned int>(max_results));
+  n = std::min<unsigned int>(n, static_cast<unsigned int>(buf));
+  if (n == 0) return out;
+
+  std::vector<float> pos(static_cast<size_t>(n) * 3);
+  std::vector<unsigned char> st(n);
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(pos.data(), d_fetch_pos_, pos.size() * sizeof(float), stream_));
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(st.data(), d_fetch_state_, st.size(), stream_));
+  sync();
+  out.reserve(n);
+  for (unsigned int i = 0; i < n; ++i) {
+    out.push_back(OccupancySample{
+        Eigen::Vector3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+        static_cast<VoxelState>(st[i])});
+  }
+  return out;
+}
+
+void OccupancyGrid::fetchOccupancyBatch(
+    const std::vector<Eigen::Vector3f> &centers, const float radius,
+    const std::optional<VoxelState> state_filter, const int per_query_cap,
+    std::vector<unsigned int> &counts, std::vector<float> &positions,
+    std::vector<unsigned char> &states) const {
+  const int nq = static_cast<int>(centers.size());
+  counts.assign(static_cast<size_t>(nq), 0u);
+  positions.clear();
+  states.clear();
+  if (nq == 0 || d_occ_ == nullptr || per_query_cap <= 0) return;
+  const int filter = state_filter ? static_cast<int>(*state_filter) : -1;
+  std::vector<cublox_fetch_cfg> cfgs(static_cast<size_t>(nq));
+  int max_diam = 0, max_nz = 0;
+  for (int q = 0; q < nq; ++q) {
+    cublox_fetch_cfg &c = cfgs[static_cast<size_t>(q)];
+    long long est = 0;
+    unsigned int cap = 0;
+    if (cublox_fetch_plan(i3(origin_i_), i3(config_.map_size_i),
+                          i3(config_.half_map_size_i), config_.resolution,
+                          config_.resolution_inv,
+                          config_.origin_at_center ? 1 : 0, l_free_,
+                          l_occupied_, l_unknown_, f3(centers[q]), radius,
+                          filter, per_query_cap, &c, &est, &cap)) {
+      max_diam = std::max(max_diam, 2 * c.disk_radius_vox + 1);
+      max_nz = std::max(max_nz, c.gz_max - c.gz_min + 1);
+    } else {
+      c = cublox_fetch_cfg{};
+      c.disk_radius_vox = -1;  // no work for this query
+      c.gz_min = 1;
+      c.gz_max = 0;
+    }
+  }
+  if (max_diam == 0) return;
+  const long long cap_total = static_cast<long long>(nq) * per_query_cap;
+  ensureFetchCapacity(cap_total);
+  if (nq > fetch_cfg_capacity_) {
+    sync();
+    freeDevice(d_fetch_cfgs_);
+    freeDevice(d_fetch_counts_);
+    fetch_cfg_capacity_ = 0;
+    CUBLOX_CHECK(cublox_malloc(&d_fetch_cfgs_, sizeof(cublox_fetch_cfg) * nq));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_counts_),
+                               sizeof(unsigned int) * nq));
+    fetch_cfg_capacity_ = nq;
+  }
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_fetch_cfgs_, cfgs.data(),
+                                       sizeof(cublox_fetch_cfg) * nq, stream_));
+  CUBLOX_CHECK(cublox_memset_async(d_fetch_counts_, 0, sizeof(unsigned int) * nq,
+                                   stream_));
+  CUBLOX_CHECK(cublox_launch_fetch_occupancy_batch(
+      static_cast<const cublox_fetch_cfg *>(d_fetch_cfgs_), nq, max_diam, max_nz,
+      d_occ_, d_fetch_counts_, d_fetch_pos_, d_fetch_state_,
+      static_cast<unsigned int>(per_query_cap), stream_));
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(counts.data(), d_fetch_counts_,
+                                       sizeof(unsigned int) * nq, stream_));
+  sync();
+  int last = -1;
+  for (int q = 0; q < nq; ++q) {
+    counts[q] = std::min(counts[q], static_cast<unsigned int>(per_query_cap));
+    if (counts[q] > 0) last = q;
+  }
+  if (last < 0) return;
+  const size_t span = static_cast<size_t>(last + 1) * per_query_cap;
+  positions.resize(span * 3);
+  states.resize(span);
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(positions.data(), d_fetch_pos_,
+                                       span * 3 * sizeof(float), stream_));
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(states.data(), d_fetch_state_, span, stream_));
+  sync();
+}
+
+float OccupancyGrid::storedAt(const int hash_id) const {
+  if (d_occ_ == nullptr || hash_id < 0 || hash_id >= config_.voxel_num) return 0.0f;
+  float v = 0.0f;
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(&v, d_occ_ + hash_id, sizeof(float), stream_));
+  sync();
+  return v;
+}
+
+bool OccupancyGrid::isOccupied(const Eigen::Vector3f &pos) const {
+  if (!inside(pos)) return false;
+  return occOccupied(storedAt(posToHashIndex(pos, config_.map_size_i,
+                                             config_.half_map_size_i,
+                                             config_.resolution_inv,
+                                             config_.origin_at_center)));
+}
+
+bool OccupancyGrid::isUnknown(const Eigen::Vector3f &pos) const {
+  if (!inside(pos)) return true;
+  return occUnknown(storedAt(posToHashIndex(pos, config_.map_size_i,
+                                            config_.half_map_size_i,
+                                            config_.resolution_inv,
+                                            config_.origin_at_center)));
+}
+
+bool OccupancyGrid::isFree(const Eigen::Vector3f &pos) const {
+  if (!inside(pos)) return false;
+  return occFree(storedAt(posToHashIndex(pos, config_.map_size_i,
+                                         config_.half_map_size_i,
+                                         config_.resolution_inv,
+                                         config_.origin_at_center)));
+}
+
+bool OccupancyGrid::isOccupied(const Eigen::Vector3i &id_g) const {
+  if (!inside(id_g)) return false;
+  return occOccupied(storedAt(
+      globalIndexToHashId(id_g, config_.map_size_i, config_.half_map_size_i)));
+}
+
+bool OccupancyGrid::isUnknown(const Eigen::Vector3i &id_g) const {
+  if (!inside(id_g)) return true;
+  return occUnknown(storedAt(
+      globalIndexToHashId(id_g, config_.map_size_i, config_.half_map_size_i)));
+}
+
+bool OccupancyGrid::isFree(const Eigen::Vector3i &id_g) const {
+  if (!inside(id_g)) return false;
+  return occFree(storedAt(
+      globalIndexToHashId(id_g, config_.map_size_i, config_.half_map_size_i)));
+}
+
+bool OccupancyGrid::isOccupied(const int hash_id) const {
+  return occOccupied(storedAt(hash_id));
+}
+bool OccupancyGrid::isUnknown(const int hash_id) const {
+  return occUnknown(storedAt(hash_id));
+}
+bool OccupancyGrid::isFree(const int hash_id) const {
+  return occFree(storedAt(hash_id));
+}
+
+void OccupancyGrid::copyOccupancyToHost(float *dst) const {
+  if (!d_occ_) return;
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(
+      dst, d_occ_, static_cast<size_t>(config_.voxel_num) * sizeof(float), stream_));
+  sync();
+}
+
+void OccupancyGrid::copyCountersToHost(int *op, int *hit) const {
+  if (!d_occ_) return;
+  const size_t bytes = static_cast<size_t>(config_.voxel_num) * sizeof(int);
+  if (op) CUBLOX_CHECK(cublox_memcpy_d2h_async(op, d_op_cnt_, bytes, stream_));
+  if (hit) CUBLOX_CHECK(cublox_memcpy_d2h_async(hit, d_hit_cnt_, bytes, stream_));
+  sync();
+}
+
+void OccupancyGrid::loadOccupancyFromHost(const float *src) {
+  if (!d_occ_) return;
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(
+      d_occ_, src, static_cast<size_t>(config_.voxel_num) * sizeof(float), stream_));
+  sync();
+}
+
+} // namespace cublox

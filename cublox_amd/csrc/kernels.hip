@@ -1,0 +1,936 @@
+// Occupancy-update kernels for CDNA4 (gfx950, wave64) and their C-ABI
+// launchers: ray cast (with the fused scan transform), apply, recenter slab
+// reset, index clear and sphere fetch.
+//
+// Reference semantics: src/kernels.cu of the reference (lines cited per
+// kernel). Arithmetic is FP32 with contraction off and IEEE division/sqrt,
+// so the traversed voxel sets are bit-identical to the CPU restatement.
+#include "hip_common.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+namespace cublox_hip {
+
+// ===================================================================
+// Wave-level helpers
+// ===================================================================
+
+__device__ __forceinline__ void atomic_add_i32(int *p, int v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned atomic_add_u32_ret(unsigned *p,
+                                                       unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Adds 1 at arr[key] for every active lane whose key is >= 0, issuing one
+// atomic per distinct key among the active lanes of the wavefront (a
+// match-any built from readfirstlane + ballot). Lanes with key < 0 take part
+// in the matching but add nothing.
+__device__ __forceinline__ void wave_add_unique(int *arr, int key) {
+  const int lane = __lane_id();
+  while (true) {
+    const int lead = __builtin_amdgcn_readfirstlane(key);
+    const bool mine = key == lead;
+    const unsigned long long m = __ballot(mine);
+    if (mine) {
+      if (lead >= 0 && lane == __ffsll(static_cast<long long>(m)) - 1) {
+        atomic_add_i32(arr + lead, __popcll(m));
+      }
+      break;
+    }
+  }
+}
+
+// ===================================================================
+// Ray cast (reference rayCastUpdateKernel, kernels.cu:33-152)
+// ===================================================================
+
+struct RayArgs {
+  float T[12];  // sensor->world, row-major [R | t]; used when transforming
+  float ox, oy, oz;
+  int oix, oiy, oiz;
+  int mx, my, mz;
+  int hx, hy, hz;
+  float res, inv_res, max_range;
+  int vox_num;
+  int center;
+};
+
+// Steps below this count are near the sensor voxel, where every ray of the
+// scan passes through the same few voxels (each ray visits exactly one voxel
+// per L1 shell around the sensor voxel). Their increments are deduplicated
+// across the wavefront before the global atomic.
+constexpr int kNearSteps = 12;
+
+template <bool kTransform>
+__global__ void __launch_bounds__(256)
+    ray_cast_kernel(const RayArgs a, const float *__restrict__ xs,
+                    const float *__restrict__ ys, const float *__restrict__ zs,
+                    const int n, int *__restrict__ op_cnt,
+                    int *__restrict__ hit_cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+
+  float ex = xs[i], ey = ys[i], ez = zs[i];
+  if (kTransform) {
+    // Scan transform (driver main.cpp:672-682): drop non-finite sensor-frame
+    // points, then p_w = R p_s + t in a fixed operand order.
+    if (!isfinite(ex) || !isfinite(ey) || !isfinite(ez)) return;
+    const float px = ex, py = ey, pz = ez;
+    ex = ((a.T[0] * px + a.T[1] * py) + a.T[2] * pz) + a.T[3];
+    ey = ((a.T[4] * px + a.T[5] * py) + a.T[6] * pz) + a.T[7];
+    ez = ((a.T[8] * px + a.T[9] * py) + a.T[10] * pz) + a.T[11];
+  }
+  const float ox = a.ox, oy = a.oy, oz = a.oz;
+
+  // kernels.cu:49-54
+  float dx = ex - ox, dy = ey - oy, dz = ez - oz;
+  float L = sqrtf((dx * dx + dy * dy) + dz * dz);
+  if (L < 1e-6f || !isfinite(L) || !isfinite(ex) || !isfinite(ey) ||
+      !isfinite(ez)) {
+    return;
+  }
+  // kernels.cu:57-67: range clip marks free space only.
+  const bool clipped = L > a.max_range;
+  if (clipped) {
+    const float s = a.max_range / L;
+    ex = ox + dx * s;
+    ey = oy + dy * s;
+    ez = oz + dz * s;
+    dx = dx * s;
+    dy = dy * s;
+    dz = dz * s;
+    L = a.max_range;
+  }
+  const float invL = 1.0f / L;
+  const float ux = dx * invL, uy = dy * invL, uz = dz * invL;
+
+  // kernels.cu:73-100: Amanatides-Woo set-up.
+  const bool ctr = a.center != 0;
+  int ix = pos_to_global(ox, a.inv_res, ctr);
+  int iy = pos_to_global(oy, a.inv_res, ctr);
+  int iz = pos_to_global(oz, a.inv_res, ctr);
+  const int eix = pos_to_global(ex, a.inv_res, ctr);
+  const int eiy = pos_to_global(ey, a.inv_res, ctr);
+  const int eiz = pos_to_global(ez, a.inv_res, ctr);
+  const int sx = (ux > 0.0f) - (ux < 0.0f);
+  const int sy = (uy > 0.0f) - (uy < 0.0f);
+  const int sz = (uz > 0.0f) - (uz < 0.0f);
+  const float inf = __builtin_inff();
+  const float tDx = sx == 0 ? inf : fabsf(a.res / ux);
+  const float tDy = sy == 0 ? inf : fabsf(a.res / uy);
+  const float tDz = sz == 0 ? inf : fabsf(a.res / uz);
+  const float bx = ctr ? (static_cast<float>(ix) + 0.5f * static_cast<float>(sx)) * a.res
+                       : (static_cast<float>(ix) + static_cast<float>(sx > 0)) * a.res;
+  const float by = ctr ? (static_cast<float>(iy) + 0.5f * static_cast<float>(sy)) * a.res
+                       : (static_cast<float>(iy) + static_cast<float>(sy > 0)) * a.res;
+  const float bz = ctr ? (static_cast<float>(iz) + 0.5f * static_cast<float>(sz)) * a.res
+                       : (static_cast<float>(iz) + static_cast<float>(sz > 0)) * a.res;
+  float tMx = sx == 0 ? inf : (bx - ox) / ux;
+  float tMy = sy == 0 ? inf : (by - oy) / uy;
+  float tMz = sz == 0 ? inf : (bz - oz) / uz;
+
+  // kernels.cu:105-137
+  const int max_steps = abs(eix - ix) + abs(eiy - iy) + abs(eiz - iz) + 4;
+  for (int step = 0; step < max_steps; ++step) {
+    if (ix == eix && iy == eiy && iz == eiz) break;
+    int h = -1;
+    if (inside_window(ix, iy, iz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
+      const int hh = global_to_hash(ix, iy, iz, a.mx, a.my, a.mz, a.hx, a.hy,
+                                    a.hz);
+      if (hh >= 0 && hh < a.vox_num) h = hh;
+    }
+    if (step < kNearSteps) {
+      wave_add_unique(op_cnt, h);
+    } else if (h >= 0) {
+      atomic_add_i32(op_cnt + h, 1);
+    }
+    if (tMx < tMy) {
+      if (tMx < tMz) {
+        ix += sx;
+        tMx += tDx;
+      } else {
+        iz += sz;
+        tMz += tDz;
+      }
+    } else {
+      if (tMy < tMz) {
+        iy += sy;
+        tMy += tDy;
+      } else {
+        iz += sz;
+        tMz += tDz;
+      }
+    }
+  }
+
+  // kernels.cu:141-151: the endpoint counts even when the walk did not reach
+  // it; a hit only for real returns.
+  if (inside_window(eix, eiy, eiz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
+    const int h =
+        global_to_hash(eix, eiy, eiz, a.mx, a.my, a.mz, a.hx, a.hy, a.hz);
+    if (h >= 0 && h < a.vox_num) {
+      atomic_add_i32(op_cnt + h, 1);
+      if (!clipped) atomic_add_i32(hit_cnt + h, 1);
+    }
+  }
+}
+
+RayArgs make_ray_args(const cublox_raycast_cfg &c) {
+  RayArgs a{};
+  a.ox = c.origin.x;
+  a.oy = c.origin.y;
+  a.oz = c.origin.z;
+  a.oix = c.origin_i.x;
+  a.oiy = c.origin_i.y;
+  a.oiz = c.origin_i.z;
+  a.mx = c.map_size_i.x;
+  a.my = c.map_size_i.y;
+  a.mz = c.map_size_i.z;
+  a.hx = c.half_map_size_i.x;
+  a.hy = c.half_map_size_i.y;
+  a.hz = c.half_map_size_i.z;
+  a.res = c.resolution;
+  a.inv_res = c.inv_resolution;
+  a.max_range = c.max_range;
+  a.vox_num = c.map_vox_num;
+  a.center = c.origin_at_center;
+  return a;
+}
+
+// ===================================================================
+// Apply (reference applyUpdateKernel, kernels.cu:167-210)
+// ===================================================================
+
+struct ApplyArgs {
+  float l_hit, l_miss, l_min, l_max, l_unk;
+};
+
+// kernels.cu:184-200 for one voxel with op != 0. Returns the new stored
+// value through *out and whether it differs from the old one.
+__device__ __forceinline__ bool apply_voxel(float *__restrict__ occ,
+                                            int *__restrict__ hit_cnt, int h,
+                                            int op, const ApplyArgs &p,
+                                            float *out) {
+  const int hit = hit_cnt[h];
+  const int miss = op > hit ? op - hit : 0;
+  const float stored0 = occ[h];
+  float l = stored0 + p.l_unk;
+  l = l + p.l_miss * static_cast<float>(miss);
+  l = l + p.l_hit * static_cast<float>(hit);
+  if (l < p.l_min) l = p.l_min;
+  if (l > p.l_max) l = p.l_max;
+  const float stored = l - p.l_unk;
+  occ[h] = stored;
+  if (hit != 0) hit_cnt[h] = 0;
+  *out = stored;
+  return stored != stored0;
+}
+
+__device__ __forceinline__ void record_modified(unsigned *count, int *idx,
+                                                float *val, unsigned cap,
+                                                int h, float v) {
+  const unsigned slot = atomic_add_u32_ret(count, 1u);
+  if (slot < cap) {
+    idx[slot] = h;
+    val[slot] = v;
+  }
+}
+
+// One pass over all V counters in 16 B loads, four chunks in flight per
+// thread; only voxels with op != 0 touch hit/occ.
+template <bool kRecord>
+__global__ void __launch_bounds__(256)
+    apply_kernel(float *__restrict__ occ, int *__restrict__ op_cnt,
+                 int *__restrict__ hit_cnt, const int V, const ApplyArgs p,
+                 unsigned *__restrict__ mod_count, int *__restrict__ mod_idx,
+                 float *__restrict__ mod_val, const unsigned mod_cap) {
+  constexpr int kUnroll = 4;
+  int4 *__restrict__ op4 = reinterpret_cast<int4 *>(op_cnt);
+  const int n4 = V / 4;
+  const int stride = gridDim.x * blockDim.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int base = t; base < n4; base += stride * kUnroll) {
+    int4 o[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int c = base + u * stride;
+      o[u] = c < n4 ? op4[c] : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if ((o[u].x | o[u].y | o[u].z | o[u].w) == 0) continue;
+      const int c = base + u * stride;
+      const int h0 = c * 4;
+      const int ov[4] = {o[u].x, o[u].y, o[u].z, o[u].w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (ov[k] == 0) continue;
+        float v;
+        const bool changed = apply_voxel(occ, hit_cnt, h0 + k, ov[k], p, &v);
+        if (kRecord && changed) {
+          record_modified(mod_count, mod_idx, mod_val, mod_cap, h0 + k, v);
+        }
+      }
+      op4[c] = make_int4(0, 0, 0, 0);
+    }
+  }
+  // Tail of V % 4 voxels.
+  const int tail = V - n4 * 4;
+  if (t < tail) {
+    const int h = n4 * 4 + t;
+    const int o = op_cnt[h];
+    if (o != 0) {
+      float v;
+      const bool changed = apply_voxel(occ, hit_cnt, h, o, p, &v);
+      if (kRecord && changed) {
+        record_modified(mod_count, mod_idx, mod_val, mod_cap, h, v);
+      }
+      op_cnt[h] = 0;
+    }
+  }
+}
+
+// ===================================================================
+// Index clear (reference clearVoxelsByIndexKernel, kernels.cu:273-284)
+// ===================================================================
+
+__global__ void __launch_bounds__(256)
+    clear_by_index_kernel(float *__restrict__ occ,
+                          const int *__restrict__ idx, const int n,
+                          const int V) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int h = idx[t];
+  if (h >= 0 && h < V) occ[h] = 0.0f;
+}
+
+// ===================================================================
+// Recenter slab reset (reference kernels.cu:404-499, re-laid out)
+//
+// The host turns the slice lists into at most a few "regions": a region is
+// n_outer runs of `run` contiguous voxels, `stride` apart, starting at
+// `base`. Merged x-slices are one contiguous range; y-slices are nx runs of
+// k*nz; z-slices are nx*ny runs of k. Regions travel by value.
+// ===================================================================
+
+struct ZeroRegion {
+  long long base;
+  long long n_outer;
+  long long stride;
+  long long run;
+};
+
+constexpr int kMaxRegions = 48;
+
+struct ZeroArgs {
+  ZeroRegion r[kMaxRegions];
+  int n;
+};
+
+template <typename T>
+__device__ __forceinline__ void zero_contiguous(T *__restrict__ p,
+                                               long long count, long long t,
+                                               long long stride) {
+  // Scalar head up to 16 B alignment, then 16 B stores, then scalar tail.
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(p);
+  long long head = static_cast<long long>(((16u - (addr & 15u)) & 15u) / 4u);
+  if (head > count) head = count;
+  const long long n4 = (count - head) / 4;
+  const long long tail0 = head + n4 * 4;
+  uint4 *__restrict__ p4 = reinterpret_cast<uint4 *>(p + head);
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (long long i = t; i < n4; i += stride) p4[i] = z;
+  if (t < head) p[t] = T(0);
+  if (t < count - tail0) p[tail0 + t] = T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_region(T *__restrict__ p,
+                                           const ZeroRegion &r, long long t,
+                                           long long stride) {
+  if (r.n_outer == 1) {
+    zero_contiguous(p + r.base, r.run, t, stride);
+    return;
+  }
+  const long long total = r.n_outer * r.run;
+  for (long long e = t; e < total; e += stride) {
+    const long long o = e / r.run;
+    const long long k = e - o * r.run;
+    p[r.base + o * r.stride + k] = T(0);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    zero_regions_kernel(float *__restrict__ occ, int *__restrict__ op_cnt,
+                        int *__restrict__ hit_cnt, const ZeroArgs z) {
+  const ZeroRegion r = z.r[blockIdx.y];
+  const long long t =
+      static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  zero_region(occ, r, t, stride);
+  if (op_cnt) zero_region(op_cnt, r, t, stride);
+  if (hit_cnt) zero_region(hit_cnt, r, t, stride);
+}
+
+// Reference-shaped slab clear from a DEVICE slice list: one block row per
+// slice, lanes along the contiguous z runs (x/y slabs) or across rows (z).
+__global__ void __launch_bounds__(256)
+    clear_slabs_list_kernel(float *__restrict__ occ, int *__restrict__ op_cnt,
+                            int *__restrict__ hit_cnt, const int mx,
+                            const int my, const int mz, const int hx,
+                            const int hy, const int hz,
+                            const int *__restrict__ slices, const int axis) {
+  const int l = slices[blockIdx.y];
+  long long base, n_outer, stride, run;
+  if (axis == 0) {
+    base = static_cast<long long>(l + hx) * my * mz;
+    n_outer = 1;
+    stride = 0;
+    run = static_cast<long long>(my) * mz;
+  } else if (axis == 1) {
+    base = static_cast<long long>(l + hy) * mz;
+    n_outer = mx;
+    stride = static_cast<long long>(my) * mz;
+    run = mz;
+  } else {
+    base = l + hz;
+    n_outer = static_cast<long long>(mx) * my;
+    stride = mz;
+    run = 1;
+  }
+  if (l < -(axis == 0 ? hx : axis == 1 ? hy : hz) ||
+      l > (axis == 0 ? hx : axis == 1 ? hy : hz)) {
+    return;
+  }
+  const ZeroRegion r{base, n_outer, stride, run};
+  const long long t =
+      static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long st = static_cast<long long>(gridDim.x) * blockDim.x;
+  zero_region(occ, r, t, st);
+  if (op_cnt) zero_region(op_cnt, r, t, st);
+  if (hit_cnt) zero_region(hit_cnt, r, t, st);
+}
+
+// ===================================================================
+// Sphere fetch (reference fetchOccupancySphereKernel, kernels.cu:506-562)
+//
+// Lanes run along z inside each (x, y) column of the disk so that occ reads
+// are contiguous; kept samples are compacted with one atomic per wavefront.
+// ===================================================================
+
+__device__ __forceinline__ void fetch_body(const cublox_fetch_cfg &c,
+                                           const float *__restrict__ occ,
+                                           unsigned *__restrict__ count,
+                                           float *__restrict__ out_pos,
+                                           unsigned char *__restrict__ out_state,
+                                           const unsigned cap,
+                                           const unsigned max_results,
+                                           const int block, const int nblocks) {
+  const int r = c.disk_radius_vox;
+  const int diam = 2 * r + 1;
+  const int nz = c.gz_max - c.gz_min + 1;
+  if (diam <= 0 || nz <= 0) return;
+  const long long total = static_cast<long long>(diam) * diam * nz;
+  const long long stride = static_cast<long long>(nblocks) * blockDim.x;
+  const bool ctr = c.origin_at_center != 0;
+  const int lane = __lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  // Wave-uniform trip count so that every lane reaches each ballot.
+  const long long first =
+      static_cast<long long>(block) * blockDim.x + (threadIdx.x & ~63);
+  for (long long w0 = first; w0 < total; w0 += stride) {
+    const long long e = w0 + lane;
+    bool keep = false;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    unsigned char st = 0;
+    if (e < total) {
+      const int tz = static_cast<int>(e % nz);
+      const long long rem = e / nz;
+      const int ty = static_cast<int>(rem % diam);
+      const int tx = static_cast<int>(rem / diam);
+      const int gx = c.center_g.x + tx - r;
+      const int gy = c.center_g.y + ty - r;
+      const int gz = c.gz_min + tz;
+      if (inside_window(gx, gy, gz, c.origin_i.x, c.origin_i.y, c.origin_i.z,
+                        c.half_map_size_i.x, c.half_map_size_i.y,
+                        c.half_map_size_i.z)) {
+        px = global_to_pos(gx, c.resolution, ctr);
+        py = global_to_pos(gy, c.resolution, ctr);
+        pz = global_to_pos(gz, c.resolution, ctr);
+        const float dx = px - c.center.x;
+        const float dy = py - c.center.y;
+        const float dz = pz - c.center.z;
+        if (!((dx * dx + dy * dy) + dz * dz > c.radius_sq)) {
+          const int h = global_to_hash(gx, gy, gz, c.map_size_i.x,
+                                       c.map_size_i.y, c.map_size_i.z,
+                                       c.half_map_size_i.x,
+                                       c.half_map_size_i.y,
+                                       c.half_map_size_i.z);
+          const float l = occ[h] + c.l_unknown;
+          st = 0;
+          if (l >= c.l_occupied) {
+            st = 2;
+          } else if (l < c.l_free) {
+            st = 1;
+          }
+          keep = c.state_filter < 0 || static_cast<int>(st) == c.state_filter;
+        }
+      }
+    }
+    const unsigned long long m = __ballot(keep);
+    if (m == 0ull) continue;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomic_add_u32_ret(count, __popcll(m));
+    base = __shfl(base, leader);
+    if (keep) {
+      const unsigned slot = base + __popcll(m & lt_mask);
+      if (!((max_results > 0u && slot >= max_results) || slot >= cap)) {
+        out_pos[3ull * slot + 0] = px;
+        out_pos[3ull * slot + 1] = py;
+        out_pos[3ull * slot + 2] = pz;
+        out_state[slot] = st;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    fetch_kernel(const cublox_fetch_cfg c, const float *__restrict__ occ,
+                 unsigned *__restrict__ count, float *__restrict__ out_pos,
+                 unsigned char *__restrict__ out_state, const unsigned cap,
+                 const unsigned max_results) {
+  fetch_body(c, occ, count, out_pos, out_state, cap, max_results, blockIdx.x,
+             gridDim.x);
+}
+
+__global__ void __launch_bounds__(256)
+    fetch_batch_kernel(const cublox_fetch_cfg *__restrict__ cfgs,
+                       const float *__restrict__ occ,
+                       unsigned *__restrict__ counts,
+                       float *__restrict__ out_pos,
+                       unsigned char *__restrict__ out_state,
+                       const unsigned per_query_cap) {
+  const int q = blockIdx.y;
+  const cublox_fetch_cfg c = cfgs[q];
+  const size_t off = static_cast<size_t>(q) * per_query_cap;
+  fetch_body(c, occ, counts + q, out_pos + 3 * off, out_state + off,
+             per_query_cap, 0u, blockIdx.x, gridDim.x);
+}
+
+// ===================================================================
+// Counter census: nnz(op), sum(op), sum(hit) with wave reductions.
+// ===================================================================
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+    census_kernel(const int *__restrict__ op, const int *__restrict__ hit,
+                  const int n, unsigned long long *__restrict__ out) {
+  unsigned long long nnz = 0, sop = 0, shit = 0, nhit = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const int o = op[i];
+    const int h = hit[i];
+    nnz += o != 0;
+    sop += static_cast<unsigned long long>(o);
+    shit += static_cast<unsigned long long>(h);
+    nhit += h != 0;
+  }
+  nnz = wave_sum_u64(nnz);
+  sop = wave_sum_u64(sop);
+  shit = wave_sum_u64(shit);
+  nhit = wave_sum_u64(nhit);
+  if (__lane_id() == 0) {
+    __hip_atomic_fetch_add(out + 0, nnz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(out + 1, sop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(out + 2, shit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(out + 3, nhit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ===================================================================
+// Host-side recenter plan (reference planRecenterKernel, kernels.cu:313-398)
+// ===================================================================
+
+// kernels.cu:303-311
+inline int normalize_slice(int x, int a, int b) {
+  const int range = b - a + 1;
+  int y = (x - a) % range;
+  if (y < 0) y += range;
+  return y + a;
+}
+
+struct HostPlan {
+  int action = 0;
+  int new_origin[3] = {0, 0, 0};
+  std::vector<int> slices[3];
+};
+
+HostPlan plan_recenter(const cublox_recenter_cfg &c) {
+  HostPlan p;
+  const float dx = c.pos.x - c.origin_f.x;
+  const float dy = c.pos.y - c.origin_f.y;
+  const float dz = c.pos.z - c.origin_f.z;
+  if ((dx * dx + dy * dy) + dz * dz < c.recenter_threshold_sq) return p;
+  const bool ctr = c.origin_at_center != 0;
+  const int no[3] = {pos_to_global(c.pos.x, c.inv_resolution, ctr),
+                     pos_to_global(c.pos.y, c.inv_resolution, ctr),
+                     pos_to_global(c.pos.z, c.inv_resolution, ctr)};
+  const int og[3] = {c.origin_i.x, c.origin_i.y, c.origin_i.z};
+  const int ms[3] = {c.map_size_i.x, c.map_size_i.y, c.map_size_i.z};
+  const int hs[3] = {c.half_map_size_i.x, c.half_map_size_i.y,
+                     c.half_map_size_i.z};
+  int max_shift = 0, max_map = 0;
+  for (int a = 0; a < 3; ++a) {
+    max_shift = std::max(max_shift, std::abs(no[a] - og[a]));
+    max_map = std::max(max_map, ms[a]);
+    p.new_origin[a] = no[a];
+  }
+  // kernels.cu:344-347: any shift >= the largest map extent resets all.
+  if (max_shift >= max_map) {
+    p.action = 2;
+    return p;
+  }
+  p.action = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int s = no[a] - og[a];
+    if (s == 0) continue;
+    const int h = hs[a];
+    const int min_l = (-h + og[a]) % ms[a];
+    std::vector<int> &out = p.slices[a];
+    if (s > 0) {
+      // Exits on the low side, then the fixed-local "entering" slices
+      // (kernels.cu:377-384), which do not depend on the origin.
+      for (int k = 0; k < s; ++k) out.push_back(normalize_slice(min_l + k, -h, h));
+      for (int k = 0; k < s; ++k) out.push_back(normalize_slice(h - k, -h, h));
+    } else {
+      for (int k = -1; k >= s; --k) out.push_back(normalize_slice(min_l + k, -h, h));
+      for (int k = 0; k < -s; ++k) out.push_back(normalize_slice(-h + k, -h, h));
+    }
+  }
+  return p;
+}
+
+// Sorted unique slice values, merged into inclusive [lo, hi] runs.
+std::vector<std::pair<int, int>> merge_runs(std::vector<int> v) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  std::vector<std::pair<int, int>> runs;
+  for (int x : v) {
+    if (!runs.empty() && runs.back().second + 1 == x) {
+      runs.back().second = x;
+    } else {
+      runs.emplace_back(x, x);
+    }
+  }
+  return runs;
+}
+
+int launch_regions(float *occ, int *op, int *hit,
+                   const std::vector<ZeroRegion> &regions, hipStream_t s) {
+  for (size_t i0 = 0; i0 < regions.size(); i0 += kMaxRegions) {
+    ZeroArgs z{};
+    z.n = static_cast<int>(std::min<size_t>(kMaxRegions, regions.size() - i0));
+    long long max_work = 0;
+    for (int k = 0; k < z.n; ++k) {
+      z.r[k] = regions[i0 + k];
+      const long long w = z.r[k].n_outer == 1 ? (z.r[k].run + 3) / 4
+                                              : z.r[k].n_outer * z.r[k].run;
+      max_work = std::max(max_work, w);
+    }
+    long long bx = (max_work + 255) / 256;
+    bx = std::max<long long>(1, std::min<long long>(bx, 2048));
+    zero_regions_kernel<<<dim3(static_cast<unsigned>(bx),
+                               static_cast<unsigned>(z.n)),
+                          dim3(256), 0, s>>>(occ, op, hit, z);
+    const int st = check_launch("zero_regions_kernel");
+    if (st != CUBLOX_OK) return st;
+  }
+  return CUBLOX_OK;
+}
+
+} // namespace cublox_hip
+
+using namespace cublox_hip;
+
+extern "C" {
+
+int cublox_launch_ray_cast_update(const cublox_raycast_cfg *cfg,
+                                  const float *d_x, const float *d_y,
+                                  const float *d_z, int n, cublox_stream s,
+                                  int *d_op_cnt, int *d_hit_cnt) {
+  if (!cfg || n <= 0 || !d_x || !d_y || !d_z || !d_op_cnt || !d_hit_cnt) {
+    return CUBLOX_OK;  // kernels.cu:223-225: no-op
+  }
+  const RayArgs a = make_ray_args(*cfg);
+  const int blocks = (n + 255) / 256;
+  ray_cast_kernel<false><<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(
+      a, d_x, d_y, d_z, n, d_op_cnt, d_hit_cnt);
+  return check_launch("ray_cast_kernel");
+}
+
+int cublox_launch_ray_cast_update_sensor(const cublox_raycast_cfg *cfg,
+                                         const float T[12], const float *d_xs,
+                                         const float *d_ys, const float *d_zs,
+                                         int n, cublox_stream s,
+                                         int *d_op_cnt, int *d_hit_cnt) {
+  if (!cfg || !T || n <= 0 || !d_xs || !d_ys || !d_zs || !d_op_cnt ||
+      !d_hit_cnt) {
+    return CUBLOX_OK;
+  }
+  RayArgs a = make_ray_args(*cfg);
+  for (int k = 0; k < 12; ++k) a.T[k] = T[k];
+  // The sensor origin is the transform's translation (main.cpp:453).
+  a.ox = T[3];
+  a.oy = T[7];
+  a.oz = T[11];
+  const int blocks = (n + 255) / 256;
+  ray_cast_kernel<true><<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(
+      a, d_xs, d_ys, d_zs, n, d_op_cnt, d_hit_cnt);
+  return check_launch("ray_cast_kernel<transform>");
+}
+
+int cublox_launch_apply_update(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
+                               int voxel_num, float l_hit, float l_miss,
+                               float l_min, float l_max, float l_unknown,
+                               cublox_stream s, unsigned int *d_mod_count,
+                               int *d_mod_idx, float *d_mod_val,
+                               unsigned int mod_cap) {
+  if (voxel_num <= 0 || !d_occ || !d_op_cnt || !d_hit_cnt) return CUBLOX_OK;
+  const ApplyArgs p{l_hit, l_miss, l_min, l_max, l_unknown};
+  // kernels.cu:255-262: record only when the buffers can hold every voxel.
+  const bool record = d_mod_count && d_mod_idx && d_mod_val &&
+                      static_cast<unsigned>(voxel_num) <= mod_cap;
+  const int n4 = voxel_num / 4;
+  int blocks = (n4 + 255) / 256;
+  blocks = std::max(1, std::min(blocks, 4096));
+  if (record) {
+    apply_kernel<true><<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(
+        d_occ, d_op_cnt, d_hit_cnt, voxel_num, p, d_mod_count, d_mod_idx,
+        d_mod_val, mod_cap);
+  } else {
+    apply_kernel<false><<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(
+        d_occ, d_op_cnt, d_hit_cnt, voxel_num, p, nullptr, nullptr, nullptr,
+        0u);
+  }
+  return check_launch("apply_kernel");
+}
+
+int cublox_launch_clear_voxels_by_index(float *d_occ, const int *d_indices,
+                                        int n, int voxel_num,
+                                        cublox_stream s) {
+  if (n <= 0 || !d_occ || !d_indices || voxel_num <= 0) return CUBLOX_OK;
+  clear_by_index_kernel<<<dim3((n + 255) / 256), dim3(256), 0,
+                          as_stream(s)>>>(d_occ, d_indices, n, voxel_num);
+  return check_launch("clear_by_index_kernel");
+}
+
+int cublox_plan_recenter(const cublox_recenter_cfg *cfg, int *slices_x,
+                         int *slices_y, int *slices_z,
+                         cublox_recenter_result *result) {
+  if (!cfg || !result) return set_error(CUBLOX_ERR_INVALID, "null argument");
+  const HostPlan p = plan_recenter(*cfg);
+  result->action = p.action;
+  result->new_origin_i = cublox_int3{p.new_origin[0], p.new_origin[1],
+                                     p.new_origin[2]};
+  int *outs[3] = {slices_x, slices_y, slices_z};
+  const long long ms[3] = {cfg->map_size_i.x, cfg->map_size_i.y,
+                           cfg->map_size_i.z};
+  long long keep = 1;
+  for (int a = 0; a < 3; ++a) {
+    result->n_slices[a] = static_cast<int>(p.slices[a].size());
+    if (outs[a]) std::copy(p.slices[a].begin(), p.slices[a].end(), outs[a]);
+    std::vector<int> u = p.slices[a];
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    keep *= ms[a] - static_cast<long long>(u.size());
+  }
+  const long long V = ms[0] * ms[1] * ms[2];
+  result->zeroed_voxels = p.action == 0 ? 0 : p.action == 2 ? V : V - keep;
+  return CUBLOX_OK;
+}
+
+int cublox_launch_recenter(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
+                           const cublox_recenter_cfg *cfg, cublox_stream s,
+                           cublox_recenter_result *result) {
+  if (!d_occ || !cfg || !result) return CUBLOX_OK;  // kernels.cu:633-636
+  const int st = cublox_plan_recenter(cfg, nullptr, nullptr, nullptr, result);
+  if (st != CUBLOX_OK) return st;
+  if (result->action == 0) return CUBLOX_OK;
+  const size_t V = static_cast<size_t>(cfg->map_vox_num);
+  hipStream_t hs = as_stream(s);
+  if (result->action == 2) {
+    // kernels.cu:652-663: full reset.
+    int r = cublox_fill_zero_async(d_occ, V, s);
+    if (r == CUBLOX_OK && d_op_cnt) r = cublox_fill_zero_async(d_op_cnt, V, s);
+    if (r == CUBLOX_OK && d_hit_cnt) r = cublox_fill_zero_async(d_hit_cnt, V, s);
+    return r;
+  }
+  const HostPlan p = plan_recenter(*cfg);
+  const long long mx = cfg->map_size_i.x, my = cfg->map_size_i.y,
+                  mz = cfg->map_size_i.z;
+  const int hx = cfg->half_map_size_i.x, hy = cfg->half_map_size_i.y,
+            hz = cfg->half_map_size_i.z;
+  std::vector<ZeroRegion> regions;
+  for (const auto &run : merge_runs(p.slices[0])) {
+    regions.push_back(ZeroRegion{(run.first + hx) * my * mz, 1, 0,
+                                 (run.second - run.first + 1) * my * mz});
+  }
+  for (const auto &run : merge_runs(p.slices[1])) {
+    regions.push_back(ZeroRegion{(run.first + hy) * mz, mx, my * mz,
+                                 (run.second - run.first + 1) * mz});
+  }
+  for (const auto &run : merge_runs(p.slices[2])) {
+    regions.push_back(ZeroRegion{run.first + hz, mx * my, mz,
+                                 static_cast<long long>(run.second - run.first + 1)});
+  }
+  return launch_regions(d_occ, d_op_cnt, d_hit_cnt, regions, hs);
+}
+
+int cublox_launch_clear_recenter_slabs_for_axis(
+    float *d_occ, int *d_op_cnt, int *d_hit_cnt, cublox_int3 ms,
+    cublox_int3 hs, const int *d_slices, int n_slices, int axis,
+    cublox_stream s) {
+  if (n_slices <= 0 || !d_occ || !d_slices) return CUBLOX_OK;  // :599-601
+  if (axis < 0 || axis > 2) return CUBLOX_OK;
+  const long long work = axis == 0   ? (static_cast<long long>(ms.y) * ms.z + 3) / 4
+                         : axis == 1 ? static_cast<long long>(ms.x) * ms.z
+                                     : static_cast<long long>(ms.x) * ms.y;
+  long long bx = (work + 255) / 256;
+  bx = std::max<long long>(1, std::min<long long>(bx, 1024));
+  clear_slabs_list_kernel<<<dim3(static_cast<unsigned>(bx),
+                                 static_cast<unsigned>(n_slices)),
+                            dim3(256), 0, as_stream(s)>>>(
+      d_occ, d_op_cnt, d_hit_cnt, ms.x, ms.y, ms.z, hs.x, hs.y, hs.z, d_slices,
+      axis);
+  return check_launch("clear_slabs_list_kernel");
+}
+
+int cublox_fetch_plan(cublox_int3 origin_i, cublox_int3 map_size_i,
+                      cublox_int3 half, float res, float inv_res, int center,
+                      float l_free, float l_occupied, float l_unknown,
+                      cublox_float3 c, float radius, int state_filter,
+                      int max_results, cublox_fetch_cfg *cfg,
+                      long long *est_voxels, unsigned int *output_cap) {
+  if (!cfg) return 0;
+  if (!(radius > 0.0f)) return 0;
+  const bool ctr = center != 0;
+  // Grid::maxHorizontalInWindowRadius (Grid.cpp:67-78)
+  const int cgx = pos_to_global(c.x, inv_res, ctr);
+  const int cgy = pos_to_global(c.y, inv_res, ctr);
+  const int cgz = pos_to_global(c.z, inv_res, ctr);
+  const int mgx = half.x - std::abs(cgx - origin_i.x);
+  const int mgy = half.y - std::abs(cgy - origin_i.y);
+  float margin = 0.0f;
+  if (mgx >= 0 && mgy >= 0) margin = static_cast<float>(std::min(mgx, mgy)) * res;
+  // OccupancyGrid.cpp:248-290
+  const float r = std::min(radius, margin);
+  if (!(r > 0.0f)) return 0;
+  int gmin[3] = {pos_to_global(c.x - r, inv_res, ctr),
+                 pos_to_global(c.y - r, inv_res, ctr),
+                 pos_to_global(c.z - r, inv_res, ctr)};
+  int gmax[3] = {pos_to_global(c.x + r, inv_res, ctr),
+                 pos_to_global(c.y + r, inv_res, ctr),
+                 pos_to_global(c.z + r, inv_res, ctr)};
+  const int o[3] = {origin_i.x, origin_i.y, origin_i.z};
+  const int h[3] = {half.x, half.y, half.z};
+  for (int a = 0; a < 3; ++a) {
+    gmin[a] = std::max(gmin[a], o[a] - h[a]);
+    gmax[a] = std::min(gmax[a], o[a] + h[a]);
+    if (gmax[a] < gmin[a]) return 0;
+  }
+  const int disk_r = static_cast<int>(std::ceil(static_cast<double>(r * inv_res)));
+  const int nz = gmax[2] - gmin[2] + 1;
+  const int diam = 2 * disk_r + 1;
+  const long long est = static_cast<long long>(diam) * diam * nz;
+  if (est <= 0 || est > static_cast<long long>(INT_MAX)) return 0;
+  cfg->center_g = cublox_int3{cgx, cgy, cgz};
+  cfg->origin_i = origin_i;
+  cfg->map_size_i = map_size_i;
+  cfg->half_map_size_i = half;
+  cfg->gz_min = gmin[2];
+  cfg->gz_max = gmax[2];
+  cfg->disk_radius_vox = disk_r;
+  cfg->center = c;
+  cfg->radius_sq = r * r;
+  cfg->resolution = res;
+  cfg->l_free = l_free;
+  cfg->l_occupied = l_occupied;
+  cfg->l_unknown = l_unknown;
+  cfg->origin_at_center = center;
+  cfg->state_filter = state_filter;
+  if (est_voxels) *est_voxels = est;
+  if (output_cap) {
+    *output_cap = max_results > 0 ? static_cast<unsigned>(max_results) : 500000u;
+  }
+  return 1;
+}
+
+int cublox_launch_fetch_occupancy_around(const cublox_fetch_cfg *cfg,
+                                         const float *d_occ,
+                                         unsigned int *d_count,
+                                         float *d_pos, unsigned char *d_state,
+                                         unsigned int cap,
+                                         unsigned int max_results,
+                                         cublox_stream s) {
+  if (!cfg || cfg->disk_radius_vox < 0 || cfg->gz_min > cfg->gz_max ||
+      !d_occ || !d_count || !d_pos || !d_state || cap == 0) {
+    return CUBLOX_OK;  // kernels.cu:570-574
+  }
+  const long long diam = 2LL * cfg->disk_radius_vox + 1;
+  const long long total = diam * diam * (cfg->gz_max - cfg->gz_min + 1);
+  long long blocks = (total + 255) / 256;
+  blocks = std::max<long long>(1, std::min<long long>(blocks, 4096));
+  fetch_kernel<<<dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                 as_stream(s)>>>(*cfg, d_occ, d_count, d_pos, d_state, cap,
+                                 max_results);
+  return check_launch("fetch_kernel");
+}
+
+int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
+                                        int n_queries, int max_disk_diam,
+                                        int max_nz, const float *d_occ,
+                                        unsigned int *d_counts,
+                                        float *d_pos,
+                                        unsigned char *d_state,
+                                        unsigned int per_query_cap,
+                                        cublox_stream s) {
+  if (n_queries <= 0 || !d_cfgs || !d_occ || !d_counts || !d_pos ||
+      !d_state || per_query_cap == 0 || max_disk_diam <= 0 || max_nz <= 0) {
+    return CUBLOX_OK;
+  }
+  if (n_queries > 65535) {
+    return set_error(CUBLOX_ERR_INVALID, "fetch batch: more than 65535 queries");
+  }
+  const long long total =
+      static_cast<long long>(max_disk_diam) * max_disk_diam * max_nz;
+  // Several waves per query; the kernel strides over the disk volume.
+  long long blocks = (total + 255) / 256;
+  blocks = std::max<long long>(1, std::min<long long>(blocks, 64));
+  fetch_batch_kernel<<<dim3(static_cast<unsigned>(blocks),
+                            static_cast<unsigned>(n_queries)),
+                       dim3(256), 0, as_stream(s)>>>(
+      d_cfgs, d_occ, d_counts, d_pos, d_state, per_query_cap);
+  return check_launch("fetch_batch_kernel");
+}
+
+int cublox_counter_census(const int *d_op, const int *d_hit, int n,
+                          unsigned long long *d_out, cublox_stream s) {
+  if (n <= 0 || !d_op || !d_hit || !d_out) return CUBLOX_OK;
+  int blocks = std::max(1, std::min((n + 255) / 256, 2048));
+  census_kernel<<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(d_op, d_hit, n, d_out);
+  return check_launch("census_kernel");
+}
+
+} // extern "C"

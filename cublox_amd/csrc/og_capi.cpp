@@ -1,0 +1,307 @@
+// Flat C binding of the drop-in cublox::OccupancyGrid for FFI callers
+// (Python ctypes in this package; cgo/JNI would bind the same symbols).
+// Every entry point catches C++ exceptions and returns a status; the message
+// of the last failure on the calling thread is in cublox_og_last_error().
+#include <cublox/OccupancyGrid.hpp>
+#include <cublox/capi.h>
+#include <cublox/utils.hpp>
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    return CUBLOX_OK;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return CUBLOX_ERR_RUNTIME;
+  } catch (...) {
+    g_err = "unknown C++ exception";
+    return CUBLOX_ERR_RUNTIME;
+  }
+}
+
+using OG = cublox::OccupancyGrid;
+
+Eigen::Vector3f v3(const float *p) { return Eigen::Vector3f(p[0], p[1], p[2]); }
+
+std::optional<OG::VoxelState> state_of(int filter) {
+  if (filter < 0) return std::nullopt;
+  return static_cast<OG::VoxelState>(filter);
+}
+} // namespace
+
+extern "C" {
+
+typedef struct cublox_og cublox_og;
+
+const char *cublox_og_last_error(void) { return g_err.c_str(); }
+
+cublox_og *cublox_og_create(int hx, int hy, int hz, float resolution,
+                            int origin_at_center, int has_threshold,
+                            double threshold, const float origin[3]) {
+  OG *g = nullptr;
+  const int st = guarded([&] {
+    g = new OG(Eigen::Vector3i(hx, hy, hz), resolution, origin_at_center != 0,
+               has_threshold ? std::optional<double>(threshold) : std::nullopt,
+               origin ? v3(origin) : Eigen::Vector3f::Zero());
+  });
+  if (st != CUBLOX_OK) return nullptr;
+  return reinterpret_cast<cublox_og *>(g);
+}
+
+void cublox_og_destroy(cublox_og *h) { delete reinterpret_cast<OG *>(h); }
+
+#define OGP reinterpret_cast<OG *>(h)
+#define OGC reinterpret_cast<const OG *>(h)
+
+int cublox_og_set_max_range(cublox_og *h, float r) {
+  return guarded([&] { OGP->setMaxRaycastRange(r); });
+}
+
+int cublox_og_set_log_odds(cublox_og *h, float l_hit, float l_miss,
+                           float l_min, float l_max, float l_free,
+                           float l_occupied) {
+  return guarded([&] {
+    OGP->setLogOddsParams(l_hit, l_miss, l_min, l_max, l_free, l_occupied);
+  });
+}
+
+int cublox_og_update(cublox_og *h, const float *x, const float *y,
+                     const float *z, int n, const float origin[3]) {
+  return guarded([&] { OGP->updateColumns(x, y, z, n, v3(origin)); });
+}
+
+int cublox_og_update_sensor(cublox_og *h, const float *xs, const float *ys,
+                            const float *zs, int n, const float T[12]) {
+  return guarded([&] { OGP->updateSensorFrameColumns(xs, ys, zs, n, T); });
+}
+
+int cublox_og_update_sensor_device(cublox_og *h, const float *d_xs,
+                                   const float *d_ys, const float *d_zs,
+                                   int n, const float T[12]) {
+  return guarded([&] { OGP->updateSensorFrameDevice(d_xs, d_ys, d_zs, n, T); });
+}
+
+int cublox_og_update_device(cublox_og *h, const float *d_x, const float *d_y,
+                            const float *d_z, int n, const float origin[3]) {
+  return guarded([&] { OGP->updateDevice(d_x, d_y, d_z, n, v3(origin)); });
+}
+
+int cublox_og_cast_rays_device(cublox_og *h, const float *d_x,
+                               const float *d_y, const float *d_z, int n,
+                               const float origin[3]) {
+  return guarded([&] { OGP->castRaysDevice(d_x, d_y, d_z, n, v3(origin)); });
+}
+
+int cublox_og_cast_rays_sensor_device(cublox_og *h, const float *d_xs,
+                                      const float *d_ys, const float *d_zs,
+                                      int n, const float T[12]) {
+  return guarded([&] { OGP->castRaysSensorFrameDevice(d_xs, d_ys, d_zs, n, T); });
+}
+
+int cublox_og_census(const cublox_og *h, unsigned long long out[4]) {
+  return guarded([&] { OGC->counterCensus(out); });
+}
+
+int cublox_og_apply_counters(cublox_og *h) {
+  return guarded([&] { OGP->applyCounters(); });
+}
+
+// *moved = 1 when the window origin changed.
+int cublox_og_recenter(cublox_og *h, const float pos[3], int *moved) {
+  return guarded([&] {
+    const Eigen::Vector3i before = OGP->getOriginIndex();
+    OGP->recenter(v3(pos));
+    if (moved) *moved = OGP->getOriginIndex() != before ? 1 : 0;
+  });
+}
+
+int cublox_og_fetch(const cublox_og *h, const float center[3], float radius,
+                    int filter, int max_results, float *out_pos,
+                    unsigned char *out_state, int cap, int *n_out) {
+  return guarded([&] {
+    const auto samples =
+        OGC->fetchOccupancyAround(v3(center), radius, state_of(filter), max_results);
+    const int n = static_cast<int>(samples.size());
+    for (int i = 0; i < n && i < cap; ++i) {
+      out_pos[3 * i + 0] = samples[i].position.x();
+      out_pos[3 * i + 1] = samples[i].position.y();
+      out_pos[3 * i + 2] = samples[i].position.z();
+      out_state[i] = static_cast<unsigned char>(samples[i].state);
+    }
+    *n_out = n;
+  });
+}
+
+// Outputs sized n_queries * per_query_cap (positions x3).
+int cublox_og_fetch_batch(const cublox_og *h, const float *centers,
+                          int n_queries, float radius, int filter,
+                          int per_query_cap, unsigned int *counts,
+                          float *out_pos, unsigned char *out_state) {
+  return guarded([&] {
+    std::vector<Eigen::Vector3f> c(static_cast<size_t>(n_queries));
+    for (int q = 0; q < n_queries; ++q) c[q] = v3(centers + 3 * q);
+    std::vector<unsigned int> cnt;
+    std::vector<float> pos;
+    std::vector<unsigned char> st;
+    OGC->fetchOccupancyBatch(c, radius, state_of(filter), per_query_cap, cnt,
+                             pos, st);
+    std::memcpy(counts, cnt.data(), sizeof(unsigned int) * cnt.size());
+    if (out_pos && !pos.empty()) std::memcpy(out_pos, pos.data(), sizeof(float) * pos.size());
+    if (out_state && !st.empty()) std::memcpy(out_state, st.data(), st.size());
+  });
+}
+
+int cublox_og_occ_to_host(const cublox_og *h, float *dst) {
+  return guarded([&] { OGC->copyOccupancyToHost(dst); });
+}
+
+int cublox_og_counters_to_host(const cublox_og *h, int *op, int *hit) {
+  return guarded([&] { OGC->copyCountersToHost(op, hit); });
+}
+
+int cublox_og_load_occ(cublox_og *h, const float *src) {
+  return guarded([&] { OGP->loadOccupancyFromHost(src); });
+}
+
+int cublox_og_reset(cublox_og *h) {
+  return guarded([&] { OGP->reset(); });
+}
+
+int cublox_og_reset_voxel(cublox_og *h, int hash_id) {
+  return guarded([&] { OGP->resetVoxel(hash_id); });
+}
+
+// Geometry: map[3], half[3], origin_i[3], origin_f[3], bound_min_i[3],
+// bound_max_i[3], voxel_num, resolution, resolution_inv.
+int cublox_og_geometry(const cublox_og *h, int map[3], int half[3],
+                       int origin_i[3], float origin_f[3], int bmin[3],
+                       int bmax[3], int *voxel_num, float *res,
+                       float *res_inv) {
+  return guarded([&] {
+    const OG &g = *OGC;
+    for (int a = 0; a < 3; ++a) {
+      map[a] = g.config_.map_size_i(a);
+      half[a] = g.config_.half_map_size_i(a);
+      origin_i[a] = g.origin_i_(a);
+      origin_f[a] = g.origin_f_(a);
+      bmin[a] = g.bound_min_i_(a);
+      bmax[a] = g.bound_max_i_(a);
+    }
+    *voxel_num = g.config_.voxel_num;
+    *res = g.config_.resolution;
+    *res_inv = g.config_.resolution_inv;
+  });
+}
+
+// kind: 0 occupied, 1 free, 2 unknown; by: 0 position, 1 global index,
+// 2 hash id (uses q[0]).
+int cublox_og_query(const cublox_og *h, int kind, int by, const float *qf,
+                    const int *qi, int *result) {
+  return guarded([&] {
+    const OG &g = *OGC;
+    bool r = false;
+    if (by == 0) {
+      const Eigen::Vector3f p = v3(qf);
+      r = kind == 0 ? g.isOccupied(p) : kind == 1 ? g.isFree(p) : g.isUnknown(p);
+    } else if (by == 1) {
+      const Eigen::Vector3i p(qi[0], qi[1], qi[2]);
+      r = kind == 0 ? g.isOccupied(p) : kind == 1 ? g.isFree(p) : g.isUnknown(p);
+    } else {
+      r = kind == 0 ? g.isOccupied(qi[0]) : kind == 1 ? g.isFree(qi[0]) : g.isUnknown(qi[0]);
+    }
+    *result = r ? 1 : 0;
+  });
+}
+
+int cublox_og_max_horizontal_radius(const cublox_og *h, const float c[3],
+                                    float *r) {
+  return guarded([&] { *r = OGC->maxHorizontalInWindowRadius(v3(c)); });
+}
+
+int cublox_og_enable_timing(cublox_og *h, int on) {
+  return guarded([&] { OGP->enableKernelTiming(on != 0); });
+}
+
+// out_ms[3] = ray cast, apply, recenter device time; out_n[4] = launches of
+// each, then voxels zeroed by recenters.
+int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
+                           long long out_n[4]) {
+  return guarded([&] {
+    const auto t = OGC->kernelTimes();
+    out_ms[0] = t.raycast_ms;
+    out_ms[1] = t.apply_ms;
+    out_ms[2] = t.recenter_ms;
+    out_n[0] = t.raycast_launches;
+    out_n[1] = t.apply_launches;
+    out_n[2] = t.recenter_launches;
+    out_n[3] = t.recenter_zeroed_voxels;
+  });
+}
+
+int cublox_og_reset_kernel_times(cublox_og *h) {
+  return guarded([&] { OGP->resetKernelTimes(); });
+}
+
+const float *cublox_og_device_occ(const cublox_og *h) {
+  return OGC->deviceOccupancy();
+}
+
+void *cublox_og_stream(const cublox_og *h) { return OGC->stream(); }
+
+} // extern "C"
+
+// Host index math of include/cublox/utils.hpp, exported for FFI checks.
+extern "C" {
+
+int cublox_utils_pos_to_global(float p, float inv_res, int center) {
+  return cublox::posToGlobalIndex(p, inv_res, center != 0);
+}
+float cublox_utils_global_to_pos(int g, float res, int center) {
+  return cublox::globalIndexToPos(g, res, center != 0);
+}
+int cublox_utils_global_to_local(int g, int map, int half) {
+  return cublox::globalIndexToLocalIndex(g, map, half);
+}
+int cublox_utils_local_to_global(int l, int map, int half, int origin_g) {
+  return cublox::localIndexToGlobalIndex(l, map, half, origin_g);
+}
+int cublox_utils_local_to_hash(const int l[3], const int map[3],
+                               const int half[3]) {
+  return cublox::localIndexToHashId(Eigen::Vector3i(l[0], l[1], l[2]),
+                                    Eigen::Vector3i(map[0], map[1], map[2]),
+                                    Eigen::Vector3i(half[0], half[1], half[2]));
+}
+void cublox_utils_hash_to_local(int h, const int map[3], const int half[3],
+                                int l[3]) {
+  Eigen::Vector3i out;
+  cublox::hashIdToLocalIndex(h, Eigen::Vector3i(map[0], map[1], map[2]),
+                             Eigen::Vector3i(half[0], half[1], half[2]), out);
+  for (int a = 0; a < 3; ++a) l[a] = out(a);
+}
+void cublox_utils_hash_to_pos(int h, const int map[3], const int half[3],
+                              const int origin[3], float res, int center,
+                              float pos[3]) {
+  Eigen::Vector3f out;
+  cublox::hashIdToPos(h, Eigen::Vector3i(map[0], map[1], map[2]),
+                      Eigen::Vector3i(half[0], half[1], half[2]),
+                      Eigen::Vector3i(origin[0], origin[1], origin[2]), res,
+                      center != 0, out);
+  for (int a = 0; a < 3; ++a) pos[a] = out(a);
+}
+int cublox_utils_pos_to_hash(const float pos[3], const int map[3],
+                             const int half[3], float inv_res, int center) {
+  return cublox::posToHashIndex(Eigen::Vector3f(pos[0], pos[1], pos[2]),
+                                Eigen::Vector3i(map[0], map[1], map[2]),
+                                Eigen::Vector3i(half[0], half[1], half[2]),
+                                inv_res, center != 0);
+}
+float cublox_logit(float p) { return cublox::logit(p); }
+
+} // extern "C"

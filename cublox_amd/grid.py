@@ -1,0 +1,298 @@
+"""Python mirror of the drop-in C++ `cublox::OccupancyGrid`.
+
+Each method calls the C++ class through its flat C binding (og_capi.cpp),
+which calls the HIP kernels through include/cublox/capi.h. Method names
+follow the reference API (include/cublox/OccupancyGrid.hpp:29-169 of the
+reference) in snake_case.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _ffi
+
+UNKNOWN, FREE, OCCUPIED = 0, 1, 2
+
+
+def logit(p: float) -> float:
+    """Probability -> log-odds in float32 (utils.hpp:185-188 of the reference)."""
+    return float(_ffi.host().cublox_logit(float(p)))
+
+
+@dataclass
+class Geometry:
+    map_size: Tuple[int, int, int]
+    half: Tuple[int, int, int]
+    origin_i: Tuple[int, int, int]
+    origin_f: Tuple[float, float, float]
+    bound_min_i: Tuple[int, int, int]
+    bound_max_i: Tuple[int, int, int]
+    voxel_num: int
+    resolution: float
+    resolution_inv: float
+
+
+def _f3(v) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(3))
+
+
+class OccupancyGrid:
+    def __init__(self, half_map_size: Sequence[int], resolution: float,
+                 origin_at_center: bool = False,
+                 recenter_threshold: Optional[float] = None,
+                 origin: Sequence[float] = (0.0, 0.0, 0.0)):
+        L = _ffi.host()
+        o = _f3(origin)
+        h = L.cublox_og_create(int(half_map_size[0]), int(half_map_size[1]),
+                               int(half_map_size[2]), float(resolution),
+                               1 if origin_at_center else 0,
+                               0 if recenter_threshold is None else 1,
+                               float(recenter_threshold or 0.0), _ffi.fptr(o))
+        if not h:
+            raise RuntimeError(L.cublox_og_last_error().decode())
+        self._h = h
+        self._L = L
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.cublox_og_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int) -> None:
+        if st != 0:
+            raise RuntimeError(self._L.cublox_og_last_error().decode())
+
+    # ---- configuration --------------------------------------------------
+    def set_max_raycast_range(self, r: float) -> None:
+        self._check(self._L.cublox_og_set_max_range(self._h, float(r)))
+
+    def set_log_odds_params(self, l_hit, l_miss, l_min, l_max, l_free,
+                            l_occupied) -> None:
+        self._check(self._L.cublox_og_set_log_odds(
+            self._h, float(l_hit), float(l_miss), float(l_min), float(l_max),
+            float(l_free), float(l_occupied)))
+
+    @property
+    def geometry(self) -> Geometry:
+        m, hf, oi, bmn, bmx = (np.zeros(3, np.int32) for _ in range(5))
+        of = np.zeros(3, np.float32)
+        V = C.c_int(0)
+        r = C.c_float(0)
+        ri = C.c_float(0)
+        self._check(self._L.cublox_og_geometry(
+            self._h, _ffi.iptr(m), _ffi.iptr(hf), _ffi.iptr(oi), _ffi.fptr(of),
+            _ffi.iptr(bmn), _ffi.iptr(bmx), C.byref(V), C.byref(r), C.byref(ri)))
+        return Geometry(tuple(m.tolist()), tuple(hf.tolist()), tuple(oi.tolist()),
+                        tuple(of.tolist()), tuple(bmn.tolist()),
+                        tuple(bmx.tolist()), V.value, r.value, ri.value)
+
+    @property
+    def voxel_num(self) -> int:
+        return self.geometry.voxel_num
+
+    # ---- update ---------------------------------------------------------
+    @staticmethod
+    def _columns(points) -> Tuple[np.ndarray, np.ndarray, np.ndarray, int]:
+        if isinstance(points, (tuple, list)) and len(points) == 3:
+            xs, ys, zs = (np.ascontiguousarray(np.asarray(c, np.float32))
+                          for c in points)
+        else:
+            p = np.asarray(points, dtype=np.float32)
+            if p.ndim != 2 or p.shape[1] != 3:
+                raise ValueError("points must be (N, 3) or a tuple of 3 columns")
+            xs, ys, zs = (np.ascontiguousarray(p[:, k]) for k in range(3))
+        if not (len(xs) == len(ys) == len(zs)):
+            raise ValueError("column lengths differ")
+        return xs, ys, zs, len(xs)
+
+    def update(self, points, sensor_origin) -> None:
+        """World-frame cloud from `sensor_origin` (OccupancyGrid::update)."""
+        xs, ys, zs, n = self._columns(points)
+        o = _f3(sensor_origin)
+        self._check(self._L.cublox_og_update(self._h, _ffi.fptr(xs), _ffi.fptr(ys),
+                                             _ffi.fptr(zs), n, _ffi.fptr(o)))
+
+    def update_sensor_frame(self, points, T) -> None:
+        """Sensor-frame cloud with pose T (3x4 [R|t]); transform fused."""
+        xs, ys, zs, n = self._columns(points)
+        Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+        self._check(self._L.cublox_og_update_sensor(
+            self._h, _ffi.fptr(xs), _ffi.fptr(ys), _ffi.fptr(zs), n, _ffi.fptr(Tm)))
+
+    def update_sensor_frame_device(self, d_xs: int, d_ys: int, d_zs: int,
+                                   n: int, T) -> None:
+        Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+        self._check(self._L.cublox_og_update_sensor_device(
+            self._h, d_xs, d_ys, d_zs, int(n), _ffi.fptr(Tm)))
+
+    def update_device(self, d_x: int, d_y: int, d_z: int, n: int,
+                      sensor_origin) -> None:
+        o = _f3(sensor_origin)
+        self._check(self._L.cublox_og_update_device(self._h, d_x, d_y, d_z,
+                                                    int(n), _ffi.fptr(o)))
+
+    def cast_rays_device(self, d_x: int, d_y: int, d_z: int, n: int,
+                         sensor_origin) -> None:
+        o = _f3(sensor_origin)
+        self._check(self._L.cublox_og_cast_rays_device(self._h, d_x, d_y, d_z,
+                                                       int(n), _ffi.fptr(o)))
+
+    def cast_rays_sensor_frame_device(self, d_xs: int, d_ys: int, d_zs: int,
+                                      n: int, T) -> None:
+        Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+        self._check(self._L.cublox_og_cast_rays_sensor_device(
+            self._h, d_xs, d_ys, d_zs, int(n), _ffi.fptr(Tm)))
+
+    def counter_census(self) -> dict:
+        """Pending-counter work: unique voxels, visits, hits, hit voxels."""
+        out = (C.c_ulonglong * 4)()
+        self._check(self._L.cublox_og_census(self._h, out))
+        return dict(touched=int(out[0]), visits=int(out[1]), hits=int(out[2]),
+                    hit_voxels=int(out[3]))
+
+    def apply_counters(self) -> None:
+        self._check(self._L.cublox_og_apply_counters(self._h))
+
+    # ---- recenter / reset -------------------------------------------------
+    def recenter(self, pos) -> bool:
+        p = _f3(pos)
+        moved = C.c_int(0)
+        self._check(self._L.cublox_og_recenter(self._h, _ffi.fptr(p), C.byref(moved)))
+        return bool(moved.value)
+
+    def reset(self) -> None:
+        self._check(self._L.cublox_og_reset(self._h))
+
+    def reset_voxel(self, hash_id: int) -> None:
+        self._check(self._L.cublox_og_reset_voxel(self._h, int(hash_id)))
+
+    # ---- reads ------------------------------------------------------------
+    def occupancy(self) -> np.ndarray:
+        out = np.empty(self.voxel_num, np.float32)
+        self._check(self._L.cublox_og_occ_to_host(self._h, _ffi.fptr(out)))
+        return out
+
+    def counters(self) -> Tuple[np.ndarray, np.ndarray]:
+        V = self.voxel_num
+        op = np.empty(V, np.int32)
+        hit = np.empty(V, np.int32)
+        self._check(self._L.cublox_og_counters_to_host(self._h, _ffi.iptr(op),
+                                                       _ffi.iptr(hit)))
+        return op, hit
+
+    def load_occupancy(self, occ: np.ndarray) -> None:
+        occ = np.ascontiguousarray(occ, np.float32)
+        assert occ.size == self.voxel_num
+        self._check(self._L.cublox_og_load_occ(self._h, _ffi.fptr(occ)))
+
+    def fetch_occupancy_around(self, center, radius: float,
+                               state_filter: Optional[int] = None,
+                               max_results: int = 0,
+                               cap: Optional[int] = None):
+        """Returns (positions (n,3) float32, states (n,) uint8)."""
+        c = _f3(center)
+        if cap is None:
+            r_vox = int(np.ceil(radius * self.geometry.resolution_inv)) + 1
+            est = (2 * r_vox + 1) ** 2 * (2 * r_vox + 1)
+            cap = min(est, max_results if max_results > 0 else 500000)
+        cap = max(int(cap), 1)
+        pos = np.empty((cap, 3), np.float32)
+        st = np.empty(cap, np.uint8)
+        n = C.c_int(0)
+        self._check(self._L.cublox_og_fetch(
+            self._h, _ffi.fptr(c), float(radius),
+            -1 if state_filter is None else int(state_filter), int(max_results),
+            _ffi.fptr(pos), _ffi.bptr(st), cap, C.byref(n)))
+        k = min(n.value, cap)
+        return pos[:k].copy(), st[:k].copy()
+
+    def fetch_occupancy_batch(self, centers, radius: float,
+                              state_filter: Optional[int], per_query_cap: int):
+        """Returns (counts (q,), positions (q, cap, 3), states (q, cap))."""
+        cs = np.ascontiguousarray(np.asarray(centers, np.float32).reshape(-1, 3))
+        q = cs.shape[0]
+        counts = np.zeros(q, np.uint32)
+        pos = np.zeros((q, per_query_cap, 3), np.float32)
+        st = np.zeros((q, per_query_cap), np.uint8)
+        self._check(self._L.cublox_og_fetch_batch(
+            self._h, _ffi.fptr(cs), q, float(radius),
+            -1 if state_filter is None else int(state_filter), int(per_query_cap),
+            _ffi.uptr(counts), _ffi.fptr(pos), _ffi.bptr(st)))
+        return counts, pos, st
+
+    def _query(self, kind: int, by: int, q) -> bool:
+        qf = np.zeros(3, np.float32)
+        qi = np.zeros(3, np.int32)
+        if by == 0:
+            qf[:] = np.asarray(q, np.float32).reshape(3)
+        elif by == 1:
+            qi[:] = np.asarray(q, np.int32).reshape(3)
+        else:
+            qi[0] = int(q)
+        r = C.c_int(0)
+        self._check(self._L.cublox_og_query(self._h, kind, by, _ffi.fptr(qf),
+                                            _ffi.iptr(qi), C.byref(r)))
+        return bool(r.value)
+
+    def is_occupied(self, pos=None, *, index=None, hash_id=None) -> bool:
+        return self._query(0, *self._which(pos, index, hash_id))
+
+    def is_free(self, pos=None, *, index=None, hash_id=None) -> bool:
+        return self._query(1, *self._which(pos, index, hash_id))
+
+    def is_unknown(self, pos=None, *, index=None, hash_id=None) -> bool:
+        return self._query(2, *self._which(pos, index, hash_id))
+
+    @staticmethod
+    def _which(pos, index, hash_id):
+        if pos is not None:
+            return 0, pos
+        if index is not None:
+            return 1, index
+        return 2, hash_id
+
+    def max_horizontal_in_window_radius(self, center) -> float:
+        c = _f3(center)
+        r = C.c_float(0)
+        self._check(self._L.cublox_og_max_horizontal_radius(self._h, _ffi.fptr(c),
+                                                            C.byref(r)))
+        return float(r.value)
+
+    def enable_kernel_timing(self, on: bool = True) -> None:
+        self._check(self._L.cublox_og_enable_timing(self._h, 1 if on else 0))
+
+    def reset_kernel_times(self) -> None:
+        self._check(self._L.cublox_og_reset_kernel_times(self._h))
+
+    def kernel_times(self) -> dict:
+        """Device ms per kernel (HIP events on this grid's stream)."""
+        ms = (C.c_double * 3)()
+        n = (C.c_longlong * 4)()
+        self._check(self._L.cublox_og_kernel_times(self._h, ms, n))
+        return dict(raycast_ms=ms[0], apply_ms=ms[1], recenter_ms=ms[2],
+                    raycast_launches=n[0], apply_launches=n[1],
+                    recenter_launches=n[2], recenter_zeroed_voxels=n[3])
+
+    @property
+    def device_occupancy_ptr(self) -> int:
+        return int(self._L.cublox_og_device_occ(self._h) or 0)
+
+    @property
+    def stream(self):
+        return self._L.cublox_og_stream(self._h)

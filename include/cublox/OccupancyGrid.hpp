@@ -1,0 +1,227 @@
+// GPU occupancy grid on MI355X (drop-in for the reference's
+// include/cublox/OccupancyGrid.hpp:29-169).
+//
+// The public interface is the reference's: update / recenter /
+// fetchOccupancyAround / point queries / log-odds setters, with the same
+// defaults and blocking semantics. Private members differ: buffers are
+// plain device pointers behind the C ABI in cublox/capi.h, every grid owns
+// its own HIP stream, and this header includes no HIP or CUDA header.
+//
+// Additions with no reference counterpart are marked "extension".
+#pragma once
+
+#include <Eigen/Dense>
+#include <optional>
+#include <vector>
+
+#include <cublox/Grid.hpp>
+
+namespace cublox {
+
+using PointCloud = Eigen::Matrix<float, Eigen::Dynamic, 3>;
+
+class OccupancyGrid : public Grid {
+public:
+  enum class VoxelState {
+    UNKNOWN = 0,
+    FREE = 1,
+    OCCUPIED = 2,
+  };
+
+  struct OccupancySample {
+    Eigen::Vector3f position;
+    VoxelState state;
+  };
+
+  OccupancyGrid(const Eigen::Vector3i &half_map_size_i, const float resolution,
+                const bool origin_at_center,
+                const std::optional<double> recenter_threshold,
+                const Eigen::Vector3f &origin = Eigen::Vector3f::Zero());
+  OccupancyGrid() = default;
+  ~OccupancyGrid() override;
+
+  // Owning device buffers: not copyable, movable.
+  OccupancyGrid(const OccupancyGrid &) = delete;
+  OccupancyGrid &operator=(const OccupancyGrid &) = delete;
+  OccupancyGrid(OccupancyGrid &&other) noexcept;
+  OccupancyGrid &operator=(OccupancyGrid &&other) noexcept;
+
+  void reset() override;
+  void resetVoxel(const int hash_id) override;
+
+  // One ray-cast pass of `input_cloud` (world frame, N x 3) from
+  // `sensor_origin`, then the clamped log-odds fold. Blocks until done.
+  void update(const PointCloud &input_cloud,
+              const Eigen::Vector3f &sensor_origin);
+
+  // Sliding-window recenter: host slab plan, one device launch. Blocks.
+  void recenter(const Eigen::Vector3f &pos);
+
+  // Voxels within `radius` [m] of `center`, optionally of one state.
+  // `max_results` caps the count (0 = the 500000-entry default cap).
+  std::vector<OccupancySample> fetchOccupancyAround(
+      const Eigen::Vector3f &center, const float radius,
+      const std::optional<VoxelState> state_filter = std::nullopt,
+      const int max_results = 0) const;
+
+  inline void setMaxRaycastRange(const float range) {
+    max_raycast_range_ = range;
+  }
+  inline float getMaxRaycastRange() const { return max_raycast_range_; }
+
+  void setLogOddsParams(const float l_hit, const float l_miss,
+                        const float l_min, const float l_max,
+                        const float l_free, const float l_occupied) {
+    l_hit_ = l_hit;
+    l_miss_ = l_miss;
+    l_min_ = l_min;
+    l_max_ = l_max;
+    l_free_ = l_free;
+    l_occupied_ = l_occupied;
+    l_unknown_ = 0.0f;
+  }
+
+  inline float getResolutionInv() const { return config_.resolution_inv; }
+  inline bool getOriginAtCenter() const { return config_.origin_at_center; }
+
+  bool isOccupied(const Eigen::Vector3f &pos) const;
+  bool isUnknown(const Eigen::Vector3f &pos) const;
+  bool isFree(const Eigen::Vector3f &pos) const;
+  bool isOccupied(const Eigen::Vector3i &id_g) const;
+  bool isUnknown(const Eigen::Vector3i &id_g) const;
+  bool isFree(const Eigen::Vector3i &id_g) const;
+  bool isOccupied(const int hash_id) const;
+  bool isUnknown(const int hash_id) const;
+  bool isFree(const int hash_id) const;
+
+  // ---- extensions -------------------------------------------------------
+  // update() with the three host columns given directly (no PointCloud).
+  void updateColumns(const float *x, const float *y, const float *z, int n,
+                     const Eigen::Vector3f &sensor_origin);
+  // updateSensorFrame() with host columns and T row-major 3x4 [R | t].
+  void updateSensorFrameColumns(const float *xs, const float *ys,
+                                const float *zs, int n, const float T[12]);
+  // Scan transform fused into the ray cast: `sensor_cloud` in the sensor
+  // frame, world pose [R | t]; non-finite points are skipped. Equivalent to
+  // update(R*p + t for finite p, t).
+  void updateSensorFrame(const PointCloud &sensor_cloud,
+                         const Eigen::Matrix3f &R, const Eigen::Vector3f &t);
+  // Same, with the sensor-frame SoA columns already in device memory and T
+  // row-major 3x4 [R | t]. Blocks.
+  void updateSensorFrameDevice(const float *d_xs, const float *d_ys,
+                               const float *d_zs, int n, const float T[12]);
+  // World-frame SoA columns already in device memory. Blocks.
+  void updateDevice(const float *d_x, const float *d_y, const float *d_z,
+                    int n, const Eigen::Vector3f &sensor_origin);
+  // The two halves of update, for stage-wise checks: castRays leaves the
+  // per-voxel counters populated, applyCounters folds and zeroes them.
+  void castRaysDevice(const float *d_x, const float *d_y, const float *d_z,
+                      int n, const Eigen::Vector3f &sensor_origin);
+  void castRaysSensorFrameDevice(const float *d_xs, const float *d_ys,
+                                 const float *d_zs, int n, const float T[12]);
+  void applyCounters();
+  // Work census of the pending counters: out = {voxels with op != 0,
+  // sum of op (voxel visits), sum of hit, voxels with hit != 0}.
+  void counterCensus(unsigned long long out[4]) const;
+
+  // Batched sphere fetch: one launch for all centres. Sample q*cap + j is
+  // the j-th sample of query q; counts[q] is the reference's returned count
+  // for that query (min(logical count, cap)). Order within a query is
+  // unspecified. Blocks.
+  void fetchOccupancyBatch(const std::vector<Eigen::Vector3f> &centers,
+                           const float radius,
+                           const std::optional<VoxelState> state_filter,
+                           const int per_query_cap,
+                           std::vector<unsigned int> &counts,
+                           std::vector<float> &positions,
+                           std::vector<unsigned char> &states) const;
+
+  // Per-kernel device time measured with HIP events on this grid's stream
+  // (off by default). Recenter also accumulates the voxels it zeroed.
+  struct KernelTimes {
+    double raycast_ms{0.0}, apply_ms{0.0}, recenter_ms{0.0};
+    long long raycast_launches{0}, apply_launches{0}, recenter_launches{0};
+    long long recenter_zeroed_voxels{0};
+  };
+  void enableKernelTiming(bool on);
+  KernelTimes kernelTimes() const { return times_; }
+  void resetKernelTimes() { times_ = KernelTimes{}; }
+
+  // Device views and host copies for integration and tests.
+  const float *deviceOccupancy() const { return d_occ_; }
+  void copyOccupancyToHost(float *dst) const;
+  void copyCountersToHost(int *op, int *hit) const;
+  void loadOccupancyFromHost(const float *src);
+  void *stream() const { return stream_; }
+  float getLogOddsFree() const { return l_free_; }
+  float getLogOddsOccupied() const { return l_occupied_; }
+
+private:
+  bool first_run_{true};
+
+  float storedToLogit(const float s) const noexcept { return s + l_unknown_; }
+  bool occOccupied(const float s) const noexcept {
+    return storedToLogit(s) >= l_occupied_;
+  }
+  bool occFree(const float s) const noexcept {
+    return storedToLogit(s) < l_free_;
+  }
+  bool occUnknown(const float s) const noexcept {
+    const float l = storedToLogit(s);
+    return l >= l_free_ && l < l_occupied_;
+  }
+
+  float storedAt(const int hash_id) const;
+  void ensureCloudCapacity(const int n);
+  void allocateVoxelBuffers();
+  void ensureFetchCapacity(const long long n) const;
+  void releaseAll() noexcept;
+  void pinFirstOrigin(const Eigen::Vector3f &sensor_origin);
+  void castRaysWorld(const float *d_x, const float *d_y, const float *d_z,
+                     int n, const Eigen::Vector3f &sensor_origin);
+  void castRaysSensor(const float *d_xs, const float *d_ys, const float *d_zs,
+                      int n, const float T[12]);
+  void launchApply();
+  void sync() const;
+  void mark(int ev) const;
+  void harvestTimes();
+
+  void *stream_{nullptr};
+
+  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter.
+  bool timing_{false};
+  void *events_[6]{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool pending_[3]{false, false, false};
+  long long pending_zeroed_{0};
+  KernelTimes times_{};
+
+  int *d_op_cnt_{nullptr};
+  int *d_hit_cnt_{nullptr};
+  float *d_occ_{nullptr};
+
+  float *d_cloud_x_{nullptr};
+  float *d_cloud_y_{nullptr};
+  float *d_cloud_z_{nullptr};
+  int cloud_capacity_{0};
+
+  mutable unsigned int *d_fetch_count_{nullptr};
+  mutable float *d_fetch_pos_{nullptr};
+  mutable unsigned char *d_fetch_state_{nullptr};
+  mutable void *d_fetch_cfgs_{nullptr};
+  mutable unsigned int *d_fetch_counts_{nullptr};
+  mutable long long fetch_capacity_{0};
+  mutable int fetch_cfg_capacity_{0};
+
+  float max_raycast_range_{25.0f};
+
+  // Defaults of the reference (OccupancyGrid.hpp:162-168).
+  float l_hit_{0.847f};
+  float l_miss_{-0.405f};
+  float l_min_{-1.992f};
+  float l_max_{3.476f};
+  float l_free_{-0.0004f};
+  float l_occupied_{1.7346f};
+  float l_unknown_{0.0f};
+};
+
+} // namespace cublox

@@ -1,0 +1,256 @@
+/*
+ * cublox C ABI — the host/device boundary of the MI355X occupancy path.
+ *
+ * This header replaces the reference launcher seam include/cublox/kernels.hpp
+ * (C++ linkage, CUDA vector types, cudaStream_t) with plain C: POD structs of
+ * float/int fields, an opaque stream handle, and an int status on every
+ * call. It includes no HIP or CUDA header, so host C++ built by g++ (or any
+ * FFI: ctypes, cgo, JNI) can bind it. All kernels are hand-written HIP for
+ * gfx950 in libcublox_hip.so.
+ *
+ * Conventions
+ *  - Launchers never allocate device memory; the caller owns every buffer
+ *    (kernels.hpp:52-53). They are asynchronous on `stream` (NULL = the
+ *    device's null stream) and enqueue nothing for empty work.
+ *  - Invalid arguments that the reference launchers silently ignore
+ *    (kernels.cu:223-225, 244-246, 288-290, 570-574, 599-601, 633-636) return
+ *    CUBLOX_OK with nothing enqueued, so the C++ layer keeps the reference's
+ *    no-op behaviour. A HIP runtime failure returns CUBLOX_ERR_RUNTIME and
+ *    cublox_last_error() holds the message for the calling thread.
+ *  - Per-launch configuration travels by value in kernel arguments, never
+ *    through module-global __constant__ symbols, so launches for different
+ *    grids on different streams do not race (reference kernels.cu:21, 301,
+ *    504 share one symbol per kernel).
+ */
+#ifndef CUBLOX_CAPI_H
+#define CUBLOX_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUBLOX_OK 0
+#define CUBLOX_ERR_INVALID 1
+#define CUBLOX_ERR_RUNTIME 2
+#define CUBLOX_ERR_CAPACITY 3
+
+typedef void *cublox_stream; /* hipStream_t; NULL selects the null stream */
+typedef void *cublox_event;  /* hipEvent_t */
+
+typedef struct {
+  float x, y, z;
+} cublox_float3;
+typedef struct {
+  int x, y, z;
+} cublox_int3;
+
+/* ------------------------------------------------------------------ */
+/* Errors and runtime primitives. The reference host library calls the CUDA
+ * runtime directly (OccupancyGrid.cpp:206-211, 319-345, 441-446); host code
+ * here must not see HIP headers, so these wrap the HIP runtime.           */
+/* ------------------------------------------------------------------ */
+const char *cublox_last_error(void);
+int cublox_device_count(int *count);
+int cublox_set_device(int device);
+int cublox_get_device(int *device);
+int cublox_device_name(char *buf, int cap);
+int cublox_mem_info(size_t *free_bytes, size_t *total_bytes);
+int cublox_malloc(void **ptr, size_t bytes);
+int cublox_free(void *ptr);
+int cublox_host_alloc(void **ptr, size_t bytes); /* pinned host memory */
+int cublox_host_free(void *ptr);
+int cublox_memset_async(void *ptr, int value, size_t bytes, cublox_stream s);
+int cublox_memcpy_h2d_async(void *dst, const void *src, size_t bytes,
+                            cublox_stream s);
+int cublox_memcpy_d2h_async(void *dst, const void *src, size_t bytes,
+                            cublox_stream s);
+int cublox_memcpy_d2d_async(void *dst, const void *src, size_t bytes,
+                            cublox_stream s);
+int cublox_stream_create(cublox_stream *s);
+int cublox_stream_destroy(cublox_stream s);
+int cublox_stream_synchronize(cublox_stream s);
+int cublox_device_synchronize(void);
+int cublox_event_create(cublox_event *e);
+int cublox_event_destroy(cublox_event e);
+int cublox_event_record(cublox_event e, cublox_stream s);
+int cublox_event_synchronize(cublox_event e);
+int cublox_event_elapsed_ms(cublox_event start, cublox_event stop, float *ms);
+/* Zero `count` floats/ints with a fill kernel (safe inside graph capture,
+ * unlike a memset node). */
+int cublox_fill_zero_async(void *ptr, size_t count32, cublox_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Ray cast. Replaces RayCastCfg (kernels.hpp:33-44).                   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  cublox_float3 origin;         /* sensor position, world frame */
+  cublox_int3 origin_i;         /* sliding-window origin, global voxel */
+  cublox_int3 map_size_i;       /* voxels per axis (2*half+1) */
+  cublox_int3 half_map_size_i;
+  float resolution;
+  float inv_resolution;
+  float max_range;              /* longer rays are clipped, free-space only */
+  int map_vox_num;
+  int origin_at_center;         /* 0 corner binning, 1 centre binning */
+} cublox_raycast_cfg;
+
+/* launchRayCastUpdate (kernels.hpp:54-57): world-frame SoA endpoints. One
+ * Amanatides-Woo walk per point; op_cnt[h] += 1 per traversed in-window
+ * voxel, endpoint op += 1 and hit += 1 unless range-clipped. */
+int cublox_launch_ray_cast_update(const cublox_raycast_cfg *cfg,
+                                  const float *d_x, const float *d_y,
+                                  const float *d_z, int n, cublox_stream s,
+                                  int *d_op_cnt, int *d_hit_cnt);
+
+/* Scan transform fused into the ray cast (reference: main.cpp:640-693 builds
+ * the world-frame cloud on the host, then kernels.hpp:54-57). Points are in
+ * the sensor frame; T is row-major 3x4 [R | t] sensor->world, evaluated as
+ * ((R0*x + R1*y) + R2*z) + t per row. Non-finite points are skipped, as the
+ * driver drops them (main.cpp:672-674). cfg->origin must equal t. */
+int cublox_launch_ray_cast_update_sensor(const cublox_raycast_cfg *cfg,
+                                         const float T[12], const float *d_xs,
+                                         const float *d_ys, const float *d_zs,
+                                         int n, cublox_stream s,
+                                         int *d_op_cnt, int *d_hit_cnt);
+
+/* ------------------------------------------------------------------ */
+/* Apply. launchApplyUpdate (kernels.hpp:69-76).                         */
+/* ------------------------------------------------------------------ */
+/* For every voxel with op != 0: stored = clamp(stored + l_unknown
+ * + l_miss*(op-hit) + l_hit*hit, l_min, l_max) - l_unknown, then zero both
+ * counters. If all three modified_* buffers are given and voxel_num <=
+ * modified_capacity, the (hash, new value) of every voxel whose stored value
+ * changed is appended (unordered) and *modified_count counts them; the
+ * caller zeroes *modified_count first. */
+int cublox_launch_apply_update(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
+                               int voxel_num, float l_hit, float l_miss,
+                               float l_min, float l_max, float l_unknown,
+                               cublox_stream s, unsigned int *d_modified_count,
+                               int *d_modified_idx, float *d_modified_val,
+                               unsigned int modified_capacity);
+
+/* launchClearVoxelsByIndex (kernels.hpp:79-80): occ[idx[i]] = 0 for valid
+ * indices. */
+int cublox_launch_clear_voxels_by_index(float *d_occ, const int *d_indices,
+                                        int n, int voxel_num, cublox_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Recenter. RecenterCfg / RecenterResult (kernels.hpp:83-100).          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  cublox_float3 pos;
+  cublox_float3 origin_f;
+  cublox_int3 origin_i;
+  cublox_int3 map_size_i;
+  cublox_int3 half_map_size_i;
+  float resolution;
+  float inv_resolution;
+  float recenter_threshold_sq;
+  int origin_at_center;
+  int map_vox_num;
+} cublox_recenter_cfg;
+
+typedef struct {
+  int action; /* 0 none, 1 clear slabs, 2 full reset */
+  cublox_int3 new_origin_i;
+  int n_slices[3];
+  long long zeroed_voxels; /* |union of cleared voxels| (0 for action 0) */
+} cublox_recenter_result;
+
+/* The slab plan of planRecenterKernel (kernels.cu:313-398), on the host: it
+ * is integer math on the config, so no device round trip is needed. Each
+ * slices_* array (may be NULL) receives the raw list for that axis in
+ * reference order and must hold 2*map_size_i.{x,y,z} ints. */
+int cublox_plan_recenter(const cublox_recenter_cfg *cfg, int *slices_x,
+                         int *slices_y, int *slices_z,
+                         cublox_recenter_result *result);
+
+/* launchRecenter (kernels.hpp:104-107): plan on the host, then one
+ * coalesced slab-zeroing launch (or a full reset for action 2). No device
+ * sync and no device->host copy. Counters are zeroed too when non-NULL. */
+int cublox_launch_recenter(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
+                           const cublox_recenter_cfg *cfg, cublox_stream s,
+                           cublox_recenter_result *result);
+
+/* launchClearRecenterSlabsForAxis (kernels.hpp:112-117): zero every voxel
+ * whose local index on `axis` is listed in the DEVICE array d_slices. */
+int cublox_launch_clear_recenter_slabs_for_axis(
+    float *d_occ, int *d_op_cnt, int *d_hit_cnt, cublox_int3 map_size_i,
+    cublox_int3 half_map_size_i, const int *d_slices, int n_slices, int axis,
+    cublox_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Sphere fetch. OccupancyFetchCfg (kernels.hpp:120-136).                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  cublox_int3 center_g;
+  cublox_int3 origin_i;
+  cublox_int3 map_size_i;
+  cublox_int3 half_map_size_i;
+  int gz_min, gz_max;   /* inclusive global z range */
+  int disk_radius_vox;  /* ceil(r/res) */
+  cublox_float3 center; /* world [m] */
+  float radius_sq;
+  float resolution;
+  float l_free;
+  float l_occupied;
+  float l_unknown;
+  int origin_at_center;
+  int state_filter; /* -1 all; 0 unknown; 1 free; 2 occupied */
+} cublox_fetch_cfg;
+
+/* Host half of OccupancyGrid::fetchOccupancyAround (OccupancyGrid.cpp:
+ * 244-317): clamps the radius to the horizontal window margin, clamps the
+ * AABB to the window and fills *cfg. Returns 1 when there is work, 0 when
+ * the reference would return an empty result. *est_voxels gets the disk
+ * volume; *output_cap the reference's buffer cap (max_results or 500000). */
+int cublox_fetch_plan(cublox_int3 origin_i, cublox_int3 map_size_i,
+                      cublox_int3 half_map_size_i, float resolution,
+                      float inv_resolution, int origin_at_center, float l_free,
+                      float l_occupied, float l_unknown, cublox_float3 center,
+                      float radius, int state_filter, int max_results,
+                      cublox_fetch_cfg *cfg, long long *est_voxels,
+                      unsigned int *output_cap);
+
+/* launchFetchOccupancyAround (kernels.hpp:142-147). out_pos holds 3 floats
+ * per sample. *d_out_count must be zeroed by the caller. Output order is
+ * unspecified (as in the reference). */
+int cublox_launch_fetch_occupancy_around(const cublox_fetch_cfg *cfg,
+                                         const float *d_occ,
+                                         unsigned int *d_out_count,
+                                         float *d_out_pos,
+                                         unsigned char *d_out_state,
+                                         unsigned int output_capacity,
+                                         unsigned int max_results,
+                                         cublox_stream s);
+
+/* Batched fetch: n_queries configs in DEVICE memory, one launch. Query q
+ * writes at most per_query_cap samples at [q*per_query_cap, ...) and its
+ * logical count to d_counts[q] (zeroed by the caller). */
+int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
+                                        int n_queries, int max_disk_diam,
+                                        int max_nz, const float *d_occ,
+                                        unsigned int *d_counts,
+                                        float *d_out_pos,
+                                        unsigned char *d_out_state,
+                                        unsigned int per_query_cap,
+                                        cublox_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Counter census (no reference counterpart): over n voxels, adds to
+ * d_out[0] the number of voxels with op != 0, to d_out[1] the sum of op, to
+ * d_out[2] the sum of hit and to d_out[3] the number of voxels with
+ * hit != 0. The caller zeroes d_out[0..3]. Used to account the algorithmic
+ * work of a ray cast (voxel visits, unique voxels touched).              */
+/* ------------------------------------------------------------------ */
+int cublox_counter_census(const int *d_op_cnt, const int *d_hit_cnt, int n,
+                          unsigned long long *d_out, cublox_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBLOX_CAPI_H */
