@@ -62,15 +62,16 @@ def upload(cb, scans):
     return bufs, n
 
 
-def new_grid(cb, w):
+def new_grid(cb, w, path=0):
     g = cb.OccupancyGrid(w.half, w.resolution, w.origin_at_center, w.threshold)
     g.set_max_raycast_range(w.max_range)
     g.set_log_odds_params(*cb.log_odds())
+    g.set_update_path(path)
     return g
 
 
-def run_gpu(cb, w, scans, bufs, n, warmup, steps):
-    g = new_grid(cb, w)
+def run_gpu(cb, w, scans, bufs, n, warmup, steps, path=0):
+    g = new_grid(cb, w, path)
     g.enable_kernel_timing(True)
 
     def step(i):
@@ -228,14 +229,14 @@ def main():
         visits = statistics.mean(c["visits"] for c in cen)
         Uh = statistics.mean(c["hit_voxels"] for c in cen)
         V = w.voxels
-        rc_ms = kt["raycast_ms"] / max(kt["raycast_launches"], 1)
-        ap_ms = kt["apply_ms"] / max(kt["apply_launches"], 1)
+        up_ms = kt["update_ms"] / max(kt["update_launches"], 1)
         re_n = max(kt["recenter_launches"], 1)
         re_ms = kt["recenter_ms"] / re_n
         re_vox = kt["recenter_zeroed_voxels"] / re_n
-        # Algorithmic bytes per launch.
-        rc_bytes = 12.0 * n + 8.0 * U + 8.0 * Uh
-        ap_bytes = 4.0 * V + 16.0 * U + 4.0 * Uh
+        # Algorithmic bytes per launch: the fused update reads the points
+        # once and reads + writes each touched voxel's log-odds once; the
+        # recenter writes each zeroed voxel once.
+        up_bytes = 12.0 * n + 8.0 * U
         re_bytes = 4.0 * re_vox
 
         def rl(bytes_, ms):
@@ -245,17 +246,23 @@ def main():
                     "traffic": None, "bytes_per_launch": int(bytes_),
                     "avg_launch_ms": round(ms, 5)}
 
-        rooflines = {"ray_cast": rl(rc_bytes, rc_ms), "apply": rl(ap_bytes, ap_ms),
-                     "recenter": rl(re_bytes, re_ms)}
-        rooflines["ray_cast"]["visits_per_s"] = visits / (rc_ms * 1e-3) if rc_ms else 0
-        totals = {"ray_cast": kt["raycast_ms"], "apply": kt["apply_ms"],
-                  "recenter": kt["recenter_ms"]}
+        rooflines = {"update": rl(up_bytes, up_ms), "recenter": rl(re_bytes, re_ms)}
+        rooflines["update"]["visits_per_s"] = visits / (up_ms * 1e-3) if up_ms else 0
+        totals = {"update": kt["update_ms"], "recenter": kt["recenter_ms"]}
         dom = max(totals, key=totals.get)
         roofline = dict(rooflines[dom], kernel=dom)
         extras["census"] = dict(visits_per_scan=visits, unique_voxels_per_scan=U,
                                 hit_voxels_per_scan=Uh,
                                 recenter_events=kt["recenter_launches"],
                                 zeroed_voxels_per_event=re_vox)
+        gl, per_l, wall_l, kt_l = run_gpu(cb, w, scans, bufs, n, args.warmup,
+                                          args.steps, path=1)
+        gl.close()
+        extras["per_voxel_atomics_path"] = dict(
+            ms_per_scan=wall_l / args.steps,
+            raycast_ms=kt_l["update_ms"] / max(kt_l["update_launches"], 1),
+            apply_ms=kt_l["apply_ms"] / max(kt_l["apply_launches"], 1))
+        log(f"per-voxel atomics path: {extras['per_voxel_atomics_path']}")
         pcie = run_pcie(cb, w, scans, args.warmup, args.steps)
         extras["pcie_inclusive_ms_per_scan"] = statistics.mean(pcie)
         extras["fetch"] = run_fetch(cb, w, g, scene, total - 1)

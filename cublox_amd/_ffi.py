@@ -151,6 +151,11 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_launch_fetch_occupancy_batch", i, vp, i, i, i, vp, vp, vp,
          vp, C.c_uint, vp)
     _sig(L, "cublox_counter_census", i, vp, vp, i, vp, vp)
+    _sig(L, "cublox_integrate_workspace_bytes", S, C.POINTER(RaycastCfg), i, i)
+    _sig(L, "cublox_integrate_workspace_init", i, vp, S, vp)
+    _sig(L, "cublox_launch_integrate", i, C.POINTER(RaycastCfg), f32p, vp, vp,
+         vp, i, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, vp,
+         S, i, i, vp, vp, vp)
     _hip = L
     return L
 
@@ -177,6 +182,9 @@ def host() -> C.CDLL:
     _sig(L, "cublox_og_apply_counters", i, vp)
     _sig(L, "cublox_og_cast_rays_sensor_device", i, vp, vp, vp, vp, i, f32p)
     _sig(L, "cublox_og_census", i, vp, C.POINTER(C.c_ulonglong))
+    _sig(L, "cublox_og_set_update_path", i, vp, i)
+    _sig(L, "cublox_og_set_near_steps", i, vp, i)
+    _sig(L, "cublox_og_cast_rays_fused_sensor_device", i, vp, vp, vp, vp, i, f32p)
     _sig(L, "cublox_og_recenter", i, vp, f32p, i32p)
     _sig(L, "cublox_og_fetch", i, vp, f32p, F, i, i, f32p, u8p, i, i32p)
     _sig(L, "cublox_og_fetch_batch", i, vp, f32p, i, F, i, i, u32p, f32p, u8p)

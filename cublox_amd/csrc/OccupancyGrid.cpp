@@ -97,6 +97,13 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   for (int k = 0; k < 3; ++k) pending_[k] = std::exchange(o.pending_[k], false);
   pending_zeroed_ = std::exchange(o.pending_zeroed_, 0);
   times_ = o.times_;
+  update_path_ = o.update_path_;
+  near_steps_ = o.near_steps_;
+  d_ws_ = std::exchange(o.d_ws_, nullptr);
+  ws_bytes_ = std::exchange(o.ws_bytes_, 0);
+  ws_points_ = std::exchange(o.ws_points_, 0);
+  ws_range_ = o.ws_range_;
+  ws_near_ = o.ws_near_;
   max_raycast_range_ = o.max_raycast_range_;
   l_hit_ = o.l_hit_;
   l_miss_ = o.l_miss_;
@@ -121,6 +128,9 @@ void OccupancyGrid::releaseAll() noexcept {
   freeDevice(d_cloud_x_);
   freeDevice(d_cloud_y_);
   freeDevice(d_cloud_z_);
+  freeDevice(d_ws_);
+  ws_bytes_ = 0;
+  ws_points_ = 0;
   cloud_capacity_ = 0;
   fetch_capacity_ = 0;
   fetch_cfg_capacity_ = 0;
@@ -151,8 +161,8 @@ void OccupancyGrid::mark(int ev) const {
 // Called after a stream sync: fold the completed event pairs into times_.
 void OccupancyGrid::harvestTimes() {
   if (!timing_) return;
-  double *acc[3] = {&times_.raycast_ms, &times_.apply_ms, &times_.recenter_ms};
-  long long *cnt[3] = {&times_.raycast_launches, &times_.apply_launches,
+  double *acc[3] = {&times_.update_ms, &times_.apply_ms, &times_.recenter_ms};
+  long long *cnt[3] = {&times_.update_launches, &times_.apply_launches,
                        &times_.recenter_launches};
   for (int k = 0; k < 3; ++k) {
     if (!pending_[k]) continue;
@@ -168,18 +178,16 @@ void OccupancyGrid::harvestTimes() {
 
 void OccupancyGrid::sync() const { CUBLOX_CHECK(cublox_stream_synchronize(stream_)); }
 
-// OccupancyGrid.cpp:431-480: three dense arrays of V, zeroed; partial
-// allocations are rolled back before throwing.
+// OccupancyGrid.cpp:431-480: the occupancy array of V, zeroed; partial
+// allocations are rolled back before throwing. The per-voxel op/hit counters
+// of the reference are only needed by the per-voxel-atomics path and the
+// stage-wise checks, so they are allocated on first use (ensureCounters).
 void OccupancyGrid::allocateVoxelBuffers() {
   const size_t n = static_cast<size_t>(config_.voxel_num);
   if (n == 0) return;
   try {
     CUBLOX_CHECK(cublox_stream_create(&stream_));
-    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_op_cnt_), n * sizeof(int)));
-    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_hit_cnt_), n * sizeof(int)));
     CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_occ_), n * sizeof(float)));
-    CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, n, stream_));
-    CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, n, stream_));
     CUBLOX_CHECK(cublox_fill_zero_async(d_occ_, n, stream_));
     sync();
   } catch (...) {
@@ -193,8 +201,8 @@ void OccupancyGrid::reset() {
   if (d_occ_ == nullptr) return;
   const size_t n = static_cast<size_t>(config_.voxel_num);
   CUBLOX_CHECK(cublox_fill_zero_async(d_occ_, n, stream_));
-  CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, n, stream_));
-  CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, n, stream_));
+  if (d_op_cnt_) CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, n, stream_));
+  if (d_hit_cnt_) CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, n, stream_));
   sync();
   first_run_ = true;
 }
@@ -202,8 +210,10 @@ void OccupancyGrid::reset() {
 void OccupancyGrid::resetVoxel(const int hash_id) {
   if (d_occ_ == nullptr || hash_id < 0 || hash_id >= config_.voxel_num) return;
   CUBLOX_CHECK(cublox_memset_async(d_occ_ + hash_id, 0, sizeof(float), stream_));
-  CUBLOX_CHECK(cublox_memset_async(d_op_cnt_ + hash_id, 0, sizeof(int), stream_));
-  CUBLOX_CHECK(cublox_memset_async(d_hit_cnt_ + hash_id, 0, sizeof(int), stream_));
+  if (d_op_cnt_) {
+    CUBLOX_CHECK(cublox_memset_async(d_op_cnt_ + hash_id, 0, sizeof(int), stream_));
+    CUBLOX_CHECK(cublox_memset_async(d_hit_cnt_ + hash_id, 0, sizeof(int), stream_));
+  }
   sync();
 }
 
@@ -253,9 +263,85 @@ void OccupancyGrid::pinFirstOrigin(const Eigen::Vector3f &sensor_origin) {
   first_run_ = false;
 }
 
+void OccupancyGrid::ensureCounters() {
+  if (d_op_cnt_) return;
+  const size_t n = static_cast<size_t>(config_.voxel_num);
+  try {
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_op_cnt_), n * sizeof(int)));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_hit_cnt_), n * sizeof(int)));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, n, stream_));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, n, stream_));
+  } catch (...) {
+    freeDevice(d_op_cnt_);
+    freeDevice(d_hit_cnt_);
+    throw;
+  }
+}
+
+void OccupancyGrid::setNearSteps(int k) {
+  if (k < 1 || k > 64) {
+    throw std::invalid_argument("cublox::OccupancyGrid: near steps must be in [1, 64]");
+  }
+  near_steps_ = k;
+}
+
+void OccupancyGrid::ensureWorkspace(int n) {
+  cublox_raycast_cfg cfg{};
+  cfg.map_size_i = i3(config_.map_size_i);
+  cfg.half_map_size_i = i3(config_.half_map_size_i);
+  cfg.resolution = config_.resolution;
+  cfg.inv_resolution = config_.resolution_inv;
+  cfg.max_range = max_raycast_range_;
+  cfg.map_vox_num = config_.voxel_num;
+  if (d_ws_ && n <= ws_points_ && ws_range_ == max_raycast_range_ &&
+      ws_near_ == near_steps_) {
+    return;
+  }
+  const int pts = std::max(n, ws_points_);
+  const size_t bytes = cublox_integrate_workspace_bytes(&cfg, pts, near_steps_);
+  sync();
+  freeDevice(d_ws_);
+  ws_bytes_ = 0;
+  ws_points_ = 0;
+  CUBLOX_CHECK(cublox_malloc(&d_ws_, bytes));
+  CUBLOX_CHECK(cublox_integrate_workspace_init(d_ws_, bytes, stream_));
+  ws_bytes_ = bytes;
+  ws_points_ = pts;
+  ws_range_ = max_raycast_range_;
+  ws_near_ = near_steps_;
+}
+
+// Fused ray cast + fold (or count export) on the grid's stream.
+void OccupancyGrid::launchIntegrate(const float *d_xs, const float *d_ys,
+                                    const float *d_zs, int n, const float *T,
+                                    const Eigen::Vector3f &origin,
+                                    bool export_counts) {
+  ensureWorkspace(n);
+  if (export_counts) ensureCounters();
+  cublox_raycast_cfg cfg{};
+  cfg.origin = f3(origin);
+  cfg.origin_i = i3(origin_i_);
+  cfg.map_size_i = i3(config_.map_size_i);
+  cfg.half_map_size_i = i3(config_.half_map_size_i);
+  cfg.resolution = config_.resolution;
+  cfg.inv_resolution = config_.resolution_inv;
+  cfg.max_range = max_raycast_range_;
+  cfg.map_vox_num = config_.voxel_num;
+  cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
+  mark(0);
+  CUBLOX_CHECK(cublox_launch_integrate(
+      &cfg, T, d_xs, d_ys, d_zs, n, d_occ_, l_hit_, l_miss_, l_min_, l_max_,
+      l_unknown_, d_ws_, ws_bytes_, ws_points_, near_steps_,
+      export_counts ? d_op_cnt_ : nullptr, export_counts ? d_hit_cnt_ : nullptr,
+      stream_));
+  mark(1);
+  pending_[0] = timing_;
+}
+
 void OccupancyGrid::castRaysWorld(const float *d_x, const float *d_y,
                                   const float *d_z, int n,
                                   const Eigen::Vector3f &sensor_origin) {
+  ensureCounters();
   cublox_raycast_cfg cfg{};
   cfg.origin = f3(sensor_origin);
   cfg.origin_i = i3(origin_i_);
@@ -276,6 +362,7 @@ void OccupancyGrid::castRaysWorld(const float *d_x, const float *d_y,
 void OccupancyGrid::castRaysSensor(const float *d_xs, const float *d_ys,
                                    const float *d_zs, int n,
                                    const float T[12]) {
+  ensureCounters();
   cublox_raycast_cfg cfg{};
   cfg.origin = cublox_float3{T[3], T[7], T[11]};
   cfg.origin_i = i3(origin_i_);
@@ -295,6 +382,7 @@ void OccupancyGrid::castRaysSensor(const float *d_xs, const float *d_ys,
 }
 
 void OccupancyGrid::launchApply() {
+  ensureCounters();
   mark(2);
   CUBLOX_CHECK(cublox_launch_apply_update(d_occ_, d_op_cnt_, d_hit_cnt_,
                                           config_.voxel_num, l_hit_, l_miss_,
@@ -337,8 +425,13 @@ void OccupancyGrid::updateColumns(const float *x, const float *y,
   CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_x_, x, bytes, stream_));
   CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_y_, y, bytes, stream_));
   CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_z_, z, bytes, stream_));
-  castRaysWorld(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, sensor_origin);
-  launchApply();
+  if (update_path_ == UpdatePath::FUSED_TILES) {
+    launchIntegrate(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, nullptr,
+                    sensor_origin, false);
+  } else {
+    castRaysWorld(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, sensor_origin);
+    launchApply();
+  }
   sync();
   harvestTimes();
 }
@@ -366,8 +459,13 @@ void OccupancyGrid::updateSensorFrameColumns(const float *xs, const float *ys,
   CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_x_, xs, bytes, stream_));
   CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_y_, ys, bytes, stream_));
   CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_z_, zs, bytes, stream_));
-  castRaysSensor(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T);
-  launchApply();
+  if (update_path_ == UpdatePath::FUSED_TILES) {
+    launchIntegrate(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T,
+                    Eigen::Vector3f(T[3], T[7], T[11]), false);
+  } else {
+    castRaysSensor(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T);
+    launchApply();
+  }
   sync();
   harvestTimes();
 }
@@ -378,8 +476,13 @@ void OccupancyGrid::updateSensorFrameDevice(const float *d_xs,
                                             const float T[12]) {
   if (config_.voxel_num <= 0 || n <= 0) return;
   pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
-  castRaysSensor(d_xs, d_ys, d_zs, n, T);
-  launchApply();
+  if (update_path_ == UpdatePath::FUSED_TILES) {
+    launchIntegrate(d_xs, d_ys, d_zs, n, T, Eigen::Vector3f(T[3], T[7], T[11]),
+                    false);
+  } else {
+    castRaysSensor(d_xs, d_ys, d_zs, n, T);
+    launchApply();
+  }
   sync();
   harvestTimes();
 }
@@ -389,8 +492,12 @@ void OccupancyGrid::updateDevice(const float *d_x, const float *d_y,
                                  const Eigen::Vector3f &sensor_origin) {
   if (config_.voxel_num <= 0 || n <= 0) return;
   pinFirstOrigin(sensor_origin);
-  castRaysWorld(d_x, d_y, d_z, n, sensor_origin);
-  launchApply();
+  if (update_path_ == UpdatePath::FUSED_TILES) {
+    launchIntegrate(d_x, d_y, d_z, n, nullptr, sensor_origin, false);
+  } else {
+    castRaysWorld(d_x, d_y, d_z, n, sensor_origin);
+    launchApply();
+  }
   sync();
   harvestTimes();
 }
@@ -416,9 +523,21 @@ void OccupancyGrid::castRaysSensorFrameDevice(const float *d_xs,
   harvestTimes();
 }
 
+void OccupancyGrid::castRaysFusedSensorFrameDevice(const float *d_xs,
+                                                   const float *d_ys,
+                                                   const float *d_zs, int n,
+                                                   const float T[12]) {
+  if (config_.voxel_num <= 0 || n <= 0) return;
+  pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
+  launchIntegrate(d_xs, d_ys, d_zs, n, T, Eigen::Vector3f(T[3], T[7], T[11]),
+                  true);
+  sync();
+  harvestTimes();
+}
+
 void OccupancyGrid::counterCensus(unsigned long long out[4]) const {
   for (int k = 0; k < 4; ++k) out[k] = 0;
-  if (!d_occ_) return;
+  if (!d_occ_ || !d_op_cnt_) return;
   unsigned long long *d = nullptr;
   CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d), 4 * sizeof(unsigned long long)));
   int st = cublox_memset_async(d, 0, 4 * sizeof(unsigned long long), stream_);
@@ -432,7 +551,7 @@ void OccupancyGrid::counterCensus(unsigned long long out[4]) const {
 }
 
 void OccupancyGrid::applyCounters() {
-  if (config_.voxel_num <= 0) return;
+  if (config_.voxel_num <= 0 || !d_op_cnt_) return;
   launchApply();
   sync();
   harvestTimes();
@@ -646,6 +765,11 @@ void OccupancyGrid::copyOccupancyToHost(float *dst) const {
 void OccupancyGrid::copyCountersToHost(int *op, int *hit) const {
   if (!d_occ_) return;
   const size_t bytes = static_cast<size_t>(config_.voxel_num) * sizeof(int);
+  if (!d_op_cnt_) {
+    if (op) std::fill(op, op + config_.voxel_num, 0);
+    if (hit) std::fill(hit, hit + config_.voxel_num, 0);
+    return;
+  }
   if (op) CUBLOX_CHECK(cublox_memcpy_d2h_async(op, d_op_cnt_, bytes, stream_));
   if (hit) CUBLOX_CHECK(cublox_memcpy_d2h_async(hit, d_hit_cnt_, bytes, stream_));
   sync();

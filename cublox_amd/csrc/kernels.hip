@@ -5,7 +5,7 @@
 // Reference semantics: src/kernels.cu of the reference (lines cited per
 // kernel). Arithmetic is FP32 with contraction off and IEEE division/sqrt,
 // so the traversed voxel sets are bit-identical to the CPU restatement.
-#include "hip_common.hpp"
+#include "raycast_common.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -15,56 +15,15 @@
 namespace cublox_hip {
 
 // ===================================================================
-// Wave-level helpers
+// Ray cast (reference rayCastUpdateKernel, kernels.cu:33-152): one ray per
+// lane, one counter atomic per traversed voxel. This is the reference-
+// shaped launcher; the grid's own update path is the tile-binned fused
+// update in integrate.hip.
 // ===================================================================
-
-__device__ __forceinline__ void atomic_add_i32(int *p, int v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned atomic_add_u32_ret(unsigned *p,
-                                                       unsigned v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Adds 1 at arr[key] for every active lane whose key is >= 0, issuing one
-// atomic per distinct key among the active lanes of the wavefront (a
-// match-any built from readfirstlane + ballot). Lanes with key < 0 take part
-// in the matching but add nothing.
-__device__ __forceinline__ void wave_add_unique(int *arr, int key) {
-  const int lane = __lane_id();
-  while (true) {
-    const int lead = __builtin_amdgcn_readfirstlane(key);
-    const bool mine = key == lead;
-    const unsigned long long m = __ballot(mine);
-    if (mine) {
-      if (lead >= 0 && lane == __ffsll(static_cast<long long>(m)) - 1) {
-        atomic_add_i32(arr + lead, __popcll(m));
-      }
-      break;
-    }
-  }
-}
-
-// ===================================================================
-// Ray cast (reference rayCastUpdateKernel, kernels.cu:33-152)
-// ===================================================================
-
-struct RayArgs {
-  float T[12];  // sensor->world, row-major [R | t]; used when transforming
-  float ox, oy, oz;
-  int oix, oiy, oiz;
-  int mx, my, mz;
-  int hx, hy, hz;
-  float res, inv_res, max_range;
-  int vox_num;
-  int center;
-};
 
 // Steps below this count are near the sensor voxel, where every ray of the
-// scan passes through the same few voxels (each ray visits exactly one voxel
-// per L1 shell around the sensor voxel). Their increments are deduplicated
+// scan passes through the same few voxels (each step moves one voxel further
+// in L1 distance from the sensor voxel). Their increments are deduplicated
 // across the wavefront before the global atomic.
 constexpr int kNearSteps = 12;
 
@@ -76,74 +35,16 @@ __global__ void __launch_bounds__(256)
                     int *__restrict__ hit_cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  RayState r;
+  if (!ray_setup<kTransform>(a, xs, ys, zs, i, r)) return;
 
-  float ex = xs[i], ey = ys[i], ez = zs[i];
-  if (kTransform) {
-    // Scan transform (driver main.cpp:672-682): drop non-finite sensor-frame
-    // points, then p_w = R p_s + t in a fixed operand order.
-    if (!isfinite(ex) || !isfinite(ey) || !isfinite(ez)) return;
-    const float px = ex, py = ey, pz = ez;
-    ex = ((a.T[0] * px + a.T[1] * py) + a.T[2] * pz) + a.T[3];
-    ey = ((a.T[4] * px + a.T[5] * py) + a.T[6] * pz) + a.T[7];
-    ez = ((a.T[8] * px + a.T[9] * py) + a.T[10] * pz) + a.T[11];
-  }
-  const float ox = a.ox, oy = a.oy, oz = a.oz;
-
-  // kernels.cu:49-54
-  float dx = ex - ox, dy = ey - oy, dz = ez - oz;
-  float L = sqrtf((dx * dx + dy * dy) + dz * dz);
-  if (L < 1e-6f || !isfinite(L) || !isfinite(ex) || !isfinite(ey) ||
-      !isfinite(ez)) {
-    return;
-  }
-  // kernels.cu:57-67: range clip marks free space only.
-  const bool clipped = L > a.max_range;
-  if (clipped) {
-    const float s = a.max_range / L;
-    ex = ox + dx * s;
-    ey = oy + dy * s;
-    ez = oz + dz * s;
-    dx = dx * s;
-    dy = dy * s;
-    dz = dz * s;
-    L = a.max_range;
-  }
-  const float invL = 1.0f / L;
-  const float ux = dx * invL, uy = dy * invL, uz = dz * invL;
-
-  // kernels.cu:73-100: Amanatides-Woo set-up.
-  const bool ctr = a.center != 0;
-  int ix = pos_to_global(ox, a.inv_res, ctr);
-  int iy = pos_to_global(oy, a.inv_res, ctr);
-  int iz = pos_to_global(oz, a.inv_res, ctr);
-  const int eix = pos_to_global(ex, a.inv_res, ctr);
-  const int eiy = pos_to_global(ey, a.inv_res, ctr);
-  const int eiz = pos_to_global(ez, a.inv_res, ctr);
-  const int sx = (ux > 0.0f) - (ux < 0.0f);
-  const int sy = (uy > 0.0f) - (uy < 0.0f);
-  const int sz = (uz > 0.0f) - (uz < 0.0f);
-  const float inf = __builtin_inff();
-  const float tDx = sx == 0 ? inf : fabsf(a.res / ux);
-  const float tDy = sy == 0 ? inf : fabsf(a.res / uy);
-  const float tDz = sz == 0 ? inf : fabsf(a.res / uz);
-  const float bx = ctr ? (static_cast<float>(ix) + 0.5f * static_cast<float>(sx)) * a.res
-                       : (static_cast<float>(ix) + static_cast<float>(sx > 0)) * a.res;
-  const float by = ctr ? (static_cast<float>(iy) + 0.5f * static_cast<float>(sy)) * a.res
-                       : (static_cast<float>(iy) + static_cast<float>(sy > 0)) * a.res;
-  const float bz = ctr ? (static_cast<float>(iz) + 0.5f * static_cast<float>(sz)) * a.res
-                       : (static_cast<float>(iz) + static_cast<float>(sz > 0)) * a.res;
-  float tMx = sx == 0 ? inf : (bx - ox) / ux;
-  float tMy = sy == 0 ? inf : (by - oy) / uy;
-  float tMz = sz == 0 ? inf : (bz - oz) / uz;
-
-  // kernels.cu:105-137
-  const int max_steps = abs(eix - ix) + abs(eiy - iy) + abs(eiz - iz) + 4;
-  for (int step = 0; step < max_steps; ++step) {
-    if (ix == eix && iy == eiy && iz == eiz) break;
+  // kernels.cu:106-137
+  for (int step = 0; step < r.max_steps; ++step) {
+    if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
     int h = -1;
-    if (inside_window(ix, iy, iz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
-      const int hh = global_to_hash(ix, iy, iz, a.mx, a.my, a.mz, a.hx, a.hy,
-                                    a.hz);
+    if (inside_window(r.ix, r.iy, r.iz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
+      const int hh = global_to_hash(r.ix, r.iy, r.iz, a.mx, a.my, a.mz, a.hx,
+                                    a.hy, a.hz);
       if (hh >= 0 && hh < a.vox_num) h = hh;
     }
     if (step < kNearSteps) {
@@ -151,66 +52,24 @@ __global__ void __launch_bounds__(256)
     } else if (h >= 0) {
       atomic_add_i32(op_cnt + h, 1);
     }
-    if (tMx < tMy) {
-      if (tMx < tMz) {
-        ix += sx;
-        tMx += tDx;
-      } else {
-        iz += sz;
-        tMz += tDz;
-      }
-    } else {
-      if (tMy < tMz) {
-        iy += sy;
-        tMy += tDy;
-      } else {
-        iz += sz;
-        tMz += tDz;
-      }
-    }
+    ray_advance(r);
   }
 
   // kernels.cu:141-151: the endpoint counts even when the walk did not reach
   // it; a hit only for real returns.
-  if (inside_window(eix, eiy, eiz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
-    const int h =
-        global_to_hash(eix, eiy, eiz, a.mx, a.my, a.mz, a.hx, a.hy, a.hz);
+  if (inside_window(r.eix, r.eiy, r.eiz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
+    const int h = global_to_hash(r.eix, r.eiy, r.eiz, a.mx, a.my, a.mz, a.hx,
+                                 a.hy, a.hz);
     if (h >= 0 && h < a.vox_num) {
       atomic_add_i32(op_cnt + h, 1);
-      if (!clipped) atomic_add_i32(hit_cnt + h, 1);
+      if (!r.clipped) atomic_add_i32(hit_cnt + h, 1);
     }
   }
-}
-
-RayArgs make_ray_args(const cublox_raycast_cfg &c) {
-  RayArgs a{};
-  a.ox = c.origin.x;
-  a.oy = c.origin.y;
-  a.oz = c.origin.z;
-  a.oix = c.origin_i.x;
-  a.oiy = c.origin_i.y;
-  a.oiz = c.origin_i.z;
-  a.mx = c.map_size_i.x;
-  a.my = c.map_size_i.y;
-  a.mz = c.map_size_i.z;
-  a.hx = c.half_map_size_i.x;
-  a.hy = c.half_map_size_i.y;
-  a.hz = c.half_map_size_i.z;
-  a.res = c.resolution;
-  a.inv_res = c.inv_resolution;
-  a.max_range = c.max_range;
-  a.vox_num = c.map_vox_num;
-  a.center = c.origin_at_center;
-  return a;
 }
 
 // ===================================================================
 // Apply (reference applyUpdateKernel, kernels.cu:167-210)
 // ===================================================================
-
-struct ApplyArgs {
-  float l_hit, l_miss, l_min, l_max, l_unk;
-};
 
 // kernels.cu:184-200 for one voxel with op != 0. Returns the new stored
 // value through *out and whether it differs from the old one.
@@ -219,14 +78,8 @@ __device__ __forceinline__ bool apply_voxel(float *__restrict__ occ,
                                             int op, const ApplyArgs &p,
                                             float *out) {
   const int hit = hit_cnt[h];
-  const int miss = op > hit ? op - hit : 0;
   const float stored0 = occ[h];
-  float l = stored0 + p.l_unk;
-  l = l + p.l_miss * static_cast<float>(miss);
-  l = l + p.l_hit * static_cast<float>(hit);
-  if (l < p.l_min) l = p.l_min;
-  if (l > p.l_max) l = p.l_max;
-  const float stored = l - p.l_unk;
+  const float stored = apply_logodds(stored0, op, hit, p);
   occ[h] = stored;
   if (hit != 0) hit_cnt[h] = 0;
   *out = stored;

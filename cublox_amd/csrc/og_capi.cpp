@@ -105,6 +105,26 @@ int cublox_og_cast_rays_sensor_device(cublox_og *h, const float *d_xs,
   return guarded([&] { OGP->castRaysSensorFrameDevice(d_xs, d_ys, d_zs, n, T); });
 }
 
+// path: 0 fused tiles, 1 per-voxel atomics.
+int cublox_og_set_update_path(cublox_og *h, int path) {
+  return guarded([&] {
+    OGP->setUpdatePath(path == 0 ? OG::UpdatePath::FUSED_TILES
+                                 : OG::UpdatePath::PER_VOXEL_ATOMICS);
+  });
+}
+
+int cublox_og_set_near_steps(cublox_og *h, int k) {
+  return guarded([&] { OGP->setNearSteps(k); });
+}
+
+int cublox_og_cast_rays_fused_sensor_device(cublox_og *h, const float *d_xs,
+                                            const float *d_ys,
+                                            const float *d_zs, int n,
+                                            const float T[12]) {
+  return guarded(
+      [&] { OGP->castRaysFusedSensorFrameDevice(d_xs, d_ys, d_zs, n, T); });
+}
+
 int cublox_og_census(const cublox_og *h, unsigned long long out[4]) {
   return guarded([&] { OGC->counterCensus(out); });
 }
@@ -235,10 +255,10 @@ int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
                            long long out_n[4]) {
   return guarded([&] {
     const auto t = OGC->kernelTimes();
-    out_ms[0] = t.raycast_ms;
+    out_ms[0] = t.update_ms;
     out_ms[1] = t.apply_ms;
     out_ms[2] = t.recenter_ms;
-    out_n[0] = t.raycast_launches;
+    out_n[0] = t.update_launches;
     out_n[1] = t.apply_launches;
     out_n[2] = t.recenter_launches;
     out_n[3] = t.recenter_zeroed_voxels;

@@ -159,6 +159,21 @@ class OccupancyGrid:
         self._check(self._L.cublox_og_cast_rays_sensor_device(
             self._h, d_xs, d_ys, d_zs, int(n), _ffi.fptr(Tm)))
 
+    FUSED_TILES, PER_VOXEL_ATOMICS = 0, 1
+
+    def set_update_path(self, path: int) -> None:
+        self._check(self._L.cublox_og_set_update_path(self._h, int(path)))
+
+    def set_near_steps(self, k: int) -> None:
+        self._check(self._L.cublox_og_set_near_steps(self._h, int(k)))
+
+    def cast_rays_fused_sensor_frame_device(self, d_xs: int, d_ys: int,
+                                            d_zs: int, n: int, T) -> None:
+        """Fused kernels, counts exported to the counter buffers."""
+        Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+        self._check(self._L.cublox_og_cast_rays_fused_sensor_device(
+            self._h, d_xs, d_ys, d_zs, int(n), _ffi.fptr(Tm)))
+
     def counter_census(self) -> dict:
         """Pending-counter work: unique voxels, visits, hits, hit voxels."""
         out = (C.c_ulonglong * 4)()
@@ -285,8 +300,8 @@ class OccupancyGrid:
         ms = (C.c_double * 3)()
         n = (C.c_longlong * 4)()
         self._check(self._L.cublox_og_kernel_times(self._h, ms, n))
-        return dict(raycast_ms=ms[0], apply_ms=ms[1], recenter_ms=ms[2],
-                    raycast_launches=n[0], apply_launches=n[1],
+        return dict(update_ms=ms[0], apply_ms=ms[1], recenter_ms=ms[2],
+                    update_launches=n[0], apply_launches=n[1],
                     recenter_launches=n[2], recenter_zeroed_voxels=n[3])
 
     @property

@@ -136,11 +136,30 @@ public:
                            std::vector<float> &positions,
                            std::vector<unsigned char> &states) const;
 
+  // How update() integrates a scan. FUSED_TILES (default) bins ray segments
+  // by tile and folds counts straight into occ; PER_VOXEL_ATOMICS is the
+  // reference's shape (global op/hit counters, then a sweep over all
+  // voxels). Both give the same occupancy.
+  enum class UpdatePath { FUSED_TILES, PER_VOXEL_ATOMICS };
+  void setUpdatePath(UpdatePath p) { update_path_ = p; }
+  UpdatePath getUpdatePath() const { return update_path_; }
+  // Steps of each ray handled in the near-sensor cube by the fused path.
+  void setNearSteps(int k);
+
+  // Fused path, counts only: the per-voxel op/hit counts the fused kernels
+  // produce are written to the counter buffers (read them with
+  // copyCountersToHost, fold them with applyCounters).
+  void castRaysFusedSensorFrameDevice(const float *d_xs, const float *d_ys,
+                                      const float *d_zs, int n,
+                                      const float T[12]);
+
   // Per-kernel device time measured with HIP events on this grid's stream
-  // (off by default). Recenter also accumulates the voxels it zeroed.
+  // (off by default). update_ms covers the ray cast (and, on the fused
+  // path, the fold); apply_ms the separate fold of the per-voxel path.
+  // Recenter also accumulates the voxels it zeroed.
   struct KernelTimes {
-    double raycast_ms{0.0}, apply_ms{0.0}, recenter_ms{0.0};
-    long long raycast_launches{0}, apply_launches{0}, recenter_launches{0};
+    double update_ms{0.0}, apply_ms{0.0}, recenter_ms{0.0};
+    long long update_launches{0}, apply_launches{0}, recenter_launches{0};
     long long recenter_zeroed_voxels{0};
   };
   void enableKernelTiming(bool on);
@@ -182,6 +201,11 @@ private:
   void castRaysSensor(const float *d_xs, const float *d_ys, const float *d_zs,
                       int n, const float T[12]);
   void launchApply();
+  void launchIntegrate(const float *d_xs, const float *d_ys, const float *d_zs,
+                       int n, const float *T, const Eigen::Vector3f &origin,
+                       bool export_counts);
+  void ensureCounters();
+  void ensureWorkspace(int n);
   void sync() const;
   void mark(int ev) const;
   void harvestTimes();
@@ -213,6 +237,14 @@ private:
   mutable int fetch_cfg_capacity_{0};
 
   float max_raycast_range_{25.0f};
+
+  UpdatePath update_path_{UpdatePath::FUSED_TILES};
+  int near_steps_{16};
+  void *d_ws_{nullptr};
+  size_t ws_bytes_{0};
+  int ws_points_{0};
+  float ws_range_{0.0f};
+  int ws_near_{0};
 
   // Defaults of the reference (OccupancyGrid.hpp:162-168).
   float l_hit_{0.847f};

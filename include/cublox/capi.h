@@ -163,7 +163,9 @@ typedef struct {
 /* The slab plan of planRecenterKernel (kernels.cu:313-398), on the host: it
  * is integer math on the config, so no device round trip is needed. Each
  * slices_* array (may be NULL) receives the raw list for that axis in
- * reference order and must hold 2*map_size_i.{x,y,z} ints. */
+ * reference order and must hold 2*max(map_size_i.x, .y, .z) ints: one
+ * axis may shift by up to the largest extent minus one before the plan
+ * becomes a full reset (kernels.cu:344-347). */
 int cublox_plan_recenter(const cublox_recenter_cfg *cfg, int *slices_x,
                          int *slices_y, int *slices_z,
                          cublox_recenter_result *result);
@@ -238,6 +240,35 @@ int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
                                         unsigned char *d_out_state,
                                         unsigned int per_query_cap,
                                         cublox_stream s);
+
+/* ------------------------------------------------------------------ */
+/* Fused scan update (no reference counterpart as one call; it replaces the
+ * pair launchRayCastUpdate + launchApplyUpdate, kernels.hpp:54-76, with the
+ * same per-voxel result). Rays are cut into segments at 16x16x32 tiles of
+ * the hash-ordered grid; each tile's counts accumulate in LDS and are folded
+ * into occ once, so no per-voxel counter arrays are needed. The first
+ * near_steps steps of every ray (voxels within that L1 distance of the
+ * sensor voxel) accumulate in a small cube with wavefront-deduplicated
+ * atomics. The caller owns the workspace: size it with
+ * cublox_integrate_workspace_bytes for at least max_points points, zero it
+ * once with cublox_integrate_workspace_init, and reuse it for launches with
+ * the same cfg geometry, max_range, max_points and near_steps (the kernels
+ * leave it ready for the next launch). T (row-major 3x4 sensor->world) may
+ * be NULL for world-frame points with cfg->origin as the sensor. When
+ * d_export_op/d_export_hit are given (both zeroed by the caller), the
+ * per-voxel counts are written there instead of being folded into occ.   */
+/* ------------------------------------------------------------------ */
+size_t cublox_integrate_workspace_bytes(const cublox_raycast_cfg *cfg,
+                                        int max_points, int near_steps);
+int cublox_integrate_workspace_init(void *d_ws, size_t bytes, cublox_stream s);
+int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
+                            const float *d_xs, const float *d_ys,
+                            const float *d_zs, int n, float *d_occ,
+                            float l_hit, float l_miss, float l_min,
+                            float l_max, float l_unknown, void *d_ws,
+                            size_t ws_bytes, int max_points, int near_steps,
+                            int *d_export_op, int *d_export_hit,
+                            cublox_stream s);
 
 /* ------------------------------------------------------------------ */
 /* Counter census (no reference counterpart): over n voxels, adds to

@@ -58,7 +58,8 @@ int orc_trace_ray(const float origin[3], const float endpoint[3],
  * Returns action (0/1/2). new_origin[3] gets the planned origin. slices[a]
  * receives the raw slice list for axis a (exits, then fixed-local entries, in
  * reference order; duplicates kept); n_slices[a] its length. Each slices[a]
- * buffer must hold 2*map[a] ints. */
+ * buffer must hold 2*max(map) ints (an axis may shift by up to the
+ * largest extent minus one). */
 int orc_plan_recenter(const float pos[3], const float origin_f[3],
                       const int origin_i[3], const int map[3],
                       const int half[3], float inv_res, float thr_sq,

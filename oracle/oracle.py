@@ -224,7 +224,8 @@ def plan_recenter(pos, origin_f, origin_i, map_size, half, inv_res, thr_sq,
                   center=False):
     L = lib()
     m = _a3(map_size, np.int32)
-    bufs = [np.zeros(2 * int(m[a]) + 2, np.int32) for a in range(3)]
+    cap = 2 * int(max(m)) + 2
+    bufs = [np.zeros(cap, np.int32) for a in range(3)]
     no = np.zeros(3, np.int32)
     ns = np.zeros(3, np.int32)
     action = L.orc_plan_recenter(_f(_a3(pos, np.float32)),

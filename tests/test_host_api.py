@@ -1,0 +1,112 @@
+"""Host-side checks that need no GPU: the C ABI libraries load and export
+every declared symbol, log-odds constants, the workload generator, and the
+drop-in C++ headers compile and run the reference's index round trips."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# logit() of the driver config probabilities (config/cublox_driver.yaml:
+# 35-41) with glibc logf on x86-64, as float32 bit patterns.
+LOGIT_BITS = {0.9: 0x400C9F53, 0.49: 0xBD23DC91, 0.12: 0xBFFF07F3,
+              0.98: 0x4079139A, 0.499: 0xBB831205, 0.85: 0x3FDE076A}
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", "cublox", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cublox_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_capi_exports_every_declared_symbol(cb):
+    from cublox_amd import _ffi
+    L = _ffi.hip()
+    names = _declared("capi.h")
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_host_library_exports(cb):
+    from cublox_amd import _ffi
+    L = _ffi.host()
+    src = open(os.path.join(ROOT, "cublox_amd", "csrc", "og_capi.cpp")).read()
+    names = sorted(set(re.findall(r"^\S.*?\b(cublox_(?:og|utils)_[a-z0-9_]+)\(", src,
+                                  flags=re.M)))
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("p", sorted(LOGIT_BITS))
+def test_logit_constants(cb, orc, p):
+    want = LOGIT_BITS[p]
+    assert np.float32(cb.logit(p)).view(np.uint32) == want
+    assert np.float32(orc.logit(p)).view(np.uint32) == want
+
+
+def test_scene_is_deterministic(cb):
+    w = cb.WORKLOADS["small"]
+    a = w.scene().scan(3, w.beams, w.azimuths, w.el_lo, w.el_hi, w.max_range)
+    b = w.scene().scan(3, w.beams, w.azimuths, w.el_lo, w.el_hi, w.max_range)
+    for x, y in ((a.xs, b.xs), (a.ys, b.ys), (a.zs, b.zs)):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    assert np.array_equal(a.T, b.T)
+    c = cb.Scene(seed=w.seed(1), n_boxes=w.n_boxes, box_extent=w.box_extent,
+                 speed=w.speed).scan(3, w.beams, w.azimuths, w.el_lo, w.el_hi,
+                                     w.max_range)
+    assert not np.array_equal(a.xs, c.xs)
+
+
+def test_scene_geometry(cb):
+    """Returns lie on the ground or a box face (within 5 sigma of noise);
+    misses sit at max_range + 1 m; the sensor is 1.8 m above the ground."""
+    w = cb.WORKLOADS["small"]
+    sc = w.scene()
+    boxes = sc.boxes()
+    for k in (0, 5):
+        s = sc.scan(k, w.beams, w.azimuths, w.el_lo, w.el_hi, w.max_range)
+        assert abs(s.T[2, 3] - 1.8) < 1e-5
+        r = np.sqrt(s.xs.astype(np.float64) ** 2 + s.ys ** 2 + s.zs ** 2)
+        miss = np.abs(r - (w.max_range + 1.0)) < 1e-3
+        assert miss.sum() == s.n_miss
+        R = s.T[:, :3].astype(np.float64)
+        t = s.T[:, 3].astype(np.float64)
+        pw = np.stack([s.xs, s.ys, s.zs], 1).astype(np.float64) @ R.T + t
+        hits = pw[~miss]
+        d_ground = np.abs(hits[:, 2])
+        d_box = np.full(len(hits), np.inf)
+        for b in boxes:
+            lo, hi = b[:3], b[3:]
+            inside = np.clip(hits, lo, hi)
+            out = np.linalg.norm(hits - inside, axis=1)
+            # distance to the surface for points inside the box
+            inner = np.min(np.minimum(hits - lo, hi - hits), axis=1)
+            dist = np.where(out > 0, out, np.maximum(inner, 0))
+            d_box = np.minimum(d_box, dist)
+        assert np.all(np.minimum(d_ground, d_box) < 0.1)
+
+
+def test_workload_shapes(cb):
+    w = cb.WORKLOADS
+    assert w["small"].map_size == (201, 201, 51) and w["small"].points == 16384
+    assert w["readme"].voxels == 74_201_551 and w["readme"].points == 65536
+    assert w["batched"].map_size == (401, 401, 101)
+    assert w["stress"].voxels == 296_382_951 and w["stress"].points == 262144
+
+
+def test_cpp_dropin_index_roundtrips(cb, tmp_path):
+    """The Eigen-typed drop-in headers compile (against the bundled Eigen
+    subset) and pass the reference's index invariants."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_utils_roundtrip.cpp")
+    exe = tmp_path / "roundtrip"
+    inc = [f"-I{ROOT}/include", f"-I{ROOT}/include/cublox/compat"]
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", *inc, src,
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "0 failures" in out.stdout
