@@ -151,6 +151,8 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_launch_fetch_occupancy_batch", i, vp, i, i, i, vp, vp, vp,
          vp, C.c_uint, vp)
     _sig(L, "cublox_counter_census", i, vp, vp, i, vp, vp)
+    _sig(L, "cublox_launch_fetch_gather", i, vp, vp, i, C.c_uint, C.c_uint, vp,
+         vp, vp, vp, vp)
     _sig(L, "cublox_integrate_workspace_bytes", S, C.POINTER(RaycastCfg), i, i)
     _sig(L, "cublox_integrate_workspace_init", i, vp, S, vp)
     _sig(L, "cublox_launch_integrate", i, C.POINTER(RaycastCfg), f32p, vp, vp,

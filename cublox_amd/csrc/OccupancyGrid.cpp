@@ -90,6 +90,11 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_state_ = std::exchange(o.d_fetch_state_, nullptr);
   d_fetch_cfgs_ = std::exchange(o.d_fetch_cfgs_, nullptr);
   d_fetch_counts_ = std::exchange(o.d_fetch_counts_, nullptr);
+  d_fetch_dense_pos_ = std::exchange(o.d_fetch_dense_pos_, nullptr);
+  d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
+  d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
+  fetch_dense_capacity_ = std::exchange(o.fetch_dense_capacity_, 0);
+  fetch_offsets_capacity_ = std::exchange(o.fetch_offsets_capacity_, 0);
   fetch_capacity_ = std::exchange(o.fetch_capacity_, 0);
   fetch_cfg_capacity_ = std::exchange(o.fetch_cfg_capacity_, 0);
   timing_ = std::exchange(o.timing_, false);
@@ -122,6 +127,11 @@ void OccupancyGrid::releaseAll() noexcept {
   freeDevice(d_fetch_count_);
   freeDevice(d_fetch_cfgs_);
   freeDevice(d_fetch_counts_);
+  freeDevice(d_fetch_dense_pos_);
+  freeDevice(d_fetch_dense_state_);
+  freeDevice(d_fetch_offsets_);
+  fetch_dense_capacity_ = 0;
+  fetch_offsets_capacity_ = 0;
   freeDevice(d_op_cnt_);
   freeDevice(d_hit_cnt_);
   freeDevice(d_occ_);
@@ -279,8 +289,8 @@ void OccupancyGrid::ensureCounters() {
 }
 
 void OccupancyGrid::setNearSteps(int k) {
-  if (k < 1 || k > 64) {
-    throw std::invalid_argument("cublox::OccupancyGrid: near steps must be in [1, 64]");
+  if (k < 1 || k > 12) {
+    throw std::invalid_argument("cublox::OccupancyGrid: near steps must be in [1, 12]");
   }
   near_steps_ = k;
 }
@@ -680,18 +690,45 @@ void OccupancyGrid::fetchOccupancyBatch(
   CUBLOX_CHECK(cublox_memcpy_d2h_async(counts.data(), d_fetch_counts_,
                                        sizeof(unsigned int) * nq, stream_));
   sync();
-  int last = -1;
+  // Compact on the device: query q's samples go to offsets[q] of a dense
+  // buffer, so only the real samples cross PCIe.
+  std::vector<unsigned int> offsets(static_cast<size_t>(nq) + 1, 0u);
+  unsigned int max_count = 0;
   for (int q = 0; q < nq; ++q) {
     counts[q] = std::min(counts[q], static_cast<unsigned int>(per_query_cap));
-    if (counts[q] > 0) last = q;
+    offsets[q + 1] = offsets[q] + counts[q];
+    max_count = std::max(max_count, counts[q]);
   }
-  if (last < 0) return;
-  const size_t span = static_cast<size_t>(last + 1) * per_query_cap;
-  positions.resize(span * 3);
-  states.resize(span);
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(positions.data(), d_fetch_pos_,
-                                       span * 3 * sizeof(float), stream_));
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(states.data(), d_fetch_state_, span, stream_));
+  const size_t total = offsets[nq];
+  if (total == 0) return;
+  if (static_cast<long long>(total) > fetch_dense_capacity_ ||
+      nq + 1 > fetch_offsets_capacity_) {
+    sync();
+    freeDevice(d_fetch_dense_pos_);
+    freeDevice(d_fetch_dense_state_);
+    freeDevice(d_fetch_offsets_);
+    fetch_dense_capacity_ = 0;
+    fetch_offsets_capacity_ = 0;
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_dense_pos_),
+                               total * 3 * sizeof(float)));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_dense_state_), total));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fetch_offsets_),
+                               sizeof(unsigned int) * (nq + 1)));
+    fetch_dense_capacity_ = static_cast<long long>(total);
+    fetch_offsets_capacity_ = nq + 1;
+  }
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_fetch_offsets_, offsets.data(),
+                                       sizeof(unsigned int) * (nq + 1), stream_));
+  CUBLOX_CHECK(cublox_launch_fetch_gather(
+      d_fetch_counts_, d_fetch_offsets_, nq,
+      static_cast<unsigned int>(per_query_cap), max_count, d_fetch_pos_,
+      d_fetch_state_, d_fetch_dense_pos_, d_fetch_dense_state_, stream_));
+  positions.resize(total * 3);
+  states.resize(total);
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(positions.data(), d_fetch_dense_pos_,
+                                       total * 3 * sizeof(float), stream_));
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(states.data(), d_fetch_dense_state_, total,
+                                       stream_));
   sync();
 }
 

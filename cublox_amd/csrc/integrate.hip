@@ -3,28 +3,33 @@
 //
 // Same results as the reference's raycast-then-apply pair (kernels.cu:33-210)
 // — the same per-voxel op/hit counts folded once per scan — organised for
-// CDNA4 instead of one global atomic per traversed voxel:
+// CDNA4, where device-scope atomics are the scarce resource (measured on
+// MI355X: ~27 G/s scattered, ~90 M/s on one address) and LDS is plentiful:
 //
-//   walk_count  one ray per lane. Steps < K (the near field: step s visits a
-//               voxel at L1 distance s from the sensor voxel) go to a small
-//               cube of counters around the sensor voxel with wavefront-
-//               deduplicated atomics. Beyond, the walk is cut into
-//               "segments" at every entry into a 16x16x32 tile of the
-//               hash-ordered grid; the segments are counted per tile.
-//   tile_scan   exclusive scan of the per-tile counts (one workgroup) and
-//               the list of non-empty tiles.
-//   walk_emit   the same walk again, writing each segment — the exact DDA
-//               state at the tile entry — into its tile's range. The state
-//               is carried, never recomputed: the walk's t_max recurrence is
-//               a running FP32 sum, so a segment resumes bit-identically.
-//   tile_apply  one workgroup per non-empty tile: resume every segment
-//               inside the tile with LDS counters (no global atomics), then
-//               fold the tile's counts into occ with coalesced z-runs.
+//   walk        one ray per lane. Steps < K (the near field: step s visits a
+//               voxel at L1 distance s from the sensor voxel) count into an
+//               LDS cube around the sensor voxel, flushed once per
+//               workgroup. Beyond, the walk is cut into "segments" at every
+//               entry into a 16x16x32 tile of the hash-ordered grid: each
+//               segment carries the exact DDA state at the tile entry plus
+//               the ray's constants and goes to the ray's own slot range.
+//   bin_count   per workgroup, an LDS hash table aggregates its segments by
+//               tile; one global atomic per (workgroup, tile).
+//   tile_scan   one workgroup: tile offsets, and the work items — one per
+//               tile, or one per 1024-segment chunk of a hot tile.
+//   bin_place   the same aggregation; one returning atomic per (workgroup,
+//               tile) reserves a range; segments are copied to tile order.
+//   tile_apply  one workgroup per work item: resume every segment inside the
+//               tile with LDS counters, then fold the tile's counts into occ
+//               with coalesced z-runs. Chunks of a hot tile add their counts
+//               into a scratch block; the last chunk to finish folds it.
 //   near_apply  fold the near-field cube into occ and clear it.
 //
-// Every voxel's visits happen at one L1 distance from the sensor voxel, so
-// the near-field cube and the tiles own disjoint voxel sets and each voxel
-// is folded exactly once.
+// The walk's t_max recurrence is a running FP32 sum, so a segment resumed
+// from its carried state is bit-identical to the uninterrupted walk. Every
+// voxel's visits happen at one L1 distance from the sensor voxel, so the
+// near-field cube and the tiles own disjoint voxel sets and each voxel is
+// folded exactly once.
 #include "raycast_common.hpp"
 
 #include <algorithm>
@@ -34,50 +39,56 @@ namespace cublox_hip {
 constexpr int TX = 16, TY = 16, TZ = 32;
 constexpr int kTileVox = TX * TY * TZ;  // 8192
 constexpr int kWalkBlock = 256;
+constexpr int kMaxNearK = 12;
+constexpr int kMaxCube = 2 * kMaxNearK - 1;  // 23
+constexpr int kMaxCubeCells = kMaxCube * kMaxCube * kMaxCube;
+constexpr int kBinRays = 128;
+constexpr int kHash = 4096;
+constexpr int kHashProbes = 64;
+constexpr int kChunk = 1024;
+constexpr int kMaxSplit = 256;
 
-struct RayInfo {  // 32 B
+struct Segment {  // 64 B, self-contained; step < 0 marks an endpoint record
+  int step;
+  int ix, iy, iz;
+  float tMx, tMy, tMz;
   float tDx, tDy, tDz;
   int eix, eiy, eiz;
   int max_steps;
   int flags;  // bits 0..5: sx+1, sy+1, sz+1 (2 bits each); bit 6: clipped
+  int tile;
 };
 
-struct Segment {  // 32 B; step < 0 marks an endpoint record
-  int ray;
-  int step;
-  int ix, iy, iz;
-  float tMx, tMy, tMz;
+struct Item {  // one tile_apply workgroup
+  int tile, beg, end, split;  // split < 0: fold directly
 };
 
 struct IntegrateArgs {
   RayArgs a;
   int n;
   int ntx, nty, ntz, n_tiles;
-  int near_k;  // K
-  int cube;    // 2K - 1
-  int max_seg; // per-ray segment bound (capacity check only)
+  int near_k;   // K
+  int cube;     // 2K - 1
+  int max_seg;  // per-ray segment slots
   ApplyArgs p;
 };
 
 struct Workspace {
-  RayInfo *rays;     // n
-  Segment *segs;     // capacity
-  int *tile_count;   // n_tiles (zero between launches)
-  int *tile_offset;  // n_tiles + 1
-  int *tile_cursor;  // n_tiles
-  int *tile_list;    // n_tiles
-  int *misc;         // [0] non-empty tiles, [1] segments, [2] overflow flag
-  int *near_op;      // cube^3 (zero between launches)
-  int *near_hit;     // cube^3 (zero between launches)
+  int *seg_count;     // n: segments written by each ray
+  Segment *segs;      // n * max_seg, per-ray slot ranges
+  Segment *sorted;    // n * max_seg, tile order
+  int *tile_count;    // n_tiles (zero between launches)
+  int *tile_offset;   // n_tiles + 1
+  int *tile_cursor;   // n_tiles
+  Item *items;        // max_items
+  int *split_left;    // kMaxSplit: chunks not yet folded
+  int *scratch;       // kMaxSplit * 2 * kTileVox (zero between launches)
+  int *misc;          // [0] items, [1] segments, [2] overflow flag
+  int *near_op;       // cube^3 (zero between launches)
+  int *near_hit;      // cube^3 (zero between launches)
   long long seg_capacity;
+  int max_items;
 };
-
-__device__ __forceinline__ int wrap_inc(int c, int s, int m) {
-  c += s;
-  if (c >= m) c -= m;
-  if (c < 0) c += m;
-  return c;
-}
 
 __device__ __forceinline__ int tile_of(const IntegrateArgs &g, int cx, int cy,
                                        int cz) {
@@ -90,58 +101,45 @@ __device__ __forceinline__ int cube_index(const IntegrateArgs &g, int dx,
   return ((dx + k1) * g.cube + (dy + k1)) * g.cube + (dz + k1);
 }
 
-// The walk shared by walk_count (kEmit = false) and walk_emit (kEmit = true).
-template <bool kTransform, bool kEmit>
+// ----------------------------------------------------------------------
+// walk
+// ----------------------------------------------------------------------
+template <bool kTransform>
 __global__ void __launch_bounds__(kWalkBlock)
     walk_kernel(const IntegrateArgs g, const float *__restrict__ xs,
                 const float *__restrict__ ys, const float *__restrict__ zs,
                 Workspace w) {
+  __shared__ int s_near[kMaxCubeCells];
+  const int cells = g.cube * g.cube * g.cube;
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
+  __syncthreads();
+
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n) return;
   const RayArgs &a = g.a;
   RayState r;
-  if (!ray_setup<kTransform>(a, xs, ys, zs, i, r)) return;
-  if (!kEmit) {
-    RayInfo ri;
-    ri.tDx = r.tDx;
-    ri.tDy = r.tDy;
-    ri.tDz = r.tDz;
-    ri.eix = r.eix;
-    ri.eiy = r.eiy;
-    ri.eiz = r.eiz;
-    ri.max_steps = r.max_steps;
-    ri.flags = (r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4) |
-               (r.clipped ? 64 : 0);
-    w.rays[i] = ri;
-  }
-  const int s0x = r.ix, s0y = r.iy, s0z = r.iz;  // sensor voxel
-  int cx = global_to_local(r.ix, a.mx, a.hx) + a.hx;
-  int cy = global_to_local(r.iy, a.my, a.hy) + a.hy;
-  int cz = global_to_local(r.iz, a.mz, a.hz) + a.hz;
-  int cur = -1;
-  const int K = g.near_k;
-
-  for (int step = 0; step < r.max_steps; ++step) {
-    if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
-    const bool inwin = abs(r.ix - a.oix) <= a.hx && abs(r.iy - a.oiy) <= a.hy &&
-                       abs(r.iz - a.oiz) <= a.hz;
-    if (step < K) {
-      if (!kEmit) {
-        const int key =
-            inwin ? cube_index(g, r.ix - s0x, r.iy - s0y, r.iz - s0z) : -1;
-        wave_add_unique(w.near_op, key);
-      }
-    } else if (inwin) {
-      const int t = tile_of(g, cx, cy, cz);
-      if (t != cur) {
-        cur = t;
-        if (!kEmit) {
-          wave_add_unique(w.tile_count, t);
-        } else {
-          const int slot = atomicAdd(w.tile_cursor + t, 1);
-          if (slot < w.seg_capacity) {
+  const bool valid = i < g.n && ray_setup<kTransform>(a, xs, ys, zs, i, r);
+  if (valid) {
+    const int flags = (r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4) |
+                      (r.clipped ? 64 : 0);
+    Segment *__restrict__ mine = w.segs + static_cast<size_t>(i) * g.max_seg;
+    int nseg = 0;
+    const int s0x = r.ix, s0y = r.iy, s0z = r.iz;  // sensor voxel
+    HashCoord hc = hash_coord_of(a, r.ix, r.iy, r.iz);
+    int cur = -1;
+    const int K = g.near_k;
+    for (int step = 0; step < r.max_steps; ++step) {
+      if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
+      const bool inwin = abs(r.ix - a.oix) <= a.hx &&
+                         abs(r.iy - a.oiy) <= a.hy && abs(r.iz - a.oiz) <= a.hz;
+      if (step < K) {
+        if (inwin) {
+          atomicAdd(s_near + cube_index(g, r.ix - s0x, r.iy - s0y, r.iz - s0z), 1);
+        }
+      } else {
+        const int t = inwin ? tile_of(g, hc.cx, hc.cy, hc.cz) : -1;
+        if (t != cur && t >= 0) {
+          if (nseg < g.max_seg) {
             Segment sg;
-            sg.ray = i;
             sg.step = step;
             sg.ix = r.ix;
             sg.iy = r.iy;
@@ -149,121 +147,253 @@ __global__ void __launch_bounds__(kWalkBlock)
             sg.tMx = r.tMx;
             sg.tMy = r.tMy;
             sg.tMz = r.tMz;
-            w.segs[slot] = sg;
+            sg.tDx = r.tDx;
+            sg.tDy = r.tDy;
+            sg.tDz = r.tDz;
+            sg.eix = r.eix;
+            sg.eiy = r.eiy;
+            sg.eiz = r.eiz;
+            sg.max_steps = r.max_steps;
+            sg.flags = flags;
+            sg.tile = t;
+            mine[nseg] = sg;
           } else {
             w.misc[2] = 1;
           }
+          ++nseg;
         }
+        cur = t;
       }
-    } else {
-      cur = -1;
+      const int ax = ray_advance(r);
+      hash_coord_step(hc, ax, r, a);
     }
-    const int ax = ray_advance(r);
-    if (ax == 0) {
-      cx = wrap_inc(cx, r.sx, a.mx);
-    } else if (ax == 1) {
-      cy = wrap_inc(cy, r.sy, a.my);
-    } else {
-      cz = wrap_inc(cz, r.sz, a.mz);
-    }
-  }
-
-  // Endpoint (kernels.cu:141-151): op += 1, hit += 1 unless clipped.
-  if (inside_window(r.eix, r.eiy, r.eiz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
-    const int ddx = r.eix - s0x, ddy = r.eiy - s0y, ddz = r.eiz - s0z;
-    if (abs(ddx) + abs(ddy) + abs(ddz) < K) {
-      if (!kEmit) {
+    // Endpoint (kernels.cu:141-151): op += 1, hit += 1 unless clipped.
+    if (inside_window(r.eix, r.eiy, r.eiz, a.oix, a.oiy, a.oiz, a.hx, a.hy,
+                      a.hz)) {
+      const int ddx = r.eix - s0x, ddy = r.eiy - s0y, ddz = r.eiz - s0z;
+      if (abs(ddx) + abs(ddy) + abs(ddz) < K) {
         const int c = cube_index(g, ddx, ddy, ddz);
-        atomic_add_i32(w.near_op + c, 1);
+        atomicAdd(s_near + c, 1);
         if (!r.clipped) atomic_add_i32(w.near_hit + c, 1);
-      }
-    } else {
-      const int ecx = global_to_local(r.eix, a.mx, a.hx) + a.hx;
-      const int ecy = global_to_local(r.eiy, a.my, a.hy) + a.hy;
-      const int ecz = global_to_local(r.eiz, a.mz, a.hz) + a.hz;
-      const int t = tile_of(g, ecx, ecy, ecz);
-      if (!kEmit) {
-        atomic_add_i32(w.tile_count + t, 1);
       } else {
-        const int slot = atomicAdd(w.tile_cursor + t, 1);
-        if (slot < w.seg_capacity) {
+        const HashCoord e = hash_coord_of(a, r.eix, r.eiy, r.eiz);
+        if (nseg < g.max_seg) {
           Segment sg;
-          sg.ray = i;
           sg.step = -1;
           sg.ix = r.eix;
           sg.iy = r.eiy;
           sg.iz = r.eiz;
-          sg.tMx = r.clipped ? 1.0f : 0.0f;
-          sg.tMy = 0.0f;
-          sg.tMz = 0.0f;
-          w.segs[slot] = sg;
+          sg.flags = flags;
+          sg.tile = tile_of(g, e.cx, e.cy, e.cz);
+          mine[nseg] = sg;
         } else {
           w.misc[2] = 1;
         }
+        ++nseg;
       }
     }
+    w.seg_count[i] = min(nseg, g.max_seg);
+  } else if (i < g.n) {
+    w.seg_count[i] = 0;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+    const int v = s_near[c];
+    if (v) atomic_add_i32(w.near_op + c, v);
   }
 }
 
-// Exclusive scan of tile counts, non-empty tile list, counts cleared.
+// ----------------------------------------------------------------------
+// binning: LDS hash aggregation of segments by tile
+// ----------------------------------------------------------------------
+
+// Finds (inserting if needed) the LDS slot of `key`; -1 when the bounded
+// probe finds neither. A key is either in the table for every caller or
+// for none: slots only fill, so a full probe window stays full.
+__device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
+  unsigned h = (static_cast<unsigned>(key) * 2654435761u) >> 20;
+  for (int p = 0; p < kHashProbes; ++p, ++h) {
+    h &= kHash - 1;
+    const int k = keys[h];
+    if (k == key) return static_cast<int>(h);
+    if (k == -1) {
+      if (!insert) return -1;
+      const int prev = atomicCAS(keys + h, -1, key);
+      if (prev == -1 || prev == key) return static_cast<int>(h);
+    }
+  }
+  return -1;
+}
+
+template <bool kPlace>
+__global__ void __launch_bounds__(256)
+    bin_kernel(const IntegrateArgs g, Workspace w) {
+  __shared__ int s_key[kHash];
+  __shared__ int s_cnt[kHash];
+  __shared__ int s_base[kHash];
+  for (int e = threadIdx.x; e < kHash; e += blockDim.x) {
+    s_key[e] = -1;
+    s_cnt[e] = 0;
+  }
+  __syncthreads();
+  const int r0 = blockIdx.x * kBinRays;
+  const int lane_ray = threadIdx.x % kBinRays;
+  const int j0 = threadIdx.x / kBinRays;
+  constexpr int kJStride = 256 / kBinRays;
+  const int ray = r0 + lane_ray;
+  const int cnt = ray < g.n ? w.seg_count[ray] : 0;
+  const size_t base = static_cast<size_t>(ray) * g.max_seg;
+
+  for (int j = j0; j < cnt; j += kJStride) {
+    const int t = w.segs[base + j].tile;
+    const int e = lds_hash_slot(s_key, t, true);
+    if (e >= 0) {
+      atomicAdd(s_cnt + e, 1);
+    } else if (!kPlace) {
+      atomic_add_i32(w.tile_count + t, 1);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kHash; e += blockDim.x) {
+    const int key = s_key[e];
+    if (key < 0) continue;
+    if (!kPlace) {
+      atomic_add_i32(w.tile_count + key, s_cnt[e]);
+    } else {
+      s_base[e] = atomicAdd(w.tile_cursor + key, s_cnt[e]);
+      s_cnt[e] = 0;
+    }
+  }
+  if (!kPlace) return;
+  __syncthreads();
+  for (int j = j0; j < cnt; j += kJStride) {
+    const Segment sg = w.segs[base + j];
+    const int e = lds_hash_slot(s_key, sg.tile, false);
+    const int slot = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
+                            : atomicAdd(w.tile_cursor + sg.tile, 1);
+    if (slot < w.seg_capacity) w.sorted[slot] = sg;
+  }
+}
+
+// ----------------------------------------------------------------------
+// tile_scan: offsets, work items, hot-tile splits
+// ----------------------------------------------------------------------
 constexpr int kScanThreads = 1024;
+
+// Inclusive block scan of one int per thread (Hillis-Steele).
+__device__ __forceinline__ int block_scan_incl(int *s, int v) {
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int a = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  const int r = s[t];
+  __syncthreads();
+  return r;
+}
 
 __global__ void __launch_bounds__(kScanThreads)
     tile_scan_kernel(const int n_tiles, Workspace w) {
-  __shared__ int s_sum[kScanThreads];
-  __shared__ int s_nz[kScanThreads];
+  __shared__ int s_buf[kScanThreads];
+  __shared__ int s_tot[3];
   const int t = threadIdx.x;
   const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
   const int beg = min(t * per, n_tiles);
   const int end = min(beg + per, n_tiles);
-  int sum = 0, nz = 0;
+  // Pass 1: segments and hot tiles, so each hot tile knows its split rank.
+  int seg = 0, hot = 0;
   for (int k = beg; k < end; ++k) {
     const int c = w.tile_count[k];
-    sum += c;
-    nz += c > 0;
+    seg += c;
+    hot += c > kChunk;
   }
-  s_sum[t] = sum;
-  s_nz[t] = nz;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 partials.
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    const int a = t >= off ? s_sum[t - off] : 0;
-    const int b = t >= off ? s_nz[t - off] : 0;
-    __syncthreads();
-    s_sum[t] += a;
-    s_nz[t] += b;
-    __syncthreads();
+  const int seg_incl = block_scan_incl(s_buf, seg);
+  const int hot_incl = block_scan_incl(s_buf, hot);
+  // Pass 2: work items, now that the split decision of each tile is known
+  // (beyond kMaxSplit hot tiles, a hot tile is folded by one workgroup).
+  int item = 0;
+  {
+    int sp = hot_incl - hot;
+    for (int k = beg; k < end; ++k) {
+      const int c = w.tile_count[k];
+      if (c <= 0) continue;
+      if (c > kChunk) {
+        item += sp < kMaxSplit ? (c + kChunk - 1) / kChunk : 1;
+        ++sp;
+      } else {
+        item += 1;
+      }
+    }
   }
-  int run = s_sum[t] - sum;
-  int pos = s_nz[t] - nz;
+  const int item_incl = block_scan_incl(s_buf, item);
+  // Pass 3: write offsets, cursors, items; clear the counts.
+  int run = seg_incl - seg;
+  int it = item_incl - item;
+  int sp = hot_incl - hot;
   for (int k = beg; k < end; ++k) {
     const int c = w.tile_count[k];
     w.tile_offset[k] = run;
     w.tile_cursor[k] = run;
-    if (c > 0) w.tile_list[pos++] = k;
+    if (c > 0) {
+      int split_id = -1;
+      if (c > kChunk) {
+        if (sp < kMaxSplit) split_id = sp;
+        ++sp;
+      }
+      const int chunks = split_id >= 0 ? (c + kChunk - 1) / kChunk : 1;
+      if (split_id >= 0) w.split_left[split_id] = chunks;
+      for (int q = 0; q < chunks; ++q) {
+        if (it < w.max_items) {
+          Item itm;
+          itm.tile = k;
+          itm.beg = run + q * kChunk;
+          itm.end = split_id >= 0 ? min(run + (q + 1) * kChunk, run + c) : run + c;
+          itm.split = split_id;
+          w.items[it] = itm;
+        }
+        ++it;
+      }
+    }
     run += c;
     w.tile_count[k] = 0;
   }
   if (t == kScanThreads - 1) {
-    w.tile_offset[n_tiles] = s_sum[t];
-    w.misc[0] = s_nz[t];
-    w.misc[1] = s_sum[t];
+    s_tot[0] = seg_incl;
+    s_tot[1] = item_incl;
+    w.tile_offset[n_tiles] = seg_incl;
+    w.misc[0] = min(item_incl, w.max_items);
+    w.misc[1] = seg_incl;
   }
 }
 
-// One workgroup per non-empty tile.
+// ----------------------------------------------------------------------
+// tile_apply
+// ----------------------------------------------------------------------
+__device__ __forceinline__ int tile_voxel_hash(const RayArgs &a, int x0, int y0,
+                                               int z0, int k) {
+  const int lz = k % TZ;
+  const int ly = (k / TZ) % TY;
+  const int lx = k / (TZ * TY);
+  return (x0 + lx) * (a.my * a.mz) + (y0 + ly) * a.mz + (z0 + lz);
+}
+
 __global__ void __launch_bounds__(256)
     tile_apply_kernel(const IntegrateArgs g, Workspace w,
                       float *__restrict__ occ, int *__restrict__ export_op,
                       int *__restrict__ export_hit) {
   __shared__ int s_op[kTileVox];
   __shared__ int s_hit[kTileVox];
+  __shared__ int s_last;
   if (static_cast<int>(blockIdx.x) >= w.misc[0]) return;
-  const int tile = w.tile_list[blockIdx.x];
+  const Item itm = w.items[blockIdx.x];
   const RayArgs &a = g.a;
-  const int tz = tile % g.ntz;
-  const int ty = (tile / g.ntz) % g.nty;
-  const int tx = tile / (g.ntz * g.nty);
+  const int tz = itm.tile % g.ntz;
+  const int ty = (itm.tile / g.ntz) % g.nty;
+  const int tx = itm.tile / (g.ntz * g.nty);
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
     s_op[k] = 0;
@@ -271,19 +401,14 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
-  const int beg = w.tile_offset[tile];
-  const int end = min(static_cast<long long>(w.tile_offset[tile + 1]),
-                      w.seg_capacity);
-  for (int k = beg + threadIdx.x; k < end; k += blockDim.x) {
-    const Segment sg = w.segs[k];
-    const RayInfo ri = w.rays[sg.ray];
+  const int end = static_cast<int>(min(static_cast<long long>(itm.end), w.seg_capacity));
+  for (int k = itm.beg + static_cast<int>(threadIdx.x); k < end; k += blockDim.x) {
+    const Segment sg = w.sorted[k];
     if (sg.step < 0) {
-      const int ecx = global_to_local(sg.ix, a.mx, a.hx) + a.hx - x0;
-      const int ecy = global_to_local(sg.iy, a.my, a.hy) + a.hy - y0;
-      const int ecz = global_to_local(sg.iz, a.mz, a.hz) + a.hz - z0;
-      const int li = (ecx * TY + ecy) * TZ + ecz;
+      const HashCoord e = hash_coord_of(a, sg.ix, sg.iy, sg.iz);
+      const int li = ((e.cx - x0) * TY + (e.cy - y0)) * TZ + (e.cz - z0);
       atomicAdd(s_op + li, 1);
-      if (sg.tMx == 0.0f) atomicAdd(s_hit + li, 1);
+      if ((sg.flags & 64) == 0) atomicAdd(s_hit + li, 1);
       continue;
     }
     RayState r;
@@ -293,66 +418,95 @@ __global__ void __launch_bounds__(256)
     r.tMx = sg.tMx;
     r.tMy = sg.tMy;
     r.tMz = sg.tMz;
-    r.tDx = ri.tDx;
-    r.tDy = ri.tDy;
-    r.tDz = ri.tDz;
-    r.eix = ri.eix;
-    r.eiy = ri.eiy;
-    r.eiz = ri.eiz;
-    r.max_steps = ri.max_steps;
-    r.sx = (ri.flags & 3) - 1;
-    r.sy = ((ri.flags >> 2) & 3) - 1;
-    r.sz = ((ri.flags >> 4) & 3) - 1;
-    // Hash coordinates of the segment's first voxel, stepped with the same
-    // seam wrap as the emitting walk so that tile membership agrees.
-    int cx = global_to_local(r.ix, a.mx, a.hx) + a.hx;
-    int cy = global_to_local(r.iy, a.my, a.hy) + a.hy;
-    int cz = global_to_local(r.iz, a.mz, a.hz) + a.hz;
+    r.tDx = sg.tDx;
+    r.tDy = sg.tDy;
+    r.tDz = sg.tDz;
+    r.eix = sg.eix;
+    r.eiy = sg.eiy;
+    r.eiz = sg.eiz;
+    r.max_steps = sg.max_steps;
+    r.sx = (sg.flags & 3) - 1;
+    r.sy = ((sg.flags >> 2) & 3) - 1;
+    r.sz = ((sg.flags >> 4) & 3) - 1;
+    // Hash coordinates stepped with the same seam wrap as the walk, so
+    // tile membership agrees with the segment cut.
+    HashCoord hc = hash_coord_of(a, r.ix, r.iy, r.iz);
     for (int step = sg.step; step < r.max_steps; ++step) {
-      if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
-      if (!(abs(r.ix - a.oix) <= a.hx && abs(r.iy - a.oiy) <= a.hy &&
-            abs(r.iz - a.oiz) <= a.hz)) {
-        break;
-      }
-      const int lx = cx - x0, ly = cy - y0, lz = cz - z0;
-      if (static_cast<unsigned>(lx) >= static_cast<unsigned>(TX) ||
-          static_cast<unsigned>(ly) >= static_cast<unsigned>(TY) ||
-          static_cast<unsigned>(lz) >= static_cast<unsigned>(TZ)) {
-        break;
-      }
+      const int lx = hc.cx - x0, ly = hc.cy - y0, lz = hc.cz - z0;
+      const bool go =
+          !(r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) &&
+          abs(r.ix - a.oix) <= a.hx && abs(r.iy - a.oiy) <= a.hy &&
+          abs(r.iz - a.oiz) <= a.hz &&
+          static_cast<unsigned>(lx) < static_cast<unsigned>(TX) &&
+          static_cast<unsigned>(ly) < static_cast<unsigned>(TY) &&
+          static_cast<unsigned>(lz) < static_cast<unsigned>(TZ);
+      if (!go) break;
       atomicAdd(s_op + (lx * TY + ly) * TZ + lz, 1);
       const int ax = ray_advance(r);
-      if (ax == 0) {
-        cx = wrap_inc(cx, r.sx, a.mx);
-      } else if (ax == 1) {
-        cy = wrap_inc(cy, r.sy, a.my);
-      } else {
-        cz = wrap_inc(cz, r.sz, a.mz);
-      }
+      hash_coord_step(hc, ax, r, a);
     }
   }
   __syncthreads();
 
-  // Fold: lanes walk z-runs of TZ voxels (contiguous in occ).
-  const int yz = a.my * a.mz;
-  for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
-    const int o = s_op[k];
-    if (o == 0) continue;
-    const int lz = k % TZ;
-    const int ly = (k / TZ) % TY;
-    const int lx = k / (TZ * TY);
-    const int h = (x0 + lx) * yz + (y0 + ly) * a.mz + (z0 + lz);
-    const int hit = s_hit[k];
-    if (export_op) {
-      export_op[h] = o;
-      export_hit[h] = hit;
-    } else {
-      occ[h] = apply_logodds(occ[h], o, hit, g.p);
+  const int *fold_op = s_op;
+  const int *fold_hit = s_hit;
+  if (itm.split >= 0) {
+    // Hot tile: add this chunk's counts into the tile's scratch block; the
+    // last chunk to arrive folds the block and clears it.
+    int *sc_op = w.scratch + static_cast<size_t>(itm.split) * 2 * kTileVox;
+    int *sc_hit = sc_op + kTileVox;
+    for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
+      if (s_op[k]) atomic_add_i32(sc_op + k, s_op[k]);
+      if (s_hit[k]) atomic_add_i32(sc_hit + k, s_hit[k]);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_last = atomicSub(w.split_left + itm.split, 1) == 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
+      s_op[k] = __hip_atomic_exchange(sc_op + k, 0, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+      s_hit[k] = __hip_atomic_exchange(sc_hit + k, 0, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+
+  // Fold: consecutive threads take consecutive tile voxels (z-runs of TZ
+  // contiguous floats); eight voxels per thread have their loads in flight
+  // together.
+  constexpr int kBatch = 8;
+  for (int b0 = 0; b0 < kTileVox; b0 += kBatch * 256) {
+    int o[kBatch], hit[kBatch], hh[kBatch];
+    float v[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int kk = b0 + j * 256 + static_cast<int>(threadIdx.x);
+      o[j] = fold_op[kk];
+      hit[j] = fold_hit[kk];
+      hh[j] = tile_voxel_hash(a, x0, y0, z0, kk);
+      v[j] = (o[j] != 0 && !export_op) ? occ[hh[j]] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      if (o[j] == 0) continue;
+      if (export_op) {
+        export_op[hh[j]] = o[j];
+        export_hit[hh[j]] = hit[j];
+      } else {
+        occ[hh[j]] = apply_logodds(v[j], o[j], hit[j], g.p);
+      }
     }
   }
 }
 
-// Fold and clear the near-field cube.
+// ----------------------------------------------------------------------
+// near_apply
+// ----------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
     near_apply_kernel(const IntegrateArgs g, Workspace w,
                       float *__restrict__ occ, int *__restrict__ export_op,
@@ -391,18 +545,18 @@ __global__ void __launch_bounds__(256)
 // ----------------------------------------------------------------------
 
 struct Layout {
-  int ntx, nty, ntz, n_tiles, cube, max_seg;
+  int ntx, nty, ntz, n_tiles, cube, max_seg, max_items;
   long long seg_capacity;
-  size_t off_rays, off_segs, off_tcount, off_toff, off_tcur, off_tlist,
-      off_misc, off_nop, off_nhit, total;
+  size_t off_count, off_segs, off_sorted, off_tcount, off_toff, off_tcur,
+      off_items, off_split, off_scratch, off_misc, off_nop, off_nhit, total;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-// Per-ray bound on segments (tile entries + the endpoint record): inside
-// the window the walk is monotone per axis, so it changes tile at most
-// ntx + nty + ntz times plus one wrap seam per axis; a walk of S steps
-// also changes tile at most S / min(tile dim) + 3 times.
+// Per-ray segment bound (tile entries + the endpoint record): inside the
+// window the walk is monotone per axis, so it changes tile at most
+// ntx + nty + ntz times plus one wrap seam per axis; a walk of S steps also
+// changes tile at most S / min(tile dim) + 3 times.
 Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
   Layout L{};
   L.ntx = (c.map_size_i.x + TX - 1) / TX;
@@ -416,15 +570,19 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
   long long range_bound = win_bound;
   if (reach < 1e7) {
     const long long steps = 3 * static_cast<long long>(reach) + 8;
-    range_bound = steps / std::min(std::min(TX, TY), TZ) + 8;
+    range_bound = steps / std::min(std::min(TX, TY), TZ) + 10;
   }
   L.max_seg = static_cast<int>(std::min(win_bound, range_bound));
-  L.seg_capacity = static_cast<long long>(std::max(max_points, 1)) * L.max_seg;
-  size_t o = 0;
   const size_t n = static_cast<size_t>(std::max(max_points, 1));
-  L.off_rays = o;
-  o = align256(o + n * sizeof(RayInfo));
+  L.seg_capacity = static_cast<long long>(n) * L.max_seg;
+  L.max_items = L.n_tiles + static_cast<int>(std::min<long long>(
+                                L.seg_capacity / kChunk + 1, 1 << 30));
+  size_t o = 0;
+  L.off_count = o;
+  o = align256(o + n * sizeof(int));
   L.off_segs = o;
+  o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(Segment));
+  L.off_sorted = o;
   o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(Segment));
   L.off_tcount = o;
   o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
@@ -432,8 +590,12 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
   o = align256(o + static_cast<size_t>(L.n_tiles + 1) * 4);
   L.off_tcur = o;
   o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
-  L.off_tlist = o;
-  o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
+  L.off_items = o;
+  o = align256(o + static_cast<size_t>(L.max_items) * sizeof(Item));
+  L.off_split = o;
+  o = align256(o + kMaxSplit * 4);
+  L.off_scratch = o;
+  o = align256(o + static_cast<size_t>(kMaxSplit) * 2 * kTileVox * 4);
   L.off_misc = o;
   o = align256(o + 64 * 4);
   const size_t cells = static_cast<size_t>(L.cube) * L.cube * L.cube;
@@ -448,16 +610,20 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
 Workspace bind(const Layout &L, void *base) {
   char *b = static_cast<char *>(base);
   Workspace w;
-  w.rays = reinterpret_cast<RayInfo *>(b + L.off_rays);
+  w.seg_count = reinterpret_cast<int *>(b + L.off_count);
   w.segs = reinterpret_cast<Segment *>(b + L.off_segs);
+  w.sorted = reinterpret_cast<Segment *>(b + L.off_sorted);
   w.tile_count = reinterpret_cast<int *>(b + L.off_tcount);
   w.tile_offset = reinterpret_cast<int *>(b + L.off_toff);
   w.tile_cursor = reinterpret_cast<int *>(b + L.off_tcur);
-  w.tile_list = reinterpret_cast<int *>(b + L.off_tlist);
+  w.items = reinterpret_cast<Item *>(b + L.off_items);
+  w.split_left = reinterpret_cast<int *>(b + L.off_split);
+  w.scratch = reinterpret_cast<int *>(b + L.off_scratch);
   w.misc = reinterpret_cast<int *>(b + L.off_misc);
   w.near_op = reinterpret_cast<int *>(b + L.off_nop);
   w.near_hit = reinterpret_cast<int *>(b + L.off_nhit);
   w.seg_capacity = L.seg_capacity;
+  w.max_items = L.max_items;
   return w;
 }
 
@@ -469,7 +635,7 @@ extern "C" {
 
 size_t cublox_integrate_workspace_bytes(const cublox_raycast_cfg *cfg,
                                         int max_points, int near_steps) {
-  if (!cfg || near_steps < 1) return 0;
+  if (!cfg || near_steps < 1 || near_steps > kMaxNearK) return 0;
   return make_layout(*cfg, max_points, near_steps).total;
 }
 
@@ -487,8 +653,8 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
                             int *d_export_op, int *d_export_hit,
                             cublox_stream s) {
   if (!cfg || n <= 0 || !d_xs || !d_ys || !d_zs || !d_occ) return CUBLOX_OK;
-  if (near_steps < 1 || near_steps > 64) {
-    return set_error(CUBLOX_ERR_INVALID, "integrate: near_steps out of [1, 64]");
+  if (near_steps < 1 || near_steps > kMaxNearK) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate: near_steps out of [1, 12]");
   }
   if (n > max_points) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
@@ -522,25 +688,28 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
   const dim3 wb(kWalkBlock);
   const dim3 wg((n + kWalkBlock - 1) / kWalkBlock);
   if (T) {
-    walk_kernel<true, false><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
+    walk_kernel<true><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
   } else {
-    walk_kernel<false, false><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
+    walk_kernel<false><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
   }
-  int st = check_launch("walk_kernel<count>");
+  int st = check_launch("walk_kernel");
+  if (st != CUBLOX_OK) return st;
+  const dim3 bg((n + kBinRays - 1) / kBinRays);
+  bin_kernel<false><<<bg, dim3(256), 0, hs>>>(g, w);
+  st = check_launch("bin_kernel<count>");
   if (st != CUBLOX_OK) return st;
   tile_scan_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(L.n_tiles, w);
   st = check_launch("tile_scan_kernel");
   if (st != CUBLOX_OK) return st;
-  if (T) {
-    walk_kernel<true, true><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
-  } else {
-    walk_kernel<false, true><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
-  }
-  st = check_launch("walk_kernel<emit>");
+  bin_kernel<true><<<bg, dim3(256), 0, hs>>>(g, w);
+  st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
-  tile_apply_kernel<<<dim3(L.n_tiles), dim3(256), 0, hs>>>(g, w, d_occ,
-                                                            d_export_op,
-                                                            d_export_hit);
+  // One workgroup per work item; items beyond the device count exit.
+  const long long est_items =
+      std::min<long long>(L.max_items, L.n_tiles + (static_cast<long long>(n) *
+                                                        L.max_seg) / kChunk + 1);
+  tile_apply_kernel<<<dim3(static_cast<unsigned>(est_items)), dim3(256), 0, hs>>>(
+      g, w, d_occ, d_export_op, d_export_hit);
   st = check_launch("tile_apply_kernel");
   if (st != CUBLOX_OK) return st;
   const int cells = L.cube * L.cube * L.cube;

@@ -38,21 +38,24 @@ __global__ void __launch_bounds__(256)
   RayState r;
   if (!ray_setup<kTransform>(a, xs, ys, zs, i, r)) return;
 
-  // kernels.cu:106-137
+  // kernels.cu:106-137. The hash coordinates are tracked incrementally
+  // (one wrap compare per step instead of three integer remainders); every
+  // lane of a wavefront is at the same step, so the near-field test is
+  // uniform.
+  HashCoord c = hash_coord_of(a, r.ix, r.iy, r.iz);
+  const int yz = a.my * a.mz;
   for (int step = 0; step < r.max_steps; ++step) {
     if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
-    int h = -1;
-    if (inside_window(r.ix, r.iy, r.iz, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
-      const int hh = global_to_hash(r.ix, r.iy, r.iz, a.mx, a.my, a.mz, a.hx,
-                                    a.hy, a.hz);
-      if (hh >= 0 && hh < a.vox_num) h = hh;
-    }
+    const bool inwin = abs(r.ix - a.oix) <= a.hx && abs(r.iy - a.oiy) <= a.hy &&
+                       abs(r.iz - a.oiz) <= a.hz;
+    const int h = inwin ? c.cx * yz + c.cy * a.mz + c.cz : -1;
     if (step < kNearSteps) {
       wave_add_unique(op_cnt, h);
     } else if (h >= 0) {
       atomic_add_i32(op_cnt + h, 1);
     }
-    ray_advance(r);
+    const int ax = ray_advance(r);
+    hash_coord_step(c, ax, r, a);
   }
 
   // kernels.cu:141-151: the endpoint counts even when the walk did not reach
@@ -376,6 +379,28 @@ __global__ void __launch_bounds__(256)
   const size_t off = static_cast<size_t>(q) * per_query_cap;
   fetch_body(c, occ, counts + q, out_pos + 3 * off, out_state + off,
              per_query_cap, 0u, blockIdx.x, gridDim.x);
+}
+
+// Dense gather of batched fetch results: query q's first min(count, cap)
+// samples move from its capacity segment to offsets[q].
+__global__ void __launch_bounds__(256)
+    fetch_gather_kernel(const unsigned *__restrict__ counts,
+                        const unsigned *__restrict__ offsets,
+                        const unsigned cap, const float *__restrict__ pos_seg,
+                        const unsigned char *__restrict__ st_seg,
+                        float *__restrict__ pos_out,
+                        unsigned char *__restrict__ st_out) {
+  const int q = blockIdx.y;
+  const unsigned n = min(counts[q], cap);
+  const size_t src = static_cast<size_t>(q) * cap;
+  const size_t dst = offsets[q];
+  for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += gridDim.x * blockDim.x) {
+    pos_out[3 * (dst + j) + 0] = pos_seg[3 * (src + j) + 0];
+    pos_out[3 * (dst + j) + 1] = pos_seg[3 * (src + j) + 1];
+    pos_out[3 * (dst + j) + 2] = pos_seg[3 * (src + j) + 2];
+    st_out[dst + j] = st_seg[src + j];
+  }
 }
 
 // ===================================================================
@@ -776,6 +801,26 @@ int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
                        dim3(256), 0, as_stream(s)>>>(
       d_cfgs, d_occ, d_counts, d_pos, d_state, per_query_cap);
   return check_launch("fetch_batch_kernel");
+}
+
+int cublox_launch_fetch_gather(const unsigned int *d_counts,
+                               const unsigned int *d_offsets, int n_queries,
+                               unsigned int per_query_cap,
+                               unsigned int max_count, const float *d_pos_seg,
+                               const unsigned char *d_state_seg,
+                               float *d_pos_out, unsigned char *d_state_out,
+                               cublox_stream s) {
+  if (n_queries <= 0 || per_query_cap == 0 || max_count == 0) return CUBLOX_OK;
+  if (n_queries > 65535) {
+    return set_error(CUBLOX_ERR_INVALID, "fetch gather: more than 65535 queries");
+  }
+  const unsigned m = std::min(max_count, per_query_cap);
+  const unsigned bx = std::max(1u, std::min((m + 255u) / 256u, 64u));
+  fetch_gather_kernel<<<dim3(bx, static_cast<unsigned>(n_queries)), dim3(256), 0,
+                        as_stream(s)>>>(d_counts, d_offsets, per_query_cap,
+                                        d_pos_seg, d_state_seg, d_pos_out,
+                                        d_state_out);
+  return check_launch("fetch_gather_kernel");
 }
 
 int cublox_counter_census(const int *d_op, const int *d_hit, int n,

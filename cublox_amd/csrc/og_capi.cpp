@@ -159,7 +159,8 @@ int cublox_og_fetch(const cublox_og *h, const float center[3], float radius,
   });
 }
 
-// Outputs sized n_queries * per_query_cap (positions x3).
+// counts[q] per query; out_pos/out_state sized n_queries * per_query_cap
+// (x3 for positions) receive the samples densely, query after query.
 int cublox_og_fetch_batch(const cublox_og *h, const float *centers,
                           int n_queries, float radius, int filter,
                           int per_query_cap, unsigned int *counts,

@@ -155,26 +155,55 @@ __device__ __forceinline__ bool ray_setup(const RayArgs &a,
 }
 
 // One Amanatides-Woo step with the reference's z-biased tie-break
-// (kernels.cu:120-136). Returns the axis stepped (0, 1, 2).
+// (kernels.cu:120-136):
+//   x if tMx < tMy and tMx < tMz; else y if !(tMx < tMy) and tMy < tMz;
+//   else z.
+// Written as selects so a wavefront whose lanes step different axes runs
+// one straight-line path; each candidate sum is formed and then selected,
+// so the kept values are exactly the reference's.
 __device__ __forceinline__ int ray_advance(RayState &r) {
-  if (r.tMx < r.tMy) {
-    if (r.tMx < r.tMz) {
-      r.ix += r.sx;
-      r.tMx += r.tDx;
-      return 0;
-    }
-    r.iz += r.sz;
-    r.tMz += r.tDz;
-    return 2;
-  }
-  if (r.tMy < r.tMz) {
-    r.iy += r.sy;
-    r.tMy += r.tDy;
-    return 1;
-  }
-  r.iz += r.sz;
-  r.tMz += r.tDz;
-  return 2;
+  const bool x_lt_y = r.tMx < r.tMy;
+  const bool step_x = x_lt_y && (r.tMx < r.tMz);
+  const bool step_y = !x_lt_y && (r.tMy < r.tMz);
+  const bool step_z = !step_x && !step_y;
+  const float nx = r.tMx + r.tDx;
+  const float ny = r.tMy + r.tDy;
+  const float nz = r.tMz + r.tDz;
+  r.ix += step_x ? r.sx : 0;
+  r.iy += step_y ? r.sy : 0;
+  r.iz += step_z ? r.sz : 0;
+  r.tMx = step_x ? nx : r.tMx;
+  r.tMy = step_y ? ny : r.tMy;
+  r.tMz = step_z ? nz : r.tMz;
+  return step_x ? 0 : (step_y ? 1 : 2);
+}
+
+// Hash coordinates (local index + half, in [0, map)) tracked along a walk:
+// a +-1 step wraps at the map edge. Branch-free.
+__device__ __forceinline__ int wrap_step(int c, int s, int m) {
+  c += s;
+  c = c >= m ? c - m : c;
+  return c < 0 ? c + m : c;
+}
+
+struct HashCoord {
+  int cx, cy, cz;
+};
+
+__device__ __forceinline__ HashCoord hash_coord_of(const RayArgs &a, int gx,
+                                                   int gy, int gz) {
+  return HashCoord{global_to_local(gx, a.mx, a.hx) + a.hx,
+                   global_to_local(gy, a.my, a.hy) + a.hy,
+                   global_to_local(gz, a.mz, a.hz) + a.hz};
+}
+
+// Applies the axis step returned by ray_advance to tracked hash coordinates.
+__device__ __forceinline__ void hash_coord_step(HashCoord &c, int ax,
+                                                const RayState &r,
+                                                const RayArgs &a) {
+  c.cx = ax == 0 ? wrap_step(c.cx, r.sx, a.mx) : c.cx;
+  c.cy = ax == 1 ? wrap_step(c.cy, r.sy, a.my) : c.cy;
+  c.cz = ax == 2 ? wrap_step(c.cz, r.sz, a.mz) : c.cz;
 }
 
 // Log-odds fold of one voxel with op != 0 (reference kernels.cu:184-197):

@@ -239,17 +239,19 @@ class OccupancyGrid:
 
     def fetch_occupancy_batch(self, centers, radius: float,
                               state_filter: Optional[int], per_query_cap: int):
-        """Returns (counts (q,), positions (q, cap, 3), states (q, cap))."""
+        """Returns (counts (q,), positions (total, 3), states (total,)) with
+        the samples of query 0 first, then query 1, ... (offsets = cumsum)."""
         cs = np.ascontiguousarray(np.asarray(centers, np.float32).reshape(-1, 3))
         q = cs.shape[0]
         counts = np.zeros(q, np.uint32)
-        pos = np.zeros((q, per_query_cap, 3), np.float32)
-        st = np.zeros((q, per_query_cap), np.uint8)
+        pos = np.zeros((q * per_query_cap, 3), np.float32)
+        st = np.zeros(q * per_query_cap, np.uint8)
         self._check(self._L.cublox_og_fetch_batch(
             self._h, _ffi.fptr(cs), q, float(radius),
             -1 if state_filter is None else int(state_filter), int(per_query_cap),
             _ffi.uptr(counts), _ffi.fptr(pos), _ffi.bptr(st)))
-        return counts, pos, st
+        total = int(counts.sum())
+        return counts, pos[:total], st[:total]
 
     def _query(self, kind: int, by: int, q) -> bool:
         qf = np.zeros(3, np.float32)

@@ -124,10 +124,10 @@ public:
   // sum of op (voxel visits), sum of hit, voxels with hit != 0}.
   void counterCensus(unsigned long long out[4]) const;
 
-  // Batched sphere fetch: one launch for all centres. Sample q*cap + j is
-  // the j-th sample of query q; counts[q] is the reference's returned count
-  // for that query (min(logical count, cap)). Order within a query is
-  // unspecified. Blocks.
+  // Batched sphere fetch: one launch for all centres. counts[q] is the
+  // reference's returned count for query q (min(logical count, cap));
+  // positions (x, y, z per sample) and states hold the samples of query 0,
+  // then query 1, and so on. Order within a query is unspecified. Blocks.
   void fetchOccupancyBatch(const std::vector<Eigen::Vector3f> &centers,
                            const float radius,
                            const std::optional<VoxelState> state_filter,
@@ -233,13 +233,18 @@ private:
   mutable unsigned char *d_fetch_state_{nullptr};
   mutable void *d_fetch_cfgs_{nullptr};
   mutable unsigned int *d_fetch_counts_{nullptr};
+  mutable float *d_fetch_dense_pos_{nullptr};
+  mutable unsigned char *d_fetch_dense_state_{nullptr};
+  mutable unsigned int *d_fetch_offsets_{nullptr};
+  mutable long long fetch_dense_capacity_{0};
+  mutable int fetch_offsets_capacity_{0};
   mutable long long fetch_capacity_{0};
   mutable int fetch_cfg_capacity_{0};
 
   float max_raycast_range_{25.0f};
 
   UpdatePath update_path_{UpdatePath::FUSED_TILES};
-  int near_steps_{16};
+  int near_steps_{8};
   void *d_ws_{nullptr};
   size_t ws_bytes_{0};
   int ws_points_{0};

@@ -241,6 +241,19 @@ int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
                                         unsigned int per_query_cap,
                                         cublox_stream s);
 
+/* Dense gather after a batched fetch: query q's first min(d_counts[q],
+ * per_query_cap) samples are copied from its segment to position
+ * d_offsets[q] of the dense outputs (offsets computed by the caller, e.g.
+ * an exclusive scan of the capped counts). max_count bounds the capped
+ * counts (sizes the launch). */
+int cublox_launch_fetch_gather(const unsigned int *d_counts,
+                               const unsigned int *d_offsets, int n_queries,
+                               unsigned int per_query_cap,
+                               unsigned int max_count, const float *d_pos_seg,
+                               const unsigned char *d_state_seg,
+                               float *d_pos_out, unsigned char *d_state_out,
+                               cublox_stream s);
+
 /* ------------------------------------------------------------------ */
 /* Fused scan update (no reference counterpart as one call; it replaces the
  * pair launchRayCastUpdate + launchApplyUpdate, kernels.hpp:54-76, with the

@@ -18,7 +18,7 @@ def _upload(cb, scan):
             for c in (scan.xs, scan.ys, scan.zs)]
 
 
-@pytest.mark.parametrize("near_steps", [1, 4, 16, 40])
+@pytest.mark.parametrize("near_steps", [1, 4, 8, 12])
 def test_fused_counts_small(cb, orc, gpu_ready, near_steps):
     w = cb.WORKLOADS["small"]
     scene = w.scene()
