@@ -16,13 +16,15 @@
 //   bin_count   per workgroup, an LDS hash table aggregates its segments by
 //               tile; one global atomic per (workgroup, tile).
 //   tile_scan   one workgroup: tile offsets, and the work items — one per
-//               tile, or one per 1024-segment chunk of a hot tile.
+//               tile, or one per 1024-segment chunk of a hot tile — ordered
+//               by decreasing cost (log2 classes).
 //   bin_place   the same aggregation; one returning atomic per (workgroup,
 //               tile) reserves a range; segments are copied to tile order.
-//   tile_apply  one workgroup per work item: resume every segment inside the
-//               tile with LDS counters, then fold the tile's counts into occ
-//               with coalesced z-runs. Chunks of a hot tile add their counts
-//               into a scratch block; the last chunk to finish folds it.
+//   tile_apply  one workgroup per work item: resume every segment inside
+//               the tile with packed 16+16-bit LDS counters (32 KB), then
+//               fold the tile's counts into occ with all loads in flight.
+//               Chunks of a hot tile store their counts to scratch slabs.
+//   hot_fold    sums each split tile's slabs and folds, 8 workgroups a tile.
 //   near_apply  fold the near-field cube into occ and clear it.
 //
 // The walk's t_max recurrence is a running FP32 sum, so a segment resumed
@@ -45,8 +47,14 @@ constexpr int kMaxCubeCells = kMaxCube * kMaxCube * kMaxCube;
 constexpr int kBinRays = 128;
 constexpr int kHash = 4096;
 constexpr int kHashProbes = 64;
-constexpr int kChunk = 1024;
-constexpr int kMaxSplit = 256;
+constexpr int kChunk = 1024;      // records per chunk of a hot tile
+constexpr int kMaxSlabs = 2048;   // scratch slabs (chunk counts), 32 KB each
+constexpr int kMaxSplit = kMaxSlabs / 2;  // a split tile has >= 2 chunks
+constexpr int kPass = 65535;      // records per LDS pass: 16-bit counts
+constexpr int kApplyThreads = 256;
+constexpr int kPerThread = kTileVox / kApplyThreads;  // 32 voxels
+constexpr int kHotSlices = 8;     // hot_fold workgroups per split tile
+constexpr int kCostBuckets = 18;  // log2 classes of item cost
 
 struct Segment {  // 64 B, self-contained; step < 0 marks an endpoint record
   int step;
@@ -59,8 +67,13 @@ struct Segment {  // 64 B, self-contained; step < 0 marks an endpoint record
   int tile;
 };
 
-struct Item {  // one tile_apply workgroup
-  int tile, beg, end, split;  // split < 0: fold directly
+struct Item {  // one tile_apply work unit: records [beg, end) of a tile
+  int tile, beg, end;
+  int slab;  // < 0: accumulate and fold; else a hot-tile chunk's slab
+};
+
+struct HotTile {  // a tile split into chunks, folded by hot_fold
+  int tile, slab0, chunks, pad;
 };
 
 struct IntegrateArgs {
@@ -80,10 +93,10 @@ struct Workspace {
   int *tile_count;    // n_tiles (zero between launches)
   int *tile_offset;   // n_tiles + 1
   int *tile_cursor;   // n_tiles
-  Item *items;        // max_items
-  int *split_left;    // kMaxSplit: chunks not yet folded
-  int *scratch;       // kMaxSplit * 2 * kTileVox (zero between launches)
-  int *misc;          // [0] items, [1] segments, [2] overflow flag
+  Item *items;        // max_items, costliest first
+  HotTile *hot;       // kMaxSplit
+  unsigned *scratch;  // kMaxSlabs * kTileVox: chunk counts, op | hit << 16
+  int *misc;          // [0] items, [1] segments, [2] overflow flag, [3] split tiles
   int *near_op;       // cube^3 (zero between launches)
   int *near_hit;      // cube^3 (zero between launches)
   long long seg_capacity;
@@ -296,119 +309,116 @@ __device__ __forceinline__ int block_scan_incl(int *s, int v) {
   return r;
 }
 
+// Cost class of a work item (its record count), larger classes first.
+__device__ __forceinline__ int cost_bucket(int records) {
+  const int lg = 31 - __clz(max(records, 1));
+  return kCostBuckets - 1 - min(lg, kCostBuckets - 1);
+}
+
+// Number of chunks a tile with c records is split into (1: not split).
+__device__ __forceinline__ int chunks_of(int c) {
+  return c > kChunk ? (c + kChunk - 1) / kChunk : 1;
+}
+
 __global__ void __launch_bounds__(kScanThreads)
     tile_scan_kernel(const int n_tiles, Workspace w) {
   __shared__ int s_buf[kScanThreads];
-  __shared__ int s_tot[3];
+  __shared__ int s_bucket[kCostBuckets];
+  __shared__ int s_cursor[kCostBuckets];
   const int t = threadIdx.x;
+  if (t < kCostBuckets) s_bucket[t] = 0;
   const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
   const int beg = min(t * per, n_tiles);
   const int end = min(beg + per, n_tiles);
-  // Pass 1: segments and hot tiles, so each hot tile knows its split rank.
-  int seg = 0, hot = 0;
+  // Pass 1: records and the scratch slabs that splitting would take.
+  int seg = 0, slabs = 0;
   for (int k = beg; k < end; ++k) {
     const int c = w.tile_count[k];
     seg += c;
-    hot += c > kChunk;
+    if (c > kChunk) slabs += chunks_of(c);
   }
   const int seg_incl = block_scan_incl(s_buf, seg);
-  const int hot_incl = block_scan_incl(s_buf, hot);
-  // Pass 2: work items, now that the split decision of each tile is known
-  // (beyond kMaxSplit hot tiles, a hot tile is folded by one workgroup).
-  int item = 0;
-  {
-    int sp = hot_incl - hot;
-    for (int k = beg; k < end; ++k) {
-      const int c = w.tile_count[k];
-      if (c <= 0) continue;
-      if (c > kChunk) {
-        item += sp < kMaxSplit ? (c + kChunk - 1) / kChunk : 1;
-        ++sp;
-      } else {
-        item += 1;
-      }
+  const int slab_incl = block_scan_incl(s_buf, slabs);
+  // A hot tile is split when its whole slab range fits the scratch area (a
+  // prefix of the hot tiles in tile order); otherwise one workgroup takes
+  // it in 16-bit passes. Pass 2: work items per cost class.
+  int sl = slab_incl - slabs;
+  int hot = 0;
+  for (int k = beg; k < end; ++k) {
+    const int c = w.tile_count[k];
+    if (c <= 0) continue;
+    const int ch = chunks_of(c);
+    if (ch > 1 && sl + ch <= kMaxSlabs) {
+      atomicAdd(s_bucket + cost_bucket((c + ch - 1) / ch), ch);
+      ++hot;
+    } else {
+      atomicAdd(s_bucket + cost_bucket(c), 1);
     }
+    if (ch > 1) sl += ch;
   }
-  const int item_incl = block_scan_incl(s_buf, item);
-  // Pass 3: write offsets, cursors, items; clear the counts.
+  const int hot_incl = block_scan_incl(s_buf, hot);
+  if (t == 0) {
+    int run = 0;
+    for (int b = 0; b < kCostBuckets; ++b) {
+      s_cursor[b] = run;
+      run += s_bucket[b];
+    }
+    w.misc[0] = min(run, w.max_items);
+  }
+  __syncthreads();
+  // Pass 3: offsets, cursors, items, split tiles; clear the counts.
   int run = seg_incl - seg;
-  int it = item_incl - item;
   int sp = hot_incl - hot;
+  sl = slab_incl - slabs;
   for (int k = beg; k < end; ++k) {
     const int c = w.tile_count[k];
     w.tile_offset[k] = run;
     w.tile_cursor[k] = run;
     if (c > 0) {
-      int split_id = -1;
-      if (c > kChunk) {
-        if (sp < kMaxSplit) split_id = sp;
-        ++sp;
-      }
-      const int chunks = split_id >= 0 ? (c + kChunk - 1) / kChunk : 1;
-      if (split_id >= 0) w.split_left[split_id] = chunks;
+      const int ch = chunks_of(c);
+      const bool split = ch > 1 && sl + ch <= kMaxSlabs;
+      const int chunks = split ? ch : 1;
+      const int per_chunk = (c + chunks - 1) / chunks;
+      if (split) w.hot[sp++] = HotTile{k, sl, ch, 0};
+      // Every chunk is classed by per_chunk, as counted in pass 2.
+      const int bucket = cost_bucket(per_chunk);
       for (int q = 0; q < chunks; ++q) {
-        if (it < w.max_items) {
-          Item itm;
-          itm.tile = k;
-          itm.beg = run + q * kChunk;
-          itm.end = split_id >= 0 ? min(run + (q + 1) * kChunk, run + c) : run + c;
-          itm.split = split_id;
-          w.items[it] = itm;
+        const int ib = run + q * per_chunk;
+        const int ie = min(ib + per_chunk, run + c);
+        const int pos = atomicAdd(s_cursor + bucket, 1);
+        if (pos < w.max_items) {
+          w.items[pos] = Item{k, ib, ie, split ? sl + q : -1};
         }
-        ++it;
       }
+      if (ch > 1) sl += ch;
     }
     run += c;
     w.tile_count[k] = 0;
   }
   if (t == kScanThreads - 1) {
-    s_tot[0] = seg_incl;
-    s_tot[1] = item_incl;
     w.tile_offset[n_tiles] = seg_incl;
-    w.misc[0] = min(item_incl, w.max_items);
     w.misc[1] = seg_incl;
+    w.misc[3] = hot_incl;
   }
 }
 
 // ----------------------------------------------------------------------
 // tile_apply
 // ----------------------------------------------------------------------
-__device__ __forceinline__ int tile_voxel_hash(const RayArgs &a, int x0, int y0,
-                                               int z0, int k) {
-  const int lz = k % TZ;
-  const int ly = (k / TZ) % TY;
-  const int lx = k / (TZ * TY);
-  return (x0 + lx) * (a.my * a.mz) + (y0 + ly) * a.mz + (z0 + lz);
-}
-
-__global__ void __launch_bounds__(256)
-    tile_apply_kernel(const IntegrateArgs g, Workspace w,
-                      float *__restrict__ occ, int *__restrict__ export_op,
-                      int *__restrict__ export_hit) {
-  __shared__ int s_op[kTileVox];
-  __shared__ int s_hit[kTileVox];
-  __shared__ int s_last;
-  if (static_cast<int>(blockIdx.x) >= w.misc[0]) return;
-  const Item itm = w.items[blockIdx.x];
+// Accumulates records [beg, end) of a tile into packed LDS counters
+// (op | hit << 16). A pass holds at most kPass records and a record visits
+// a voxel at most once, so neither field carries into the other.
+__device__ __forceinline__ void tile_accumulate(const IntegrateArgs &g,
+                                                const Workspace &w, int beg,
+                                                int end, int x0, int y0,
+                                                int z0, unsigned *s_cnt) {
   const RayArgs &a = g.a;
-  const int tz = itm.tile % g.ntz;
-  const int ty = (itm.tile / g.ntz) % g.nty;
-  const int tx = itm.tile / (g.ntz * g.nty);
-  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-  for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
-    s_op[k] = 0;
-    s_hit[k] = 0;
-  }
-  __syncthreads();
-
-  const int end = static_cast<int>(min(static_cast<long long>(itm.end), w.seg_capacity));
-  for (int k = itm.beg + static_cast<int>(threadIdx.x); k < end; k += blockDim.x) {
+  for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
     const Segment sg = w.sorted[k];
     if (sg.step < 0) {
       const HashCoord e = hash_coord_of(a, sg.ix, sg.iy, sg.iz);
       const int li = ((e.cx - x0) * TY + (e.cy - y0)) * TZ + (e.cz - z0);
-      atomicAdd(s_op + li, 1);
-      if ((sg.flags & 64) == 0) atomicAdd(s_hit + li, 1);
+      atomicAdd(s_cnt + li, (sg.flags & 64) ? 1u : 0x10001u);
       continue;
     }
     RayState r;
@@ -441,65 +451,174 @@ __global__ void __launch_bounds__(256)
           static_cast<unsigned>(ly) < static_cast<unsigned>(TY) &&
           static_cast<unsigned>(lz) < static_cast<unsigned>(TZ);
       if (!go) break;
-      atomicAdd(s_op + (lx * TY + ly) * TZ + lz, 1);
+      atomicAdd(s_cnt + (lx * TY + ly) * TZ + lz, 1u);
       const int ax = ray_advance(r);
       hash_coord_step(hc, ax, r, a);
     }
   }
-  __syncthreads();
+}
 
-  const int *fold_op = s_op;
-  const int *fold_hit = s_hit;
-  if (itm.split >= 0) {
-    // Hot tile: add this chunk's counts into the tile's scratch block; the
-    // last chunk to arrive folds the block and clears it.
-    int *sc_op = w.scratch + static_cast<size_t>(itm.split) * 2 * kTileVox;
-    int *sc_hit = sc_op + kTileVox;
-    for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
-      if (s_op[k]) atomic_add_i32(sc_op + k, s_op[k]);
-      if (s_hit[k]) atomic_add_i32(sc_hit + k, s_hit[k]);
+__device__ __forceinline__ void lds_zero(uint4 *s4) {
+  for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
+    s4[k] = make_uint4(0, 0, 0, 0);
+  }
+}
+
+// Thread t owns tile voxels j * 256 + t (j < 32): a z-run of 32 contiguous
+// floats per 32 lanes. Hash offset of its j-th voxel:
+__device__ __forceinline__ int owned_hash(const RayArgs &a, int x0, int y0,
+                                          int z0, int j) {
+  const int t = threadIdx.x;
+  const int lx = j >> 1;
+  const int ly = ((j & 1) << 3) + (t >> 5);
+  const int lz = t & 31;
+  return (x0 + lx) * (a.my * a.mz) + (y0 + ly) * a.mz + (z0 + lz);
+}
+
+// Fold of a thread's owned counts: every occ load is issued before the
+// first update, so a tile costs one memory round trip.
+__device__ __forceinline__ void fold_owned(const IntegrateArgs &g, int x0,
+                                           int y0, int z0, const int *op,
+                                           const int *hit,
+                                           float *__restrict__ occ,
+                                           int *__restrict__ export_op,
+                                           int *__restrict__ export_hit) {
+  const RayArgs &a = g.a;
+  if (export_op) {
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      if (op[j] == 0) continue;
+      const int h = owned_hash(a, x0, y0, z0, j);
+      export_op[h] = op[j];
+      export_hit[h] = hit[j];
     }
-    __threadfence();
+    return;
+  }
+  float v[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    v[j] = op[j] != 0 ? occ[owned_hash(a, x0, y0, z0, j)] : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    if (op[j] == 0) continue;
+    occ[owned_hash(a, x0, y0, z0, j)] = apply_logodds(v[j], op[j], hit[j], g.p);
+  }
+}
+
+// One workgroup per work item, items in decreasing cost order so the
+// hardware dispatcher balances them. A plain item accumulates its tile in
+// LDS passes of kPass records, sums the passes in registers and folds. A
+// chunk of a hot tile stores its packed counts to its scratch slab;
+// hot_fold sums a tile's slabs and folds.
+__global__ void __launch_bounds__(kApplyThreads)
+    tile_apply_kernel(const IntegrateArgs g, Workspace w,
+                      float *__restrict__ occ, int *__restrict__ export_op,
+                      int *__restrict__ export_hit) {
+  __shared__ uint4 s_cnt4[kTileVox / 4];
+  unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
+  if (static_cast<int>(blockIdx.x) >= w.misc[0]) return;
+  const Item itm = w.items[blockIdx.x];
+  const int tz = itm.tile % g.ntz;
+  const int ty = (itm.tile / g.ntz) % g.nty;
+  const int tx = itm.tile / (g.ntz * g.nty);
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  const int end = static_cast<int>(
+      min(static_cast<long long>(itm.end), w.seg_capacity));
+  if (itm.slab >= 0) {
+    lds_zero(s_cnt4);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      s_last = atomicSub(w.split_left + itm.split, 1) == 1;
+    tile_accumulate(g, w, itm.beg, end, x0, y0, z0, s_cnt);
+    __syncthreads();
+    uint4 *slab = reinterpret_cast<uint4 *>(
+        w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
+    for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
+      slab[k] = s_cnt4[k];
     }
+    return;
+  }
+  int op[kPerThread], hit[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    op[j] = 0;
+    hit[j] = 0;
+  }
+  for (int b = itm.beg; b < end; b += kPass) {
+    lds_zero(s_cnt4);
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    for (int k = threadIdx.x; k < kTileVox; k += blockDim.x) {
-      s_op[k] = __hip_atomic_exchange(sc_op + k, 0, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-      s_hit[k] = __hip_atomic_exchange(sc_hit + k, 0, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+    tile_accumulate(g, w, b, min(b + kPass, end), x0, y0, z0, s_cnt);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
+      op[j] += static_cast<int>(c & 0xFFFFu);
+      hit[j] += static_cast<int>(c >> 16);
     }
     __syncthreads();
   }
+  fold_owned(g, x0, y0, z0, op, hit, occ, export_op, export_hit);
+}
 
-  // Fold: consecutive threads take consecutive tile voxels (z-runs of TZ
-  // contiguous floats); eight voxels per thread have their loads in flight
-  // together.
-  constexpr int kBatch = 8;
-  for (int b0 = 0; b0 < kTileVox; b0 += kBatch * 256) {
-    int o[kBatch], hit[kBatch], hh[kBatch];
-    float v[kBatch];
+// Sums the scratch slabs of each split tile and folds: kHotSlices
+// workgroups per tile, four consecutive voxels (one z-run piece) per lane.
+__global__ void __launch_bounds__(256)
+    hot_fold_kernel(const IntegrateArgs g, Workspace w,
+                    float *__restrict__ occ, int *__restrict__ export_op,
+                    int *__restrict__ export_hit) {
+  const int hot = blockIdx.x / kHotSlices;
+  if (hot >= w.misc[3]) return;
+  const HotTile ht = w.hot[hot];
+  const int k4 = (blockIdx.x % kHotSlices) * 256 + threadIdx.x;  // < 2048
+  const uint4 *src = reinterpret_cast<const uint4 *>(
+                         w.scratch + static_cast<size_t>(ht.slab0) * kTileVox) +
+                     k4;
+  unsigned op[4] = {0, 0, 0, 0}, hit[4] = {0, 0, 0, 0};
+  int q = 0;
+  for (; q + 4 <= ht.chunks; q += 4) {
+    uint4 v[4];
 #pragma unroll
-    for (int j = 0; j < kBatch; ++j) {
-      const int kk = b0 + j * 256 + static_cast<int>(threadIdx.x);
-      o[j] = fold_op[kk];
-      hit[j] = fold_hit[kk];
-      hh[j] = tile_voxel_hash(a, x0, y0, z0, kk);
-      v[j] = (o[j] != 0 && !export_op) ? occ[hh[j]] : 0.0f;
-    }
+    for (int u = 0; u < 4; ++u) v[u] = src[(q + u) * (kTileVox / 4)];
 #pragma unroll
-    for (int j = 0; j < kBatch; ++j) {
-      if (o[j] == 0) continue;
-      if (export_op) {
-        export_op[hh[j]] = o[j];
-        export_hit[hh[j]] = hit[j];
-      } else {
-        occ[hh[j]] = apply_logodds(v[j], o[j], hit[j], g.p);
+    for (int u = 0; u < 4; ++u) {
+      const unsigned vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        op[e] += vv[e] & 0xFFFFu;
+        hit[e] += vv[e] >> 16;
       }
+    }
+  }
+  for (; q < ht.chunks; ++q) {
+    const uint4 v = src[q * (kTileVox / 4)];
+    const unsigned vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      op[e] += vv[e] & 0xFFFFu;
+      hit[e] += vv[e] >> 16;
+    }
+  }
+  const RayArgs &a = g.a;
+  const int tz = ht.tile % g.ntz;
+  const int ty = (ht.tile / g.ntz) % g.nty;
+  const int tx = ht.tile / (g.ntz * g.nty);
+  const int k = 4 * k4;
+  const int lz = k % TZ, ly = (k / TZ) % TY, lx = k / (TZ * TY);
+  const int h0 = (tx * TX + lx) * (a.my * a.mz) + (ty * TY + ly) * a.mz +
+                 (tz * TZ + lz);
+  float v[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[e] = (op[e] != 0 && !export_op) ? occ[h0 + e] : 0.0f;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (op[e] == 0) continue;
+    if (export_op) {
+      export_op[h0 + e] = static_cast<int>(op[e]);
+      export_hit[h0 + e] = static_cast<int>(hit[e]);
+    } else {
+      occ[h0 + e] = apply_logodds(v[e], static_cast<int>(op[e]),
+                                  static_cast<int>(hit[e]), g.p);
     }
   }
 }
@@ -548,7 +667,7 @@ struct Layout {
   int ntx, nty, ntz, n_tiles, cube, max_seg, max_items;
   long long seg_capacity;
   size_t off_count, off_segs, off_sorted, off_tcount, off_toff, off_tcur,
-      off_items, off_split, off_scratch, off_misc, off_nop, off_nhit, total;
+      off_items, off_hot, off_scratch, off_misc, off_nop, off_nhit, total;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
@@ -575,8 +694,8 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
   L.max_seg = static_cast<int>(std::min(win_bound, range_bound));
   const size_t n = static_cast<size_t>(std::max(max_points, 1));
   L.seg_capacity = static_cast<long long>(n) * L.max_seg;
-  L.max_items = L.n_tiles + static_cast<int>(std::min<long long>(
-                                L.seg_capacity / kChunk + 1, 1 << 30));
+  // An unsplit tile is one item; a split tile's chunks each own a slab.
+  L.max_items = L.n_tiles + kMaxSlabs;
   size_t o = 0;
   L.off_count = o;
   o = align256(o + n * sizeof(int));
@@ -592,10 +711,10 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
   o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
   L.off_items = o;
   o = align256(o + static_cast<size_t>(L.max_items) * sizeof(Item));
-  L.off_split = o;
-  o = align256(o + kMaxSplit * 4);
+  L.off_hot = o;
+  o = align256(o + kMaxSplit * sizeof(HotTile));
   L.off_scratch = o;
-  o = align256(o + static_cast<size_t>(kMaxSplit) * 2 * kTileVox * 4);
+  o = align256(o + static_cast<size_t>(kMaxSlabs) * kTileVox * 4);
   L.off_misc = o;
   o = align256(o + 64 * 4);
   const size_t cells = static_cast<size_t>(L.cube) * L.cube * L.cube;
@@ -617,8 +736,8 @@ Workspace bind(const Layout &L, void *base) {
   w.tile_offset = reinterpret_cast<int *>(b + L.off_toff);
   w.tile_cursor = reinterpret_cast<int *>(b + L.off_tcur);
   w.items = reinterpret_cast<Item *>(b + L.off_items);
-  w.split_left = reinterpret_cast<int *>(b + L.off_split);
-  w.scratch = reinterpret_cast<int *>(b + L.off_scratch);
+  w.hot = reinterpret_cast<HotTile *>(b + L.off_hot);
+  w.scratch = reinterpret_cast<unsigned *>(b + L.off_scratch);
   w.misc = reinterpret_cast<int *>(b + L.off_misc);
   w.near_op = reinterpret_cast<int *>(b + L.off_nop);
   w.near_hit = reinterpret_cast<int *>(b + L.off_nhit);
@@ -704,13 +823,13 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
   bin_kernel<true><<<bg, dim3(256), 0, hs>>>(g, w);
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
-  // One workgroup per work item; items beyond the device count exit.
-  const long long est_items =
-      std::min<long long>(L.max_items, L.n_tiles + (static_cast<long long>(n) *
-                                                        L.max_seg) / kChunk + 1);
-  tile_apply_kernel<<<dim3(static_cast<unsigned>(est_items)), dim3(256), 0, hs>>>(
+  tile_apply_kernel<<<dim3(L.max_items), dim3(kApplyThreads), 0, hs>>>(
       g, w, d_occ, d_export_op, d_export_hit);
   st = check_launch("tile_apply_kernel");
+  if (st != CUBLOX_OK) return st;
+  hot_fold_kernel<<<dim3(kMaxSplit * kHotSlices), dim3(256), 0, hs>>>(
+      g, w, d_occ, d_export_op, d_export_hit);
+  st = check_launch("hot_fold_kernel");
   if (st != CUBLOX_OK) return st;
   const int cells = L.cube * L.cube * L.cube;
   near_apply_kernel<<<dim3(std::max(1, std::min((cells + 255) / 256, 1024))),
