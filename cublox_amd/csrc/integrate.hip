@@ -11,8 +11,9 @@
 //               LDS cube around the sensor voxel, flushed once per
 //               workgroup. Beyond, the walk is cut into "segments" at every
 //               entry into a 16x16x32 tile of the hash-ordered grid: each
-//               segment carries the exact DDA state at the tile entry plus
-//               the ray's constants and goes to the ray's own slot range.
+//               segment (32 B) carries the exact DDA state at the tile entry
+//               and goes to the ray's own slot range, its tile key to a
+//               parallel array; the ray's constants are stored once.
 //   bin_count   per workgroup, an LDS hash table aggregates its segments by
 //               tile; one global atomic per (workgroup, tile).
 //   tile_scan   one workgroup: tile offsets, and the work items — one per
@@ -42,8 +43,6 @@ constexpr int TX = 16, TY = 16, TZ = 32;
 constexpr int kTileVox = TX * TY * TZ;  // 8192
 constexpr int kWalkBlock = 256;
 constexpr int kMaxNearK = 12;
-constexpr int kMaxCube = 2 * kMaxNearK - 1;  // 23
-constexpr int kMaxCubeCells = kMaxCube * kMaxCube * kMaxCube;
 constexpr int kBinRays = 128;
 constexpr int kHash = 4096;
 constexpr int kHashProbes = 64;
@@ -56,16 +55,24 @@ constexpr int kPerThread = kTileVox / kApplyThreads;  // 32 voxels
 constexpr int kHotSlices = 8;     // hot_fold workgroups per split tile
 constexpr int kCostBuckets = 18;  // log2 classes of item cost
 
-struct Segment {  // 64 B, self-contained; step < 0 marks an endpoint record
-  int step;
-  int ix, iy, iz;
+// A segment is the part of one ray inside one tile span: its DDA state at
+// the span entry, the increments, and in `meta` the entry voxel's index in
+// the tile (bits 0..12), the number of voxels it visits (13..19), the step
+// signs (20..25, s + 1 two bits each), an endpoint flag (26) and the
+// clipped flag (27). An endpoint record adds op (and hit unless clipped) at
+// its voxel.
+struct Segment {  // 32 B
   float tMx, tMy, tMz;
   float tDx, tDy, tDz;
-  int eix, eiy, eiz;
-  int max_steps;
-  int flags;  // bits 0..5: sx+1, sy+1, sz+1 (2 bits each); bit 6: clipped
-  int tile;
+  unsigned meta;
+  unsigned pad;
 };
+
+constexpr unsigned kMetaLi = 0x1FFFu;
+constexpr int kMetaN = 13;
+constexpr int kMetaSign = 20;
+constexpr unsigned kMetaEndpoint = 1u << 26;
+constexpr unsigned kMetaClipped = 1u << 27;
 
 struct Item {  // one tile_apply work unit: records [beg, end) of a tile
   int tile, beg, end;
@@ -83,15 +90,44 @@ struct IntegrateArgs {
   int near_k;   // K
   int cube;     // 2K - 1
   int max_seg;  // per-ray segment slots
+  int qx, qy, qz;  // window origin modulo the map size (non-negative)
   ApplyArgs p;
 };
 
+// Hash coordinate of an in-window global coordinate g (|g - o| <= h, map
+// m = 2h + 1): (g + h) mod m, i.e. global_to_local(g) + h, as
+// (q + (g - o + h)) mod m with q = o mod m and q + (g - o + h) < 2m.
+__device__ __forceinline__ int window_hash(int g, int o, int h, int m, int q) {
+  const int v = q + (g - o + h);
+  return v >= m ? v - m : v;
+}
+
+__device__ __forceinline__ HashCoord window_hash_coord(const IntegrateArgs &g,
+                                                       int gx, int gy, int gz) {
+  const RayArgs &a = g.a;
+  return HashCoord{window_hash(gx, a.oix, a.hx, a.mx, g.qx),
+                   window_hash(gy, a.oiy, a.hy, a.my, g.qy),
+                   window_hash(gz, a.oiz, a.hz, a.mz, g.qz)};
+}
+
+// Tile span origin entered when a walk in direction s leaves the span at
+// t0 of an axis with map m and tile width T (the seam joins m - 1 and 0),
+// and the width of a span.
+__device__ __forceinline__ int next_span(int t0, int s, int m, int T) {
+  if (s > 0) return t0 + T >= m ? 0 : t0 + T;
+  return t0 == 0 ? ((m - 1) / T) * T : t0 - T;
+}
+
+__device__ __forceinline__ int span_width(int t0, int m, int T) {
+  return min(T, m - t0);
+}
+
 struct Workspace {
   int *seg_count;     // n: segments written by each ray
+  int *seg_tile;      // n * max_seg: tile of each slot's segment
   Segment *segs;      // n * max_seg, per-ray slot ranges
   Segment *sorted;    // n * max_seg, tile order
   int *tile_count;    // n_tiles (zero between launches)
-  int *tile_offset;   // n_tiles + 1
   int *tile_cursor;   // n_tiles
   Item *items;        // max_items, costliest first
   HotTile *hot;       // kMaxSplit
@@ -108,21 +144,247 @@ __device__ __forceinline__ int tile_of(const IntegrateArgs &g, int cx, int cy,
   return ((cx / TX) * g.nty + (cy / TY)) * g.ntz + (cz / TZ);
 }
 
+// Tile of the spans starting at hash coordinates (t0x, t0y, t0z).
+__device__ __forceinline__ int span_tile(const IntegrateArgs &g, int t0x,
+                                         int t0y, int t0z) {
+  const unsigned tx = static_cast<unsigned>(t0x) / TX;
+  const unsigned ty = static_cast<unsigned>(t0y) / TY;
+  const unsigned tz = static_cast<unsigned>(t0z) / TZ;
+  return static_cast<int>((tx * g.nty + ty) * g.ntz + tz);
+}
+
 __device__ __forceinline__ int cube_index(const IntegrateArgs &g, int dx,
                                           int dy, int dz) {
   const int k1 = g.near_k - 1;
   return ((dx + k1) * g.cube + (dy + k1)) * g.cube + (dz + k1);
 }
 
+constexpr int kNever = INT_MIN;  // a coordinate no walk reaches
+
+// First global coordinate past the window [o - h, o + h] for travel
+// direction s (kNever when the axis does not move).
+__device__ __forceinline__ int window_exit(int s, int o, int h) {
+  return s > 0 ? o + h + 1 : (s < 0 ? o - h - 1 : kNever);
+}
+
+// First global coordinate, travelling in direction s from global g with
+// hash coordinate c, outside both the tile span [t0, t0 + tw) of hash
+// coordinates and the window (wexit from window_exit). Inside a tile span
+// the hash coordinate moves with the global one (the wrap seam is a tile
+// boundary).
+__device__ __forceinline__ int exit_coord(int g, int s, int c, int t0, int tw,
+                                          int wexit) {
+  if (s > 0) return min(g + (t0 + tw - c), wexit);
+  if (s < 0) return max(g - (c - t0) - 1, wexit);
+  return kNever;
+}
+
 // ----------------------------------------------------------------------
 // walk
 // ----------------------------------------------------------------------
+constexpr int kBig = 1 << 20;  // steps to the exit of an axis that never moves
+
+// A_k of the running sum A_0 = h, A_{j+1} = A_j + d, formed in order as the
+// walk forms its crossing parameters.
+__device__ __forceinline__ float seq_at(float h, float d, int k) {
+  for (int j = 0; j < k; ++j) h += d;
+  return h;
+}
+
+// The walk steps the axis with the smallest next crossing parameter, z
+// before y before x on ties (reference kernels.cu:120-136), so its steps
+// are the merge of the three per-axis running sums in that order. Steps of
+// one axis (next parameter h, increment d, at most nb of them) that come
+// before an event at parameter e: those below e, or equal to it when the
+// axis goes first on ties. h is left at the axis's next parameter.
+__device__ __forceinline__ int count_before(float &h, float d, int nb, float e,
+                                            bool first_on_tie) {
+  int c = 0;
+  while (c < nb && (h < e || (first_on_tie && h == e))) {
+    h += d;
+    ++c;
+  }
+  return c;
+}
+
+// Whether the walk, from its current state, is at the end voxel after
+// dx + dy + dz more steps (dx, dy, dz: steps still to go on each axis):
+// every step it needs must come before every step of another axis past
+// the end voxel.
+__device__ __forceinline__ bool end_reached(const RayState &r, int dx, int dy,
+                                            int dz) {
+  const float inf = __builtin_inff();
+  const float inx = dx > 0 ? seq_at(r.tMx, r.tDx, dx - 1) : -inf;
+  const float iny = dy > 0 ? seq_at(r.tMy, r.tDy, dy - 1) : -inf;
+  const float inz = dz > 0 ? seq_at(r.tMz, r.tDz, dz - 1) : -inf;
+  if (!(inx < inf) || !(iny < inf) || !(inz < inf)) return false;
+  const float ox = dx > 0 ? inx + r.tDx : r.tMx;
+  const float oy = dy > 0 ? iny + r.tDy : r.tMy;
+  const float oz = dz > 0 ? inz + r.tDz : r.tMz;
+  return inx < oy && inx < oz && iny <= ox && iny < oz && inz <= ox &&
+         inz <= oy;
+}
+
+// Tile-local index of hash coordinates in their own tile.
+__device__ __forceinline__ int tile_local(int cx, int cy, int cz) {
+  return ((cx % TX) * TY + cy % TY) * TZ + cz % TZ;
+}
+
+// Far field one step at a time from `step` (r: the walk's state there):
+// one segment per maximal run of in-window voxels inside one tile span,
+// cut at the wrap seam. Used when the sensor voxel is outside the window
+// and for the last span of a walk that passes its end voxel by.
+template <typename Emit>
+__device__ __forceinline__ void far_by_step(const IntegrateArgs &g,
+                                            RayState &r, int step,
+                                            Emit &emit) {
+  const RayArgs &a = g.a;
+  HashCoord hc = hash_coord_of(a, r.ix, r.iy, r.iz);
+  int cur = -1, open_step = 0, open_li = 0;
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  for (; step < r.max_steps; ++step) {
+    if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
+    const bool inwin = abs(r.ix - a.oix) <= a.hx &&
+                       abs(r.iy - a.oiy) <= a.hy && abs(r.iz - a.oiz) <= a.hz;
+    const int t = inwin ? tile_of(g, hc.cx, hc.cy, hc.cz) : -1;
+    if (t != cur) {
+      if (cur >= 0) emit(ox, oy, oz, open_li, step - open_step, cur);
+      if (t >= 0) {
+        open_step = step;
+        open_li = tile_local(hc.cx, hc.cy, hc.cz);
+        ox = r.tMx;
+        oy = r.tMy;
+        oz = r.tMz;
+      }
+      cur = t;
+    }
+    const int ax = ray_advance(r);
+    const int before = hc.cx + hc.cy + hc.cz;
+    hash_coord_step(hc, ax, r, a);
+    if (abs(hc.cx + hc.cy + hc.cz - before) != 1 && cur >= 0) {  // seam
+      emit(ox, oy, oz, open_li, step + 1 - open_step, cur);
+      cur = -1;
+    }
+  }
+  if (cur >= 0) emit(ox, oy, oz, open_li, step - open_step, cur);
+}
+
+// Far field from `step` when the sensor voxel is inside the window (r is
+// inside too and not at its end voxel), one tile span per iteration. The
+// walk is monotone per axis, so each axis reaches the exit of the span
+// (tile span or window) after a known number of its own steps; the exit
+// comes with the first of those exit steps in merge order, and the other
+// axes' steps before it are counted on their running sums. The exit axis
+// is guessed from rounded estimates and confirmed exactly. The end voxel
+// can only be met at step lend; the span holding that step is decided
+// exactly by end_reached, or walked one step at a time when it is passed.
+template <typename Emit>
+__device__ __forceinline__ void far_by_span(const IntegrateArgs &g,
+                                            RayState &r, int step, int lend,
+                                            int wx, int wy, int wz,
+                                            Emit &emit) {
+  const RayArgs &a = g.a;
+  const float inf = __builtin_inff();
+  const HashCoord hc = window_hash_coord(g, r.ix, r.iy, r.iz);
+  int t0x = hc.cx - hc.cx % TX, t0y = hc.cy - hc.cy % TY,
+      t0z = hc.cz - hc.cz % TZ;
+  int lx = hc.cx - t0x, ly = hc.cy - t0y, lz = hc.cz - t0z;
+  int bx = exit_coord(r.ix, r.sx, hc.cx, t0x, span_width(t0x, a.mx, TX), wx);
+  int by = exit_coord(r.iy, r.sy, hc.cy, t0y, span_width(t0y, a.my, TY), wy);
+  int bz = exit_coord(r.iz, r.sz, hc.cz, t0z, span_width(t0z, a.mz, TZ), wz);
+  while (true) {
+    const int li = (lx * TY + ly) * TZ + lz;
+    const int tile = span_tile(g, t0x, t0y, t0z);
+    const int nbx = r.sx != 0 ? abs(bx - r.ix) : kBig;
+    const int nby = r.sy != 0 ? abs(by - r.iy) : kBig;
+    const int nbz = r.sz != 0 ? abs(bz - r.iz) : kBig;
+    const float gxe = nbx == 1 ? r.tMx : r.tMx + static_cast<float>(nbx - 1) * r.tDx;
+    const float gye = nby == 1 ? r.tMy : r.tMy + static_cast<float>(nby - 1) * r.tDy;
+    const float gze = nbz == 1 ? r.tMz : r.tMz + static_cast<float>(nbz - 1) * r.tDz;
+    int ax = (gze <= gxe && gze <= gye) ? 2 : (gye <= gxe ? 1 : 0);
+    float e = ax == 0 ? seq_at(r.tMx, r.tDx, nbx - 1)
+                      : (ax == 1 ? seq_at(r.tMy, r.tDy, nby - 1)
+                                 : seq_at(r.tMz, r.tDz, nbz - 1));
+    float hx = r.tMx, hy = r.tMy, hz = r.tMz;
+    int cx = count_before(hx, r.tDx, ax == 0 ? 0 : nbx, e, false);
+    int cy = count_before(hy, r.tDy, ax == 1 ? 0 : nby, e, ax == 0);
+    int cz = count_before(hz, r.tDz, ax == 2 ? 0 : nbz, e, true);
+    if ((ax != 0 && cx == nbx) || (ax != 1 && cy == nby) ||
+        (ax != 2 && cz == nbz)) {
+      // Wrong guess: exact exit parameters of every axis.
+      const float ex = r.sx != 0 ? seq_at(r.tMx, r.tDx, nbx - 1) : inf;
+      const float ey = r.sy != 0 ? seq_at(r.tMy, r.tDy, nby - 1) : inf;
+      const float ez = r.sz != 0 ? seq_at(r.tMz, r.tDz, nbz - 1) : inf;
+      ax = (ez <= ex && ez <= ey) ? 2 : (ey <= ex ? 1 : 0);
+      e = ax == 0 ? ex : (ax == 1 ? ey : ez);
+      hx = r.tMx;
+      hy = r.tMy;
+      hz = r.tMz;
+      cx = count_before(hx, r.tDx, ax == 0 ? 0 : nbx, e, false);
+      cy = count_before(hy, r.tDy, ax == 1 ? 0 : nby, e, ax == 0);
+      cz = count_before(hz, r.tDz, ax == 2 ? 0 : nbz, e, true);
+    }
+    if (ax == 0) {
+      cx = nbx;
+      hx = e + r.tDx;
+    } else if (ax == 1) {
+      cy = nby;
+      hy = e + r.tDy;
+    } else {
+      cz = nbz;
+      hz = e + r.tDz;
+    }
+    const int n = cx + cy + cz;  // voxels of this span
+    if (lend <= step + n) {
+      const int tn = lend - step;
+      const int dx = abs(r.eix - r.ix), dy = abs(r.eiy - r.iy),
+                dz = abs(r.eiz - r.iz);
+      if (dx + dy + dz == tn && end_reached(r, dx, dy, dz)) {
+        if (tn > 0) emit(r.tMx, r.tMy, r.tMz, li, tn, tile);
+      } else {
+        far_by_step(g, r, step, emit);
+      }
+      return;
+    }
+    emit(r.tMx, r.tMy, r.tMz, li, n, tile);
+    if ((ax == 0 && bx == wx) || (ax == 1 && by == wy) || (ax == 2 && bz == wz)) {
+      return;  // the next voxel is outside the window
+    }
+    r.ix += r.sx * cx;
+    r.iy += r.sy * cy;
+    r.iz += r.sz * cz;
+    lx += r.sx * cx;
+    ly += r.sy * cy;
+    lz += r.sz * cz;
+    r.tMx = hx;
+    r.tMy = hy;
+    r.tMz = hz;
+    step += n;
+    if (ax == 0) {
+      t0x = next_span(t0x, r.sx, a.mx, TX);
+      const int tw = span_width(t0x, a.mx, TX);
+      bx = r.sx > 0 ? min(r.ix + tw, wx) : max(r.ix - tw, wx);
+      lx = r.sx > 0 ? 0 : tw - 1;
+    } else if (ax == 1) {
+      t0y = next_span(t0y, r.sy, a.my, TY);
+      const int tw = span_width(t0y, a.my, TY);
+      by = r.sy > 0 ? min(r.iy + tw, wy) : max(r.iy - tw, wy);
+      ly = r.sy > 0 ? 0 : tw - 1;
+    } else {
+      t0z = next_span(t0z, r.sz, a.mz, TZ);
+      const int tw = span_width(t0z, a.mz, TZ);
+      bz = r.sz > 0 ? min(r.iz + tw, wz) : max(r.iz - tw, wz);
+      lz = r.sz > 0 ? 0 : tw - 1;
+    }
+  }
+}
+
 template <bool kTransform>
 __global__ void __launch_bounds__(kWalkBlock)
     walk_kernel(const IntegrateArgs g, const float *__restrict__ xs,
                 const float *__restrict__ ys, const float *__restrict__ zs,
                 Workspace w) {
-  __shared__ int s_near[kMaxCubeCells];
+  extern __shared__ int s_near[];  // cube^3 cells, sized at launch
   const int cells = g.cube * g.cube * g.cube;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
   __syncthreads();
@@ -132,53 +394,82 @@ __global__ void __launch_bounds__(kWalkBlock)
   RayState r;
   const bool valid = i < g.n && ray_setup<kTransform>(a, xs, ys, zs, i, r);
   if (valid) {
-    const int flags = (r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4) |
-                      (r.clipped ? 64 : 0);
-    Segment *__restrict__ mine = w.segs + static_cast<size_t>(i) * g.max_seg;
+    const unsigned signs =
+        static_cast<unsigned>((r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4))
+        << kMetaSign;
+    const size_t slot0 = static_cast<size_t>(i) * g.max_seg;
+    Segment *__restrict__ mine = w.segs + slot0;
+    int *__restrict__ mine_tile = w.seg_tile + slot0;
     int nseg = 0;
+    auto put = [&](const Segment &sg, int tile) {
+      if (nseg < g.max_seg) {
+        mine[nseg] = sg;
+        mine_tile[nseg] = tile;
+      } else {
+        w.misc[2] = 1;
+      }
+      ++nseg;
+    };
+    auto emit = [&](float tmx, float tmy, float tmz, int li, int n, int tile) {
+      Segment sg;
+      sg.tMx = tmx;
+      sg.tMy = tmy;
+      sg.tMz = tmz;
+      sg.tDx = r.tDx;
+      sg.tDy = r.tDy;
+      sg.tDz = r.tDz;
+      sg.meta = static_cast<unsigned>(li) |
+                (static_cast<unsigned>(n) << kMetaN) | signs;
+      sg.pad = 0;
+      put(sg, tile);
+    };
     const int s0x = r.ix, s0y = r.iy, s0z = r.iz;  // sensor voxel
-    HashCoord hc = hash_coord_of(a, r.ix, r.iy, r.iz);
-    int cur = -1;
     const int K = g.near_k;
-    for (int step = 0; step < r.max_steps; ++step) {
-      if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) break;
-      const bool inwin = abs(r.ix - a.oix) <= a.hx &&
-                         abs(r.iy - a.oiy) <= a.hy && abs(r.iz - a.oiz) <= a.hz;
-      if (step < K) {
-        if (inwin) {
+    const int near_end = min(K, r.max_steps);
+    int step = 0;
+    if (inside_window(s0x, s0y, s0z, a.oix, a.oiy, a.oiz, a.hx, a.hy, a.hz)) {
+      // Each step moves one axis one voxel in its travel direction, so the
+      // walk is monotone per axis: step s sits at L1 distance s from the
+      // sensor voxel, the end voxel can only be met at step lend, and once
+      // an axis reaches its window exit coordinate no later voxel is inside.
+      const int lend = abs(r.eix - s0x) + abs(r.eiy - s0y) + abs(r.eiz - s0z);
+      const int wx = window_exit(r.sx, a.oix, a.hx);
+      const int wy = window_exit(r.sy, a.oiy, a.hy);
+      const int wz = window_exit(r.sz, a.oiz, a.hz);
+      const int C = g.cube;
+      const int cdx = r.sx * C * C, cdy = r.sy * C, cdz = r.sz;
+      int ci = cube_index(g, 0, 0, 0);
+      bool done = false;
+      for (; step < near_end; ++step) {
+        if ((step == lend && r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) ||
+            r.ix == wx || r.iy == wy || r.iz == wz) {
+          done = true;
+          break;
+        }
+        atomicAdd(s_near + ci, 1);
+        const int ax = ray_advance(r);
+        ci += ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
+      }
+      if (!done && step < r.max_steps &&
+          !(step == lend && r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) &&
+          !(r.ix == wx || r.iy == wy || r.iz == wz)) {
+        far_by_span(g, r, step, lend, wx, wy, wz, emit);
+      }
+    } else {
+      // Sensor outside the window: the ray may enter it later.
+      bool ended = false;
+      for (; step < near_end; ++step) {
+        if (r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) {
+          ended = true;
+          break;
+        }
+        if (inside_window(r.ix, r.iy, r.iz, a.oix, a.oiy, a.oiz, a.hx, a.hy,
+                          a.hz)) {
           atomicAdd(s_near + cube_index(g, r.ix - s0x, r.iy - s0y, r.iz - s0z), 1);
         }
-      } else {
-        const int t = inwin ? tile_of(g, hc.cx, hc.cy, hc.cz) : -1;
-        if (t != cur && t >= 0) {
-          if (nseg < g.max_seg) {
-            Segment sg;
-            sg.step = step;
-            sg.ix = r.ix;
-            sg.iy = r.iy;
-            sg.iz = r.iz;
-            sg.tMx = r.tMx;
-            sg.tMy = r.tMy;
-            sg.tMz = r.tMz;
-            sg.tDx = r.tDx;
-            sg.tDy = r.tDy;
-            sg.tDz = r.tDz;
-            sg.eix = r.eix;
-            sg.eiy = r.eiy;
-            sg.eiz = r.eiz;
-            sg.max_steps = r.max_steps;
-            sg.flags = flags;
-            sg.tile = t;
-            mine[nseg] = sg;
-          } else {
-            w.misc[2] = 1;
-          }
-          ++nseg;
-        }
-        cur = t;
+        ray_advance(r);
       }
-      const int ax = ray_advance(r);
-      hash_coord_step(hc, ax, r, a);
+      if (!ended) far_by_step(g, r, step, emit);
     }
     // Endpoint (kernels.cu:141-151): op += 1, hit += 1 unless clipped.
     if (inside_window(r.eix, r.eiy, r.eiz, a.oix, a.oiy, a.oiz, a.hx, a.hy,
@@ -189,20 +480,14 @@ __global__ void __launch_bounds__(kWalkBlock)
         atomicAdd(s_near + c, 1);
         if (!r.clipped) atomic_add_i32(w.near_hit + c, 1);
       } else {
-        const HashCoord e = hash_coord_of(a, r.eix, r.eiy, r.eiz);
-        if (nseg < g.max_seg) {
-          Segment sg;
-          sg.step = -1;
-          sg.ix = r.eix;
-          sg.iy = r.eiy;
-          sg.iz = r.eiz;
-          sg.flags = flags;
-          sg.tile = tile_of(g, e.cx, e.cy, e.cz);
-          mine[nseg] = sg;
-        } else {
-          w.misc[2] = 1;
-        }
-        ++nseg;
+        const HashCoord e = window_hash_coord(g, r.eix, r.eiy, r.eiz);
+        Segment sg;
+        sg.tMx = sg.tMy = sg.tMz = 0.0f;
+        sg.tDx = sg.tDy = sg.tDz = 0.0f;
+        sg.meta = static_cast<unsigned>(tile_local(e.cx, e.cy, e.cz)) |
+                  kMetaEndpoint | (r.clipped ? kMetaClipped : 0u);
+        sg.pad = 0;
+        put(sg, tile_of(g, e.cx, e.cy, e.cz));
       }
     }
     w.seg_count[i] = min(nseg, g.max_seg);
@@ -258,7 +543,7 @@ __global__ void __launch_bounds__(256)
   const size_t base = static_cast<size_t>(ray) * g.max_seg;
 
   for (int j = j0; j < cnt; j += kJStride) {
-    const int t = w.segs[base + j].tile;
+    const int t = w.seg_tile[base + j];
     const int e = lds_hash_slot(s_key, t, true);
     if (e >= 0) {
       atomicAdd(s_cnt + e, 1);
@@ -280,10 +565,11 @@ __global__ void __launch_bounds__(256)
   if (!kPlace) return;
   __syncthreads();
   for (int j = j0; j < cnt; j += kJStride) {
+    const int t = w.seg_tile[base + j];
     const Segment sg = w.segs[base + j];
-    const int e = lds_hash_slot(s_key, sg.tile, false);
+    const int e = lds_hash_slot(s_key, t, false);
     const int slot = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
-                            : atomicAdd(w.tile_cursor + sg.tile, 1);
+                            : atomicAdd(w.tile_cursor + t, 1);
     if (slot < w.seg_capacity) w.sorted[slot] = sg;
   }
 }
@@ -292,19 +578,32 @@ __global__ void __launch_bounds__(256)
 // tile_scan: offsets, work items, hot-tile splits
 // ----------------------------------------------------------------------
 constexpr int kScanThreads = 1024;
+constexpr int kScanWaves = kScanThreads / 64;
+constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
 
-// Inclusive block scan of one int per thread (Hillis-Steele).
-__device__ __forceinline__ int block_scan_incl(int *s, int v) {
-  const int t = threadIdx.x;
-  s[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    const int a = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += a;
-    __syncthreads();
+__device__ __forceinline__ int wave_scan_incl(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
   }
-  const int r = s[t];
+  return v;
+}
+
+// Inclusive block scan of one int per thread: within waves by shuffles,
+// then across the wave totals.
+__device__ __forceinline__ int block_scan_incl(int *s_wave, int v) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  v = wave_scan_incl(v);
+  if (lane == 63) s_wave[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    const int x = wave_scan_incl(lane < kScanWaves ? s_wave[lane] : 0);
+    if (lane < kScanWaves) s_wave[lane] = x;
+  }
+  __syncthreads();
+  const int r = v + (wv > 0 ? s_wave[wv - 1] : 0);
   __syncthreads();
   return r;
 }
@@ -322,18 +621,30 @@ __device__ __forceinline__ int chunks_of(int c) {
 
 __global__ void __launch_bounds__(kScanThreads)
     tile_scan_kernel(const int n_tiles, Workspace w) {
-  __shared__ int s_buf[kScanThreads];
+  __shared__ int s_buf[kScanWaves];
   __shared__ int s_bucket[kCostBuckets];
   __shared__ int s_cursor[kCostBuckets];
+  __shared__ int s_count[kScanStage];
   const int t = threadIdx.x;
   if (t < kCostBuckets) s_bucket[t] = 0;
+  // Counts staged in LDS with coalesced loads (and cleared for the next
+  // launch) when they fit; otherwise every pass reads them from memory.
+  const bool staged = n_tiles <= kScanStage;
+  if (staged) {
+    for (int k = t; k < n_tiles; k += kScanThreads) {
+      s_count[k] = w.tile_count[k];
+      w.tile_count[k] = 0;
+    }
+    __syncthreads();
+  }
+  auto count = [&](int k) { return staged ? s_count[k] : w.tile_count[k]; };
   const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
   const int beg = min(t * per, n_tiles);
   const int end = min(beg + per, n_tiles);
   // Pass 1: records and the scratch slabs that splitting would take.
   int seg = 0, slabs = 0;
   for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
+    const int c = count(k);
     seg += c;
     if (c > kChunk) slabs += chunks_of(c);
   }
@@ -345,7 +656,7 @@ __global__ void __launch_bounds__(kScanThreads)
   int sl = slab_incl - slabs;
   int hot = 0;
   for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
+    const int c = count(k);
     if (c <= 0) continue;
     const int ch = chunks_of(c);
     if (ch > 1 && sl + ch <= kMaxSlabs) {
@@ -366,13 +677,12 @@ __global__ void __launch_bounds__(kScanThreads)
     w.misc[0] = min(run, w.max_items);
   }
   __syncthreads();
-  // Pass 3: offsets, cursors, items, split tiles; clear the counts.
+  // Pass 3: cursors, items, split tiles; clear the counts.
   int run = seg_incl - seg;
   int sp = hot_incl - hot;
   sl = slab_incl - slabs;
   for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
-    w.tile_offset[k] = run;
+    const int c = count(k);
     w.tile_cursor[k] = run;
     if (c > 0) {
       const int ch = chunks_of(c);
@@ -393,10 +703,9 @@ __global__ void __launch_bounds__(kScanThreads)
       if (ch > 1) sl += ch;
     }
     run += c;
-    w.tile_count[k] = 0;
+    if (!staged) w.tile_count[k] = 0;
   }
   if (t == kScanThreads - 1) {
-    w.tile_offset[n_tiles] = seg_incl;
     w.misc[1] = seg_incl;
     w.misc[3] = hot_incl;
   }
@@ -406,54 +715,37 @@ __global__ void __launch_bounds__(kScanThreads)
 // tile_apply
 // ----------------------------------------------------------------------
 // Accumulates records [beg, end) of a tile into packed LDS counters
-// (op | hit << 16). A pass holds at most kPass records and a record visits
-// a voxel at most once, so neither field carries into the other.
-__device__ __forceinline__ void tile_accumulate(const IntegrateArgs &g,
-                                                const Workspace &w, int beg,
-                                                int end, int x0, int y0,
-                                                int z0, unsigned *s_cnt) {
-  const RayArgs &a = g.a;
+// (op | hit << 16): each segment replays its n voxels from its entry state
+// (the same crossing parameters and tie-break as the walk). A pass holds at
+// most kPass records and a record visits a voxel at most once, so neither
+// field carries into the other.
+__device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
+                                                int end, unsigned *s_cnt) {
   for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
     const Segment sg = w.sorted[k];
-    if (sg.step < 0) {
-      const HashCoord e = hash_coord_of(a, sg.ix, sg.iy, sg.iz);
-      const int li = ((e.cx - x0) * TY + (e.cy - y0)) * TZ + (e.cz - z0);
-      atomicAdd(s_cnt + li, (sg.flags & 64) ? 1u : 0x10001u);
+    int li = static_cast<int>(sg.meta & kMetaLi);
+    if (sg.meta & kMetaEndpoint) {
+      atomicAdd(s_cnt + li, (sg.meta & kMetaClipped) ? 1u : 0x10001u);
       continue;
     }
-    RayState r;
-    r.ix = sg.ix;
-    r.iy = sg.iy;
-    r.iz = sg.iz;
-    r.tMx = sg.tMx;
-    r.tMy = sg.tMy;
-    r.tMz = sg.tMz;
-    r.tDx = sg.tDx;
-    r.tDy = sg.tDy;
-    r.tDz = sg.tDz;
-    r.eix = sg.eix;
-    r.eiy = sg.eiy;
-    r.eiz = sg.eiz;
-    r.max_steps = sg.max_steps;
-    r.sx = (sg.flags & 3) - 1;
-    r.sy = ((sg.flags >> 2) & 3) - 1;
-    r.sz = ((sg.flags >> 4) & 3) - 1;
-    // Hash coordinates stepped with the same seam wrap as the walk, so
-    // tile membership agrees with the segment cut.
-    HashCoord hc = hash_coord_of(a, r.ix, r.iy, r.iz);
-    for (int step = sg.step; step < r.max_steps; ++step) {
-      const int lx = hc.cx - x0, ly = hc.cy - y0, lz = hc.cz - z0;
-      const bool go =
-          !(r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) &&
-          abs(r.ix - a.oix) <= a.hx && abs(r.iy - a.oiy) <= a.hy &&
-          abs(r.iz - a.oiz) <= a.hz &&
-          static_cast<unsigned>(lx) < static_cast<unsigned>(TX) &&
-          static_cast<unsigned>(ly) < static_cast<unsigned>(TY) &&
-          static_cast<unsigned>(lz) < static_cast<unsigned>(TZ);
-      if (!go) break;
-      atomicAdd(s_cnt + (lx * TY + ly) * TZ + lz, 1u);
-      const int ax = ray_advance(r);
-      hash_coord_step(hc, ax, r, a);
+    int n = static_cast<int>((sg.meta >> kMetaN) & 0x7Fu);
+    const int sx = static_cast<int>((sg.meta >> kMetaSign) & 3u) - 1;
+    const int sy = static_cast<int>((sg.meta >> (kMetaSign + 2)) & 3u) - 1;
+    const int sz = static_cast<int>((sg.meta >> (kMetaSign + 4)) & 3u) - 1;
+    const int ldx = sx * (TY * TZ), ldy = sy * TZ, ldz = sz;
+    float tMx = sg.tMx, tMy = sg.tMy, tMz = sg.tMz;
+    for (; n > 0; --n) {
+      atomicAdd(s_cnt + li, 1u);
+      const bool x_lt_y = tMx < tMy;
+      const bool step_x = x_lt_y && (tMx < tMz);
+      const bool step_y = !x_lt_y && (tMy < tMz);
+      const float nx = tMx + sg.tDx;
+      const float ny = tMy + sg.tDy;
+      const float nz = tMz + sg.tDz;
+      tMx = step_x ? nx : tMx;
+      tMy = step_y ? ny : tMy;
+      tMz = (!step_x && !step_y) ? nz : tMz;
+      li += step_x ? ldx : (step_y ? ldy : ldz);
     }
   }
 }
@@ -528,7 +820,7 @@ __global__ void __launch_bounds__(kApplyThreads)
   if (itm.slab >= 0) {
     lds_zero(s_cnt4);
     __syncthreads();
-    tile_accumulate(g, w, itm.beg, end, x0, y0, z0, s_cnt);
+    tile_accumulate(w, itm.beg, end, s_cnt);
     __syncthreads();
     uint4 *slab = reinterpret_cast<uint4 *>(
         w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
@@ -546,7 +838,7 @@ __global__ void __launch_bounds__(kApplyThreads)
   for (int b = itm.beg; b < end; b += kPass) {
     lds_zero(s_cnt4);
     __syncthreads();
-    tile_accumulate(g, w, b, min(b + kPass, end), x0, y0, z0, s_cnt);
+    tile_accumulate(w, b, min(b + kPass, end), s_cnt);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
@@ -666,7 +958,7 @@ __global__ void __launch_bounds__(256)
 struct Layout {
   int ntx, nty, ntz, n_tiles, cube, max_seg, max_items;
   long long seg_capacity;
-  size_t off_count, off_segs, off_sorted, off_tcount, off_toff, off_tcur,
+  size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_hot, off_scratch, off_misc, off_nop, off_nhit, total;
 };
 
@@ -699,14 +991,14 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
   size_t o = 0;
   L.off_count = o;
   o = align256(o + n * sizeof(int));
+  L.off_stile = o;
+  o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(int));
   L.off_segs = o;
   o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(Segment));
   L.off_sorted = o;
   o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(Segment));
   L.off_tcount = o;
   o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
-  L.off_toff = o;
-  o = align256(o + static_cast<size_t>(L.n_tiles + 1) * 4);
   L.off_tcur = o;
   o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
   L.off_items = o;
@@ -730,10 +1022,10 @@ Workspace bind(const Layout &L, void *base) {
   char *b = static_cast<char *>(base);
   Workspace w;
   w.seg_count = reinterpret_cast<int *>(b + L.off_count);
+  w.seg_tile = reinterpret_cast<int *>(b + L.off_stile);
   w.segs = reinterpret_cast<Segment *>(b + L.off_segs);
   w.sorted = reinterpret_cast<Segment *>(b + L.off_sorted);
   w.tile_count = reinterpret_cast<int *>(b + L.off_tcount);
-  w.tile_offset = reinterpret_cast<int *>(b + L.off_toff);
   w.tile_cursor = reinterpret_cast<int *>(b + L.off_tcur);
   w.items = reinterpret_cast<Item *>(b + L.off_items);
   w.hot = reinterpret_cast<HotTile *>(b + L.off_hot);
@@ -778,6 +1070,11 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
   if (n > max_points) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
   }
+  if (cfg->map_size_i.x != 2 * cfg->half_map_size_i.x + 1 ||
+      cfg->map_size_i.y != 2 * cfg->half_map_size_i.y + 1 ||
+      cfg->map_size_i.z != 2 * cfg->half_map_size_i.z + 1) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate: map size must be 2 * half + 1");
+  }
   const Layout L = make_layout(*cfg, max_points, near_steps);
   if (!d_ws || ws_bytes < L.total) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
@@ -801,15 +1098,20 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
   g.near_k = near_steps;
   g.cube = L.cube;
   g.max_seg = L.max_seg;
+  auto pmod = [](int v, int m) { const int r = v % m; return r < 0 ? r + m : r; };
+  g.qx = pmod(g.a.oix, g.a.mx);
+  g.qy = pmod(g.a.oiy, g.a.my);
+  g.qz = pmod(g.a.oiz, g.a.mz);
   g.p = ApplyArgs{l_hit, l_miss, l_min, l_max, l_unknown};
   const Workspace w = bind(L, d_ws);
   hipStream_t hs = as_stream(s);
   const dim3 wb(kWalkBlock);
   const dim3 wg((n + kWalkBlock - 1) / kWalkBlock);
+  const size_t near_lds = static_cast<size_t>(L.cube) * L.cube * L.cube * 4;
   if (T) {
-    walk_kernel<true><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
+    walk_kernel<true><<<wg, wb, near_lds, hs>>>(g, d_xs, d_ys, d_zs, w);
   } else {
-    walk_kernel<false><<<wg, wb, 0, hs>>>(g, d_xs, d_ys, d_zs, w);
+    walk_kernel<false><<<wg, wb, near_lds, hs>>>(g, d_xs, d_ys, d_zs, w);
   }
   int st = check_launch("walk_kernel");
   if (st != CUBLOX_OK) return st;

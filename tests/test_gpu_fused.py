@@ -91,3 +91,32 @@ def test_fused_centre_binning_and_long_range(cb, orc, gpu_ready):
         g.recenter(s.base)
         o.recenter(s.base)
         assert_occ_bitexact(g.occupancy(), o.occupancy(), f"centre scan {k}")
+
+
+def test_fused_sensor_outside_window(cb, orc, gpu_ready):
+    """No recentering while the robot drives away: after the first scans the
+    sensor voxel lies outside the window and rays enter it from outside."""
+    from cublox_amd.scene import Scene
+    scene = Scene(seed=77, n_boxes=40, box_extent=10.0, speed=1.0)
+    g = cb.OccupancyGrid((15, 10, 6), 0.1, False, 0.5)
+    o = orc.OracleGrid((15, 10, 6), 0.1, False, 0.5)
+    for x in (g, o):
+        x.set_max_raycast_range(8.0)
+        x.set_log_odds_params(*cb.log_odds())
+    outside = 0
+    for k in range(8):
+        s = scene.scan(k, 16, 1024, -15.0, 15.0, 8.0)
+        bx, by, bz = _upload(cb, s)
+        g.cast_rays_fused_sensor_frame_device(bx.ptr, by.ptr, bz.ptr, s.n, s.T)
+        o.raycast(s.world(), s.sensor_origin)
+        g_op, g_hit = g.counters()
+        o_op, o_hit = o.counters()
+        assert_counters_equal(g_op, g_hit, o_op, o_hit, f"outside scan {k}")
+        g.apply_counters()
+        o.apply()
+        assert_occ_bitexact(g.occupancy(), o.occupancy(), f"outside scan {k}")
+        oi = np.asarray(o.origin()[0])
+        sv = np.floor(np.asarray(s.sensor_origin, np.float64) / 0.1).astype(int)
+        if np.any(np.abs(sv - oi) > [15, 10, 6]):
+            outside += 1
+    assert outside >= 3
