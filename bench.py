@@ -199,6 +199,12 @@ def main():
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    # The drop-in C++ API prints to stdout (as the reference's does); keep
+    # stdout for the one JSON line by pointing fd 1 at stderr meanwhile.
+    sys.stdout.flush()
+    json_fd = os.dup(1)
+    os.dup2(2, 1)
+
     import cublox_amd as cb
 
     if cb.device_count() < 1:
@@ -306,7 +312,9 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extras)
-    print(json.dumps(out), flush=True)
+    sys.stdout.flush()
+    os.write(json_fd, (json.dumps(out) + "\n").encode())
+    os.close(json_fd)
 
 
 if __name__ == "__main__":

@@ -4,6 +4,7 @@
 // of the last failure on the calling thread is in cublox_og_last_error().
 #include <cublox/OccupancyGrid.hpp>
 #include <cublox/capi.h>
+#include <cublox/og_capi.h>
 #include <cublox/utils.hpp>
 
 #include <cstring>
@@ -37,8 +38,6 @@ std::optional<OG::VoxelState> state_of(int filter) {
 } // namespace
 
 extern "C" {
-
-typedef struct cublox_og cublox_og;
 
 const char *cublox_og_last_error(void) { return g_err.c_str(); }
 

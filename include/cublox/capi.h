@@ -261,8 +261,9 @@ int cublox_launch_fetch_gather(const unsigned int *d_counts,
  * the hash-ordered grid; each tile's counts accumulate in LDS and are folded
  * into occ once, so no per-voxel counter arrays are needed. The first
  * near_steps steps of every ray (voxels within that L1 distance of the
- * sensor voxel) accumulate in a small cube with wavefront-deduplicated
- * atomics. The caller owns the workspace: size it with
+ * sensor voxel, near_steps in [1, 12]) accumulate in a per-workgroup LDS
+ * cube around the sensor voxel, flushed once per workgroup. The map size
+ * must be 2 * half + 1 on every axis. The caller owns the workspace: size it with
  * cublox_integrate_workspace_bytes for at least max_points points, zero it
  * once with cublox_integrate_workspace_init, and reuse it for launches with
  * the same cfg geometry, max_range, max_points and near_steps (the kernels

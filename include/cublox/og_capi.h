@@ -1,0 +1,152 @@
+/*
+ * Flat C binding of the drop-in cublox::OccupancyGrid (libcublox.so), for
+ * FFI callers that cannot bind C++ (Python ctypes, cgo, JNI, N-API).
+ *
+ * Each entry point wraps one method of the C++ API in
+ * include/cublox/OccupancyGrid.hpp, itself the reference class
+ * (reference include/cublox/OccupancyGrid.hpp:29-98), and behaves as the
+ * method does, blocking included. Every call catches C++ exceptions and
+ * returns CUBLOX_OK or CUBLOX_ERR_RUNTIME; cublox_og_last_error() holds the
+ * message of the calling thread's last failure, in the reference's
+ * "cublox::cudaCheck: CUDA error in `expr`: msg (file:line)" shape for
+ * device errors (reference src/OccupancyGrid.cpp:32-49).
+ *
+ * Host arrays are plain pointers; d_-prefixed ones are device pointers
+ * (from cublox_malloc in capi.h or any HIP allocation on the same device).
+ * A handle is not thread-safe, as the reference object is not.
+ */
+#ifndef CUBLOX_OG_CAPI_H
+#define CUBLOX_OG_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cublox_og cublox_og;
+
+const char *cublox_og_last_error(void);
+
+/* OccupancyGrid(half_map_size_i, resolution, origin_at_center,
+ * recenter_threshold) (reference OccupancyGrid.hpp:42-45); has_threshold = 0
+ * is std::nullopt. `origin` (may be NULL) sets an initial origin; NULL keeps
+ * the reference's first-update origin pinning. Returns NULL on failure. */
+cublox_og *cublox_og_create(int hx, int hy, int hz, float resolution,
+                            int origin_at_center, int has_threshold,
+                            double threshold, const float origin[3]);
+void cublox_og_destroy(cublox_og *h); /* ~OccupancyGrid (hpp:47) */
+
+/* setMaxRaycastRange (hpp:68-70), setLogOddsParams (hpp:74-85). */
+int cublox_og_set_max_range(cublox_og *h, float r);
+int cublox_og_set_log_odds(cublox_og *h, float l_hit, float l_miss,
+                           float l_min, float l_max, float l_free,
+                           float l_occupied);
+
+/* update(input_cloud, sensor_origin) (hpp:53-54) with the cloud as three
+ * host columns of n floats. */
+int cublox_og_update(cublox_og *h, const float *x, const float *y,
+                     const float *z, int n, const float origin[3]);
+/* The driver's scan transform (main.cpp:640-693) fused on device: sensor-
+ * frame host columns and T = sensor->world, row-major 3x4 [R | t]. */
+int cublox_og_update_sensor(cublox_og *h, const float *xs, const float *ys,
+                            const float *zs, int n, const float T[12]);
+/* The same two with device-resident columns (no host copy). */
+int cublox_og_update_sensor_device(cublox_og *h, const float *d_xs,
+                                   const float *d_ys, const float *d_zs,
+                                   int n, const float T[12]);
+int cublox_og_update_device(cublox_og *h, const float *d_x, const float *d_y,
+                            const float *d_z, int n, const float origin[3]);
+
+/* The halves of update (reference launchRayCastUpdate, launchApplyUpdate,
+ * kernels.hpp:54-76) for stage-wise parity: cast_rays leaves per-voxel
+ * counters populated, apply_counters folds and zeroes them. The fused cast
+ * exports the counts the fused path would fold. */
+int cublox_og_cast_rays_device(cublox_og *h, const float *d_x,
+                               const float *d_y, const float *d_z, int n,
+                               const float origin[3]);
+int cublox_og_cast_rays_sensor_device(cublox_og *h, const float *d_xs,
+                                      const float *d_ys, const float *d_zs,
+                                      int n, const float T[12]);
+int cublox_og_cast_rays_fused_sensor_device(cublox_og *h, const float *d_xs,
+                                            const float *d_ys,
+                                            const float *d_zs, int n,
+                                            const float T[12]);
+int cublox_og_apply_counters(cublox_og *h);
+/* out: voxels with op != 0, sum of op, sum of hit, voxels with hit != 0. */
+int cublox_og_census(const cublox_og *h, unsigned long long out[4]);
+/* 0: fused tile-binned update (default); 1: per-voxel atomic counters. */
+int cublox_og_set_update_path(cublox_og *h, int path);
+/* Near-field steps of the fused update, 1..12. */
+int cublox_og_set_near_steps(cublox_og *h, int k);
+
+/* recenter(pos) (hpp:57); *moved = 1 when the origin changed. */
+int cublox_og_recenter(cublox_og *h, const float pos[3], int *moved);
+
+/* fetchOccupancyAround(center, radius, state_filter, max_results)
+ * (hpp:62-65); filter < 0 is std::nullopt, else VoxelState (hpp:31-35).
+ * Writes up to cap samples (xyz float triples and states). */
+int cublox_og_fetch(const cublox_og *h, const float center[3], float radius,
+                    int filter, int max_results, float *out_pos,
+                    unsigned char *out_state, int cap, int *n_out);
+/* Many queries in one launch (no reference counterpart): counts[q] samples
+ * of query q, densely packed in query order in out_pos / out_state; each
+ * query keeps at most per_query_cap samples. */
+int cublox_og_fetch_batch(const cublox_og *h, const float *centers,
+                          int n_queries, float radius, int filter,
+                          int per_query_cap, unsigned int *counts,
+                          float *out_pos, unsigned char *out_state);
+
+/* Voxel data access (occupancy log-odds and counters, hash order). */
+int cublox_og_occ_to_host(const cublox_og *h, float *dst);
+int cublox_og_counters_to_host(const cublox_og *h, int *op, int *hit);
+int cublox_og_load_occ(cublox_og *h, const float *src);
+const float *cublox_og_device_occ(const cublox_og *h);
+void *cublox_og_stream(const cublox_og *h);
+
+/* reset() / resetVoxel(hash_id) (hpp:49-50). */
+int cublox_og_reset(cublox_og *h);
+int cublox_og_reset_voxel(cublox_og *h, int hash_id);
+
+/* Grid::Config and window state (reference Grid.hpp:22-60). */
+int cublox_og_geometry(const cublox_og *h, int map[3], int half[3],
+                       int origin_i[3], float origin_f[3], int bmin[3],
+                       int bmax[3], int *voxel_num, float *res,
+                       float *res_inv);
+/* isOccupied / isFree / isUnknown (hpp:90-98): kind 0/1/2 = occupied /
+ * free / unknown; by 0/1/2 = position qf[3] / global index qi[3] / hash
+ * id qi[0]. */
+int cublox_og_query(const cublox_og *h, int kind, int by, const float *qf,
+                    const int *qi, int *result);
+/* Grid::maxHorizontalInWindowRadius (Grid.hpp:42). */
+int cublox_og_max_horizontal_radius(const cublox_og *h, const float c[3],
+                                    float *r);
+
+/* Per-kernel device timing (no reference counterpart): out_ms = ray cast
+ * (or fused update), apply, recenter device-time totals; out_n = their
+ * launch counts, then the voxels zeroed by recenters. */
+int cublox_og_enable_timing(cublox_og *h, int on);
+int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
+                           long long out_n[4]);
+int cublox_og_reset_kernel_times(cublox_og *h);
+
+/* Index math (reference include/cublox/utils.hpp:24-181). */
+int cublox_utils_pos_to_global(float p, float inv_res, int center);
+float cublox_utils_global_to_pos(int g, float res, int center);
+int cublox_utils_global_to_local(int g, int map, int half);
+int cublox_utils_local_to_global(int l, int map, int half, int origin_g);
+int cublox_utils_local_to_hash(const int l[3], const int map[3],
+                               const int half[3]);
+void cublox_utils_hash_to_local(int h, const int map[3], const int half[3],
+                                int l[3]);
+void cublox_utils_hash_to_pos(int h, const int map[3], const int half[3],
+                              const int origin[3], float res, int center,
+                              float pos[3]);
+int cublox_utils_pos_to_hash(const float pos[3], const int map[3],
+                             const int half[3], float inv_res, int center);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBLOX_OG_CAPI_H */
