@@ -6,6 +6,7 @@ Python layer over both. Importing the package loads the built libraries and
 raises if they are missing: there is no CPU fallback.
 """
 from . import _ffi
+from .batch import GridBatch
 from .device import DeviceBuffer, Event, Stream, device_count, device_name, synchronize
 from .grid import FREE, OCCUPIED, UNKNOWN, OccupancyGrid, logit
 from .scene import Scan, Scene
@@ -18,7 +19,7 @@ _ffi.host()
 _ffi.gen()
 
 __all__ = [
-    "OccupancyGrid", "Scene", "Scan", "DeviceBuffer", "Stream", "Event",
+    "OccupancyGrid", "GridBatch", "Scene", "Scan", "DeviceBuffer", "Stream", "Event",
     "device_count", "device_name", "synchronize", "logit", "log_odds",
     "WORKLOADS", "UNKNOWN", "FREE", "OCCUPIED",
 ]

@@ -158,6 +158,20 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_launch_integrate", i, C.POINTER(RaycastCfg), f32p, vp, vp,
          vp, i, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, vp,
          S, i, i, vp, vp, vp)
+    _sig(L, "cublox_integrate_segment_bound", i, C.POINTER(RaycastCfg))
+    _sig(L, "cublox_integrate_tile_count", i, C.POINTER(RaycastCfg))
+    _sig(L, "cublox_integrate_batch_workspace_bytes", S, vp)
+    _sig(L, "cublox_integrate_batch_staging_bytes", S, vp)
+    _sig(L, "cublox_integrate_batch_stage", i, vp, i, vp, i, vp)
+    _sig(L, "cublox_launch_integrate_batch_staged", i, vp, vp, S, vp, vp)
+    _sig(L, "cublox_launch_integrate_batch", i, vp, i, vp, vp, vp, S, vp)
+    _sig(L, "cublox_recenter_batch_staging_bytes", S, i)
+    _sig(L, "cublox_recenter_batch_stage", i, vp, i, i, vp, vp)
+    _sig(L, "cublox_launch_recenter_batch_staged", i, vp, vp, i, vp)
+    _sig(L, "cublox_graph_begin_capture", i, vp)
+    _sig(L, "cublox_graph_end_capture", i, vp, C.POINTER(vp))
+    _sig(L, "cublox_graph_launch", i, vp, vp)
+    _sig(L, "cublox_graph_destroy", i, vp)
     _hip = L
     return L
 
@@ -205,6 +219,19 @@ def host() -> C.CDLL:
     _sig(L, "cublox_og_reset_kernel_times", i, vp)
     _sig(L, "cublox_og_device_occ", vp, vp)
     _sig(L, "cublox_og_stream", vp, vp)
+    pp = C.POINTER(vp)
+    _sig(L, "cublox_gb_create", vp, i32p)
+    _sig(L, "cublox_gb_destroy", None, vp)
+    _sig(L, "cublox_gb_segment_bound", i, vp)
+    _sig(L, "cublox_gb_tile_count", i, vp)
+    _sig(L, "cublox_gb_tick", i, vp, i, pp, pp, pp, pp, i32p, f32p, i, pp, f32p)
+    _sig(L, "cublox_gb_set_graph_mode", i, vp, i)
+    _sig(L, "cublox_gb_set_tuning", i, vp, i, i32p, i)
+    _sig(L, "cublox_gb_tuning", i, vp, i32p)
+    _sig(L, "cublox_gb_setting", i, vp, i32p)
+    _sig(L, "cublox_gb_set_setting", i, vp, i, i)
+    _sig(L, "cublox_gb_tune_results", i, vp, C.POINTER(C.c_double), i)
+    _sig(L, "cublox_gb_last", i, vp, C.POINTER(C.c_double))
     _sig(L, "cublox_utils_pos_to_global", i, F, F, i)
     _sig(L, "cublox_utils_global_to_pos", F, i, F, i)
     _sig(L, "cublox_utils_global_to_local", i, i, i, i)

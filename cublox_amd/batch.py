@@ -1,0 +1,124 @@
+"""Batched updates of many grids (the C++ cublox::GridBatch).
+
+One tick runs the fused update of every listed scan on its grid and then
+the listed recenters, as one launch sequence each for all grids, with an
+optional HIP-graph replay and an online autotuner. Each grid ends a tick
+exactly as its own update + recenter would leave it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _ffi
+
+
+class GridBatch:
+    def __init__(self, *, max_grids: int, max_points: int, max_tiles: int,
+                 max_segments_per_ray: int, max_near_steps: int = 12,
+                 min_chunk: int = 256, max_slabs: int = 16384,
+                 max_regions: int = 0):
+        L = _ffi.host()
+        lim = (C.c_int * 8)(int(max_grids), int(max_points), int(max_tiles),
+                            int(max_segments_per_ray), int(max_near_steps),
+                            int(min_chunk), int(max_slabs), int(max_regions))
+        h = L.cublox_gb_create(lim)
+        if not h:
+            raise RuntimeError(L.cublox_og_last_error().decode())
+        self._h = h
+        self._L = L
+        self.max_grids = int(max_grids)
+
+    @staticmethod
+    def segment_bound(grid) -> int:
+        return _ffi.host().cublox_gb_segment_bound(grid._h)
+
+    @staticmethod
+    def tile_count(grid) -> int:
+        return _ffi.host().cublox_gb_tile_count(grid._h)
+
+    @classmethod
+    def for_grids(cls, grids: Sequence, max_points: int, **kw) -> "GridBatch":
+        """Limits that fit one tick over `grids` with max_points in total."""
+        kw.setdefault("max_tiles", sum(cls.tile_count(g) for g in grids))
+        kw.setdefault("max_segments_per_ray",
+                      max(cls.segment_bound(g) for g in grids))
+        return cls(max_grids=len(grids), max_points=max_points, **kw)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.cublox_gb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> None:
+        if st != 0:
+            raise RuntimeError(self._L.cublox_og_last_error().decode())
+
+    def tick(self, scans: Sequence[Tuple], recenters: Iterable[Tuple] = ()) -> None:
+        """scans: (grid, d_x, d_y, d_z, n, T 3x4 sensor->world) with device
+        buffers (objects with .ptr or ints); recenters: (grid, pos)."""
+        scans = list(scans)
+        recenters = list(recenters)
+        k = len(scans)
+        P = C.c_void_p * max(k, 1)
+        ptr = lambda b: b if isinstance(b, int) else b.ptr
+        grids = P(*[s[0]._h for s in scans])
+        xs = P(*[ptr(s[1]) for s in scans])
+        ys = P(*[ptr(s[2]) for s in scans])
+        zs = P(*[ptr(s[3]) for s in scans])
+        n = np.array([int(s[4]) for s in scans] or [0], np.int32)
+        T = np.ascontiguousarray(
+            np.array([np.asarray(s[5], np.float32).reshape(12) for s in scans]
+                     or [np.zeros(12, np.float32)], np.float32))
+        R = C.c_void_p * max(len(recenters), 1)
+        rg = R(*[r[0]._h for r in recenters])
+        rp = np.ascontiguousarray(
+            np.array([np.asarray(r[1], np.float32).reshape(3) for r in recenters]
+                     or [np.zeros(3, np.float32)], np.float32))
+        self._check(self._L.cublox_gb_tick(
+            self._h, k, grids, xs, ys, zs, _ffi.iptr(n), _ffi.fptr(T),
+            len(recenters), rg, _ffi.fptr(rp)))
+
+    def set_graph_mode(self, on: bool) -> None:
+        self._check(self._L.cublox_gb_set_graph_mode(self._h, 1 if on else 0))
+
+    def set_tuning(self, candidates: Sequence[Tuple[int, int]], ticks_each: int) -> None:
+        arr = np.array([v for c in candidates for v in c] or [0], np.int32)
+        self._check(self._L.cublox_gb_set_tuning(self._h, len(candidates),
+                                                 _ffi.iptr(arr), int(ticks_each)))
+
+    @property
+    def tuning(self) -> bool:
+        v = C.c_int(0)
+        self._check(self._L.cublox_gb_tuning(self._h, C.byref(v)))
+        return bool(v.value)
+
+    @property
+    def setting(self) -> Tuple[int, int]:
+        out = np.zeros(2, np.int32)
+        self._check(self._L.cublox_gb_setting(self._h, _ffi.iptr(out)))
+        return int(out[0]), int(out[1])
+
+    def set_setting(self, near_steps: int, chunk: int) -> None:
+        self._check(self._L.cublox_gb_set_setting(self._h, int(near_steps), int(chunk)))
+
+    def tune_results(self, n: int) -> List[float]:
+        out = (C.c_double * max(n, 1))()
+        self._check(self._L.cublox_gb_tune_results(self._h, out, n))
+        return [out[i] for i in range(n)]
+
+    @property
+    def last(self) -> Tuple[float, float, int]:
+        """(update ms, recenter ms, zeroed voxels) of the last tick; in graph
+        mode the whole tick counts as update."""
+        out = (C.c_double * 3)()
+        self._check(self._L.cublox_gb_last(self._h, out))
+        return out[0], out[1], int(out[2])

@@ -1,0 +1,315 @@
+// Batched updates of many occupancy grids: host orchestration over the
+// batched C ABI (cublox_integrate_batch_*, cublox_recenter_batch_*).
+#include <cublox/GridBatch.hpp>
+#include <cublox/capi.h>
+#include <cublox/utils.hpp>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace cublox {
+
+namespace {
+
+void statusCheck(int status, const char *expr, const char *file, int line) {
+  if (status == CUBLOX_OK) return;
+  std::string msg = "cublox::cudaCheck: CUDA error in `";
+  msg += expr;
+  msg += "`: ";
+  msg += cublox_last_error();
+  msg += " (";
+  msg += file;
+  msg += ":";
+  msg += std::to_string(line);
+  msg += ")";
+  throw std::runtime_error(msg);
+}
+
+#define CUBLOX_CHECK(call) ::cublox::statusCheck((call), #call, __FILE__, __LINE__)
+
+cublox_int3 i3(const Eigen::Vector3i &v) { return cublox_int3{v.x(), v.y(), v.z()}; }
+cublox_float3 f3(const Eigen::Vector3f &v) {
+  return cublox_float3{v.x(), v.y(), v.z()};
+}
+
+cublox_raycast_cfg raycastCfg(const OccupancyGrid &g, float max_range) {
+  cublox_raycast_cfg c{};
+  c.origin_i = i3(g.origin_i_);
+  c.map_size_i = i3(g.config_.map_size_i);
+  c.half_map_size_i = i3(g.config_.half_map_size_i);
+  c.resolution = g.config_.resolution;
+  c.inv_resolution = g.config_.resolution_inv;
+  c.max_range = max_range;
+  c.map_vox_num = g.config_.voxel_num;
+  c.origin_at_center = g.config_.origin_at_center ? 1 : 0;
+  return c;
+}
+
+} // namespace
+
+static cublox_batch_limits abiLimits(const BatchLimits &l) {
+  cublox_batch_limits c{};
+  c.max_grids = l.max_grids;
+  c.max_points = l.max_points;
+  c.max_tiles = l.max_tiles;
+  c.max_segments_per_ray = l.max_segments_per_ray;
+  c.max_near_steps = l.max_near_steps;
+  c.chunk = l.min_chunk;
+  c.max_slabs = l.max_slabs;
+  return c;
+}
+
+GridBatch::GridBatch(const BatchLimits &limits) : lim_(limits) {
+  if (lim_.max_grids < 1 || lim_.max_points < 1 || lim_.max_tiles < 1 ||
+      lim_.max_segments_per_ray < 1 || lim_.max_near_steps < 1 ||
+      lim_.max_near_steps > 12) {
+    throw std::invalid_argument("cublox::GridBatch: invalid limits");
+  }
+  setting_.near_steps = std::min(setting_.near_steps, lim_.max_near_steps);
+  setting_.chunk = std::max(setting_.chunk, lim_.min_chunk);
+  max_regions_ = lim_.max_regions > 0 ? lim_.max_regions : 6 * lim_.max_grids;
+  const cublox_batch_limits c = abiLimits(lim_);
+  try {
+    CUBLOX_CHECK(cublox_stream_create(&stream_));
+    for (auto &e : ev_) CUBLOX_CHECK(cublox_event_create(&e));
+    ws_bytes_ = cublox_integrate_batch_workspace_bytes(&c);
+    CUBLOX_CHECK(cublox_malloc(&d_ws_, ws_bytes_));
+    CUBLOX_CHECK(cublox_integrate_workspace_init(d_ws_, ws_bytes_, stream_));
+    CUBLOX_CHECK(cublox_host_alloc(&h_stage_, cublox_integrate_batch_staging_bytes(&c)));
+    const size_t rb = cublox_recenter_batch_staging_bytes(max_regions_);
+    CUBLOX_CHECK(cublox_host_alloc(&h_rstage_, rb));
+    CUBLOX_CHECK(cublox_malloc(&d_rarea_, rb));
+    CUBLOX_CHECK(cublox_stream_synchronize(stream_));
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+GridBatch::~GridBatch() { release(); }
+
+void GridBatch::release() noexcept {
+  if (stream_) cublox_stream_synchronize(stream_);
+  if (graph_) cublox_graph_destroy(graph_);
+  graph_ = nullptr;
+  if (d_ws_) cublox_free(d_ws_);
+  if (d_rarea_) cublox_free(d_rarea_);
+  if (h_stage_) cublox_host_free(h_stage_);
+  if (h_rstage_) cublox_host_free(h_rstage_);
+  d_ws_ = d_rarea_ = h_stage_ = h_rstage_ = nullptr;
+  for (auto &e : ev_) {
+    if (e) cublox_event_destroy(e);
+    e = nullptr;
+  }
+  if (stream_) cublox_stream_destroy(stream_);
+  stream_ = nullptr;
+}
+
+int GridBatch::segmentBound(const OccupancyGrid &g) {
+  const cublox_raycast_cfg c = raycastCfg(g, g.getMaxRaycastRange());
+  return cublox_integrate_segment_bound(&c);
+}
+
+int GridBatch::tileCount(const OccupancyGrid &g) {
+  const cublox_raycast_cfg c = raycastCfg(g, g.getMaxRaycastRange());
+  return cublox_integrate_tile_count(&c);
+}
+
+void GridBatch::setGraphMode(bool on) {
+  if (!on && graph_) {
+    CUBLOX_CHECK(cublox_stream_synchronize(stream_));
+    CUBLOX_CHECK(cublox_graph_destroy(graph_));
+    graph_ = nullptr;
+  }
+  graph_mode_ = on;
+}
+
+void GridBatch::setSetting(const TuneSetting &s) {
+  if (s.near_steps < 1 || s.near_steps > lim_.max_near_steps ||
+      s.chunk < lim_.min_chunk || s.chunk > 65535) {
+    throw std::invalid_argument("cublox::GridBatch: setting outside the limits");
+  }
+  setting_ = s;
+}
+
+void GridBatch::setTuning(const std::vector<TuneSetting> &candidates,
+                          int ticks_each) {
+  for (const auto &s : candidates) {
+    if (s.near_steps < 1 || s.near_steps > lim_.max_near_steps ||
+        s.chunk < lim_.min_chunk || s.chunk > 65535) {
+      throw std::invalid_argument("cublox::GridBatch: candidate outside the limits");
+    }
+  }
+  tune_ = candidates;
+  tune_ms_.assign(candidates.size(), 0.0);
+  tune_n_.assign(candidates.size(), 0);
+  tune_pos_ = 0;
+  tune_ticks_ = std::max(ticks_each, 1);
+  tune_count_ = 0;
+}
+
+std::vector<std::pair<TuneSetting, double>> GridBatch::tuneResults() const {
+  std::vector<std::pair<TuneSetting, double>> out;
+  for (size_t k = 0; k < tune_.size(); ++k) {
+    out.emplace_back(tune_[k], tune_n_[k] ? tune_ms_[k] / tune_n_[k] : 0.0);
+  }
+  return out;
+}
+
+// Host staging of one tick: the update descriptors (pinning each grid's
+// first origin, as update() does) and the recenter plans.
+void GridBatch::stage(
+    const std::vector<GridScan> &scans,
+    const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
+  std::vector<cublox_integrate_grid> items(scans.size());
+  for (size_t k = 0; k < scans.size(); ++k) {
+    const GridScan &s = scans[k];
+    if (!s.grid) throw std::invalid_argument("cublox::GridBatch: null grid");
+    OccupancyGrid &g = *s.grid;
+    const Eigen::Vector3f sensor =
+        s.sensor_frame ? Eigen::Vector3f(s.T[3], s.T[7], s.T[11]) : s.origin;
+    g.pinFirstOrigin(sensor);
+    cublox_integrate_grid &q = items[k];
+    q.cfg = raycastCfg(g, g.max_raycast_range_);
+    q.cfg.origin = f3(sensor);
+    for (int i = 0; i < 12; ++i) q.T[i] = s.T[static_cast<size_t>(i)];
+    q.use_T = s.sensor_frame ? 1 : 0;
+    q.d_xs = s.d_x;
+    q.d_ys = s.d_y;
+    q.d_zs = s.d_z;
+    q.n = s.n;
+    q.d_occ = g.d_occ_;
+    q.l_hit = g.l_hit_;
+    q.l_miss = g.l_miss_;
+    q.l_min = g.l_min_;
+    q.l_max = g.l_max_;
+    q.l_unknown = g.l_unknown_;
+    q.near_steps = setting_.near_steps;
+    q.d_export_op = nullptr;
+    q.d_export_hit = nullptr;
+  }
+  const cublox_batch_limits c = abiLimits(lim_);
+  CUBLOX_CHECK(cublox_integrate_batch_stage(items.data(), static_cast<int>(items.size()),
+                                            &c, setting_.chunk, h_stage_));
+
+  // OccupancyGrid.cpp:141-179 per grid: threshold pre-check on the host,
+  // then the plan; origins move once the tick has run.
+  std::vector<cublox_recenter_item> ritems;
+  std::vector<OccupancyGrid *> rgrids;
+  for (const auto &[gp, pos] : recenters) {
+    if (!gp) throw std::invalid_argument("cublox::GridBatch: null grid");
+    OccupancyGrid &g = *gp;
+    if (!g.config_.recenter_threshold.has_value() || g.d_occ_ == nullptr) continue;
+    if ((pos - g.origin_f_).squaredNorm() < g.config_.recenter_threshold_squared) continue;
+    cublox_recenter_item it{};
+    it.cfg.pos = f3(pos);
+    it.cfg.origin_f = f3(g.origin_f_);
+    it.cfg.origin_i = i3(g.origin_i_);
+    it.cfg.map_size_i = i3(g.config_.map_size_i);
+    it.cfg.half_map_size_i = i3(g.config_.half_map_size_i);
+    it.cfg.resolution = g.config_.resolution;
+    it.cfg.inv_resolution = g.config_.resolution_inv;
+    it.cfg.recenter_threshold_sq = g.config_.recenter_threshold_squared;
+    it.cfg.origin_at_center = g.config_.origin_at_center ? 1 : 0;
+    it.cfg.map_vox_num = g.config_.voxel_num;
+    it.d_occ = g.d_occ_;
+    ritems.push_back(it);
+    rgrids.push_back(gp);
+  }
+  std::vector<cublox_recenter_result> res(ritems.size());
+  CUBLOX_CHECK(cublox_recenter_batch_stage(ritems.data(), static_cast<int>(ritems.size()),
+                                           max_regions_, h_rstage_, res.data()));
+  rc_grids_.clear();
+  rc_origins_.clear();
+  rc_zeroed_.clear();
+  for (size_t k = 0; k < res.size(); ++k) {
+    if (res[k].action == 0) continue;
+    rc_grids_.push_back(rgrids[k]);
+    rc_origins_.emplace_back(res[k].new_origin_i.x, res[k].new_origin_i.y,
+                             res[k].new_origin_i.z);
+    rc_zeroed_.push_back(res[k].zeroed_voxels);
+  }
+}
+
+void GridBatch::launch() {
+  const cublox_batch_limits c = abiLimits(lim_);
+  auto enqueue = [&]() {
+    CUBLOX_CHECK(cublox_launch_integrate_batch_staged(h_stage_, d_ws_, ws_bytes_, &c,
+                                                      stream_));
+    if (!graph_mode_) CUBLOX_CHECK(cublox_event_record(ev_[1], stream_));
+    CUBLOX_CHECK(cublox_launch_recenter_batch_staged(h_rstage_, d_rarea_, max_regions_,
+                                                     stream_));
+  };
+  CUBLOX_CHECK(cublox_event_record(ev_[0], stream_));
+  if (graph_mode_) {
+    if (!graph_) {
+      CUBLOX_CHECK(cublox_graph_begin_capture(stream_));
+      try {
+        enqueue();
+      } catch (...) {
+        cublox_graph g = nullptr;
+        cublox_graph_end_capture(stream_, &g);  // leave capture mode
+        if (g) cublox_graph_destroy(g);
+        throw;
+      }
+      CUBLOX_CHECK(cublox_graph_end_capture(stream_, &graph_));
+    }
+    CUBLOX_CHECK(cublox_graph_launch(graph_, stream_));
+  } else {
+    enqueue();
+  }
+  CUBLOX_CHECK(cublox_event_record(ev_[2], stream_));
+}
+
+void GridBatch::finish() {
+  CUBLOX_CHECK(cublox_stream_synchronize(stream_));
+  float total = 0.0f, upd = 0.0f;
+  CUBLOX_CHECK(cublox_event_elapsed_ms(ev_[0], ev_[2], &total));
+  if (!graph_mode_) {
+    CUBLOX_CHECK(cublox_event_elapsed_ms(ev_[0], ev_[1], &upd));
+  } else {
+    upd = total;
+  }
+  last_update_ms_ = upd;
+  last_recenter_ms_ = graph_mode_ ? 0.0 : static_cast<double>(total - upd);
+  last_zeroed_ = 0;
+  for (size_t k = 0; k < rc_grids_.size(); ++k) {
+    rc_grids_[k]->updateOriginAndBound(rc_origins_[k]);
+    last_zeroed_ += rc_zeroed_[k];
+  }
+  if (tuning()) {
+    // One warm-up tick per candidate, then ticks_each measured ones.
+    if (tune_count_ > 0) {
+      tune_ms_[tune_pos_] += total;
+      tune_n_[tune_pos_] += 1;
+    }
+    if (++tune_count_ > tune_ticks_) {
+      tune_count_ = 0;
+      if (++tune_pos_ == tune_.size()) {
+        size_t best = 0;
+        for (size_t k = 1; k < tune_.size(); ++k) {
+          if (tune_ms_[k] / std::max(tune_n_[k], 1) <
+              tune_ms_[best] / std::max(tune_n_[best], 1)) {
+            best = k;
+          }
+        }
+        setting_ = tune_[best];
+      }
+    }
+  }
+}
+
+void GridBatch::tick(
+    const std::vector<GridScan> &scans,
+    const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
+  if (static_cast<int>(scans.size()) > lim_.max_grids) {
+    throw std::invalid_argument("cublox::GridBatch: more scans than max_grids");
+  }
+  if (tuning()) setting_ = tune_[tune_pos_];
+  stage(scans, recenters);
+  launch();
+  finish();
+}
+
+}  // namespace cublox

@@ -16,17 +16,24 @@
 //               parallel array; the ray's constants are stored once.
 //   bin_count   per workgroup, an LDS hash table aggregates its segments by
 //               tile; one global atomic per (workgroup, tile).
-//   tile_scan   one workgroup: tile offsets, and the work items — one per
-//               tile, or one per 1024-segment chunk of a hot tile — ordered
-//               by decreasing cost (log2 classes).
+//   tile_scan   tile cursors and work items — one per tile, or one per
+//               1024-record chunk of a hot tile — in decreasing cost order
+//               (log2 classes); one workgroup for up to 12288 tiles, three
+//               multi-workgroup passes beyond.
 //   bin_place   the same aggregation; one returning atomic per (workgroup,
 //               tile) reserves a range; segments are copied to tile order.
-//   tile_apply  one workgroup per work item: resume every segment inside
-//               the tile with packed 16+16-bit LDS counters (32 KB), then
-//               fold the tile's counts into occ with all loads in flight.
-//               Chunks of a hot tile store their counts to scratch slabs.
+//   tile_apply  one workgroup per work item: replay every segment's voxels
+//               into packed 16+16-bit LDS counters (32 KB), then fold the
+//               tile's counts into occ with all loads in flight. Chunks of
+//               a hot tile store their counts to scratch slabs.
 //   hot_fold    sums each split tile's slabs and folds, 8 workgroups a tile.
 //   near_apply  fold the near-field cube into occ and clear it.
+//
+// One launch sequence serves one grid or a batch of grids (each with its
+// own geometry, sensor pose, points and occupancy buffer): per-grid
+// descriptors live in device memory and every kernel maps its rays, tiles
+// or cells to their grid, so a batch of any mix of grids drains through
+// the same seven-to-nine launches.
 //
 // The walk's t_max recurrence is a running FP32 sum, so a segment resumed
 // from its carried state is bit-identical to the uninterrupted walk. Every
@@ -46,10 +53,7 @@ constexpr int kMaxNearK = 12;
 constexpr int kBinRays = 128;
 constexpr int kHash = 4096;
 constexpr int kHashProbes = 64;
-constexpr int kChunk = 1024;      // records per chunk of a hot tile
-constexpr int kMaxSlabs = 2048;   // scratch slabs (chunk counts), 32 KB each
-constexpr int kMaxSplit = kMaxSlabs / 2;  // a split tile has >= 2 chunks
-constexpr int kPass = 65535;      // records per LDS pass: 16-bit counts
+constexpr int kChunk = 1024;      // records per chunk of a hot tile (default)
 constexpr int kApplyThreads = 256;
 constexpr int kPerThread = kTileVox / kApplyThreads;  // 32 voxels
 constexpr int kHotSlices = 8;     // hot_fold workgroups per split tile
@@ -130,13 +134,17 @@ struct Workspace {
   int *tile_count;    // n_tiles (zero between launches)
   int *tile_cursor;   // n_tiles
   Item *items;        // max_items, costliest first
-  HotTile *hot;       // kMaxSplit
-  unsigned *scratch;  // kMaxSlabs * kTileVox: chunk counts, op | hit << 16
+  HotTile *hot;       // split tiles
+  unsigned *scratch;  // one kTileVox slab per hot-tile chunk, op | hit << 16
   int *misc;          // [0] items, [1] segments, [2] overflow flag, [3] split tiles
-  int *near_op;       // cube^3 (zero between launches)
-  int *near_hit;      // cube^3 (zero between launches)
+  int *near_op;       // near-field cells (zero between launches)
+  int *near_hit;      // near-field cells (zero between launches)
+  int *block_sums;    // multi-block tile scan: per-block sums
+  int *block_bases;   // multi-block tile scan: per-block bases
   long long seg_capacity;
   int max_items;
+  int slabs_cap;      // scratch slabs
+  int chunk;          // records per hot-tile chunk (single-grid launches)
 };
 
 __device__ __forceinline__ int tile_of(const IntegrateArgs &g, int cx, int cy,
@@ -379,32 +387,118 @@ __device__ __forceinline__ void far_by_span(const IntegrateArgs &g,
   }
 }
 
-template <bool kTransform>
+// ----------------------------------------------------------------------
+// Grids of one launch
+// ----------------------------------------------------------------------
+// A launch updates one grid or a batch of grids. Rays, segment slots,
+// tiles and near-field cells of all grids live in one index space each;
+// every grid owns a contiguous range of them (its *_base fields). A
+// workgroup's rays, a work item's tile and a near cell belong to one grid,
+// found by binary search over the bases.
+struct GridDesc {
+  IntegrateArgs g;
+  const float *xs, *ys, *zs;
+  float *occ;
+  int *export_op, *export_hit;
+  int transform;       // points are sensor-frame, world = T * p
+  int ray_base;        // first ray; a multiple of kWalkBlock
+  long long slot_base; // first segment slot; rays own g.max_seg slots each
+  int tile_base;       // first tile
+  int near_base;       // first near-field cell
+};
+
+struct BatchHeader {  // device-resident in batch mode
+  int n_grids;
+  int n_rays;   // end of the ray range (padded)
+  int n_tiles;  // tiles of all grids
+  int n_near;   // near-field cells of all grids
+  int chunk;    // records per hot-tile work item
+  int pad[3];
+};
+
+struct Batch {
+  GridDesc one;              // the grid of a single-grid launch
+  const GridDesc *descs;     // batch: n_grids descriptors (device)
+  const BatchHeader *hdr;    // batch: counts (device)
+};
+
+__device__ __forceinline__ int batch_grids(const Batch &b) {
+  return b.descs ? b.hdr->n_grids : 1;
+}
+__device__ __forceinline__ int batch_rays(const Batch &b) {
+  return b.descs ? b.hdr->n_rays : b.one.ray_base + b.one.g.n;
+}
+__device__ __forceinline__ int batch_tiles(const Batch &b) {
+  return b.descs ? b.hdr->n_tiles : b.one.g.n_tiles;
+}
+__device__ __forceinline__ int batch_near(const Batch &b) {
+  return b.descs ? b.hdr->n_near : b.one.g.cube * b.one.g.cube * b.one.g.cube;
+}
+__device__ __forceinline__ int batch_chunk(const Batch &b, const Workspace &w) {
+  return b.descs ? b.hdr->chunk : w.chunk;
+}
+
+// Last grid whose base (selected by Field) is <= key. Keys are uniform
+// across the calling workgroup.
+template <typename Field>
+__device__ __forceinline__ const GridDesc &grid_at(const Batch &b, long long key,
+                                                   Field field) {
+  if (!b.descs) return b.one;
+  int lo = 0, hi = b.hdr->n_grids - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (field(b.descs[mid]) <= key) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return b.descs[lo];
+}
+
+__device__ __forceinline__ const GridDesc &grid_of_ray(const Batch &b, int ray) {
+  return grid_at(b, ray, [](const GridDesc &d) { return (long long)d.ray_base; });
+}
+__device__ __forceinline__ const GridDesc &grid_of_tile(const Batch &b, int tile) {
+  return grid_at(b, tile, [](const GridDesc &d) { return (long long)d.tile_base; });
+}
+__device__ __forceinline__ const GridDesc &grid_of_near(const Batch &b, int cell) {
+  return grid_at(b, cell, [](const GridDesc &d) { return (long long)d.near_base; });
+}
+
+// ----------------------------------------------------------------------
+// walk kernel
+// ----------------------------------------------------------------------
 __global__ void __launch_bounds__(kWalkBlock)
-    walk_kernel(const IntegrateArgs g, const float *__restrict__ xs,
-                const float *__restrict__ ys, const float *__restrict__ zs,
-                Workspace w) {
-  extern __shared__ int s_near[];  // cube^3 cells, sized at launch
+    walk_kernel(const Batch b, Workspace w) {
+  extern __shared__ int s_near[];  // the grid's near cube, sized at launch
+  const int ray0 = blockIdx.x * kWalkBlock;
+  if (ray0 >= batch_rays(b)) return;
+  const GridDesc &d = grid_of_ray(b, ray0);
+  const IntegrateArgs &g = d.g;
   const int cells = g.cube * g.cube * g.cube;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
   __syncthreads();
 
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ray = ray0 + threadIdx.x;
+  const int i = ray - d.ray_base;  // index within the grid's points
   const RayArgs &a = g.a;
   RayState r;
-  const bool valid = i < g.n && ray_setup<kTransform>(a, xs, ys, zs, i, r);
+  const bool valid =
+      i < g.n && (d.transform ? ray_setup<true>(a, d.xs, d.ys, d.zs, i, r)
+                              : ray_setup<false>(a, d.xs, d.ys, d.zs, i, r));
   if (valid) {
     const unsigned signs =
         static_cast<unsigned>((r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4))
         << kMetaSign;
-    const size_t slot0 = static_cast<size_t>(i) * g.max_seg;
+    const long long slot0 = d.slot_base + static_cast<long long>(i) * g.max_seg;
     Segment *__restrict__ mine = w.segs + slot0;
     int *__restrict__ mine_tile = w.seg_tile + slot0;
     int nseg = 0;
     auto put = [&](const Segment &sg, int tile) {
       if (nseg < g.max_seg) {
         mine[nseg] = sg;
-        mine_tile[nseg] = tile;
+        mine_tile[nseg] = d.tile_base + tile;
       } else {
         w.misc[2] = 1;
       }
@@ -478,7 +572,7 @@ __global__ void __launch_bounds__(kWalkBlock)
       if (abs(ddx) + abs(ddy) + abs(ddz) < K) {
         const int c = cube_index(g, ddx, ddy, ddz);
         atomicAdd(s_near + c, 1);
-        if (!r.clipped) atomic_add_i32(w.near_hit + c, 1);
+        if (!r.clipped) atomic_add_i32(w.near_hit + d.near_base + c, 1);
       } else {
         const HashCoord e = window_hash_coord(g, r.eix, r.eiy, r.eiz);
         Segment sg;
@@ -490,14 +584,14 @@ __global__ void __launch_bounds__(kWalkBlock)
         put(sg, tile_of(g, e.cx, e.cy, e.cz));
       }
     }
-    w.seg_count[i] = min(nseg, g.max_seg);
-  } else if (i < g.n) {
-    w.seg_count[i] = 0;
+    w.seg_count[ray] = min(nseg, g.max_seg);
+  } else {
+    w.seg_count[ray] = 0;
   }
   __syncthreads();
   for (int c = threadIdx.x; c < cells; c += blockDim.x) {
     const int v = s_near[c];
-    if (v) atomic_add_i32(w.near_op + c, v);
+    if (v) atomic_add_i32(w.near_op + d.near_base + c, v);
   }
 }
 
@@ -525,22 +619,25 @@ __device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
 
 template <bool kPlace>
 __global__ void __launch_bounds__(256)
-    bin_kernel(const IntegrateArgs g, Workspace w) {
+    bin_kernel(const Batch b, Workspace w) {
   __shared__ int s_key[kHash];
   __shared__ int s_cnt[kHash];
   __shared__ int s_base[kHash];
+  const int r0 = blockIdx.x * kBinRays;
+  if (r0 >= batch_rays(b)) return;
+  const GridDesc &d = grid_of_ray(b, r0);
   for (int e = threadIdx.x; e < kHash; e += blockDim.x) {
     s_key[e] = -1;
     s_cnt[e] = 0;
   }
   __syncthreads();
-  const int r0 = blockIdx.x * kBinRays;
   const int lane_ray = threadIdx.x % kBinRays;
   const int j0 = threadIdx.x / kBinRays;
   constexpr int kJStride = 256 / kBinRays;
   const int ray = r0 + lane_ray;
-  const int cnt = ray < g.n ? w.seg_count[ray] : 0;
-  const size_t base = static_cast<size_t>(ray) * g.max_seg;
+  const int cnt = w.seg_count[ray];
+  const long long base =
+      d.slot_base + static_cast<long long>(ray - d.ray_base) * d.g.max_seg;
 
   for (int j = j0; j < cnt; j += kJStride) {
     const int t = w.seg_tile[base + j];
@@ -568,18 +665,27 @@ __global__ void __launch_bounds__(256)
     const int t = w.seg_tile[base + j];
     const Segment sg = w.segs[base + j];
     const int e = lds_hash_slot(s_key, t, false);
-    const int slot = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
-                            : atomicAdd(w.tile_cursor + t, 1);
+    const long long slot = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
+                                  : atomicAdd(w.tile_cursor + t, 1);
     if (slot < w.seg_capacity) w.sorted[slot] = sg;
   }
 }
 
 // ----------------------------------------------------------------------
-// tile_scan: offsets, work items, hot-tile splits
+// tile_scan: cursors, work items, hot-tile splits
 // ----------------------------------------------------------------------
+// A tile with more than `chunk` records is split into chunks of about
+// `chunk` records, each its own work item with its own scratch slab, while
+// the split tiles' slabs (taken in tile order) fit the scratch area; a hot
+// tile beyond that is one item, accumulated in 16-bit passes. Items are
+// placed in decreasing cost classes (log2 of records per item) so the
+// heaviest start first.
 constexpr int kScanThreads = 1024;
 constexpr int kScanWaves = kScanThreads / 64;
 constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
+constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
+constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
+constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
 
 __device__ __forceinline__ int wave_scan_incl(int v) {
   const int lane = threadIdx.x & 63;
@@ -595,12 +701,13 @@ __device__ __forceinline__ int wave_scan_incl(int v) {
 // then across the wave totals.
 __device__ __forceinline__ int block_scan_incl(int *s_wave, int v) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
   v = wave_scan_incl(v);
   if (lane == 63) s_wave[wv] = v;
   __syncthreads();
   if (wv == 0) {
-    const int x = wave_scan_incl(lane < kScanWaves ? s_wave[lane] : 0);
-    if (lane < kScanWaves) s_wave[lane] = x;
+    const int x = wave_scan_incl(lane < nw ? s_wave[lane] : 0);
+    if (lane < nw) s_wave[lane] = x;
   }
   __syncthreads();
   const int r = v + (wv > 0 ? s_wave[wv - 1] : 0);
@@ -615,51 +722,71 @@ __device__ __forceinline__ int cost_bucket(int records) {
 }
 
 // Number of chunks a tile with c records is split into (1: not split).
-__device__ __forceinline__ int chunks_of(int c) {
-  return c > kChunk ? (c + kChunk - 1) / kChunk : 1;
+__device__ __forceinline__ int chunks_of(int c, int chunk) {
+  return c > chunk ? (c + chunk - 1) / chunk : 1;
 }
 
+// Items of tile k (c records from `run`; its slab range would start at sl
+// and its split index is sp): ch chunk items when split, else one item.
+// `slots` item positions are taken from the class cursor `bucket`; those
+// an unsplit hot tile leaves unused become empty items.
+__device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
+                                           long long run, int sl, int sp,
+                                           int chunk, bool split, int bucket,
+                                           int slots, int *s_cursor) {
+  w.tile_cursor[k] = static_cast<int>(run);
+  if (c <= 0) return;
+  const int ch = split ? chunks_of(c, chunk) : 1;
+  const int per_chunk = (c + ch - 1) / ch;
+  if (split) w.hot[sp] = HotTile{k, sl, ch, 0};
+  for (int q = 0; q < slots; ++q) {
+    const int pos = atomicAdd(s_cursor + bucket, 1);
+    if (pos >= w.max_items) continue;
+    if (q < ch) {
+      const int ib = static_cast<int>(run) + q * per_chunk;
+      const int ie = min(ib + per_chunk, static_cast<int>(run) + c);
+      w.items[pos] = Item{k, ib, ie, split ? sl + q : -1};
+    } else {
+      w.items[pos] = Item{-1, 0, 0, -1};
+    }
+  }
+}
+
+// Single-workgroup scan for up to kScanStage tiles.
 __global__ void __launch_bounds__(kScanThreads)
-    tile_scan_kernel(const int n_tiles, Workspace w) {
+    tile_scan_kernel(const Batch b, Workspace w) {
   __shared__ int s_buf[kScanWaves];
   __shared__ int s_bucket[kCostBuckets];
   __shared__ int s_cursor[kCostBuckets];
   __shared__ int s_count[kScanStage];
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
   const int t = threadIdx.x;
   if (t < kCostBuckets) s_bucket[t] = 0;
-  // Counts staged in LDS with coalesced loads (and cleared for the next
-  // launch) when they fit; otherwise every pass reads them from memory.
-  const bool staged = n_tiles <= kScanStage;
-  if (staged) {
-    for (int k = t; k < n_tiles; k += kScanThreads) {
-      s_count[k] = w.tile_count[k];
-      w.tile_count[k] = 0;
-    }
-    __syncthreads();
+  for (int k = t; k < n_tiles; k += kScanThreads) {
+    s_count[k] = w.tile_count[k];
+    w.tile_count[k] = 0;
   }
-  auto count = [&](int k) { return staged ? s_count[k] : w.tile_count[k]; };
+  __syncthreads();
   const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
   const int beg = min(t * per, n_tiles);
   const int end = min(beg + per, n_tiles);
-  // Pass 1: records and the scratch slabs that splitting would take.
   int seg = 0, slabs = 0;
   for (int k = beg; k < end; ++k) {
-    const int c = count(k);
+    const int c = s_count[k];
     seg += c;
-    if (c > kChunk) slabs += chunks_of(c);
+    if (c > chunk) slabs += chunks_of(c, chunk);
   }
   const int seg_incl = block_scan_incl(s_buf, seg);
   const int slab_incl = block_scan_incl(s_buf, slabs);
-  // A hot tile is split when its whole slab range fits the scratch area (a
-  // prefix of the hot tiles in tile order); otherwise one workgroup takes
-  // it in 16-bit passes. Pass 2: work items per cost class.
-  int sl = slab_incl - slabs;
-  int hot = 0;
+  // Split decisions (slab prefix over all hot tiles in tile order) and the
+  // items per cost class they give.
+  int sl = slab_incl - slabs, hot = 0;
   for (int k = beg; k < end; ++k) {
-    const int c = count(k);
+    const int c = s_count[k];
     if (c <= 0) continue;
-    const int ch = chunks_of(c);
-    if (ch > 1 && sl + ch <= kMaxSlabs) {
+    const int ch = chunks_of(c, chunk);
+    if (ch > 1 && sl + ch <= w.slabs_cap) {
       atomicAdd(s_bucket + cost_bucket((c + ch - 1) / ch), ch);
       ++hot;
     } else {
@@ -670,40 +797,26 @@ __global__ void __launch_bounds__(kScanThreads)
   const int hot_incl = block_scan_incl(s_buf, hot);
   if (t == 0) {
     int run = 0;
-    for (int b = 0; b < kCostBuckets; ++b) {
-      s_cursor[b] = run;
-      run += s_bucket[b];
+    for (int q = 0; q < kCostBuckets; ++q) {
+      s_cursor[q] = run;
+      run += s_bucket[q];
     }
     w.misc[0] = min(run, w.max_items);
   }
   __syncthreads();
-  // Pass 3: cursors, items, split tiles; clear the counts.
-  int run = seg_incl - seg;
+  long long run = seg_incl - seg;
   int sp = hot_incl - hot;
   sl = slab_incl - slabs;
   for (int k = beg; k < end; ++k) {
-    const int c = count(k);
-    w.tile_cursor[k] = run;
-    if (c > 0) {
-      const int ch = chunks_of(c);
-      const bool split = ch > 1 && sl + ch <= kMaxSlabs;
-      const int chunks = split ? ch : 1;
-      const int per_chunk = (c + chunks - 1) / chunks;
-      if (split) w.hot[sp++] = HotTile{k, sl, ch, 0};
-      // Every chunk is classed by per_chunk, as counted in pass 2.
-      const int bucket = cost_bucket(per_chunk);
-      for (int q = 0; q < chunks; ++q) {
-        const int ib = run + q * per_chunk;
-        const int ie = min(ib + per_chunk, run + c);
-        const int pos = atomicAdd(s_cursor + bucket, 1);
-        if (pos < w.max_items) {
-          w.items[pos] = Item{k, ib, ie, split ? sl + q : -1};
-        }
-      }
-      if (ch > 1) sl += ch;
-    }
+    const int c = s_count[k];
+    const int ch = chunks_of(c, chunk);
+    const bool split = ch > 1 && sl + ch <= w.slabs_cap;
+    const int bucket = cost_bucket(split ? (c + ch - 1) / ch : c);
+    place_tile(w, k, c, run, sl, sp, chunk, split, bucket, split ? ch : 1,
+               s_cursor);
+    if (split) ++sp;
+    if (ch > 1) sl += ch;
     run += c;
-    if (!staged) w.tile_count[k] = 0;
   }
   if (t == kScanThreads - 1) {
     w.misc[1] = seg_incl;
@@ -711,14 +824,133 @@ __global__ void __launch_bounds__(kScanThreads)
   }
 }
 
+// Multi-workgroup scan, pass 1: per block of kScanTilesPerBlock tiles, the
+// records, the slabs its hot tiles would take, and the items per cost class
+// as if every hot tile were split.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_sums_kernel(const Batch b, Workspace w) {
+  __shared__ int s_sum[kScanSums];
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
+  const int k0 = blockIdx.x * kScanTilesPerBlock;
+  if (threadIdx.x < kScanSums) s_sum[threadIdx.x] = 0;
+  __syncthreads();
+  int seg = 0, slabs = 0;
+  for (int k = k0 + threadIdx.x; k < min(k0 + kScanTilesPerBlock, n_tiles);
+       k += kScanThreads) {
+    const int c = w.tile_count[k];
+    seg += c;
+    const int ch = chunks_of(c, chunk);
+    if (c > 0) atomicAdd(s_sum + 3 + cost_bucket((c + ch - 1) / ch), ch);
+    if (ch > 1) slabs += ch;
+  }
+  atomicAdd(s_sum + 0, seg);
+  atomicAdd(s_sum + 1, slabs);
+  __syncthreads();
+  if (threadIdx.x < kScanSums) {
+    w.block_sums[blockIdx.x * kScanSums + threadIdx.x] = s_sum[threadIdx.x];
+  }
+}
+
+// Pass 2 (one workgroup): exclusive per-block bases of records and slabs,
+// and for every cost class the block's first item slot.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_bases_kernel(const Batch b, Workspace w) {
+  __shared__ int s_buf[kScanWaves];
+  __shared__ int s_total[kScanSums];
+  const int n_blocks =
+      (batch_tiles(b) + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
+  const int t = threadIdx.x;  // one thread per block (at most kScanThreads)
+  for (int q = 0; q < kScanSums; ++q) {
+    const int v = t < n_blocks ? w.block_sums[t * kScanSums + q] : 0;
+    const int incl = block_scan_incl(s_buf, v);
+    if (t < n_blocks) w.block_bases[t * kScanSums + q] = incl - v;
+    if (t == kScanThreads - 1) s_total[q] = incl;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int q = 0; q < kCostBuckets; ++q) {
+      const int n = s_total[3 + q];
+      s_total[3 + q] = run;  // first slot of class q
+      run += n;
+    }
+    w.misc[0] = min(run, w.max_items);
+    w.misc[1] = s_total[0];
+  }
+  __syncthreads();
+  for (int q = 0; q < kCostBuckets; ++q) {
+    if (t < n_blocks) w.block_bases[t * kScanSums + 3 + q] += s_total[3 + q];
+  }
+  if (t == 0) w.misc[3] = 0;  // split tiles, counted by pass 3
+}
+
+// Pass 3: per block, cursors, items and split tiles from the block bases.
+// A hot tile whose slab range does not fit keeps one item (in its class
+// slot) and turns its other slots into empty items.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_place_kernel(const Batch b, Workspace w) {
+  __shared__ int s_buf[kScanWaves];
+  __shared__ int s_cursor[kCostBuckets];
+  __shared__ int s_hot;
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
+  const int k0 = blockIdx.x * kScanTilesPerBlock;
+  const int k1 = min(k0 + kScanTilesPerBlock, n_tiles);
+  const int t = threadIdx.x;
+  const int *base = w.block_bases + blockIdx.x * kScanSums;
+  if (t < kCostBuckets) s_cursor[t] = base[3 + t];
+  constexpr int kPer = kScanTilesPerBlock / kScanThreads;
+  const int beg = min(k0 + t * kPer, k1);
+  const int end = min(beg + kPer, k1);
+  int seg = 0, slabs = 0, hot = 0;
+  {
+    // Split decisions need the slab prefix; count split tiles under them.
+    for (int k = beg; k < end; ++k) {
+      const int c = w.tile_count[k];
+      seg += c;
+      if (c > chunk) slabs += chunks_of(c, chunk);
+    }
+  }
+  const int seg_incl = block_scan_incl(s_buf, seg);
+  const int slab_incl = block_scan_incl(s_buf, slabs);
+  int sl = base[1] + slab_incl - slabs;
+  for (int k = beg; k < end; ++k) {
+    const int c = w.tile_count[k];
+    const int ch = chunks_of(c, chunk);
+    if (ch > 1) {
+      if (sl + ch <= w.slabs_cap) ++hot;
+      sl += ch;
+    }
+  }
+  const int hot_incl = block_scan_incl(s_buf, hot);
+  if (t == kScanThreads - 1) s_hot = atomicAdd(w.misc + 3, hot_incl);
+  __syncthreads();
+  long long run = static_cast<long long>(base[0]) + seg_incl - seg;
+  sl = base[1] + slab_incl - slabs;
+  int sp = s_hot + hot_incl - hot;
+  for (int k = beg; k < end; ++k) {
+    const int c = w.tile_count[k];
+    const int ch = chunks_of(c, chunk);
+    const bool split = ch > 1 && sl + ch <= w.slabs_cap;
+    place_tile(w, k, c, run, sl, sp, chunk, split,
+               cost_bucket((c + ch - 1) / ch), ch, s_cursor);
+    if (split) ++sp;
+    if (ch > 1) sl += ch;
+    run += c;
+    w.tile_count[k] = 0;
+  }
+}
+
 // ----------------------------------------------------------------------
 // tile_apply
 // ----------------------------------------------------------------------
+
 // Accumulates records [beg, end) of a tile into packed LDS counters
 // (op | hit << 16): each segment replays its n voxels from its entry state
-// (the same crossing parameters and tie-break as the walk). A pass holds at
-// most kPass records and a record visits a voxel at most once, so neither
-// field carries into the other.
+// (the same crossing parameters and tie-break as the walk). A call covers
+// at most kPass records and a record visits a voxel at most once, so
+// neither field carries into the other.
 __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
                                                 int end, unsigned *s_cnt) {
   for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
@@ -756,197 +988,200 @@ __device__ __forceinline__ void lds_zero(uint4 *s4) {
   }
 }
 
-// Thread t owns tile voxels j * 256 + t (j < 32): a z-run of 32 contiguous
-// floats per 32 lanes. Hash offset of its j-th voxel:
-__device__ __forceinline__ int owned_hash(const RayArgs &a, int x0, int y0,
-                                          int z0, int j) {
-  const int t = threadIdx.x;
-  const int lx = j >> 1;
-  const int ly = ((j & 1) << 3) + (t >> 5);
-  const int lz = t & 31;
+// Hash offset of the grid voxel at tile-local index k of the tile whose
+// hash corner is (x0, y0, z0).
+__device__ __forceinline__ int tile_voxel_hash(const RayArgs &a, int x0,
+                                               int y0, int z0, int k) {
+  const int lz = k % TZ, ly = (k / TZ) % TY, lx = k / (TZ * TY);
   return (x0 + lx) * (a.my * a.mz) + (y0 + ly) * a.mz + (z0 + lz);
 }
 
-// Fold of a thread's owned counts: every occ load is issued before the
-// first update, so a tile costs one memory round trip.
-__device__ __forceinline__ void fold_owned(const IntegrateArgs &g, int x0,
-                                           int y0, int z0, const int *op,
-                                           const int *hit,
-                                           float *__restrict__ occ,
-                                           int *__restrict__ export_op,
-                                           int *__restrict__ export_hit) {
-  const RayArgs &a = g.a;
-  if (export_op) {
+// Fold of the tile counts a thread owns (voxels j * 256 + t, j < 32: a
+// z-run of 32 contiguous floats per 32 lanes): every occ load is issued
+// before the first update, so a tile costs one memory round trip.
+__device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
+                                           int z0, const unsigned *op,
+                                           const unsigned *hit) {
+  const RayArgs &a = d.g.a;
+  const int t = threadIdx.x;
+  auto hash = [&](int j) {
+    return (x0 + (j >> 1)) * (a.my * a.mz) +
+           (y0 + ((j & 1) << 3) + (t >> 5)) * a.mz + (z0 + (t & 31));
+  };
+  if (d.export_op) {
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
       if (op[j] == 0) continue;
-      const int h = owned_hash(a, x0, y0, z0, j);
-      export_op[h] = op[j];
-      export_hit[h] = hit[j];
+      d.export_op[hash(j)] = static_cast<int>(op[j]);
+      d.export_hit[hash(j)] = static_cast<int>(hit[j]);
     }
     return;
   }
   float v[kPerThread];
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
-    v[j] = op[j] != 0 ? occ[owned_hash(a, x0, y0, z0, j)] : 0.0f;
+    v[j] = op[j] != 0 ? d.occ[hash(j)] : 0.0f;
   }
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
     if (op[j] == 0) continue;
-    occ[owned_hash(a, x0, y0, z0, j)] = apply_logodds(v[j], op[j], hit[j], g.p);
+    d.occ[hash(j)] = apply_logodds(v[j], static_cast<int>(op[j]),
+                                   static_cast<int>(hit[j]), d.g.p);
   }
 }
 
-// One workgroup per work item, items in decreasing cost order so the
-// hardware dispatcher balances them. A plain item accumulates its tile in
-// LDS passes of kPass records, sums the passes in registers and folds. A
-// chunk of a hot tile stores its packed counts to its scratch slab;
-// hot_fold sums a tile's slabs and folds.
-__global__ void __launch_bounds__(kApplyThreads)
-    tile_apply_kernel(const IntegrateArgs g, Workspace w,
-                      float *__restrict__ occ, int *__restrict__ export_op,
-                      int *__restrict__ export_hit) {
+// Work items in decreasing cost order, one per workgroup while they last
+// (then strided), so the hardware dispatcher balances them. A plain item
+// accumulates its tile in LDS (in passes of kPass records, summed in
+// registers, for an unsplit hot tile) and folds it. A chunk of a hot tile
+// stores its packed counts to its scratch slab; hot_fold sums a tile's
+// slabs. Empty items (slots an unsplit hot tile left) do nothing.
+__global__ void __launch_bounds__(kApplyThreads, 4)
+    tile_apply_kernel(const Batch b, Workspace w) {
   __shared__ uint4 s_cnt4[kTileVox / 4];
   unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
-  if (static_cast<int>(blockIdx.x) >= w.misc[0]) return;
-  const Item itm = w.items[blockIdx.x];
-  const int tz = itm.tile % g.ntz;
-  const int ty = (itm.tile / g.ntz) % g.nty;
-  const int tx = itm.tile / (g.ntz * g.nty);
-  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-  const int end = static_cast<int>(
-      min(static_cast<long long>(itm.end), w.seg_capacity));
-  if (itm.slab >= 0) {
-    lds_zero(s_cnt4);
-    __syncthreads();
-    tile_accumulate(w, itm.beg, end, s_cnt);
-    __syncthreads();
-    uint4 *slab = reinterpret_cast<uint4 *>(
-        w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
-    for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
-      slab[k] = s_cnt4[k];
+  const int n_items = w.misc[0];
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const Item itm = w.items[item];
+    if (itm.tile < 0) continue;  // uniform across the workgroup
+    const int end = static_cast<int>(
+        min(static_cast<long long>(itm.end), w.seg_capacity));
+    if (itm.slab >= 0) {
+      lds_zero(s_cnt4);
+      __syncthreads();
+      tile_accumulate(w, itm.beg, end, s_cnt);
+      __syncthreads();
+      uint4 *slab = reinterpret_cast<uint4 *>(
+          w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
+      for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
+        slab[k] = s_cnt4[k];
+      }
+      __syncthreads();
+      continue;
     }
-    return;
-  }
-  int op[kPerThread], hit[kPerThread];
-#pragma unroll
-  for (int j = 0; j < kPerThread; ++j) {
-    op[j] = 0;
-    hit[j] = 0;
-  }
-  for (int b = itm.beg; b < end; b += kPass) {
-    lds_zero(s_cnt4);
-    __syncthreads();
-    tile_accumulate(w, b, min(b + kPass, end), s_cnt);
-    __syncthreads();
+    unsigned op[kPerThread], hit[kPerThread];
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
-      const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
-      op[j] += static_cast<int>(c & 0xFFFFu);
-      hit[j] += static_cast<int>(c >> 16);
+      op[j] = 0;
+      hit[j] = 0;
     }
-    __syncthreads();
+    for (int p0 = itm.beg; p0 < end; p0 += kPass) {
+      lds_zero(s_cnt4);
+      __syncthreads();
+      tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j) {
+        const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
+        op[j] += c & 0xFFFFu;
+        hit[j] += c >> 16;
+      }
+      __syncthreads();
+    }
+    const GridDesc &d = grid_of_tile(b, itm.tile);
+    const int tile = itm.tile - d.tile_base;
+    const int tz = tile % d.g.ntz;
+    const int ty = (tile / d.g.ntz) % d.g.nty;
+    const int tx = tile / (d.g.ntz * d.g.nty);
+    fold_owned(d, tx * TX, ty * TY, tz * TZ, op, hit);
   }
-  fold_owned(g, x0, y0, z0, op, hit, occ, export_op, export_hit);
 }
 
 // Sums the scratch slabs of each split tile and folds: kHotSlices
 // workgroups per tile, four consecutive voxels (one z-run piece) per lane.
 __global__ void __launch_bounds__(256)
-    hot_fold_kernel(const IntegrateArgs g, Workspace w,
-                    float *__restrict__ occ, int *__restrict__ export_op,
-                    int *__restrict__ export_hit) {
-  const int hot = blockIdx.x / kHotSlices;
-  if (hot >= w.misc[3]) return;
-  const HotTile ht = w.hot[hot];
-  const int k4 = (blockIdx.x % kHotSlices) * 256 + threadIdx.x;  // < 2048
-  const uint4 *src = reinterpret_cast<const uint4 *>(
-                         w.scratch + static_cast<size_t>(ht.slab0) * kTileVox) +
-                     k4;
-  unsigned op[4] = {0, 0, 0, 0}, hit[4] = {0, 0, 0, 0};
-  int q = 0;
-  for (; q + 4 <= ht.chunks; q += 4) {
-    uint4 v[4];
+    hot_fold_kernel(const Batch b, Workspace w) {
+  const int n_hot = w.misc[3];
+  for (int blk = blockIdx.x; blk < n_hot * kHotSlices; blk += gridDim.x) {
+    const HotTile ht = w.hot[blk / kHotSlices];
+    const int k4 = (blk % kHotSlices) * 256 + threadIdx.x;  // < 2048
+    const uint4 *src =
+        reinterpret_cast<const uint4 *>(w.scratch +
+                                        static_cast<size_t>(ht.slab0) * kTileVox) +
+        k4;
+    unsigned op[4] = {0, 0, 0, 0}, hit[4] = {0, 0, 0, 0};
+    int q = 0;
+    for (; q + 4 <= ht.chunks; q += 4) {
+      uint4 v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = src[(q + u) * (kTileVox / 4)];
+      for (int u = 0; u < 4; ++u) v[u] = src[(q + u) * (kTileVox / 4)];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const unsigned vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+      for (int u = 0; u < 4; ++u) {
+        const unsigned vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          op[e] += vv[e] & 0xFFFFu;
+          hit[e] += vv[e] >> 16;
+        }
+      }
+    }
+    for (; q < ht.chunks; ++q) {
+      const uint4 v = src[q * (kTileVox / 4)];
+      const unsigned vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         op[e] += vv[e] & 0xFFFFu;
         hit[e] += vv[e] >> 16;
       }
     }
-  }
-  for (; q < ht.chunks; ++q) {
-    const uint4 v = src[q * (kTileVox / 4)];
-    const unsigned vv[4] = {v.x, v.y, v.z, v.w};
+    const GridDesc &d = grid_of_tile(b, ht.tile);
+    const RayArgs &a = d.g.a;
+    const int tile = ht.tile - d.tile_base;
+    const int tz = tile % d.g.ntz;
+    const int ty = (tile / d.g.ntz) % d.g.nty;
+    const int tx = tile / (d.g.ntz * d.g.nty);
+    const int h0 = tile_voxel_hash(a, tx * TX, ty * TY, tz * TZ, 4 * k4);
+    float v[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      op[e] += vv[e] & 0xFFFFu;
-      hit[e] += vv[e] >> 16;
+      v[e] = (op[e] != 0 && !d.export_op) ? d.occ[h0 + e] : 0.0f;
     }
-  }
-  const RayArgs &a = g.a;
-  const int tz = ht.tile % g.ntz;
-  const int ty = (ht.tile / g.ntz) % g.nty;
-  const int tx = ht.tile / (g.ntz * g.nty);
-  const int k = 4 * k4;
-  const int lz = k % TZ, ly = (k / TZ) % TY, lx = k / (TZ * TY);
-  const int h0 = (tx * TX + lx) * (a.my * a.mz) + (ty * TY + ly) * a.mz +
-                 (tz * TZ + lz);
-  float v[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v[e] = (op[e] != 0 && !export_op) ? occ[h0 + e] : 0.0f;
-  }
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    if (op[e] == 0) continue;
-    if (export_op) {
-      export_op[h0 + e] = static_cast<int>(op[e]);
-      export_hit[h0 + e] = static_cast<int>(hit[e]);
-    } else {
-      occ[h0 + e] = apply_logodds(v[e], static_cast<int>(op[e]),
-                                  static_cast<int>(hit[e]), g.p);
+    for (int e = 0; e < 4; ++e) {
+      if (op[e] == 0) continue;
+      if (d.export_op) {
+        d.export_op[h0 + e] = static_cast<int>(op[e]);
+        d.export_hit[h0 + e] = static_cast<int>(hit[e]);
+      } else {
+        d.occ[h0 + e] = apply_logodds(v[e], static_cast<int>(op[e]),
+                                      static_cast<int>(hit[e]), d.g.p);
+      }
     }
   }
 }
 
 // ----------------------------------------------------------------------
-// near_apply
+// near_apply: fold the near-field cubes, clear them
 // ----------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-    near_apply_kernel(const IntegrateArgs g, Workspace w,
-                      float *__restrict__ occ, int *__restrict__ export_op,
-                      int *__restrict__ export_hit) {
-  const RayArgs &a = g.a;
-  const bool ctr = a.center != 0;
-  const int s0x = pos_to_global(a.ox, a.inv_res, ctr);
-  const int s0y = pos_to_global(a.oy, a.inv_res, ctr);
-  const int s0z = pos_to_global(a.oz, a.inv_res, ctr);
-  const int C = g.cube;
-  const int cells = C * C * C;
-  const int k1 = g.near_k - 1;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells;
-       c += gridDim.x * blockDim.x) {
-    const int o = w.near_op[c];
+    near_apply_kernel(const Batch b, Workspace w) {
+  const int n_near = batch_near(b);
+  for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < n_near;
+       cell += gridDim.x * blockDim.x) {
+    const int o = w.near_op[cell];
     if (o == 0) continue;
-    const int hit = w.near_hit[c];
-    w.near_op[c] = 0;
-    w.near_hit[c] = 0;
+    const int hit = w.near_hit[cell];
+    w.near_op[cell] = 0;
+    w.near_hit[cell] = 0;
+    const GridDesc &d = grid_of_near(b, cell);
+    const IntegrateArgs &g = d.g;
+    const RayArgs &a = g.a;
+    const bool ctr = a.center != 0;
+    const int c = cell - d.near_base;
+    const int C = g.cube;
+    const int k1 = g.near_k - 1;
     const int dz = c % C - k1;
     const int dy = (c / C) % C - k1;
     const int dx = c / (C * C) - k1;
-    const int h = global_to_hash(s0x + dx, s0y + dy, s0z + dz, a.mx, a.my,
-                                 a.mz, a.hx, a.hy, a.hz);
-    if (export_op) {
-      export_op[h] = o;
-      export_hit[h] = hit;
+    const int h = global_to_hash(pos_to_global(a.ox, a.inv_res, ctr) + dx,
+                                 pos_to_global(a.oy, a.inv_res, ctr) + dy,
+                                 pos_to_global(a.oz, a.inv_res, ctr) + dz,
+                                 a.mx, a.my, a.mz, a.hx, a.hy, a.hz);
+    if (d.export_op) {
+      d.export_op[h] = o;
+      d.export_hit[h] = hit;
     } else {
-      occ[h] = apply_logodds(occ[h], o, hit, g.p);
+      d.occ[h] = apply_logodds(d.occ[h], o, hit, g.p);
     }
   }
 }
@@ -955,27 +1190,24 @@ __global__ void __launch_bounds__(256)
 // Host side: geometry, workspace layout, launch sequence.
 // ----------------------------------------------------------------------
 
-struct Layout {
-  int ntx, nty, ntz, n_tiles, cube, max_seg, max_items;
-  long long seg_capacity;
-  size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
-      off_items, off_hot, off_scratch, off_misc, off_nop, off_nhit, total;
-};
-
 inline size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+struct GridGeom {
+  int ntx, nty, ntz, n_tiles, cube, max_seg;
+};
 
 // Per-ray segment bound (tile entries + the endpoint record): inside the
 // window the walk is monotone per axis, so it changes tile at most
 // ntx + nty + ntz times plus one wrap seam per axis; a walk of S steps also
 // changes tile at most S / min(tile dim) + 3 times.
-Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
-  Layout L{};
-  L.ntx = (c.map_size_i.x + TX - 1) / TX;
-  L.nty = (c.map_size_i.y + TY - 1) / TY;
-  L.ntz = (c.map_size_i.z + TZ - 1) / TZ;
-  L.n_tiles = L.ntx * L.nty * L.ntz;
-  L.cube = 2 * near_k - 1;
-  const long long win_bound = L.ntx + L.nty + L.ntz + 8;
+GridGeom grid_geom(const cublox_raycast_cfg &c, int near_k) {
+  GridGeom G{};
+  G.ntx = (c.map_size_i.x + TX - 1) / TX;
+  G.nty = (c.map_size_i.y + TY - 1) / TY;
+  G.ntz = (c.map_size_i.z + TZ - 1) / TZ;
+  G.n_tiles = G.ntx * G.nty * G.ntz;
+  G.cube = 2 * near_k - 1;
+  const long long win_bound = G.ntx + G.nty + G.ntz + 8;
   const double reach = static_cast<double>(c.max_range) *
                            static_cast<double>(c.inv_resolution) + 2.0;
   long long range_bound = win_bound;
@@ -983,44 +1215,85 @@ Layout make_layout(const cublox_raycast_cfg &c, int max_points, int near_k) {
     const long long steps = 3 * static_cast<long long>(reach) + 8;
     range_bound = steps / std::min(std::min(TX, TY), TZ) + 10;
   }
-  L.max_seg = static_cast<int>(std::min(win_bound, range_bound));
-  const size_t n = static_cast<size_t>(std::max(max_points, 1));
-  L.seg_capacity = static_cast<long long>(n) * L.max_seg;
-  // An unsplit tile is one item; a split tile's chunks each own a slab.
-  L.max_items = L.n_tiles + kMaxSlabs;
+  G.max_seg = static_cast<int>(std::min(win_bound, range_bound));
+  return G;
+}
+
+// Batch staging: a header and the descriptors, copied to the workspace in
+// one transfer at the head of the launch sequence.
+struct Staging {
+  BatchHeader hdr;
+  int pad[60];
+  GridDesc desc[1];  // max_grids of them
+};
+
+size_t staging_bytes(int max_grids) {
+  return offsetof(Staging, desc) + static_cast<size_t>(std::max(max_grids, 1)) * sizeof(GridDesc);
+}
+
+// Capacities of a workspace.
+struct Caps {
+  int max_grids;
+  long long max_rays;      // padded ray range
+  long long seg_capacity;  // segment slots
+  int max_tiles;
+  int max_cube;            // largest near cube edge
+  int max_near;            // near cells
+  int chunk;               // smallest records per hot-tile chunk
+  int max_slabs;           // scratch slab cap
+};
+
+struct Layout {
+  Caps cap;
+  long long slabs;
+  int max_items, scan_blocks;
+  size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
+      off_items, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
+      off_bbase, off_hdr, off_desc, total;
+};
+
+constexpr int kMinChunk = 256;
+
+Layout make_layout(const Caps &c) {
+  Layout L{};
+  L.cap = c;
+  // Each hot tile has more than `chunk` records, so the chunks of all hot
+  // tiles number at most 2 * segments / chunk: the item bound. Slabs are
+  // capped; hot tiles past the cap run unsplit.
+  const long long chunks = 2 * c.seg_capacity / std::max(c.chunk, kMinChunk) + 2;
+  L.slabs = std::max<long long>(2, std::min<long long>(chunks, c.max_slabs));
+  L.max_items = static_cast<int>(std::min<long long>(c.max_tiles + chunks, 1LL << 30));
+  L.scan_blocks = (c.max_tiles + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
   size_t o = 0;
-  L.off_count = o;
-  o = align256(o + n * sizeof(int));
-  L.off_stile = o;
-  o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(int));
-  L.off_segs = o;
-  o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(Segment));
-  L.off_sorted = o;
-  o = align256(o + static_cast<size_t>(L.seg_capacity) * sizeof(Segment));
-  L.off_tcount = o;
-  o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
-  L.off_tcur = o;
-  o = align256(o + static_cast<size_t>(L.n_tiles) * 4);
-  L.off_items = o;
-  o = align256(o + static_cast<size_t>(L.max_items) * sizeof(Item));
-  L.off_hot = o;
-  o = align256(o + kMaxSplit * sizeof(HotTile));
-  L.off_scratch = o;
-  o = align256(o + static_cast<size_t>(kMaxSlabs) * kTileVox * 4);
-  L.off_misc = o;
-  o = align256(o + 64 * 4);
-  const size_t cells = static_cast<size_t>(L.cube) * L.cube * L.cube;
-  L.off_nop = o;
-  o = align256(o + cells * 4);
-  L.off_nhit = o;
-  o = align256(o + cells * 4);
+  auto take = [&](size_t &off, size_t bytes) {
+    off = o;
+    o = align256(o + bytes);
+  };
+  take(L.off_count, static_cast<size_t>(c.max_rays) * 4);
+  take(L.off_stile, static_cast<size_t>(c.seg_capacity) * 4);
+  take(L.off_segs, static_cast<size_t>(c.seg_capacity) * sizeof(Segment));
+  take(L.off_sorted, static_cast<size_t>(c.seg_capacity) * sizeof(Segment));
+  take(L.off_tcount, static_cast<size_t>(c.max_tiles) * 4);
+  take(L.off_tcur, static_cast<size_t>(c.max_tiles) * 4);
+  take(L.off_items, static_cast<size_t>(L.max_items) * sizeof(Item));
+  take(L.off_hot, static_cast<size_t>(L.slabs) * sizeof(HotTile));
+  take(L.off_scratch, static_cast<size_t>(L.slabs) * kTileVox * 4);
+  take(L.off_misc, 64 * 4);
+  take(L.off_nop, static_cast<size_t>(c.max_near) * 4);
+  take(L.off_nhit, static_cast<size_t>(c.max_near) * 4);
+  take(L.off_bsum, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
+  take(L.off_bbase, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
+  // Header and descriptors, laid out as the host staging buffer so one
+  // copy moves both.
+  take(L.off_hdr, staging_bytes(c.max_grids));
+  L.off_desc = L.off_hdr + offsetof(Staging, desc);
   L.total = o;
   return L;
 }
 
 Workspace bind(const Layout &L, void *base) {
   char *b = static_cast<char *>(base);
-  Workspace w;
+  Workspace w{};
   w.seg_count = reinterpret_cast<int *>(b + L.off_count);
   w.seg_tile = reinterpret_cast<int *>(b + L.off_stile);
   w.segs = reinterpret_cast<Segment *>(b + L.off_segs);
@@ -1033,9 +1306,142 @@ Workspace bind(const Layout &L, void *base) {
   w.misc = reinterpret_cast<int *>(b + L.off_misc);
   w.near_op = reinterpret_cast<int *>(b + L.off_nop);
   w.near_hit = reinterpret_cast<int *>(b + L.off_nhit);
-  w.seg_capacity = L.seg_capacity;
+  w.block_sums = reinterpret_cast<int *>(b + L.off_bsum);
+  w.block_bases = reinterpret_cast<int *>(b + L.off_bbase);
+  w.seg_capacity = L.cap.seg_capacity;
   w.max_items = L.max_items;
+  w.slabs_cap = static_cast<int>(L.slabs);
+  w.chunk = L.cap.chunk;
   return w;
+}
+
+// Single-grid capacities.
+Caps single_caps(const cublox_raycast_cfg &cfg, int max_points, int near_k) {
+  const GridGeom G = grid_geom(cfg, near_k);
+  Caps c{};
+  c.max_grids = 1;
+  c.max_rays = static_cast<long long>(std::max(max_points, 1) + kWalkBlock);
+  c.seg_capacity = static_cast<long long>(std::max(max_points, 1)) * G.max_seg;
+  c.max_tiles = G.n_tiles;
+  c.max_cube = G.cube;
+  c.max_near = G.cube * G.cube * G.cube;
+  c.chunk = kChunk;
+  c.max_slabs = 4096;
+  return c;
+}
+
+int check_cfg(const cublox_raycast_cfg &cfg, int near_k) {
+  if (near_k < 1 || near_k > kMaxNearK) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate: near_steps out of [1, 12]");
+  }
+  if (cfg.map_size_i.x != 2 * cfg.half_map_size_i.x + 1 ||
+      cfg.map_size_i.y != 2 * cfg.half_map_size_i.y + 1 ||
+      cfg.map_size_i.z != 2 * cfg.half_map_size_i.z + 1) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate: map size must be 2 * half + 1");
+  }
+  return CUBLOX_OK;
+}
+
+// Fills the descriptor of one grid (bases set by the caller).
+GridDesc make_desc(const cublox_raycast_cfg &cfg, const float *T,
+                   const float *xs, const float *ys, const float *zs, int n,
+                   float *occ, const ApplyArgs &p, int near_k, int *eop,
+                   int *ehit) {
+  const GridGeom G = grid_geom(cfg, near_k);
+  GridDesc d{};
+  IntegrateArgs &g = d.g;
+  g.a = make_ray_args(cfg);
+  if (T) {
+    for (int k = 0; k < 12; ++k) g.a.T[k] = T[k];
+    g.a.ox = T[3];
+    g.a.oy = T[7];
+    g.a.oz = T[11];
+  }
+  g.n = n;
+  g.ntx = G.ntx;
+  g.nty = G.nty;
+  g.ntz = G.ntz;
+  g.n_tiles = G.n_tiles;
+  g.near_k = near_k;
+  g.cube = G.cube;
+  g.max_seg = G.max_seg;
+  auto pmod = [](int v, int m) {
+    const int r = v % m;
+    return r < 0 ? r + m : r;
+  };
+  g.qx = pmod(g.a.oix, g.a.mx);
+  g.qy = pmod(g.a.oiy, g.a.my);
+  g.qz = pmod(g.a.oiz, g.a.mz);
+  g.p = p;
+  d.xs = xs;
+  d.ys = ys;
+  d.zs = zs;
+  d.occ = occ;
+  d.export_op = eop;
+  d.export_hit = ehit;
+  d.transform = T ? 1 : 0;
+  return d;
+}
+
+// The launch sequence for a workspace of layout L. Launch shapes depend on
+// the capacities only (so the sequence can be captured once and replayed);
+// the kernels read the actual counts from the batch.
+int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
+                    hipStream_t hs) {
+  const Caps &c = L.cap;
+  const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
+  const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
+  walk_kernel<<<dim3(walk_blocks), dim3(kWalkBlock), near_lds, hs>>>(b, w);
+  int st = check_launch("walk_kernel");
+  if (st != CUBLOX_OK) return st;
+  const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + kBinRays - 1) / kBinRays);
+  bin_kernel<false><<<dim3(bin_blocks), dim3(256), 0, hs>>>(b, w);
+  st = check_launch("bin_kernel<count>");
+  if (st != CUBLOX_OK) return st;
+  if (c.max_tiles <= kScanStage) {
+    tile_scan_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(b, w);
+    st = check_launch("tile_scan_kernel");
+  } else {
+    tile_sums_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w);
+    st = check_launch("tile_sums_kernel");
+    if (st != CUBLOX_OK) return st;
+    tile_bases_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(b, w);
+    st = check_launch("tile_bases_kernel");
+    if (st != CUBLOX_OK) return st;
+    tile_place_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w);
+    st = check_launch("tile_place_kernel");
+  }
+  if (st != CUBLOX_OK) return st;
+  bin_kernel<true><<<dim3(bin_blocks), dim3(256), 0, hs>>>(b, w);
+  st = check_launch("bin_kernel<place>");
+  if (st != CUBLOX_OK) return st;
+  const int apply_blocks = std::min(L.max_items, 1 << 16);
+  tile_apply_kernel<<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  st = check_launch("tile_apply_kernel");
+  if (st != CUBLOX_OK) return st;
+  const long long hot_blocks =
+      std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
+  hot_fold_kernel<<<dim3(static_cast<unsigned>(hot_blocks)), dim3(256), 0, hs>>>(b, w);
+  st = check_launch("hot_fold_kernel");
+  if (st != CUBLOX_OK) return st;
+  near_apply_kernel<<<dim3(std::max(1, std::min((c.max_near + 255) / 256, 1024))),
+                      dim3(256), 0, hs>>>(b, w);
+  return check_launch("near_apply_kernel");
+}
+
+Caps batch_caps(const cublox_batch_limits &lim) {
+  Caps c{};
+  c.max_grids = std::max(lim.max_grids, 1);
+  c.max_rays = static_cast<long long>(std::max(lim.max_points, 1)) +
+               static_cast<long long>(c.max_grids) * kWalkBlock;
+  c.seg_capacity = static_cast<long long>(std::max(lim.max_points, 1)) *
+                   std::max(lim.max_segments_per_ray, 1);
+  c.max_tiles = std::max(lim.max_tiles, 1);
+  c.max_cube = 2 * std::max(1, std::min(lim.max_near_steps, kMaxNearK)) - 1;
+  c.max_near = c.max_grids * c.max_cube * c.max_cube * c.max_cube;
+  c.chunk = std::max(kMinChunk, std::min(lim.chunk > 0 ? lim.chunk : kChunk, 8192));
+  c.max_slabs = lim.max_slabs > 0 ? lim.max_slabs : 16384;
+  return c;
 }
 
 } // namespace cublox_hip
@@ -1047,7 +1453,7 @@ extern "C" {
 size_t cublox_integrate_workspace_bytes(const cublox_raycast_cfg *cfg,
                                         int max_points, int near_steps) {
   if (!cfg || near_steps < 1 || near_steps > kMaxNearK) return 0;
-  return make_layout(*cfg, max_points, near_steps).total;
+  return make_layout(single_caps(*cfg, max_points, near_steps)).total;
 }
 
 int cublox_integrate_workspace_init(void *d_ws, size_t bytes, cublox_stream s) {
@@ -1064,80 +1470,140 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
                             int *d_export_op, int *d_export_hit,
                             cublox_stream s) {
   if (!cfg || n <= 0 || !d_xs || !d_ys || !d_zs || !d_occ) return CUBLOX_OK;
-  if (near_steps < 1 || near_steps > kMaxNearK) {
-    return set_error(CUBLOX_ERR_INVALID, "integrate: near_steps out of [1, 12]");
-  }
+  int st = check_cfg(*cfg, near_steps);
+  if (st != CUBLOX_OK) return st;
   if (n > max_points) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
   }
-  if (cfg->map_size_i.x != 2 * cfg->half_map_size_i.x + 1 ||
-      cfg->map_size_i.y != 2 * cfg->half_map_size_i.y + 1 ||
-      cfg->map_size_i.z != 2 * cfg->half_map_size_i.z + 1) {
-    return set_error(CUBLOX_ERR_INVALID, "integrate: map size must be 2 * half + 1");
-  }
-  const Layout L = make_layout(*cfg, max_points, near_steps);
+  const Layout L = make_layout(single_caps(*cfg, max_points, near_steps));
   if (!d_ws || ws_bytes < L.total) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
   }
   if ((d_export_op == nullptr) != (d_export_hit == nullptr)) {
     return set_error(CUBLOX_ERR_INVALID, "integrate: export needs both arrays");
   }
-  IntegrateArgs g{};
-  g.a = make_ray_args(*cfg);
-  if (T) {
-    for (int k = 0; k < 12; ++k) g.a.T[k] = T[k];
-    g.a.ox = T[3];
-    g.a.oy = T[7];
-    g.a.oz = T[11];
+  Batch b{};
+  b.one = make_desc(*cfg, T, d_xs, d_ys, d_zs, n, d_occ,
+                    ApplyArgs{l_hit, l_miss, l_min, l_max, l_unknown},
+                    near_steps, d_export_op, d_export_hit);
+  return launch_sequence(b, L, bind(L, d_ws), as_stream(s));
+}
+
+int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg) {
+  if (!cfg) return 0;
+  return grid_geom(*cfg, 1).max_seg;
+}
+
+int cublox_integrate_tile_count(const cublox_raycast_cfg *cfg) {
+  if (!cfg) return 0;
+  return grid_geom(*cfg, 1).n_tiles;
+}
+
+// The multi-block tile scan's base pass runs one thread per block.
+constexpr int kMaxBatchTiles = kScanThreads * kScanTilesPerBlock;
+
+size_t cublox_integrate_batch_workspace_bytes(const cublox_batch_limits *lim) {
+  if (!lim || lim->max_tiles > kMaxBatchTiles) return 0;
+  return make_layout(batch_caps(*lim)).total;
+}
+
+size_t cublox_integrate_batch_staging_bytes(const cublox_batch_limits *lim) {
+  if (!lim) return 0;
+  return staging_bytes(lim->max_grids);
+}
+
+int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
+                                 int n_grids, const cublox_batch_limits *lim,
+                                 int chunk_override, void *h_staging) {
+  if (!lim || !h_staging || n_grids < 0 || (n_grids > 0 && !grids)) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_batch: bad arguments");
   }
-  g.n = n;
-  g.ntx = L.ntx;
-  g.nty = L.nty;
-  g.ntz = L.ntz;
-  g.n_tiles = L.n_tiles;
-  g.near_k = near_steps;
-  g.cube = L.cube;
-  g.max_seg = L.max_seg;
-  auto pmod = [](int v, int m) { const int r = v % m; return r < 0 ? r + m : r; };
-  g.qx = pmod(g.a.oix, g.a.mx);
-  g.qy = pmod(g.a.oiy, g.a.my);
-  g.qz = pmod(g.a.oiz, g.a.mz);
-  g.p = ApplyArgs{l_hit, l_miss, l_min, l_max, l_unknown};
-  const Workspace w = bind(L, d_ws);
+  if (lim->max_tiles > kMaxBatchTiles) {
+    return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: max_tiles above 4194304");
+  }
+  const Caps c = batch_caps(*lim);
+  if (n_grids > c.max_grids) {
+    return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: too many grids");
+  }
+  Staging *sg = static_cast<Staging *>(h_staging);
+  long long ray = 0, slot = 0, points = 0;
+  int tile = 0, near = 0;
+  for (int k = 0; k < n_grids; ++k) {
+    const cublox_integrate_grid &q = grids[k];
+    const int near_k = q.near_steps > 0 ? q.near_steps : 8;
+    int st = check_cfg(q.cfg, near_k);
+    if (st != CUBLOX_OK) return st;
+    if (2 * near_k - 1 > c.max_cube) {
+      return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: near_steps above the limit");
+    }
+    if ((q.d_export_op == nullptr) != (q.d_export_hit == nullptr)) {
+      return set_error(CUBLOX_ERR_INVALID, "integrate_batch: export needs both arrays");
+    }
+    const bool live = q.n > 0 && q.d_xs && q.d_ys && q.d_zs && q.d_occ;
+    GridDesc d = make_desc(q.cfg, q.use_T ? q.T : nullptr, q.d_xs, q.d_ys, q.d_zs,
+                           live ? q.n : 0, q.d_occ,
+                           ApplyArgs{q.l_hit, q.l_miss, q.l_min, q.l_max, q.l_unknown},
+                           near_k, q.d_export_op, q.d_export_hit);
+    if (d.g.max_seg > lim->max_segments_per_ray) {
+      return set_error(CUBLOX_ERR_CAPACITY,
+                       "integrate_batch: a grid needs more segments per ray than the limit");
+    }
+    d.ray_base = static_cast<int>(ray);
+    d.slot_base = slot;
+    d.tile_base = tile;
+    d.near_base = near;
+    sg->desc[k] = d;
+    const long long n = d.g.n;
+    points += n;
+    ray += (n + kWalkBlock - 1) / kWalkBlock * kWalkBlock;
+    slot += n * d.g.max_seg;
+    tile += d.g.n_tiles;
+    near += d.g.cube * d.g.cube * d.g.cube;
+  }
+  if (points > lim->max_points || ray > c.max_rays || slot > c.seg_capacity ||
+      tile > c.max_tiles || near > c.max_near) {
+    return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: batch exceeds the limits");
+  }
+  const int chunk = chunk_override > 0 ? chunk_override : c.chunk;
+  if (chunk < c.chunk || chunk > 65535) {
+    return set_error(CUBLOX_ERR_INVALID,
+                     "integrate_batch: chunk below the limits' chunk or above 65535");
+  }
+  sg->hdr = BatchHeader{n_grids, static_cast<int>(ray), tile, near, chunk, {0, 0, 0}};
+  return CUBLOX_OK;
+}
+
+int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
+                                         size_t ws_bytes,
+                                         const cublox_batch_limits *lim,
+                                         cublox_stream s) {
+  if (!lim || !h_staging || lim->max_tiles > kMaxBatchTiles) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_batch: bad arguments");
+  }
+  const Layout L = make_layout(batch_caps(*lim));
+  if (!d_ws || ws_bytes < L.total) {
+    return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: workspace too small");
+  }
   hipStream_t hs = as_stream(s);
-  const dim3 wb(kWalkBlock);
-  const dim3 wg((n + kWalkBlock - 1) / kWalkBlock);
-  const size_t near_lds = static_cast<size_t>(L.cube) * L.cube * L.cube * 4;
-  if (T) {
-    walk_kernel<true><<<wg, wb, near_lds, hs>>>(g, d_xs, d_ys, d_zs, w);
-  } else {
-    walk_kernel<false><<<wg, wb, near_lds, hs>>>(g, d_xs, d_ys, d_zs, w);
-  }
-  int st = check_launch("walk_kernel");
+  char *base = static_cast<char *>(d_ws);
+  const int st = check(hipMemcpyAsync(base + L.off_hdr, h_staging,
+                                      staging_bytes(L.cap.max_grids),
+                                      hipMemcpyHostToDevice, hs),
+                       "hipMemcpyAsync(batch descriptors)");
   if (st != CUBLOX_OK) return st;
-  const dim3 bg((n + kBinRays - 1) / kBinRays);
-  bin_kernel<false><<<bg, dim3(256), 0, hs>>>(g, w);
-  st = check_launch("bin_kernel<count>");
+  Batch b{};
+  b.descs = reinterpret_cast<const GridDesc *>(base + L.off_desc);
+  b.hdr = reinterpret_cast<const BatchHeader *>(base + L.off_hdr);
+  return launch_sequence(b, L, bind(L, d_ws), hs);
+}
+
+int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,
+                                  int n_grids, const cublox_batch_limits *lim,
+                                  void *h_staging, void *d_ws, size_t ws_bytes,
+                                  cublox_stream s) {
+  int st = cublox_integrate_batch_stage(grids, n_grids, lim, 0, h_staging);
   if (st != CUBLOX_OK) return st;
-  tile_scan_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(L.n_tiles, w);
-  st = check_launch("tile_scan_kernel");
-  if (st != CUBLOX_OK) return st;
-  bin_kernel<true><<<bg, dim3(256), 0, hs>>>(g, w);
-  st = check_launch("bin_kernel<place>");
-  if (st != CUBLOX_OK) return st;
-  tile_apply_kernel<<<dim3(L.max_items), dim3(kApplyThreads), 0, hs>>>(
-      g, w, d_occ, d_export_op, d_export_hit);
-  st = check_launch("tile_apply_kernel");
-  if (st != CUBLOX_OK) return st;
-  hot_fold_kernel<<<dim3(kMaxSplit * kHotSlices), dim3(256), 0, hs>>>(
-      g, w, d_occ, d_export_op, d_export_hit);
-  st = check_launch("hot_fold_kernel");
-  if (st != CUBLOX_OK) return st;
-  const int cells = L.cube * L.cube * L.cube;
-  near_apply_kernel<<<dim3(std::max(1, std::min((cells + 255) / 256, 1024))),
-                      dim3(256), 0, hs>>>(g, w, d_occ, d_export_op,
-                                          d_export_hit);
-  return check_launch("near_apply_kernel");
+  return cublox_launch_integrate_batch_staged(h_staging, d_ws, ws_bytes, lim, s);
 }
 
 } // extern "C"

@@ -207,19 +207,35 @@ __device__ __forceinline__ void zero_contiguous(T *__restrict__ p,
   if (t < count - tail0) p[tail0 + t] = T(0);
 }
 
+// Zeroes one region with the rows of the region spread over the
+// workgroups of this region (blk of nblk): runs of at least 256 elements
+// take a workgroup per row with 16 B stores; shorter runs pack
+// 256 / run rows into a workgroup, one element per lane (one division per
+// lane, none per element).
 template <typename T>
 __device__ __forceinline__ void zero_region(T *__restrict__ p,
-                                           const ZeroRegion &r, long long t,
-                                           long long stride) {
+                                           const ZeroRegion &r, int blk,
+                                           int nblk) {
   if (r.n_outer == 1) {
-    zero_contiguous(p + r.base, r.run, t, stride);
+    zero_contiguous(p + r.base, r.run,
+                    static_cast<long long>(blk) * blockDim.x + threadIdx.x,
+                    static_cast<long long>(nblk) * blockDim.x);
     return;
   }
-  const long long total = r.n_outer * r.run;
-  for (long long e = t; e < total; e += stride) {
-    const long long o = e / r.run;
-    const long long k = e - o * r.run;
-    p[r.base + o * r.stride + k] = T(0);
+  if (r.run >= 256) {
+    for (long long o = blk; o < r.n_outer; o += nblk) {
+      zero_contiguous(p + r.base + o * r.stride, r.run, threadIdx.x, blockDim.x);
+    }
+    return;
+  }
+  const int run = static_cast<int>(r.run);
+  const int rows = static_cast<int>(blockDim.x) / run;
+  const int my_row = static_cast<int>(threadIdx.x) / run;
+  const int col = static_cast<int>(threadIdx.x) - my_row * run;
+  if (my_row >= rows) return;
+  for (long long o = static_cast<long long>(blk) * rows + my_row; o < r.n_outer;
+       o += static_cast<long long>(nblk) * rows) {
+    p[r.base + o * r.stride + col] = T(0);
   }
 }
 
@@ -227,12 +243,28 @@ __global__ void __launch_bounds__(256)
     zero_regions_kernel(float *__restrict__ occ, int *__restrict__ op_cnt,
                         int *__restrict__ hit_cnt, const ZeroArgs z) {
   const ZeroRegion r = z.r[blockIdx.y];
-  const long long t =
-      static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
-  zero_region(occ, r, t, stride);
-  if (op_cnt) zero_region(op_cnt, r, t, stride);
-  if (hit_cnt) zero_region(hit_cnt, r, t, stride);
+  zero_region(occ, r, blockIdx.x, gridDim.x);
+  if (op_cnt) zero_region(op_cnt, r, blockIdx.x, gridDim.x);
+  if (hit_cnt) zero_region(hit_cnt, r, blockIdx.x, gridDim.x);
+}
+
+// Batched recenter: regions of many grids, each with its occupancy base.
+struct ZeroRegionP {
+  float *occ;
+  ZeroRegion r;
+};
+
+struct RecenterBatchHeader {
+  int n_regions;
+  int pad[7];
+};
+
+__global__ void __launch_bounds__(256)
+    zero_regions_batch_kernel(const RecenterBatchHeader *__restrict__ hdr,
+                              const ZeroRegionP *__restrict__ regions) {
+  if (static_cast<int>(blockIdx.y) >= hdr->n_regions) return;
+  const ZeroRegionP q = regions[blockIdx.y];
+  zero_region(q.occ, q.r, blockIdx.x, gridDim.x);
 }
 
 // Reference-shaped slab clear from a DEVICE slice list: one block row per
@@ -266,12 +298,9 @@ __global__ void __launch_bounds__(256)
     return;
   }
   const ZeroRegion r{base, n_outer, stride, run};
-  const long long t =
-      static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const long long st = static_cast<long long>(gridDim.x) * blockDim.x;
-  zero_region(occ, r, t, st);
-  if (op_cnt) zero_region(op_cnt, r, t, st);
-  if (hit_cnt) zero_region(hit_cnt, r, t, st);
+  zero_region(occ, r, blockIdx.x, gridDim.x);
+  if (op_cnt) zero_region(op_cnt, r, blockIdx.x, gridDim.x);
+  if (hit_cnt) zero_region(hit_cnt, r, blockIdx.x, gridDim.x);
 }
 
 // ===================================================================
@@ -500,6 +529,19 @@ HostPlan plan_recenter(const cublox_recenter_cfg &c) {
   return p;
 }
 
+// Workgroups that cover a region with zero_region's mapping.
+long long region_blocks(const ZeroRegion &r) {
+  long long b;
+  if (r.n_outer == 1) {
+    b = (r.run / 4 + 255) / 256;
+  } else if (r.run >= 256) {
+    b = r.n_outer;
+  } else {
+    b = (r.n_outer + 256 / r.run - 1) / (256 / r.run);
+  }
+  return std::max<long long>(1, std::min<long long>(b, 2048));
+}
+
 // Sorted unique slice values, merged into inclusive [lo, hi] runs.
 std::vector<std::pair<int, int>> merge_runs(std::vector<int> v) {
   std::sort(v.begin(), v.end());
@@ -515,20 +557,43 @@ std::vector<std::pair<int, int>> merge_runs(std::vector<int> v) {
   return runs;
 }
 
+// The zero regions of a slab-clearing recenter plan: merged runs of the
+// exit and entering slices per axis (x: contiguous; y: one run per x row;
+// z: one run per column).
+std::vector<ZeroRegion> recenter_regions(const cublox_recenter_cfg &cfg) {
+  const HostPlan p = plan_recenter(cfg);
+  const long long mx = cfg.map_size_i.x, my = cfg.map_size_i.y,
+                  mz = cfg.map_size_i.z;
+  const int hx = cfg.half_map_size_i.x, hy = cfg.half_map_size_i.y,
+            hz = cfg.half_map_size_i.z;
+  std::vector<ZeroRegion> regions;
+  for (const auto &run : merge_runs(p.slices[0])) {
+    regions.push_back(ZeroRegion{(run.first + hx) * my * mz, 1, 0,
+                                 (run.second - run.first + 1) * my * mz});
+  }
+  for (const auto &run : merge_runs(p.slices[1])) {
+    regions.push_back(ZeroRegion{(run.first + hy) * mz, mx, my * mz,
+                                 (run.second - run.first + 1) * mz});
+  }
+  for (const auto &run : merge_runs(p.slices[2])) {
+    regions.push_back(ZeroRegion{run.first + hz, mx * my, mz,
+                                 static_cast<long long>(run.second - run.first + 1)});
+  }
+  return regions;
+}
+
+constexpr unsigned kBatchRegionBlocks = 64;
+
 int launch_regions(float *occ, int *op, int *hit,
                    const std::vector<ZeroRegion> &regions, hipStream_t s) {
   for (size_t i0 = 0; i0 < regions.size(); i0 += kMaxRegions) {
     ZeroArgs z{};
     z.n = static_cast<int>(std::min<size_t>(kMaxRegions, regions.size() - i0));
-    long long max_work = 0;
+    long long bx = 1;
     for (int k = 0; k < z.n; ++k) {
       z.r[k] = regions[i0 + k];
-      const long long w = z.r[k].n_outer == 1 ? (z.r[k].run + 3) / 4
-                                              : z.r[k].n_outer * z.r[k].run;
-      max_work = std::max(max_work, w);
+      bx = std::max(bx, region_blocks(z.r[k]));
     }
-    long long bx = (max_work + 255) / 256;
-    bx = std::max<long long>(1, std::min<long long>(bx, 2048));
     zero_regions_kernel<<<dim3(static_cast<unsigned>(bx),
                                static_cast<unsigned>(z.n)),
                           dim3(256), 0, s>>>(occ, op, hit, z);
@@ -655,25 +720,65 @@ int cublox_launch_recenter(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
     if (r == CUBLOX_OK && d_hit_cnt) r = cublox_fill_zero_async(d_hit_cnt, V, s);
     return r;
   }
-  const HostPlan p = plan_recenter(*cfg);
-  const long long mx = cfg->map_size_i.x, my = cfg->map_size_i.y,
-                  mz = cfg->map_size_i.z;
-  const int hx = cfg->half_map_size_i.x, hy = cfg->half_map_size_i.y,
-            hz = cfg->half_map_size_i.z;
-  std::vector<ZeroRegion> regions;
-  for (const auto &run : merge_runs(p.slices[0])) {
-    regions.push_back(ZeroRegion{(run.first + hx) * my * mz, 1, 0,
-                                 (run.second - run.first + 1) * my * mz});
+  return launch_regions(d_occ, d_op_cnt, d_hit_cnt, recenter_regions(*cfg), hs);
+}
+
+size_t cublox_recenter_batch_staging_bytes(int max_regions) {
+  return sizeof(RecenterBatchHeader) +
+         static_cast<size_t>(std::max(max_regions, 1)) * sizeof(ZeroRegionP);
+}
+
+int cublox_recenter_batch_stage(const cublox_recenter_item *items, int n,
+                                int max_regions, void *h_staging,
+                                cublox_recenter_result *results) {
+  if (n < 0 || (n > 0 && (!items || !results)) || !h_staging) {
+    return set_error(CUBLOX_ERR_INVALID, "recenter_batch: bad arguments");
   }
-  for (const auto &run : merge_runs(p.slices[1])) {
-    regions.push_back(ZeroRegion{(run.first + hy) * mz, mx, my * mz,
-                                 (run.second - run.first + 1) * mz});
+  auto *hdr = static_cast<RecenterBatchHeader *>(h_staging);
+  auto *regs = reinterpret_cast<ZeroRegionP *>(hdr + 1);
+  int count = 0;
+  for (int k = 0; k < n; ++k) {
+    const cublox_recenter_item &it = items[k];
+    results[k] = cublox_recenter_result{};
+    if (!it.d_occ) continue;
+    const int st = cublox_plan_recenter(&it.cfg, nullptr, nullptr, nullptr,
+                                        results + k);
+    if (st != CUBLOX_OK) return st;
+    if (results[k].action == 0) continue;
+    std::vector<ZeroRegion> rs;
+    if (results[k].action == 2) {
+      rs.push_back(ZeroRegion{0, 1, 0, static_cast<long long>(it.cfg.map_vox_num)});
+    } else {
+      rs = recenter_regions(it.cfg);
+    }
+    for (const ZeroRegion &r : rs) {
+      if (count >= max_regions) {
+        return set_error(CUBLOX_ERR_CAPACITY, "recenter_batch: too many regions");
+      }
+      regs[count++] = ZeroRegionP{it.d_occ, r};
+    }
   }
-  for (const auto &run : merge_runs(p.slices[2])) {
-    regions.push_back(ZeroRegion{run.first + hz, mx * my, mz,
-                                 static_cast<long long>(run.second - run.first + 1)});
+  *hdr = RecenterBatchHeader{count, {0, 0, 0, 0, 0, 0, 0}};
+  return CUBLOX_OK;
+}
+
+int cublox_launch_recenter_batch_staged(const void *h_staging, void *d_area,
+                                        int max_regions, cublox_stream s) {
+  if (!h_staging || !d_area || max_regions <= 0) {
+    return set_error(CUBLOX_ERR_INVALID, "recenter_batch: bad arguments");
   }
-  return launch_regions(d_occ, d_op_cnt, d_hit_cnt, regions, hs);
+  hipStream_t hs = as_stream(s);
+  const int st = check(hipMemcpyAsync(d_area, h_staging,
+                                      cublox_recenter_batch_staging_bytes(max_regions),
+                                      hipMemcpyHostToDevice, hs),
+                       "hipMemcpyAsync(recenter batch)");
+  if (st != CUBLOX_OK) return st;
+  const auto *hdr = static_cast<const RecenterBatchHeader *>(d_area);
+  zero_regions_batch_kernel<<<dim3(kBatchRegionBlocks,
+                                   static_cast<unsigned>(max_regions)),
+                              dim3(256), 0, hs>>>(
+      hdr, reinterpret_cast<const ZeroRegionP *>(hdr + 1));
+  return check_launch("zero_regions_batch_kernel");
 }
 
 int cublox_launch_clear_recenter_slabs_for_axis(

@@ -2,6 +2,7 @@
 // (Python ctypes in this package; cgo/JNI would bind the same symbols).
 // Every entry point catches C++ exceptions and returns a status; the message
 // of the last failure on the calling thread is in cublox_og_last_error().
+#include <cublox/GridBatch.hpp>
 #include <cublox/OccupancyGrid.hpp>
 #include <cublox/capi.h>
 #include <cublox/og_capi.h>
@@ -267,6 +268,110 @@ int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
 
 int cublox_og_reset_kernel_times(cublox_og *h) {
   return guarded([&] { OGP->resetKernelTimes(); });
+}
+
+#define GBC reinterpret_cast<const cublox::GridBatch *>(b)
+#define GBP reinterpret_cast<cublox::GridBatch *>(b)
+
+cublox_gb *cublox_gb_create(const int limits[8]) {
+  cublox_gb *out = nullptr;
+  g_err.clear();
+  guarded([&] {
+    cublox::BatchLimits l;
+    l.max_grids = limits[0];
+    l.max_points = limits[1];
+    l.max_tiles = limits[2];
+    l.max_segments_per_ray = limits[3];
+    l.max_near_steps = limits[4];
+    l.min_chunk = limits[5];
+    l.max_slabs = limits[6];
+    l.max_regions = limits[7];
+    out = reinterpret_cast<cublox_gb *>(new cublox::GridBatch(l));
+  });
+  return out;
+}
+
+void cublox_gb_destroy(cublox_gb *b) { delete GBP; }
+
+int cublox_gb_segment_bound(const cublox_og *h) {
+  return cublox::GridBatch::segmentBound(*OGC);
+}
+
+int cublox_gb_tile_count(const cublox_og *h) {
+  return cublox::GridBatch::tileCount(*OGC);
+}
+
+int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
+                   const float *const *d_x, const float *const *d_y,
+                   const float *const *d_z, const int *n, const float *T,
+                   int n_recenter, cublox_og *const *rgrids,
+                   const float *rpos) {
+  return guarded([&] {
+    std::vector<cublox::GridScan> scans(static_cast<size_t>(std::max(n_scans, 0)));
+    for (int k = 0; k < n_scans; ++k) {
+      cublox::GridScan &s = scans[static_cast<size_t>(k)];
+      s.grid = reinterpret_cast<OG *>(grids[k]);
+      s.d_x = d_x[k];
+      s.d_y = d_y[k];
+      s.d_z = d_z[k];
+      s.n = n[k];
+      s.sensor_frame = true;
+      for (int i = 0; i < 12; ++i) s.T[static_cast<size_t>(i)] = T[12 * k + i];
+    }
+    std::vector<std::pair<OG *, Eigen::Vector3f>> rc;
+    for (int k = 0; k < n_recenter; ++k) {
+      rc.emplace_back(reinterpret_cast<OG *>(rgrids[k]), v3(rpos + 3 * k));
+    }
+    GBP->tick(scans, rc);
+  });
+}
+
+int cublox_gb_set_graph_mode(cublox_gb *b, int on) {
+  return guarded([&] { GBP->setGraphMode(on != 0); });
+}
+
+int cublox_gb_set_tuning(cublox_gb *b, int n, const int *settings,
+                         int ticks_each) {
+  return guarded([&] {
+    std::vector<cublox::TuneSetting> c;
+    for (int k = 0; k < n; ++k) {
+      c.push_back(cublox::TuneSetting{settings[2 * k], settings[2 * k + 1]});
+    }
+    GBP->setTuning(c, ticks_each);
+  });
+}
+
+int cublox_gb_tuning(const cublox_gb *b, int *active) {
+  return guarded([&] { *active = GBC->tuning() ? 1 : 0; });
+}
+
+int cublox_gb_setting(const cublox_gb *b, int out[2]) {
+  return guarded([&] {
+    const auto s = GBC->setting();
+    out[0] = s.near_steps;
+    out[1] = s.chunk;
+  });
+}
+
+int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk) {
+  return guarded([&] { GBP->setSetting(cublox::TuneSetting{near_steps, chunk}); });
+}
+
+int cublox_gb_tune_results(const cublox_gb *b, double *out_ms, int cap) {
+  return guarded([&] {
+    const auto r = GBC->tuneResults();
+    for (int k = 0; k < cap && k < static_cast<int>(r.size()); ++k) {
+      out_ms[k] = r[static_cast<size_t>(k)].second;
+    }
+  });
+}
+
+int cublox_gb_last(const cublox_gb *b, double out[3]) {
+  return guarded([&] {
+    out[0] = GBC->lastUpdateMs();
+    out[1] = GBC->lastRecenterMs();
+    out[2] = static_cast<double>(GBC->lastZeroedVoxels());
+  });
 }
 
 const float *cublox_og_device_occ(const cublox_og *h) {

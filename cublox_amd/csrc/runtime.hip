@@ -187,6 +187,34 @@ int cublox_event_elapsed_ms(cublox_event a, cublox_event b, float *ms) {
   return CUBLOX_OK;
 }
 
+int cublox_graph_begin_capture(cublox_stream s) {
+  CUBLOX_TRY(hipStreamBeginCapture(as_stream(s), hipStreamCaptureModeThreadLocal));
+  return CUBLOX_OK;
+}
+
+int cublox_graph_end_capture(cublox_stream s, cublox_graph *exec) {
+  if (!exec) return set_error(CUBLOX_ERR_INVALID, "graph: null output");
+  hipGraph_t g = nullptr;
+  CUBLOX_TRY(hipStreamEndCapture(as_stream(s), &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  CUBLOX_TRY(err);
+  *exec = e;
+  return CUBLOX_OK;
+}
+
+int cublox_graph_launch(cublox_graph exec, cublox_stream s) {
+  CUBLOX_TRY(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(exec), as_stream(s)));
+  return CUBLOX_OK;
+}
+
+int cublox_graph_destroy(cublox_graph exec) {
+  if (!exec) return CUBLOX_OK;
+  CUBLOX_TRY(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(exec)));
+  return CUBLOX_OK;
+}
+
 int cublox_fill_zero_async(void *ptr, size_t count32, cublox_stream s) {
   if (!ptr || count32 == 0) return CUBLOX_OK;
   if (reinterpret_cast<uintptr_t>(ptr) % 4 != 0) {

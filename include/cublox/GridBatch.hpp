@@ -1,0 +1,129 @@
+// Batched updates of many occupancy grids (extension; the reference updates
+// one grid per call, OccupancyGrid.cpp:181-238).
+//
+// One tick runs the fused update of every listed scan, each on its own
+// grid, and then the recenter of every listed grid, as one launch sequence
+// each on the batch's stream: every grid's rays, tiles and slabs drain
+// through the same kernels, so a tick fills the GPU with any mix of grids.
+// Each grid ends the tick exactly as its own update() + recenter() would
+// leave it. The per-grid descriptors travel in one host->device copy at the
+// head of each sequence; with graph mode on, the two sequences are captured
+// once and each tick restages and replays them.
+//
+// The online autotuner cycles candidate settings over the first ticks
+// (results do not depend on them) and keeps the fastest by device time.
+#pragma once
+
+#include <Eigen/Dense>
+#include <array>
+#include <utility>
+#include <vector>
+
+#include <cublox/OccupancyGrid.hpp>
+
+namespace cublox {
+
+struct BatchLimits {
+  int max_grids{64};
+  int max_points{1 << 22};          // points per tick, all grids
+  int max_tiles{1 << 20};           // sum of the grids' tile counts
+  int max_segments_per_ray{96};     // max over grids (segmentBound)
+  int max_near_steps{12};
+  int min_chunk{256};               // smallest hot-tile chunk any tick uses
+  int max_slabs{16384};             // 32 KB scratch slabs
+  int max_regions{0};               // recenter regions per tick (0: 6 per grid)
+};
+
+// One grid's scan of one tick: device-resident columns, sensor-frame with
+// T (row-major 3x4 [R | t] sensor->world) or world-frame from `origin`.
+struct GridScan {
+  OccupancyGrid *grid{nullptr};
+  const float *d_x{nullptr}, *d_y{nullptr}, *d_z{nullptr};
+  int n{0};
+  bool sensor_frame{true};
+  std::array<float, 12> T{};
+  Eigen::Vector3f origin{Eigen::Vector3f::Zero()};
+};
+
+struct TuneSetting {
+  int near_steps{8};
+  int chunk{1024};
+};
+
+class GridBatch {
+public:
+  explicit GridBatch(const BatchLimits &limits);
+  ~GridBatch();
+  GridBatch(const GridBatch &) = delete;
+  GridBatch &operator=(const GridBatch &) = delete;
+
+  // Per-ray segment bound and tile count of a grid (for the limits).
+  static int segmentBound(const OccupancyGrid &g);
+  static int tileCount(const OccupancyGrid &g);
+
+  // One tick: update every scan's grid, then recenter each listed grid
+  // toward its position (as recenter(pos)). Blocks until done.
+  void tick(const std::vector<GridScan> &scans,
+            const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
+                &recenters);
+
+  // Capture the tick's launch sequences in a HIP graph on the next tick and
+  // replay it afterwards.
+  void setGraphMode(bool on);
+  bool graphMode() const { return graph_mode_; }
+
+  // Online autotuning: each candidate runs `ticks_each` ticks (after one
+  // warm-up tick per candidate), then the fastest by mean device time is
+  // kept. An empty list disables tuning.
+  void setTuning(const std::vector<TuneSetting> &candidates, int ticks_each);
+  bool tuning() const { return tune_pos_ < tune_.size(); }
+  TuneSetting setting() const { return setting_; }
+  void setSetting(const TuneSetting &s);
+  std::vector<std::pair<TuneSetting, double>> tuneResults() const;
+
+  // Device time of the last tick's update and recenter sequences [ms],
+  // and the voxels its recenters zeroed.
+  double lastUpdateMs() const { return last_update_ms_; }
+  double lastRecenterMs() const { return last_recenter_ms_; }
+  long long lastZeroedVoxels() const { return last_zeroed_; }
+  void *stream() const { return stream_; }
+
+private:
+  void stage(const std::vector<GridScan> &scans,
+             const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
+                 &recenters);
+  void launch();
+  void finish();
+  void release() noexcept;
+
+  BatchLimits lim_;
+  void *stream_{nullptr};
+  void *d_ws_{nullptr};
+  size_t ws_bytes_{0};
+  void *h_stage_{nullptr};      // integrate staging (pinned)
+  void *h_rstage_{nullptr};     // recenter staging (pinned)
+  void *d_rarea_{nullptr};      // recenter staging, device copy
+  int max_regions_{0};
+  void *graph_{nullptr};
+  bool graph_mode_{false};
+  void *ev_[3]{nullptr, nullptr, nullptr};
+
+  // Recenter outcomes of the staged tick, applied to the grids after it.
+  std::vector<OccupancyGrid *> rc_grids_;
+  std::vector<Eigen::Vector3i> rc_origins_;
+  std::vector<long long> rc_zeroed_;
+
+  TuneSetting setting_{};
+  std::vector<TuneSetting> tune_;
+  std::vector<double> tune_ms_;
+  std::vector<int> tune_n_;
+  size_t tune_pos_{0};
+  int tune_ticks_{0};
+  int tune_count_{0};
+
+  double last_update_ms_{0.0};
+  double last_recenter_ms_{0.0};
+  long long last_zeroed_{0};
+};
+
+}  // namespace cublox

@@ -20,6 +20,8 @@ namespace cublox {
 
 using PointCloud = Eigen::Matrix<float, Eigen::Dynamic, 3>;
 
+class GridBatch;
+
 class OccupancyGrid : public Grid {
 public:
   enum class VoxelState {
@@ -176,6 +178,8 @@ public:
   float getLogOddsOccupied() const { return l_occupied_; }
 
 private:
+  friend class GridBatch;  // batched updates read the grid's state
+
   bool first_run_{true};
 
   float storedToLogit(const float s) const noexcept { return s + l_unknown_; }

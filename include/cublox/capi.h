@@ -82,6 +82,13 @@ int cublox_event_elapsed_ms(cublox_event start, cublox_event stop, float *ms);
  * unlike a memset node). */
 int cublox_fill_zero_async(void *ptr, size_t count32, cublox_stream s);
 
+/* HIP graphs: capture the work enqueued on a stream, replay it. */
+typedef void *cublox_graph; /* hipGraphExec_t */
+int cublox_graph_begin_capture(cublox_stream s);
+int cublox_graph_end_capture(cublox_stream s, cublox_graph *exec);
+int cublox_graph_launch(cublox_graph exec, cublox_stream s);
+int cublox_graph_destroy(cublox_graph exec);
+
 /* ------------------------------------------------------------------ */
 /* Ray cast. Replaces RayCastCfg (kernels.hpp:33-44).                   */
 /* ------------------------------------------------------------------ */
@@ -179,6 +186,26 @@ int cublox_launch_recenter(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
 
 /* launchClearRecenterSlabsForAxis (kernels.hpp:112-117): zero every voxel
  * whose local index on `axis` is listed in the DEVICE array d_slices. */
+/* Batched recenter (no reference counterpart): the plans of many grids,
+ * made on the host as cublox_plan_recenter does, zeroed in one launch.
+ * Stage (host only; results[k] is grid k's plan outcome, as
+ * cublox_launch_recenter reports it), then launch from the staging buffer
+ * with d_area (device, cublox_recenter_batch_staging_bytes bytes) as the
+ * device copy. The launch shape depends on max_regions only, so it can be
+ * captured in a HIP graph. Counters are not cleared (they are zero
+ * between updates). */
+typedef struct {
+  cublox_recenter_cfg cfg;
+  float *d_occ;
+} cublox_recenter_item;
+
+size_t cublox_recenter_batch_staging_bytes(int max_regions);
+int cublox_recenter_batch_stage(const cublox_recenter_item *items, int n,
+                                int max_regions, void *h_staging,
+                                cublox_recenter_result *results);
+int cublox_launch_recenter_batch_staged(const void *h_staging, void *d_area,
+                                        int max_regions, cublox_stream s);
+
 int cublox_launch_clear_recenter_slabs_for_axis(
     float *d_occ, int *d_op_cnt, int *d_hit_cnt, cublox_int3 map_size_i,
     cublox_int3 half_map_size_i, const int *d_slices, int n_slices, int axis,
@@ -283,6 +310,66 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
                             size_t ws_bytes, int max_points, int near_steps,
                             int *d_export_op, int *d_export_hit,
                             cublox_stream s);
+
+/* Per-ray segment bound and tile count of a grid geometry (for sizing
+ * batch limits). */
+int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg);
+int cublox_integrate_tile_count(const cublox_raycast_cfg *cfg);
+
+/* ------------------------------------------------------------------ */
+/* Batched fused update (no reference counterpart; the reference updates
+ * one grid per call, kernels.hpp:54-76). One launch sequence updates a
+ * batch of grids, each with its own geometry, window, sensor pose, points,
+ * occupancy buffer and log-odds, with the same per-grid result as
+ * cublox_launch_integrate on that grid alone. The launch shapes depend on
+ * the limits only and the per-grid descriptors travel in one host->device
+ * copy at its head, so cublox_launch_integrate_batch_staged can be
+ * captured once in a HIP graph and replayed per tick after restaging.
+ * The staging buffer must be pinned host memory (cublox_host_alloc) when
+ * the launch is captured or asynchronous with respect to restaging.      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int max_grids;            /* grids per launch                           */
+  int max_points;           /* points per launch, all grids               */
+  int max_tiles;            /* sum of cublox_integrate_tile_count         */
+  int max_segments_per_ray; /* max of cublox_integrate_segment_bound      */
+  int max_near_steps;       /* largest near_steps of any grid, 1..12      */
+  int chunk;                /* smallest records per hot-tile work item
+                               any launch will use (0: 1024)             */
+  int max_slabs;            /* hot-tile scratch slabs, 32 KB each (0:
+                               16384); hot tiles past them run unsplit   */
+} cublox_batch_limits;
+
+typedef struct {
+  cublox_raycast_cfg cfg;   /* geometry, window; origin = sensor if !use_T */
+  float T[12];              /* row-major 3x4 sensor->world                 */
+  int use_T;                /* points are sensor-frame                     */
+  const float *d_xs, *d_ys, *d_zs;
+  int n;
+  float *d_occ;
+  float l_hit, l_miss, l_min, l_max, l_unknown;
+  int near_steps;           /* 1..max_near_steps (0: 8)                    */
+  int *d_export_op, *d_export_hit; /* optional: counts instead of a fold  */
+} cublox_integrate_grid;
+
+size_t cublox_integrate_batch_workspace_bytes(const cublox_batch_limits *lim);
+size_t cublox_integrate_batch_staging_bytes(const cublox_batch_limits *lim);
+/* Host only: writes the batch's descriptors into h_staging. chunk > 0
+ * sets this launch's records per hot-tile item (>= lim->chunk), else the
+ * limits' chunk; results do not depend on it. */
+int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
+                                 int n_grids, const cublox_batch_limits *lim,
+                                 int chunk, void *h_staging);
+/* Copies h_staging to the workspace and launches the update sequence. */
+int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
+                                         size_t ws_bytes,
+                                         const cublox_batch_limits *lim,
+                                         cublox_stream s);
+/* stage + staged launch. */
+int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,
+                                  int n_grids, const cublox_batch_limits *lim,
+                                  void *h_staging, void *d_ws,
+                                  size_t ws_bytes, cublox_stream s);
 
 /* ------------------------------------------------------------------ */
 /* Counter census (no reference counterpart): over n voxels, adds to

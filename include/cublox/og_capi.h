@@ -130,6 +130,35 @@ int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
                            long long out_n[4]);
 int cublox_og_reset_kernel_times(cublox_og *h);
 
+/* Batched updates of many grids (cublox::GridBatch, no reference
+ * counterpart). limits: max_grids, max_points, max_tiles,
+ * max_segments_per_ray, max_near_steps, min_chunk, max_slabs, max_regions
+ * (8 ints; see include/cublox/GridBatch.hpp). */
+typedef struct cublox_gb cublox_gb;
+cublox_gb *cublox_gb_create(const int limits[8]);
+void cublox_gb_destroy(cublox_gb *b);
+int cublox_gb_segment_bound(const cublox_og *g);
+int cublox_gb_tile_count(const cublox_og *g);
+/* One tick: n_scans scans (grids[k]: device columns d_x/d_y/d_z[k] of
+ * n[k] points, sensor-frame with T[12k..12k+11]), then the recenter of
+ * rgrids[k] toward rpos[3k..3k+2]. Blocks until done. */
+int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
+                   const float *const *d_x, const float *const *d_y,
+                   const float *const *d_z, const int *n, const float *T,
+                   int n_recenter, cublox_og *const *rgrids,
+                   const float *rpos);
+int cublox_gb_set_graph_mode(cublox_gb *b, int on);
+/* settings: n pairs (near_steps, chunk); ticks_each measured ticks each. */
+int cublox_gb_set_tuning(cublox_gb *b, int n, const int *settings,
+                         int ticks_each);
+int cublox_gb_tuning(const cublox_gb *b, int *active);
+int cublox_gb_setting(const cublox_gb *b, int out[2]);
+int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk);
+/* out_ms[k]: mean device ms of candidate k (cap entries). */
+int cublox_gb_tune_results(const cublox_gb *b, double *out_ms, int cap);
+/* out: update ms, recenter ms (0 in graph mode: all in update), zeroed. */
+int cublox_gb_last(const cublox_gb *b, double out[3]);
+
 /* Index math (reference include/cublox/utils.hpp:24-181). */
 int cublox_utils_pos_to_global(float p, float inv_res, int center);
 float cublox_utils_global_to_pos(int g, float res, int center);

@@ -41,7 +41,7 @@ def test_host_library_exports(cb):
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     src = open(os.path.join(ROOT, "cublox_amd", "csrc", "og_capi.cpp")).read()
-    defined = sorted(set(re.findall(r"^\S.*?\b(cublox_(?:og|utils)_[a-z0-9_]+)\(", src,
+    defined = sorted(set(re.findall(r"^\S.*?\b(cublox_(?:og|gb|utils)_[a-z0-9_]+)\(", src,
                                     flags=re.M)))
     assert sorted(set(defined) - set(names)) == []
 
