@@ -134,6 +134,95 @@ def run_pcie(cb, w, scans, warmup, steps):
     return per
 
 
+TUNE_CANDIDATES = [(4, 512), (8, 512), (8, 1024), (12, 1024), (8, 2048)]
+
+
+def run_batched(cb, tune_ticks_each, ticks):
+    """The batched workload: 64 grids of 401x401x101, per-grid beams,
+    azimuths, range and scene. The same scans run through the plain form
+    (each grid's blocking update + recenter in turn) and through one
+    GridBatch tick per step (graph replay, after online tuning)."""
+    from cublox_amd.workloads import batched_grid
+    w = cb.WORKLOADS["batched"]
+    G = w.extra["grids"]
+    tune_ticks = len(TUNE_CANDIDATES) * (tune_ticks_each + 1)
+    total = tune_ticks + ticks
+    draws = [batched_grid(w, g) for g in range(G)]
+    t = time.perf_counter()
+    scans = [[d[3].scan(k, d[0], d[1], w.el_lo, w.el_hi, d[2]) for k in range(total)]
+             for d in draws]
+    bufs = [[[cb.DeviceBuffer.from_array(np.ascontiguousarray(c))
+              for c in (s.xs, s.ys, s.zs)] for s in gs] for gs in scans]
+    pts = [sum(scans[g][k].n for g in range(G)) for k in range(total)]
+    log(f"batched: {G} grids, {statistics.mean(pts):.0f} points/tick, generated + "
+        f"uploaded {G * total} scans in {time.perf_counter() - t:.1f} s")
+
+    def grids():
+        out = []
+        for g in range(G):
+            x = cb.OccupancyGrid(w.half, w.resolution, w.origin_at_center, w.threshold)
+            x.set_max_raycast_range(draws[g][2])
+            x.set_log_odds_params(*cb.log_odds())
+            out.append(x)
+        return out
+
+    # Plain form.
+    plain = grids()
+    per_scan, tick_ms = [], []
+    for k in range(total):
+        a = time.perf_counter()
+        for g in range(G):
+            s = scans[g][k]
+            b = bufs[g][k]
+            c0 = time.perf_counter()
+            plain[g].update_sensor_frame_device(b[0].ptr, b[1].ptr, b[2].ptr, s.n, s.T)
+            plain[g].recenter(s.base)
+            if k >= tune_ticks:
+                per_scan.append((time.perf_counter() - c0) * 1e3)
+        if k >= tune_ticks:
+            tick_ms.append((time.perf_counter() - a) * 1e3)
+    for x in plain:
+        x.close()
+    plain_sps = G * len(tick_ms) / (sum(tick_ms) * 1e-3)
+
+    # Batched form.
+    bg = grids()
+    batch = cb.GridBatch.for_grids(bg, max(pts), max_near_steps=12, min_chunk=512)
+    batch.set_graph_mode(True)
+    batch.set_tuning(TUNE_CANDIDATES, tune_ticks_each)
+    btick, bdev = [], []
+    for k in range(total):
+        sc = [(bg[g], *bufs[g][k], scans[g][k].n, scans[g][k].T) for g in range(G)]
+        rc = [(bg[g], scans[g][k].base) for g in range(G)]
+        a = time.perf_counter()
+        batch.tick(sc, rc)
+        if k >= tune_ticks:
+            btick.append((time.perf_counter() - a) * 1e3)
+            bdev.append(batch.last[0])
+    batch_sps = G * len(btick) / (sum(btick) * 1e-3)
+    tune = batch.tune_results(len(TUNE_CANDIDATES))
+    out = dict(
+        grids=G, grid=f"{w.map_size[0]}x{w.map_size[1]}x{w.map_size[2]} @ {w.resolution} m",
+        points_per_tick=round(statistics.mean(pts)), ticks=len(btick),
+        plain=dict(scans_per_s=round(plain_sps, 1), tick_ms=round(statistics.mean(tick_ms), 3),
+                   scan_p50_ms=round(pct(per_scan, 50), 4),
+                   scan_p95_ms=round(pct(per_scan, 95), 4),
+                   scan_p99_ms=round(pct(per_scan, 99), 4)),
+        batched=dict(scans_per_s=round(batch_sps, 1),
+                     tick_p50_ms=round(pct(btick, 50), 4),
+                     tick_p95_ms=round(pct(btick, 95), 4),
+                     tick_p99_ms=round(pct(btick, 99), 4),
+                     device_tick_ms=round(statistics.mean(bdev), 4),
+                     graph=True, setting=dict(zip(("near_steps", "chunk"), batch.setting)),
+                     tuning_ms={f"{a}/{b}": round(m, 4) for (a, b), m in zip(TUNE_CANDIDATES, tune)}),
+        speedup_scans_per_s=round(batch_sps / plain_sps, 2),
+    )
+    batch.close()
+    for x in bg:
+        x.close()
+    return out
+
+
 def run_fetch(cb, w, g, scene, k, reps=5):
     if not w.fetch_queries:
         return None
@@ -190,7 +279,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip census, PCIe-inclusive and fetch passes")
+                    help="skip census, PCIe-inclusive, fetch and batched passes")
+    ap.add_argument("--batched-ticks", type=int, default=20,
+                    help="timed ticks of the 64-grid batched workload (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -272,6 +363,9 @@ def main():
         pcie = run_pcie(cb, w, scans, args.warmup, args.steps)
         extras["pcie_inclusive_ms_per_scan"] = statistics.mean(pcie)
         extras["fetch"] = run_fetch(cb, w, g, scene, total - 1)
+        if args.batched_ticks > 0:
+            extras["batched_workload"] = run_batched(cb, 1, args.batched_ticks)
+            log(f"batched: {extras['batched_workload']}")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -165,6 +165,9 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_integrate_batch_stage", i, vp, i, vp, i, vp)
     _sig(L, "cublox_launch_integrate_batch_staged", i, vp, vp, S, vp, vp)
     _sig(L, "cublox_launch_integrate_batch", i, vp, i, vp, vp, vp, S, vp)
+    _sig(L, "cublox_integrate_batch_counters", i, vp, S, vp, vp, vp)
+    _sig(L, "cublox_integrate_counters", i, vp, S, C.POINTER(RaycastCfg), i, i,
+         vp, vp)
     _sig(L, "cublox_recenter_batch_staging_bytes", S, i)
     _sig(L, "cublox_recenter_batch_stage", i, vp, i, i, vp, vp)
     _sig(L, "cublox_launch_recenter_batch_staged", i, vp, vp, i, vp)
@@ -232,6 +235,7 @@ def host() -> C.CDLL:
     _sig(L, "cublox_gb_set_setting", i, vp, i, i)
     _sig(L, "cublox_gb_tune_results", i, vp, C.POINTER(C.c_double), i)
     _sig(L, "cublox_gb_last", i, vp, C.POINTER(C.c_double))
+    _sig(L, "cublox_gb_counters", i, vp, C.POINTER(C.c_int))
     _sig(L, "cublox_utils_pos_to_global", i, F, F, i)
     _sig(L, "cublox_utils_global_to_pos", F, i, F, i)
     _sig(L, "cublox_utils_global_to_local", i, i, i, i)

@@ -122,3 +122,11 @@ class GridBatch:
         out = (C.c_double * 3)()
         self._check(self._L.cublox_gb_last(self._h, out))
         return out[0], out[1], int(out[2])
+
+    def counters(self) -> dict:
+        """Counters of the last tick's update: work items, tile records,
+        overflow flag, split and unsplit hot tiles."""
+        out = (C.c_int * 5)()
+        self._check(self._L.cublox_gb_counters(self._h, out))
+        return dict(items=out[0], records=out[1], overflow=out[2],
+                    split_tiles=out[3], unsplit_hot=out[4])

@@ -106,6 +106,14 @@ void GridBatch::release() noexcept {
   stream_ = nullptr;
 }
 
+std::array<int, 5> GridBatch::lastCounters() const {
+  std::array<int, 5> out{};
+  const cublox_batch_limits c = abiLimits(lim_);
+  CUBLOX_CHECK(cublox_integrate_batch_counters(d_ws_, ws_bytes_, &c, out.data(),
+                                               stream_));
+  return out;
+}
+
 int GridBatch::segmentBound(const OccupancyGrid &g) {
   const cublox_raycast_cfg c = raycastCfg(g, g.getMaxRaycastRange());
   return cublox_integrate_segment_bound(&c);

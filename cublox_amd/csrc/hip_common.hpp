@@ -36,6 +36,35 @@ inline hipStream_t as_stream(cublox_stream s) {
   return reinterpret_cast<hipStream_t>(s);
 }
 
+// Global-memory view of a device pointer that a kernel reads from memory (a
+// descriptor field) rather than receives as an argument. Without it the
+// compiler cannot prove the address space and emits flat_* accesses, which
+// also count against the LDS wait counter and so serialise with LDS work.
+#define CUBLOX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ CUBLOX_GLOBAL T *as_global(T *p) {
+  return (CUBLOX_GLOBAL T *)p;
+}
+
+// Memory that no thread writes while a kernel runs (launch descriptors) is
+// read through the constant address space: uniform addresses then compile to
+// scalar loads.
+#define CUBLOX_CONST __attribute__((address_space(4)))
+
+// Copy of *p read word by word through the constant address space; pointer
+// fields of the copy are then known to address global memory.
+template <typename T>
+__device__ __forceinline__ T load_const(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "load_const: whole 32-bit words");
+  T v;
+  int *o = reinterpret_cast<int *>(&v);
+#pragma unroll
+  for (int i = 0; i < static_cast<int>(sizeof(T) / 4); ++i) {
+    o[i] = ((const CUBLOX_CONST int *)p)[i];
+  }
+  return v;
+}
+
 // ---------------------------------------------------------------------
 // Index math of the toroidal window (reference utils.cuh:24-176). Every FP32
 // expression keeps the reference's operand order; the library is built with

@@ -43,6 +43,7 @@
 #include "raycast_common.hpp"
 
 #include <algorithm>
+#include <cstddef>
 
 namespace cublox_hip {
 
@@ -136,9 +137,11 @@ struct Workspace {
   Item *items;        // max_items, costliest first
   HotTile *hot;       // split tiles
   unsigned *scratch;  // one kTileVox slab per hot-tile chunk, op | hit << 16
-  int *misc;          // [0] items, [1] segments, [2] overflow flag, [3] split tiles
+  int *misc;          // [0] items, [1] segments, [2] overflow flag, [3] split
+                      // tiles, [4] unsplit hot tiles
   int *near_op;       // near-field cells (zero between launches)
   int *near_hit;      // near-field cells (zero between launches)
+  int *big;           // item slots of unsplit hot tiles (passes)
   int *block_sums;    // multi-block tile scan: per-block sums
   int *block_bases;   // multi-block tile scan: per-block bases
   long long seg_capacity;
@@ -422,48 +425,58 @@ struct Batch {
   const BatchHeader *hdr;    // batch: counts (device)
 };
 
+// The descriptors and the header stay unchanged while a launch runs, so
+// they are read through the constant address space: scalar loads into
+// SGPRs, not vector loads into every lane's registers.
+template <typename T>
+__device__ __forceinline__ int const_int(const T *p, int word) {
+  return ((const CUBLOX_CONST int *)(p))[word];
+}
+__device__ __forceinline__ int batch_word(const Batch &b, int word) {
+  return const_int(b.hdr, word);
+}
 __device__ __forceinline__ int batch_grids(const Batch &b) {
-  return b.descs ? b.hdr->n_grids : 1;
+  return b.descs ? batch_word(b, 0) : 1;
 }
 __device__ __forceinline__ int batch_rays(const Batch &b) {
-  return b.descs ? b.hdr->n_rays : b.one.ray_base + b.one.g.n;
+  return b.descs ? batch_word(b, 1) : b.one.ray_base + b.one.g.n;
 }
 __device__ __forceinline__ int batch_tiles(const Batch &b) {
-  return b.descs ? b.hdr->n_tiles : b.one.g.n_tiles;
+  return b.descs ? batch_word(b, 2) : b.one.g.n_tiles;
 }
 __device__ __forceinline__ int batch_near(const Batch &b) {
-  return b.descs ? b.hdr->n_near : b.one.g.cube * b.one.g.cube * b.one.g.cube;
+  return b.descs ? batch_word(b, 3) : b.one.g.cube * b.one.g.cube * b.one.g.cube;
 }
 __device__ __forceinline__ int batch_chunk(const Batch &b, const Workspace &w) {
-  return b.descs ? b.hdr->chunk : w.chunk;
+  return b.descs ? batch_word(b, 4) : w.chunk;
 }
+static_assert(offsetof(BatchHeader, n_rays) == 4 && offsetof(BatchHeader, chunk) == 16,
+              "BatchHeader words");
 
-// Last grid whose base (selected by Field) is <= key. Keys are uniform
-// across the calling workgroup.
-template <typename Field>
-__device__ __forceinline__ const GridDesc &grid_at(const Batch &b, long long key,
-                                                   Field field) {
+// Last grid whose base (the int field at byte offset `off`) is <= key.
+// Keys are uniform across the calling workgroup.
+__device__ __forceinline__ GridDesc grid_at(const Batch &b, int key, size_t off) {
   if (!b.descs) return b.one;
-  int lo = 0, hi = b.hdr->n_grids - 1;
+  int lo = 0, hi = batch_grids(b) - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (field(b.descs[mid]) <= key) {
+    if (const_int(b.descs + mid, static_cast<int>(off / 4)) <= key) {
       lo = mid;
     } else {
       hi = mid - 1;
     }
   }
-  return b.descs[lo];
+  return load_const(b.descs + lo);
 }
 
-__device__ __forceinline__ const GridDesc &grid_of_ray(const Batch &b, int ray) {
-  return grid_at(b, ray, [](const GridDesc &d) { return (long long)d.ray_base; });
+__device__ __forceinline__ GridDesc grid_of_ray(const Batch &b, int ray) {
+  return grid_at(b, ray, offsetof(GridDesc, ray_base));
 }
-__device__ __forceinline__ const GridDesc &grid_of_tile(const Batch &b, int tile) {
-  return grid_at(b, tile, [](const GridDesc &d) { return (long long)d.tile_base; });
+__device__ __forceinline__ GridDesc grid_of_tile(const Batch &b, int tile) {
+  return grid_at(b, tile, offsetof(GridDesc, tile_base));
 }
-__device__ __forceinline__ const GridDesc &grid_of_near(const Batch &b, int cell) {
-  return grid_at(b, cell, [](const GridDesc &d) { return (long long)d.near_base; });
+__device__ __forceinline__ GridDesc grid_of_near(const Batch &b, int cell) {
+  return grid_at(b, cell, offsetof(GridDesc, near_base));
 }
 
 // ----------------------------------------------------------------------
@@ -474,7 +487,7 @@ __global__ void __launch_bounds__(kWalkBlock)
   extern __shared__ int s_near[];  // the grid's near cube, sized at launch
   const int ray0 = blockIdx.x * kWalkBlock;
   if (ray0 >= batch_rays(b)) return;
-  const GridDesc &d = grid_of_ray(b, ray0);
+  const GridDesc d = grid_of_ray(b, ray0);
   const IntegrateArgs &g = d.g;
   const int cells = g.cube * g.cube * g.cube;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
@@ -485,8 +498,10 @@ __global__ void __launch_bounds__(kWalkBlock)
   const RayArgs &a = g.a;
   RayState r;
   const bool valid =
-      i < g.n && (d.transform ? ray_setup<true>(a, d.xs, d.ys, d.zs, i, r)
-                              : ray_setup<false>(a, d.xs, d.ys, d.zs, i, r));
+      i < g.n &&
+      (d.transform
+           ? ray_setup<true>(a, as_global(d.xs), as_global(d.ys), as_global(d.zs), i, r)
+           : ray_setup<false>(a, as_global(d.xs), as_global(d.ys), as_global(d.zs), i, r));
   if (valid) {
     const unsigned signs =
         static_cast<unsigned>((r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4))
@@ -625,7 +640,7 @@ __global__ void __launch_bounds__(256)
   __shared__ int s_base[kHash];
   const int r0 = blockIdx.x * kBinRays;
   if (r0 >= batch_rays(b)) return;
-  const GridDesc &d = grid_of_ray(b, r0);
+  const GridDesc d = grid_of_ray(b, r0);
   for (int e = threadIdx.x; e < kHash; e += blockDim.x) {
     s_key[e] = -1;
     s_cnt[e] = 0;
@@ -686,6 +701,7 @@ constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
 constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
 constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
 constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
+constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
 
 __device__ __forceinline__ int wave_scan_incl(int v) {
   const int lane = threadIdx.x & 63;
@@ -745,7 +761,9 @@ __device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
     if (q < ch) {
       const int ib = static_cast<int>(run) + q * per_chunk;
       const int ie = min(ib + per_chunk, static_cast<int>(run) + c);
-      w.items[pos] = Item{k, ib, ie, split ? sl + q : -1};
+      const bool big = !split && c > chunk;  // hot tile without slabs
+      w.items[pos] = Item{k, ib, ie, split ? sl + q : (big ? -2 : -1)};
+      if (big) w.big[atomicAdd(w.misc + 4, 1)] = pos;
     } else {
       w.items[pos] = Item{-1, 0, 0, -1};
     }
@@ -763,6 +781,7 @@ __global__ void __launch_bounds__(kScanThreads)
   const int chunk = batch_chunk(b, w);
   const int t = threadIdx.x;
   if (t < kCostBuckets) s_bucket[t] = 0;
+  if (t == 0) w.misc[4] = 0;
   for (int k = t; k < n_tiles; k += kScanThreads) {
     s_count[k] = w.tile_count[k];
     w.tile_count[k] = 0;
@@ -882,7 +901,10 @@ __global__ void __launch_bounds__(kScanThreads)
   for (int q = 0; q < kCostBuckets; ++q) {
     if (t < n_blocks) w.block_bases[t * kScanSums + 3 + q] += s_total[3 + q];
   }
-  if (t == 0) w.misc[3] = 0;  // split tiles, counted by pass 3
+  if (t == 0) {  // split and unsplit hot tiles, counted by pass 3
+    w.misc[3] = 0;
+    w.misc[4] = 0;
+  }
 }
 
 // Pass 3: per block, cursors, items and split tiles from the block bases.
@@ -946,39 +968,97 @@ __global__ void __launch_bounds__(kScanThreads)
 // tile_apply
 // ----------------------------------------------------------------------
 
+// Replays one record into the packed LDS counters.
+__device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt) {
+  int li = static_cast<int>(sg.meta & kMetaLi);
+  if (sg.meta & kMetaEndpoint) {
+    atomicAdd(s_cnt + li, (sg.meta & kMetaClipped) ? 1u : 0x10001u);
+    return;
+  }
+  int n = static_cast<int>((sg.meta >> kMetaN) & 0x7Fu);
+  const int sx = static_cast<int>((sg.meta >> kMetaSign) & 3u) - 1;
+  const int sy = static_cast<int>((sg.meta >> (kMetaSign + 2)) & 3u) - 1;
+  const int sz = static_cast<int>((sg.meta >> (kMetaSign + 4)) & 3u) - 1;
+  const int ldx = sx * (TY * TZ), ldy = sy * TZ, ldz = sz;
+  float tMx = sg.tMx, tMy = sg.tMy, tMz = sg.tMz;
+  for (; n > 0; --n) {
+    atomicAdd(s_cnt + li, 1u);
+    const bool x_lt_y = tMx < tMy;
+    const bool step_x = x_lt_y && (tMx < tMz);
+    const bool step_y = !x_lt_y && (tMy < tMz);
+    const float nx = tMx + sg.tDx;
+    const float ny = tMy + sg.tDy;
+    const float nz = tMz + sg.tDz;
+    tMx = step_x ? nx : tMx;
+    tMy = step_y ? ny : tMy;
+    tMz = (!step_x && !step_y) ? nz : tMz;
+    li += step_x ? ldx : (step_y ? ldy : ldz);
+  }
+}
+
+constexpr int kRound = 1024;  // records ordered per round, 4 per thread
+constexpr int kLenClasses = 64;
+
+struct ReplayScratch {  // LDS of the length ordering
+  int hist[kLenClasses];
+  unsigned short order[kRound];
+};
+
 // Accumulates records [beg, end) of a tile into packed LDS counters
 // (op | hit << 16): each segment replays its n voxels from its entry state
-// (the same crossing parameters and tie-break as the walk). A call covers
-// at most kPass records and a record visits a voxel at most once, so
-// neither field carries into the other.
+// (the same crossing parameters and tie-break as the walk). Records are
+// taken in rounds of kRound, counting-sorted by length (longest first) so
+// the 64 lanes of a wavefront replay segments of similar length. A call
+// covers at most kPass records and a record visits a voxel at most once,
+// so neither field carries into the other.
 __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
-                                                int end, unsigned *s_cnt) {
-  for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
-    const Segment sg = w.sorted[k];
-    int li = static_cast<int>(sg.meta & kMetaLi);
-    if (sg.meta & kMetaEndpoint) {
-      atomicAdd(s_cnt + li, (sg.meta & kMetaClipped) ? 1u : 0x10001u);
+                                                int end, unsigned *s_cnt,
+                                                ReplayScratch &rs) {
+  const int t = threadIdx.x;
+  constexpr int kPer = kRound / kApplyThreads;
+  for (int r0 = beg; r0 < end; r0 += kRound) {
+    const int m = min(kRound, end - r0);
+    if (m <= kApplyThreads) {
+      // One record per thread at most: the round's latency is its longest
+      // record whatever the order.
+      if (t < m) replay_record(w.sorted[r0 + t], s_cnt);
+      __syncthreads();
       continue;
     }
-    int n = static_cast<int>((sg.meta >> kMetaN) & 0x7Fu);
-    const int sx = static_cast<int>((sg.meta >> kMetaSign) & 3u) - 1;
-    const int sy = static_cast<int>((sg.meta >> (kMetaSign + 2)) & 3u) - 1;
-    const int sz = static_cast<int>((sg.meta >> (kMetaSign + 4)) & 3u) - 1;
-    const int ldx = sx * (TY * TZ), ldy = sy * TZ, ldz = sz;
-    float tMx = sg.tMx, tMy = sg.tMy, tMz = sg.tMz;
-    for (; n > 0; --n) {
-      atomicAdd(s_cnt + li, 1u);
-      const bool x_lt_y = tMx < tMy;
-      const bool step_x = x_lt_y && (tMx < tMz);
-      const bool step_y = !x_lt_y && (tMy < tMz);
-      const float nx = tMx + sg.tDx;
-      const float ny = tMy + sg.tDy;
-      const float nz = tMz + sg.tDz;
-      tMx = step_x ? nx : tMx;
-      tMy = step_y ? ny : tMy;
-      tMz = (!step_x && !step_y) ? nz : tMz;
-      li += step_x ? ldx : (step_y ? ldy : ldz);
+    if (t < kLenClasses) rs.hist[t] = 0;
+    __syncthreads();
+    int cls[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int k = t + j * kApplyThreads;
+      cls[j] = -1;
+      if (k < m) {
+        const unsigned meta = w.sorted[r0 + k].meta;
+        const int n = (meta & kMetaEndpoint) ? 0 : static_cast<int>((meta >> kMetaN) & 0x7Fu);
+        cls[j] = kLenClasses - 1 - min(n, kLenClasses - 1);
+        atomicAdd(rs.hist + cls[j], 1);
+      }
     }
+    __syncthreads();
+    if (t < 64) {  // one wavefront: exclusive scan of the classes
+      const int v = rs.hist[t];
+      rs.hist[t] = wave_scan_incl(v) - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (cls[j] >= 0) {
+        const int pos = atomicAdd(rs.hist + cls[j], 1);
+        rs.order[pos] = static_cast<unsigned short>(t + j * kApplyThreads);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int pos = t + j * kApplyThreads;
+      if (pos < m) replay_record(w.sorted[r0 + rs.order[pos]], s_cnt);
+    }
+    __syncthreads();
   }
 }
 
@@ -998,10 +1078,11 @@ __device__ __forceinline__ int tile_voxel_hash(const RayArgs &a, int x0,
 
 // Fold of the tile counts a thread owns (voxels j * 256 + t, j < 32: a
 // z-run of 32 contiguous floats per 32 lanes): every occ load is issued
-// before the first update, so a tile costs one memory round trip.
+// before the first update, so a tile costs one memory round trip. Counts
+// come from op(j) and hit(j).
+template <typename Op, typename Hit>
 __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
-                                           int z0, const unsigned *op,
-                                           const unsigned *hit) {
+                                           int z0, Op op, Hit hit) {
   const RayArgs &a = d.g.a;
   const int t = threadIdx.x;
   auto hash = [&](int j) {
@@ -1009,48 +1090,52 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
            (y0 + ((j & 1) << 3) + (t >> 5)) * a.mz + (z0 + (t & 31));
   };
   if (d.export_op) {
+    CUBLOX_GLOBAL int *eop = as_global(d.export_op);
+    CUBLOX_GLOBAL int *ehit = as_global(d.export_hit);
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
-      if (op[j] == 0) continue;
-      d.export_op[hash(j)] = static_cast<int>(op[j]);
-      d.export_hit[hash(j)] = static_cast<int>(hit[j]);
+      if (op(j) == 0) continue;
+      eop[hash(j)] = static_cast<int>(op(j));
+      ehit[hash(j)] = static_cast<int>(hit(j));
     }
     return;
   }
+  CUBLOX_GLOBAL float *occ = as_global(d.occ);
   float v[kPerThread];
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
-    v[j] = op[j] != 0 ? d.occ[hash(j)] : 0.0f;
+    v[j] = op(j) != 0 ? occ[hash(j)] : 0.0f;
   }
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
-    if (op[j] == 0) continue;
-    d.occ[hash(j)] = apply_logodds(v[j], static_cast<int>(op[j]),
-                                   static_cast<int>(hit[j]), d.g.p);
+    if (op(j) == 0) continue;
+    occ[hash(j)] = apply_logodds(v[j], static_cast<int>(op(j)),
+                                 static_cast<int>(hit(j)), d.g.p);
   }
 }
 
 // Work items in decreasing cost order, one per workgroup while they last
 // (then strided), so the hardware dispatcher balances them. A plain item
-// accumulates its tile in LDS (in passes of kPass records, summed in
-// registers, for an unsplit hot tile) and folds it. A chunk of a hot tile
-// stores its packed counts to its scratch slab; hot_fold sums a tile's
-// slabs. Empty items (slots an unsplit hot tile left) do nothing.
+// accumulates its tile in LDS and folds it; a chunk of a hot tile stores
+// its packed counts to its scratch slab (hot_fold sums a tile's slabs).
+// kBig: the unsplit hot tiles instead, accumulated in passes of kPass
+// records summed in registers. Empty items do nothing.
+template <bool kBig>
 __global__ void __launch_bounds__(kApplyThreads, 4)
     tile_apply_kernel(const Batch b, Workspace w) {
   __shared__ uint4 s_cnt4[kTileVox / 4];
+  __shared__ ReplayScratch s_rs;
   unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
-  const int n_items = w.misc[0];
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const Item itm = w.items[item];
-    if (itm.tile < 0) continue;  // uniform across the workgroup
+  const int n_items = kBig ? w.misc[4] : w.misc[0];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const Item itm = w.items[kBig ? w.big[it] : it];
+    if (itm.tile < 0 || (!kBig && itm.slab == -2)) continue;  // uniform
     const int end = static_cast<int>(
         min(static_cast<long long>(itm.end), w.seg_capacity));
-    if (itm.slab >= 0) {
+    if (!kBig && itm.slab >= 0) {
       lds_zero(s_cnt4);
       __syncthreads();
-      tile_accumulate(w, itm.beg, end, s_cnt);
-      __syncthreads();
+      tile_accumulate(w, itm.beg, end, s_cnt, s_rs);
       uint4 *slab = reinterpret_cast<uint4 *>(
           w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
       for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
@@ -1059,31 +1144,45 @@ __global__ void __launch_bounds__(kApplyThreads, 4)
       __syncthreads();
       continue;
     }
-    unsigned op[kPerThread], hit[kPerThread];
-#pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-      op[j] = 0;
-      hit[j] = 0;
-    }
-    for (int p0 = itm.beg; p0 < end; p0 += kPass) {
-      lds_zero(s_cnt4);
-      __syncthreads();
-      tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt);
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < kPerThread; ++j) {
-        const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
-        op[j] += c & 0xFFFFu;
-        hit[j] += c >> 16;
-      }
-      __syncthreads();
-    }
-    const GridDesc &d = grid_of_tile(b, itm.tile);
+    const GridDesc d = grid_of_tile(b, itm.tile);
     const int tile = itm.tile - d.tile_base;
     const int tz = tile % d.g.ntz;
     const int ty = (tile / d.g.ntz) % d.g.nty;
     const int tx = tile / (d.g.ntz * d.g.nty);
-    fold_owned(d, tx * TX, ty * TY, tz * TZ, op, hit);
+    if (!kBig) {
+      // One pass: the packed 16-bit counts fold as they are.
+      lds_zero(s_cnt4);
+      __syncthreads();
+      tile_accumulate(w, itm.beg, end, s_cnt, s_rs);
+      unsigned c[kPerThread];
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j) c[j] = s_cnt[j * kApplyThreads + threadIdx.x];
+      __syncthreads();
+      fold_owned(d, tx * TX, ty * TY, tz * TZ,
+                 [&](int j) { return c[j] & 0xFFFFu; },
+                 [&](int j) { return c[j] >> 16; });
+    } else {
+      unsigned op[kPerThread], hit[kPerThread];
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j) {
+        op[j] = 0;
+        hit[j] = 0;
+      }
+      for (int p0 = itm.beg; p0 < end; p0 += kPass) {
+        lds_zero(s_cnt4);
+        __syncthreads();
+        tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt, s_rs);
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+          const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
+          op[j] += c & 0xFFFFu;
+          hit[j] += c >> 16;
+        }
+        __syncthreads();
+      }
+      fold_owned(d, tx * TX, ty * TY, tz * TZ, [&](int j) { return op[j]; },
+                 [&](int j) { return hit[j]; });
+    }
   }
 }
 
@@ -1124,27 +1223,28 @@ __global__ void __launch_bounds__(256)
         hit[e] += vv[e] >> 16;
       }
     }
-    const GridDesc &d = grid_of_tile(b, ht.tile);
+    const GridDesc d = grid_of_tile(b, ht.tile);
     const RayArgs &a = d.g.a;
     const int tile = ht.tile - d.tile_base;
     const int tz = tile % d.g.ntz;
     const int ty = (tile / d.g.ntz) % d.g.nty;
     const int tx = tile / (d.g.ntz * d.g.nty);
     const int h0 = tile_voxel_hash(a, tx * TX, ty * TY, tz * TZ, 4 * k4);
+    CUBLOX_GLOBAL float *occ = as_global(d.occ);
     float v[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      v[e] = (op[e] != 0 && !d.export_op) ? d.occ[h0 + e] : 0.0f;
+      v[e] = (op[e] != 0 && !d.export_op) ? occ[h0 + e] : 0.0f;
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       if (op[e] == 0) continue;
       if (d.export_op) {
-        d.export_op[h0 + e] = static_cast<int>(op[e]);
-        d.export_hit[h0 + e] = static_cast<int>(hit[e]);
+        as_global(d.export_op)[h0 + e] = static_cast<int>(op[e]);
+        as_global(d.export_hit)[h0 + e] = static_cast<int>(hit[e]);
       } else {
-        d.occ[h0 + e] = apply_logodds(v[e], static_cast<int>(op[e]),
-                                      static_cast<int>(hit[e]), d.g.p);
+        occ[h0 + e] = apply_logodds(v[e], static_cast<int>(op[e]),
+                                    static_cast<int>(hit[e]), d.g.p);
       }
     }
   }
@@ -1163,7 +1263,7 @@ __global__ void __launch_bounds__(256)
     const int hit = w.near_hit[cell];
     w.near_op[cell] = 0;
     w.near_hit[cell] = 0;
-    const GridDesc &d = grid_of_near(b, cell);
+    const GridDesc d = grid_of_near(b, cell);
     const IntegrateArgs &g = d.g;
     const RayArgs &a = g.a;
     const bool ctr = a.center != 0;
@@ -1178,10 +1278,11 @@ __global__ void __launch_bounds__(256)
                                  pos_to_global(a.oz, a.inv_res, ctr) + dz,
                                  a.mx, a.my, a.mz, a.hx, a.hy, a.hz);
     if (d.export_op) {
-      d.export_op[h] = o;
-      d.export_hit[h] = hit;
+      as_global(d.export_op)[h] = o;
+      as_global(d.export_hit)[h] = hit;
     } else {
-      d.occ[h] = apply_logodds(d.occ[h], o, hit, g.p);
+      CUBLOX_GLOBAL float *occ = as_global(d.occ);
+      occ[h] = apply_logodds(occ[h], o, hit, g.p);
     }
   }
 }
@@ -1248,7 +1349,7 @@ struct Layout {
   long long slabs;
   int max_items, scan_blocks;
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
-      off_items, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
+      off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
       off_bbase, off_hdr, off_desc, total;
 };
 
@@ -1276,6 +1377,7 @@ Layout make_layout(const Caps &c) {
   take(L.off_tcount, static_cast<size_t>(c.max_tiles) * 4);
   take(L.off_tcur, static_cast<size_t>(c.max_tiles) * 4);
   take(L.off_items, static_cast<size_t>(L.max_items) * sizeof(Item));
+  take(L.off_big, static_cast<size_t>(c.max_tiles) * sizeof(int));
   take(L.off_hot, static_cast<size_t>(L.slabs) * sizeof(HotTile));
   take(L.off_scratch, static_cast<size_t>(L.slabs) * kTileVox * 4);
   take(L.off_misc, 64 * 4);
@@ -1301,6 +1403,7 @@ Workspace bind(const Layout &L, void *base) {
   w.tile_count = reinterpret_cast<int *>(b + L.off_tcount);
   w.tile_cursor = reinterpret_cast<int *>(b + L.off_tcur);
   w.items = reinterpret_cast<Item *>(b + L.off_items);
+  w.big = reinterpret_cast<int *>(b + L.off_big);
   w.hot = reinterpret_cast<HotTile *>(b + L.off_hot);
   w.scratch = reinterpret_cast<unsigned *>(b + L.off_scratch);
   w.misc = reinterpret_cast<int *>(b + L.off_misc);
@@ -1416,8 +1519,11 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
   const int apply_blocks = std::min(L.max_items, 1 << 16);
-  tile_apply_kernel<<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  tile_apply_kernel<false><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
   st = check_launch("tile_apply_kernel");
+  if (st != CUBLOX_OK) return st;
+  tile_apply_kernel<true><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  st = check_launch("tile_apply_kernel<big>");
   if (st != CUBLOX_OK) return st;
   const long long hot_blocks =
       std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
@@ -1442,6 +1548,21 @@ Caps batch_caps(const cublox_batch_limits &lim) {
   c.chunk = std::max(kMinChunk, std::min(lim.chunk > 0 ? lim.chunk : kChunk, 8192));
   c.max_slabs = lim.max_slabs > 0 ? lim.max_slabs : 16384;
   return c;
+}
+
+} // namespace cublox_hip
+
+namespace cublox_hip {
+
+int read_counters(const Layout &L, const void *d_ws, size_t ws_bytes, int *out,
+                  hipStream_t s) {
+  if (!d_ws || ws_bytes < L.total) {
+    return set_error(CUBLOX_ERR_CAPACITY, "integrate_counters: workspace too small");
+  }
+  CUBLOX_TRY(hipMemcpyAsync(out, static_cast<const char *>(d_ws) + L.off_misc,
+                            5 * sizeof(int), hipMemcpyDeviceToHost, s));
+  CUBLOX_TRY(hipStreamSynchronize(s));
+  return CUBLOX_OK;
 }
 
 } // namespace cublox_hip
@@ -1604,6 +1725,26 @@ int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,
   int st = cublox_integrate_batch_stage(grids, n_grids, lim, 0, h_staging);
   if (st != CUBLOX_OK) return st;
   return cublox_launch_integrate_batch_staged(h_staging, d_ws, ws_bytes, lim, s);
+}
+
+int cublox_integrate_batch_counters(const void *d_ws, size_t ws_bytes,
+                                    const cublox_batch_limits *lim, int *out,
+                                    cublox_stream s) {
+  if (!lim || !out || lim->max_tiles > kMaxBatchTiles) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_batch_counters: bad arguments");
+  }
+  return read_counters(make_layout(batch_caps(*lim)), d_ws, ws_bytes, out,
+                       as_stream(s));
+}
+
+int cublox_integrate_counters(const void *d_ws, size_t ws_bytes,
+                              const cublox_raycast_cfg *cfg, int max_points,
+                              int near_steps, int *out, cublox_stream s) {
+  if (!cfg || !out || near_steps < 1 || near_steps > kMaxNearK) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_counters: bad arguments");
+  }
+  return read_counters(make_layout(single_caps(*cfg, max_points, near_steps)),
+                       d_ws, ws_bytes, out, as_stream(s));
 }
 
 } // extern "C"

@@ -191,7 +191,7 @@ struct ZeroArgs {
 };
 
 template <typename T>
-__device__ __forceinline__ void zero_contiguous(T *__restrict__ p,
+__device__ __forceinline__ void zero_contiguous(CUBLOX_GLOBAL T *__restrict__ p,
                                                long long count, long long t,
                                                long long stride) {
   // Scalar head up to 16 B alignment, then 16 B stores, then scalar tail.
@@ -200,8 +200,9 @@ __device__ __forceinline__ void zero_contiguous(T *__restrict__ p,
   if (head > count) head = count;
   const long long n4 = (count - head) / 4;
   const long long tail0 = head + n4 * 4;
-  uint4 *__restrict__ p4 = reinterpret_cast<uint4 *>(p + head);
-  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  CUBLOX_GLOBAL u32x4 *__restrict__ p4 = (CUBLOX_GLOBAL u32x4 *)(p + head);
+  const u32x4 z = {0u, 0u, 0u, 0u};
   for (long long i = t; i < n4; i += stride) p4[i] = z;
   if (t < head) p[t] = T(0);
   if (t < count - tail0) p[tail0 + t] = T(0);
@@ -213,7 +214,7 @@ __device__ __forceinline__ void zero_contiguous(T *__restrict__ p,
 // 256 / run rows into a workgroup, one element per lane (one division per
 // lane, none per element).
 template <typename T>
-__device__ __forceinline__ void zero_region(T *__restrict__ p,
+__device__ __forceinline__ void zero_region(CUBLOX_GLOBAL T *__restrict__ p,
                                            const ZeroRegion &r, int blk,
                                            int nblk) {
   if (r.n_outer == 1) {
@@ -243,9 +244,9 @@ __global__ void __launch_bounds__(256)
     zero_regions_kernel(float *__restrict__ occ, int *__restrict__ op_cnt,
                         int *__restrict__ hit_cnt, const ZeroArgs z) {
   const ZeroRegion r = z.r[blockIdx.y];
-  zero_region(occ, r, blockIdx.x, gridDim.x);
-  if (op_cnt) zero_region(op_cnt, r, blockIdx.x, gridDim.x);
-  if (hit_cnt) zero_region(hit_cnt, r, blockIdx.x, gridDim.x);
+  zero_region(as_global(occ), r, blockIdx.x, gridDim.x);
+  if (op_cnt) zero_region(as_global(op_cnt), r, blockIdx.x, gridDim.x);
+  if (hit_cnt) zero_region(as_global(hit_cnt), r, blockIdx.x, gridDim.x);
 }
 
 // Batched recenter: regions of many grids, each with its occupancy base.
@@ -262,9 +263,9 @@ struct RecenterBatchHeader {
 __global__ void __launch_bounds__(256)
     zero_regions_batch_kernel(const RecenterBatchHeader *__restrict__ hdr,
                               const ZeroRegionP *__restrict__ regions) {
-  if (static_cast<int>(blockIdx.y) >= hdr->n_regions) return;
-  const ZeroRegionP q = regions[blockIdx.y];
-  zero_region(q.occ, q.r, blockIdx.x, gridDim.x);
+  if (static_cast<int>(blockIdx.y) >= load_const(hdr).n_regions) return;
+  const ZeroRegionP q = load_const(regions + blockIdx.y);
+  zero_region(as_global(q.occ), q.r, blockIdx.x, gridDim.x);
 }
 
 // Reference-shaped slab clear from a DEVICE slice list: one block row per
@@ -298,9 +299,9 @@ __global__ void __launch_bounds__(256)
     return;
   }
   const ZeroRegion r{base, n_outer, stride, run};
-  zero_region(occ, r, blockIdx.x, gridDim.x);
-  if (op_cnt) zero_region(op_cnt, r, blockIdx.x, gridDim.x);
-  if (hit_cnt) zero_region(hit_cnt, r, blockIdx.x, gridDim.x);
+  zero_region(as_global(occ), r, blockIdx.x, gridDim.x);
+  if (op_cnt) zero_region(as_global(op_cnt), r, blockIdx.x, gridDim.x);
+  if (hit_cnt) zero_region(as_global(hit_cnt), r, blockIdx.x, gridDim.x);
 }
 
 // ===================================================================

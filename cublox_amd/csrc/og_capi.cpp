@@ -374,6 +374,13 @@ int cublox_gb_last(const cublox_gb *b, double out[3]) {
   });
 }
 
+int cublox_gb_counters(const cublox_gb *b, int out[5]) {
+  return guarded([&] {
+    const auto c = GBC->lastCounters();
+    for (int k = 0; k < 5; ++k) out[k] = c[static_cast<size_t>(k)];
+  });
+}
+
 const float *cublox_og_device_occ(const cublox_og *h) {
   return OGC->deviceOccupancy();
 }

@@ -82,12 +82,9 @@ struct RayState {
 
 // Loads point i (transforming it when kTransform) and sets up its walk.
 // Returns false when the ray is skipped (non-finite, zero length).
-template <bool kTransform>
-__device__ __forceinline__ bool ray_setup(const RayArgs &a,
-                                          const float *__restrict__ xs,
-                                          const float *__restrict__ ys,
-                                          const float *__restrict__ zs, int i,
-                                          RayState &r) {
+template <bool kTransform, typename P>
+__device__ __forceinline__ bool ray_setup(const RayArgs &a, P xs, P ys, P zs,
+                                          int i, RayState &r) {
   float ex = xs[i], ey = ys[i], ez = zs[i];
   if (kTransform) {
     // Scan transform (driver main.cpp:672-682): drop non-finite sensor-frame

@@ -92,3 +92,20 @@ def log_odds():
     p = PROBABILITIES
     return (logit(p["hit"]), logit(p["miss"]), logit(p["min"]), logit(p["max"]),
             logit(p["free"]), logit(p["occupied"]))
+
+
+def batched_grid(w: Workload, grid_id: int):
+    """Per-grid draw of the batched workload: (beams, azimuths, max range,
+    scene) from a generator seeded with the grid's seed."""
+    import numpy as np
+    from .scene import Scene
+    rng = np.random.default_rng(w.seed(grid_id))
+    lo_b, hi_b = w.extra["beams_range"]
+    lo_a, hi_a = w.extra["az_range"]
+    lo_r, hi_r = w.extra["range_range"]
+    beams = int(rng.integers(lo_b, hi_b + 1))
+    az = int(rng.integers(lo_a, hi_a + 1))
+    rmax = float(np.float32(rng.uniform(lo_r, hi_r)))
+    scene = Scene(seed=w.seed(grid_id), n_boxes=w.n_boxes, box_extent=w.box_extent,
+                  speed=w.speed, yaw_rate_deg=w.yaw_rate_deg)
+    return beams, az, rmax, scene

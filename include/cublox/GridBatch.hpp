@@ -86,6 +86,9 @@ public:
   double lastUpdateMs() const { return last_update_ms_; }
   double lastRecenterMs() const { return last_recenter_ms_; }
   long long lastZeroedVoxels() const { return last_zeroed_; }
+  // Counters of the last tick's update: work items, tile records, overflow
+  // flag, split hot tiles, unsplit hot tiles.
+  std::array<int, 5> lastCounters() const;
   void *stream() const { return stream_; }
 
 private:

@@ -371,6 +371,21 @@ int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,
                                   void *h_staging, void *d_ws,
                                   size_t ws_bytes, cublox_stream s);
 
+/* Diagnostics (no reference counterpart): after the last update launched on
+ * stream s with this workspace, synchronises s and writes to out[0..4] the
+ * work items, the tile records (segments and endpoints), the overflow flag
+ * (sticky since cublox_integrate_workspace_init: nonzero once a ray needed
+ * more records than its slot bound), the split hot tiles and the unsplit
+ * hot tiles of that launch. The batch form takes
+ * the limits the workspace was sized with; the single-grid form the
+ * arguments of cublox_integrate_workspace_bytes.                         */
+int cublox_integrate_batch_counters(const void *d_ws, size_t ws_bytes,
+                                    const cublox_batch_limits *lim, int *out,
+                                    cublox_stream s);
+int cublox_integrate_counters(const void *d_ws, size_t ws_bytes,
+                              const cublox_raycast_cfg *cfg, int max_points,
+                              int near_steps, int *out, cublox_stream s);
+
 /* ------------------------------------------------------------------ */
 /* Counter census (no reference counterpart): over n voxels, adds to
  * d_out[0] the number of voxels with op != 0, to d_out[1] the sum of op, to

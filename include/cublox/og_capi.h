@@ -158,6 +158,8 @@ int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk);
 int cublox_gb_tune_results(const cublox_gb *b, double *out_ms, int cap);
 /* out: update ms, recenter ms (0 in graph mode: all in update), zeroed. */
 int cublox_gb_last(const cublox_gb *b, double out[3]);
+/* Counters of the last tick's update (cublox_integrate_batch_counters). */
+int cublox_gb_counters(const cublox_gb *b, int out[5]);
 
 /* Index math (reference include/cublox/utils.hpp:24-181). */
 int cublox_utils_pos_to_global(float p, float inv_res, int center);
