@@ -223,6 +223,109 @@ def run_batched(cb, tune_ticks_each, ticks):
     return out
 
 
+SERVED_SHAPES = ("small", "batched", "readme")  # 201x201x51, 401x401x101, 701x701x151
+
+
+def _lat(c):
+    d = c["done_ms"] - c["submit_ms"]
+    if len(d) == 0:
+        return None
+    return dict(n=int(len(d)), p50_ms=round(pct(d, 50), 3), p95_ms=round(pct(d, 95), 3),
+                p99_ms=round(pct(d, 99), 3))
+
+
+def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
+               tick_points=1 << 21):
+    """The map server workload: n_sessions sessions cycling over three grid
+    shapes, HBM-budget admission (200 GB), continuous batching. Saturation
+    is the drain rate of `depth` queued scans per session; then open-loop
+    Poisson scan arrivals at each load fraction of it (fetches at 20 % of the
+    scan rate, r = 1.5 m) for `seconds`, with latency = completion - submit
+    (host copy, H2D, batched update + recenter, fetch). The same sessions
+    served one scan per tick give the unbatched saturation."""
+    from cublox_amd.workloads import batched_grid
+    W = cb.WORKLOADS
+    t = time.perf_counter()
+    specs, templates = [], []
+    for k in range(n_sessions):
+        w = W[SERVED_SHAPES[k % len(SERVED_SHAPES)]]
+        if w.name == "batched":
+            beams, az, rmax, scene = batched_grid(w, k)
+        else:
+            beams, az, rmax = w.beams, w.azimuths, w.max_range
+            scene = cb.Scene(seed=w.seed(k), n_boxes=w.n_boxes, box_extent=w.box_extent,
+                             speed=w.speed, yaw_rate_deg=w.yaw_rate_deg)
+        s = scene.scan(0, beams, az, w.el_lo, w.el_hi, rmax)
+        specs.append((w.half, w.resolution, w.origin_at_center, w.threshold, rmax))
+        templates.append((s.xs, s.ys, s.zs, s.T, s.base))
+    pts = [t_[0].size for t_ in templates]
+    log(f"served: {n_sessions} sessions, {statistics.mean(pts):.0f} points/scan mean, "
+        f"templates in {time.perf_counter() - t:.1f} s")
+
+    def serve(max_grids):
+        srv = cb.MapServer(hbm_budget_bytes=200 << 30, max_sessions=n_sessions,
+                           max_points_per_scan=131072, max_points_per_tick=tick_points,
+                           max_grids_per_tick=max_grids, max_tiles_per_tick=1 << 21,
+                           max_segments_per_ray=72, graph=True)
+        sids = [srv.open_session(h, r, c, th, mr, cb.log_odds()) for (h, r, c, th, mr) in specs]
+        return srv, sids
+
+    def queued(srv, sids, k0):
+        """depth scans per session queued while paused, then drained."""
+        srv.pause()
+        for j in range(depth):
+            for sid, (xs, ys, zs, T, base) in zip(sids, templates):
+                Tj = np.array(T, np.float32)
+                Tj[0, 3] += 0.5 * (k0 + j)
+                b = np.array(base, np.float32)
+                b[0] += 0.5 * (k0 + j)
+                srv.submit_scan(sid, xs, ys, zs, Tj, b)
+        srv.resume()
+        srv.drain()
+        c = srv.completions()
+        return len(c) / ((c["done_ms"].max() - c["start_ms"].min()) * 1e-3)
+
+    srv, sids = serve(256)
+    t = time.perf_counter()
+    queued(srv, sids, 0)  # warm-up: first origins, graph capture
+    st0 = srv.stats()
+    sat = queued(srv, sids, depth)
+    st1 = srv.stats()
+    log(f"served: saturation {sat:.0f} scans/s over {st1['ticks'] - st0['ticks']} ticks "
+        f"({time.perf_counter() - t:.1f} s), resident {st1['resident_bytes'] / 2**30:.1f} GiB")
+    levels = []
+    for f in loads:
+        rate = f * sat
+        n_sub = srv.run_load(sids, templates, scan_rate_hz=rate, duration_s=seconds,
+                             fetch_ratio=0.2, fetch_radius=1.5, speed=0.5, seed=17)
+        srv.drain()
+        c = srv.completions()
+        scans = c[c["kind"] == 0]
+        span = (c["done_ms"].max() - c["submit_ms"].min()) * 1e-3
+        levels.append(dict(load=f, offered_scans_per_s=round(rate, 1),
+                           achieved_scans_per_s=round(len(scans) / span, 1),
+                           requests=int(n_sub), scan_latency=_lat(scans),
+                           fetch_latency=_lat(c[c["kind"] == 1])))
+        log(f"served: load {f}: {levels[-1]}")
+    st = srv.stats()
+    srv.close()
+    # Unbatched: the same sessions, one scan per tick.
+    srv1, sids1 = serve(1)
+    queued(srv1, sids1, 0)
+    plain_sat = queued(srv1, sids1, depth)
+    srv1.close()
+    log(f"served: unbatched saturation {plain_sat:.0f} scans/s")
+    return dict(sessions=n_sessions,
+                shapes=[f"{W[s].map_size[0]}x{W[s].map_size[1]}x{W[s].map_size[2]}"
+                        for s in SERVED_SHAPES],
+                points_per_scan_mean=round(statistics.mean(pts)),
+                hbm_budget_gib=200, resident_gib=round(st["resident_bytes"] / 2**30, 2),
+                tick_points=tick_points, saturation_scans_per_s=round(sat, 1),
+                unbatched_saturation_scans_per_s=round(plain_sat, 1),
+                batching_speedup=round(sat / plain_sat, 2), levels=levels,
+                seconds_per_level=seconds)
+
+
 def run_fetch(cb, w, g, scene, k, reps=5):
     if not w.fetch_queries:
         return None

@@ -9,6 +9,7 @@ from . import _ffi
 from .batch import GridBatch
 from .device import DeviceBuffer, Event, Stream, device_count, device_name, synchronize
 from .grid import FREE, OCCUPIED, UNKNOWN, OccupancyGrid, logit
+from .server import MapServer
 from .scene import Scan, Scene
 from .workloads import ALL as WORKLOADS
 from .workloads import log_odds
@@ -19,7 +20,7 @@ _ffi.host()
 _ffi.gen()
 
 __all__ = [
-    "OccupancyGrid", "GridBatch", "Scene", "Scan", "DeviceBuffer", "Stream", "Event",
+    "OccupancyGrid", "GridBatch", "MapServer", "Scene", "Scan", "DeviceBuffer", "Stream", "Event",
     "device_count", "device_name", "synchronize", "logit", "log_odds",
     "WORKLOADS", "UNKNOWN", "FREE", "OCCUPIED",
 ]

@@ -236,6 +236,29 @@ def host() -> C.CDLL:
     _sig(L, "cublox_gb_tune_results", i, vp, C.POINTER(C.c_double), i)
     _sig(L, "cublox_gb_last", i, vp, C.POINTER(C.c_double))
     _sig(L, "cublox_gb_counters", i, vp, C.POINTER(C.c_int))
+    ll = C.c_longlong
+    llp = C.POINTER(ll)
+    dp = C.POINTER(C.c_double)
+    _sig(L, "cublox_ms_create", vp, llp)
+    _sig(L, "cublox_ms_destroy", None, vp)
+    _sig(L, "cublox_ms_session_bytes", ll, i32p)
+    _sig(L, "cublox_ms_open", i, vp, i32p, F, i, i, C.c_double, F, f32p, i32p)
+    _sig(L, "cublox_ms_close", i, vp, i)
+    _sig(L, "cublox_ms_admitted", i, vp, i, i32p)
+    _sig(L, "cublox_ms_submit_scan", i, vp, i, vp, vp, vp, i, f32p, f32p, llp)
+    _sig(L, "cublox_ms_submit_fetch", i, vp, i, f32p, F, i, i, llp)
+    _sig(L, "cublox_ms_drain", i, vp)
+    _sig(L, "cublox_ms_pause", i, vp)
+    _sig(L, "cublox_ms_resume", i, vp)
+    _sig(L, "cublox_ms_take_completions", i, vp, llp, i32p, i32p, dp, llp, llp,
+         i, i32p)
+    _sig(L, "cublox_ms_keep_fetch_results", i, vp, i)
+    _sig(L, "cublox_ms_fetch_result", i, vp, ll, f32p, u8p, i, i32p)
+    _sig(L, "cublox_ms_copy_occupancy", i, vp, i, f32p)
+    _sig(L, "cublox_ms_origin", i, vp, i, i32p)
+    _sig(L, "cublox_ms_stats", i, vp, dp)
+    _sig(L, "cublox_ms_load_run", i, vp, i32p, i, vp, vp, vp, i32p, f32p, f32p, vp,
+         llp)
     _sig(L, "cublox_utils_pos_to_global", i, F, F, i)
     _sig(L, "cublox_utils_global_to_pos", F, i, F, i)
     _sig(L, "cublox_utils_global_to_local", i, i, i, i)

@@ -282,10 +282,7 @@ void GridBatch::finish() {
   last_update_ms_ = upd;
   last_recenter_ms_ = graph_mode_ ? 0.0 : static_cast<double>(total - upd);
   last_zeroed_ = 0;
-  for (size_t k = 0; k < rc_grids_.size(); ++k) {
-    rc_grids_[k]->updateOriginAndBound(rc_origins_[k]);
-    last_zeroed_ += rc_zeroed_[k];
-  }
+  for (size_t k = 0; k < rc_grids_.size(); ++k) last_zeroed_ += rc_zeroed_[k];
   if (tuning()) {
     // One warm-up tick per candidate, then ticks_each measured ones.
     if (tune_count_ > 0) {
@@ -308,16 +305,36 @@ void GridBatch::finish() {
   }
 }
 
-void GridBatch::tick(
+void GridBatch::submit(
     const std::vector<GridScan> &scans,
     const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
+  if (in_flight_) {
+    throw std::logic_error("cublox::GridBatch: submit() before wait() of the last tick");
+  }
   if (static_cast<int>(scans.size()) > lim_.max_grids) {
     throw std::invalid_argument("cublox::GridBatch: more scans than max_grids");
   }
   if (tuning()) setting_ = tune_[tune_pos_];
   stage(scans, recenters);
   launch();
+  in_flight_ = true;
+  // The grids' host state describes them after all work enqueued so far.
+  for (size_t k = 0; k < rc_grids_.size(); ++k) {
+    rc_grids_[k]->updateOriginAndBound(rc_origins_[k]);
+  }
+}
+
+void GridBatch::wait() {
+  if (!in_flight_) return;
+  in_flight_ = false;
   finish();
+}
+
+void GridBatch::tick(
+    const std::vector<GridScan> &scans,
+    const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
+  submit(scans, recenters);
+  wait();
 }
 
 }  // namespace cublox

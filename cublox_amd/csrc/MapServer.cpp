@@ -1,0 +1,603 @@
+// Occupancy map server: sessions, HBM-budget admission, continuous batching
+// of scans over GridBatch, fetches in submission order.
+#include <cublox/MapServer.hpp>
+#include <cublox/capi.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace cublox {
+
+namespace {
+
+void statusCheck(int status, const char *expr) {
+  if (status == CUBLOX_OK) return;
+  throw std::runtime_error(std::string("cublox::MapServer: `") + expr +
+                           "` failed: " + cublox_last_error());
+}
+#define MS_CHECK(call) statusCheck((call), #call)
+
+// Fetch output reserve of a session: the reference's default cap of 500000
+// samples at 12 B position + 1 B state, plus the count word.
+constexpr std::size_t kFetchReserve = 500000ull * 13ull + 256ull;
+
+cublox_raycast_cfg shapeCfg(const SessionSpec &s) {
+  cublox_raycast_cfg c{};
+  c.map_size_i = cublox_int3{2 * s.half_map_size_i.x() + 1, 2 * s.half_map_size_i.y() + 1,
+                             2 * s.half_map_size_i.z() + 1};
+  c.half_map_size_i =
+      cublox_int3{s.half_map_size_i.x(), s.half_map_size_i.y(), s.half_map_size_i.z()};
+  c.resolution = s.resolution;
+  c.inv_resolution = 1.0f / s.resolution;
+  c.max_range = s.max_range;
+  c.map_vox_num = c.map_size_i.x * c.map_size_i.y * c.map_size_i.z;
+  c.origin_at_center = s.origin_at_center ? 1 : 0;
+  return c;
+}
+
+}  // namespace
+
+std::size_t MapServer::sessionBytes(const SessionSpec &s) {
+  const std::size_t v = static_cast<std::size_t>(2 * s.half_map_size_i.x() + 1) *
+                        static_cast<std::size_t>(2 * s.half_map_size_i.y() + 1) *
+                        static_cast<std::size_t>(2 * s.half_map_size_i.z() + 1);
+  return v * sizeof(float) + kFetchReserve;
+}
+
+MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
+  if (lim_.max_sessions < 1 || lim_.max_points_per_scan < 1 ||
+      lim_.max_points_per_tick < lim_.max_points_per_scan || lim_.max_grids_per_tick < 1 ||
+      lim_.max_tiles_per_tick < 1 || lim_.max_segments_per_ray < 1) {
+    throw std::invalid_argument("cublox::MapServer: invalid limits");
+  }
+  BatchLimits bl;
+  bl.max_grids = lim_.max_grids_per_tick;
+  bl.max_points = lim_.max_points_per_tick;
+  bl.max_tiles = lim_.max_tiles_per_tick;
+  bl.max_segments_per_ray = lim_.max_segments_per_ray;
+  bl.max_near_steps = 12;
+  bl.min_chunk = 512;
+  batch_ = std::make_unique<GridBatch>(bl);
+  batch_->setGraphMode(lim_.graph);
+  const std::size_t pts_bytes = 3ull * sizeof(float) * lim_.max_points_per_tick;
+  try {
+    MS_CHECK(cublox_stream_create(&copy_stream_));
+    for (int b = 0; b < 2; ++b) {
+      MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_pts_[b]), pts_bytes));
+      MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_pts_[b]), pts_bytes));
+    }
+  } catch (...) {
+    for (int b = 0; b < 2; ++b) {
+      if (h_pts_[b]) cublox_host_free(h_pts_[b]);
+      if (d_pts_[b]) cublox_free(d_pts_[b]);
+    }
+    if (copy_stream_) cublox_stream_destroy(copy_stream_);
+    throw;
+  }
+  stats_.workspace_bytes = 2 * pts_bytes;
+  t0_ = std::chrono::steady_clock::now();
+  worker_ = std::thread([this] { run(); });
+}
+
+MapServer::~MapServer() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  if (worker_.joinable()) worker_.join();
+  batch_.reset();
+  for (int b = 0; b < 2; ++b) {
+    if (h_pts_[b]) cublox_host_free(h_pts_[b]);
+    if (d_pts_[b]) cublox_free(d_pts_[b]);
+  }
+  growFetchArena(0, 0);  // frees
+  if (copy_stream_) cublox_stream_destroy(copy_stream_);
+  sessions_.clear();  // frees the grids
+}
+
+double MapServer::nowMs() const {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_)
+      .count();
+}
+
+int MapServer::openSession(const SessionSpec &spec) {
+  const cublox_raycast_cfg c = shapeCfg(spec);
+  Session s;
+  s.spec = spec;
+  s.bytes = sessionBytes(spec);
+  s.tiles = cublox_integrate_tile_count(&c);
+  s.seg_bound = cublox_integrate_segment_bound(&c);
+  if (spec.resolution <= 0.0f || spec.half_map_size_i.minCoeff() < 0) {
+    throw std::invalid_argument("cublox::MapServer: invalid session shape");
+  }
+  if (s.seg_bound > lim_.max_segments_per_ray || s.tiles > lim_.max_tiles_per_tick ||
+      s.bytes > lim_.hbm_budget_bytes) {
+    throw std::invalid_argument(
+        "cublox::MapServer: session shape above the server limits (segments per ray, "
+        "tiles per tick or HBM budget)");
+  }
+  std::lock_guard<std::mutex> lk(mu_);
+  if (worker_error_) std::rethrow_exception(worker_error_);
+  const int sid = next_sid_++;
+  sessions_.emplace(sid, std::move(s));
+  waiting_.push_back(sid);
+  ++gen_;
+  cv_work_.notify_all();
+  return sid;
+}
+
+void MapServer::closeSession(int sid) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sessions_.find(sid);
+  if (it == sessions_.end()) throw std::invalid_argument("cublox::MapServer: no such session");
+  it->second.closing = true;
+  ++gen_;
+  cv_work_.notify_all();
+}
+
+bool MapServer::admitted(int sid) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sessions_.find(sid);
+  return it != sessions_.end() && it->second.admitted;
+}
+
+long long MapServer::submitScan(int sid, const float *xs, const float *ys, const float *zs,
+                                int n, const float T[12], const Eigen::Vector3f &base) {
+  if (n < 0 || n > lim_.max_points_per_scan) {
+    throw std::invalid_argument("cublox::MapServer: scan size outside [0, max_points_per_scan]");
+  }
+  Request r;
+  r.kind = Kind::SCAN;
+  r.n = n;
+  r.pts.resize(3 * static_cast<std::size_t>(n));
+  std::memcpy(r.pts.data(), xs, sizeof(float) * n);
+  std::memcpy(r.pts.data() + n, ys, sizeof(float) * n);
+  std::memcpy(r.pts.data() + 2 * static_cast<std::size_t>(n), zs, sizeof(float) * n);
+  std::copy(T, T + 12, r.T.begin());
+  r.pos = base;
+  std::lock_guard<std::mutex> lk(mu_);
+  if (worker_error_) std::rethrow_exception(worker_error_);
+  auto it = sessions_.find(sid);
+  if (it == sessions_.end() || it->second.closing) {
+    throw std::invalid_argument("cublox::MapServer: scan for a closed or unknown session");
+  }
+  r.id = next_id_++;
+  r.submit_ms = nowMs();
+  it->second.queue.push_back(std::move(r));
+  ++gen_;
+  cv_work_.notify_all();
+  return next_id_ - 1;
+}
+
+long long MapServer::submitFetch(int sid, const Eigen::Vector3f &center, float radius,
+                                 std::optional<OccupancyGrid::VoxelState> filter,
+                                 int max_results) {
+  Request r;
+  r.kind = Kind::FETCH;
+  r.pos = center;
+  r.radius = radius;
+  r.filter = filter;
+  r.max_results = max_results;
+  std::lock_guard<std::mutex> lk(mu_);
+  if (worker_error_) std::rethrow_exception(worker_error_);
+  auto it = sessions_.find(sid);
+  if (it == sessions_.end() || it->second.closing) {
+    throw std::invalid_argument("cublox::MapServer: fetch for a closed or unknown session");
+  }
+  r.id = next_id_++;
+  r.submit_ms = nowMs();
+  it->second.queue.push_back(std::move(r));
+  ++gen_;
+  cv_work_.notify_all();
+  return next_id_ - 1;
+}
+
+bool MapServer::quiescent() const {
+  for (const auto &[sid, s] : sessions_) {
+    if (s.busy > 0) return false;
+    if (s.admitted && !s.queue.empty()) return false;
+    if (s.closing && s.admitted) return false;  // not yet retired
+  }
+  for (int sid : waiting_) {
+    const Session &s = sessions_.at(sid);
+    if (admittable(s)) return false;
+  }
+  return true;
+}
+
+void MapServer::pause() {
+  std::lock_guard<std::mutex> lk(mu_);
+  paused_ = true;
+}
+
+void MapServer::resume() {
+  std::lock_guard<std::mutex> lk(mu_);
+  paused_ = false;
+  ++gen_;
+  cv_work_.notify_all();
+}
+
+void MapServer::drain() {
+  std::unique_lock<std::mutex> lk(mu_);
+  if (paused_) throw std::logic_error("cublox::MapServer: drain() while paused");
+  ++gen_;
+  cv_work_.notify_all();
+  cv_idle_.wait(lk, [&] { return worker_error_ || (quiescent() && !worker_busy_); });
+  if (worker_error_) std::rethrow_exception(worker_error_);
+}
+
+std::vector<MapServer::Completion> MapServer::takeCompletions() {
+  std::lock_guard<std::mutex> lk(mu_);
+  std::vector<Completion> out;
+  out.swap(done_);
+  return out;
+}
+
+void MapServer::keepFetchResults(bool on) {
+  std::lock_guard<std::mutex> lk(mu_);
+  keep_fetch_ = on;
+}
+
+std::vector<OccupancyGrid::OccupancySample> MapServer::takeFetchResult(long long id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = fetch_results_.find(id);
+  if (it == fetch_results_.end()) return {};
+  auto v = std::move(it->second);
+  fetch_results_.erase(it);
+  return v;
+}
+
+const OccupancyGrid &MapServer::grid(int sid) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sessions_.find(sid);
+  if (it == sessions_.end() || !it->second.grid) {
+    throw std::invalid_argument("cublox::MapServer: session has no resident grid");
+  }
+  return *it->second.grid;
+}
+
+MapServer::Stats MapServer::stats() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  Stats s = stats_;
+  s.resident_bytes = resident_;
+  s.admitted = admitted_count_;
+  s.waiting = static_cast<int>(waiting_.size());
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// worker
+// ---------------------------------------------------------------------------
+
+bool MapServer::admittable(const Session &s) const {
+  return admitted_count_ < lim_.max_sessions && resident_ + s.bytes <= lim_.hbm_budget_bytes;
+}
+
+void MapServer::retireClosed() {
+  for (auto it = sessions_.begin(); it != sessions_.end();) {
+    Session &s = it->second;
+    if (s.closing && s.busy == 0 && (s.queue.empty() || !s.admitted)) {
+      if (s.admitted) {
+        resident_ -= s.bytes;
+        --admitted_count_;
+      } else {
+        waiting_.erase(std::remove(waiting_.begin(), waiting_.end(), it->first),
+                       waiting_.end());
+      }
+      it = sessions_.erase(it);  // frees the grid
+    } else {
+      ++it;
+    }
+  }
+}
+
+void MapServer::admitWaiting() {
+  while (!waiting_.empty()) {
+    Session &s = sessions_.at(waiting_.front());
+    if (!admittable(s)) break;  // FIFO: later sessions wait behind it
+    const SessionSpec &p = s.spec;
+    s.grid = std::make_unique<OccupancyGrid>(p.half_map_size_i, p.resolution,
+                                             p.origin_at_center, p.recenter_threshold);
+    s.grid->setMaxRaycastRange(p.max_range);
+    s.grid->setLogOddsParams(p.log_odds[0], p.log_odds[1], p.log_odds[2], p.log_odds[3],
+                             p.log_odds[4], p.log_odds[5]);
+    s.admitted = true;
+    resident_ += s.bytes;
+    ++admitted_count_;
+    waiting_.pop_front();
+  }
+}
+
+// Takes the tick's requests: per admitted session (round-robin from rr_),
+// a leading run of fetches, or one scan and the fetches right after it,
+// while the scan fits the tick's caps.
+bool MapServer::formTick(Tick &t) {
+  long long points = 0, slots = 0;
+  int tiles = 0, grids = 0;
+  const long long slot_cap =
+      static_cast<long long>(lim_.max_points_per_tick) * lim_.max_segments_per_ray;
+  std::vector<int> ids;
+  ids.reserve(sessions_.size());
+  for (const auto &kv : sessions_) ids.push_back(kv.first);
+  if (ids.empty()) return false;
+  const std::size_t start =
+      static_cast<std::size_t>(std::lower_bound(ids.begin(), ids.end(), rr_) - ids.begin());
+  for (std::size_t k = 0; k < ids.size(); ++k) {
+    const int sid = ids[(start + k) % ids.size()];
+    Session &s = sessions_.at(sid);
+    if (!s.admitted || s.queue.empty()) continue;
+    if (s.queue.front().kind == Kind::SCAN) {
+      const Request &r = s.queue.front();
+      const long long need = static_cast<long long>(r.n) * s.seg_bound;
+      if (grids + 1 > lim_.max_grids_per_tick || points + r.n > lim_.max_points_per_tick ||
+          tiles + s.tiles > lim_.max_tiles_per_tick || slots + need > slot_cap) {
+        continue;
+      }
+      grids += 1;
+      points += r.n;
+      tiles += s.tiles;
+      slots += need;
+      t.scans.emplace_back(sid, std::move(s.queue.front()));
+      s.queue.pop_front();
+      ++s.busy;
+    }
+    while (!s.queue.empty() && s.queue.front().kind == Kind::FETCH) {
+      t.fetches.emplace_back(sid, std::move(s.queue.front()));
+      s.queue.pop_front();
+      ++s.busy;
+    }
+  }
+  if (t.scans.empty() && t.fetches.empty()) return false;
+  rr_ = ids[(start + 1) % ids.size()];
+  return true;
+}
+
+void MapServer::upload(Tick &t) {
+  float *h = h_pts_[t.buf];
+  std::size_t off = 0;
+  for (auto &[sid, r] : t.scans) {
+    std::memcpy(h + off, r.pts.data(), sizeof(float) * r.pts.size());
+    off += r.pts.size();
+  }
+  if (off > 0) {
+    MS_CHECK(cublox_memcpy_h2d_async(d_pts_[t.buf], h, sizeof(float) * off, copy_stream_));
+  }
+}
+
+void MapServer::submitTick(Tick &t) {
+  std::vector<GridScan> scans;
+  std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> rc;
+  std::vector<OccupancyGrid *> grids;
+  {
+    std::lock_guard<std::mutex> lk(mu_);  // a busy session's grid stays put
+    for (auto &[sid, r] : t.scans) grids.push_back(sessions_.at(sid).grid.get());
+  }
+  const float *d = d_pts_[t.buf];
+  std::size_t off = 0;
+  std::size_t k = 0;
+  for (auto &[sid, r] : t.scans) {
+    OccupancyGrid *g = grids[k++];
+    GridScan s;
+    s.grid = g;
+    s.d_x = d + off;
+    s.d_y = d + off + r.n;
+    s.d_z = d + off + 2 * static_cast<std::size_t>(r.n);
+    s.n = r.n;
+    s.sensor_frame = true;
+    s.T = r.T;
+    off += r.pts.size();
+    scans.push_back(s);
+    rc.emplace_back(g, r.pos);
+  }
+  batch_->submit(scans, rc);
+}
+
+// Grows (samples > 0) or frees (0, 0) the fetch output arena. Called
+// between ticks only, when no device work uses it.
+void MapServer::growFetchArena(std::size_t samples, std::size_t fetches) {
+  if (samples <= fcap_ && fetches <= fcnt_cap_ && samples > 0) return;
+  if (d_fpos_) cublox_free(d_fpos_);
+  if (d_fst_) cublox_free(d_fst_);
+  if (d_fcnt_) cublox_free(d_fcnt_);
+  if (h_fpos_) cublox_host_free(h_fpos_);
+  if (h_fst_) cublox_host_free(h_fst_);
+  if (h_fcnt_) cublox_host_free(h_fcnt_);
+  d_fpos_ = h_fpos_ = nullptr;
+  d_fst_ = h_fst_ = nullptr;
+  d_fcnt_ = h_fcnt_ = nullptr;
+  fcap_ = fcnt_cap_ = 0;
+  if (samples == 0 && fetches == 0) return;
+  const std::size_t s = std::max<std::size_t>(samples, 1 << 16) * 2;  // headroom
+  const std::size_t f = std::max<std::size_t>(fetches, 256) * 2;
+  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fpos_), 3 * sizeof(float) * s));
+  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fst_), s));
+  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fcnt_), sizeof(unsigned) * f));
+  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_fpos_), 3 * sizeof(float) * s));
+  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_fst_), s));
+  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_fcnt_), sizeof(unsigned) * f));
+  fcap_ = s;
+  fcnt_cap_ = f;
+}
+
+// Plans the tick's fetches against the grids as the tick leaves them (the
+// host plan of OccupancyGrid::fetchOccupancyAround) and enqueues them on the
+// batch stream after the tick's update and recenter, with their outputs
+// copied to pinned memory: no host wait per fetch.
+void MapServer::enqueueFetches(Tick &t) {
+  if (t.fetches.empty()) return;
+  std::vector<const OccupancyGrid *> grids;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &[sid, r] : t.fetches) grids.push_back(sessions_.at(sid).grid.get());
+  }
+  std::vector<cublox_fetch_cfg> cfgs(t.fetches.size());
+  t.jobs.assign(t.fetches.size(), FetchJob{});
+  std::size_t total = 0;
+  for (std::size_t k = 0; k < t.fetches.size(); ++k) {
+    const Request &r = t.fetches[k].second;
+    const OccupancyGrid &g = *grids[k];
+    FetchJob &j = t.jobs[k];
+    if (g.deviceOccupancy() == nullptr || r.radius <= 0.0f) continue;
+    long long est = 0;
+    unsigned output_cap = 0;
+    const auto &c = g.config_;
+    j.live = cublox_fetch_plan(
+                 cublox_int3{g.origin_i_.x(), g.origin_i_.y(), g.origin_i_.z()},
+                 cublox_int3{c.map_size_i.x(), c.map_size_i.y(), c.map_size_i.z()},
+                 cublox_int3{c.half_map_size_i.x(), c.half_map_size_i.y(), c.half_map_size_i.z()},
+                 c.resolution, c.resolution_inv, c.origin_at_center ? 1 : 0,
+                 g.getLogOddsFree(), g.getLogOddsOccupied(), 0.0f,
+                 cublox_float3{r.pos.x(), r.pos.y(), r.pos.z()}, r.radius,
+                 r.filter ? static_cast<int>(*r.filter) : -1, r.max_results, &cfgs[k], &est,
+                 &output_cap) != 0;
+    if (!j.live) continue;
+    j.cap = static_cast<unsigned>(std::min<long long>(output_cap, est));
+    j.limit = r.max_results > 0 ? std::min(static_cast<unsigned>(r.max_results), output_cap)
+                                : output_cap;
+    j.off = total;
+    total += j.cap;
+  }
+  growFetchArena(total, t.fetches.size());
+  void *s = batch_->stream();
+  MS_CHECK(cublox_fill_zero_async(d_fcnt_, t.fetches.size(), s));
+  for (std::size_t k = 0; k < t.fetches.size(); ++k) {
+    const FetchJob &j = t.jobs[k];
+    if (!j.live || j.cap == 0) continue;
+    MS_CHECK(cublox_launch_fetch_occupancy_around(&cfgs[k], grids[k]->deviceOccupancy(),
+                                                  d_fcnt_ + k, d_fpos_ + 3 * j.off,
+                                                  d_fst_ + j.off, j.cap, j.limit, s));
+  }
+  MS_CHECK(cublox_memcpy_d2h_async(h_fcnt_, d_fcnt_, sizeof(unsigned) * t.fetches.size(), s));
+  if (total > 0) {
+    MS_CHECK(cublox_memcpy_d2h_async(h_fpos_, d_fpos_, 3 * sizeof(float) * total, s));
+    MS_CHECK(cublox_memcpy_d2h_async(h_fst_, d_fst_, total, s));
+  }
+}
+
+void MapServer::completeTick(Tick &t) {
+  if (!t.scans.empty()) {
+    batch_->wait();
+  } else {
+    MS_CHECK(cublox_stream_synchronize(batch_->stream()));
+  }
+  const double done = nowMs();
+  std::vector<Completion> out;
+  for (auto &[sid, r] : t.scans) {
+    out.push_back(Completion{r.id, sid, Kind::SCAN, r.submit_ms, t.start_ms, done, t.index,
+                             static_cast<long long>(r.n)});
+  }
+  bool keep = false;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    keep = keep_fetch_;
+  }
+  std::vector<std::pair<long long, std::vector<OccupancyGrid::OccupancySample>>> kept;
+  for (std::size_t k = 0; k < t.fetches.size(); ++k) {
+    const Request &r = t.fetches[k].second;
+    const FetchJob &j = t.jobs[k];
+    // OccupancyGrid.cpp: n = min(count, output cap, max_results, buffer).
+    const unsigned n = j.live ? std::min({h_fcnt_[k], j.limit, j.cap}) : 0u;
+    out.push_back(Completion{r.id, t.fetches[k].first, Kind::FETCH, r.submit_ms, t.start_ms,
+                             done, t.index, static_cast<long long>(n)});
+    if (!keep) continue;
+    std::vector<OccupancyGrid::OccupancySample> v;
+    v.reserve(n);
+    for (unsigned i = 0; i < n; ++i) {
+      const float *p = h_fpos_ + 3 * (j.off + i);
+      v.push_back(OccupancyGrid::OccupancySample{
+          Eigen::Vector3f(p[0], p[1], p[2]),
+          static_cast<OccupancyGrid::VoxelState>(h_fst_[j.off + i])});
+    }
+    kept.emplace_back(r.id, std::move(v));
+  }
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto &[sid, r] : t.scans) {
+    --sessions_.at(sid).busy;
+    stats_.points += r.n;
+  }
+  for (auto &[sid, r] : t.fetches) --sessions_.at(sid).busy;
+  stats_.scans += static_cast<long long>(t.scans.size());
+  stats_.fetches += static_cast<long long>(t.fetches.size());
+  if (!t.scans.empty()) {
+    ++stats_.ticks;
+    stats_.device_ms += batch_->lastUpdateMs() + batch_->lastRecenterMs();
+  }
+  for (auto &kv : kept) fetch_results_[kv.first] = std::move(kv.second);
+  done_.insert(done_.end(), out.begin(), out.end());
+}
+
+// Software pipeline: the next tick is formed and its points copied while
+// the current one runs on the device; the current one then completes
+// (its fetches run after its update) before the next is launched.
+void MapServer::run() {
+  Tick prev;
+  bool have_prev = false;
+  int buf = 0;
+  long long seen = -1;
+  try {
+    while (true) {
+      Tick cur;
+      bool have_cur = false;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        retireClosed();
+        admitWaiting();
+        if (!have_prev) {
+          worker_busy_ = false;
+          cv_idle_.notify_all();
+          cv_work_.wait(lk, [&] { return stop_ || (gen_ != seen && !paused_); });
+          seen = gen_;
+          retireClosed();
+          admitWaiting();
+          worker_busy_ = true;
+        }
+        have_cur = !paused_ && formTick(cur);
+        if (!have_cur && !have_prev) {
+          if (stop_) break;
+          continue;
+        }
+        if (have_cur) cur.index = ticks_++;
+      }
+      if (have_cur) {
+        cur.buf = buf;
+        cur.start_ms = nowMs();
+        upload(cur);
+      }
+      if (have_prev) {
+        completeTick(prev);
+        have_prev = false;
+      }
+      if (have_cur) {
+        MS_CHECK(cublox_stream_synchronize(copy_stream_));
+        if (!cur.scans.empty()) {
+          submitTick(cur);
+          enqueueFetches(cur);
+          prev = std::move(cur);
+          have_prev = true;
+          buf ^= 1;
+        } else {
+          enqueueFetches(cur);
+          completeTick(cur);
+        }
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        cv_idle_.notify_all();
+      }
+    }
+  } catch (...) {
+    std::lock_guard<std::mutex> lk(mu_);
+    worker_error_ = std::current_exception();
+    worker_busy_ = false;
+    cv_idle_.notify_all();
+    return;
+  }
+  std::lock_guard<std::mutex> lk(mu_);
+  worker_busy_ = false;
+  cv_idle_.notify_all();
+}
+
+}  // namespace cublox

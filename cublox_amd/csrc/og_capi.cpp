@@ -38,6 +38,11 @@ std::optional<OG::VoxelState> state_of(int filter) {
 }
 } // namespace
 
+namespace cublox_ffi {
+// Shared with the MapServer binding (ms_capi.cpp).
+void set_last_error(const char *msg) { g_err = msg; }
+} // namespace cublox_ffi
+
 extern "C" {
 
 const char *cublox_og_last_error(void) { return g_err.c_str(); }

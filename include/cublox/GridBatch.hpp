@@ -66,6 +66,17 @@ public:
   void tick(const std::vector<GridScan> &scans,
             const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
                 &recenters);
+  // tick() in two halves: submit() stages and launches the tick on stream()
+  // and returns at once, with the recentered grids' origins already moved
+  // (work the caller enqueues on stream() afterwards sees the grids after
+  // the tick); wait() blocks until the stream is idle. Every submit() needs
+  // its wait() before the next submit(), and the scans' device columns must
+  // stay valid until then.
+  void submit(const std::vector<GridScan> &scans,
+              const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
+                  &recenters);
+  void wait();
+  bool inFlight() const { return in_flight_; }
 
   // Capture the tick's launch sequences in a HIP graph on the next tick and
   // replay it afterwards.
@@ -109,6 +120,7 @@ private:
   int max_regions_{0};
   void *graph_{nullptr};
   bool graph_mode_{false};
+  bool in_flight_{false};
   void *ev_[3]{nullptr, nullptr, nullptr};
 
   // Recenter outcomes of the staged tick, applied to the grids after it.

@@ -46,6 +46,30 @@ def test_host_library_exports(cb):
     assert sorted(set(defined) - set(names)) == []
 
 
+def test_server_library_exports(cb):
+    """ms_capi.h (the map server binding) and ms_capi.cpp agree, and every
+    declared entry point is exported."""
+    from cublox_amd import _ffi
+    L = _ffi.host()
+    names = _declared("ms_capi.h")
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    src = "".join(open(os.path.join(ROOT, "cublox_amd", "csrc", f)).read()
+                  for f in ("ms_capi.cpp", "ServerLoad.cpp"))
+    defined = sorted(set(re.findall(r"^\S.*?\b(cublox_ms_[a-z0-9_]+)\(", src, flags=re.M)))
+    assert defined == names
+
+
+def test_server_session_bytes(cb):
+    """Admission charge: 4 B per voxel of the realised map (2 * half + 1 per
+    axis) plus the fetch output reserve (500000 samples of 13 B)."""
+    reserve = 500000 * 13 + 256
+    for half, vox in [((100, 100, 25), 201 * 201 * 51), ((350, 350, 75), 74201551),
+                      ((200, 200, 50), 16240901)]:
+        assert cb.MapServer.session_bytes(half) == 4 * vox + reserve
+
+
 @pytest.mark.parametrize("p", sorted(LOGIT_BITS))
 def test_logit_constants(cb, orc, p):
     want = LOGIT_BITS[p]
