@@ -146,7 +146,9 @@ def run_batched(cb, tune_ticks_each, ticks):
     w = cb.WORKLOADS["batched"]
     G = w.extra["grids"]
     tune_ticks = len(TUNE_CANDIDATES) * (tune_ticks_each + 1)
-    total = tune_ticks + ticks
+    rc_ticks = 4  # after the timed ticks, without graph: recenter timed alone
+    total = tune_ticks + ticks + rc_ticks
+    timed = range(tune_ticks, tune_ticks + ticks)
     draws = [batched_grid(w, g) for g in range(G)]
     t = time.perf_counter()
     scans = [[d[3].scan(k, d[0], d[1], w.el_lo, w.el_hi, d[2]) for k in range(total)]
@@ -177,9 +179,9 @@ def run_batched(cb, tune_ticks_each, ticks):
             c0 = time.perf_counter()
             plain[g].update_sensor_frame_device(b[0].ptr, b[1].ptr, b[2].ptr, s.n, s.T)
             plain[g].recenter(s.base)
-            if k >= tune_ticks:
+            if k in timed:
                 per_scan.append((time.perf_counter() - c0) * 1e3)
-        if k >= tune_ticks:
+        if k in timed:
             tick_ms.append((time.perf_counter() - a) * 1e3)
     for x in plain:
         x.close()
@@ -190,20 +192,31 @@ def run_batched(cb, tune_ticks_each, ticks):
     batch = cb.GridBatch.for_grids(bg, max(pts), max_near_steps=12, min_chunk=512)
     batch.set_graph_mode(True)
     batch.set_tuning(TUNE_CANDIDATES, tune_ticks_each)
-    btick, bdev = [], []
+    btick, bdev, zeroed, rc_ms, rc_vox = [], [], [], [], []
     for k in range(total):
+        if k == tune_ticks + ticks:
+            batch.set_graph_mode(False)
         sc = [(bg[g], *bufs[g][k], scans[g][k].n, scans[g][k].T) for g in range(G)]
         rc = [(bg[g], scans[g][k].base) for g in range(G)]
         a = time.perf_counter()
         batch.tick(sc, rc)
-        if k >= tune_ticks:
+        if k in timed:
             btick.append((time.perf_counter() - a) * 1e3)
             bdev.append(batch.last[0])
+            zeroed.append(batch.last[2])
+        elif k >= tune_ticks + ticks:
+            rc_ms.append(batch.last[1])
+            rc_vox.append(batch.last[2])
+            records = batch.counters()["records"]
     batch_sps = G * len(btick) / (sum(btick) * 1e-3)
+    # Batched recenter: 4 B per zeroed voxel over the recenter sequence's
+    # device time (descriptor copy + one zeroing launch for all grids).
+    rc_gbs = 4.0 * sum(rc_vox) / (sum(rc_ms) * 1e-3) / 1e9 if sum(rc_ms) > 0 else 0.0
     tune = batch.tune_results(len(TUNE_CANDIDATES))
     out = dict(
         grids=G, grid=f"{w.map_size[0]}x{w.map_size[1]}x{w.map_size[2]} @ {w.resolution} m",
         points_per_tick=round(statistics.mean(pts)), ticks=len(btick),
+        tile_records_per_tick=records,
         plain=dict(scans_per_s=round(plain_sps, 1), tick_ms=round(statistics.mean(tick_ms), 3),
                    scan_p50_ms=round(pct(per_scan, 50), 4),
                    scan_p95_ms=round(pct(per_scan, 95), 4),
@@ -216,6 +229,11 @@ def run_batched(cb, tune_ticks_each, ticks):
                      graph=True, setting=dict(zip(("near_steps", "chunk"), batch.setting)),
                      tuning_ms={f"{a}/{b}": round(m, 4) for (a, b), m in zip(TUNE_CANDIDATES, tune)}),
         speedup_scans_per_s=round(batch_sps / plain_sps, 2),
+        recenter=dict(zeroed_voxels_per_tick=round(statistics.mean(zeroed)),
+                      batched_recenter_ms=round(statistics.mean(rc_ms), 4),
+                      batched_recenter_zeroed_voxels=round(statistics.mean(rc_vox)),
+                      batched_recenter_gbs=round(rc_gbs, 1),
+                      batched_recenter_frac_of_hbm_peak=round(rc_gbs / HBM_PEAK_GBS, 4)),
     )
     batch.close()
     for x in bg:
@@ -385,6 +403,8 @@ def main():
                     help="skip census, PCIe-inclusive, fetch and batched passes")
     ap.add_argument("--batched-ticks", type=int, default=20,
                     help="timed ticks of the 64-grid batched workload (0: skip)")
+    ap.add_argument("--served-seconds", type=float, default=2.0,
+                    help="seconds per load level of the 256-session map server (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -469,6 +489,8 @@ def main():
         if args.batched_ticks > 0:
             extras["batched_workload"] = run_batched(cb, 1, args.batched_ticks)
             log(f"batched: {extras['batched_workload']}")
+        if args.served_seconds > 0:
+            extras["served_workload"] = run_served(cb, seconds=args.served_seconds)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -51,7 +51,8 @@ constexpr int TX = 16, TY = 16, TZ = 32;
 constexpr int kTileVox = TX * TY * TZ;  // 8192
 constexpr int kWalkBlock = 256;
 constexpr int kMaxNearK = 12;
-constexpr int kBinRays = 128;
+constexpr int kRayAlign = 1024;   // grids' ray ranges start at multiples of it
+constexpr int kBinThreads = 512;
 constexpr int kHash = 4096;
 constexpr int kHashProbes = 64;
 constexpr int kChunk = 1024;      // records per chunk of a hot tile (default)
@@ -72,6 +73,7 @@ struct Segment {  // 32 B
   unsigned meta;
   unsigned pad;
 };
+static_assert(sizeof(Segment) == 32, "records move as two 16 B words");
 
 constexpr unsigned kMetaLi = 0x1FFFu;
 constexpr int kMetaN = 13;
@@ -390,6 +392,133 @@ __device__ __forceinline__ void far_by_span(const IntegrateArgs &g,
   }
 }
 
+// Value of v in lane k of this lane's quad (lanes 4j..4j+3): a DPP quad
+// permute, exact for any 32-bit value. Every lane of the quad is active.
+template <int k>
+__device__ __forceinline__ int quad_get(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, k | (k << 2) | (k << 4) | (k << 6), 0xF, 0xF,
+                                     false);
+}
+template <int k>
+__device__ __forceinline__ float quad_get(float v) {
+  return __int_as_float(quad_get<k>(__float_as_int(v)));
+}
+
+// v of axis q (x, y, z; q = 3 takes z), chosen by value: a conditional over
+// members would select their addresses and keep the walk state in memory.
+template <typename T>
+__device__ __forceinline__ T pick3(int q, T x, T y, T z) {
+  T v = z;
+  if (q == 0) {
+    v = x;
+  } else if (q == 1) {
+    v = y;
+  }
+  return v;
+}
+
+// far_by_span with the ray's quad of lanes sharing the per-axis work: lane
+// q < 3 owns axis q (its exit parameter, its steps before the span exit,
+// its end-voxel parameters), lane 3 stands by; the quad exchanges the
+// per-axis results and every lane then holds the same walk state and takes
+// the same branches. Same spans, same records, same order as far_by_span
+// (every axis's exit parameter is formed exactly, so no guess is needed).
+template <typename Emit>
+__device__ __forceinline__ void far_by_span_quad(const IntegrateArgs &g,
+                                                 RayState &r, int step, int lend,
+                                                 int wx, int wy, int wz, int q,
+                                                 Emit &emit) {
+  const RayArgs &a = g.a;
+  const float inf = __builtin_inff();
+  const HashCoord hc = window_hash_coord(g, r.ix, r.iy, r.iz);
+  int t0x = hc.cx - hc.cx % TX, t0y = hc.cy - hc.cy % TY,
+      t0z = hc.cz - hc.cz % TZ;
+  int lx = hc.cx - t0x, ly = hc.cy - t0y, lz = hc.cz - t0z;
+  int bx = exit_coord(r.ix, r.sx, hc.cx, t0x, span_width(t0x, a.mx, TX), wx);
+  int by = exit_coord(r.iy, r.sy, hc.cy, t0y, span_width(t0y, a.my, TY), wy);
+  int bz = exit_coord(r.iz, r.sz, hc.cz, t0z, span_width(t0z, a.mz, TZ), wz);
+  // This lane's axis (lane 3: an axis that never moves).
+  const int ms = q == 3 ? 0 : pick3(q, r.sx, r.sy, r.sz);
+  const float md = pick3(q, r.tDx, r.tDy, r.tDz);
+  while (true) {
+    const int li = (lx * TY + ly) * TZ + lz;
+    const int tile = span_tile(g, t0x, t0y, t0z);
+    const int nbx = r.sx != 0 ? abs(bx - r.ix) : kBig;
+    const int nby = r.sy != 0 ? abs(by - r.iy) : kBig;
+    const int nbz = r.sz != 0 ? abs(bz - r.iz) : kBig;
+    const int mnb = pick3(q, nbx, nby, nbz);
+    const float mh0 = pick3(q, r.tMx, r.tMy, r.tMz);
+    const float me = ms != 0 ? seq_at(mh0, md, mnb - 1) : inf;
+    const float ex = quad_get<0>(me), ey = quad_get<1>(me), ez = quad_get<2>(me);
+    const int ax = (ez <= ex && ez <= ey) ? 2 : (ey <= ex ? 1 : 0);
+    const float e = ax == 0 ? ex : (ax == 1 ? ey : ez);
+    float mh = mh0;
+    int mc = 0;
+    if (q == ax) {
+      mc = mnb;
+      mh = e + md;
+    } else if (ms != 0) {
+      // z before y before x on ties: z always first, y first against x.
+      mc = count_before(mh, md, mnb, e, q == 2 || (q == 1 && ax == 0));
+    }
+    const int cx = quad_get<0>(mc), cy = quad_get<1>(mc), cz = quad_get<2>(mc);
+    const float hx = quad_get<0>(mh), hy = quad_get<1>(mh), hz = quad_get<2>(mh);
+    const int n = cx + cy + cz;  // voxels of this span
+    if (lend <= step + n) {
+      const int tn = lend - step;
+      const int dx = abs(r.eix - r.ix), dy = abs(r.eiy - r.iy),
+                dz = abs(r.eiz - r.iz);
+      bool reached = false;
+      if (dx + dy + dz == tn) {
+        // end_reached, its three running sums one per lane.
+        const int md_steps = q == 3 ? 0 : pick3(q, dx, dy, dz);
+        const float min_ = md_steps > 0 ? seq_at(mh0, md, md_steps - 1) : -inf;
+        const float mout = md_steps > 0 ? min_ + md : mh0;
+        const float inx = quad_get<0>(min_), iny = quad_get<1>(min_), inz = quad_get<2>(min_);
+        const float ox = quad_get<0>(mout), oy = quad_get<1>(mout), oz = quad_get<2>(mout);
+        reached = inx < inf && iny < inf && inz < inf && inx < oy && inx < oz &&
+                  iny <= ox && iny < oz && inz <= ox && inz <= oy;
+      }
+      if (reached) {
+        if (tn > 0) emit(r.tMx, r.tMy, r.tMz, li, tn, tile);
+      } else {
+        far_by_step(g, r, step, emit);
+      }
+      return;
+    }
+    emit(r.tMx, r.tMy, r.tMz, li, n, tile);
+    if ((ax == 0 && bx == wx) || (ax == 1 && by == wy) || (ax == 2 && bz == wz)) {
+      return;  // the next voxel is outside the window
+    }
+    r.ix += r.sx * cx;
+    r.iy += r.sy * cy;
+    r.iz += r.sz * cz;
+    lx += r.sx * cx;
+    ly += r.sy * cy;
+    lz += r.sz * cz;
+    r.tMx = hx;
+    r.tMy = hy;
+    r.tMz = hz;
+    step += n;
+    if (ax == 0) {
+      t0x = next_span(t0x, r.sx, a.mx, TX);
+      const int tw = span_width(t0x, a.mx, TX);
+      bx = r.sx > 0 ? min(r.ix + tw, wx) : max(r.ix - tw, wx);
+      lx = r.sx > 0 ? 0 : tw - 1;
+    } else if (ax == 1) {
+      t0y = next_span(t0y, r.sy, a.my, TY);
+      const int tw = span_width(t0y, a.my, TY);
+      by = r.sy > 0 ? min(r.iy + tw, wy) : max(r.iy - tw, wy);
+      ly = r.sy > 0 ? 0 : tw - 1;
+    } else {
+      t0z = next_span(t0z, r.sz, a.mz, TZ);
+      const int tw = span_width(t0z, a.mz, TZ);
+      bz = r.sz > 0 ? min(r.iz + tw, wz) : max(r.iz - tw, wz);
+      lz = r.sz > 0 ? 0 : tw - 1;
+    }
+  }
+}
+
 // ----------------------------------------------------------------------
 // Grids of one launch
 // ----------------------------------------------------------------------
@@ -404,7 +533,7 @@ struct GridDesc {
   float *occ;
   int *export_op, *export_hit;
   int transform;       // points are sensor-frame, world = T * p
-  int ray_base;        // first ray; a multiple of kWalkBlock
+  int ray_base;        // first ray; a multiple of kRayAlign
   long long slot_base; // first segment slot; rays own g.max_seg slots each
   int tile_base;       // first tile
   int near_base;       // first near-field cell
@@ -482,7 +611,11 @@ __device__ __forceinline__ GridDesc grid_of_near(const Batch &b, int cell) {
 // ----------------------------------------------------------------------
 // walk kernel
 // ----------------------------------------------------------------------
-__global__ void __launch_bounds__(kWalkBlock)
+// kLanes lanes per ray: 1, or 4 (a quad shares each span's per-axis work,
+// for launches too small to fill the GPU with one lane per ray). All lanes
+// of a ray hold the same walk state; lane 0 writes its records and counts.
+template <int kLanes>
+__global__ void __launch_bounds__(kWalkBlock * kLanes)
     walk_kernel(const Batch b, Workspace w) {
   extern __shared__ int s_near[];  // the grid's near cube, sized at launch
   const int ray0 = blockIdx.x * kWalkBlock;
@@ -493,7 +626,9 @@ __global__ void __launch_bounds__(kWalkBlock)
   for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
   __syncthreads();
 
-  const int ray = ray0 + threadIdx.x;
+  const int q = static_cast<int>(threadIdx.x) % kLanes;
+  const bool lead = q == 0;
+  const int ray = ray0 + static_cast<int>(threadIdx.x) / kLanes;
   const int i = ray - d.ray_base;  // index within the grid's points
   const RayArgs &a = g.a;
   RayState r;
@@ -511,7 +646,8 @@ __global__ void __launch_bounds__(kWalkBlock)
     int *__restrict__ mine_tile = w.seg_tile + slot0;
     int nseg = 0;
     auto put = [&](const Segment &sg, int tile) {
-      if (nseg < g.max_seg) {
+      if (!lead) {
+      } else if (nseg < g.max_seg) {
         mine[nseg] = sg;
         mine_tile[nseg] = d.tile_base + tile;
       } else {
@@ -555,14 +691,18 @@ __global__ void __launch_bounds__(kWalkBlock)
           done = true;
           break;
         }
-        atomicAdd(s_near + ci, 1);
+        if (lead) atomicAdd(s_near + ci, 1);
         const int ax = ray_advance(r);
         ci += ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
       }
       if (!done && step < r.max_steps &&
           !(step == lend && r.ix == r.eix && r.iy == r.eiy && r.iz == r.eiz) &&
           !(r.ix == wx || r.iy == wy || r.iz == wz)) {
-        far_by_span(g, r, step, lend, wx, wy, wz, emit);
+        if (kLanes == 4) {
+          far_by_span_quad(g, r, step, lend, wx, wy, wz, q, emit);
+        } else {
+          far_by_span(g, r, step, lend, wx, wy, wz, emit);
+        }
       }
     } else {
       // Sensor outside the window: the ray may enter it later.
@@ -572,8 +712,8 @@ __global__ void __launch_bounds__(kWalkBlock)
           ended = true;
           break;
         }
-        if (inside_window(r.ix, r.iy, r.iz, a.oix, a.oiy, a.oiz, a.hx, a.hy,
-                          a.hz)) {
+        if (lead && inside_window(r.ix, r.iy, r.iz, a.oix, a.oiy, a.oiz, a.hx, a.hy,
+                                  a.hz)) {
           atomicAdd(s_near + cube_index(g, r.ix - s0x, r.iy - s0y, r.iz - s0z), 1);
         }
         ray_advance(r);
@@ -586,8 +726,10 @@ __global__ void __launch_bounds__(kWalkBlock)
       const int ddx = r.eix - s0x, ddy = r.eiy - s0y, ddz = r.eiz - s0z;
       if (abs(ddx) + abs(ddy) + abs(ddz) < K) {
         const int c = cube_index(g, ddx, ddy, ddz);
-        atomicAdd(s_near + c, 1);
-        if (!r.clipped) atomic_add_i32(w.near_hit + d.near_base + c, 1);
+        if (lead) {
+          atomicAdd(s_near + c, 1);
+          if (!r.clipped) atomic_add_i32(w.near_hit + d.near_base + c, 1);
+        }
       } else {
         const HashCoord e = window_hash_coord(g, r.eix, r.eiy, r.eiz);
         Segment sg;
@@ -599,8 +741,8 @@ __global__ void __launch_bounds__(kWalkBlock)
         put(sg, tile_of(g, e.cx, e.cy, e.cz));
       }
     }
-    w.seg_count[ray] = min(nseg, g.max_seg);
-  } else {
+    if (lead) w.seg_count[ray] = min(nseg, g.max_seg);
+  } else if (lead) {
     w.seg_count[ray] = 0;
   }
   __syncthreads();
@@ -613,95 +755,6 @@ __global__ void __launch_bounds__(kWalkBlock)
 // ----------------------------------------------------------------------
 // binning: LDS hash aggregation of segments by tile
 // ----------------------------------------------------------------------
-
-// Finds (inserting if needed) the LDS slot of `key`; -1 when the bounded
-// probe finds neither. A key is either in the table for every caller or
-// for none: slots only fill, so a full probe window stays full.
-__device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
-  unsigned h = (static_cast<unsigned>(key) * 2654435761u) >> 20;
-  for (int p = 0; p < kHashProbes; ++p, ++h) {
-    h &= kHash - 1;
-    const int k = keys[h];
-    if (k == key) return static_cast<int>(h);
-    if (k == -1) {
-      if (!insert) return -1;
-      const int prev = atomicCAS(keys + h, -1, key);
-      if (prev == -1 || prev == key) return static_cast<int>(h);
-    }
-  }
-  return -1;
-}
-
-template <bool kPlace>
-__global__ void __launch_bounds__(256)
-    bin_kernel(const Batch b, Workspace w) {
-  __shared__ int s_key[kHash];
-  __shared__ int s_cnt[kHash];
-  __shared__ int s_base[kHash];
-  const int r0 = blockIdx.x * kBinRays;
-  if (r0 >= batch_rays(b)) return;
-  const GridDesc d = grid_of_ray(b, r0);
-  for (int e = threadIdx.x; e < kHash; e += blockDim.x) {
-    s_key[e] = -1;
-    s_cnt[e] = 0;
-  }
-  __syncthreads();
-  const int lane_ray = threadIdx.x % kBinRays;
-  const int j0 = threadIdx.x / kBinRays;
-  constexpr int kJStride = 256 / kBinRays;
-  const int ray = r0 + lane_ray;
-  const int cnt = w.seg_count[ray];
-  const long long base =
-      d.slot_base + static_cast<long long>(ray - d.ray_base) * d.g.max_seg;
-
-  for (int j = j0; j < cnt; j += kJStride) {
-    const int t = w.seg_tile[base + j];
-    const int e = lds_hash_slot(s_key, t, true);
-    if (e >= 0) {
-      atomicAdd(s_cnt + e, 1);
-    } else if (!kPlace) {
-      atomic_add_i32(w.tile_count + t, 1);
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kHash; e += blockDim.x) {
-    const int key = s_key[e];
-    if (key < 0) continue;
-    if (!kPlace) {
-      atomic_add_i32(w.tile_count + key, s_cnt[e]);
-    } else {
-      s_base[e] = atomicAdd(w.tile_cursor + key, s_cnt[e]);
-      s_cnt[e] = 0;
-    }
-  }
-  if (!kPlace) return;
-  __syncthreads();
-  for (int j = j0; j < cnt; j += kJStride) {
-    const int t = w.seg_tile[base + j];
-    const Segment sg = w.segs[base + j];
-    const int e = lds_hash_slot(s_key, t, false);
-    const long long slot = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
-                                  : atomicAdd(w.tile_cursor + t, 1);
-    if (slot < w.seg_capacity) w.sorted[slot] = sg;
-  }
-}
-
-// ----------------------------------------------------------------------
-// tile_scan: cursors, work items, hot-tile splits
-// ----------------------------------------------------------------------
-// A tile with more than `chunk` records is split into chunks of about
-// `chunk` records, each its own work item with its own scratch slab, while
-// the split tiles' slabs (taken in tile order) fit the scratch area; a hot
-// tile beyond that is one item, accumulated in 16-bit passes. Items are
-// placed in decreasing cost classes (log2 of records per item) so the
-// heaviest start first.
-constexpr int kScanThreads = 1024;
-constexpr int kScanWaves = kScanThreads / 64;
-constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
-constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
-constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
-constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
-constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
 
 __device__ __forceinline__ int wave_scan_incl(int v) {
   const int lane = threadIdx.x & 63;
@@ -730,6 +783,139 @@ __device__ __forceinline__ int block_scan_incl(int *s_wave, int v) {
   __syncthreads();
   return r;
 }
+
+
+// Finds (inserting if needed) the LDS slot of `key`; -1 when the bounded
+// probe finds neither. A key is either in the table for every caller or
+// for none: slots only fill, so a full probe window stays full.
+__device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
+  unsigned h = (static_cast<unsigned>(key) * 2654435761u) >> 20;
+  for (int p = 0; p < kHashProbes; ++p, ++h) {
+    h &= kHash - 1;
+    const int k = keys[h];
+    if (k == key) return static_cast<int>(h);
+    if (k == -1) {
+      if (!insert) return -1;
+      const int prev = atomicCAS(keys + h, -1, key);
+      if (prev == -1 || prev == key) return static_cast<int>(h);
+    }
+  }
+  return -1;
+}
+
+// A block bins the records of `rays` consecutive rays (a divisor of
+// kRayAlign, so all of one grid), taken in flattened order -- an LDS prefix
+// of the rays' record counts maps record f to its ray and slot -- so
+// consecutive lanes read consecutive slots. Counts aggregate per tile in an
+// LDS hash: one global atomic per distinct tile of the block (records that
+// find no hash slot go to the global counters directly).
+template <bool kPlace>
+__global__ void __launch_bounds__(kBinThreads)
+    bin_kernel(const Batch b, Workspace w, int rays) {
+  __shared__ int s_key[kHash];
+  __shared__ int s_cnt[kHash];
+  __shared__ int s_base[kHash];
+  __shared__ int s_pre[kRayAlign + 1];
+  __shared__ int s_wave[kBinThreads / 64];
+  const int r0 = blockIdx.x * rays;
+  const int n_rays = batch_rays(b);
+  if (r0 >= n_rays) return;
+  const GridDesc d = grid_of_ray(b, r0);
+  const int t = threadIdx.x;
+  for (int e = t; e < kHash; e += kBinThreads) {
+    s_key[e] = -1;
+    s_cnt[e] = 0;
+  }
+  constexpr int kPerT = kRayAlign / kBinThreads;
+  int c[kPerT];
+  int sum = 0;
+#pragma unroll
+  for (int j = 0; j < kPerT; ++j) {
+    const int r = t * kPerT + j;
+    c[j] = (r < rays && r0 + r < n_rays) ? w.seg_count[r0 + r] : 0;
+    sum += c[j];
+  }
+  const int incl = block_scan_incl(s_wave, sum);
+  int run = incl - sum;
+#pragma unroll
+  for (int j = 0; j < kPerT; ++j) {
+    s_pre[t * kPerT + j] = run;
+    run += c[j];
+  }
+  if (t == kBinThreads - 1) s_pre[kRayAlign] = incl;
+  __syncthreads();
+  const int total = s_pre[kRayAlign];
+  const long long base0 =
+      d.slot_base + static_cast<long long>(r0 - d.ray_base) * d.g.max_seg;
+  const int max_seg = d.g.max_seg;
+  // Slot of record f: its ray is the last with s_pre <= f.
+  auto slot_of = [&](int f) {
+    int lo = 0, hi = rays - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= f) {
+        lo = mid;
+      } else {
+        hi = mid - 1;
+      }
+    }
+    return base0 + static_cast<long long>(lo) * max_seg + (f - s_pre[lo]);
+  };
+
+  for (int f = t; f < total; f += kBinThreads) {
+    const int tile = w.seg_tile[slot_of(f)];
+    const int e = lds_hash_slot(s_key, tile, true);
+    if (e >= 0) {
+      atomicAdd(s_cnt + e, 1);
+    } else if (!kPlace) {
+      atomic_add_i32(w.tile_count + tile, 1);
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < kHash; e += kBinThreads) {
+    const int key = s_key[e];
+    if (key < 0) continue;
+    if (!kPlace) {
+      atomic_add_i32(w.tile_count + key, s_cnt[e]);
+    } else {
+      s_base[e] = atomicAdd(w.tile_cursor + key, s_cnt[e]);
+      s_cnt[e] = 0;
+    }
+  }
+  if (!kPlace) return;
+  __syncthreads();
+  for (int f = t; f < total; f += kBinThreads) {
+    const long long slot = slot_of(f);
+    const int tile = w.seg_tile[slot];
+    const uint4 *src = reinterpret_cast<const uint4 *>(w.segs + slot);
+    const uint4 lo = src[0], hi = src[1];  // the 32 B record, in registers
+    const int e = lds_hash_slot(s_key, tile, false);
+    const long long pos = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
+                                 : atomicAdd(w.tile_cursor + tile, 1);
+    if (pos < w.seg_capacity) {
+      uint4 *dst = reinterpret_cast<uint4 *>(w.sorted + pos);
+      dst[0] = lo;
+      dst[1] = hi;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------
+// tile_scan: cursors, work items, hot-tile splits
+// ----------------------------------------------------------------------
+// A tile with more than `chunk` records is split into chunks of about
+// `chunk` records, each its own work item with its own scratch slab, while
+// the split tiles' slabs (taken in tile order) fit the scratch area; a hot
+// tile beyond that is one item, accumulated in 16-bit passes. Items are
+// placed in decreasing cost classes (log2 of records per item) so the
+// heaviest start first.
+constexpr int kScanThreads = 1024;
+constexpr int kScanWaves = kScanThreads / 64;
+constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
+constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
+constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
+constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
+constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
 
 // Cost class of a work item (its record count), larger classes first.
 __device__ __forceinline__ int cost_bucket(int records) {
@@ -1486,6 +1672,18 @@ GridDesc make_desc(const cublox_raycast_cfg &cfg, const float *T,
   return d;
 }
 
+// Launches of at most this many rays walk with a quad of lanes per ray.
+constexpr long long kQuadWalkRays = 1 << 18;
+
+// Rays per bin workgroup: larger blocks share more tiles (fewer global
+// atomics), smaller ones fill the GPU on small launches. A power of two
+// dividing kRayAlign, from the capacities only.
+int bin_rays_per_block(const Caps &c) {
+  int r = 256;
+  while (r < kRayAlign && c.max_rays / (2 * r) >= 1024) r *= 2;
+  return r;
+}
+
 // The launch sequence for a workspace of layout L. Launch shapes depend on
 // the capacities only (so the sequence can be captured once and replayed);
 // the kernels read the actual counts from the batch.
@@ -1494,11 +1692,16 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   const Caps &c = L.cap;
   const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
-  walk_kernel<<<dim3(walk_blocks), dim3(kWalkBlock), near_lds, hs>>>(b, w);
+  if (c.max_rays <= kQuadWalkRays) {
+    walk_kernel<4><<<dim3(walk_blocks), dim3(4 * kWalkBlock), near_lds, hs>>>(b, w);
+  } else {
+    walk_kernel<1><<<dim3(walk_blocks), dim3(kWalkBlock), near_lds, hs>>>(b, w);
+  }
   int st = check_launch("walk_kernel");
   if (st != CUBLOX_OK) return st;
-  const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + kBinRays - 1) / kBinRays);
-  bin_kernel<false><<<dim3(bin_blocks), dim3(256), 0, hs>>>(b, w);
+  const int bin_rays = bin_rays_per_block(c);
+  const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + bin_rays - 1) / bin_rays);
+  bin_kernel<false><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
   st = check_launch("bin_kernel<count>");
   if (st != CUBLOX_OK) return st;
   if (c.max_tiles <= kScanStage) {
@@ -1515,7 +1718,7 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     st = check_launch("tile_place_kernel");
   }
   if (st != CUBLOX_OK) return st;
-  bin_kernel<true><<<dim3(bin_blocks), dim3(256), 0, hs>>>(b, w);
+  bin_kernel<true><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
   const int apply_blocks = std::min(L.max_items, 1 << 16);
@@ -1539,7 +1742,7 @@ Caps batch_caps(const cublox_batch_limits &lim) {
   Caps c{};
   c.max_grids = std::max(lim.max_grids, 1);
   c.max_rays = static_cast<long long>(std::max(lim.max_points, 1)) +
-               static_cast<long long>(c.max_grids) * kWalkBlock;
+               static_cast<long long>(c.max_grids) * kRayAlign;
   c.seg_capacity = static_cast<long long>(std::max(lim.max_points, 1)) *
                    std::max(lim.max_segments_per_ray, 1);
   c.max_tiles = std::max(lim.max_tiles, 1);
@@ -1676,7 +1879,7 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
     sg->desc[k] = d;
     const long long n = d.g.n;
     points += n;
-    ray += (n + kWalkBlock - 1) / kWalkBlock * kWalkBlock;
+    ray += (n + kRayAlign - 1) / kRayAlign * kRayAlign;
     slot += n * d.g.max_seg;
     tile += d.g.n_tiles;
     near += d.g.cube * d.g.cube * d.g.cube;

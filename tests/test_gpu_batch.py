@@ -41,9 +41,9 @@ def _setup(cb, orc):
     return grids, oracles, scenes
 
 
-def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None):
+def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=None):
     grids, oracles, scenes = _setup(cb, orc)
-    total = sum(b * a for (_, _, _, _, _, b, a, _, _) in GRIDS)
+    total = max_points or sum(b * a for (_, _, _, _, _, b, a, _, _) in GRIDS)
     kw = {} if max_tiles is None else {"max_tiles": max_tiles}
     batch = cb.GridBatch.for_grids(grids, total, max_near_steps=12, min_chunk=256,
                                    **kw)
@@ -82,6 +82,12 @@ def test_batch_graph_replay_matches(cb, orc, gpu_ready):
 def test_batch_multiblock_tile_scan(cb, orc, gpu_ready):
     """Tile capacity above one workgroup's stage: the three-pass scan."""
     _run(cb, orc, 4, max_tiles=20000)
+
+
+def test_batch_large_bin_blocks(cb, orc, gpu_ready):
+    """Limits for 2M points per tick: the binning takes 1024 rays per
+    workgroup (each grid's ray range starts at a multiple of 1024)."""
+    _run(cb, orc, 3, max_points=1 << 21)
 
 
 def test_batch_autotuner_keeps_results(cb, orc, gpu_ready):
