@@ -915,7 +915,7 @@ constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
 constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
 constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
 constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
-constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
+
 
 // Cost class of a work item (its record count), larger classes first.
 __device__ __forceinline__ int cost_bucket(int records) {
@@ -941,9 +941,10 @@ __device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
   const int ch = split ? chunks_of(c, chunk) : 1;
   const int per_chunk = (c + ch - 1) / ch;
   if (split) w.hot[sp] = HotTile{k, sl, ch, 0};
+  const int pos0 = atomicAdd(s_cursor + bucket, slots);  // one claim per tile
   for (int q = 0; q < slots; ++q) {
-    const int pos = atomicAdd(s_cursor + bucket, 1);
-    if (pos >= w.max_items) continue;
+    const int pos = pos0 + q;
+    if (pos >= w.max_items) break;
     if (q < ch) {
       const int ib = static_cast<int>(run) + q * per_chunk;
       const int ie = min(ib + per_chunk, static_cast<int>(run) + c);
@@ -1300,25 +1301,27 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
   }
 }
 
-// Work items in decreasing cost order, one per workgroup while they last
-// (then strided), so the hardware dispatcher balances them. A plain item
-// accumulates its tile in LDS and folds it; a chunk of a hot tile stores
-// its packed counts to its scratch slab (hot_fold sums a tile's slabs).
-// kBig: the unsplit hot tiles instead, accumulated in passes of kPass
+// Work items in order -- first the unsplit hot tiles, the longest items,
+// then the plain items in decreasing cost order -- one per workgroup while
+// they last (then strided), so the hardware dispatcher balances them.
+// A plain item accumulates its tile in LDS and folds it; a chunk of a hot
+// tile stores its packed counts to its scratch slab (hot_fold sums a
+// tile's slabs); an unsplit hot tile accumulates in passes of kPass
 // records summed in registers. Empty items do nothing.
-template <bool kBig>
 __global__ void __launch_bounds__(kApplyThreads, 4)
     tile_apply_kernel(const Batch b, Workspace w) {
   __shared__ uint4 s_cnt4[kTileVox / 4];
   __shared__ ReplayScratch s_rs;
   unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
-  const int n_items = kBig ? w.misc[4] : w.misc[0];
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const Item itm = w.items[kBig ? w.big[it] : it];
-    if (itm.tile < 0 || (!kBig && itm.slab == -2)) continue;  // uniform
+  const int n_big = w.misc[4];
+  const int n_all = n_big + w.misc[0];
+  for (int it = blockIdx.x; it < n_all; it += gridDim.x) {
+    const bool big = it < n_big;  // uniform
+    const Item itm = w.items[big ? w.big[it] : it - n_big];
+    if (itm.tile < 0 || (!big && itm.slab == -2)) continue;  // uniform
     const int end = static_cast<int>(
         min(static_cast<long long>(itm.end), w.seg_capacity));
-    if (!kBig && itm.slab >= 0) {
+    if (!big && itm.slab >= 0) {
       lds_zero(s_cnt4);
       __syncthreads();
       tile_accumulate(w, itm.beg, end, s_cnt, s_rs);
@@ -1335,7 +1338,7 @@ __global__ void __launch_bounds__(kApplyThreads, 4)
     const int tz = tile % d.g.ntz;
     const int ty = (tile / d.g.ntz) % d.g.nty;
     const int tx = tile / (d.g.ntz * d.g.nty);
-    if (!kBig) {
+    if (!big) {
       // One pass: the packed 16-bit counts fold as they are.
       lds_zero(s_cnt4);
       __syncthreads();
@@ -1722,11 +1725,8 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
   const int apply_blocks = std::min(L.max_items, 1 << 16);
-  tile_apply_kernel<false><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  tile_apply_kernel<<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
   st = check_launch("tile_apply_kernel");
-  if (st != CUBLOX_OK) return st;
-  tile_apply_kernel<true><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
-  st = check_launch("tile_apply_kernel<big>");
   if (st != CUBLOX_OK) return st;
   const long long hot_blocks =
       std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
