@@ -200,7 +200,8 @@ constexpr int kBig = 1 << 20;  // steps to the exit of an axis that never moves
 // A_k of the running sum A_0 = h, A_{j+1} = A_j + d, formed in order as the
 // walk forms its crossing parameters.
 __device__ __forceinline__ float seq_at(float h, float d, int k) {
-  for (int j = 0; j < k; ++j) h += d;
+  for (; k >= 4; k -= 4) h = (((h + d) + d) + d) + d;  // each add rounded in turn
+  for (; k > 0; --k) h += d;
   return h;
 }
 
@@ -212,8 +213,31 @@ __device__ __forceinline__ float seq_at(float h, float d, int k) {
 // axis goes first on ties. h is left at the axis's next parameter.
 __device__ __forceinline__ int count_before(float &h, float d, int nb, float e,
                                             bool first_on_tie) {
+  // The sums increase (d > 0), so "before e" holds for a prefix of them:
+  // four candidates per iteration, counted by how many hold.
+  auto before = [&](float x) { return x < e || (first_on_tie && x == e); };
   int c = 0;
-  while (c < nb && (h < e || (first_on_tie && h == e))) {
+  while (c + 4 <= nb) {
+    const float h1 = h + d, h2 = h1 + d, h3 = h2 + d;
+    if (!before(h3)) {
+      if (before(h)) {
+        ++c;
+        h = h1;
+        if (before(h1)) {
+          ++c;
+          h = h2;
+          if (before(h2)) {
+            ++c;
+            h = h3;
+          }
+        }
+      }
+      return c;
+    }
+    h = h3 + d;
+    c += 4;
+  }
+  while (c < nb && before(h)) {
     h += d;
     ++c;
   }
