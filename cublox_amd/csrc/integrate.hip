@@ -563,6 +563,8 @@ struct GridDesc {
   int near_base;       // first near-field cell
 };
 
+static_assert(sizeof(GridDesc) == 256, "descriptor size (documented)");
+
 struct BatchHeader {  // device-resident in batch mode
   int n_grids;
   int n_rays;   // end of the ray range (padded)
