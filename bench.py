@@ -258,7 +258,7 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
     shapes, HBM-budget admission (200 GB), continuous batching. Saturation
     is the drain rate of `depth` queued scans per session; then open-loop
     Poisson scan arrivals at each load fraction of it (fetches at 20 % of the
-    scan rate, r = 1.5 m) for `seconds`, with latency = completion - submit
+    scan rate, r uniform in 0.5-3 m) for `seconds`, with latency = completion - submit
     (host copy, H2D, batched update + recenter, fetch). The same sessions
     served one scan per tick give the unbatched saturation."""
     from cublox_amd.workloads import batched_grid
@@ -314,15 +314,17 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
     levels = []
     for f in loads:
         rate = f * sat
-        n_sub = srv.run_load(sids, templates, scan_rate_hz=rate, duration_s=seconds,
-                             fetch_ratio=0.2, fetch_radius=1.5, speed=0.5, seed=17)
+        n_sub, lag = srv.run_load(sids, templates, scan_rate_hz=rate, duration_s=seconds,
+                                  fetch_ratio=0.2, fetch_radius=0.5, fetch_radius_max=3.0,
+                                  speed=0.5, seed=17, clients=8)
         srv.drain()
         c = srv.completions()
         scans = c[c["kind"] == 0]
         span = (c["done_ms"].max() - c["submit_ms"].min()) * 1e-3
         levels.append(dict(load=f, offered_scans_per_s=round(rate, 1),
                            achieved_scans_per_s=round(len(scans) / span, 1),
-                           requests=int(n_sub), scan_latency=_lat(scans),
+                           requests=int(n_sub), client_max_lag_ms=round(lag, 3),
+                           scan_latency=_lat(scans),
                            fetch_latency=_lat(c[c["kind"] == 1])))
         log(f"served: load {f}: {levels[-1]}")
     st = srv.stats()
@@ -405,6 +407,8 @@ def main():
                     help="timed ticks of the 64-grid batched workload (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="seconds per load level of the 256-session map server (0: skip)")
+    ap.add_argument("--only", choices=("batched", "served"),
+                    help="run only that workload and print its record (for profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -424,6 +428,13 @@ def main():
     if cb.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    if args.only:
+        rec = (run_batched(cb, 1, args.batched_ticks) if args.only == "batched"
+               else run_served(cb, seconds=args.served_seconds))
+        sys.stdout.flush()
+        os.write(json_fd, (json.dumps({"workload": args.only, **rec}) + "\n").encode())
+        os.close(json_fd)
+        return
     w = cb.WORKLOADS[args.workload]
     log(f"device: {cb.device_name()}")
     log(f"workload {w.name}: grid {w.map_size} ({w.voxels} voxels) @ {w.resolution} m, "

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <stdexcept>
 #include <string>
 
@@ -59,24 +60,20 @@ MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
   bl.max_segments_per_ray = lim_.max_segments_per_ray;
   bl.max_near_steps = 12;
   bl.min_chunk = 512;
+  if (lim_.point_arena_bytes < 3ull * sizeof(float) * lim_.max_points_per_scan + 256) {
+    throw std::invalid_argument("cublox::MapServer: point arena below one largest scan");
+  }
   batch_ = std::make_unique<GridBatch>(bl);
   batch_->setGraphMode(lim_.graph);
-  const std::size_t pts_bytes = 3ull * sizeof(float) * lim_.max_points_per_tick;
   try {
-    MS_CHECK(cublox_stream_create(&copy_stream_));
-    for (int b = 0; b < 2; ++b) {
-      MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_pts_[b]), pts_bytes));
-      MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_pts_[b]), pts_bytes));
-    }
+    MS_CHECK(cublox_stream_create(&upload_stream_));
+    MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_arena_), lim_.point_arena_bytes));
   } catch (...) {
-    for (int b = 0; b < 2; ++b) {
-      if (h_pts_[b]) cublox_host_free(h_pts_[b]);
-      if (d_pts_[b]) cublox_free(d_pts_[b]);
-    }
-    if (copy_stream_) cublox_stream_destroy(copy_stream_);
+    if (upload_stream_) cublox_stream_destroy(upload_stream_);
     throw;
   }
-  stats_.workspace_bytes = 2 * pts_bytes;
+  arena_free_[0] = lim_.point_arena_bytes;
+  stats_.workspace_bytes = lim_.point_arena_bytes;
   t0_ = std::chrono::steady_clock::now();
   worker_ = std::thread([this] { run(); });
 }
@@ -88,19 +85,57 @@ MapServer::~MapServer() {
   }
   cv_work_.notify_all();
   if (worker_.joinable()) worker_.join();
-  batch_.reset();
-  for (int b = 0; b < 2; ++b) {
-    if (h_pts_[b]) cublox_host_free(h_pts_[b]);
-    if (d_pts_[b]) cublox_free(d_pts_[b]);
+  {
+    std::lock_guard<std::mutex> lk(arena_mu_);
+    arena_closed_ = true;
   }
-  growFetchArena(0, 0);  // frees
-  if (copy_stream_) cublox_stream_destroy(copy_stream_);
+  arena_cv_.notify_all();
+  batch_.reset();
+  for (auto &a : farena_) growFetchArena(a, 0, 0);  // frees
+  if (d_arena_) cublox_free(d_arena_);
+  if (upload_stream_) cublox_stream_destroy(upload_stream_);
   sessions_.clear();  // frees the grids
 }
 
 double MapServer::nowMs() const {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_)
       .count();
+}
+
+// First fit over the free list; waits while no range fits.
+std::size_t MapServer::allocPoints(std::size_t bytes) {
+  std::unique_lock<std::mutex> lk(arena_mu_);
+  while (true) {
+    if (arena_closed_) throw std::runtime_error("cublox::MapServer: shutting down");
+    for (auto it = arena_free_.begin(); it != arena_free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const std::size_t off = it->first, size = it->second;
+      arena_free_.erase(it);
+      if (size > bytes) arena_free_[off + bytes] = size - bytes;
+      return off;
+    }
+    arena_cv_.wait(lk);
+  }
+}
+
+void MapServer::releasePoints(std::size_t off, std::size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(arena_mu_);
+    auto next = arena_free_.lower_bound(off);
+    if (next != arena_free_.end() && off + bytes == next->first) {  // merge right
+      bytes += next->second;
+      next = arena_free_.erase(next);
+    }
+    if (next != arena_free_.begin()) {  // merge left
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += bytes;
+        bytes = 0;
+      }
+    }
+    if (bytes) arena_free_[off] = bytes;
+  }
+  arena_cv_.notify_all();
 }
 
 int MapServer::openSession(const SessionSpec &spec) {
@@ -150,22 +185,43 @@ long long MapServer::submitScan(int sid, const float *xs, const float *ys, const
     throw std::invalid_argument("cublox::MapServer: scan size outside [0, max_points_per_scan]");
   }
   Request r;
+  r.submit_ms = nowMs();
   r.kind = Kind::SCAN;
   r.n = n;
-  r.pts.resize(3 * static_cast<std::size_t>(n));
-  std::memcpy(r.pts.data(), xs, sizeof(float) * n);
-  std::memcpy(r.pts.data() + n, ys, sizeof(float) * n);
-  std::memcpy(r.pts.data() + 2 * static_cast<std::size_t>(n), zs, sizeof(float) * n);
   std::copy(T, T + 12, r.T.begin());
   r.pos = base;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = sessions_.find(sid);
+    if (it == sessions_.end() || it->second.closing) {
+      throw std::invalid_argument("cublox::MapServer: scan for a closed or unknown session");
+    }
+  }
+  if (n > 0) {
+    r.arena_bytes = (3 * sizeof(float) * static_cast<std::size_t>(n) + 255) & ~std::size_t{255};
+    r.arena_off = allocPoints(r.arena_bytes);
+    r.d_pts = reinterpret_cast<float *>(d_arena_ + r.arena_off);
+    try {
+      std::lock_guard<std::mutex> lk(upload_mu_);
+      const std::size_t b = sizeof(float) * static_cast<std::size_t>(n);
+      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts, xs, b, upload_stream_));
+      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts + n, ys, b, upload_stream_));
+      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts + 2 * static_cast<std::size_t>(n), zs, b,
+                                       upload_stream_));
+      MS_CHECK(cublox_stream_synchronize(upload_stream_));
+    } catch (...) {
+      releasePoints(r.arena_off, r.arena_bytes);
+      throw;
+    }
+  }
   std::lock_guard<std::mutex> lk(mu_);
-  if (worker_error_) std::rethrow_exception(worker_error_);
   auto it = sessions_.find(sid);
-  if (it == sessions_.end() || it->second.closing) {
+  if (worker_error_ || it == sessions_.end() || it->second.closing) {
+    if (r.arena_bytes) releasePoints(r.arena_off, r.arena_bytes);
+    if (worker_error_) std::rethrow_exception(worker_error_);
     throw std::invalid_argument("cublox::MapServer: scan for a closed or unknown session");
   }
   r.id = next_id_++;
-  r.submit_ms = nowMs();
   it->second.queue.push_back(std::move(r));
   ++gen_;
   cv_work_.notify_all();
@@ -280,6 +336,9 @@ void MapServer::retireClosed() {
   for (auto it = sessions_.begin(); it != sessions_.end();) {
     Session &s = it->second;
     if (s.closing && s.busy == 0 && (s.queue.empty() || !s.admitted)) {
+      for (const Request &r : s.queue) {  // dropped with a never-admitted session
+        if (r.arena_bytes) releasePoints(r.arena_off, r.arena_bytes);
+      }
       if (s.admitted) {
         resident_ -= s.bytes;
         --admitted_count_;
@@ -355,18 +414,6 @@ bool MapServer::formTick(Tick &t) {
   return true;
 }
 
-void MapServer::upload(Tick &t) {
-  float *h = h_pts_[t.buf];
-  std::size_t off = 0;
-  for (auto &[sid, r] : t.scans) {
-    std::memcpy(h + off, r.pts.data(), sizeof(float) * r.pts.size());
-    off += r.pts.size();
-  }
-  if (off > 0) {
-    MS_CHECK(cublox_memcpy_h2d_async(d_pts_[t.buf], h, sizeof(float) * off, copy_stream_));
-  }
-}
-
 void MapServer::submitTick(Tick &t) {
   std::vector<GridScan> scans;
   std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> rc;
@@ -375,20 +422,17 @@ void MapServer::submitTick(Tick &t) {
     std::lock_guard<std::mutex> lk(mu_);  // a busy session's grid stays put
     for (auto &[sid, r] : t.scans) grids.push_back(sessions_.at(sid).grid.get());
   }
-  const float *d = d_pts_[t.buf];
-  std::size_t off = 0;
   std::size_t k = 0;
   for (auto &[sid, r] : t.scans) {
     OccupancyGrid *g = grids[k++];
     GridScan s;
     s.grid = g;
-    s.d_x = d + off;
-    s.d_y = d + off + r.n;
-    s.d_z = d + off + 2 * static_cast<std::size_t>(r.n);
+    s.d_x = r.d_pts;
+    s.d_y = r.d_pts ? r.d_pts + r.n : nullptr;
+    s.d_z = r.d_pts ? r.d_pts + 2 * static_cast<std::size_t>(r.n) : nullptr;
     s.n = r.n;
     s.sensor_frame = true;
     s.T = r.T;
-    off += r.pts.size();
     scans.push_back(s);
     rc.emplace_back(g, r.pos);
   }
@@ -397,29 +441,26 @@ void MapServer::submitTick(Tick &t) {
 
 // Grows (samples > 0) or frees (0, 0) the fetch output arena. Called
 // between ticks only, when no device work uses it.
-void MapServer::growFetchArena(std::size_t samples, std::size_t fetches) {
-  if (samples <= fcap_ && fetches <= fcnt_cap_ && samples > 0) return;
-  if (d_fpos_) cublox_free(d_fpos_);
-  if (d_fst_) cublox_free(d_fst_);
-  if (d_fcnt_) cublox_free(d_fcnt_);
-  if (h_fpos_) cublox_host_free(h_fpos_);
-  if (h_fst_) cublox_host_free(h_fst_);
-  if (h_fcnt_) cublox_host_free(h_fcnt_);
-  d_fpos_ = h_fpos_ = nullptr;
-  d_fst_ = h_fst_ = nullptr;
-  d_fcnt_ = h_fcnt_ = nullptr;
-  fcap_ = fcnt_cap_ = 0;
+void MapServer::growFetchArena(FetchArena &a, std::size_t samples, std::size_t fetches) {
+  if (samples <= a.cap && fetches <= a.cnt_cap && (samples > 0 || fetches > 0)) return;
+  if (a.d_pos) cublox_free(a.d_pos);
+  if (a.d_st) cublox_free(a.d_st);
+  if (a.d_cnt) cublox_free(a.d_cnt);
+  if (a.h_pos) cublox_host_free(a.h_pos);
+  if (a.h_st) cublox_host_free(a.h_st);
+  if (a.h_cnt) cublox_host_free(a.h_cnt);
+  a = FetchArena{};
   if (samples == 0 && fetches == 0) return;
   const std::size_t s = std::max<std::size_t>(samples, 1 << 16) * 2;  // headroom
   const std::size_t f = std::max<std::size_t>(fetches, 256) * 2;
-  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fpos_), 3 * sizeof(float) * s));
-  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fst_), s));
-  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_fcnt_), sizeof(unsigned) * f));
-  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_fpos_), 3 * sizeof(float) * s));
-  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_fst_), s));
-  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&h_fcnt_), sizeof(unsigned) * f));
-  fcap_ = s;
-  fcnt_cap_ = f;
+  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&a.d_pos), 3 * sizeof(float) * s));
+  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&a.d_st), s));
+  MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&a.d_cnt), sizeof(unsigned) * f));
+  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&a.h_pos), 3 * sizeof(float) * s));
+  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&a.h_st), s));
+  MS_CHECK(cublox_host_alloc(reinterpret_cast<void **>(&a.h_cnt), sizeof(unsigned) * f));
+  a.cap = s;
+  a.cnt_cap = f;
 }
 
 // Plans the tick's fetches against the grids as the tick leaves them (the
@@ -460,30 +501,40 @@ void MapServer::enqueueFetches(Tick &t) {
     j.off = total;
     total += j.cap;
   }
-  growFetchArena(total, t.fetches.size());
+  FetchArena &a = farena_[t.buf];
+  growFetchArena(a, total, t.fetches.size());
   void *s = batch_->stream();
-  MS_CHECK(cublox_fill_zero_async(d_fcnt_, t.fetches.size(), s));
+  MS_CHECK(cublox_fill_zero_async(a.d_cnt, t.fetches.size(), s));
   for (std::size_t k = 0; k < t.fetches.size(); ++k) {
     const FetchJob &j = t.jobs[k];
     if (!j.live || j.cap == 0) continue;
     MS_CHECK(cublox_launch_fetch_occupancy_around(&cfgs[k], grids[k]->deviceOccupancy(),
-                                                  d_fcnt_ + k, d_fpos_ + 3 * j.off,
-                                                  d_fst_ + j.off, j.cap, j.limit, s));
+                                                  a.d_cnt + k, a.d_pos + 3 * j.off,
+                                                  a.d_st + j.off, j.cap, j.limit, s));
   }
-  MS_CHECK(cublox_memcpy_d2h_async(h_fcnt_, d_fcnt_, sizeof(unsigned) * t.fetches.size(), s));
+  MS_CHECK(cublox_memcpy_d2h_async(a.h_cnt, a.d_cnt, sizeof(unsigned) * t.fetches.size(), s));
   if (total > 0) {
-    MS_CHECK(cublox_memcpy_d2h_async(h_fpos_, d_fpos_, 3 * sizeof(float) * total, s));
-    MS_CHECK(cublox_memcpy_d2h_async(h_fst_, d_fst_, total, s));
+    MS_CHECK(cublox_memcpy_d2h_async(a.h_pos, a.d_pos, 3 * sizeof(float) * total, s));
+    MS_CHECK(cublox_memcpy_d2h_async(a.h_st, a.d_st, total, s));
   }
 }
 
-void MapServer::completeTick(Tick &t) {
+// Blocks until the tick's device work (update, recenter, fetches) is done.
+void MapServer::waitTick(Tick &t) {
   if (!t.scans.empty()) {
     batch_->wait();
+    t.device_ms = batch_->lastUpdateMs() + batch_->lastRecenterMs();
   } else {
     MS_CHECK(cublox_stream_synchronize(batch_->stream()));
   }
-  const double done = nowMs();
+  t.done_ms = nowMs();
+}
+
+// Records a finished tick's completions and fetch results and frees its
+// points (runs while the next tick is on the device).
+void MapServer::finishTick(Tick &t) {
+  const double done = t.done_ms;
+  const FetchArena &a = farena_[t.buf];
   std::vector<Completion> out;
   for (auto &[sid, r] : t.scans) {
     out.push_back(Completion{r.id, sid, Kind::SCAN, r.submit_ms, t.start_ms, done, t.index,
@@ -499,19 +550,22 @@ void MapServer::completeTick(Tick &t) {
     const Request &r = t.fetches[k].second;
     const FetchJob &j = t.jobs[k];
     // OccupancyGrid.cpp: n = min(count, output cap, max_results, buffer).
-    const unsigned n = j.live ? std::min({h_fcnt_[k], j.limit, j.cap}) : 0u;
+    const unsigned n = j.live ? std::min({a.h_cnt[k], j.limit, j.cap}) : 0u;
     out.push_back(Completion{r.id, t.fetches[k].first, Kind::FETCH, r.submit_ms, t.start_ms,
                              done, t.index, static_cast<long long>(n)});
     if (!keep) continue;
     std::vector<OccupancyGrid::OccupancySample> v;
     v.reserve(n);
     for (unsigned i = 0; i < n; ++i) {
-      const float *p = h_fpos_ + 3 * (j.off + i);
+      const float *p = a.h_pos + 3 * (j.off + i);
       v.push_back(OccupancyGrid::OccupancySample{
           Eigen::Vector3f(p[0], p[1], p[2]),
-          static_cast<OccupancyGrid::VoxelState>(h_fst_[j.off + i])});
+          static_cast<OccupancyGrid::VoxelState>(a.h_st[j.off + i])});
     }
     kept.emplace_back(r.id, std::move(v));
+  }
+  for (auto &[sid, r] : t.scans) {
+    if (r.arena_bytes) releasePoints(r.arena_off, r.arena_bytes);
   }
   std::lock_guard<std::mutex> lk(mu_);
   for (auto &[sid, r] : t.scans) {
@@ -523,15 +577,15 @@ void MapServer::completeTick(Tick &t) {
   stats_.fetches += static_cast<long long>(t.fetches.size());
   if (!t.scans.empty()) {
     ++stats_.ticks;
-    stats_.device_ms += batch_->lastUpdateMs() + batch_->lastRecenterMs();
+    stats_.device_ms += t.device_ms;
   }
   for (auto &kv : kept) fetch_results_[kv.first] = std::move(kv.second);
   done_.insert(done_.end(), out.begin(), out.end());
 }
 
-// Software pipeline: the next tick is formed and its points copied while
-// the current one runs on the device; the current one then completes
-// (its fetches run after its update) before the next is launched.
+// The next tick is formed the moment the device finishes the current one,
+// from everything queued by then; the finished tick's completions are then
+// recorded while the new tick runs.
 void MapServer::run() {
   Tick prev;
   bool have_prev = false;
@@ -539,6 +593,7 @@ void MapServer::run() {
   long long seen = -1;
   try {
     while (true) {
+      if (have_prev) waitTick(prev);
       Tick cur;
       bool have_cur = false;
       {
@@ -555,33 +610,23 @@ void MapServer::run() {
           worker_busy_ = true;
         }
         have_cur = !paused_ && formTick(cur);
-        if (!have_cur && !have_prev) {
-          if (stop_) break;
-          continue;
-        }
         if (have_cur) cur.index = ticks_++;
+        if (!have_cur && !have_prev && stop_) break;
       }
       if (have_cur) {
         cur.buf = buf;
+        buf ^= 1;
         cur.start_ms = nowMs();
-        upload(cur);
+        if (!cur.scans.empty()) submitTick(cur);
+        enqueueFetches(cur);
       }
       if (have_prev) {
-        completeTick(prev);
+        finishTick(prev);
         have_prev = false;
       }
       if (have_cur) {
-        MS_CHECK(cublox_stream_synchronize(copy_stream_));
-        if (!cur.scans.empty()) {
-          submitTick(cur);
-          enqueueFetches(cur);
-          prev = std::move(cur);
-          have_prev = true;
-          buf ^= 1;
-        } else {
-          enqueueFetches(cur);
-          completeTick(cur);
-        }
+        prev = std::move(cur);
+        have_prev = true;
       }
       {
         std::lock_guard<std::mutex> lk(mu_);

@@ -60,7 +60,7 @@ cublox::MapServer &server_of(cublox_ms *h) { return *H->server; }
 
 extern "C" {
 
-cublox_ms *cublox_ms_create(const long long limits[8]) {
+cublox_ms *cublox_ms_create(const long long limits[9]) {
   cublox_ms *out = nullptr;
   guarded([&] {
     cublox::ServerLimits l;
@@ -72,6 +72,7 @@ cublox_ms *cublox_ms_create(const long long limits[8]) {
     l.max_tiles_per_tick = static_cast<int>(limits[5]);
     l.max_segments_per_ray = static_cast<int>(limits[6]);
     l.graph = limits[7] != 0;
+    if (limits[8] > 0) l.point_arena_bytes = static_cast<std::size_t>(limits[8]);
     auto *hd = new Handle;
     try {
       hd->server = std::make_unique<MS>(l);

@@ -29,12 +29,14 @@ class MapServer:
     def __init__(self, *, hbm_budget_bytes: int = 200 << 30, max_sessions: int = 256,
                  max_points_per_scan: int = 131072, max_points_per_tick: int = 1 << 21,
                  max_grids_per_tick: int = 256, max_tiles_per_tick: int = 1 << 21,
-                 max_segments_per_ray: int = 96, graph: bool = True):
+                 max_segments_per_ray: int = 96, graph: bool = True,
+                 point_arena_bytes: int = 0):
         L = _ffi.host()
-        lim = (C.c_longlong * 8)(int(hbm_budget_bytes), int(max_sessions),
+        lim = (C.c_longlong * 9)(int(hbm_budget_bytes), int(max_sessions),
                                  int(max_points_per_scan), int(max_points_per_tick),
                                  int(max_grids_per_tick), int(max_tiles_per_tick),
-                                 int(max_segments_per_ray), int(bool(graph)))
+                                 int(max_segments_per_ray), int(bool(graph)),
+                                 int(point_arena_bytes))
         h = L.cublox_ms_create(lim)
         if not h:
             raise RuntimeError(L.cublox_og_last_error().decode())
@@ -154,11 +156,14 @@ class MapServer:
 
     def run_load(self, sids: Sequence[int], templates: Sequence[Tuple], *,
                  scan_rate_hz: float, duration_s: float, fetch_ratio: float = 0.2,
-                 fetch_radius: float = 1.5, speed: float = 0.5, seed: int = 1) -> int:
+                 fetch_radius: float = 1.5, fetch_radius_max: float = 0.0,
+                 speed: float = 0.5, seed: int = 1, clients: int = 8) -> Tuple[int, float]:
         """Open-loop Poisson load from a native client thread (this call):
         templates[k] = (xs, ys, zs, T 3x4, base) of session sids[k], replayed
-        with the pose moved `speed` m along x per scan. Returns the number of
-        requests submitted; drain() waits for them."""
+        with the pose moved `speed` m along x per scan; fetch radii uniform in
+        [fetch_radius, fetch_radius_max]; `clients` submitting threads share
+        the sessions. Returns (requests submitted, largest submission delay
+        behind schedule in ms); drain() waits for the requests."""
         k = len(sids)
         cols = [tuple(_f32(c) for c in t[:3]) for t in templates]
         ptr = C.c_void_p * k
@@ -172,17 +177,20 @@ class MapServer:
         class Cfg(C.Structure):
             _fields_ = [("scan_rate_hz", C.c_double), ("fetch_ratio", C.c_double),
                         ("duration_s", C.c_double), ("fetch_radius", C.c_float),
-                        ("speed", C.c_float), ("seed", C.c_ulonglong)]
+                        ("speed", C.c_float), ("seed", C.c_ulonglong),
+                        ("fetch_radius_max", C.c_float), ("clients", C.c_int)]
 
         cfg = Cfg(float(scan_rate_hz), float(fetch_ratio), float(duration_s),
-                  float(fetch_radius), float(speed), int(seed))
+                  float(fetch_radius), float(speed), int(seed), float(fetch_radius_max),
+                  int(clients))
         out = C.c_longlong()
+        lag = C.c_double()
         self._check(self._L.cublox_ms_load_run(
             self._h, (C.c_int * k)(*map(int, sids)), k, C.cast(xs, C.c_void_p),
             C.cast(ys, C.c_void_p), C.cast(zs, C.c_void_p), n,
             T.ctypes.data_as(_ffi.f32p), base.ctypes.data_as(_ffi.f32p), C.byref(cfg),
-            C.byref(out)))
-        return out.value
+            C.byref(out), C.byref(lag)))
+        return out.value, lag.value
 
     def keep_fetch_results(self, on: bool) -> None:
         self._check(self._L.cublox_ms_keep_fetch_results(self._h, int(bool(on))))

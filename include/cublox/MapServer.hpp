@@ -5,13 +5,15 @@
 //   * admission: a session is admitted while its device bytes fit the HBM
 //     budget and a session slot is free; otherwise it waits (FIFO) until
 //     closed sessions free room. Requests to a waiting session queue up.
-//   * continuous batching: a worker thread forms each tick from the requests
-//     ready at that moment -- at most one scan per session, in session
-//     round-robin order, within the tick's point, grid, tile and segment
-//     caps -- and runs it as one GridBatch tick (update, then the recenter
-//     of every scanned grid toward its base position).
-//   * overlap: the next tick's points are packed into pinned memory and
-//     copied to the device on a copy stream while the current tick runs.
+//   * upload at submission: a scan's points go to a device arena in the
+//     submitting thread, while earlier ticks run.
+//   * continuous batching: the moment a tick finishes on the device, a
+//     worker thread forms the next from the requests queued then -- at most
+//     one scan per session, in session round-robin order, within the tick's
+//     point, grid, tile and segment caps -- and launches it as one GridBatch
+//     tick (update, then the recenter of every scanned grid toward its base
+//     position); the finished tick's completions are recorded while the
+//     next one runs.
 //   * fetches: a fetch request runs once every scan its session submitted
 //     before it has been applied, so it sees exactly those scans.
 //
@@ -57,6 +59,7 @@ struct ServerLimits {
   int max_tiles_per_tick{1 << 21};   // sum of the tick's grids' tile counts
   int max_segments_per_ray{96};      // sessions above it are refused
   bool graph{true};                  // replay the tick as a HIP graph
+  std::size_t point_arena_bytes{4ull << 30};  // queued scans' device points
 };
 
 class MapServer {
@@ -95,7 +98,9 @@ public:
   void closeSession(int sid);
   bool admitted(int sid) const;
 
-  // Thread-safe; the points are copied. T is row-major 3x4 sensor->world.
+  // Thread-safe; the points are copied to the device before the call
+  // returns (it waits while the point arena is full). T is row-major 3x4
+  // sensor->world.
   long long submitScan(int sid, const float *xs, const float *ys,
                        const float *zs, int n, const float T[12],
                        const Eigen::Vector3f &base);
@@ -126,7 +131,8 @@ private:
     long long id;
     Kind kind;
     double submit_ms;
-    std::vector<float> pts;  // x block, y block, z block
+    float *d_pts{nullptr};   // device: x block, y block, z block
+    std::size_t arena_off{0}, arena_bytes{0};
     int n{0};
     std::array<float, 12> T{};
     Eigen::Vector3f pos{Eigen::Vector3f::Zero()};  // base or fetch centre
@@ -154,32 +160,43 @@ private:
     std::vector<std::pair<int, Request>> scans;
     std::vector<std::pair<int, Request>> fetches;
     std::vector<FetchJob> jobs;  // one per fetch, planned at enqueue
-    double start_ms{0.0};
-    int buf{0};
+    double start_ms{0.0}, done_ms{0.0};
+    double device_ms{0.0};
+    int buf{0};                  // fetch output arena
+  };
+  struct FetchArena {            // fetch outputs of one tick
+    float *d_pos{nullptr}, *h_pos{nullptr};
+    unsigned char *d_st{nullptr}, *h_st{nullptr};
+    unsigned *d_cnt{nullptr}, *h_cnt{nullptr};
+    std::size_t cap{0}, cnt_cap{0};
   };
 
   void run();
   void admitWaiting();                       // worker thread, lock held
   void retireClosed();                       // worker thread, lock held
   bool formTick(Tick &t);                    // lock held
-  void upload(Tick &t);
   void submitTick(Tick &t);
   void enqueueFetches(Tick &t);
-  void completeTick(Tick &t);
-  void growFetchArena(std::size_t samples, std::size_t fetches);
+  void waitTick(Tick &t);
+  void finishTick(Tick &t);
+  void growFetchArena(FetchArena &a, std::size_t samples, std::size_t fetches);
+  std::size_t allocPoints(std::size_t bytes);  // blocks while full
+  void releasePoints(std::size_t off, std::size_t bytes);
   bool admittable(const Session &s) const;  // lock held
   bool quiescent() const;                    // lock held
 
   ServerLimits lim_;
   std::unique_ptr<GridBatch> batch_;
-  void *copy_stream_{nullptr};
-  float *h_pts_[2]{nullptr, nullptr};  // pinned
-  float *d_pts_[2]{nullptr, nullptr};
-  // Fetch outputs of one tick (device, and pinned host copies).
-  float *d_fpos_{nullptr}, *h_fpos_{nullptr};
-  unsigned char *d_fst_{nullptr}, *h_fst_{nullptr};
-  unsigned *d_fcnt_{nullptr}, *h_fcnt_{nullptr};
-  std::size_t fcap_{0}, fcnt_cap_{0};
+  FetchArena farena_[2];
+  // Device arena of queued scans' points: first-fit free list (offset ->
+  // size), its own lock, and the stream the submitting threads copy on.
+  char *d_arena_{nullptr};
+  std::map<std::size_t, std::size_t> arena_free_;
+  std::mutex arena_mu_;
+  std::condition_variable arena_cv_;
+  bool arena_closed_{false};
+  void *upload_stream_{nullptr};
+  std::mutex upload_mu_;
 
   mutable std::mutex mu_;
   std::condition_variable cv_work_, cv_idle_;

@@ -20,9 +20,10 @@ typedef struct cublox_ms cublox_ms;
 /* limits: [0] HBM budget bytes for sessions, [1] max admitted sessions,
  * [2] max points per scan, [3] max points per tick, [4] max grids per
  * tick, [5] max tiles per tick, [6] max segments per ray (sessions whose
- * segment bound is larger are refused), [7] graph replay (0/1). NULL on
+ * segment bound is larger are refused), [7] graph replay (0/1), [8] bytes
+ * of the device arena holding queued scans' points (0: 4 GiB). NULL on
  * failure. */
-cublox_ms *cublox_ms_create(const long long limits[8]);
+cublox_ms *cublox_ms_create(const long long limits[9]);
 void cublox_ms_destroy(cublox_ms *h);
 /* Device bytes a session of this half size is charged (4 B per voxel plus
  * the fetch output reserve). */
@@ -70,14 +71,17 @@ int cublox_ms_origin(cublox_ms *h, int sid, int origin_i[3]);
  * workspace bytes, admitted, waiting, ms since start. */
 int cublox_ms_stats(cublox_ms *h, double out[10]);
 
-/* Open-loop load (benchmark client): Poisson scan arrivals at scan_rate_hz
- * in total, each to a uniformly drawn session (so every session sees a
- * Poisson stream), plus independent Poisson fetch arrivals at fetch_ratio
- * times that rate. Session k replays its template scan (xs[k], ys[k],
+/* Open-loop load (benchmark client): `clients` threads, each owning every
+ * clients-th session, each with Poisson scan arrivals at its share of
+ * scan_rate_hz, each arrival to a uniformly drawn session of its own (so
+ * every session sees a Poisson stream), plus independent Poisson fetch
+ * arrivals at fetch_ratio times that rate. Session k replays its template scan (xs[k], ys[k],
  * zs[k], n[k], T[12k..], base[3k..]) with the pose moved by `speed` m
- * along x per scan; fetches ask for fetch_radius around the current sensor
- * position. Returns once the last arrival of duration_s is submitted (the
- * caller drains); *submitted = requests submitted.                       */
+ * along x per scan; fetches ask for a radius uniform in [fetch_radius,
+ * fetch_radius_max] (fetch_radius when the max is not larger) around the
+ * current sensor position. Returns once the last arrival of duration_s is
+ * submitted (the caller drains); *submitted = requests submitted and
+ * *max_lag_ms the largest delay of a submission behind its arrival time. */
 typedef struct {
   double scan_rate_hz;
   double fetch_ratio;
@@ -85,13 +89,15 @@ typedef struct {
   float fetch_radius;
   float speed;
   unsigned long long seed;
+  float fetch_radius_max;
+  int clients;
 } cublox_ms_load_cfg;
 
 int cublox_ms_load_run(cublox_ms *h, const int *sids, int n_sessions,
                        const float *const *xs, const float *const *ys,
                        const float *const *zs, const int *n, const float *T,
                        const float *base, const cublox_ms_load_cfg *cfg,
-                       long long *submitted);
+                       long long *submitted, double *max_lag_ms);
 
 #ifdef __cplusplus
 }

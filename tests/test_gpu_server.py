@@ -35,6 +35,7 @@ def _server(cb, **kw):
     kw.setdefault("max_points_per_tick", 1 << 16)
     kw.setdefault("max_tiles_per_tick", 1 << 16)
     kw.setdefault("max_segments_per_ray", 96)
+    kw.setdefault("point_arena_bytes", 64 << 20)
     return cb.MapServer(**kw)
 
 
@@ -198,5 +199,25 @@ def test_server_queued_scans_of_one_session(cb, gpu_ready):
     assert len(set(comp["tick"].tolist())) == len(reqs)
     g, _ = _alone(cb, 2, reqs)
     assert_occ_bitexact(srv.occupancy(sid), g.occupancy(), "queued")
+    g.close()
+    srv.close()
+
+
+def test_server_small_point_arena(cb, gpu_ready):
+    """A point arena for about two scans: submissions wait for finished
+    ticks to free their ranges (first-fit with coalescing), and the result
+    still matches the session run alone."""
+    reqs = [r for r in _requests(cb)[4] if r[0] == "scan"]
+    n = max(x.n for _, x in reqs)
+    srv = _server(cb, max_points_per_scan=n, max_points_per_tick=4 * n,
+                  point_arena_bytes=2 * (12 * n + 256) + 256)
+    half, res, ctr, thr, rng, *_ = SESSIONS[4]
+    sid = srv.open_session(half, res, ctr, thr, rng, cb.log_odds())
+    for _, x in reqs:
+        srv.submit_scan(sid, x.xs, x.ys, x.zs, x.T, x.base)
+    srv.drain()
+    assert len(srv.completions()) == len(reqs)
+    g, _ = _alone(cb, 4, reqs)
+    assert_occ_bitexact(srv.occupancy(sid), g.occupancy(), "small arena")
     g.close()
     srv.close()
