@@ -313,17 +313,23 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
         f"({time.perf_counter() - t:.1f} s), resident {st1['resident_bytes'] / 2**30:.1f} GiB")
     levels = []
     for f in loads:
+        st_a = srv.stats()
         rate = f * sat
         n_sub, lag = srv.run_load(sids, templates, scan_rate_hz=rate, duration_s=seconds,
                                   fetch_ratio=0.2, fetch_radius=0.5, fetch_radius_max=3.0,
                                   speed=0.5, seed=17, clients=8)
         srv.drain()
         c = srv.completions()
+        st_b = srv.stats()
+        nt = max(st_b["ticks"] - st_a["ticks"], 1)
         scans = c[c["kind"] == 0]
         span = (c["done_ms"].max() - c["submit_ms"].min()) * 1e-3
         levels.append(dict(load=f, offered_scans_per_s=round(rate, 1),
                            achieved_scans_per_s=round(len(scans) / span, 1),
                            requests=int(n_sub), client_max_lag_ms=round(lag, 3),
+                           ticks=nt, scans_per_tick=round(len(scans) / nt, 1),
+                           device_ms_per_tick=round((st_b["device_ms"] - st_a["device_ms"]) / nt, 3),
+                           host_ms_between_ticks=round((st_b["launch_ms"] - st_a["launch_ms"]) / nt, 3),
                            scan_latency=_lat(scans),
                            fetch_latency=_lat(c[c["kind"] == 1])))
         log(f"served: load {f}: {levels[-1]}")

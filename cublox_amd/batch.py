@@ -18,7 +18,7 @@ from . import _ffi
 class GridBatch:
     def __init__(self, *, max_grids: int, max_points: int, max_tiles: int,
                  max_segments_per_ray: int, max_near_steps: int = 12,
-                 min_chunk: int = 256, max_slabs: int = 16384,
+                 min_chunk: int = 256, max_slabs: int = 65536,
                  max_regions: int = 0):
         L = _ffi.host()
         lim = (C.c_int * 8)(int(max_grids), int(max_points), int(max_tiles),

@@ -619,6 +619,11 @@ void MapServer::run() {
         cur.start_ms = nowMs();
         if (!cur.scans.empty()) submitTick(cur);
         enqueueFetches(cur);
+        cur.launched_ms = nowMs();
+        if (have_prev) {
+          std::lock_guard<std::mutex> lk(mu_);
+          stats_.launch_ms += cur.launched_ms - prev.done_ms;
+        }
       }
       if (have_prev) {
         finishTick(prev);

@@ -941,6 +941,7 @@ constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
 constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
 constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
 constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
+constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
 
 
 // Cost class of a work item (its record count), larger classes first.
@@ -1209,70 +1210,17 @@ __device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt
   }
 }
 
-constexpr int kRound = 1024;  // records ordered per round, 4 per thread
-constexpr int kLenClasses = 64;
-
-struct ReplayScratch {  // LDS of the length ordering
-  int hist[kLenClasses];
-  unsigned short order[kRound];
-};
-
 // Accumulates records [beg, end) of a tile into packed LDS counters
-// (op | hit << 16): each segment replays its n voxels from its entry state
-// (the same crossing parameters and tie-break as the walk). Records are
-// taken in rounds of kRound, counting-sorted by length (longest first) so
-// the 64 lanes of a wavefront replay segments of similar length. A call
-// covers at most kPass records and a record visits a voxel at most once,
-// so neither field carries into the other.
+// (op | hit << 16): each record replays its n voxels from its entry state
+// (the same crossing parameters and tie-break as the walk), one record per
+// thread at a time. A call covers at most kPass records and a record visits
+// a voxel at most once, so neither field carries into the other.
 __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
-                                                int end, unsigned *s_cnt,
-                                                ReplayScratch &rs) {
-  const int t = threadIdx.x;
-  constexpr int kPer = kRound / kApplyThreads;
-  for (int r0 = beg; r0 < end; r0 += kRound) {
-    const int m = min(kRound, end - r0);
-    if (m <= kApplyThreads) {
-      // One record per thread at most: the round's latency is its longest
-      // record whatever the order.
-      if (t < m) replay_record(w.sorted[r0 + t], s_cnt);
-      __syncthreads();
-      continue;
-    }
-    if (t < kLenClasses) rs.hist[t] = 0;
-    __syncthreads();
-    int cls[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int k = t + j * kApplyThreads;
-      cls[j] = -1;
-      if (k < m) {
-        const unsigned meta = w.sorted[r0 + k].meta;
-        const int n = (meta & kMetaEndpoint) ? 0 : static_cast<int>((meta >> kMetaN) & 0x7Fu);
-        cls[j] = kLenClasses - 1 - min(n, kLenClasses - 1);
-        atomicAdd(rs.hist + cls[j], 1);
-      }
-    }
-    __syncthreads();
-    if (t < 64) {  // one wavefront: exclusive scan of the classes
-      const int v = rs.hist[t];
-      rs.hist[t] = wave_scan_incl(v) - v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      if (cls[j] >= 0) {
-        const int pos = atomicAdd(rs.hist + cls[j], 1);
-        rs.order[pos] = static_cast<unsigned short>(t + j * kApplyThreads);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int pos = t + j * kApplyThreads;
-      if (pos < m) replay_record(w.sorted[r0 + rs.order[pos]], s_cnt);
-    }
-    __syncthreads();
+                                                int end, unsigned *s_cnt) {
+  for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
+    replay_record(w.sorted[k], s_cnt);
   }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void lds_zero(uint4 *s4) {
@@ -1327,30 +1275,32 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
   }
 }
 
-// Work items in order -- first the unsplit hot tiles, the longest items,
-// then the plain items in decreasing cost order -- one per workgroup while
-// they last (then strided), so the hardware dispatcher balances them.
+// Work items in decreasing cost order, one per workgroup while they last
+// (then strided), so the hardware dispatcher balances them. kBig = false:
+// the plain items (split chunks and whole tiles); kBig = true: the unsplit
+// hot tiles, in passes of kPass records summed in registers (a separate
+// kernel, so the common one keeps a register budget for 5 workgroups per
+// CU).
 // A plain item accumulates its tile in LDS and folds it; a chunk of a hot
 // tile stores its packed counts to its scratch slab (hot_fold sums a
 // tile's slabs); an unsplit hot tile accumulates in passes of kPass
 // records summed in registers. Empty items do nothing.
-__global__ void __launch_bounds__(kApplyThreads, 4)
+template <bool kBig>
+__global__ void __launch_bounds__(kApplyThreads)
     tile_apply_kernel(const Batch b, Workspace w) {
   __shared__ uint4 s_cnt4[kTileVox / 4];
-  __shared__ ReplayScratch s_rs;
   unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
-  const int n_big = w.misc[4];
-  const int n_all = n_big + w.misc[0];
+  constexpr bool big = kBig;
+  const int n_all = kBig ? w.misc[4] : w.misc[0];
   for (int it = blockIdx.x; it < n_all; it += gridDim.x) {
-    const bool big = it < n_big;  // uniform
-    const Item itm = w.items[big ? w.big[it] : it - n_big];
-    if (itm.tile < 0 || (!big && itm.slab == -2)) continue;  // uniform
+    const Item itm = w.items[kBig ? w.big[it] : it];
+    if (itm.tile < 0 || (!kBig && itm.slab == -2)) continue;  // uniform
     const int end = static_cast<int>(
         min(static_cast<long long>(itm.end), w.seg_capacity));
     if (!big && itm.slab >= 0) {
       lds_zero(s_cnt4);
       __syncthreads();
-      tile_accumulate(w, itm.beg, end, s_cnt, s_rs);
+      tile_accumulate(w, itm.beg, end, s_cnt);
       uint4 *slab = reinterpret_cast<uint4 *>(
           w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
       for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
@@ -1368,7 +1318,7 @@ __global__ void __launch_bounds__(kApplyThreads, 4)
       // One pass: the packed 16-bit counts fold as they are.
       lds_zero(s_cnt4);
       __syncthreads();
-      tile_accumulate(w, itm.beg, end, s_cnt, s_rs);
+      tile_accumulate(w, itm.beg, end, s_cnt);
       unsigned c[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) c[j] = s_cnt[j * kApplyThreads + threadIdx.x];
@@ -1386,7 +1336,7 @@ __global__ void __launch_bounds__(kApplyThreads, 4)
       for (int p0 = itm.beg; p0 < end; p0 += kPass) {
         lds_zero(s_cnt4);
         __syncthreads();
-        tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt, s_rs);
+        tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt);
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
           const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
@@ -1751,8 +1701,11 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
   const int apply_blocks = std::min(L.max_items, 1 << 16);
-  tile_apply_kernel<<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  tile_apply_kernel<false><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
   st = check_launch("tile_apply_kernel");
+  if (st != CUBLOX_OK) return st;
+  tile_apply_kernel<true><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  st = check_launch("tile_apply_kernel<big>");
   if (st != CUBLOX_OK) return st;
   const long long hot_blocks =
       std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
@@ -1775,7 +1728,7 @@ Caps batch_caps(const cublox_batch_limits &lim) {
   c.max_cube = 2 * std::max(1, std::min(lim.max_near_steps, kMaxNearK)) - 1;
   c.max_near = c.max_grids * c.max_cube * c.max_cube * c.max_cube;
   c.chunk = std::max(kMinChunk, std::min(lim.chunk > 0 ? lim.chunk : kChunk, 8192));
-  c.max_slabs = lim.max_slabs > 0 ? lim.max_slabs : 16384;
+  c.max_slabs = lim.max_slabs > 0 ? lim.max_slabs : 65536;
   return c;
 }
 

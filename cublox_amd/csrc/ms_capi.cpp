@@ -199,7 +199,7 @@ int cublox_ms_origin(cublox_ms *h, int sid, int origin_i[3]) {
   });
 }
 
-int cublox_ms_stats(cublox_ms *h, double out[10]) {
+int cublox_ms_stats(cublox_ms *h, double out[11]) {
   return guarded([&] {
     const MS::Stats s = H->server->stats();
     out[0] = static_cast<double>(s.ticks);
@@ -212,6 +212,7 @@ int cublox_ms_stats(cublox_ms *h, double out[10]) {
     out[7] = s.admitted;
     out[8] = s.waiting;
     out[9] = H->server->nowMs();
+    out[10] = s.launch_ms;
   });
 }
 

@@ -220,9 +220,9 @@ class MapServer:
         return o[0], o[1], o[2]
 
     def stats(self) -> dict:
-        out = (C.c_double * 10)()
+        out = (C.c_double * 11)()
         self._check(self._L.cublox_ms_stats(self._h, out))
         keys = ("ticks", "scans", "fetches", "points", "device_ms", "resident_bytes",
-                "workspace_bytes", "admitted", "waiting", "now_ms")
-        return {k: (out[i] if k in ("device_ms", "now_ms") else int(out[i]))
+                "workspace_bytes", "admitted", "waiting", "now_ms", "launch_ms")
+        return {k: (out[i] if k in ("device_ms", "now_ms", "launch_ms") else int(out[i]))
                 for i, k in enumerate(keys)}

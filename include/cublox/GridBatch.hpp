@@ -30,7 +30,7 @@ struct BatchLimits {
   int max_segments_per_ray{96};     // max over grids (segmentBound)
   int max_near_steps{12};
   int min_chunk{256};               // smallest hot-tile chunk any tick uses
-  int max_slabs{16384};             // 32 KB scratch slabs
+  int max_slabs{65536};             // 32 KB scratch slabs
   int max_regions{0};               // recenter regions per tick (0: 6 per grid)
 };
 

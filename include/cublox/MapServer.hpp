@@ -77,7 +77,9 @@ public:
 
   struct Stats {
     long long ticks{0}, scans{0}, fetches{0}, points{0};
-    double device_ms{0.0};      // summed tick device time
+    double device_ms{0.0};      // summed tick device time (HIP events)
+    double launch_ms{0.0};      // summed host time from a tick's end on the
+                                // device to the next tick's launch
     std::size_t resident_bytes{0}, workspace_bytes{0};
     int admitted{0}, waiting{0};
   };
@@ -160,7 +162,7 @@ private:
     std::vector<std::pair<int, Request>> scans;
     std::vector<std::pair<int, Request>> fetches;
     std::vector<FetchJob> jobs;  // one per fetch, planned at enqueue
-    double start_ms{0.0}, done_ms{0.0};
+    double start_ms{0.0}, done_ms{0.0}, launched_ms{0.0};
     double device_ms{0.0};
     int buf{0};                  // fetch output arena
   };

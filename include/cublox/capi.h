@@ -337,7 +337,7 @@ typedef struct {
   int chunk;                /* smallest records per hot-tile work item
                                any launch will use (0: 1024)             */
   int max_slabs;            /* hot-tile scratch slabs, 32 KB each (0:
-                               16384); hot tiles past them run unsplit   */
+                               65536); hot tiles past them run unsplit   */
 } cublox_batch_limits;
 
 typedef struct {

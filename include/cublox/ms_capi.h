@@ -68,8 +68,9 @@ int cublox_ms_fetch_result(cublox_ms *h, long long id, float *pos,
 int cublox_ms_copy_occupancy(cublox_ms *h, int sid, float *dst);
 int cublox_ms_origin(cublox_ms *h, int sid, int origin_i[3]);
 /* out: ticks, scans, fetches, points, device ms, resident bytes,
- * workspace bytes, admitted, waiting, ms since start. */
-int cublox_ms_stats(cublox_ms *h, double out[10]);
+ * workspace bytes, admitted, waiting, ms since start, host ms between a
+ * tick's end on the device and the next tick's launch (summed). */
+int cublox_ms_stats(cublox_ms *h, double out[11]);
 
 /* Open-loop load (benchmark client): `clients` threads, each owning every
  * clients-th session, each with Poisson scan arrivals at its share of
