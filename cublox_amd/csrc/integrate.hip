@@ -133,7 +133,7 @@ struct Workspace {
   int *seg_count;     // n: segments written by each ray
   int *seg_tile;      // n * max_seg: tile of each slot's segment
   Segment *segs;      // n * max_seg, per-ray slot ranges
-  Segment *sorted;    // n * max_seg, tile order
+  unsigned *order;    // n * max_seg: record slots in tile order
   int *tile_count;    // n_tiles (zero between launches)
   int *tile_cursor;   // n_tiles
   Item *items;        // max_items, costliest first
@@ -913,16 +913,10 @@ __global__ void __launch_bounds__(kBinThreads)
   for (int f = t; f < total; f += kBinThreads) {
     const long long slot = slot_of(f);
     const int tile = w.seg_tile[slot];
-    const uint4 *src = reinterpret_cast<const uint4 *>(w.segs + slot);
-    const uint4 lo = src[0], hi = src[1];  // the 32 B record, in registers
     const int e = lds_hash_slot(s_key, tile, false);
     const long long pos = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
                                  : atomicAdd(w.tile_cursor + tile, 1);
-    if (pos < w.seg_capacity) {
-      uint4 *dst = reinterpret_cast<uint4 *>(w.sorted + pos);
-      dst[0] = lo;
-      dst[1] = hi;
-    }
+    if (pos < w.seg_capacity) w.order[pos] = static_cast<unsigned>(slot);
   }
 }
 
@@ -1218,7 +1212,7 @@ __device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt
 __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
                                                 int end, unsigned *s_cnt) {
   for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
-    replay_record(w.sorted[k], s_cnt);
+    replay_record(w.segs[w.order[k]], s_cnt);
   }
   __syncthreads();
 }
@@ -1520,6 +1514,9 @@ struct Layout {
 
 constexpr int kMinChunk = 256;
 
+// Record slots are indexed by 32-bit unsigned integers in tile order.
+bool caps_fit(const Caps &c) { return c.seg_capacity <= 0xFFFFFFFFll; }
+
 Layout make_layout(const Caps &c) {
   Layout L{};
   L.cap = c;
@@ -1538,7 +1535,7 @@ Layout make_layout(const Caps &c) {
   take(L.off_count, static_cast<size_t>(c.max_rays) * 4);
   take(L.off_stile, static_cast<size_t>(c.seg_capacity) * 4);
   take(L.off_segs, static_cast<size_t>(c.seg_capacity) * sizeof(Segment));
-  take(L.off_sorted, static_cast<size_t>(c.seg_capacity) * sizeof(Segment));
+  take(L.off_sorted, static_cast<size_t>(c.seg_capacity) * sizeof(unsigned));
   take(L.off_tcount, static_cast<size_t>(c.max_tiles) * 4);
   take(L.off_tcur, static_cast<size_t>(c.max_tiles) * 4);
   take(L.off_items, static_cast<size_t>(L.max_items) * sizeof(Item));
@@ -1564,7 +1561,7 @@ Workspace bind(const Layout &L, void *base) {
   w.seg_count = reinterpret_cast<int *>(b + L.off_count);
   w.seg_tile = reinterpret_cast<int *>(b + L.off_stile);
   w.segs = reinterpret_cast<Segment *>(b + L.off_segs);
-  w.sorted = reinterpret_cast<Segment *>(b + L.off_sorted);
+  w.order = reinterpret_cast<unsigned *>(b + L.off_sorted);
   w.tile_count = reinterpret_cast<int *>(b + L.off_tcount);
   w.tile_cursor = reinterpret_cast<int *>(b + L.off_tcur);
   w.items = reinterpret_cast<Item *>(b + L.off_items);
@@ -1756,7 +1753,8 @@ extern "C" {
 size_t cublox_integrate_workspace_bytes(const cublox_raycast_cfg *cfg,
                                         int max_points, int near_steps) {
   if (!cfg || near_steps < 1 || near_steps > kMaxNearK) return 0;
-  return make_layout(single_caps(*cfg, max_points, near_steps)).total;
+  const Caps c = single_caps(*cfg, max_points, near_steps);
+  return caps_fit(c) ? make_layout(c).total : 0;
 }
 
 int cublox_integrate_workspace_init(void *d_ws, size_t bytes, cublox_stream s) {
@@ -1778,7 +1776,12 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
   if (n > max_points) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
   }
-  const Layout L = make_layout(single_caps(*cfg, max_points, near_steps));
+  const Caps caps = single_caps(*cfg, max_points, near_steps);
+  if (!caps_fit(caps)) {
+    return set_error(CUBLOX_ERR_CAPACITY,
+                     "integrate: max_points x segment bound above 2^32 slots");
+  }
+  const Layout L = make_layout(caps);
   if (!d_ws || ws_bytes < L.total) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
   }
@@ -1807,7 +1810,8 @@ constexpr int kMaxBatchTiles = kScanThreads * kScanTilesPerBlock;
 
 size_t cublox_integrate_batch_workspace_bytes(const cublox_batch_limits *lim) {
   if (!lim || lim->max_tiles > kMaxBatchTiles) return 0;
-  return make_layout(batch_caps(*lim)).total;
+  const Caps c = batch_caps(*lim);
+  return caps_fit(c) ? make_layout(c).total : 0;
 }
 
 size_t cublox_integrate_batch_staging_bytes(const cublox_batch_limits *lim) {
@@ -1825,6 +1829,10 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
     return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: max_tiles above 4194304");
   }
   const Caps c = batch_caps(*lim);
+  if (!caps_fit(c)) {
+    return set_error(CUBLOX_ERR_CAPACITY,
+                     "integrate_batch: max_points x max_segments_per_ray above 2^32 slots");
+  }
   if (n_grids > c.max_grids) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate_batch: too many grids");
   }
@@ -1880,7 +1888,7 @@ int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
                                          size_t ws_bytes,
                                          const cublox_batch_limits *lim,
                                          cublox_stream s) {
-  if (!lim || !h_staging || lim->max_tiles > kMaxBatchTiles) {
+  if (!lim || !h_staging || lim->max_tiles > kMaxBatchTiles || !caps_fit(batch_caps(*lim))) {
     return set_error(CUBLOX_ERR_INVALID, "integrate_batch: bad arguments");
   }
   const Layout L = make_layout(batch_caps(*lim));
