@@ -1209,9 +1209,21 @@ __device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt
 // (the same crossing parameters and tie-break as the walk), one record per
 // thread at a time. A call covers at most kPass records and a record visits
 // a voxel at most once, so neither field carries into the other.
+__device__ __forceinline__ void lds_zero(uint4 *s4);
+
+// Clears the tile's counters and accumulates records [beg, end) into them.
+// Each thread's first record is gathered before the clear, so that load's
+// latency overlaps it.
 __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
-                                                int end, unsigned *s_cnt) {
-  for (int k = beg + static_cast<int>(threadIdx.x); k < end; k += kApplyThreads) {
+                                                int end, uint4 *s_cnt4) {
+  unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
+  const int k0 = beg + static_cast<int>(threadIdx.x);
+  Segment first{};
+  if (k0 < end) first = w.segs[w.order[k0]];
+  lds_zero(s_cnt4);
+  __syncthreads();
+  if (k0 < end) replay_record(first, s_cnt);
+  for (int k = k0 + kApplyThreads; k < end; k += kApplyThreads) {
     replay_record(w.segs[w.order[k]], s_cnt);
   }
   __syncthreads();
@@ -1292,9 +1304,7 @@ __global__ void __launch_bounds__(kApplyThreads)
     const int end = static_cast<int>(
         min(static_cast<long long>(itm.end), w.seg_capacity));
     if (!big && itm.slab >= 0) {
-      lds_zero(s_cnt4);
-      __syncthreads();
-      tile_accumulate(w, itm.beg, end, s_cnt);
+      tile_accumulate(w, itm.beg, end, s_cnt4);
       uint4 *slab = reinterpret_cast<uint4 *>(
           w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
       for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
@@ -1310,9 +1320,7 @@ __global__ void __launch_bounds__(kApplyThreads)
     const int tx = tile / (d.g.ntz * d.g.nty);
     if (!big) {
       // One pass: the packed 16-bit counts fold as they are.
-      lds_zero(s_cnt4);
-      __syncthreads();
-      tile_accumulate(w, itm.beg, end, s_cnt);
+      tile_accumulate(w, itm.beg, end, s_cnt4);
       unsigned c[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) c[j] = s_cnt[j * kApplyThreads + threadIdx.x];
@@ -1328,9 +1336,7 @@ __global__ void __launch_bounds__(kApplyThreads)
         hit[j] = 0;
       }
       for (int p0 = itm.beg; p0 < end; p0 += kPass) {
-        lds_zero(s_cnt4);
-        __syncthreads();
-        tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt);
+        tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt4);
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
           const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
