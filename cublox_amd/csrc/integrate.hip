@@ -637,24 +637,94 @@ __device__ __forceinline__ GridDesc grid_of_near(const Batch &b, int cell) {
 // ----------------------------------------------------------------------
 // walk kernel
 // ----------------------------------------------------------------------
+__device__ __forceinline__ int wave_scan_incl(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// Inclusive block scan of one int per thread: within waves by shuffles,
+// then across the wave totals.
+__device__ __forceinline__ int block_scan_incl(int *s_wave, int v) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  v = wave_scan_incl(v);
+  if (lane == 63) s_wave[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    const int x = wave_scan_incl(lane < nw ? s_wave[lane] : 0);
+    if (lane < nw) s_wave[lane] = x;
+  }
+  __syncthreads();
+  const int r = v + (wv > 0 ? s_wave[wv - 1] : 0);
+  __syncthreads();
+  return r;
+}
+
+
+constexpr int kLenBuckets = 64;
+
+// Length class of point i for the in-block ordering: its range in voxels
+// (clipped to the max range), 8 voxels per class. Any value is correct;
+// similar classes in one wave only shorten the longest lane's walk.
+__device__ __forceinline__ int length_class(const GridDesc &d, int i) {
+  const RayArgs &a = d.g.a;
+  float x = as_global(d.xs)[i], y = as_global(d.ys)[i], z = as_global(d.zs)[i];
+  if (!d.transform) {  // world frame: relative to the sensor
+    x -= a.ox;
+    y -= a.oy;
+    z -= a.oz;
+  }
+  const float r = sqrtf(x * x + y * y + z * z);
+  if (!(r < a.max_range)) return kLenBuckets - 1;  // clipped, or not finite
+  return min(kLenBuckets - 1, static_cast<int>(r * a.inv_res) >> 3);
+}
+
 // kLanes lanes per ray: 1, or 4 (a quad shares each span's per-axis work,
 // for launches too small to fill the GPU with one lane per ray). All lanes
 // of a ray hold the same walk state; lane 0 writes its records and counts.
+// The block's rays are taken in order of length class (a counting sort in
+// LDS), so each wave's rays walk similar distances; every ray's records are
+// its own, whichever lane walks it.
 template <int kLanes>
 __global__ void __launch_bounds__(kWalkBlock * kLanes)
     walk_kernel(const Batch b, Workspace w) {
   extern __shared__ int s_near[];  // the grid's near cube, sized at launch
+  __shared__ int s_hist[kLenBuckets];
+  __shared__ unsigned short s_perm[kWalkBlock];
   const int ray0 = blockIdx.x * kWalkBlock;
   if (ray0 >= batch_rays(b)) return;
   const GridDesc d = grid_of_ray(b, ray0);
   const IntegrateArgs &g = d.g;
   const int cells = g.cube * g.cube * g.cube;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
+  if (threadIdx.x < kLenBuckets) s_hist[threadIdx.x] = 0;
   __syncthreads();
+  {
+    const int t = threadIdx.x;
+    const int n_block = min(kWalkBlock, g.n - (ray0 - d.ray_base));
+    int cls = -1;
+    if (t < kWalkBlock) {
+      cls = t < n_block ? length_class(d, ray0 - d.ray_base + t) : kLenBuckets - 1;
+      atomicAdd(s_hist + cls, 1);
+    }
+    __syncthreads();
+    if (t < 64) {  // one wavefront: exclusive scan of the classes
+      const int v = s_hist[t];
+      s_hist[t] = wave_scan_incl(v) - v;
+    }
+    __syncthreads();
+    if (cls >= 0) s_perm[atomicAdd(s_hist + cls, 1)] = static_cast<unsigned short>(t);
+    __syncthreads();
+  }
 
   const int q = static_cast<int>(threadIdx.x) % kLanes;
   const bool lead = q == 0;
-  const int ray = ray0 + static_cast<int>(threadIdx.x) / kLanes;
+  const int ray = ray0 + s_perm[static_cast<int>(threadIdx.x) / kLanes];
   const int i = ray - d.ray_base;  // index within the grid's points
   const RayArgs &a = g.a;
   RayState r;
@@ -781,35 +851,6 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
 // ----------------------------------------------------------------------
 // binning: LDS hash aggregation of segments by tile
 // ----------------------------------------------------------------------
-
-__device__ __forceinline__ int wave_scan_incl(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(v, off, 64);
-    if (lane >= off) v += u;
-  }
-  return v;
-}
-
-// Inclusive block scan of one int per thread: within waves by shuffles,
-// then across the wave totals.
-__device__ __forceinline__ int block_scan_incl(int *s_wave, int v) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
-  v = wave_scan_incl(v);
-  if (lane == 63) s_wave[wv] = v;
-  __syncthreads();
-  if (wv == 0) {
-    const int x = wave_scan_incl(lane < nw ? s_wave[lane] : 0);
-    if (lane < nw) s_wave[lane] = x;
-  }
-  __syncthreads();
-  const int r = v + (wv > 0 ? s_wave[wv - 1] : 0);
-  __syncthreads();
-  return r;
-}
-
 
 // Finds (inserting if needed) the LDS slot of `key`; -1 when the bounded
 // probe finds neither. A key is either in the table for every caller or
