@@ -253,14 +253,16 @@ def _lat(c):
 
 
 def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
-               tick_points=1 << 21):
+               tick_points=1 << 20):
     """The map server workload: n_sessions sessions cycling over three grid
     shapes, HBM-budget admission (200 GB), continuous batching. Saturation
     is the drain rate of `depth` queued scans per session; then open-loop
     Poisson scan arrivals at each load fraction of it (fetches at 20 % of the
     scan rate, r uniform in 0.5-3 m) for `seconds`, with latency = completion - submit
     (host copy, H2D, batched update + recenter, fetch). The same sessions
-    served one scan per tick give the unbatched saturation."""
+    served one scan per tick give the unbatched saturation, and the same
+    scans through one drop-in grid per session (blocking update + recenter
+    per scan) the plain rate."""
     from cublox_amd.workloads import batched_grid
     W = cb.WORKLOADS
     t = time.perf_counter()
@@ -341,12 +343,41 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
     plain_sat = queued(srv1, sids1, depth)
     srv1.close()
     log(f"served: unbatched saturation {plain_sat:.0f} scans/s")
+    # Plain form: one drop-in grid per session, the same scans as blocking
+    # update (host columns, so the H2D copy is included as in the server)
+    # + recenter calls, one after another.
+    plain = []
+    for (h, r, c, th, mr) in specs:
+        x = cb.OccupancyGrid(h, r, c, th)
+        x.set_max_raycast_range(mr)
+        x.set_log_odds_params(*cb.log_odds())
+        plain.append(x)
+
+    def plain_round(j):
+        for x, (xs, ys, zs, T, base) in zip(plain, templates):
+            Tj = np.array(T, np.float32)
+            Tj[0, 3] += 0.5 * j
+            b = np.array(base, np.float32)
+            b[0] += 0.5 * j
+            x.update_sensor_frame((xs, ys, zs), Tj)
+            x.recenter(b)
+
+    plain_round(0)  # first origins, workspaces
+    t = time.perf_counter()
+    for j in range(1, depth + 1):
+        plain_round(j)
+    plain_sps = n_sessions * depth / (time.perf_counter() - t)
+    for x in plain:
+        x.close()
+    log(f"served: plain per-scan calls {plain_sps:.0f} scans/s")
     return dict(sessions=n_sessions,
                 shapes=[f"{W[s].map_size[0]}x{W[s].map_size[1]}x{W[s].map_size[2]}"
                         for s in SERVED_SHAPES],
                 points_per_scan_mean=round(statistics.mean(pts)),
                 hbm_budget_gib=200, resident_gib=round(st["resident_bytes"] / 2**30, 2),
                 tick_points=tick_points, saturation_scans_per_s=round(sat, 1),
+                plain_scans_per_s=round(plain_sps, 1),
+                speedup_vs_plain=round(sat / plain_sps, 2),
                 unbatched_saturation_scans_per_s=round(plain_sat, 1),
                 batching_speedup=round(sat / plain_sat, 2), levels=levels,
                 seconds_per_level=seconds)

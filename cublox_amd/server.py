@@ -27,7 +27,7 @@ def _f32(a) -> np.ndarray:
 
 class MapServer:
     def __init__(self, *, hbm_budget_bytes: int = 200 << 30, max_sessions: int = 256,
-                 max_points_per_scan: int = 131072, max_points_per_tick: int = 1 << 21,
+                 max_points_per_scan: int = 131072, max_points_per_tick: int = 1 << 20,
                  max_grids_per_tick: int = 256, max_tiles_per_tick: int = 1 << 21,
                  max_segments_per_ray: int = 96, graph: bool = True,
                  point_arena_bytes: int = 0):

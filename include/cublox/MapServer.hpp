@@ -54,7 +54,7 @@ struct ServerLimits {
   std::size_t hbm_budget_bytes{200ull << 30};  // sessions' device bytes
   int max_sessions{256};             // admitted at once
   int max_points_per_scan{131072};
-  int max_points_per_tick{1 << 21};
+  int max_points_per_tick{1 << 20};  // bounds a tick's device time
   int max_grids_per_tick{256};
   int max_tiles_per_tick{1 << 21};   // sum of the tick's grids' tile counts
   int max_segments_per_ray{96};      // sessions above it are refused
