@@ -383,6 +383,25 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
                 seconds_per_level=seconds)
 
 
+def fill_bandwidth(cb, nbytes=256 << 20, reps=20):
+    """Write bandwidth of a plain zero fill (cublox_fill_zero_async, 16 B
+    stores) over nbytes, back to back on the default stream: the practical
+    ceiling for the store-only recenter."""
+    from cublox_amd import _ffi
+    L = _ffi.hip()
+    buf = cb.DeviceBuffer(nbytes)
+    buf.zero()
+    a, b = cb.Event(), cb.Event()
+    a.record()
+    for _ in range(reps):
+        _ffi.check(L.cublox_fill_zero_async(buf.ptr, nbytes // 4, None))
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_ms(b) / reps
+    buf.free()
+    return dict(bytes=nbytes, ms=round(ms, 5), gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+
+
 def run_fetch(cb, w, g, scene, k, reps=5):
     if not w.fetch_queries:
         return None
@@ -534,6 +553,8 @@ def main():
         pcie = run_pcie(cb, w, scans, args.warmup, args.steps)
         extras["pcie_inclusive_ms_per_scan"] = statistics.mean(pcie)
         extras["fetch"] = run_fetch(cb, w, g, scene, total - 1)
+        extras["zero_fill_256MiB"] = fill_bandwidth(cb)
+        log(f"zero fill: {extras['zero_fill_256MiB']}")
         if args.batched_ticks > 0:
             extras["batched_workload"] = run_batched(cb, 1, args.batched_ticks)
             log(f"batched: {extras['batched_workload']}")

@@ -243,11 +243,12 @@ void GridBatch::stage(
 void GridBatch::launch() {
   const cublox_batch_limits c = abiLimits(lim_);
   auto enqueue = [&]() {
+    // Both staging copies first, so the recenter sequence is one launch.
+    CUBLOX_CHECK(cublox_recenter_batch_upload(h_rstage_, d_rarea_, max_regions_, stream_));
     CUBLOX_CHECK(cublox_launch_integrate_batch_staged(h_stage_, d_ws_, ws_bytes_, &c,
                                                       stream_));
     if (!graph_mode_) CUBLOX_CHECK(cublox_event_record(ev_[1], stream_));
-    CUBLOX_CHECK(cublox_launch_recenter_batch_staged(h_rstage_, d_rarea_, max_regions_,
-                                                     stream_));
+    CUBLOX_CHECK(cublox_launch_recenter_batch_uploaded(d_rarea_, max_regions_, stream_));
   };
   CUBLOX_CHECK(cublox_event_record(ev_[0], stream_));
   if (graph_mode_) {

@@ -183,10 +183,20 @@ struct ZeroRegion {
   long long run;
 };
 
+// A region cut into chunks of about `piece` elements, each zeroed by one
+// workgroup: the whole store set is spread evenly over the launch whatever
+// the regions' shapes. Chunks of the region are numbered from chunk0.
+struct ChunkedRegion {
+  ZeroRegion r;
+  long long chunk0;
+  long long piece;     // elements per chunk (a multiple of 64)
+  long long per_row;   // chunks per row (run >= piece) or rows per chunk
+};
+
 constexpr int kMaxRegions = 48;
 
 struct ZeroArgs {
-  ZeroRegion r[kMaxRegions];
+  ChunkedRegion r[kMaxRegions];
   int n;
 };
 
@@ -240,32 +250,64 @@ __device__ __forceinline__ void zero_region(CUBLOX_GLOBAL T *__restrict__ p,
   }
 }
 
+// The part of region c.r that chunk j (0-based within the region) covers.
+__host__ __device__ __forceinline__ ZeroRegion chunk_part(const ChunkedRegion &c,
+                                                        long long j) {
+  const ZeroRegion &r = c.r;
+  if (r.n_outer == 1 || r.run >= c.piece) {  // pieces of rows
+    const long long row = j / c.per_row, part = j - row * c.per_row;
+    const long long off = part * c.piece;
+    return ZeroRegion{r.base + row * r.stride + off, 1, 0,
+                      r.run - off < c.piece ? r.run - off : c.piece};
+  }
+  const long long row0 = j * c.per_row;  // whole short rows
+  const long long rows = r.n_outer - row0 < c.per_row ? r.n_outer - row0 : c.per_row;
+  return ZeroRegion{r.base + row0 * r.stride, rows, r.stride, r.run};
+}
+
 __global__ void __launch_bounds__(256)
     zero_regions_kernel(float *__restrict__ occ, int *__restrict__ op_cnt,
                         int *__restrict__ hit_cnt, const ZeroArgs z) {
-  const ZeroRegion r = z.r[blockIdx.y];
-  zero_region(as_global(occ), r, blockIdx.x, gridDim.x);
-  if (op_cnt) zero_region(as_global(op_cnt), r, blockIdx.x, gridDim.x);
-  if (hit_cnt) zero_region(as_global(hit_cnt), r, blockIdx.x, gridDim.x);
+  const long long j = blockIdx.x;
+  int k = 0;  // the region holding chunk j (chunk0 ascending)
+  while (k + 1 < z.n && z.r[k + 1].chunk0 <= j) ++k;
+  const ZeroRegion part = chunk_part(z.r[k], j - z.r[k].chunk0);
+  zero_region(as_global(occ), part, 0, 1);
+  if (op_cnt) zero_region(as_global(op_cnt), part, 0, 1);
+  if (hit_cnt) zero_region(as_global(hit_cnt), part, 0, 1);
 }
 
 // Batched recenter: regions of many grids, each with its occupancy base.
 struct ZeroRegionP {
   float *occ;
-  ZeroRegion r;
+  ChunkedRegion c;
 };
 
 struct RecenterBatchHeader {
   int n_regions;
-  int pad[7];
+  int pad;
+  long long n_chunks;
+  int pad2[4];
 };
 
 __global__ void __launch_bounds__(256)
     zero_regions_batch_kernel(const RecenterBatchHeader *__restrict__ hdr,
                               const ZeroRegionP *__restrict__ regions) {
-  if (static_cast<int>(blockIdx.y) >= load_const(hdr).n_regions) return;
-  const ZeroRegionP q = load_const(regions + blockIdx.y);
-  zero_region(as_global(q.occ), q.r, blockIdx.x, gridDim.x);
+  const RecenterBatchHeader h = load_const(hdr);
+  for (long long j = blockIdx.x; j < h.n_chunks; j += gridDim.x) {
+    int lo = 0, hi = h.n_regions - 1;  // last region with chunk0 <= j
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      const long long c0 = ((const CUBLOX_CONST long long *)&regions[mid].c.chunk0)[0];
+      if (c0 <= j) {
+        lo = mid;
+      } else {
+        hi = mid - 1;
+      }
+    }
+    const ZeroRegionP q = load_const(regions + lo);
+    zero_region(as_global(q.occ), chunk_part(q.c, j - q.c.chunk0), 0, 1);
+  }
 }
 
 // Reference-shaped slab clear from a DEVICE slice list: one block row per
@@ -530,17 +572,29 @@ HostPlan plan_recenter(const cublox_recenter_cfg &c) {
   return p;
 }
 
-// Workgroups that cover a region with zero_region's mapping.
-long long region_blocks(const ZeroRegion &r) {
-  long long b;
-  if (r.n_outer == 1) {
-    b = (r.run / 4 + 255) / 256;
-  } else if (r.run >= 256) {
-    b = r.n_outer;
-  } else {
-    b = (r.n_outer + 256 / r.run - 1) / (256 / r.run);
+constexpr long long kChunkElems = 4096;     // at least 16 KB of float per workgroup
+constexpr long long kSingleChunkBlocks = 2048;  // one grid's event: ~8 workgroups per CU
+
+// Cuts a region into chunks of about `piece` elements (rows are split when
+// longer, whole rows grouped when shorter); returns the chunk count.
+long long chunk_region(const ZeroRegion &r, long long piece, long long chunk0,
+                       ChunkedRegion *out) {
+  out->r = r;
+  out->chunk0 = chunk0;
+  out->piece = piece;
+  if (r.n_outer == 1 || r.run >= piece) {
+    out->per_row = (r.run + piece - 1) / piece;
+    return r.n_outer * out->per_row;
   }
-  return std::max<long long>(1, std::min<long long>(b, 2048));
+  out->per_row = std::max<long long>(1, piece / r.run);
+  return (r.n_outer + out->per_row - 1) / out->per_row;
+}
+
+// Elements per chunk so that a launch of `blocks` workgroups takes about
+// one chunk each (at least kChunkElems, a multiple of 64).
+long long chunk_piece(long long elems, long long blocks) {
+  const long long p = (elems + blocks - 1) / std::max<long long>(blocks, 1);
+  return std::max(kChunkElems, (p + 63) / 64 * 64);
 }
 
 // Sorted unique slice values, merged into inclusive [lo, hi] runs.
@@ -583,21 +637,23 @@ std::vector<ZeroRegion> recenter_regions(const cublox_recenter_cfg &cfg) {
   return regions;
 }
 
-constexpr unsigned kBatchRegionBlocks = 64;
+constexpr unsigned kBatchChunksPerRegion = 32;  // batch launch: blocks per region slot
 
 int launch_regions(float *occ, int *op, int *hit,
                    const std::vector<ZeroRegion> &regions, hipStream_t s) {
   for (size_t i0 = 0; i0 < regions.size(); i0 += kMaxRegions) {
     ZeroArgs z{};
     z.n = static_cast<int>(std::min<size_t>(kMaxRegions, regions.size() - i0));
-    long long bx = 1;
+    long long elems = 0;
+    for (int k = 0; k < z.n; ++k) elems += regions[i0 + k].n_outer * regions[i0 + k].run;
+    const long long piece = chunk_piece(elems, kSingleChunkBlocks);
+    long long chunks = 0;
     for (int k = 0; k < z.n; ++k) {
-      z.r[k] = regions[i0 + k];
-      bx = std::max(bx, region_blocks(z.r[k]));
+      chunks += chunk_region(regions[i0 + k], piece, chunks, &z.r[k]);
     }
-    zero_regions_kernel<<<dim3(static_cast<unsigned>(bx),
-                               static_cast<unsigned>(z.n)),
-                          dim3(256), 0, s>>>(occ, op, hit, z);
+    if (chunks <= 0) continue;
+    zero_regions_kernel<<<dim3(static_cast<unsigned>(chunks)), dim3(256), 0, s>>>(
+        occ, op, hit, z);
     const int st = check_launch("zero_regions_kernel");
     if (st != CUBLOX_OK) return st;
   }
@@ -725,6 +781,7 @@ int cublox_launch_recenter(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
 }
 
 size_t cublox_recenter_batch_staging_bytes(int max_regions) {
+  static_assert(sizeof(RecenterBatchHeader) == 32, "header size");
   return sizeof(RecenterBatchHeader) +
          static_cast<size_t>(std::max(max_regions, 1)) * sizeof(ZeroRegionP);
 }
@@ -737,7 +794,7 @@ int cublox_recenter_batch_stage(const cublox_recenter_item *items, int n,
   }
   auto *hdr = static_cast<RecenterBatchHeader *>(h_staging);
   auto *regs = reinterpret_cast<ZeroRegionP *>(hdr + 1);
-  int count = 0;
+  std::vector<std::pair<float *, ZeroRegion>> all;
   for (int k = 0; k < n; ++k) {
     const cublox_recenter_item &it = items[k];
     results[k] = cublox_recenter_result{};
@@ -753,33 +810,53 @@ int cublox_recenter_batch_stage(const cublox_recenter_item *items, int n,
       rs = recenter_regions(it.cfg);
     }
     for (const ZeroRegion &r : rs) {
-      if (count >= max_regions) {
+      if (static_cast<int>(all.size()) >= max_regions) {
         return set_error(CUBLOX_ERR_CAPACITY, "recenter_batch: too many regions");
       }
-      regs[count++] = ZeroRegionP{it.d_occ, r};
+      all.emplace_back(it.d_occ, r);
     }
   }
-  *hdr = RecenterBatchHeader{count, {0, 0, 0, 0, 0, 0, 0}};
+  long long elems = 0;
+  for (const auto &pr : all) elems += pr.second.n_outer * pr.second.run;
+  const long long piece =
+      chunk_piece(elems, static_cast<long long>(max_regions) * kBatchChunksPerRegion);
+  long long chunks = 0;
+  for (size_t k = 0; k < all.size(); ++k) {
+    regs[k].occ = all[k].first;
+    chunks += chunk_region(all[k].second, piece, chunks, &regs[k].c);
+  }
+  *hdr = RecenterBatchHeader{static_cast<int>(all.size()), 0, chunks, {0, 0, 0, 0}};
   return CUBLOX_OK;
+}
+
+int cublox_recenter_batch_upload(const void *h_staging, void *d_area, int max_regions,
+                                 cublox_stream s) {
+  if (!h_staging || !d_area || max_regions <= 0) {
+    return set_error(CUBLOX_ERR_INVALID, "recenter_batch: bad arguments");
+  }
+  return check(hipMemcpyAsync(d_area, h_staging,
+                              cublox_recenter_batch_staging_bytes(max_regions),
+                              hipMemcpyHostToDevice, as_stream(s)),
+               "hipMemcpyAsync(recenter batch)");
+}
+
+int cublox_launch_recenter_batch_uploaded(const void *d_area, int max_regions,
+                                          cublox_stream s) {
+  if (!d_area || max_regions <= 0) {
+    return set_error(CUBLOX_ERR_INVALID, "recenter_batch: bad arguments");
+  }
+  const auto *hdr = static_cast<const RecenterBatchHeader *>(d_area);
+  zero_regions_batch_kernel<<<dim3(kBatchChunksPerRegion * static_cast<unsigned>(max_regions)),
+                              dim3(256), 0, as_stream(s)>>>(
+      hdr, reinterpret_cast<const ZeroRegionP *>(hdr + 1));
+  return check_launch("zero_regions_batch_kernel");
 }
 
 int cublox_launch_recenter_batch_staged(const void *h_staging, void *d_area,
                                         int max_regions, cublox_stream s) {
-  if (!h_staging || !d_area || max_regions <= 0) {
-    return set_error(CUBLOX_ERR_INVALID, "recenter_batch: bad arguments");
-  }
-  hipStream_t hs = as_stream(s);
-  const int st = check(hipMemcpyAsync(d_area, h_staging,
-                                      cublox_recenter_batch_staging_bytes(max_regions),
-                                      hipMemcpyHostToDevice, hs),
-                       "hipMemcpyAsync(recenter batch)");
+  const int st = cublox_recenter_batch_upload(h_staging, d_area, max_regions, s);
   if (st != CUBLOX_OK) return st;
-  const auto *hdr = static_cast<const RecenterBatchHeader *>(d_area);
-  zero_regions_batch_kernel<<<dim3(kBatchRegionBlocks,
-                                   static_cast<unsigned>(max_regions)),
-                              dim3(256), 0, hs>>>(
-      hdr, reinterpret_cast<const ZeroRegionP *>(hdr + 1));
-  return check_launch("zero_regions_batch_kernel");
+  return cublox_launch_recenter_batch_uploaded(d_area, max_regions, s);
 }
 
 int cublox_launch_clear_recenter_slabs_for_axis(

@@ -205,6 +205,13 @@ int cublox_recenter_batch_stage(const cublox_recenter_item *items, int n,
                                 cublox_recenter_result *results);
 int cublox_launch_recenter_batch_staged(const void *h_staging, void *d_area,
                                         int max_regions, cublox_stream s);
+/* The two halves of cublox_launch_recenter_batch_staged: the staging copy
+ * to the device area, and the zeroing launch that reads it (for callers
+ * that move the copy ahead of other work on the stream). */
+int cublox_recenter_batch_upload(const void *h_staging, void *d_area,
+                                 int max_regions, cublox_stream s);
+int cublox_launch_recenter_batch_uploaded(const void *d_area, int max_regions,
+                                          cublox_stream s);
 
 int cublox_launch_clear_recenter_slabs_for_axis(
     float *d_occ, int *d_op_cnt, int *d_hit_cnt, cublox_int3 map_size_i,
