@@ -398,8 +398,19 @@ def fill_bandwidth(cb, nbytes=256 << 20, reps=20):
     b.record()
     b.synchronize()
     ms = a.elapsed_ms(b) / reps
+    # A device-to-device copy of the same size: a known read + write for
+    # scaling hardware byte counters (FETCH_SIZE / WRITE_SIZE).
+    dst = cb.DeviceBuffer(nbytes)
+    a.record()
+    _ffi.check(L.cublox_memcpy_d2d_async(dst.ptr, buf.ptr, nbytes, None))
+    b.record()
+    b.synchronize()
+    copy_ms = a.elapsed_ms(b)
+    dst.free()
     buf.free()
-    return dict(bytes=nbytes, ms=round(ms, 5), gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+    return dict(bytes=nbytes, ms=round(ms, 5), gbs=round(nbytes / (ms * 1e-3) / 1e9, 1),
+                copy_ms=round(copy_ms, 5),
+                copy_gbs=round(2 * nbytes / (copy_ms * 1e-3) / 1e9, 1))
 
 
 def run_fetch(cb, w, g, scene, k, reps=5):
@@ -463,6 +474,8 @@ def main():
                     help="timed ticks of the 64-grid batched workload (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="seconds per load level of the 256-session map server (0: skip)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run the zero-fill and copy calibration kernels (with --no-extras)")
     ap.add_argument("--only", choices=("batched", "served"),
                     help="run only that workload and print its record (for profiling)")
     args = ap.parse_args()
@@ -534,6 +547,12 @@ def main():
                     "avg_launch_ms": round(ms, 5)}
 
         rooflines = {"update": rl(up_bytes, up_ms), "recenter": rl(re_bytes, re_ms)}
+        # HBM bytes per update launch from the FETCH_SIZE / WRITE_SIZE runs
+        # recorded in profiles/ (readme workload only).
+        tpath = os.path.join(ROOT, "profiles", "update_traffic_readme_701x701x151.json")
+        if w.name == "readme" and os.path.exists(tpath):
+            with open(tpath) as fh:
+                rooflines["update"]["traffic"] = json.load(fh)["update_traffic_bytes"]
         rooflines["update"]["visits_per_s"] = visits / (up_ms * 1e-3) if up_ms else 0
         totals = {"update": kt["update_ms"], "recenter": kt["recenter_ms"]}
         dom = max(totals, key=totals.get)
@@ -560,6 +579,9 @@ def main():
             log(f"batched: {extras['batched_workload']}")
         if args.served_seconds > 0:
             extras["served_workload"] = run_served(cb, seconds=args.served_seconds)
+
+    if args.no_extras and args.calibrate:
+        extras["zero_fill_256MiB"] = fill_bandwidth(cb)
 
     cpu = None
     if not args.no_cpu_baseline:
