@@ -19,11 +19,12 @@ class GridBatch:
     def __init__(self, *, max_grids: int, max_points: int, max_tiles: int,
                  max_segments_per_ray: int, max_near_steps: int = 12,
                  min_chunk: int = 256, max_slabs: int = 65536,
-                 max_regions: int = 0):
+                 max_regions: int = 0, changed_voxels: bool = False):
         L = _ffi.host()
-        lim = (C.c_int * 8)(int(max_grids), int(max_points), int(max_tiles),
+        lim = (C.c_int * 9)(int(max_grids), int(max_points), int(max_tiles),
                             int(max_segments_per_ray), int(max_near_steps),
-                            int(min_chunk), int(max_slabs), int(max_regions))
+                            int(min_chunk), int(max_slabs), int(max_regions),
+                            int(bool(changed_voxels)))
         h = L.cublox_gb_create(lim)
         if not h:
             raise RuntimeError(L.cublox_og_last_error().decode())

@@ -57,6 +57,7 @@ static cublox_batch_limits abiLimits(const BatchLimits &l) {
   c.max_near_steps = l.max_near_steps;
   c.chunk = l.min_chunk;
   c.max_slabs = l.max_slabs;
+  c.delta = l.changed_voxels ? 1 : 0;
   return c;
 }
 
@@ -171,6 +172,13 @@ void GridBatch::stage(
     const std::vector<GridScan> &scans,
     const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
   std::vector<cublox_integrate_grid> items(scans.size());
+  for (const GridScan &s : scans) {
+    if (s.grid && s.grid->delta_capacity_ > 0 && !lim_.changed_voxels) {
+      throw std::invalid_argument(
+          "cublox::GridBatch: a grid records changed voxels; set "
+          "BatchLimits::changed_voxels");
+    }
+  }
   for (size_t k = 0; k < scans.size(); ++k) {
     const GridScan &s = scans[k];
     if (!s.grid) throw std::invalid_argument("cublox::GridBatch: null grid");
@@ -196,6 +204,12 @@ void GridBatch::stage(
     q.near_steps = setting_.near_steps;
     q.d_export_op = nullptr;
     q.d_export_hit = nullptr;
+    if (g.delta_capacity_ > 0) {
+      q.d_delta_count = g.d_delta_count_;
+      q.d_delta_idx = g.d_delta_idx_;
+      q.d_delta_val = g.d_delta_val_;
+      q.delta_capacity = g.delta_capacity_;
+    }
   }
   const cublox_batch_limits c = abiLimits(lim_);
   CUBLOX_CHECK(cublox_integrate_batch_stage(items.data(), static_cast<int>(items.size()),

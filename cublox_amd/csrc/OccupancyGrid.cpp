@@ -93,6 +93,10 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_dense_pos_ = std::exchange(o.d_fetch_dense_pos_, nullptr);
   d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
+  d_delta_count_ = std::exchange(o.d_delta_count_, nullptr);
+  d_delta_idx_ = std::exchange(o.d_delta_idx_, nullptr);
+  d_delta_val_ = std::exchange(o.d_delta_val_, nullptr);
+  delta_capacity_ = std::exchange(o.delta_capacity_, 0u);
   fetch_dense_capacity_ = std::exchange(o.fetch_dense_capacity_, 0);
   fetch_offsets_capacity_ = std::exchange(o.fetch_offsets_capacity_, 0);
   fetch_capacity_ = std::exchange(o.fetch_capacity_, 0);
@@ -132,6 +136,10 @@ void OccupancyGrid::releaseAll() noexcept {
   freeDevice(d_fetch_offsets_);
   fetch_dense_capacity_ = 0;
   fetch_offsets_capacity_ = 0;
+  freeDevice(d_delta_count_);
+  freeDevice(d_delta_idx_);
+  freeDevice(d_delta_val_);
+  delta_capacity_ = 0;
   freeDevice(d_op_cnt_);
   freeDevice(d_hit_cnt_);
   freeDevice(d_occ_);
@@ -288,6 +296,67 @@ void OccupancyGrid::ensureCounters() {
   }
 }
 
+// The per-voxel apply keeps the reference's gate (kernels.cu:251-262): it
+// records changed voxels only when the capacity covers every voxel.
+static void checkDeltaGate(OccupancyGrid::UpdatePath p, unsigned int capacity,
+                           int voxel_num) {
+  if (p == OccupancyGrid::UpdatePath::PER_VOXEL_ATOMICS && capacity > 0 &&
+      capacity < static_cast<unsigned int>(voxel_num)) {
+    throw std::invalid_argument(
+        "cublox::OccupancyGrid: the per-voxel apply records changed voxels only "
+        "with a capacity of at least the voxel count");
+  }
+}
+
+void OccupancyGrid::setUpdatePath(UpdatePath p) {
+  checkDeltaGate(p, delta_capacity_, config_.voxel_num);
+  update_path_ = p;
+}
+
+void OccupancyGrid::setChangedVoxelCapacity(unsigned int capacity) {
+  checkDeltaGate(update_path_, capacity, config_.voxel_num);
+  if (capacity == delta_capacity_) return;
+  sync();
+  freeDevice(d_delta_count_);
+  freeDevice(d_delta_idx_);
+  freeDevice(d_delta_val_);
+  delta_capacity_ = 0;
+  if (capacity == 0) return;
+  try {
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_delta_count_), sizeof(unsigned int)));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_delta_idx_),
+                               static_cast<size_t>(capacity) * sizeof(int)));
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_delta_val_),
+                               static_cast<size_t>(capacity) * sizeof(float)));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_delta_count_, 1, stream_));
+  } catch (...) {
+    freeDevice(d_delta_count_);
+    freeDevice(d_delta_idx_);
+    freeDevice(d_delta_val_);
+    throw;
+  }
+  delta_capacity_ = capacity;
+}
+
+unsigned long long OccupancyGrid::changedVoxels(std::vector<ChangedVoxel> &out) const {
+  out.clear();
+  if (delta_capacity_ == 0) return 0;
+  unsigned int count = 0;
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(&count, d_delta_count_, sizeof(count), stream_));
+  sync();
+  const unsigned int n = std::min(count, delta_capacity_);
+  std::vector<int> ids(n);
+  std::vector<float> vals(n);
+  if (n > 0) {
+    CUBLOX_CHECK(cublox_memcpy_d2h_async(ids.data(), d_delta_idx_, n * sizeof(int), stream_));
+    CUBLOX_CHECK(cublox_memcpy_d2h_async(vals.data(), d_delta_val_, n * sizeof(float), stream_));
+    sync();
+  }
+  out.resize(n);
+  for (unsigned int k = 0; k < n; ++k) out[k] = ChangedVoxel{ids[k], vals[k]};
+  return count;
+}
+
 void OccupancyGrid::setNearSteps(int k) {
   if (k < 1 || k > 12) {
     throw std::invalid_argument("cublox::OccupancyGrid: near steps must be in [1, 12]");
@@ -338,12 +407,14 @@ void OccupancyGrid::launchIntegrate(const float *d_xs, const float *d_ys,
   cfg.max_range = max_raycast_range_;
   cfg.map_vox_num = config_.voxel_num;
   cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
+  const bool delta = !export_counts && delta_capacity_ > 0;
   mark(0);
-  CUBLOX_CHECK(cublox_launch_integrate(
+  CUBLOX_CHECK(cublox_launch_integrate_delta(
       &cfg, T, d_xs, d_ys, d_zs, n, d_occ_, l_hit_, l_miss_, l_min_, l_max_,
       l_unknown_, d_ws_, ws_bytes_, ws_points_, near_steps_,
       export_counts ? d_op_cnt_ : nullptr, export_counts ? d_hit_cnt_ : nullptr,
-      stream_));
+      delta ? d_delta_count_ : nullptr, delta ? d_delta_idx_ : nullptr,
+      delta ? d_delta_val_ : nullptr, delta ? delta_capacity_ : 0u, stream_));
   mark(1);
   pending_[0] = timing_;
 }
@@ -393,11 +464,15 @@ void OccupancyGrid::castRaysSensor(const float *d_xs, const float *d_ys,
 
 void OccupancyGrid::launchApply() {
   ensureCounters();
+  checkDeltaGate(UpdatePath::PER_VOXEL_ATOMICS, delta_capacity_, config_.voxel_num);
+  const bool delta = delta_capacity_ > 0;
+  if (delta) CUBLOX_CHECK(cublox_fill_zero_async(d_delta_count_, 1, stream_));
   mark(2);
-  CUBLOX_CHECK(cublox_launch_apply_update(d_occ_, d_op_cnt_, d_hit_cnt_,
-                                          config_.voxel_num, l_hit_, l_miss_,
-                                          l_min_, l_max_, l_unknown_, stream_,
-                                          nullptr, nullptr, nullptr, 0u));
+  CUBLOX_CHECK(cublox_launch_apply_update(
+      d_occ_, d_op_cnt_, d_hit_cnt_, config_.voxel_num, l_hit_, l_miss_, l_min_,
+      l_max_, l_unknown_, stream_, delta ? d_delta_count_ : nullptr,
+      delta ? d_delta_idx_ : nullptr, delta ? d_delta_val_ : nullptr,
+      delta ? delta_capacity_ : 0u));
   mark(3);
   pending_[1] = timing_;
 }

@@ -561,9 +561,16 @@ struct GridDesc {
   long long slot_base; // first segment slot; rays own g.max_seg slots each
   int tile_base;       // first tile
   int near_base;       // first near-field cell
+  // Changed-voxel stream (optional, delta_count non-null): the (hash, new
+  // value) of every voxel whose stored value changes is appended; the
+  // count counts all of them, entries past delta_cap are not written.
+  unsigned *delta_count;
+  int *delta_idx;
+  float *delta_val;
+  unsigned delta_cap;
 };
 
-static_assert(sizeof(GridDesc) == 256, "descriptor size (documented)");
+static_assert(sizeof(GridDesc) == 288, "descriptor size (documented)");
 
 struct BatchHeader {  // device-resident in batch mode
   int n_grids;
@@ -886,6 +893,14 @@ __global__ void __launch_bounds__(kBinThreads)
   __shared__ int s_wave[kBinThreads / 64];
   const int r0 = blockIdx.x * rays;
   const int n_rays = batch_rays(b);
+  if (!kPlace && blockIdx.x == 0) {
+    // The changed-voxel counts start from zero (the folds run later).
+    const int n_grids = batch_grids(b);
+    for (int k = threadIdx.x; k < n_grids; k += kBinThreads) {
+      unsigned *c = b.descs ? load_const(b.descs + k).delta_count : b.one.delta_count;
+      if (c) *c = 0u;
+    }
+  }
   if (r0 >= n_rays) return;
   const GridDesc d = grid_of_ray(b, r0);
   const int t = threadIdx.x;
@@ -1284,11 +1299,53 @@ __device__ __forceinline__ int tile_voxel_hash(const RayArgs &a, int x0,
   return (x0 + lx) * (a.my * a.mz) + (y0 + ly) * a.mz + (z0 + lz);
 }
 
+// ----------------------------------------------------------------------
+// Changed-voxel stream (the reference's modified-voxel compaction,
+// kernels.cu:202-209, per fused update): a wave reserves the slots of all
+// its changed voxels with one atomic on the grid's count, then writes them
+// voxel-index-major, so the lanes of one index write one run.
+// ----------------------------------------------------------------------
+__device__ __forceinline__ unsigned lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0u));
+}
+
+__device__ __forceinline__ void delta_put(const GridDesc &d, unsigned slot,
+                                          int h, float v) {
+  if (slot < d.delta_cap) {
+    as_global(d.delta_idx)[slot] = h;
+    as_global(d.delta_val)[slot] = v;
+  }
+}
+
+// Appends the voxels of this thread marked in `changed` (bit j: hash(j),
+// new value v[j], j < kN). Every lane of the wave takes part; d is the
+// same in all of them.
+template <int kN, typename Hash>
+__device__ __forceinline__ void delta_append(const GridDesc &d,
+                                             unsigned changed, const float *v,
+                                             Hash hash) {
+  int total = __popc(changed);
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off, 64);
+  if (total == 0) return;  // wave-uniform
+  unsigned base = 0;
+  if (__lane_id() == 0) base = atomic_add_u32_ret(d.delta_count, static_cast<unsigned>(total));
+  base = static_cast<unsigned>(__shfl(static_cast<int>(base), 0, 64));
+#pragma unroll
+  for (int j = 0; j < kN; ++j) {
+    const bool on = (changed >> j) & 1u;
+    const unsigned long long m = __ballot(on);
+    if (on) delta_put(d, base + lanes_below(m), hash(j), v[j]);
+    base += static_cast<unsigned>(__popcll(m));
+  }
+}
+
 // Fold of the tile counts a thread owns (voxels j * 256 + t, j < 32: a
 // z-run of 32 contiguous floats per 32 lanes): every occ load is issued
 // before the first update, so a tile costs one memory round trip. Counts
-// come from op(j) and hit(j).
-template <typename Op, typename Hit>
+// come from op(j) and hit(j). kDelta: voxels whose value changes go to the
+// grid's changed-voxel stream when it has one.
+template <bool kDelta, typename Op, typename Hit>
 __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
                                            int z0, Op op, Hit hit) {
   const RayArgs &a = d.g.a;
@@ -1314,6 +1371,20 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
   for (int j = 0; j < kPerThread; ++j) {
     v[j] = op(j) != 0 ? occ[hash(j)] : 0.0f;
   }
+  if (kDelta && d.delta_count) {
+    unsigned changed = 0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      if (op(j) == 0) continue;
+      const float nv = apply_logodds(v[j], static_cast<int>(op(j)),
+                                     static_cast<int>(hit(j)), d.g.p);
+      occ[hash(j)] = nv;
+      if (nv != v[j]) changed |= 1u << j;
+      v[j] = nv;
+    }
+    delta_append<kPerThread>(d, changed, v, hash);
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
     if (op(j) == 0) continue;
@@ -1332,7 +1403,7 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
 // tile stores its packed counts to its scratch slab (hot_fold sums a
 // tile's slabs); an unsplit hot tile accumulates in passes of kPass
 // records summed in registers. Empty items do nothing.
-template <bool kBig>
+template <bool kBig, bool kDelta>
 __global__ void __launch_bounds__(kApplyThreads)
     tile_apply_kernel(const Batch b, Workspace w) {
   __shared__ uint4 s_cnt4[kTileVox / 4];
@@ -1366,7 +1437,7 @@ __global__ void __launch_bounds__(kApplyThreads)
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) c[j] = s_cnt[j * kApplyThreads + threadIdx.x];
       __syncthreads();
-      fold_owned(d, tx * TX, ty * TY, tz * TZ,
+      fold_owned<kDelta>(d, tx * TX, ty * TY, tz * TZ,
                  [&](int j) { return c[j] & 0xFFFFu; },
                  [&](int j) { return c[j] >> 16; });
     } else {
@@ -1386,7 +1457,7 @@ __global__ void __launch_bounds__(kApplyThreads)
         }
         __syncthreads();
       }
-      fold_owned(d, tx * TX, ty * TY, tz * TZ, [&](int j) { return op[j]; },
+      fold_owned<kDelta>(d, tx * TX, ty * TY, tz * TZ, [&](int j) { return op[j]; },
                  [&](int j) { return hit[j]; });
     }
   }
@@ -1394,6 +1465,7 @@ __global__ void __launch_bounds__(kApplyThreads)
 
 // Sums the scratch slabs of each split tile and folds: kHotSlices
 // workgroups per tile, four consecutive voxels (one z-run piece) per lane.
+template <bool kDelta>
 __global__ void __launch_bounds__(256)
     hot_fold_kernel(const Batch b, Workspace w) {
   const int n_hot = w.misc[3];
@@ -1442,6 +1514,7 @@ __global__ void __launch_bounds__(256)
     for (int e = 0; e < 4; ++e) {
       v[e] = (op[e] != 0 && !d.export_op) ? occ[h0 + e] : 0.0f;
     }
+    unsigned changed = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       if (op[e] == 0) continue;
@@ -1449,9 +1522,15 @@ __global__ void __launch_bounds__(256)
         as_global(d.export_op)[h0 + e] = static_cast<int>(op[e]);
         as_global(d.export_hit)[h0 + e] = static_cast<int>(hit[e]);
       } else {
-        occ[h0 + e] = apply_logodds(v[e], static_cast<int>(op[e]),
-                                    static_cast<int>(hit[e]), d.g.p);
+        const float nv = apply_logodds(v[e], static_cast<int>(op[e]),
+                                       static_cast<int>(hit[e]), d.g.p);
+        occ[h0 + e] = nv;
+        if (kDelta && nv != v[e]) changed |= 1u << e;
+        v[e] = nv;
       }
+    }
+    if (kDelta && d.delta_count) {
+      delta_append<4>(d, changed, v, [&](int e) { return h0 + e; });
     }
   }
 }
@@ -1459,6 +1538,7 @@ __global__ void __launch_bounds__(256)
 // ----------------------------------------------------------------------
 // near_apply: fold the near-field cubes, clear them
 // ----------------------------------------------------------------------
+template <bool kDelta>
 __global__ void __launch_bounds__(256)
     near_apply_kernel(const Batch b, Workspace w) {
   const int n_near = batch_near(b);
@@ -1488,7 +1568,13 @@ __global__ void __launch_bounds__(256)
       as_global(d.export_hit)[h] = hit;
     } else {
       CUBLOX_GLOBAL float *occ = as_global(d.occ);
-      occ[h] = apply_logodds(occ[h], o, hit, g.p);
+      const float v0 = occ[h];
+      const float nv = apply_logodds(v0, o, hit, g.p);
+      occ[h] = nv;
+      // Cells of one wave may belong to different grids: one atomic each.
+      if (kDelta && d.delta_count && nv != v0) {
+        delta_put(d, atomic_add_u32_ret(d.delta_count, 1u), h, nv);
+      }
     }
   }
 }
@@ -1642,6 +1728,30 @@ Caps single_caps(const cublox_raycast_cfg &cfg, int max_points, int near_k) {
   return c;
 }
 
+// A changed-voxel stream: a count (zeroed by the caller) and, for a
+// nonzero capacity, both entry arrays; not with a count export (no value
+// is stored then).
+int check_delta(const unsigned *count, const int *idx, const float *val,
+                unsigned cap, bool exporting, const char *who) {
+  if (!count) return CUBLOX_OK;
+  if (exporting) {
+    return set_error(CUBLOX_ERR_INVALID,
+                     std::string(who) + ": a changed-voxel stream with a count export");
+  }
+  if (cap > 0 && (!idx || !val)) {
+    return set_error(CUBLOX_ERR_INVALID,
+                     std::string(who) + ": changed-voxel stream needs both arrays");
+  }
+  return CUBLOX_OK;
+}
+
+void set_delta(GridDesc &d, unsigned *count, int *idx, float *val, unsigned cap) {
+  d.delta_count = count;
+  d.delta_idx = count ? idx : nullptr;
+  d.delta_val = count ? val : nullptr;
+  d.delta_cap = count ? cap : 0u;
+}
+
 int check_cfg(const cublox_raycast_cfg &cfg, int near_k) {
   if (near_k < 1 || near_k > kMaxNearK) {
     return set_error(CUBLOX_ERR_INVALID, "integrate: near_steps out of [1, 12]");
@@ -1707,11 +1817,34 @@ int bin_rays_per_block(const Caps &c) {
   return r;
 }
 
+// The fold launches; kDelta: the variants that feed changed-voxel streams.
+template <bool kDelta>
+int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
+                 hipStream_t hs) {
+  const Caps &c = L.cap;
+  const int apply_blocks = std::min(L.max_items, 1 << 16);
+  tile_apply_kernel<false, kDelta><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  int st = check_launch("tile_apply_kernel");
+  if (st != CUBLOX_OK) return st;
+  tile_apply_kernel<true, kDelta><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+  st = check_launch("tile_apply_kernel<big>");
+  if (st != CUBLOX_OK) return st;
+  const long long hot_blocks =
+      std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
+  hot_fold_kernel<kDelta><<<dim3(static_cast<unsigned>(hot_blocks)), dim3(256), 0, hs>>>(b, w);
+  st = check_launch("hot_fold_kernel");
+  if (st != CUBLOX_OK) return st;
+  near_apply_kernel<kDelta><<<dim3(std::max(1, std::min((c.max_near + 255) / 256, 1024))),
+                              dim3(256), 0, hs>>>(b, w);
+  return check_launch("near_apply_kernel");
+}
+
 // The launch sequence for a workspace of layout L. Launch shapes depend on
 // the capacities only (so the sequence can be captured once and replayed);
-// the kernels read the actual counts from the batch.
+// the kernels read the actual counts from the batch. delta selects the fold
+// variants that feed changed-voxel streams (grids without one skip it).
 int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
-                    hipStream_t hs) {
+                    bool delta, hipStream_t hs) {
   const Caps &c = L.cap;
   const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
@@ -1744,21 +1877,7 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   bin_kernel<true><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
-  const int apply_blocks = std::min(L.max_items, 1 << 16);
-  tile_apply_kernel<false><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
-  st = check_launch("tile_apply_kernel");
-  if (st != CUBLOX_OK) return st;
-  tile_apply_kernel<true><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
-  st = check_launch("tile_apply_kernel<big>");
-  if (st != CUBLOX_OK) return st;
-  const long long hot_blocks =
-      std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
-  hot_fold_kernel<<<dim3(static_cast<unsigned>(hot_blocks)), dim3(256), 0, hs>>>(b, w);
-  st = check_launch("hot_fold_kernel");
-  if (st != CUBLOX_OK) return st;
-  near_apply_kernel<<<dim3(std::max(1, std::min((c.max_near + 255) / 256, 1024))),
-                      dim3(256), 0, hs>>>(b, w);
-  return check_launch("near_apply_kernel");
+  return delta ? launch_folds<true>(b, L, w, hs) : launch_folds<false>(b, L, w, hs);
 }
 
 Caps batch_caps(const cublox_batch_limits &lim) {
@@ -1817,7 +1936,24 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
                             size_t ws_bytes, int max_points, int near_steps,
                             int *d_export_op, int *d_export_hit,
                             cublox_stream s) {
-  if (!cfg || n <= 0 || !d_xs || !d_ys || !d_zs || !d_occ) return CUBLOX_OK;
+  return cublox_launch_integrate_delta(cfg, T, d_xs, d_ys, d_zs, n, d_occ, l_hit,
+                                       l_miss, l_min, l_max, l_unknown, d_ws,
+                                       ws_bytes, max_points, near_steps,
+                                       d_export_op, d_export_hit, nullptr,
+                                       nullptr, nullptr, 0u, s);
+}
+
+int cublox_launch_integrate_delta(
+    const cublox_raycast_cfg *cfg, const float *T, const float *d_xs,
+    const float *d_ys, const float *d_zs, int n, float *d_occ, float l_hit,
+    float l_miss, float l_min, float l_max, float l_unknown, void *d_ws,
+    size_t ws_bytes, int max_points, int near_steps, int *d_export_op,
+    int *d_export_hit, unsigned int *d_delta_count, int *d_delta_idx,
+    float *d_delta_val, unsigned int delta_capacity, cublox_stream s) {
+  if (!cfg || n <= 0 || !d_xs || !d_ys || !d_zs || !d_occ) {
+    // Nothing to integrate: nothing changed.
+    return d_delta_count ? cublox_fill_zero_async(d_delta_count, 1, s) : CUBLOX_OK;
+  }
   int st = check_cfg(*cfg, near_steps);
   if (st != CUBLOX_OK) return st;
   if (n > max_points) {
@@ -1835,11 +1971,16 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
   if ((d_export_op == nullptr) != (d_export_hit == nullptr)) {
     return set_error(CUBLOX_ERR_INVALID, "integrate: export needs both arrays");
   }
+  st = check_delta(d_delta_count, d_delta_idx, d_delta_val, delta_capacity,
+                   d_export_op != nullptr, "integrate");
+  if (st != CUBLOX_OK) return st;
   Batch b{};
   b.one = make_desc(*cfg, T, d_xs, d_ys, d_zs, n, d_occ,
                     ApplyArgs{l_hit, l_miss, l_min, l_max, l_unknown},
                     near_steps, d_export_op, d_export_hit);
-  return launch_sequence(b, L, bind(L, d_ws), as_stream(s));
+  set_delta(b.one, d_delta_count, d_delta_idx, d_delta_val, delta_capacity);
+  return launch_sequence(b, L, bind(L, d_ws), d_delta_count != nullptr,
+                         as_stream(s));
 }
 
 int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg) {
@@ -1897,11 +2038,19 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
     if ((q.d_export_op == nullptr) != (q.d_export_hit == nullptr)) {
       return set_error(CUBLOX_ERR_INVALID, "integrate_batch: export needs both arrays");
     }
+    if (q.d_delta_count && !lim->delta) {
+      return set_error(CUBLOX_ERR_INVALID,
+                       "integrate_batch: a changed-voxel stream needs limits.delta");
+    }
+    st = check_delta(q.d_delta_count, q.d_delta_idx, q.d_delta_val,
+                     q.delta_capacity, q.d_export_op != nullptr, "integrate_batch");
+    if (st != CUBLOX_OK) return st;
     const bool live = q.n > 0 && q.d_xs && q.d_ys && q.d_zs && q.d_occ;
     GridDesc d = make_desc(q.cfg, q.use_T ? q.T : nullptr, q.d_xs, q.d_ys, q.d_zs,
                            live ? q.n : 0, q.d_occ,
                            ApplyArgs{q.l_hit, q.l_miss, q.l_min, q.l_max, q.l_unknown},
                            near_k, q.d_export_op, q.d_export_hit);
+    set_delta(d, q.d_delta_count, q.d_delta_idx, q.d_delta_val, q.delta_capacity);
     if (d.g.max_seg > lim->max_segments_per_ray) {
       return set_error(CUBLOX_ERR_CAPACITY,
                        "integrate_batch: a grid needs more segments per ray than the limit");
@@ -1952,7 +2101,7 @@ int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
   Batch b{};
   b.descs = reinterpret_cast<const GridDesc *>(base + L.off_desc);
   b.hdr = reinterpret_cast<const BatchHeader *>(base + L.off_hdr);
-  return launch_sequence(b, L, bind(L, d_ws), hs);
+  return launch_sequence(b, L, bind(L, d_ws), lim->delta != 0, hs);
 }
 
 int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,

@@ -122,6 +122,26 @@ int cublox_og_set_near_steps(cublox_og *h, int k) {
   return guarded([&] { OGP->setNearSteps(k); });
 }
 
+int cublox_og_set_changed_capacity(cublox_og *h, unsigned int capacity) {
+  return guarded([&] { OGP->setChangedVoxelCapacity(capacity); });
+}
+
+int cublox_og_changed_voxels(const cublox_og *h, int *ids, float *vals,
+                             unsigned int out_cap, unsigned long long *count,
+                             unsigned int *written) {
+  return guarded([&] {
+    std::vector<OG::ChangedVoxel> v;
+    const unsigned long long c = OGC->changedVoxels(v);
+    const size_t n = std::min<size_t>(v.size(), out_cap);
+    for (size_t k = 0; k < n; ++k) {
+      ids[k] = v[k].hash_id;
+      vals[k] = v[k].log_odds;
+    }
+    if (count) *count = c;
+    if (written) *written = static_cast<unsigned int>(n);
+  });
+}
+
 int cublox_og_cast_rays_fused_sensor_device(cublox_og *h, const float *d_xs,
                                             const float *d_ys,
                                             const float *d_zs, int n,
@@ -278,7 +298,7 @@ int cublox_og_reset_kernel_times(cublox_og *h) {
 #define GBC reinterpret_cast<const cublox::GridBatch *>(b)
 #define GBP reinterpret_cast<cublox::GridBatch *>(b)
 
-cublox_gb *cublox_gb_create(const int limits[8]) {
+cublox_gb *cublox_gb_create(const int limits[9]) {
   cublox_gb *out = nullptr;
   g_err.clear();
   guarded([&] {
@@ -291,6 +311,7 @@ cublox_gb *cublox_gb_create(const int limits[8]) {
     l.min_chunk = limits[5];
     l.max_slabs = limits[6];
     l.max_regions = limits[7];
+    l.changed_voxels = limits[8] != 0;
     out = reinterpret_cast<cublox_gb *>(new cublox::GridBatch(l));
   });
   return out;

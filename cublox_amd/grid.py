@@ -167,6 +167,25 @@ class OccupancyGrid:
     def set_near_steps(self, k: int) -> None:
         self._check(self._L.cublox_og_set_near_steps(self._h, int(k)))
 
+    def set_changed_capacity(self, capacity: int) -> None:
+        """Record (hash id, new log-odds) of every voxel whose value an
+        update changes, at most `capacity` per update (0: off)."""
+        self._check(self._L.cublox_og_set_changed_capacity(self._h, int(capacity)))
+        self._changed_cap = int(capacity)
+
+    def changed_voxels(self):
+        """(ids int32, values float32, count) of the last update; count is
+        how many voxels changed (more than len(ids) when truncated)."""
+        cap = getattr(self, "_changed_cap", 0)
+        ids = np.zeros(max(cap, 1), np.int32)
+        vals = np.zeros(max(cap, 1), np.float32)
+        cnt = C.c_ulonglong(0)
+        wr = C.c_uint(0)
+        self._check(self._L.cublox_og_changed_voxels(
+            self._h, ids.ctypes.data_as(_ffi.i32p), vals.ctypes.data_as(_ffi.f32p),
+            cap, C.byref(cnt), C.byref(wr)))
+        return ids[:wr.value].copy(), vals[:wr.value].copy(), int(cnt.value)
+
     def cast_rays_fused_sensor_frame_device(self, d_xs: int, d_ys: int,
                                             d_zs: int, n: int, T) -> None:
         """Fused kernels, counts exported to the counter buffers."""

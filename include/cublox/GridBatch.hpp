@@ -32,6 +32,10 @@ struct BatchLimits {
   int min_chunk{256};               // smallest hot-tile chunk any tick uses
   int max_slabs{65536};             // 32 KB scratch slabs
   int max_regions{0};               // recenter regions per tick (0: 6 per grid)
+  // Grids with a changed-voxel stream (OccupancyGrid::setChangedVoxelCapacity)
+  // get it filled by each tick; needs this set (other grids pay nothing,
+  // but the folds of every tick take the stream-feeding variants).
+  bool changed_voxels{false};
 };
 
 // One grid's scan of one tick: device-resident columns, sensor-frame with

@@ -143,7 +143,7 @@ public:
   // reference's shape (global op/hit counters, then a sweep over all
   // voxels). Both give the same occupancy.
   enum class UpdatePath { FUSED_TILES, PER_VOXEL_ATOMICS };
-  void setUpdatePath(UpdatePath p) { update_path_ = p; }
+  void setUpdatePath(UpdatePath p);
   UpdatePath getUpdatePath() const { return update_path_; }
   // Steps of each ray handled in the near-sensor cube by the fused path.
   void setNearSteps(int k);
@@ -154,6 +154,29 @@ public:
   void castRaysFusedSensorFrameDevice(const float *d_xs, const float *d_ys,
                                       const float *d_zs, int n,
                                       const float T[12]);
+
+  // Changed-voxel stream (the reference's modified-voxel compaction,
+  // kernels.cu:202-209, which it compiles but never enables). capacity > 0:
+  // every later update also records (hash id, new log-odds) of each voxel
+  // whose stored value changed, at most `capacity` entries per update; 0
+  // turns it off. Recentering is not recorded. On PER_VOXEL_ATOMICS the
+  // capacity must be at least the voxel count (the reference's gate,
+  // kernels.cu:251-262).
+  void setChangedVoxelCapacity(unsigned int capacity);
+  unsigned int getChangedVoxelCapacity() const { return delta_capacity_; }
+  struct ChangedVoxel {
+    int hash_id;
+    float log_odds;
+  };
+  // The changes of the last update, in unspecified order (blocks). Returns
+  // how many voxels changed: more than out.size() when the capacity was too
+  // small.
+  unsigned long long changedVoxels(std::vector<ChangedVoxel> &out) const;
+  // Device views of the stream (count, hash ids, values), valid until the
+  // next update on this grid's stream.
+  const unsigned int *deviceChangedCount() const { return d_delta_count_; }
+  const int *deviceChangedIds() const { return d_delta_idx_; }
+  const float *deviceChangedValues() const { return d_delta_val_; }
 
   // Per-kernel device time measured with HIP events on this grid's stream
   // (off by default). update_ms covers the ray cast (and, on the fused
@@ -226,6 +249,11 @@ private:
   int *d_op_cnt_{nullptr};
   int *d_hit_cnt_{nullptr};
   float *d_occ_{nullptr};
+
+  unsigned int *d_delta_count_{nullptr};
+  int *d_delta_idx_{nullptr};
+  float *d_delta_val_{nullptr};
+  unsigned int delta_capacity_{0};
 
   float *d_cloud_x_{nullptr};
   float *d_cloud_y_{nullptr};

@@ -318,6 +318,24 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
                             int *d_export_op, int *d_export_hit,
                             cublox_stream s);
 
+/* Fused update with a changed-voxel stream (the reference's modified-voxel
+ * compaction in applyUpdateKernel, kernels.cu:202-209, gated off at
+ * kernels.cu:251-262, here enabled for the fused update): for every voxel
+ * whose stored value changes, (hash, new log-odds value) is appended to
+ * d_delta_idx/d_delta_val in unspecified order, and *d_delta_count counts
+ * them all (the launch sets it; the caller need not zero it). Entries at
+ * positions >= capacity
+ * are not written (count > capacity means the stream is truncated);
+ * capacity 0 with NULL arrays counts only. NULL d_delta_count: the same as
+ * cublox_launch_integrate. Not with a count export.                      */
+int cublox_launch_integrate_delta(
+    const cublox_raycast_cfg *cfg, const float *T, const float *d_xs,
+    const float *d_ys, const float *d_zs, int n, float *d_occ, float l_hit,
+    float l_miss, float l_min, float l_max, float l_unknown, void *d_ws,
+    size_t ws_bytes, int max_points, int near_steps, int *d_export_op,
+    int *d_export_hit, unsigned int *d_delta_count, int *d_delta_idx,
+    float *d_delta_val, unsigned int delta_capacity, cublox_stream s);
+
 /* Per-ray segment bound and tile count of a grid geometry (for sizing
  * batch limits). */
 int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg);
@@ -345,6 +363,8 @@ typedef struct {
                                any launch will use (0: 1024)             */
   int max_slabs;            /* hot-tile scratch slabs, 32 KB each (0:
                                65536); hot tiles past them run unsplit   */
+  int delta;                /* nonzero: grids may carry changed-voxel
+                               streams (selects the folds that feed them) */
 } cublox_batch_limits;
 
 typedef struct {
@@ -357,6 +377,12 @@ typedef struct {
   float l_hit, l_miss, l_min, l_max, l_unknown;
   int near_steps;           /* 1..max_near_steps (0: 8)                    */
   int *d_export_op, *d_export_hit; /* optional: counts instead of a fold  */
+  /* optional changed-voxel stream, as in cublox_launch_integrate_delta
+   * (needs limits.delta)                                                 */
+  unsigned int *d_delta_count;
+  int *d_delta_idx;
+  float *d_delta_val;
+  unsigned int delta_capacity;
 } cublox_integrate_grid;
 
 size_t cublox_integrate_batch_workspace_bytes(const cublox_batch_limits *lim);

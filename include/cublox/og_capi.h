@@ -79,6 +79,15 @@ int cublox_og_census(const cublox_og *h, unsigned long long out[4]);
 int cublox_og_set_update_path(cublox_og *h, int path);
 /* Near-field steps of the fused update, 1..12. */
 int cublox_og_set_near_steps(cublox_og *h, int k);
+/* Changed-voxel stream (no reference counterpart as a call; it enables the
+ * compaction of kernels.cu:202-209): capacity > 0 records the changes of
+ * each later update, 0 turns it off. */
+int cublox_og_set_changed_capacity(cublox_og *h, unsigned int capacity);
+/* Changes of the last update: *count = voxels that changed; the first
+ * min(*count, capacity, out_cap) go to ids/vals, *written of them. */
+int cublox_og_changed_voxels(const cublox_og *h, int *ids, float *vals,
+                             unsigned int out_cap, unsigned long long *count,
+                             unsigned int *written);
 
 /* recenter(pos) (hpp:57); *moved = 1 when the origin changed. */
 int cublox_og_recenter(cublox_og *h, const float pos[3], int *moved);
@@ -132,10 +141,10 @@ int cublox_og_reset_kernel_times(cublox_og *h);
 
 /* Batched updates of many grids (cublox::GridBatch, no reference
  * counterpart). limits: max_grids, max_points, max_tiles,
- * max_segments_per_ray, max_near_steps, min_chunk, max_slabs, max_regions
- * (8 ints; see include/cublox/GridBatch.hpp). */
+ * max_segments_per_ray, max_near_steps, min_chunk, max_slabs, max_regions,
+ * changed_voxels (9 ints; see include/cublox/GridBatch.hpp). */
 typedef struct cublox_gb cublox_gb;
-cublox_gb *cublox_gb_create(const int limits[8]);
+cublox_gb *cublox_gb_create(const int limits[9]);
 void cublox_gb_destroy(cublox_gb *b);
 int cublox_gb_segment_bound(const cublox_og *g);
 int cublox_gb_tile_count(const cublox_og *g);
