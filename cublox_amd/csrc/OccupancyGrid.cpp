@@ -93,6 +93,9 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_dense_pos_ = std::exchange(o.d_fetch_dense_pos_, nullptr);
   d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
+  d_raw_ = std::exchange(o.d_raw_, nullptr);
+  raw_capacity_ = std::exchange(o.raw_capacity_, 0);
+  d_kept_ = std::exchange(o.d_kept_, nullptr);
   d_delta_count_ = std::exchange(o.d_delta_count_, nullptr);
   d_delta_idx_ = std::exchange(o.d_delta_idx_, nullptr);
   d_delta_val_ = std::exchange(o.d_delta_val_, nullptr);
@@ -140,6 +143,9 @@ void OccupancyGrid::releaseAll() noexcept {
   freeDevice(d_delta_idx_);
   freeDevice(d_delta_val_);
   delta_capacity_ = 0;
+  freeDevice(d_raw_);
+  raw_capacity_ = 0;
+  freeDevice(d_kept_);
   freeDevice(d_op_cnt_);
   freeDevice(d_hit_cnt_);
   freeDevice(d_occ_);
@@ -394,7 +400,8 @@ void OccupancyGrid::ensureWorkspace(int n) {
 void OccupancyGrid::launchIntegrate(const float *d_xs, const float *d_ys,
                                     const float *d_zs, int n, const float *T,
                                     const Eigen::Vector3f &origin,
-                                    bool export_counts) {
+                                    bool export_counts, int stride,
+                                    float drop_sq) {
   ensureWorkspace(n);
   if (export_counts) ensureCounters();
   cublox_raycast_cfg cfg{};
@@ -408,13 +415,37 @@ void OccupancyGrid::launchIntegrate(const float *d_xs, const float *d_ys,
   cfg.map_vox_num = config_.voxel_num;
   cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
   const bool delta = !export_counts && delta_capacity_ > 0;
+  cublox_integrate_grid q{};
+  q.cfg = cfg;
+  if (T) {
+    for (int k = 0; k < 12; ++k) q.T[k] = T[k];
+    q.use_T = 1;
+  }
+  q.d_xs = d_xs;
+  q.d_ys = d_ys;
+  q.d_zs = d_zs;
+  q.n = n;
+  q.d_occ = d_occ_;
+  q.l_hit = l_hit_;
+  q.l_miss = l_miss_;
+  q.l_min = l_min_;
+  q.l_max = l_max_;
+  q.l_unknown = l_unknown_;
+  q.near_steps = near_steps_;
+  if (export_counts) {
+    q.d_export_op = d_op_cnt_;
+    q.d_export_hit = d_hit_cnt_;
+  }
+  if (delta) {
+    q.d_delta_count = d_delta_count_;
+    q.d_delta_idx = d_delta_idx_;
+    q.d_delta_val = d_delta_val_;
+    q.delta_capacity = delta_capacity_;
+  }
+  q.point_stride = stride;
+  q.range_drop_sq = drop_sq;
   mark(0);
-  CUBLOX_CHECK(cublox_launch_integrate_delta(
-      &cfg, T, d_xs, d_ys, d_zs, n, d_occ_, l_hit_, l_miss_, l_min_, l_max_,
-      l_unknown_, d_ws_, ws_bytes_, ws_points_, near_steps_,
-      export_counts ? d_op_cnt_ : nullptr, export_counts ? d_hit_cnt_ : nullptr,
-      delta ? d_delta_count_ : nullptr, delta ? d_delta_idx_ : nullptr,
-      delta ? d_delta_val_ : nullptr, delta ? delta_capacity_ : 0u, stream_));
+  CUBLOX_CHECK(cublox_launch_integrate_grid(&q, d_ws_, ws_bytes_, ws_points_, stream_));
   mark(1);
   pending_[0] = timing_;
 }
@@ -570,6 +601,120 @@ void OccupancyGrid::updateSensorFrameDevice(const float *d_xs,
   }
   sync();
   harvestTimes();
+}
+
+// ---- driver ingest ------------------------------------------------------
+
+const float *OccupancyGrid::rawFloats(const void *d_data, uint32_t off) const {
+  return reinterpret_cast<const float *>(static_cast<const unsigned char *>(d_data) + off);
+}
+
+// buildCloudInOdomFrame (main.cpp:640-693) fused into the update: the
+// driver's filter runs in the ray set-up; the census of kept points decides
+// the return value (and, on the first run, whether the origin is pinned).
+bool OccupancyGrid::ingest(const float *d_xs, const float *d_ys,
+                           const float *d_zs, int stride, int n,
+                           const float T[12]) {
+  if (update_path_ != UpdatePath::FUSED_TILES) {
+    throw std::logic_error("cublox::OccupancyGrid: sensor ingest runs on the fused path");
+  }
+  if (config_.voxel_num <= 0 || n <= 0) return false;
+  if (!d_kept_) {
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_kept_), sizeof(unsigned int)));
+  }
+  // main.cpp:451: the range filter is the grid's max range, squared in float.
+  const float drop_sq = max_raycast_range_ * max_raycast_range_;
+  CUBLOX_CHECK(cublox_launch_count_kept_points(d_xs, d_ys, d_zs, stride, n, drop_sq,
+                                               d_kept_, stream_));
+  unsigned int kept = 0;
+  if (first_run_) {
+    CUBLOX_CHECK(cublox_memcpy_d2h_async(&kept, d_kept_, sizeof(kept), stream_));
+    sync();
+    if (kept == 0) return false;
+  }
+  const Eigen::Vector3f t(T[3], T[7], T[11]);
+  pinFirstOrigin(t);
+  launchIntegrate(d_xs, d_ys, d_zs, n, T, t, false, stride, drop_sq);
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(&kept, d_kept_, sizeof(kept), stream_));
+  sync();
+  harvestTimes();
+  return kept > 0;
+}
+
+static void checkLayout(const OccupancyGrid::PointCloud2Layout &l) {
+  if (l.point_step == 0 || l.point_step % 4 != 0 || l.off_x % 4 != 0 ||
+      l.off_y % 4 != 0 || l.off_z % 4 != 0 || l.off_x + 4 > l.point_step ||
+      l.off_y + 4 > l.point_step || l.off_z + 4 > l.point_step) {
+    throw std::invalid_argument(
+        "cublox::OccupancyGrid: PointCloud2 layout needs a point step and "
+        "float32 x/y/z offsets that are multiples of 4 inside the record");
+  }
+}
+
+static int cappedPoints(uint32_t width, uint32_t height) {
+  const unsigned long long n = static_cast<unsigned long long>(width) * height;
+  return static_cast<int>(std::min<unsigned long long>(
+      n, static_cast<unsigned long long>(OccupancyGrid::kMaxCloudPoints)));
+}
+
+bool OccupancyGrid::updatePointCloud2Device(const void *d_data,
+                                            const PointCloud2Layout &layout,
+                                            const float T[12]) {
+  checkLayout(layout);
+  const int n = cappedPoints(layout.width, layout.height);
+  if (n == 0) return false;
+  return ingest(rawFloats(d_data, layout.off_x), rawFloats(d_data, layout.off_y),
+                rawFloats(d_data, layout.off_z),
+                static_cast<int>(layout.point_step / 4), n, T);
+}
+
+bool OccupancyGrid::updatePointCloud2(const void *data,
+                                      const PointCloud2Layout &layout,
+                                      const float T[12]) {
+  checkLayout(layout);
+  const int n = cappedPoints(layout.width, layout.height);
+  if (n == 0) return false;
+  const size_t bytes = static_cast<size_t>(n) * layout.point_step;
+  if (bytes > raw_capacity_) {
+    sync();
+    freeDevice(d_raw_);
+    raw_capacity_ = 0;
+    CUBLOX_CHECK(cublox_malloc(&d_raw_, bytes));
+    raw_capacity_ = bytes;
+  }
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_raw_, data, bytes, stream_));
+  return updatePointCloud2Device(d_raw_, layout, T);
+}
+
+bool OccupancyGrid::updateDepthImageDevice(const void *d_data, uint32_t width,
+                                           uint32_t height, uint32_t step,
+                                           DepthEncoding encoding,
+                                           const float K[4], const float T[12]) {
+  const int n = cappedPoints(width, height);
+  if (n == 0) return false;
+  ensureCloudCapacity(n);
+  CUBLOX_CHECK(cublox_launch_depth_to_points(
+      d_data, static_cast<int>(width), static_cast<int>(height), step,
+      encoding == DepthEncoding::MILLIMETRES_U16 ? 0 : 1, K, n, d_cloud_x_,
+      d_cloud_y_, d_cloud_z_, stream_));
+  return ingest(d_cloud_x_, d_cloud_y_, d_cloud_z_, 1, n, T);
+}
+
+bool OccupancyGrid::updateDepthImage(const void *data, uint32_t width,
+                                     uint32_t height, uint32_t step,
+                                     DepthEncoding encoding, const float K[4],
+                                     const float T[12]) {
+  if (width == 0 || height == 0) return false;
+  const size_t bytes = static_cast<size_t>(step) * height;
+  if (bytes > raw_capacity_) {
+    sync();
+    freeDevice(d_raw_);
+    raw_capacity_ = 0;
+    CUBLOX_CHECK(cublox_malloc(&d_raw_, bytes));
+    raw_capacity_ = bytes;
+  }
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_raw_, data, bytes, stream_));
+  return updateDepthImageDevice(d_raw_, width, height, step, encoding, K, T);
 }
 
 void OccupancyGrid::updateDevice(const float *d_x, const float *d_y,

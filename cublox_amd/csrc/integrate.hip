@@ -570,7 +570,7 @@ struct GridDesc {
   unsigned delta_cap;
 };
 
-static_assert(sizeof(GridDesc) == 288, "descriptor size (documented)");
+static_assert(sizeof(GridDesc) == 296, "descriptor size (documented)");
 
 struct BatchHeader {  // device-resident in batch mode
   int n_grids;
@@ -680,7 +680,8 @@ constexpr int kLenBuckets = 64;
 // similar classes in one wave only shorten the longest lane's walk.
 __device__ __forceinline__ int length_class(const GridDesc &d, int i) {
   const RayArgs &a = d.g.a;
-  float x = as_global(d.xs)[i], y = as_global(d.ys)[i], z = as_global(d.zs)[i];
+  const size_t at = static_cast<size_t>(i) * static_cast<size_t>(a.stride);
+  float x = as_global(d.xs)[at], y = as_global(d.ys)[at], z = as_global(d.zs)[at];
   if (!d.transform) {  // world frame: relative to the sensor
     x -= a.ox;
     y -= a.oy;
@@ -1745,6 +1746,24 @@ int check_delta(const unsigned *count, const int *idx, const float *val,
   return CUBLOX_OK;
 }
 
+// Point layout: point_stride floats between consecutive points (0: 1,
+// separate columns); a range drop applies to sensor-frame points only.
+int check_input(const cublox_integrate_grid &q, const char *who) {
+  if (q.point_stride < 0) {
+    return set_error(CUBLOX_ERR_INVALID, std::string(who) + ": negative point stride");
+  }
+  if (q.range_drop_sq > 0.0f && !q.use_T) {
+    return set_error(CUBLOX_ERR_INVALID,
+                     std::string(who) + ": a range drop needs sensor-frame points");
+  }
+  return CUBLOX_OK;
+}
+
+void set_input(GridDesc &d, const cublox_integrate_grid &q) {
+  d.g.a.stride = q.point_stride > 0 ? q.point_stride : 1;
+  d.g.a.drop_sq = q.range_drop_sq > 0.0f ? q.range_drop_sq : 0.0f;
+}
+
 void set_delta(GridDesc &d, unsigned *count, int *idx, float *val, unsigned cap) {
   d.delta_count = count;
   d.delta_idx = count ? idx : nullptr;
@@ -1950,16 +1969,53 @@ int cublox_launch_integrate_delta(
     size_t ws_bytes, int max_points, int near_steps, int *d_export_op,
     int *d_export_hit, unsigned int *d_delta_count, int *d_delta_idx,
     float *d_delta_val, unsigned int delta_capacity, cublox_stream s) {
-  if (!cfg || n <= 0 || !d_xs || !d_ys || !d_zs || !d_occ) {
-    // Nothing to integrate: nothing changed.
+  if (!cfg) {
     return d_delta_count ? cublox_fill_zero_async(d_delta_count, 1, s) : CUBLOX_OK;
   }
-  int st = check_cfg(*cfg, near_steps);
+  if (near_steps < 1 || near_steps > kMaxNearK) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate: near_steps out of [1, 12]");
+  }
+  cublox_integrate_grid q{};
+  q.cfg = *cfg;
+  if (T) {
+    for (int k = 0; k < 12; ++k) q.T[k] = T[k];
+    q.use_T = 1;
+  }
+  q.d_xs = d_xs;
+  q.d_ys = d_ys;
+  q.d_zs = d_zs;
+  q.n = n;
+  q.d_occ = d_occ;
+  q.l_hit = l_hit;
+  q.l_miss = l_miss;
+  q.l_min = l_min;
+  q.l_max = l_max;
+  q.l_unknown = l_unknown;
+  q.near_steps = near_steps;
+  q.d_export_op = d_export_op;
+  q.d_export_hit = d_export_hit;
+  q.d_delta_count = d_delta_count;
+  q.d_delta_idx = d_delta_idx;
+  q.d_delta_val = d_delta_val;
+  q.delta_capacity = delta_capacity;
+  return cublox_launch_integrate_grid(&q, d_ws, ws_bytes, max_points, s);
+}
+
+int cublox_launch_integrate_grid(const cublox_integrate_grid *q, void *d_ws,
+                                 size_t ws_bytes, int max_points,
+                                 cublox_stream s) {
+  if (!q) return set_error(CUBLOX_ERR_INVALID, "integrate: null grid");
+  if (q->n <= 0 || !q->d_xs || !q->d_ys || !q->d_zs || !q->d_occ) {
+    // Nothing to integrate: nothing changed.
+    return q->d_delta_count ? cublox_fill_zero_async(q->d_delta_count, 1, s) : CUBLOX_OK;
+  }
+  const int near_steps = q->near_steps > 0 ? q->near_steps : 8;
+  int st = check_cfg(q->cfg, near_steps);
   if (st != CUBLOX_OK) return st;
-  if (n > max_points) {
+  if (q->n > max_points) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
   }
-  const Caps caps = single_caps(*cfg, max_points, near_steps);
+  const Caps caps = single_caps(q->cfg, max_points, near_steps);
   if (!caps_fit(caps)) {
     return set_error(CUBLOX_ERR_CAPACITY,
                      "integrate: max_points x segment bound above 2^32 slots");
@@ -1968,18 +2024,23 @@ int cublox_launch_integrate_delta(
   if (!d_ws || ws_bytes < L.total) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
   }
-  if ((d_export_op == nullptr) != (d_export_hit == nullptr)) {
+  if ((q->d_export_op == nullptr) != (q->d_export_hit == nullptr)) {
     return set_error(CUBLOX_ERR_INVALID, "integrate: export needs both arrays");
   }
-  st = check_delta(d_delta_count, d_delta_idx, d_delta_val, delta_capacity,
-                   d_export_op != nullptr, "integrate");
+  st = check_delta(q->d_delta_count, q->d_delta_idx, q->d_delta_val,
+                   q->delta_capacity, q->d_export_op != nullptr, "integrate");
+  if (st != CUBLOX_OK) return st;
+  st = check_input(*q, "integrate");
   if (st != CUBLOX_OK) return st;
   Batch b{};
-  b.one = make_desc(*cfg, T, d_xs, d_ys, d_zs, n, d_occ,
-                    ApplyArgs{l_hit, l_miss, l_min, l_max, l_unknown},
-                    near_steps, d_export_op, d_export_hit);
-  set_delta(b.one, d_delta_count, d_delta_idx, d_delta_val, delta_capacity);
-  return launch_sequence(b, L, bind(L, d_ws), d_delta_count != nullptr,
+  b.one = make_desc(q->cfg, q->use_T ? q->T : nullptr, q->d_xs, q->d_ys, q->d_zs,
+                    q->n, q->d_occ,
+                    ApplyArgs{q->l_hit, q->l_miss, q->l_min, q->l_max, q->l_unknown},
+                    near_steps, q->d_export_op, q->d_export_hit);
+  set_input(b.one, *q);
+  set_delta(b.one, q->d_delta_count, q->d_delta_idx, q->d_delta_val,
+            q->delta_capacity);
+  return launch_sequence(b, L, bind(L, d_ws), q->d_delta_count != nullptr,
                          as_stream(s));
 }
 
@@ -2051,6 +2112,9 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
                            ApplyArgs{q.l_hit, q.l_miss, q.l_min, q.l_max, q.l_unknown},
                            near_k, q.d_export_op, q.d_export_hit);
     set_delta(d, q.d_delta_count, q.d_delta_idx, q.d_delta_val, q.delta_capacity);
+    st = check_input(q, "integrate_batch");
+    if (st != CUBLOX_OK) return st;
+    set_input(d, q);
     if (d.g.max_seg > lim->max_segments_per_ray) {
       return set_error(CUBLOX_ERR_CAPACITY,
                        "integrate_batch: a grid needs more segments per ray than the limit");

@@ -122,6 +122,40 @@ int cublox_og_set_near_steps(cublox_og *h, int k) {
   return guarded([&] { OGP->setNearSteps(k); });
 }
 
+int cublox_og_update_cloud2(cublox_og *h, const void *data, int on_device,
+                            const unsigned int layout[6], const float T[12],
+                            int *updated) {
+  return guarded([&] {
+    OG::PointCloud2Layout l;
+    l.width = layout[0];
+    l.height = layout[1];
+    l.point_step = layout[2];
+    l.off_x = layout[3];
+    l.off_y = layout[4];
+    l.off_z = layout[5];
+    const bool u = on_device ? OGP->updatePointCloud2Device(data, l, T)
+                             : OGP->updatePointCloud2(data, l, T);
+    if (updated) *updated = u ? 1 : 0;
+  });
+}
+
+int cublox_og_update_depth(cublox_og *h, const void *data, int on_device,
+                           unsigned int width, unsigned int height,
+                           unsigned int step, int encoding, const float K[4],
+                           const float T[12], int *updated) {
+  return guarded([&] {
+    if (encoding != 0 && encoding != 1) {
+      throw std::invalid_argument("cublox_og_update_depth: encoding must be 0 or 1");
+    }
+    const OG::DepthEncoding e = encoding == 0 ? OG::DepthEncoding::MILLIMETRES_U16
+                                              : OG::DepthEncoding::METRES_F32;
+    const bool u = on_device
+                       ? OGP->updateDepthImageDevice(data, width, height, step, e, K, T)
+                       : OGP->updateDepthImage(data, width, height, step, e, K, T);
+    if (updated) *updated = u ? 1 : 0;
+  });
+}
+
 int cublox_og_set_changed_capacity(cublox_og *h, unsigned int capacity) {
   return guarded([&] { OGP->setChangedVoxelCapacity(capacity); });
 }

@@ -45,6 +45,8 @@ struct RayArgs {
   float res, inv_res, max_range;
   int vox_num;
   int center;
+  int stride;      // floats from one point's x (y, z) to the next one's
+  float drop_sq;   // > 0: sensor-frame points with |p|^2 above it are dropped
 };
 
 inline RayArgs make_ray_args(const cublox_raycast_cfg &c) {
@@ -66,6 +68,8 @@ inline RayArgs make_ray_args(const cublox_raycast_cfg &c) {
   a.max_range = c.max_range;
   a.vox_num = c.map_vox_num;
   a.center = c.origin_at_center;
+  a.stride = 1;
+  a.drop_sq = 0.0f;
   return a;
 }
 
@@ -85,12 +89,15 @@ struct RayState {
 template <bool kTransform, typename P>
 __device__ __forceinline__ bool ray_setup(const RayArgs &a, P xs, P ys, P zs,
                                           int i, RayState &r) {
-  float ex = xs[i], ey = ys[i], ez = zs[i];
+  const size_t at = static_cast<size_t>(i) * static_cast<size_t>(a.stride);
+  float ex = xs[at], ey = ys[at], ez = zs[at];
   if (kTransform) {
     // Scan transform (driver main.cpp:672-682): drop non-finite sensor-frame
-    // points, then p_w = R p_s + t in a fixed operand order.
+    // points and, when asked, points beyond the range (|p|^2 > r^2,
+    // main.cpp:676-679), then p_w = R p_s + t in a fixed operand order.
     if (!isfinite(ex) || !isfinite(ey) || !isfinite(ez)) return false;
     const float px = ex, py = ey, pz = ez;
+    if (a.drop_sq > 0.0f && (px * px + py * py) + pz * pz > a.drop_sq) return false;
     ex = ((a.T[0] * px + a.T[1] * py) + a.T[2] * pz) + a.T[3];
     ey = ((a.T[4] * px + a.T[5] * py) + a.T[6] * pz) + a.T[7];
     ez = ((a.T[8] * px + a.T[9] * py) + a.T[10] * pz) + a.T[11];

@@ -167,6 +167,58 @@ class OccupancyGrid:
     def set_near_steps(self, k: int) -> None:
         self._check(self._L.cublox_og_set_near_steps(self._h, int(k)))
 
+    # ---- driver ingest (reference main.cpp:558-693, on the device) ------
+    def update_point_cloud2(self, data, width: int, height: int,
+                            point_step: int, offsets=(0, 4, 8), T=None,
+                            device: bool = False) -> bool:
+        """A PointCloud2 xyz payload (host bytes, or a device pointer with
+        device=True) in the sensor frame: the first 1,000,000 records,
+        finite and within the max raycast range, transformed by T and
+        integrated. False (nothing updated) when no point is kept."""
+        lay = (C.c_uint * 6)(int(width), int(height), int(point_step),
+                             int(offsets[0]), int(offsets[1]), int(offsets[2]))
+        Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+        upd = C.c_int(0)
+        if device:
+            ptr, keep = C.c_void_p(int(data)), None
+        else:
+            keep = np.ascontiguousarray(np.frombuffer(memoryview(data).cast("B"), np.uint8))
+            ptr = C.c_void_p(keep.ctypes.data)
+        self._check(self._L.cublox_og_update_cloud2(
+            self._h, ptr, 1 if device else 0, lay, _ffi.fptr(Tm), C.byref(upd)))
+        return bool(upd.value)
+
+    def update_depth_image(self, image, K, T, *, width: int = 0, height: int = 0,
+                           step: int = 0, encoding: Optional[int] = None,
+                           device: bool = False) -> bool:
+        """A depth image (uint16 millimetres or float32 metres; a 2-D numpy
+        array, or a device pointer with width/height/step/encoding given)
+        back-projected with K = (fx, fy, cx, cy) as the driver does, then
+        update_point_cloud2 of the result."""
+        if device:
+            ptr, keep = C.c_void_p(int(image)), None
+            if encoding is None or not width or not height or not step:
+                raise ValueError("a device image needs width, height, step and encoding")
+        else:
+            keep = np.ascontiguousarray(image)
+            if keep.dtype == np.uint16:
+                enc = 0
+            elif keep.dtype == np.float32:
+                enc = 1
+            else:
+                raise ValueError("depth images are uint16 (mm) or float32 (m)")
+            encoding = enc if encoding is None else encoding
+            height, width = keep.shape
+            step = keep.strides[0]
+            ptr = C.c_void_p(keep.ctypes.data)
+        Km = np.ascontiguousarray(np.asarray(K, np.float32).reshape(4))
+        Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+        upd = C.c_int(0)
+        self._check(self._L.cublox_og_update_depth(
+            self._h, ptr, 1 if device else 0, int(width), int(height), int(step),
+            int(encoding), _ffi.fptr(Km), _ffi.fptr(Tm), C.byref(upd)))
+        return bool(upd.value)
+
     def set_changed_capacity(self, capacity: int) -> None:
         """Record (hash id, new log-odds) of every voxel whose value an
         update changes, at most `capacity` per update (0: off)."""

@@ -11,6 +11,7 @@
 #pragma once
 
 #include <Eigen/Dense>
+#include <cstdint>
 #include <optional>
 #include <vector>
 
@@ -155,6 +156,38 @@ public:
                                       const float *d_zs, int n,
                                       const float T[12]);
 
+  // ---- driver ingest on the device (reference src/main.cpp) -------------
+  // A PointCloud2 xyz payload: width * height records of point_step bytes,
+  // float32 x/y/z at byte offsets off_x/off_y/off_z (all four multiples of
+  // 4), little-endian, sensor frame.
+  struct PointCloud2Layout {
+    uint32_t width{0}, height{1}, point_step{12};
+    uint32_t off_x{0}, off_y{4}, off_z{8};
+  };
+  static constexpr int kMaxCloudPoints = 1000000;  // main.cpp:54
+  // buildCloudInOdomFrame (main.cpp:640-693) then update(): the first
+  // kMaxCloudPoints records (main.cpp:655-660), kept when finite and within
+  // the max raycast range (main.cpp:672-679), transformed by T (row-major
+  // 3x4 [R | t], sensor -> world) and integrated with t as the sensor
+  // origin. Returns false, updating nothing, when no point is kept
+  // (main.cpp:686-690). Runs on the fused path. Blocks.
+  bool updatePointCloud2(const void *data, const PointCloud2Layout &layout,
+                         const float T[12]);
+  bool updatePointCloud2Device(const void *d_data,
+                               const PointCloud2Layout &layout,
+                               const float T[12]);
+  // depthCallback (main.cpp:558-638) then the above: the image (row pitch
+  // `step` bytes) is back-projected with K = {fx, fy, cx, cy} into the
+  // sensor-frame cloud the driver publishes, invalid pixels as NaN points.
+  enum class DepthEncoding { MILLIMETRES_U16, METRES_F32 };
+  bool updateDepthImage(const void *data, uint32_t width, uint32_t height,
+                        uint32_t step, DepthEncoding encoding,
+                        const float K[4], const float T[12]);
+  bool updateDepthImageDevice(const void *d_data, uint32_t width,
+                              uint32_t height, uint32_t step,
+                              DepthEncoding encoding, const float K[4],
+                              const float T[12]);
+
   // Changed-voxel stream (the reference's modified-voxel compaction,
   // kernels.cu:202-209, which it compiles but never enables). capacity > 0:
   // every later update also records (hash id, new log-odds) of each voxel
@@ -230,7 +263,10 @@ private:
   void launchApply();
   void launchIntegrate(const float *d_xs, const float *d_ys, const float *d_zs,
                        int n, const float *T, const Eigen::Vector3f &origin,
-                       bool export_counts);
+                       bool export_counts, int stride = 1, float drop_sq = 0.0f);
+  bool ingest(const float *d_xs, const float *d_ys, const float *d_zs,
+              int stride, int n, const float T[12]);
+  const float *rawFloats(const void *d_data, uint32_t off) const;
   void ensureCounters();
   void ensureWorkspace(int n);
   void sync() const;
@@ -249,6 +285,10 @@ private:
   int *d_op_cnt_{nullptr};
   int *d_hit_cnt_{nullptr};
   float *d_occ_{nullptr};
+
+  void *d_raw_{nullptr};  // ingest payload staging
+  size_t raw_capacity_{0};
+  unsigned int *d_kept_{nullptr};
 
   unsigned int *d_delta_count_{nullptr};
   int *d_delta_idx_{nullptr};

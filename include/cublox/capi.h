@@ -336,6 +336,32 @@ int cublox_launch_integrate_delta(
     int *d_export_hit, unsigned int *d_delta_count, int *d_delta_idx,
     float *d_delta_val, unsigned int delta_capacity, cublox_stream s);
 
+/* ------------------------------------------------------------------ */
+/* Sensor ingest (the driver's input side, reference src/main.cpp; no
+ * counterpart in the library API).                                       */
+/* ------------------------------------------------------------------ */
+/* Depth image back-projection (depthCallback, main.cpp:558-638): the first
+ * min(width*height, max_points) pixels in row-major order become
+ * sensor-frame points x = ((u - cx) * z) / fx, y = ((v - cy) * z) / fy, z.
+ * encoding 0: 16-bit depth in millimetres (16UC1, mono16; z = raw * 0.001,
+ * raw 0 invalid); 1: 32-bit float metres (32FC1; non-finite or <= 0
+ * invalid). Invalid pixels give quiet-NaN points, which the update drops.
+ * K = {fx, fy, cx, cy}; step_bytes is the image row pitch.               */
+int cublox_launch_depth_to_points(const void *d_image, int width, int height,
+                                  size_t step_bytes, int encoding,
+                                  const float K[4], int max_points,
+                                  float *d_xs, float *d_ys, float *d_zs,
+                                  cublox_stream s);
+/* Census of the driver's point filter (main.cpp:672-679): *d_count (set by
+ * the launch) = points that are finite and, for range_drop_sq > 0, have
+ * x*x + y*y + z*z <= range_drop_sq. Points laid out as in
+ * cublox_integrate_grid (point_stride 0: 1). buildCloudInOdomFrame returns
+ * false, and the driver skips the update, when this is 0 (main.cpp:686-690). */
+int cublox_launch_count_kept_points(const float *d_xs, const float *d_ys,
+                                    const float *d_zs, int point_stride, int n,
+                                    float range_drop_sq, unsigned int *d_count,
+                                    cublox_stream s);
+
 /* Per-ray segment bound and tile count of a grid geometry (for sizing
  * batch limits). */
 int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg);
@@ -383,7 +409,23 @@ typedef struct {
   int *d_delta_idx;
   float *d_delta_val;
   unsigned int delta_capacity;
+  /* Point layout (the driver's ingest, main.cpp:640-693): point k is
+   * (d_xs[k*s], d_ys[k*s], d_zs[k*s]) with s = point_stride (0: 1, separate
+   * columns). A packed PointCloud2 payload of point_step bytes with float32
+   * x/y/z at byte offsets ox/oy/oz is d_xs = data + ox (and so on) with
+   * point_stride = point_step / 4 (all four multiples of 4).              */
+  int point_stride;
+  /* > 0 (sensor-frame points only): points with x*x + y*y + z*z above it
+   * are dropped before the transform (main.cpp:676-679).                  */
+  float range_drop_sq;
 } cublox_integrate_grid;
+
+/* Single-grid fused update from a batch item (every option above; no
+ * reference counterpart as one call). Workspace as for
+ * cublox_launch_integrate (near_steps 0 means 8).                        */
+int cublox_launch_integrate_grid(const cublox_integrate_grid *grid, void *d_ws,
+                                 size_t ws_bytes, int max_points,
+                                 cublox_stream s);
 
 size_t cublox_integrate_batch_workspace_bytes(const cublox_batch_limits *lim);
 size_t cublox_integrate_batch_staging_bytes(const cublox_batch_limits *lim);

@@ -79,6 +79,18 @@ int cublox_og_census(const cublox_og *h, unsigned long long out[4]);
 int cublox_og_set_update_path(cublox_og *h, int path);
 /* Near-field steps of the fused update, 1..12. */
 int cublox_og_set_near_steps(cublox_og *h, int k);
+/* Driver ingest (reference main.cpp:558-693, run on the device):
+ * OccupancyGrid::updatePointCloud2 / updateDepthImage. layout = {width,
+ * height, point_step, off_x, off_y, off_z}; on_device: data is device
+ * memory. encoding 0: 16-bit millimetres, 1: 32-bit float metres; K = {fx,
+ * fy, cx, cy}. *updated = 1 when some point was kept (the grid updated). */
+int cublox_og_update_cloud2(cublox_og *h, const void *data, int on_device,
+                            const unsigned int layout[6], const float T[12],
+                            int *updated);
+int cublox_og_update_depth(cublox_og *h, const void *data, int on_device,
+                           unsigned int width, unsigned int height,
+                           unsigned int step, int encoding, const float K[4],
+                           const float T[12], int *updated);
 /* Changed-voxel stream (no reference counterpart as a call; it enables the
  * compaction of kernels.cu:202-209): capacity > 0 records the changes of
  * each later update, 0 turns it off. */

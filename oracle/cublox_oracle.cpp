@@ -370,6 +370,66 @@ extern "C" int orc_plan_recenter(const float pos[3], const float origin_f[3],
   return p.action;
 }
 
+/* main.cpp:54, 640-693. Eigen's squaredNorm of a Vector3f and its
+ * Isometry3f * Vector3f are taken as (x*x + y*y) + z*z and the row sums in
+ * column order (Eigen is not available here to pin either). */
+extern "C" int orc_build_cloud(const uint8_t *data, long long n_records,
+                               int point_step, int off_x, int off_y, int off_z,
+                               const float T[12], float max_range_sq,
+                               float *xw, float *yw, float *zw) {
+  if (n_records <= 0) return 0;
+  const long long n = std::min<long long>(n_records, 1000000);
+  int row = 0;
+  for (long long i = 0; i < n; ++i) {
+    const uint8_t *rec = data + i * point_step;
+    float lx, ly, lz;
+    std::memcpy(&lx, rec + off_x, 4);
+    std::memcpy(&ly, rec + off_y, 4);
+    std::memcpy(&lz, rec + off_z, 4);
+    if (!std::isfinite(lx) || !std::isfinite(ly) || !std::isfinite(lz)) continue;
+    if ((lx * lx + ly * ly) + lz * lz > max_range_sq) continue;
+    xw[row] = ((T[0] * lx + T[1] * ly) + T[2] * lz) + T[3];
+    yw[row] = ((T[4] * lx + T[5] * ly) + T[6] * lz) + T[7];
+    zw[row] = ((T[8] * lx + T[9] * ly) + T[10] * lz) + T[11];
+    ++row;
+  }
+  return row;
+}
+
+/* main.cpp:558-638. */
+extern "C" void orc_depth_to_cloud(const uint8_t *img, int width, int height,
+                                   long long step, int encoding,
+                                   const float K[4], float *xs, float *ys,
+                                   float *zs) {
+  const float fx = K[0], fy = K[1], cx = K[2], cy = K[3];
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  long long k = 0;
+  for (int v = 0; v < height; ++v) {
+    const uint8_t *row = img + static_cast<long long>(v) * step;
+    for (int u = 0; u < width; ++u, ++k) {
+      float z;
+      if (encoding == 0) {
+        uint16_t raw;
+        std::memcpy(&raw, row + 2 * u, 2);
+        if (raw == 0) {
+          xs[k] = ys[k] = zs[k] = nan;
+          continue;
+        }
+        z = static_cast<float>(raw) * 0.001f;
+      } else {
+        std::memcpy(&z, row + 4 * u, 4);
+        if (!std::isfinite(z) || z <= 0.f) {
+          xs[k] = ys[k] = zs[k] = nan;
+          continue;
+        }
+      }
+      xs[k] = (static_cast<float>(u) - cx) * z / fx;
+      ys[k] = (static_cast<float>(v) - cy) * z / fy;
+      zs[k] = z;
+    }
+  }
+}
+
 extern "C" void orc_transform_points(const float T[12], const float *xs,
                                      const float *ys, const float *zs, int n,
                                      float *xw, float *yw, float *zw) {

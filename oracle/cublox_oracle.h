@@ -20,7 +20,10 @@
  *   apply          — exhaustive closed form over small (stored, op, hit);
  *   recenter       — brute-force window-membership model of the slab plan;
  *   fetch          — brute-force enumeration of the window (multiset);
- *   transform      — closed form R·p + t in a fixed operand order.
+ *   transform      — closed form R·p + t in a fixed operand order;
+ *   ingest         — hand-worked decode/filter/back-projection cases (exact
+ *                    special values, range boundary, the 1 M cap, field
+ *                    offsets, invalid-pixel rules) in tests/test_oracle_ingest.py.
  * The reference CUDA cannot be compiled here (no nvcc, CUDA headers, Eigen),
  * so there is no oracle/_ref build.
  */
@@ -107,6 +110,23 @@ int *orc_grid_hit(orc_grid *g);
 int orc_grid_voxel_num(const orc_grid *g);
 void orc_grid_origin(const orc_grid *g, int origin_i[3], float origin_f[3]);
 void orc_grid_load_occ(orc_grid *g, const float *occ);
+
+/* ---- driver ingest (main.cpp:558-693) ---- */
+/* buildCloudInOdomFrame (main.cpp:640-693) over a PointCloud2 payload of
+ * n_records records of point_step bytes with float32 x/y/z at byte offsets
+ * off_x/off_y/off_z: the first min(n_records, 1000000) records, non-finite
+ * and ||p||^2 > max_range_sq dropped, the rest transformed by T (row-major
+ * 3x4 [R | t]) into xw/yw/zw (capacity min(n_records, 1000000)). Returns
+ * the rows written; 0 means the driver skips the update. */
+int orc_build_cloud(const uint8_t *data, long long n_records, int point_step,
+                    int off_x, int off_y, int off_z, const float T[12],
+                    float max_range_sq, float *xw, float *yw, float *zw);
+/* depthCallback (main.cpp:558-638): width*height sensor-frame points in
+ * row-major order, NaN for invalid pixels. encoding 0: 16-bit millimetres,
+ * 1: 32-bit float metres. K = {fx, fy, cx, cy}; step = row pitch in bytes. */
+void orc_depth_to_cloud(const uint8_t *img, int width, int height,
+                        long long step, int encoding, const float K[4],
+                        float *xs, float *ys, float *zs);
 
 /* Transform only (a1), used by the transform tests. */
 void orc_transform_points(const float T[12], const float *xs, const float *ys,

@@ -68,6 +68,10 @@ def lib() -> C.CDLL:
     sig("orc_grid_load_occ", None, vp, f32p)
     sig("orc_transform_points", None, f32p, f32p, f32p, f32p, i, f32p, f32p,
         f32p)
+    sig("orc_build_cloud", i, u8p, C.c_longlong, i, i, i, i, f32p, F, f32p,
+        f32p, f32p)
+    sig("orc_depth_to_cloud", None, u8p, i, i, C.c_longlong, i, f32p, f32p,
+        f32p, f32p)
     _lib = L
     return L
 
@@ -120,7 +124,28 @@ class OracleGrid:
         self._L.orc_grid_set_threads(self._h, int(n))
 
     def set_max_raycast_range(self, r: float) -> None:
+        self._max_range = float(r)
         self._L.orc_grid_set_max_range(self._h, float(r))
+
+    def update_cloud2(self, data, n_records, point_step, offsets, T) -> bool:
+        """The driver's run() step for a PointCloud2 payload: build the
+        odom-frame cloud (main.cpp:640-693) with the grid's max range, and
+        update only when a row was kept (main.cpp:450-453). Returns that."""
+        r = np.float32(getattr(self, "_max_range", 25.0))
+        xw, yw, zw = build_cloud(data, n_records, point_step, offsets, T, r * r)
+        if len(xw) == 0:
+            return False
+        Tm = np.asarray(T, np.float32).reshape(12)
+        self.update((xw, yw, zw), (Tm[3], Tm[7], Tm[11]))
+        return True
+
+    def update_depth(self, img, width, height, step, encoding, K, T) -> bool:
+        """depthCallback (main.cpp:558-638), then update_cloud2 on the cloud
+        it publishes."""
+        xs, ys, zs = depth_to_cloud(img, width, height, step, encoding, K)
+        packed = np.ascontiguousarray(np.stack([xs, ys, zs], 1), np.float32)
+        return self.update_cloud2(packed.view(np.uint8).reshape(-1), len(xs), 12,
+                                  (0, 4, 8), T)
 
     def set_log_odds_params(self, *vals) -> None:
         self._L.orc_grid_set_log_odds(self._h, *[float(v) for v in vals])
@@ -246,6 +271,35 @@ def transform_points(T, xs, ys, zs):
     L.orc_transform_points(_f(Tm), _f(xs), _f(ys), _f(zs), n, _f(out[0]),
                            _f(out[1]), _f(out[2]))
     return out
+
+
+def build_cloud(data, n_records, point_step, offsets, T, max_range_sq):
+    """orc_build_cloud: (xw, yw, zw) of the kept, transformed points."""
+    L = lib()
+    buf = np.ascontiguousarray(np.frombuffer(memoryview(data).cast("B"), np.uint8))
+    need = int(n_records) * int(point_step)
+    assert buf.size >= min(need, 1000000 * int(point_step)), "payload too short"
+    Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
+    cap = max(1, min(int(n_records), 1000000))
+    out = [np.empty(cap, np.float32) for _ in range(3)]
+    rows = L.orc_build_cloud(buf.ctypes.data_as(u8p), int(n_records), int(point_step),
+                             int(offsets[0]), int(offsets[1]), int(offsets[2]),
+                             _f(Tm), float(max_range_sq), _f(out[0]), _f(out[1]),
+                             _f(out[2]))
+    return [o[:rows].copy() for o in out]
+
+
+def depth_to_cloud(img, width, height, step, encoding, K):
+    """orc_depth_to_cloud: width*height sensor-frame points (NaN invalid)."""
+    L = lib()
+    buf = np.ascontiguousarray(np.frombuffer(memoryview(img).cast("B"), np.uint8))
+    assert buf.size >= int(step) * int(height), "image too short"
+    Km = np.ascontiguousarray(np.asarray(K, np.float32).reshape(4))
+    n = int(width) * int(height)
+    out = [np.empty(max(n, 1), np.float32) for _ in range(3)]
+    L.orc_depth_to_cloud(buf.ctypes.data_as(u8p), int(width), int(height), int(step),
+                         int(encoding), _f(Km), _f(out[0]), _f(out[1]), _f(out[2]))
+    return [o[:n].copy() for o in out]
 
 
 def logit(p: float) -> float:
