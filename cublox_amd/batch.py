@@ -64,8 +64,9 @@ class GridBatch:
             raise RuntimeError(self._L.cublox_og_last_error().decode())
 
     def tick(self, scans: Sequence[Tuple], recenters: Iterable[Tuple] = ()) -> None:
-        """scans: (grid, d_x, d_y, d_z, n, T 3x4 sensor->world) with device
-        buffers (objects with .ptr or ints); recenters: (grid, pos)."""
+        """scans: (grid, d_x, d_y, d_z, n, T 3x4 sensor->world[, point
+        stride[, range drop squared]]) with device buffers (objects with
+        .ptr or ints); recenters: (grid, pos)."""
         scans = list(scans)
         recenters = list(recenters)
         k = len(scans)
@@ -79,6 +80,9 @@ class GridBatch:
         T = np.ascontiguousarray(
             np.array([np.asarray(s[5], np.float32).reshape(12) for s in scans]
                      or [np.zeros(12, np.float32)], np.float32))
+        stride = np.array([int(s[6]) if len(s) > 6 else 0 for s in scans] or [0], np.int32)
+        drop = np.array([float(s[7]) if len(s) > 7 else 0.0 for s in scans] or [0.0],
+                        np.float32)
         R = C.c_void_p * max(len(recenters), 1)
         rg = R(*[r[0]._h for r in recenters])
         rp = np.ascontiguousarray(
@@ -86,7 +90,7 @@ class GridBatch:
                      or [np.zeros(3, np.float32)], np.float32))
         self._check(self._L.cublox_gb_tick(
             self._h, k, grids, xs, ys, zs, _ffi.iptr(n), _ffi.fptr(T),
-            len(recenters), rg, _ffi.fptr(rp)))
+            _ffi.iptr(stride), _ffi.fptr(drop), len(recenters), rg, _ffi.fptr(rp)))
 
     def set_graph_mode(self, on: bool) -> None:
         self._check(self._L.cublox_gb_set_graph_mode(self._h, 1 if on else 0))

@@ -204,6 +204,8 @@ void GridBatch::stage(
     q.near_steps = setting_.near_steps;
     q.d_export_op = nullptr;
     q.d_export_hit = nullptr;
+    q.point_stride = s.point_stride;
+    q.range_drop_sq = s.range_drop_sq;
     if (g.delta_capacity_ > 0) {
       q.d_delta_count = g.d_delta_count_;
       q.d_delta_idx = g.d_delta_idx_;

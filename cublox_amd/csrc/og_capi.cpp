@@ -364,6 +364,7 @@ int cublox_gb_tile_count(const cublox_og *h) {
 int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
                    const float *const *d_x, const float *const *d_y,
                    const float *const *d_z, const int *n, const float *T,
+                   const int *point_stride, const float *range_drop_sq,
                    int n_recenter, cublox_og *const *rgrids,
                    const float *rpos) {
   return guarded([&] {
@@ -377,6 +378,8 @@ int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
       s.n = n[k];
       s.sensor_frame = true;
       for (int i = 0; i < 12; ++i) s.T[static_cast<size_t>(i)] = T[12 * k + i];
+      if (point_stride) s.point_stride = point_stride[k];
+      if (range_drop_sq) s.range_drop_sq = range_drop_sq[k];
     }
     std::vector<std::pair<OG *, Eigen::Vector3f>> rc;
     for (int k = 0; k < n_recenter; ++k) {

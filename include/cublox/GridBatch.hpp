@@ -47,6 +47,13 @@ struct GridScan {
   bool sensor_frame{true};
   std::array<float, 12> T{};
   Eigen::Vector3f origin{Eigen::Vector3f::Zero()};
+  // Point layout and the driver's range drop (cublox_integrate_grid):
+  // point k is (d_x[k*s], d_y[k*s], d_z[k*s]), s = point_stride (0: 1), so
+  // a PointCloud2 payload is read in place; range_drop_sq > 0 drops
+  // sensor-frame points with |p|^2 above it (main.cpp:676-679). The
+  // driver's skip of a cloud with no kept point is not applied here.
+  int point_stride{0};
+  float range_drop_sq{0.0f};
 };
 
 struct TuneSetting {

@@ -161,11 +161,13 @@ void cublox_gb_destroy(cublox_gb *b);
 int cublox_gb_segment_bound(const cublox_og *g);
 int cublox_gb_tile_count(const cublox_og *g);
 /* One tick: n_scans scans (grids[k]: device columns d_x/d_y/d_z[k] of
- * n[k] points, sensor-frame with T[12k..12k+11]), then the recenter of
- * rgrids[k] toward rpos[3k..3k+2]. Blocks until done. */
+ * n[k] points, sensor-frame with T[12k..12k+11]; point_stride[k] and
+ * range_drop_sq[k] as in GridScan, either array may be NULL), then the
+ * recenter of rgrids[k] toward rpos[3k..3k+2]. Blocks until done. */
 int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
                    const float *const *d_x, const float *const *d_y,
                    const float *const *d_z, const int *n, const float *T,
+                   const int *point_stride, const float *range_drop_sq,
                    int n_recenter, cublox_og *const *rgrids,
                    const float *rpos);
 int cublox_gb_set_graph_mode(cublox_gb *b, int on);

@@ -168,3 +168,54 @@ def test_depth_images(cb, orc, gpu_ready, kind):
         g.recenter(base)
         o.recenter(base)
         assert_occ_bitexact(g.occupancy(), o.occupancy(), f"depth {kind} recenter {k}")
+
+
+def test_batched_ingest(cb, orc, gpu_ready):
+    """GridBatch ticks with PointCloud2 payloads read in place (record
+    stride, range drop) beside a plain column scan: every grid equals its
+    oracle run alone."""
+    specs = [  # half, resolution, max range, beams, azimuths, seed, layout
+        ((40, 30, 12), 0.1, 6.0, 16, 512, 41, (32, (16, 0, 8))),
+        ((37, 21, 9), 0.1, 12.0, 16, 256, 42, None),
+        ((50, 50, 16), 0.05, 5.0, 16, 1024, 43, (16, (0, 4, 8))),
+    ]
+    lo = cb.log_odds()
+    grids, oracles, scenes = [], [], []
+    for half, res, rng_, _b, _a, seed, _l in specs:
+        g = cb.OccupancyGrid(half, res, False, 1.0)
+        o = orc.OracleGrid(half, res, False, 1.0)
+        for x in (g, o):
+            x.set_max_raycast_range(rng_)
+            x.set_log_odds_params(*lo)
+        grids.append(g)
+        oracles.append(o)
+        scenes.append(cb.Scene(seed=seed, n_boxes=40, box_extent=10.0, speed=0.5))
+    total = sum(b * a for (_, _, _, b, a, _, _) in specs)
+    batch = cb.GridBatch.for_grids(grids, total, max_near_steps=12)
+    for k in range(4):
+        scans, recenters, keep = [], [], []
+        for i, (half, res, rng_, beams, az, seed, layout) in enumerate(specs):
+            s = scenes[i].scan(k, beams, az, -15.0, 15.0, rng_)
+            if layout is None:
+                bufs = [cb.DeviceBuffer.from_array(np.ascontiguousarray(c))
+                        for c in (s.xs, s.ys, s.zs)]
+                keep.append(bufs)
+                scans.append((grids[i], bufs[0], bufs[1], bufs[2], s.n, s.T))
+                oracles[i].update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+            else:
+                step, offs = layout
+                rec = _records((s.xs, s.ys, s.zs), step, offs)
+                buf = cb.DeviceBuffer.from_array(rec)
+                keep.append(buf)
+                r2 = float(np.float32(rng_) * np.float32(rng_))
+                scans.append((grids[i], buf.ptr + offs[0], buf.ptr + offs[1],
+                              buf.ptr + offs[2], s.n, s.T, step // 4, r2))
+                assert oracles[i].update_cloud2(rec, s.n, step, offs, s.T)
+            recenters.append((grids[i], s.base))
+            oracles[i].recenter(s.base)
+        print(f"[ingest batch] tick {k}", flush=True)
+        batch.tick(scans, recenters)
+        for i in range(len(specs)):
+            assert_occ_bitexact(grids[i].occupancy(), oracles[i].occupancy(),
+                                f"batched ingest tick {k} grid {i}")
+    batch.close()
