@@ -93,6 +93,9 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_dense_pos_ = std::exchange(o.d_fetch_dense_pos_, nullptr);
   d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
+  d_viz_ = std::exchange(o.d_viz_, nullptr);
+  d_viz_count_ = std::exchange(o.d_viz_count_, nullptr);
+  viz_capacity_ = std::exchange(o.viz_capacity_, 0);
   d_raw_ = std::exchange(o.d_raw_, nullptr);
   raw_capacity_ = std::exchange(o.raw_capacity_, 0);
   d_kept_ = std::exchange(o.d_kept_, nullptr);
@@ -145,6 +148,9 @@ void OccupancyGrid::releaseAll() noexcept {
   delta_capacity_ = 0;
   freeDevice(d_raw_);
   raw_capacity_ = 0;
+  freeDevice(d_viz_);
+  freeDevice(d_viz_count_);
+  viz_capacity_ = 0;
   freeDevice(d_kept_);
   freeDevice(d_op_cnt_);
   freeDevice(d_hit_cnt_);
@@ -852,6 +858,72 @@ std::vector<OccupancyGrid::OccupancySample> OccupancyGrid::fetchOccupancyAround(
         static_cast<VoxelState>(st[i])});
   }
   return out;
+}
+
+size_t OccupancyGrid::buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
+                                             const float max_viz_radius,
+                                             const int subsample,
+                                             const int max_points,
+                                             std::vector<VizPoint> &out) const {
+  out.clear();
+  if (d_occ_ == nullptr || config_.voxel_num <= 0) return 0;
+  // main.cpp:484-485; fetchOccupancyAround returns nothing for r <= 0.
+  const float radius = std::min(max_viz_radius, maxHorizontalInWindowRadius(robot_pos));
+  if (!(radius > 0.0f)) return 0;
+  cublox_fetch_cfg cfg{};
+  long long est = 0;
+  unsigned int output_cap = 0;
+  if (!cublox_fetch_plan(i3(origin_i_), i3(config_.map_size_i),
+                         i3(config_.half_map_size_i), config_.resolution,
+                         config_.resolution_inv, config_.origin_at_center ? 1 : 0,
+                         l_free_, l_occupied_, l_unknown_, f3(robot_pos), radius,
+                         static_cast<int>(VoxelState::OCCUPIED), max_points, &cfg,
+                         &est, &output_cap)) {
+    return 0;
+  }
+  const long long buf = std::min<long long>(output_cap, est);
+  if (buf <= 0) return 0;
+  const unsigned int max_out =
+      max_points > 0 ? std::min(static_cast<unsigned int>(max_points), output_cap)
+                     : output_cap;
+  const long long out_cap = std::min<long long>(buf, max_out);
+  if (out_cap > viz_capacity_) {
+    sync();
+    freeDevice(d_viz_);
+    viz_capacity_ = 0;
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_viz_),
+                               static_cast<size_t>(out_cap) * 4 * sizeof(float)));
+    viz_capacity_ = out_cap;
+  }
+  if (!d_viz_count_) {
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_viz_count_), 2 * sizeof(unsigned int)));
+  }
+  cublox_viz_cfg viz{};
+  viz.robot_pos = f3(robot_pos);
+  viz.max_viz_radius = max_viz_radius;
+  viz.inv_resolution = config_.resolution_inv;
+  viz.origin_at_center = config_.origin_at_center ? 1 : 0;
+  viz.subsample = subsample;
+  viz.max_points = static_cast<int>(max_out);
+  CUBLOX_CHECK(cublox_launch_viz_cloud(&cfg, d_occ_, &viz, static_cast<unsigned int>(buf),
+                                       d_viz_count_, d_viz_count_ + 1, d_viz_,
+                                       static_cast<unsigned int>(out_cap), stream_));
+  unsigned int counts[2] = {0, 0};
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(counts, d_viz_count_, sizeof(counts), stream_));
+  sync();
+  const unsigned int written =
+      std::min<unsigned int>(counts[1], static_cast<unsigned int>(out_cap));
+  if (written > 0) {
+    out.resize(written);
+    CUBLOX_CHECK(cublox_memcpy_d2h_async(out.data(), d_viz_,
+                                         static_cast<size_t>(written) * sizeof(VizPoint),
+                                         stream_));
+    sync();
+  }
+  // fetchOccupancyAround's sample count (OccupancyGrid.cpp:335-356).
+  unsigned int n = std::min(counts[0], output_cap);
+  if (max_points > 0) n = std::min(n, static_cast<unsigned int>(max_points));
+  return std::min<unsigned int>(n, static_cast<unsigned int>(buf));
 }
 
 void OccupancyGrid::fetchOccupancyBatch(

@@ -353,14 +353,16 @@ __global__ void __launch_bounds__(256)
 // are contiguous; kept samples are compacted with one atomic per wavefront.
 // ===================================================================
 
-__device__ __forceinline__ void fetch_body(const cublox_fetch_cfg &c,
+// Enumerates the disk x z-range of a fetch, classifies each in-sphere,
+// in-window voxel and numbers the kept ones with one atomic on *count per
+// wave; sink(keep, slot, px, py, pz, state) then runs in every lane of the
+// wave (slot is meaningful where keep is set).
+template <typename Sink>
+__device__ __forceinline__ void fetch_scan(const cublox_fetch_cfg &c,
                                            const float *__restrict__ occ,
                                            unsigned *__restrict__ count,
-                                           float *__restrict__ out_pos,
-                                           unsigned char *__restrict__ out_state,
-                                           const unsigned cap,
-                                           const unsigned max_results,
-                                           const int block, const int nblocks) {
+                                           const int block, const int nblocks,
+                                           Sink sink) {
   const int r = c.disk_radius_vox;
   const int diam = 2 * r + 1;
   const int nz = c.gz_max - c.gz_min + 1;
@@ -418,16 +420,106 @@ __device__ __forceinline__ void fetch_body(const cublox_fetch_cfg &c,
     unsigned base = 0;
     if (lane == leader) base = atomic_add_u32_ret(count, __popcll(m));
     base = __shfl(base, leader);
-    if (keep) {
-      const unsigned slot = base + __popcll(m & lt_mask);
-      if (!((max_results > 0u && slot >= max_results) || slot >= cap)) {
-        out_pos[3ull * slot + 0] = px;
-        out_pos[3ull * slot + 1] = py;
-        out_pos[3ull * slot + 2] = pz;
-        out_state[slot] = st;
-      }
-    }
+    sink(keep, base + __popcll(m & lt_mask), px, py, pz, st);
   }
+}
+
+__device__ __forceinline__ void fetch_body(const cublox_fetch_cfg &c,
+                                           const float *__restrict__ occ,
+                                           unsigned *__restrict__ count,
+                                           float *__restrict__ out_pos,
+                                           unsigned char *__restrict__ out_state,
+                                           const unsigned cap,
+                                           const unsigned max_results,
+                                           const int block, const int nblocks) {
+  fetch_scan(c, occ, count, block, nblocks,
+             [&](bool keep, unsigned slot, float px, float py, float pz,
+                 unsigned char st) {
+               if (keep && !((max_results > 0u && slot >= max_results) || slot >= cap)) {
+                 out_pos[3ull * slot + 0] = px;
+                 out_pos[3ull * slot + 1] = py;
+                 out_pos[3ull * slot + 2] = pz;
+                 out_state[slot] = st;
+               }
+             });
+}
+
+// ===================================================================
+// Occupancy viz cloud (the driver's publishOccupancyCloudAsync,
+// main.cpp:740-817, over the OCCUPIED fetch of main.cpp:487-491), fused
+// into the fetch: a fetched sample (slot below the fetch's max_results and
+// capacity) is dropped by the global-index subsample (main.cpp:71-84),
+// else coloured by its distance to the robot (main.cpp:275-291), packed
+// as a float (main.cpp:293-304) and appended as (x, y, z, rgb).
+// ===================================================================
+
+struct VizArgs {
+  float px, py, pz;   // robot position (distances are measured from it)
+  float d_max;        // colour-map range
+  float inv_res;      // grid resolution inverse and binning, for the
+  int center;         // subsample's global index
+  int step;           // subsample step (<= 1: keep every sample)
+  unsigned max_fetch; // the fetch's max_results (0: no limit)
+  unsigned fetch_cap; // the fetch's output capacity
+};
+
+// main.cpp:275-304; std::clamp order kept (a NaN passes through).
+__device__ __forceinline__ float viz_rgb(float d, float d_max) {
+  const float dm = d_max < 1.0f ? 1.0f : d_max;  // std::max(d_max, 1.0f)
+  const float q = d / dm;
+  const float t = q < 0.0f ? 0.0f : (1.0f < q ? 1.0f : q);
+  float r, g, b;
+  if (t <= 0.5f) {
+    const float k = t * 2.0f;
+    r = 0.0f;
+    g = 1.0f - k;
+    b = k;
+  } else {
+    const float k = (t - 0.5f) * 2.0f;
+    r = k;
+    g = 0.0f;
+    b = 1.0f - k;
+  }
+  auto u8 = [](float c) {
+    const float cc = c < 0.0f ? 0.0f : (1.0f < c ? 1.0f : c);
+    return static_cast<unsigned>(cc * 255.0f);
+  };
+  return __uint_as_float((u8(r) << 16) | (u8(g) << 8) | u8(b));
+}
+
+__global__ void __launch_bounds__(256)
+    viz_kernel(const cublox_fetch_cfg c, const float *__restrict__ occ,
+               const VizArgs v, unsigned *__restrict__ fetch_count,
+               unsigned *__restrict__ viz_count, float4 *__restrict__ out,
+               const unsigned out_cap) {
+  const int lane = __lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const bool ctr = v.center != 0;
+  fetch_scan(c, occ, fetch_count, blockIdx.x, gridDim.x,
+             [&](bool keep, unsigned slot, float px, float py, float pz,
+                 unsigned char) {
+               bool take = keep && !((v.max_fetch > 0u && slot >= v.max_fetch) ||
+                                     slot >= v.fetch_cap);
+               if (take && v.step > 1) {
+                 const unsigned mix =
+                     static_cast<unsigned>(pos_to_global(px, v.inv_res, ctr)) ^
+                     static_cast<unsigned>(pos_to_global(py, v.inv_res, ctr)) ^
+                     static_cast<unsigned>(pos_to_global(pz, v.inv_res, ctr));
+                 take = mix % static_cast<unsigned>(v.step) == 0u;
+               }
+               const unsigned long long m = __ballot(take);
+               if (m == 0ull) return;
+               const int leader = __ffsll(static_cast<long long>(m)) - 1;
+               unsigned base = 0;
+               if (lane == leader) base = atomic_add_u32_ret(viz_count, __popcll(m));
+               base = __shfl(base, leader);
+               if (!take) return;
+               const unsigned k = base + __popcll(m & lt_mask);
+               if (k >= out_cap) return;
+               const float dx = px - v.px, dy = py - v.py, dz = pz - v.pz;
+               const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+               out[k] = make_float4(px, py, pz, viz_rgb(d, v.d_max));
+             });
 }
 
 __global__ void __launch_bounds__(256)
@@ -957,6 +1049,41 @@ int cublox_launch_fetch_occupancy_around(const cublox_fetch_cfg *cfg,
                  as_stream(s)>>>(*cfg, d_occ, d_count, d_pos, d_state, cap,
                                  max_results);
   return check_launch("fetch_kernel");
+}
+
+int cublox_launch_viz_cloud(const cublox_fetch_cfg *cfg, const float *d_occ,
+                            const cublox_viz_cfg *viz,
+                            unsigned int fetch_capacity,
+                            unsigned int *d_fetch_count,
+                            unsigned int *d_viz_count, float *d_out,
+                            unsigned int out_capacity, cublox_stream s) {
+  if (!cfg || !viz || !d_occ || !d_fetch_count || !d_viz_count || !d_out) {
+    return set_error(CUBLOX_ERR_INVALID, "viz_cloud: null argument");
+  }
+  int st = cublox_fill_zero_async(d_fetch_count, 1, s);
+  if (st == CUBLOX_OK) st = cublox_fill_zero_async(d_viz_count, 1, s);
+  if (st != CUBLOX_OK) return st;
+  if (cfg->disk_radius_vox < 0 || cfg->gz_min > cfg->gz_max || fetch_capacity == 0) {
+    return CUBLOX_OK;  // kernels.cu:570-574: nothing fetched
+  }
+  VizArgs v{};
+  v.px = viz->robot_pos.x;
+  v.py = viz->robot_pos.y;
+  v.pz = viz->robot_pos.z;
+  v.d_max = viz->max_viz_radius;
+  v.inv_res = viz->inv_resolution;
+  v.center = viz->origin_at_center;
+  v.step = viz->subsample;
+  v.max_fetch = viz->max_points > 0 ? static_cast<unsigned>(viz->max_points) : 0u;
+  v.fetch_cap = fetch_capacity;
+  const long long diam = 2LL * cfg->disk_radius_vox + 1;
+  const long long total = diam * diam * (cfg->gz_max - cfg->gz_min + 1);
+  long long blocks = (total + 255) / 256;
+  blocks = std::max<long long>(1, std::min<long long>(blocks, 4096));
+  viz_kernel<<<dim3(static_cast<unsigned>(blocks)), dim3(256), 0, as_stream(s)>>>(
+      *cfg, d_occ, v, d_fetch_count, d_viz_count, reinterpret_cast<float4 *>(d_out),
+      out_capacity);
+  return check_launch("viz_kernel");
 }
 
 int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,

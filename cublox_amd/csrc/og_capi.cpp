@@ -156,6 +156,26 @@ int cublox_og_update_depth(cublox_og *h, const void *data, int on_device,
   });
 }
 
+int cublox_og_viz_cloud(const cublox_og *h, const float robot_pos[3],
+                        float max_viz_radius, int subsample, int max_points,
+                        float *out4, unsigned int out_cap, unsigned int *written,
+                        unsigned long long *fetched) {
+  return guarded([&] {
+    std::vector<OG::VizPoint> v;
+    const size_t f = OGC->buildOccupancyVizCloud(v3(robot_pos), max_viz_radius,
+                                                 subsample, max_points, v);
+    const size_t n = std::min<size_t>(v.size(), out_cap);
+    for (size_t k = 0; k < n; ++k) {
+      out4[4 * k + 0] = v[k].x;
+      out4[4 * k + 1] = v[k].y;
+      out4[4 * k + 2] = v[k].z;
+      out4[4 * k + 3] = v[k].rgb;
+    }
+    if (written) *written = static_cast<unsigned int>(n);
+    if (fetched) *fetched = f;
+  });
+}
+
 int cublox_og_set_changed_capacity(cublox_og *h, unsigned int capacity) {
   return guarded([&] { OGP->setChangedVoxelCapacity(capacity); });
 }

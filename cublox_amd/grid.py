@@ -219,6 +219,19 @@ class OccupancyGrid:
             int(encoding), _ffi.fptr(Km), _ffi.fptr(Tm), C.byref(upd)))
         return bool(upd.value)
 
+    def occupancy_viz_cloud(self, robot_pos, max_viz_radius: float,
+                            subsample: int = 1, max_points: int = 100000):
+        """The driver's viz cloud: ((k, 4) float32 records x, y, z, rgb in
+        unspecified order, the driver's fetched sample count)."""
+        cap = max(int(max_points), 1) if max_points > 0 else 1 << 22
+        out = np.empty((cap, 4), np.float32)
+        wr = C.c_uint(0)
+        fetched = C.c_ulonglong(0)
+        self._check(self._L.cublox_og_viz_cloud(
+            self._h, _ffi.fptr(_f3(robot_pos)), float(max_viz_radius), int(subsample),
+            int(max_points), _ffi.fptr(out), cap, C.byref(wr), C.byref(fetched)))
+        return out[:wr.value].copy(), int(fetched.value)
+
     def set_changed_capacity(self, capacity: int) -> None:
         """Record (hash id, new log-odds) of every voxel whose value an
         update changes, at most `capacity` per update (0: off)."""

@@ -188,6 +188,21 @@ public:
                               DepthEncoding encoding, const float K[4],
                               const float T[12]);
 
+  // The driver's occupancy viz cloud (main.cpp:484-491 fetch, 740-817
+  // build) in one launch: the OCCUPIED fetch around robot_pos with radius
+  // min(max_viz_radius, maxHorizontalInWindowRadius(robot_pos)) and
+  // max_points results, each fetched sample dropped by the global-index
+  // subsample (main.cpp:71-84) or kept as (x, y, z, rgb) with the driver's
+  // distance colour packed into a float (main.cpp:275-304). Record order is
+  // unspecified. Returns the driver's fetched sample count. Blocks.
+  struct VizPoint {
+    float x, y, z, rgb;
+  };
+  size_t buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
+                                float max_viz_radius, int subsample,
+                                int max_points,
+                                std::vector<VizPoint> &out) const;
+
   // Changed-voxel stream (the reference's modified-voxel compaction,
   // kernels.cu:202-209, which it compiles but never enables). capacity > 0:
   // every later update also records (hash id, new log-odds) of each voxel
@@ -308,6 +323,9 @@ private:
   mutable float *d_fetch_dense_pos_{nullptr};
   mutable unsigned char *d_fetch_dense_state_{nullptr};
   mutable unsigned int *d_fetch_offsets_{nullptr};
+  mutable float *d_viz_{nullptr};
+  mutable unsigned int *d_viz_count_{nullptr};
+  mutable long long viz_capacity_{0};
   mutable long long fetch_dense_capacity_{0};
   mutable int fetch_offsets_capacity_{0};
   mutable long long fetch_capacity_{0};

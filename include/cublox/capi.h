@@ -263,6 +263,32 @@ int cublox_launch_fetch_occupancy_around(const cublox_fetch_cfg *cfg,
                                          unsigned int max_results,
                                          cublox_stream s);
 
+/* Occupancy viz cloud (the driver's publishOccupancyCloudAsync,
+ * main.cpp:740-817, over its OCCUPIED fetch, main.cpp:487-491; no library
+ * counterpart): one launch fetches as cublox_launch_fetch_occupancy_around
+ * with max_results = max_points (cfg from cublox_fetch_plan with the
+ * OCCUPIED filter), drops fetched samples by the global-index subsample
+ * (main.cpp:71-84) and writes the rest as (x, y, z, rgb) float records,
+ * rgb the driver's distance colour packed into a float (main.cpp:275-304).
+ * *d_fetch_count (the fetch's logical count; the driver's sample count is
+ * min(it, max_points)) and *d_viz_count (records written, past
+ * out_capacity not stored) are set by the launch. Record order is
+ * unspecified, as the fetch order is in the reference.                   */
+typedef struct {
+  cublox_float3 robot_pos;  /* distances are measured from it             */
+  float max_viz_radius;     /* colour-map range (values below 1 act as 1) */
+  float inv_resolution;     /* grid binning for the subsample's index     */
+  int origin_at_center;
+  int subsample;            /* keep where (gx ^ gy ^ gz) % subsample == 0 */
+  int max_points;           /* the fetch's max_results (<= 0: none)       */
+} cublox_viz_cfg;
+int cublox_launch_viz_cloud(const cublox_fetch_cfg *cfg, const float *d_occ,
+                            const cublox_viz_cfg *viz,
+                            unsigned int fetch_capacity,
+                            unsigned int *d_fetch_count,
+                            unsigned int *d_viz_count, float *d_out,
+                            unsigned int out_capacity, cublox_stream s);
+
 /* Batched fetch: n_queries configs in DEVICE memory, one launch. Query q
  * writes at most per_query_cap samples at [q*per_query_cap, ...) and its
  * logical count to d_counts[q] (zeroed by the caller). */

@@ -91,6 +91,13 @@ int cublox_og_update_depth(cublox_og *h, const void *data, int on_device,
                            unsigned int width, unsigned int height,
                            unsigned int step, int encoding, const float K[4],
                            const float T[12], int *updated);
+/* The driver's occupancy viz cloud (OccupancyGrid::buildOccupancyVizCloud,
+ * main.cpp:484-491 and 740-817): up to out_cap (x, y, z, rgb) records to
+ * out4; *written records, *fetched the driver's sample count. */
+int cublox_og_viz_cloud(const cublox_og *h, const float robot_pos[3],
+                        float max_viz_radius, int subsample, int max_points,
+                        float *out4, unsigned int out_cap, unsigned int *written,
+                        unsigned long long *fetched);
 /* Changed-voxel stream (no reference counterpart as a call; it enables the
  * compaction of kernels.cu:202-209): capacity > 0 records the changes of
  * each later update, 0 turns it off. */

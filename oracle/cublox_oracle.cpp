@@ -430,6 +430,56 @@ extern "C" void orc_depth_to_cloud(const uint8_t *img, int width, int height,
   }
 }
 
+/* main.cpp:275-291 then 293-304. */
+static float viz_color_packed(float d, float d_max) {
+  const float d_max_clamped = std::max(d_max, 1.0f);
+  const float t = std::clamp(d / d_max_clamped, 0.0f, 1.0f);
+  float r, g, b;
+  if (t <= 0.5f) {
+    const float k = t * 2.0f;
+    r = 0.0f;
+    g = 1.0f - k;
+    b = k;
+  } else {
+    const float k = (t - 0.5f) * 2.0f;
+    r = k;
+    g = 0.0f;
+    b = 1.0f - k;
+  }
+  const uint32_t ur = static_cast<uint32_t>(std::clamp(r, 0.f, 1.f) * 255.f);
+  const uint32_t ug = static_cast<uint32_t>(std::clamp(g, 0.f, 1.f) * 255.f);
+  const uint32_t ub = static_cast<uint32_t>(std::clamp(b, 0.f, 1.f) * 255.f);
+  const uint32_t packed = (ur << 16) | (ug << 8) | ub;
+  float rgb = 0.f;
+  std::memcpy(&rgb, &packed, sizeof(float));
+  return rgb;
+}
+
+/* main.cpp:71-84 and 763-790. */
+extern "C" int orc_viz_points(const float *pos, int n, const float robot[3],
+                              float d_max, float inv_res, int origin_at_center,
+                              int step, int max_pts, float *out4) {
+  int written = 0;
+  for (int i = 0; i < n && written < max_pts; ++i) {
+    const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    if (step > 1) {
+      const unsigned mix =
+          static_cast<unsigned>(orc_pos_to_global(x, inv_res, origin_at_center)) ^
+          static_cast<unsigned>(orc_pos_to_global(y, inv_res, origin_at_center)) ^
+          static_cast<unsigned>(orc_pos_to_global(z, inv_res, origin_at_center));
+      if (mix % static_cast<unsigned>(step) != 0u) continue;
+    }
+    const float dx = x - robot[0], dy = y - robot[1], dz = z - robot[2];
+    const float d = std::sqrt((dx * dx + dy * dy) + dz * dz);
+    out4[4 * written + 0] = x;
+    out4[4 * written + 1] = y;
+    out4[4 * written + 2] = z;
+    out4[4 * written + 3] = viz_color_packed(d, d_max);
+    ++written;
+  }
+  return written;
+}
+
 extern "C" void orc_transform_points(const float T[12], const float *xs,
                                      const float *ys, const float *zs, int n,
                                      float *xw, float *yw, float *zw) {

@@ -128,6 +128,16 @@ void orc_depth_to_cloud(const uint8_t *img, int width, int height,
                         long long step, int encoding, const float K[4],
                         float *xs, float *ys, float *zs);
 
+/* publishOccupancyCloudAsync (main.cpp:740-817) over fetched samples
+ * (pos: 3 floats each, the fetch's order): in order, while fewer than
+ * max_pts are written, a sample is dropped by the global-index subsample
+ * (main.cpp:71-84), else written to out4 as (x, y, z, rgb) with rgb the
+ * distance colour of main.cpp:275-304 packed into a float. Returns the
+ * records written. */
+int orc_viz_points(const float *pos, int n, const float robot[3], float d_max,
+                   float inv_res, int origin_at_center, int step, int max_pts,
+                   float *out4);
+
 /* Transform only (a1), used by the transform tests. */
 void orc_transform_points(const float T[12], const float *xs, const float *ys,
                           const float *zs, int n, float *xw, float *yw,

@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
         f32p, f32p)
     sig("orc_depth_to_cloud", None, u8p, i, i, C.c_longlong, i, f32p, f32p,
         f32p, f32p)
+    sig("orc_viz_points", i, f32p, i, f32p, F, F, i, i, i, f32p)
     _lib = L
     return L
 
@@ -300,6 +301,18 @@ def depth_to_cloud(img, width, height, step, encoding, K):
     L.orc_depth_to_cloud(buf.ctypes.data_as(u8p), int(width), int(height), int(step),
                          int(encoding), _f(Km), _f(out[0]), _f(out[1]), _f(out[2]))
     return [o[:n].copy() for o in out]
+
+
+def viz_points(pos, robot, d_max, inv_res, center, step, max_pts):
+    """orc_viz_points: (k, 4) float32 records (x, y, z, rgb) in sample order."""
+    L = lib()
+    pos = np.ascontiguousarray(np.asarray(pos, np.float32).reshape(-1, 3))
+    r = _a3(robot, np.float32)
+    n = len(pos)
+    out = np.empty((max(min(n, int(max_pts)), 1), 4), np.float32)
+    k = L.orc_viz_points(_f(pos), n, _f(r), float(d_max), float(inv_res),
+                         1 if center else 0, int(step), int(max_pts), _f(out))
+    return out[:k].copy()
 
 
 def logit(p: float) -> float:
