@@ -204,6 +204,11 @@ def host() -> C.CDLL:
     _sig(L, "cublox_og_set_update_path", i, vp, i)
     _sig(L, "cublox_og_set_near_steps", i, vp, i)
     _sig(L, "cublox_og_set_changed_capacity", i, vp, C.c_uint)
+    _sig(L, "cublox_og_save_snapshot", i, vp, C.POINTER(vp), C.POINTER(C.c_size_t))
+    _sig(L, "cublox_og_load_snapshot", i, vp, vp, C.c_size_t)
+    _sig(L, "cublox_og_free_buffer", None, vp)
+    _sig(L, "cublox_og_save_snapshot_file", i, vp, C.c_char_p)
+    _sig(L, "cublox_og_load_snapshot_file", i, vp, C.c_char_p)
     _sig(L, "cublox_og_viz_cloud", i, vp, f32p, C.c_float, i, i, f32p, C.c_uint,
          u32p, C.POINTER(C.c_ulonglong))
     _sig(L, "cublox_og_update_cloud2", i, vp, vp, i, u32p, f32p, i32p)

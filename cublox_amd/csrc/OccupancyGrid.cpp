@@ -3,12 +3,15 @@
 // device only through the C ABI of cublox/capi.h.
 #include <cublox/OccupancyGrid.hpp>
 #include <cublox/capi.h>
+#include <cublox/snapshot.h>
 #include <cublox/utils.hpp>
 
 #include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -924,6 +927,196 @@ size_t OccupancyGrid::buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
   unsigned int n = std::min(counts[0], output_cap);
   if (max_points > 0) n = std::min(n, static_cast<unsigned int>(max_points));
   return std::min<unsigned int>(n, static_cast<unsigned int>(buf));
+}
+
+// ---- map snapshots ----------------------------------------------------
+
+std::vector<uint8_t> OccupancyGrid::saveSnapshot() const {
+  cublox_snapshot_header h{};
+  std::memcpy(h.magic, "CBLXSNAP", 8);
+  h.version = CUBLOX_SNAPSHOT_VERSION;
+  for (int a = 0; a < 3; ++a) {
+    h.half_map_size[a] = config_.half_map_size_i[a];
+    h.origin_i[a] = origin_i_[a];
+    h.origin_f[a] = origin_f_[a];
+  }
+  h.resolution = config_.resolution;
+  h.origin_at_center = config_.origin_at_center ? 1 : 0;
+  h.has_threshold = config_.recenter_threshold.has_value() ? 1 : 0;
+  h.recenter_threshold = config_.recenter_threshold.value_or(0.0f);
+  h.first_run = first_run_ ? 1 : 0;
+  h.l_hit = l_hit_;
+  h.l_miss = l_miss_;
+  h.l_min = l_min_;
+  h.l_max = l_max_;
+  h.l_free = l_free_;
+  h.l_occupied = l_occupied_;
+  h.l_unknown = l_unknown_;
+  h.max_raycast_range = max_raycast_range_;
+  h.voxel_num = config_.voxel_num;
+  const size_t V = static_cast<size_t>(std::max(config_.voxel_num, 0));
+  std::vector<uint8_t> out;
+  if (V == 0 || d_occ_ == nullptr) {
+    h.encoding = CUBLOX_SNAPSHOT_DENSE;
+    h.payload_bytes = 0;
+    out.resize(sizeof(h));
+    std::memcpy(out.data(), &h, sizeof(h));
+    return out;
+  }
+  // Count the nonzero voxels; sparse when 8 bytes each beat 4 per voxel.
+  const size_t scratch = cublox_sparse_scratch_bytes(config_.voxel_num);
+  void *d_scratch = nullptr;
+  int *d_ids = nullptr;
+  float *d_vals = nullptr;
+  try {
+    CUBLOX_CHECK(cublox_malloc(&d_scratch, scratch + 256));
+    unsigned int *d_count = reinterpret_cast<unsigned int *>(
+        static_cast<char *>(d_scratch) + scratch);
+    CUBLOX_CHECK(cublox_launch_sparse_compact(d_occ_, config_.voxel_num, d_scratch,
+                                              scratch, nullptr, nullptr, 0u,
+                                              d_count, stream_));
+    unsigned int nnz = 0;
+    CUBLOX_CHECK(cublox_memcpy_d2h_async(&nnz, d_count, sizeof(nnz), stream_));
+    sync();
+    if (static_cast<size_t>(nnz) * 8 + 8 < V * 4) {
+      h.encoding = CUBLOX_SNAPSHOT_SPARSE;
+      h.payload_bytes = 8 + static_cast<uint64_t>(nnz) * 8;
+      out.resize(sizeof(h) + h.payload_bytes);
+      uint8_t *p = out.data() + sizeof(h);
+      const uint32_t head[2] = {nnz, 0u};
+      std::memcpy(p, head, 8);
+      if (nnz > 0) {
+        CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_ids), nnz * sizeof(int)));
+        CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_vals), nnz * sizeof(float)));
+        CUBLOX_CHECK(cublox_launch_sparse_compact(d_occ_, config_.voxel_num, d_scratch,
+                                                  scratch, d_ids, d_vals, nnz,
+                                                  d_count, stream_));
+        CUBLOX_CHECK(cublox_memcpy_d2h_async(p + 8, d_ids, nnz * sizeof(int), stream_));
+        CUBLOX_CHECK(cublox_memcpy_d2h_async(p + 8 + static_cast<size_t>(nnz) * 4, d_vals,
+                                             nnz * sizeof(float), stream_));
+        sync();
+      }
+    } else {
+      h.encoding = CUBLOX_SNAPSHOT_DENSE;
+      h.payload_bytes = V * 4;
+      out.resize(sizeof(h) + h.payload_bytes);
+      CUBLOX_CHECK(cublox_memcpy_d2h_async(out.data() + sizeof(h), d_occ_, V * 4, stream_));
+      sync();
+    }
+  } catch (...) {
+    freeDevice(d_ids);
+    freeDevice(d_vals);
+    freeDevice(d_scratch);
+    throw;
+  }
+  freeDevice(d_ids);
+  freeDevice(d_vals);
+  freeDevice(d_scratch);
+  std::memcpy(out.data(), &h, sizeof(h));
+  return out;
+}
+
+void OccupancyGrid::loadSnapshot(const uint8_t *data, size_t bytes) {
+  auto bad = [](const char *why) {
+    throw std::invalid_argument(std::string("cublox::OccupancyGrid::loadSnapshot: ") + why);
+  };
+  if (data == nullptr || bytes < sizeof(cublox_snapshot_header)) bad("truncated header");
+  cublox_snapshot_header h;
+  std::memcpy(&h, data, sizeof(h));
+  if (std::memcmp(h.magic, "CBLXSNAP", 8) != 0) bad("not a cublox snapshot");
+  if (h.version != CUBLOX_SNAPSHOT_VERSION) bad("unsupported version");
+  for (int a = 0; a < 3; ++a) {
+    if (h.half_map_size[a] != config_.half_map_size_i[a]) bad("half map size differs");
+  }
+  if (h.resolution != config_.resolution) bad("resolution differs");
+  if ((h.origin_at_center != 0) != config_.origin_at_center) bad("binning differs");
+  if (h.voxel_num != config_.voxel_num) bad("voxel count differs");
+  if (h.payload_bytes != bytes - sizeof(h)) bad("payload size differs from the header");
+  const size_t V = static_cast<size_t>(std::max(config_.voxel_num, 0));
+  const uint8_t *p = data + sizeof(h);
+  uint32_t nnz = 0;
+  if (h.encoding == CUBLOX_SNAPSHOT_DENSE) {
+    if (h.payload_bytes != V * 4 && !(h.payload_bytes == 0 && d_occ_ == nullptr)) {
+      bad("dense payload is not one float per voxel");
+    }
+  } else if (h.encoding == CUBLOX_SNAPSHOT_SPARSE) {
+    if (h.payload_bytes < 8) bad("sparse payload without a count");
+    std::memcpy(&nnz, p, 4);
+    if (h.payload_bytes != 8 + static_cast<uint64_t>(nnz) * 8 || nnz > V) {
+      bad("sparse payload size differs from its count");
+    }
+  } else {
+    bad("unknown encoding");
+  }
+  if (V > 0 && d_occ_ != nullptr) {
+    sync();
+    if (h.encoding == CUBLOX_SNAPSHOT_DENSE) {
+      CUBLOX_CHECK(cublox_memcpy_h2d_async(d_occ_, p, V * 4, stream_));
+      sync();
+    } else {
+      int *d_ids = nullptr;
+      float *d_vals = nullptr;
+      try {
+        if (nnz > 0) {
+          CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_ids), nnz * sizeof(int)));
+          CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_vals), nnz * sizeof(float)));
+          CUBLOX_CHECK(cublox_memcpy_h2d_async(d_ids, p + 8, nnz * sizeof(int), stream_));
+          CUBLOX_CHECK(cublox_memcpy_h2d_async(d_vals, p + 8 + static_cast<size_t>(nnz) * 4,
+                                               nnz * sizeof(float), stream_));
+        }
+        CUBLOX_CHECK(cublox_launch_sparse_restore(d_occ_, config_.voxel_num, d_ids, d_vals,
+                                                  nnz, stream_));
+        sync();
+      } catch (...) {
+        freeDevice(d_ids);
+        freeDevice(d_vals);
+        throw;
+      }
+      freeDevice(d_ids);
+      freeDevice(d_vals);
+    }
+  }
+  // Counters (per-voxel path) start from zero after a restore.
+  if (d_op_cnt_) {
+    CUBLOX_CHECK(cublox_fill_zero_async(d_op_cnt_, V, stream_));
+    CUBLOX_CHECK(cublox_fill_zero_async(d_hit_cnt_, V, stream_));
+    sync();
+  }
+  updateOriginAndBound(Eigen::Vector3i(h.origin_i[0], h.origin_i[1], h.origin_i[2]));
+  first_run_ = h.first_run != 0;
+  if (h.has_threshold) {
+    config_.recenter_threshold = h.recenter_threshold;
+    config_.recenter_threshold_squared = h.recenter_threshold * h.recenter_threshold;
+  } else {
+    config_.recenter_threshold.reset();
+    config_.recenter_threshold_squared = 0.0f;
+  }
+  l_hit_ = h.l_hit;
+  l_miss_ = h.l_miss;
+  l_min_ = h.l_min;
+  l_max_ = h.l_max;
+  l_free_ = h.l_free;
+  l_occupied_ = h.l_occupied;
+  l_unknown_ = h.l_unknown;
+  max_raycast_range_ = h.max_raycast_range;
+}
+
+void OccupancyGrid::saveSnapshotFile(const std::string &path) const {
+  const std::vector<uint8_t> b = saveSnapshot();
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  f.write(reinterpret_cast<const char *>(b.data()), static_cast<std::streamsize>(b.size()));
+  if (!f) throw std::runtime_error("cublox::OccupancyGrid::saveSnapshotFile: cannot write " + path);
+}
+
+void OccupancyGrid::loadSnapshotFile(const std::string &path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) throw std::runtime_error("cublox::OccupancyGrid::loadSnapshotFile: cannot read " + path);
+  const std::streamsize n = f.tellg();
+  f.seekg(0);
+  std::vector<uint8_t> b(static_cast<size_t>(std::max<std::streamsize>(n, 0)));
+  f.read(reinterpret_cast<char *>(b.data()), n);
+  if (!f) throw std::runtime_error("cublox::OccupancyGrid::loadSnapshotFile: cannot read " + path);
+  loadSnapshot(b.data(), b.size());
 }
 
 void OccupancyGrid::fetchOccupancyBatch(

@@ -119,4 +119,35 @@ __host__ __device__ __forceinline__ bool inside_window(int gx, int gy, int gz,
   return abs(gx - ox) <= hx && abs(gy - oy) <= hy && abs(gz - oz) <= hz;
 }
 
+// ---------------------------------------------------------------------
+// Wave and block scans (wave64).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int wave_scan_incl(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// Inclusive block scan of one int per thread: within waves by shuffles,
+// then across the wave totals (s_wave: one int per wave).
+__device__ __forceinline__ int block_scan_incl(int *s_wave, int v) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  v = wave_scan_incl(v);
+  if (lane == 63) s_wave[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    const int x = wave_scan_incl(lane < nw ? s_wave[lane] : 0);
+    if (lane < nw) s_wave[lane] = x;
+  }
+  __syncthreads();
+  const int r = v + (wv > 0 ? s_wave[wv - 1] : 0);
+  __syncthreads();
+  return r;
+}
+
 } // namespace cublox_hip

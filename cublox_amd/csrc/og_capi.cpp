@@ -8,6 +8,7 @@
 #include <cublox/og_capi.h>
 #include <cublox/utils.hpp>
 
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -174,6 +175,35 @@ int cublox_og_viz_cloud(const cublox_og *h, const float robot_pos[3],
     if (written) *written = static_cast<unsigned int>(n);
     if (fetched) *fetched = f;
   });
+}
+
+int cublox_og_save_snapshot(const cublox_og *h, void **out, size_t *bytes) {
+  return guarded([&] {
+    if (!out || !bytes) throw std::invalid_argument("cublox_og_save_snapshot: null output");
+    *out = nullptr;
+    *bytes = 0;
+    const std::vector<uint8_t> b = OGC->saveSnapshot();
+    void *p = std::malloc(b.size());
+    if (!p) throw std::bad_alloc();
+    std::memcpy(p, b.data(), b.size());
+    *out = p;
+    *bytes = b.size();
+  });
+}
+
+int cublox_og_load_snapshot(cublox_og *h, const void *data, size_t bytes) {
+  return guarded(
+      [&] { OGP->loadSnapshot(static_cast<const uint8_t *>(data), bytes); });
+}
+
+void cublox_og_free_buffer(void *p) { std::free(p); }
+
+int cublox_og_save_snapshot_file(const cublox_og *h, const char *path) {
+  return guarded([&] { OGC->saveSnapshotFile(path ? path : ""); });
+}
+
+int cublox_og_load_snapshot_file(cublox_og *h, const char *path) {
+  return guarded([&] { OGP->loadSnapshotFile(path ? path : ""); });
 }
 
 int cublox_og_set_changed_capacity(cublox_og *h, unsigned int capacity) {

@@ -232,6 +232,27 @@ class OccupancyGrid:
             int(max_points), _ffi.fptr(out), cap, C.byref(wr), C.byref(fetched)))
         return out[:wr.value].copy(), int(fetched.value)
 
+    # ---- map snapshots (format: include/cublox/snapshot.h) ---------------
+    def save_snapshot(self) -> bytes:
+        out = C.c_void_p(None)
+        n = C.c_size_t(0)
+        self._check(self._L.cublox_og_save_snapshot(self._h, C.byref(out), C.byref(n)))
+        try:
+            return C.string_at(out, n.value)
+        finally:
+            self._L.cublox_og_free_buffer(out)
+
+    def load_snapshot(self, data: bytes) -> None:
+        buf = np.frombuffer(data, np.uint8)
+        self._check(self._L.cublox_og_load_snapshot(
+            self._h, C.c_void_p(buf.ctypes.data), buf.size))
+
+    def save_snapshot_file(self, path: str) -> None:
+        self._check(self._L.cublox_og_save_snapshot_file(self._h, str(path).encode()))
+
+    def load_snapshot_file(self, path: str) -> None:
+        self._check(self._L.cublox_og_load_snapshot_file(self._h, str(path).encode()))
+
     def set_changed_capacity(self, capacity: int) -> None:
         """Record (hash id, new log-odds) of every voxel whose value an
         update changes, at most `capacity` per update (0: off)."""

@@ -13,6 +13,7 @@
 #include <Eigen/Dense>
 #include <cstdint>
 #include <optional>
+#include <string>
 #include <vector>
 
 #include <cublox/Grid.hpp>
@@ -202,6 +203,18 @@ public:
                                 float max_viz_radius, int subsample,
                                 int max_points,
                                 std::vector<VizPoint> &out) const;
+
+  // Map snapshot (no reference counterpart; format in cublox/snapshot.h):
+  // geometry, window origin, settings and the occupancy, dense or as its
+  // nonzero voxels, whichever is smaller. Blocks.
+  std::vector<uint8_t> saveSnapshot() const;
+  // Restores a snapshot of a grid with the same half map size, resolution
+  // and binning (std::invalid_argument otherwise, or for a damaged
+  // snapshot): occupancy, origin, first-run flag, recenter threshold,
+  // log-odds parameters and ray-cast range. Blocks.
+  void loadSnapshot(const uint8_t *data, size_t bytes);
+  void saveSnapshotFile(const std::string &path) const;
+  void loadSnapshotFile(const std::string &path);
 
   // Changed-voxel stream (the reference's modified-voxel compaction,
   // kernels.cu:202-209, which it compiles but never enables). capacity > 0:

@@ -388,6 +388,26 @@ int cublox_launch_count_kept_points(const float *d_xs, const float *d_ys,
                                     float range_drop_sq, unsigned int *d_count,
                                     cublox_stream s);
 
+/* ------------------------------------------------------------------ */
+/* Map snapshots (no reference counterpart: the reference cannot save or
+ * restore a grid). The sparse form of an occupancy buffer is the (hash,
+ * value) pairs of its voxels with a nonzero bit pattern, ascending hash.  */
+/* ------------------------------------------------------------------ */
+size_t cublox_sparse_scratch_bytes(int voxel_num);
+/* *d_count (set by the launch) = the nonzero voxels; the first
+ * min(count, capacity) pairs go to d_ids/d_vals (capacity 0: count only).
+ * d_scratch holds cublox_sparse_scratch_bytes(voxel_num) bytes.          */
+int cublox_launch_sparse_compact(const float *d_occ, int voxel_num,
+                                 void *d_scratch, size_t scratch_bytes,
+                                 int *d_ids, float *d_vals,
+                                 unsigned int capacity, unsigned int *d_count,
+                                 cublox_stream s);
+/* occ := 0, then occ[ids[k]] = vals[k] for k < n (ids outside [0, V)
+ * skipped).                                                              */
+int cublox_launch_sparse_restore(float *d_occ, int voxel_num, const int *d_ids,
+                                 const float *d_vals, unsigned int n,
+                                 cublox_stream s);
+
 /* Per-ray segment bound and tile count of a grid geometry (for sizing
  * batch limits). */
 int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg);

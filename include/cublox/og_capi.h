@@ -98,6 +98,14 @@ int cublox_og_viz_cloud(const cublox_og *h, const float robot_pos[3],
                         float max_viz_radius, int subsample, int max_points,
                         float *out4, unsigned int out_cap, unsigned int *written,
                         unsigned long long *fetched);
+/* Map snapshots (OccupancyGrid::saveSnapshot / loadSnapshot; format in
+ * cublox/snapshot.h). save: *out is malloc'ed (free it with
+ * cublox_og_free_buffer), *bytes its size. */
+int cublox_og_save_snapshot(const cublox_og *h, void **out, size_t *bytes);
+int cublox_og_load_snapshot(cublox_og *h, const void *data, size_t bytes);
+void cublox_og_free_buffer(void *p);
+int cublox_og_save_snapshot_file(const cublox_og *h, const char *path);
+int cublox_og_load_snapshot_file(cublox_og *h, const char *path);
 /* Changed-voxel stream (no reference counterpart as a call; it enables the
  * compaction of kernels.cu:202-209): capacity > 0 records the changes of
  * each later update, 0 turns it off. */

@@ -138,3 +138,23 @@ def test_cpp_dropin_index_roundtrips(cb, tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "0 failures" in out.stdout
+
+
+def test_snapshot_header_layout(tmp_path):
+    """The C header's struct has the field offsets the format documents
+    (the tests' numpy reader uses the same table)."""
+    from snapshot_format import HEADER
+    fields = [n for n in HEADER.names]
+    src = tmp_path / "off.c"
+    src.write_text(
+        "#include <stddef.h>\n#include <stdio.h>\n#include <cublox/snapshot.h>\n"
+        "int main(void){printf(\"%zu\\n\", sizeof(cublox_snapshot_header));\n"
+        + "".join(f"printf(\"%zu\\n\", offsetof(cublox_snapshot_header, {f}));\n"
+                  for f in fields) + "return 0;}\n")
+    exe = tmp_path / "off"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert out[0] == HEADER.itemsize == 120
+    assert out[1:] == [HEADER.fields[f][1] for f in fields]
