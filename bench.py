@@ -430,6 +430,135 @@ def run_fetch(cb, w, g, scene, k, reps=5):
                 samples=int(counts.sum()))
 
 
+def _records(scans, n, step=16):
+    """All scans as PointCloud2 payloads (x, y, z, intensity; 16 B/point)."""
+    rec = np.zeros((len(scans) * n, step // 4), np.float32)
+    for i, s in enumerate(scans):
+        rec[i * n:(i + 1) * n, 0] = s.xs
+        rec[i * n:(i + 1) * n, 1] = s.ys
+        rec[i * n:(i + 1) * n, 2] = s.zs
+        rec[i * n:(i + 1) * n, 3] = 1.0
+    return rec.reshape(-1)
+
+
+def _depth_frames(count, W=640, H=480, seed=5):
+    rng = np.random.default_rng(seed)
+    u = np.arange(W, dtype=np.float32)[None, :]
+    v = np.arange(H, dtype=np.float32)[:, None]
+    out = []
+    for k in range(count):
+        d = 1.5 + 5.5 * (v / H) + 0.8 * np.sin(u / 37.0 + k) + rng.normal(0, 0.02, (H, W))
+        img = np.round(d * 1000).astype(np.uint16)
+        img[rng.random((H, W)) < 0.1] = 0
+        out.append(img)
+    return out
+
+
+def run_driver_steps(cb, w, scans, bufs, n, warmup, steps, g, base_update_ms):
+    """The driver-side rows on the device: ingest of PointCloud2 payloads
+    and depth images, the changed-voxel stream's cost, the viz cloud and
+    map snapshots (all on the measured workload's grid shape)."""
+    out = {}
+    total = warmup + steps
+
+    # Changed-voxel stream: the same update with the stream on.
+    gd = new_grid(cb, w)
+    gd.set_changed_capacity(1 << 24)
+    gd.enable_kernel_timing(True)
+    changed = []
+    for i in range(total):
+        off = 4 * n * i
+        s = scans[i]
+        if i == warmup:
+            gd.reset_kernel_times()
+        gd.update_sensor_frame_device(bufs[0].ptr + off, bufs[1].ptr + off,
+                                      bufs[2].ptr + off, n, s.T)
+        if i >= warmup:
+            changed.append(gd.changed_voxels()[2])
+        gd.recenter(s.base)
+    kt = gd.kernel_times()
+    upd = kt["update_ms"] / max(kt["update_launches"], 1)
+    out["changed_voxels"] = dict(update_ms=upd, plain_update_ms=base_update_ms,
+                                 overhead=upd / base_update_ms - 1.0,
+                                 changed_per_scan=statistics.mean(changed))
+    gd.close()
+
+    # Ingest: PointCloud2 payloads read in place (range drop, transform).
+    rec = cb.DeviceBuffer.from_array(_records(scans, n))
+    gi = new_grid(cb, w)
+    gi.enable_kernel_timing(True)
+    per = []
+    for i in range(total):
+        s = scans[i]
+        if i == warmup:
+            gi.reset_kernel_times()
+            cb.synchronize()
+        a = time.perf_counter()
+        gi.update_point_cloud2(rec.ptr + 16 * n * i, n, 1, 16, (0, 4, 8), s.T, device=True)
+        gi.recenter(s.base)
+        if i >= warmup:
+            per.append((time.perf_counter() - a) * 1e3)
+    kt = gi.kernel_times()
+    r2 = np.float32(w.max_range) ** 2
+    kept = [int(np.count_nonzero((s.xs * s.xs + s.ys * s.ys) + s.zs * s.zs <= r2))
+            for s in scans[warmup:total]]
+    out["ingest_point_cloud2"] = dict(
+        ms_per_scan=statistics.mean(per), p99_ms=pct(per, 99),
+        update_ms=kt["update_ms"] / max(kt["update_launches"], 1),
+        point_step=16, kept_points_per_scan=statistics.mean(kept),
+        note="misses beyond the range are dropped, as the driver does")
+    gi.close()
+    del rec
+
+    # Depth images (640x480, 16-bit mm), device-resident.
+    frames = _depth_frames(min(total, 30))
+    dbuf = cb.DeviceBuffer.from_array(np.stack(frames).reshape(-1))
+    gz = new_grid(cb, w)
+    K = (525.0, 525.0, 319.5, 239.5)
+    per = []
+    for k in range(len(frames)):
+        T = np.array([[0, 0, 1, 0.4 * k], [-1, 0, 0, 0.0], [0, -1, 0, 1.3]], np.float32)
+        a = time.perf_counter()
+        gz.update_depth_image(dbuf.ptr + k * 640 * 480 * 2, K, T, width=640, height=480,
+                              step=1280, encoding=0, device=True)
+        if k >= 3:
+            per.append((time.perf_counter() - a) * 1e3)
+    out["ingest_depth_640x480"] = dict(ms_per_frame=statistics.mean(per),
+                                      p99_ms=pct(per, 99), frames=len(per))
+    gz.close()
+    del dbuf
+
+    # Viz cloud on the measured grid (the driver's 25 m, every point).
+    base = np.asarray(scans[total - 1].base, np.float32)
+    times = []
+    for _ in range(5):
+        a = time.perf_counter()
+        recs, fetched = g.occupancy_viz_cloud(base, 25.0, 1, 1_000_000)
+        times.append((time.perf_counter() - a) * 1e3)
+    out["viz_cloud"] = dict(radius_m=25.0, ms=statistics.median(times),
+                            fetched=fetched, points=int(len(recs)))
+
+    # Snapshot save / restore of the measured grid.
+    times_s, times_l = [], []
+    blob = None
+    g2 = new_grid(cb, w)
+    for _ in range(3):
+        a = time.perf_counter()
+        blob = g.save_snapshot()
+        times_s.append((time.perf_counter() - a) * 1e3)
+        a = time.perf_counter()
+        g2.load_snapshot(blob)
+        times_l.append((time.perf_counter() - a) * 1e3)
+    same = bool(np.array_equal(g2.occupancy(), g.occupancy()))
+    g2.close()
+    enc = int(np.frombuffer(blob[12:16], "<u4")[0])
+    out["snapshot"] = dict(bytes=len(blob), dense_bytes=4 * g.voxel_num,
+                           encoding="sparse" if enc == 1 else "dense",
+                           save_ms=statistics.median(times_s),
+                           load_ms=statistics.median(times_l), round_trip_equal=same)
+    return out
+
+
 def cpu_baseline(cb, w, scans, budget_s, threads):
     """The oracle (CPU restatement, OpenMP) on the same scans."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -572,6 +701,10 @@ def main():
         pcie = run_pcie(cb, w, scans, args.warmup, args.steps)
         extras["pcie_inclusive_ms_per_scan"] = statistics.mean(pcie)
         extras["fetch"] = run_fetch(cb, w, g, scene, total - 1)
+        extras["driver_steps"] = run_driver_steps(
+            cb, w, scans, bufs, n, args.warmup, args.steps, g,
+            kt["update_ms"] / max(kt["update_launches"], 1))
+        log(f"driver steps: {extras['driver_steps']}")
         extras["zero_fill_256MiB"] = fill_bandwidth(cb)
         log(f"zero fill: {extras['zero_fill_256MiB']}")
         if args.batched_ticks > 0:

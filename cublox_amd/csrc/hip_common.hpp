@@ -120,6 +120,28 @@ __host__ __device__ __forceinline__ bool inside_window(int gx, int gy, int gz,
 }
 
 // ---------------------------------------------------------------------
+// Division by a divisor fixed for a launch (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994,
+// Fig. 4.1): one high multiply, exact for every 32-bit dividend.
+// ---------------------------------------------------------------------
+struct FastDiv {
+  unsigned m, s1, s2;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(unsigned d) {
+  unsigned l = 0;  // ceil(log2(d))
+  while (l < 32 && (1ull << l) < d) ++l;
+  const unsigned long long m =
+      ((1ull << 32) * ((1ull << l) - d)) / d + 1ull;
+  return FastDiv{static_cast<unsigned>(m), l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
+}
+
+__device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv &f) {
+  const unsigned t = __umulhi(n, f.m);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// ---------------------------------------------------------------------
 // Wave and block scans (wave64).
 // ---------------------------------------------------------------------
 __device__ __forceinline__ int wave_scan_incl(int v) {

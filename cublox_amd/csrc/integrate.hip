@@ -1291,18 +1291,33 @@ __device__ __forceinline__ void delta_put(const GridDesc &d, unsigned slot,
 }
 
 // Appends the voxels of this thread marked in `changed` (bit j: hash(j),
-// new value v[j], j < kN). Every lane of the wave takes part; d is the
-// same in all of them.
+// new value v[j], j < kN). Every thread of the workgroup (at most four
+// waves) takes part, d is the same in all of them, and s_tmp holds 8 ints
+// of LDS. The workgroup reserves its slots with one atomic: the grid's
+// count is one address that every workgroup of the launch hits.
 template <int kN, typename Hash>
 __device__ __forceinline__ void delta_append(const GridDesc &d,
                                              unsigned changed, const float *v,
-                                             Hash hash) {
+                                             Hash hash, int *s_tmp) {
   int total = __popc(changed);
   for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off, 64);
+  const int wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  if (__lane_id() == 0) s_tmp[wv] = total;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int k = 0; k < nw; ++k) {
+    const int c = s_tmp[k];
+    before += k < wv ? c : 0;
+    all += c;
+  }
+  if (threadIdx.x == 0 && all > 0) {
+    s_tmp[4] = static_cast<int>(atomic_add_u32_ret(d.delta_count, static_cast<unsigned>(all)));
+  }
+  __syncthreads();
+  unsigned base = static_cast<unsigned>(s_tmp[4]) + static_cast<unsigned>(before);
+  __syncthreads();  // s_tmp is reused by the next call
   if (total == 0) return;  // wave-uniform
-  unsigned base = 0;
-  if (__lane_id() == 0) base = atomic_add_u32_ret(d.delta_count, static_cast<unsigned>(total));
-  base = static_cast<unsigned>(__shfl(static_cast<int>(base), 0, 64));
 #pragma unroll
   for (int j = 0; j < kN; ++j) {
     const bool on = (changed >> j) & 1u;
@@ -1319,21 +1334,23 @@ __device__ __forceinline__ void delta_append(const GridDesc &d,
 // grid's changed-voxel stream when it has one.
 template <bool kDelta, typename Op, typename Hit>
 __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
-                                           int z0, Op op, Hit hit) {
+                                           int z0, Op op, Hit hit, int *s_tmp) {
   const RayArgs &a = d.g.a;
   const int t = threadIdx.x;
-  auto hash = [&](int j) {
-    return (x0 + (j >> 1)) * (a.my * a.mz) +
-           (y0 + ((j & 1) << 3) + (t >> 5)) * a.mz + (z0 + (t & 31));
-  };
+  // hash(j) = hbase (this thread) + joff(j) (the same in every lane), so
+  // each access is a uniform base plus one per-thread offset.
+  const int hbase = (x0 * a.my + y0 + (t >> 5)) * a.mz + z0 + (t & 31);
+  const int plane = a.my * a.mz;
+  auto joff = [&](int j) { return (j >> 1) * plane + ((j & 1) << 3) * a.mz; };
+  auto hash = [&](int j) { return hbase + joff(j); };
   if (d.export_op) {
     CUBLOX_GLOBAL int *eop = as_global(d.export_op);
     CUBLOX_GLOBAL int *ehit = as_global(d.export_hit);
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
       if (op(j) == 0) continue;
-      eop[hash(j)] = static_cast<int>(op(j));
-      ehit[hash(j)] = static_cast<int>(hit(j));
+      (eop + joff(j))[hbase] = static_cast<int>(op(j));
+      (ehit + joff(j))[hbase] = static_cast<int>(hit(j));
     }
     return;
   }
@@ -1341,7 +1358,7 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
   float v[kPerThread];
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
-    v[j] = op(j) != 0 ? occ[hash(j)] : 0.0f;
+    v[j] = op(j) != 0 ? (occ + joff(j))[hbase] : 0.0f;
   }
   if (kDelta && d.delta_count) {
     unsigned changed = 0;
@@ -1350,18 +1367,18 @@ __device__ __forceinline__ void fold_owned(const GridDesc &d, int x0, int y0,
       if (op(j) == 0) continue;
       const float nv = apply_logodds(v[j], static_cast<int>(op(j)),
                                      static_cast<int>(hit(j)), d.g.p);
-      occ[hash(j)] = nv;
+      (occ + joff(j))[hbase] = nv;
       if (nv != v[j]) changed |= 1u << j;
       v[j] = nv;
     }
-    delta_append<kPerThread>(d, changed, v, hash);
+    delta_append<kPerThread>(d, changed, v, hash, s_tmp);
     return;
   }
 #pragma unroll
   for (int j = 0; j < kPerThread; ++j) {
     if (op(j) == 0) continue;
-    occ[hash(j)] = apply_logodds(v[j], static_cast<int>(op(j)),
-                                 static_cast<int>(hit(j)), d.g.p);
+    (occ + joff(j))[hbase] = apply_logodds(v[j], static_cast<int>(op(j)),
+                                           static_cast<int>(hit(j)), d.g.p);
   }
 }
 
@@ -1379,11 +1396,15 @@ template <bool kBig, bool kDelta>
 __global__ void __launch_bounds__(kApplyThreads)
     tile_apply_kernel(const Batch b, Workspace w) {
   __shared__ uint4 s_cnt4[kTileVox / 4];
+  __shared__ int s_tmp[8];
   unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
   constexpr bool big = kBig;
-  const int n_all = kBig ? w.misc[4] : w.misc[0];
+  // Items, their count and the hot list are fixed while this kernel runs:
+  // read them through the constant address space (scalar loads), so the
+  // item's tile, its grid and every address derived from them are uniform.
+  const int n_all = const_int(w.misc, kBig ? 4 : 0);
   for (int it = blockIdx.x; it < n_all; it += gridDim.x) {
-    const Item itm = w.items[kBig ? w.big[it] : it];
+    const Item itm = load_const(w.items + (kBig ? const_int(w.big, it) : it));
     if (itm.tile < 0 || (!kBig && itm.slab == -2)) continue;  // uniform
     const int end = static_cast<int>(
         min(static_cast<long long>(itm.end), w.seg_capacity));
@@ -1411,7 +1432,7 @@ __global__ void __launch_bounds__(kApplyThreads)
       __syncthreads();
       fold_owned<kDelta>(d, tx * TX, ty * TY, tz * TZ,
                  [&](int j) { return c[j] & 0xFFFFu; },
-                 [&](int j) { return c[j] >> 16; });
+                 [&](int j) { return c[j] >> 16; }, s_tmp);
     } else {
       unsigned op[kPerThread], hit[kPerThread];
 #pragma unroll
@@ -1430,7 +1451,7 @@ __global__ void __launch_bounds__(kApplyThreads)
         __syncthreads();
       }
       fold_owned<kDelta>(d, tx * TX, ty * TY, tz * TZ, [&](int j) { return op[j]; },
-                 [&](int j) { return hit[j]; });
+                 [&](int j) { return hit[j]; }, s_tmp);
     }
   }
 }
@@ -1440,9 +1461,10 @@ __global__ void __launch_bounds__(kApplyThreads)
 template <bool kDelta>
 __global__ void __launch_bounds__(256)
     hot_fold_kernel(const Batch b, Workspace w) {
+  __shared__ int s_tmp[8];
   const int n_hot = w.misc[3];
   for (int blk = blockIdx.x; blk < n_hot * kHotSlices; blk += gridDim.x) {
-    const HotTile ht = w.hot[blk / kHotSlices];
+    const HotTile ht = load_const(w.hot + blk / kHotSlices);
     const int k4 = (blk % kHotSlices) * 256 + threadIdx.x;  // < 2048
     const uint4 *src =
         reinterpret_cast<const uint4 *>(w.scratch +
@@ -1502,7 +1524,7 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (kDelta && d.delta_count) {
-      delta_append<4>(d, changed, v, [&](int e) { return h0 + e; });
+      delta_append<4>(d, changed, v, [&](int e) { return h0 + e; }, s_tmp);
     }
   }
 }

@@ -353,67 +353,108 @@ __global__ void __launch_bounds__(256)
 // are contiguous; kept samples are compacted with one atomic per wavefront.
 // ===================================================================
 
-// Enumerates the disk x z-range of a fetch, classifies each in-sphere,
-// in-window voxel and numbers the kept ones with one atomic on *count per
-// wave; sink(keep, slot, px, py, pz, state) then runs in every lane of the
-// wave (slot is meaningful where keep is set).
+// Geometry of a fetch's enumeration: the disk x z-range, row-major with z
+// fastest (lanes along z read contiguous occ). The host plan keeps
+// diam * diam * nz below 2^31 (kernels.cu:575-580).
+struct FetchSpace {
+  int r, diam, nz;
+  unsigned total;
+  FastDiv by_nz, by_diam;
+  bool ctr;
+  int qx, qy, qz;  // window origin modulo the map size (non-negative)
+};
+
+// Hash coordinate of an in-window global index g on an axis with window
+// origin o, half size h, map m = 2h + 1 and q = o mod m: (g + h) mod m,
+// which is global_to_local(g) + h (utils.cuh:60-70), with one compare.
+__device__ __forceinline__ int in_window_coord(int g, int o, int h, int m, int q) {
+  const int v = q + (g - o + h);
+  return v >= m ? v - m : v;
+}
+
+__device__ __forceinline__ bool fetch_space(const cublox_fetch_cfg &c, FetchSpace &f) {
+  f.r = c.disk_radius_vox;
+  f.diam = 2 * f.r + 1;
+  f.nz = c.gz_max - c.gz_min + 1;
+  if (f.diam <= 0 || f.nz <= 0) return false;
+  f.total = static_cast<unsigned>(f.diam) * f.diam * f.nz;
+  f.by_nz = make_fastdiv(static_cast<unsigned>(f.nz));
+  f.by_diam = make_fastdiv(static_cast<unsigned>(f.diam));
+  f.ctr = c.origin_at_center != 0;
+  auto pmod = [](int v, int m) {
+    const int r = v % m;
+    return r < 0 ? r + m : r;
+  };
+  f.qx = pmod(c.origin_i.x, c.map_size_i.x);
+  f.qy = pmod(c.origin_i.y, c.map_size_i.y);
+  f.qz = pmod(c.origin_i.z, c.map_size_i.z);
+  return true;
+}
+
+// kernels.cu:520-561 for element e < total: in the window, in the sphere
+// and passing the state filter; writes the voxel position and state.
+__device__ __forceinline__ bool fetch_classify(const cublox_fetch_cfg &c,
+                                               const float *__restrict__ occ,
+                                               const FetchSpace &f, unsigned e,
+                                               float &px, float &py, float &pz,
+                                               unsigned char &st) {
+  const unsigned rem = fast_div(e, f.by_nz);
+  const int tz = static_cast<int>(e - rem * static_cast<unsigned>(f.nz));
+  const unsigned tx_u = fast_div(rem, f.by_diam);
+  const int ty = static_cast<int>(rem - tx_u * static_cast<unsigned>(f.diam));
+  const int tx = static_cast<int>(tx_u);
+  const int gx = c.center_g.x + tx - f.r;
+  const int gy = c.center_g.y + ty - f.r;
+  const int gz = c.gz_min + tz;
+  if (!inside_window(gx, gy, gz, c.origin_i.x, c.origin_i.y, c.origin_i.z,
+                     c.half_map_size_i.x, c.half_map_size_i.y, c.half_map_size_i.z)) {
+    return false;
+  }
+  px = global_to_pos(gx, c.resolution, f.ctr);
+  py = global_to_pos(gy, c.resolution, f.ctr);
+  pz = global_to_pos(gz, c.resolution, f.ctr);
+  const float dx = px - c.center.x;
+  const float dy = py - c.center.y;
+  const float dz = pz - c.center.z;
+  if ((dx * dx + dy * dy) + dz * dz > c.radius_sq) return false;
+  const int h =
+      (in_window_coord(gx, c.origin_i.x, c.half_map_size_i.x, c.map_size_i.x, f.qx) *
+           c.map_size_i.y +
+       in_window_coord(gy, c.origin_i.y, c.half_map_size_i.y, c.map_size_i.y, f.qy)) *
+          c.map_size_i.z +
+      in_window_coord(gz, c.origin_i.z, c.half_map_size_i.z, c.map_size_i.z, f.qz);
+  const float l = occ[h] + c.l_unknown;
+  st = 0;
+  if (l >= c.l_occupied) {
+    st = 2;
+  } else if (l < c.l_free) {
+    st = 1;
+  }
+  return c.state_filter < 0 || static_cast<int>(st) == c.state_filter;
+}
+
+// Enumerates a fetch and numbers the kept samples with one atomic on
+// *count per wave; sink(keep, slot, px, py, pz, state) then runs in every
+// lane of the wave (slot is meaningful where keep is set). Used where many
+// queries share a launch (each query's counter sees few waves).
 template <typename Sink>
 __device__ __forceinline__ void fetch_scan(const cublox_fetch_cfg &c,
                                            const float *__restrict__ occ,
                                            unsigned *__restrict__ count,
                                            const int block, const int nblocks,
                                            Sink sink) {
-  const int r = c.disk_radius_vox;
-  const int diam = 2 * r + 1;
-  const int nz = c.gz_max - c.gz_min + 1;
-  if (diam <= 0 || nz <= 0) return;
-  const long long total = static_cast<long long>(diam) * diam * nz;
-  const long long stride = static_cast<long long>(nblocks) * blockDim.x;
-  const bool ctr = c.origin_at_center != 0;
+  FetchSpace f;
+  if (!fetch_space(c, f)) return;
+  const unsigned stride = static_cast<unsigned>(nblocks) * blockDim.x;
   const int lane = __lane_id();
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   // Wave-uniform trip count so that every lane reaches each ballot.
-  const long long first =
-      static_cast<long long>(block) * blockDim.x + (threadIdx.x & ~63);
-  for (long long w0 = first; w0 < total; w0 += stride) {
-    const long long e = w0 + lane;
-    bool keep = false;
+  const unsigned first = static_cast<unsigned>(block) * blockDim.x + (threadIdx.x & ~63u);
+  for (unsigned w0 = first; w0 < f.total; w0 += stride) {
+    const unsigned e = w0 + lane;
     float px = 0.f, py = 0.f, pz = 0.f;
     unsigned char st = 0;
-    if (e < total) {
-      const int tz = static_cast<int>(e % nz);
-      const long long rem = e / nz;
-      const int ty = static_cast<int>(rem % diam);
-      const int tx = static_cast<int>(rem / diam);
-      const int gx = c.center_g.x + tx - r;
-      const int gy = c.center_g.y + ty - r;
-      const int gz = c.gz_min + tz;
-      if (inside_window(gx, gy, gz, c.origin_i.x, c.origin_i.y, c.origin_i.z,
-                        c.half_map_size_i.x, c.half_map_size_i.y,
-                        c.half_map_size_i.z)) {
-        px = global_to_pos(gx, c.resolution, ctr);
-        py = global_to_pos(gy, c.resolution, ctr);
-        pz = global_to_pos(gz, c.resolution, ctr);
-        const float dx = px - c.center.x;
-        const float dy = py - c.center.y;
-        const float dz = pz - c.center.z;
-        if (!((dx * dx + dy * dy) + dz * dz > c.radius_sq)) {
-          const int h = global_to_hash(gx, gy, gz, c.map_size_i.x,
-                                       c.map_size_i.y, c.map_size_i.z,
-                                       c.half_map_size_i.x,
-                                       c.half_map_size_i.y,
-                                       c.half_map_size_i.z);
-          const float l = occ[h] + c.l_unknown;
-          st = 0;
-          if (l >= c.l_occupied) {
-            st = 2;
-          } else if (l < c.l_free) {
-            st = 1;
-          }
-          keep = c.state_filter < 0 || static_cast<int>(st) == c.state_filter;
-        }
-      }
-    }
+    const bool keep = e < f.total && fetch_classify(c, occ, f, e, px, py, pz, st);
     const unsigned long long m = __ballot(keep);
     if (m == 0ull) continue;
     const int leader = __ffsll(static_cast<long long>(m)) - 1;
@@ -422,6 +463,57 @@ __device__ __forceinline__ void fetch_scan(const cublox_fetch_cfg &c,
     base = __shfl(base, leader);
     sink(keep, base + __popcll(m & lt_mask), px, py, pz, st);
   }
+}
+
+// Workgroup-staged form for one large query: kept samples (x, y, z, state)
+// gather in LDS and each flush reserves its slots on *count with one
+// atomic (a flush every kFetchStage - 256 samples at most, usually one per
+// workgroup), so the shared counter is not hit once per wave. The loop is
+// workgroup-uniform; flush(n, base) runs in every thread with the staged
+// samples s_stage[0, n) holding slots base + k.
+constexpr int kFetchThreads = 256;
+constexpr int kFetchStage = 1024;
+
+struct FetchLds {
+  float4 stage[kFetchStage];
+  int wave[kFetchThreads / 64];
+  int misc[4];
+};
+
+template <typename Flush>
+__device__ __forceinline__ void staged_flush(unsigned *__restrict__ count, int n,
+                                             FetchLds &L, Flush flush) {
+  if (threadIdx.x == 0) L.misc[1] = static_cast<int>(atomic_add_u32_ret(count, static_cast<unsigned>(n)));
+  __syncthreads();
+  flush(n, static_cast<unsigned>(L.misc[1]));
+  __syncthreads();
+}
+
+template <typename Flush>
+__device__ __forceinline__ void fetch_scan_staged(const cublox_fetch_cfg &c,
+                                                  const float *__restrict__ occ,
+                                                  unsigned *__restrict__ count,
+                                                  FetchLds &L, Flush flush) {
+  FetchSpace f;
+  if (!fetch_space(c, f)) return;  // uniform: c is a launch argument
+  const unsigned stride = gridDim.x * blockDim.x;
+  int n = 0;  // staged samples (the same in every thread)
+  for (unsigned b0 = blockIdx.x * blockDim.x; b0 < f.total; b0 += stride) {
+    const unsigned e = b0 + threadIdx.x;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    unsigned char st = 0;
+    const bool keep = e < f.total && fetch_classify(c, occ, f, e, px, py, pz, st);
+    const int incl = block_scan_incl(L.wave, keep ? 1 : 0);
+    const int tot = L.wave[kFetchThreads / 64 - 1];
+    if (keep) L.stage[n + incl - 1] = make_float4(px, py, pz, __int_as_float(st));
+    n += tot;
+    __syncthreads();
+    if (n > kFetchStage - kFetchThreads) {
+      staged_flush(count, n, L, flush);
+      n = 0;
+    }
+  }
+  if (n > 0) staged_flush(count, n, L, flush);
 }
 
 __device__ __forceinline__ void fetch_body(const cublox_fetch_cfg &c,
@@ -487,48 +579,74 @@ __device__ __forceinline__ float viz_rgb(float d, float d_max) {
   return __uint_as_float((u8(r) << 16) | (u8(g) << 8) | u8(b));
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kFetchThreads)
     viz_kernel(const cublox_fetch_cfg c, const float *__restrict__ occ,
                const VizArgs v, unsigned *__restrict__ fetch_count,
                unsigned *__restrict__ viz_count, float4 *__restrict__ out,
                const unsigned out_cap) {
-  const int lane = __lane_id();
-  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  __shared__ FetchLds L;
   const bool ctr = v.center != 0;
-  fetch_scan(c, occ, fetch_count, blockIdx.x, gridDim.x,
-             [&](bool keep, unsigned slot, float px, float py, float pz,
-                 unsigned char) {
-               bool take = keep && !((v.max_fetch > 0u && slot >= v.max_fetch) ||
-                                     slot >= v.fetch_cap);
-               if (take && v.step > 1) {
-                 const unsigned mix =
-                     static_cast<unsigned>(pos_to_global(px, v.inv_res, ctr)) ^
-                     static_cast<unsigned>(pos_to_global(py, v.inv_res, ctr)) ^
-                     static_cast<unsigned>(pos_to_global(pz, v.inv_res, ctr));
-                 take = mix % static_cast<unsigned>(v.step) == 0u;
-               }
-               const unsigned long long m = __ballot(take);
-               if (m == 0ull) return;
-               const int leader = __ffsll(static_cast<long long>(m)) - 1;
-               unsigned base = 0;
-               if (lane == leader) base = atomic_add_u32_ret(viz_count, __popcll(m));
-               base = __shfl(base, leader);
-               if (!take) return;
-               const unsigned k = base + __popcll(m & lt_mask);
-               if (k >= out_cap) return;
-               const float dx = px - v.px, dy = py - v.py, dz = pz - v.pz;
-               const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
-               out[k] = make_float4(px, py, pz, viz_rgb(d, v.d_max));
-             });
+  constexpr int kPer = kFetchStage / kFetchThreads;
+  fetch_scan_staged(c, occ, fetch_count, L, [&](int n, unsigned base) {
+    // Fetched samples (slot below max_results and the capacity) that pass
+    // the subsample; this thread's are staged k = t + j * 256.
+    unsigned takes = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int k = threadIdx.x + j * kFetchThreads;
+      bool take = false;
+      if (k < n) {
+        const unsigned slot = base + static_cast<unsigned>(k);
+        take = !((v.max_fetch > 0u && slot >= v.max_fetch) || slot >= v.fetch_cap);
+        if (take && v.step > 1) {
+          const float4 q = L.stage[k];
+          const unsigned mix = static_cast<unsigned>(pos_to_global(q.x, v.inv_res, ctr)) ^
+                               static_cast<unsigned>(pos_to_global(q.y, v.inv_res, ctr)) ^
+                               static_cast<unsigned>(pos_to_global(q.z, v.inv_res, ctr));
+          take = mix % static_cast<unsigned>(v.step) == 0u;
+        }
+      }
+      takes |= (take ? 1u : 0u) << j;
+    }
+    const int mine = __popc(takes);
+    const int incl = block_scan_incl(L.wave, mine);
+    const int total = L.wave[kFetchThreads / 64 - 1];
+    if (threadIdx.x == 0 && total > 0) {
+      L.misc[2] = static_cast<int>(atomic_add_u32_ret(viz_count, static_cast<unsigned>(total)));
+    }
+    __syncthreads();
+    unsigned idx = static_cast<unsigned>(L.misc[2]) + static_cast<unsigned>(incl - mine);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (!((takes >> j) & 1u)) continue;
+      const float4 q = L.stage[threadIdx.x + j * kFetchThreads];
+      if (idx < out_cap) {
+        const float dx = q.x - v.px, dy = q.y - v.py, dz = q.z - v.pz;
+        const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+        out[idx] = make_float4(q.x, q.y, q.z, viz_rgb(d, v.d_max));
+      }
+      ++idx;
+    }
+  });
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kFetchThreads)
     fetch_kernel(const cublox_fetch_cfg c, const float *__restrict__ occ,
                  unsigned *__restrict__ count, float *__restrict__ out_pos,
                  unsigned char *__restrict__ out_state, const unsigned cap,
                  const unsigned max_results) {
-  fetch_body(c, occ, count, out_pos, out_state, cap, max_results, blockIdx.x,
-             gridDim.x);
+  __shared__ FetchLds L;
+  fetch_scan_staged(c, occ, count, L, [&](int n, unsigned base) {
+    for (int k = threadIdx.x; k < n; k += kFetchThreads) {
+      const unsigned slot = base + static_cast<unsigned>(k);
+      if ((max_results > 0u && slot >= max_results) || slot >= cap) continue;
+      const float4 q = L.stage[k];
+      out_pos[3ull * slot + 0] = q.x;
+      out_pos[3ull * slot + 1] = q.y;
+      out_pos[3ull * slot + 2] = q.z;
+      out_state[slot] = static_cast<unsigned char>(__float_as_int(q.w));
+    }
+  });
 }
 
 __global__ void __launch_bounds__(256)
@@ -1043,6 +1161,9 @@ int cublox_launch_fetch_occupancy_around(const cublox_fetch_cfg *cfg,
   }
   const long long diam = 2LL * cfg->disk_radius_vox + 1;
   const long long total = diam * diam * (cfg->gz_max - cfg->gz_min + 1);
+  if (total > INT_MAX) {
+    return set_error(CUBLOX_ERR_INVALID, "fetch: more than 2^31 - 1 voxels to test");
+  }
   long long blocks = (total + 255) / 256;
   blocks = std::max<long long>(1, std::min<long long>(blocks, 4096));
   fetch_kernel<<<dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
@@ -1078,6 +1199,9 @@ int cublox_launch_viz_cloud(const cublox_fetch_cfg *cfg, const float *d_occ,
   v.fetch_cap = fetch_capacity;
   const long long diam = 2LL * cfg->disk_radius_vox + 1;
   const long long total = diam * diam * (cfg->gz_max - cfg->gz_min + 1);
+  if (total > INT_MAX) {
+    return set_error(CUBLOX_ERR_INVALID, "viz_cloud: more than 2^31 - 1 voxels to test");
+  }
   long long blocks = (total + 255) / 256;
   blocks = std::max<long long>(1, std::min<long long>(blocks, 4096));
   viz_kernel<<<dim3(static_cast<unsigned>(blocks)), dim3(256), 0, as_stream(s)>>>(
@@ -1103,6 +1227,9 @@ int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
   }
   const long long total =
       static_cast<long long>(max_disk_diam) * max_disk_diam * max_nz;
+  if (total > INT_MAX) {
+    return set_error(CUBLOX_ERR_INVALID, "fetch batch: more than 2^31 - 1 voxels per query");
+  }
   // Several waves per query; the kernel strides over the disk volume.
   long long blocks = (total + 255) / 256;
   blocks = std::max<long long>(1, std::min<long long>(blocks, 64));
