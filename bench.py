@@ -605,7 +605,7 @@ def main():
                     help="seconds per load level of the 256-session map server (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the zero-fill and copy calibration kernels (with --no-extras)")
-    ap.add_argument("--only", choices=("batched", "served"),
+    ap.add_argument("--only", choices=("batched", "served", "driver"),
                     help="run only that workload and print its record (for profiling)")
     args = ap.parse_args()
 
@@ -626,6 +626,18 @@ def main():
     if cb.device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    if args.only == "driver":
+        w = cb.WORKLOADS[args.workload]
+        scene = w.scene()
+        scans = make_scans(w, args.warmup + args.steps, scene)
+        bufs, n = upload(cb, scans)
+        g, _per, _wall, kt = run_gpu(cb, w, scans, bufs, n, args.warmup, args.steps)
+        rec = run_driver_steps(cb, w, scans, bufs, n, args.warmup, args.steps, g,
+                               kt["update_ms"] / max(kt["update_launches"], 1))
+        sys.stdout.flush()
+        os.write(json_fd, (json.dumps({"workload": "driver", **rec}) + "\n").encode())
+        os.close(json_fd)
+        return
     if args.only:
         rec = (run_batched(cb, 1, args.batched_ticks) if args.only == "batched"
                else run_served(cb, seconds=args.served_seconds))

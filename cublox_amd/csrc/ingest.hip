@@ -50,22 +50,29 @@ __global__ void __launch_bounds__(256)
 }
 
 // main.cpp:672-679: a point is kept when finite and, for drop_sq > 0, its
-// squared norm ((x*x + y*y) + z*z) is at most drop_sq. One atomic per wave.
+// squared norm ((x*x + y*y) + z*z) is at most drop_sq. One atomic per
+// workgroup (a counter every workgroup hits serialises at the L2, so the
+// launch is kept to a few workgroups).
+constexpr int kCountBlocks = 64;
+
 __global__ void __launch_bounds__(256)
     count_kept_kernel(const float *__restrict__ xs, const float *__restrict__ ys,
                       const float *__restrict__ zs, int stride, int n,
                       float drop_sq, unsigned *__restrict__ count) {
-  unsigned kept = 0;
+  __shared__ int s_wave[4];
+  int kept = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
     const size_t at = static_cast<size_t>(i) * static_cast<size_t>(stride);
     const float x = xs[at], y = ys[at], z = zs[at];
     bool keep = isfinite(x) && isfinite(y) && isfinite(z);
     if (keep && drop_sq > 0.0f) keep = !((x * x + y * y) + z * z > drop_sq);
-    kept += keep ? 1u : 0u;
+    kept += keep ? 1 : 0;
   }
-  for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off, 64);
-  if (__lane_id() == 0 && kept != 0) atomic_add_u32_ret(count, kept);
+  const int total = block_scan_incl(s_wave, kept);
+  if (threadIdx.x == blockDim.x - 1 && total != 0) {
+    atomic_add_u32_ret(count, static_cast<unsigned>(total));
+  }
 }
 
 }  // namespace
@@ -118,7 +125,7 @@ int cublox_launch_count_kept_points(const float *d_xs, const float *d_ys,
     return set_error(CUBLOX_ERR_INVALID, "count_kept_points: bad arguments");
   }
   const int stride = point_stride > 0 ? point_stride : 1;
-  const unsigned blocks = static_cast<unsigned>(std::min(1024, (n + 255) / 256));
+  const unsigned blocks = static_cast<unsigned>(std::min(kCountBlocks, (n + 255) / 256));
   count_kept_kernel<<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(
       d_xs, d_ys, d_zs, stride, n, range_drop_sq, d_count);
   return check_launch("count_kept_kernel");
