@@ -418,7 +418,9 @@ def run_fetch(cb, w, g, scene, k, reps=5):
         return None
     qs = scene.queries(k, w.fetch_queries, w.fetch_spread)
     r_vox = int(np.ceil(w.fetch_radius / w.resolution))
-    cap = (2 * r_vox + 1) ** 3
+    # Per-query output segments: the disk volume, at most 16,384 samples
+    # (an OCCUPIED query returns surface voxels only; counts stay exact).
+    cap = min((2 * r_vox + 1) ** 3, 16384)
     times = []
     counts = None
     for _ in range(reps):
@@ -427,7 +429,8 @@ def run_fetch(cb, w, g, scene, k, reps=5):
         times.append((time.perf_counter() - a) * 1e3)
     return dict(queries=int(len(qs)), radius_m=w.fetch_radius, filter="occupied",
                 ms_per_batch=statistics.median(times),
-                samples=int(counts.sum()))
+                samples=int(counts.sum()), per_query_cap=int(cap),
+                queries_at_cap=int((counts >= cap).sum()))
 
 
 def _records(scans, n, step=16):

@@ -663,6 +663,8 @@ __global__ void __launch_bounds__(256)
              per_query_cap, 0u, blockIdx.x, gridDim.x);
 }
 
+constexpr int kMaxGridY = 65535;  // queries per batched launch
+
 // Dense gather of batched fetch results: query q's first min(count, cap)
 // samples move from its capacity segment to offsets[q].
 __global__ void __launch_bounds__(256)
@@ -1222,22 +1224,25 @@ int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
       !d_state || per_query_cap == 0 || max_disk_diam <= 0 || max_nz <= 0) {
     return CUBLOX_OK;
   }
-  if (n_queries > 65535) {
-    return set_error(CUBLOX_ERR_INVALID, "fetch batch: more than 65535 queries");
-  }
   const long long total =
       static_cast<long long>(max_disk_diam) * max_disk_diam * max_nz;
   if (total > INT_MAX) {
     return set_error(CUBLOX_ERR_INVALID, "fetch batch: more than 2^31 - 1 voxels per query");
   }
   // Several waves per query; the kernel strides over the disk volume.
+  // Queries go in launches of at most 65,535 (the grid's y extent).
   long long blocks = (total + 255) / 256;
   blocks = std::max<long long>(1, std::min<long long>(blocks, 64));
-  fetch_batch_kernel<<<dim3(static_cast<unsigned>(blocks),
-                            static_cast<unsigned>(n_queries)),
-                       dim3(256), 0, as_stream(s)>>>(
-      d_cfgs, d_occ, d_counts, d_pos, d_state, per_query_cap);
-  return check_launch("fetch_batch_kernel");
+  for (int q0 = 0; q0 < n_queries; q0 += kMaxGridY) {
+    const int nq = std::min(kMaxGridY, n_queries - q0);
+    const size_t off = static_cast<size_t>(q0) * per_query_cap;
+    fetch_batch_kernel<<<dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(nq)),
+                         dim3(256), 0, as_stream(s)>>>(
+        d_cfgs + q0, d_occ, d_counts + q0, d_pos + 3 * off, d_state + off, per_query_cap);
+    const int st = check_launch("fetch_batch_kernel");
+    if (st != CUBLOX_OK) return st;
+  }
+  return CUBLOX_OK;
 }
 
 int cublox_launch_fetch_gather(const unsigned int *d_counts,
@@ -1248,16 +1253,19 @@ int cublox_launch_fetch_gather(const unsigned int *d_counts,
                                float *d_pos_out, unsigned char *d_state_out,
                                cublox_stream s) {
   if (n_queries <= 0 || per_query_cap == 0 || max_count == 0) return CUBLOX_OK;
-  if (n_queries > 65535) {
-    return set_error(CUBLOX_ERR_INVALID, "fetch gather: more than 65535 queries");
-  }
   const unsigned m = std::min(max_count, per_query_cap);
   const unsigned bx = std::max(1u, std::min((m + 255u) / 256u, 64u));
-  fetch_gather_kernel<<<dim3(bx, static_cast<unsigned>(n_queries)), dim3(256), 0,
-                        as_stream(s)>>>(d_counts, d_offsets, per_query_cap,
-                                        d_pos_seg, d_state_seg, d_pos_out,
-                                        d_state_out);
-  return check_launch("fetch_gather_kernel");
+  for (int q0 = 0; q0 < n_queries; q0 += kMaxGridY) {
+    const int nq = std::min(kMaxGridY, n_queries - q0);
+    const size_t off = static_cast<size_t>(q0) * per_query_cap;
+    fetch_gather_kernel<<<dim3(bx, static_cast<unsigned>(nq)), dim3(256), 0,
+                          as_stream(s)>>>(d_counts + q0, d_offsets + q0, per_query_cap,
+                                          d_pos_seg + 3 * off, d_state_seg + off,
+                                          d_pos_out, d_state_out);
+    const int st = check_launch("fetch_gather_kernel");
+    if (st != CUBLOX_OK) return st;
+  }
+  return CUBLOX_OK;
 }
 
 int cublox_counter_census(const int *d_op, const int *d_hit, int n,
