@@ -120,3 +120,32 @@ def test_readme_shape_first_scans(cb, orc, gpu_ready):
         op_, os_, on = o.fetch(c, w.fetch_radius, cb.OCCUPIED)
         assert len(gs) == on
         assert np.array_equal(sample_multiset(gp, gs), sample_multiset(op_, os_))
+
+
+def test_batched_fetch_beyond_one_launch(cb, orc, gpu_ready):
+    """70,000 queries: the batch runs in launches of 65,535 queries; every
+    query's count equals the oracle's, and queries of the second launch
+    give the oracle's samples."""
+    from helpers import sample_multiset
+    w = cb.WORKLOADS["small"]
+    scene = w.scene()
+    g, o = make_pair(cb, orc, w)
+    for k in range(3):
+        s = scene.scan(k, w.beams, w.azimuths, w.el_lo, w.el_hi, w.max_range)
+        g.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+        o.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+    rng = np.random.default_rng(8)
+    base = np.asarray(s.base, np.float32)
+    q = 70_000
+    cs = (base + rng.uniform([-8, -8, -1], [8, 8, 2], (q, 3))).astype(np.float32)
+    cap = 512
+    counts, pos, st = g.fetch_occupancy_batch(cs, 0.35, None, cap)
+    offs = np.zeros(q + 1, np.int64)
+    offs[1:] = np.cumsum(np.minimum(counts, cap).astype(np.int64))
+    want = np.array([o.fetch(c, 0.35, None, cap, cap=cap)[2] for c in cs])
+    assert np.array_equal(counts, np.minimum(want, cap)), "per-query counts"
+    for j in list(range(65_530, 65_545)) + list(rng.integers(65_535, q, 40)):
+        op_, os_, on = o.fetch(cs[j], 0.35, None, cap, cap=cap)
+        a, b = offs[j], offs[j + 1]
+        assert np.array_equal(sample_multiset(pos[a:b], st[a:b]),
+                              sample_multiset(op_, os_)), j
