@@ -271,6 +271,7 @@ def host() -> C.CDLL:
     _sig(L, "cublox_ms_copy_occupancy", i, vp, i, f32p)
     _sig(L, "cublox_ms_origin", i, vp, i, i32p)
     _sig(L, "cublox_ms_stats", i, vp, dp)
+    _sig(L, "cublox_ms_evict", i, vp, i)
     _sig(L, "cublox_ms_load_run", i, vp, i32p, i, vp, vp, vp, i32p, f32p, f32p, vp,
          llp, dp)
     _sig(L, "cublox_utils_pos_to_global", i, F, F, i)

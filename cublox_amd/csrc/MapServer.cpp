@@ -157,6 +157,7 @@ int MapServer::openSession(const SessionSpec &spec) {
   std::lock_guard<std::mutex> lk(mu_);
   if (worker_error_) std::rethrow_exception(worker_error_);
   const int sid = next_sid_++;
+  s.in_waiting = true;
   sessions_.emplace(sid, std::move(s));
   waiting_.push_back(sid);
   ++gen_;
@@ -171,6 +172,22 @@ void MapServer::closeSession(int sid) {
   it->second.closing = true;
   ++gen_;
   cv_work_.notify_all();
+}
+
+void MapServer::evictSession(int sid) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sessions_.find(sid);
+  if (it == sessions_.end()) throw std::invalid_argument("cublox::MapServer: no such session");
+  it->second.evict_requested = true;
+  ++gen_;
+  cv_work_.notify_all();
+}
+
+std::vector<uint8_t> MapServer::evictedSnapshot(int sid) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = sessions_.find(sid);
+  if (it == sessions_.end()) throw std::invalid_argument("cublox::MapServer: no such session");
+  return it->second.snapshot;
 }
 
 bool MapServer::admitted(int sid) const {
@@ -223,6 +240,10 @@ long long MapServer::submitScan(int sid, const float *xs, const float *ys, const
   }
   r.id = next_id_++;
   it->second.queue.push_back(std::move(r));
+  if (!it->second.admitted && !it->second.in_waiting) {  // evicted: readmit
+    it->second.in_waiting = true;
+    waiting_.push_back(sid);
+  }
   ++gen_;
   cv_work_.notify_all();
   return next_id_ - 1;
@@ -246,6 +267,10 @@ long long MapServer::submitFetch(int sid, const Eigen::Vector3f &center, float r
   r.id = next_id_++;
   r.submit_ms = nowMs();
   it->second.queue.push_back(std::move(r));
+  if (!it->second.admitted && !it->second.in_waiting) {  // evicted: readmit
+    it->second.in_waiting = true;
+    waiting_.push_back(sid);
+  }
   ++gen_;
   cv_work_.notify_all();
   return next_id_ - 1;
@@ -257,9 +282,15 @@ bool MapServer::quiescent() const {
     if (s.admitted && !s.queue.empty()) return false;
     if (s.closing && s.admitted) return false;  // not yet retired
   }
+  for (const auto &[sid, s] : sessions_) {
+    if (s.evict_requested && s.admitted && !s.closing) return false;
+  }
+  // The worker would admit someone (the same walk as admitWaiting).
   for (int sid : waiting_) {
     const Session &s = sessions_.at(sid);
     if (admittable(s)) return false;
+    if (lim_.evict_idle && !s.queue.empty() && idleVictim()) return false;
+    if (!(lim_.evict_idle && s.queue.empty())) break;  // FIFO: blocked here
   }
   return true;
 }
@@ -321,6 +352,8 @@ MapServer::Stats MapServer::stats() const {
   s.resident_bytes = resident_;
   s.admitted = admitted_count_;
   s.waiting = static_cast<int>(waiting_.size());
+  s.snapshot_bytes = 0;
+  for (const auto &kv : sessions_) s.snapshot_bytes += kv.second.snapshot.size();
   return s;
 }
 
@@ -342,7 +375,8 @@ void MapServer::retireClosed() {
       if (s.admitted) {
         resident_ -= s.bytes;
         --admitted_count_;
-      } else {
+      }
+      if (s.in_waiting) {
         waiting_.erase(std::remove(waiting_.begin(), waiting_.end(), it->first),
                        waiting_.end());
       }
@@ -354,19 +388,81 @@ void MapServer::retireClosed() {
 }
 
 void MapServer::admitWaiting() {
-  while (!waiting_.empty()) {
-    Session &s = sessions_.at(waiting_.front());
-    if (!admittable(s)) break;  // FIFO: later sessions wait behind it
+  for (auto it = waiting_.begin(); it != waiting_.end();) {
+    Session &s = sessions_.at(*it);
+    if (!admittable(s)) {
+      // FIFO: later sessions wait behind it. With idle eviction, an idle
+      // session makes room for one with queued requests, and a waiting
+      // session with nothing queued does not hold up the ones behind it.
+      if (lim_.evict_idle && !s.queue.empty() && evictIdle()) continue;
+      if (lim_.evict_idle && s.queue.empty()) {
+        ++it;
+        continue;
+      }
+      break;
+    }
     const SessionSpec &p = s.spec;
     s.grid = std::make_unique<OccupancyGrid>(p.half_map_size_i, p.resolution,
                                              p.origin_at_center, p.recenter_threshold);
     s.grid->setMaxRaycastRange(p.max_range);
     s.grid->setLogOddsParams(p.log_odds[0], p.log_odds[1], p.log_odds[2], p.log_odds[3],
                              p.log_odds[4], p.log_odds[5]);
+    if (!s.snapshot.empty()) {  // an evicted session comes back as it left
+      s.grid->loadSnapshot(s.snapshot.data(), s.snapshot.size());
+      std::vector<uint8_t>().swap(s.snapshot);
+      ++stats_.restores;
+    }
     s.admitted = true;
+    s.in_waiting = false;
     resident_ += s.bytes;
     ++admitted_count_;
-    waiting_.pop_front();
+    it = waiting_.erase(it);
+  }
+}
+
+// An idle session: resident, with nothing queued, in flight or closing.
+bool MapServer::idleVictim() const {
+  for (const auto &kv : sessions_) {
+    const Session &s = kv.second;
+    if (s.admitted && !s.closing && s.busy == 0 && s.queue.empty()) return true;
+  }
+  return false;
+}
+
+void MapServer::evict(int sid, Session &s) {
+  s.snapshot = s.grid->saveSnapshot();  // nothing of it is in flight
+  s.grid.reset();
+  s.admitted = false;
+  s.evict_requested = false;
+  resident_ -= s.bytes;
+  --admitted_count_;
+  ++stats_.evictions;
+  if (!s.queue.empty() && !s.in_waiting) {
+    s.in_waiting = true;
+    waiting_.push_back(sid);
+  }
+}
+
+// The least recently used idle session.
+bool MapServer::evictIdle() {
+  int best = -1;
+  long long best_tick = 0;
+  for (auto &[sid, s] : sessions_) {
+    if (!(s.admitted && !s.closing && s.busy == 0 && s.queue.empty())) continue;
+    if (best < 0 || s.last_tick < best_tick) {
+      best = sid;
+      best_tick = s.last_tick;
+    }
+  }
+  if (best < 0) return false;
+  evict(best, sessions_.at(best));
+  return true;
+}
+
+void MapServer::evictRequested() {
+  for (auto &[sid, s] : sessions_) {
+    if (s.evict_requested && s.admitted && !s.closing && s.busy == 0) evict(sid, s);
+    if (s.evict_requested && (s.closing || !s.admitted)) s.evict_requested = false;
   }
 }
 
@@ -402,11 +498,13 @@ bool MapServer::formTick(Tick &t) {
       t.scans.emplace_back(sid, std::move(s.queue.front()));
       s.queue.pop_front();
       ++s.busy;
+      s.last_tick = ticks_;
     }
     while (!s.queue.empty() && s.queue.front().kind == Kind::FETCH) {
       t.fetches.emplace_back(sid, std::move(s.queue.front()));
       s.queue.pop_front();
       ++s.busy;
+      s.last_tick = ticks_;
     }
   }
   if (t.scans.empty() && t.fetches.empty()) return false;
@@ -599,6 +697,7 @@ void MapServer::run() {
       {
         std::unique_lock<std::mutex> lk(mu_);
         retireClosed();
+        evictRequested();
         admitWaiting();
         if (!have_prev) {
           worker_busy_ = false;
@@ -606,6 +705,7 @@ void MapServer::run() {
           cv_work_.wait(lk, [&] { return stop_ || (gen_ != seen && !paused_); });
           seen = gen_;
           retireClosed();
+          evictRequested();
           admitWaiting();
           worker_busy_ = true;
         }

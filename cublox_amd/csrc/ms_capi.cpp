@@ -5,8 +5,10 @@
 #include <cublox/capi.h>
 #include <cublox/ms_capi.h>
 #include <cublox/og_capi.h>
+#include <cublox/snapshot.h>
 
 #include <algorithm>
+#include <cstring>
 #include <exception>
 #include <string>
 #include <vector>
@@ -60,7 +62,7 @@ cublox::MapServer &server_of(cublox_ms *h) { return *H->server; }
 
 extern "C" {
 
-cublox_ms *cublox_ms_create(const long long limits[9]) {
+cublox_ms *cublox_ms_create(const long long limits[10]) {
   cublox_ms *out = nullptr;
   guarded([&] {
     cublox::ServerLimits l;
@@ -73,6 +75,7 @@ cublox_ms *cublox_ms_create(const long long limits[9]) {
     l.max_segments_per_ray = static_cast<int>(limits[6]);
     l.graph = limits[7] != 0;
     if (limits[8] > 0) l.point_arena_bytes = static_cast<std::size_t>(limits[8]);
+    l.evict_idle = limits[9] != 0;
     auto *hd = new Handle;
     try {
       hd->server = std::make_unique<MS>(l);
@@ -104,6 +107,10 @@ int cublox_ms_open(cublox_ms *h, const int half[3], float resolution, int origin
 
 int cublox_ms_close(cublox_ms *h, int sid) {
   return guarded([&] { H->server->closeSession(sid); });
+}
+
+int cublox_ms_evict(cublox_ms *h, int sid) {
+  return guarded([&] { H->server->evictSession(sid); });
 }
 
 int cublox_ms_admitted(cublox_ms *h, int sid, int *out) {
@@ -185,21 +192,63 @@ int cublox_ms_fetch_result(cublox_ms *h, long long id, float *pos, unsigned char
   });
 }
 
+// An evicted session's map, read from its snapshot (cublox/snapshot.h).
+static cublox_snapshot_header evictedMap(const cublox::MapServer &srv, int sid,
+                                         float *dst) {
+  const std::vector<uint8_t> b = srv.evictedSnapshot(sid);
+  if (b.size() < sizeof(cublox_snapshot_header)) {
+    throw std::invalid_argument("cublox::MapServer: session has no map yet");
+  }
+  cublox_snapshot_header hd;
+  std::memcpy(&hd, b.data(), sizeof(hd));
+  if (dst) {
+    const std::size_t V = static_cast<std::size_t>(std::max(hd.voxel_num, 0));
+    const uint8_t *p = b.data() + sizeof(hd);
+    if (hd.encoding == CUBLOX_SNAPSHOT_DENSE) {
+      if (hd.payload_bytes == V * 4) std::memcpy(dst, p, V * 4);
+      else std::fill(dst, dst + V, 0.0f);
+    } else {
+      uint32_t n = 0;
+      std::memcpy(&n, p, 4);
+      std::fill(dst, dst + V, 0.0f);
+      for (uint32_t k = 0; k < n; ++k) {
+        int32_t id;
+        float v;
+        std::memcpy(&id, p + 8 + 4ull * k, 4);
+        std::memcpy(&v, p + 8 + 4ull * n + 4ull * k, 4);
+        if (id >= 0 && static_cast<std::size_t>(id) < V) dst[id] = v;
+      }
+    }
+  }
+  return hd;
+}
+
 int cublox_ms_copy_occupancy(cublox_ms *h, int sid, float *dst) {
-  return guarded([&] { H->server->grid(sid).copyOccupancyToHost(dst); });
+  return guarded([&] {
+    if (H->server->admitted(sid)) {
+      H->server->grid(sid).copyOccupancyToHost(dst);
+    } else {
+      evictedMap(*H->server, sid, dst);
+    }
+  });
 }
 
 int cublox_ms_origin(cublox_ms *h, int sid, int origin_i[3]) {
   return guarded([&] {
-    const auto &g = H->server->grid(sid);
-    const Eigen::Vector3i o = g.getOriginIndex();
-    origin_i[0] = o.x();
-    origin_i[1] = o.y();
-    origin_i[2] = o.z();
+    if (H->server->admitted(sid)) {
+      const auto &g = H->server->grid(sid);
+      const Eigen::Vector3i o = g.getOriginIndex();
+      origin_i[0] = o.x();
+      origin_i[1] = o.y();
+      origin_i[2] = o.z();
+    } else {
+      const cublox_snapshot_header hd = evictedMap(*H->server, sid, nullptr);
+      for (int a = 0; a < 3; ++a) origin_i[a] = hd.origin_i[a];
+    }
   });
 }
 
-int cublox_ms_stats(cublox_ms *h, double out[11]) {
+int cublox_ms_stats(cublox_ms *h, double out[14]) {
   return guarded([&] {
     const MS::Stats s = H->server->stats();
     out[0] = static_cast<double>(s.ticks);
@@ -213,6 +262,9 @@ int cublox_ms_stats(cublox_ms *h, double out[11]) {
     out[8] = s.waiting;
     out[9] = H->server->nowMs();
     out[10] = s.launch_ms;
+    out[11] = static_cast<double>(s.evictions);
+    out[12] = static_cast<double>(s.restores);
+    out[13] = static_cast<double>(s.snapshot_bytes);
   });
 }
 

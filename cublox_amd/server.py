@@ -30,13 +30,13 @@ class MapServer:
                  max_points_per_scan: int = 131072, max_points_per_tick: int = 1 << 20,
                  max_grids_per_tick: int = 256, max_tiles_per_tick: int = 1 << 21,
                  max_segments_per_ray: int = 96, graph: bool = True,
-                 point_arena_bytes: int = 0):
+                 point_arena_bytes: int = 0, evict_idle: bool = False):
         L = _ffi.host()
-        lim = (C.c_longlong * 9)(int(hbm_budget_bytes), int(max_sessions),
-                                 int(max_points_per_scan), int(max_points_per_tick),
-                                 int(max_grids_per_tick), int(max_tiles_per_tick),
-                                 int(max_segments_per_ray), int(bool(graph)),
-                                 int(point_arena_bytes))
+        lim = (C.c_longlong * 10)(int(hbm_budget_bytes), int(max_sessions),
+                                  int(max_points_per_scan), int(max_points_per_tick),
+                                  int(max_grids_per_tick), int(max_tiles_per_tick),
+                                  int(max_segments_per_ray), int(bool(graph)),
+                                  int(point_arena_bytes), int(bool(evict_idle)))
         h = L.cublox_ms_create(lim)
         if not h:
             raise RuntimeError(L.cublox_og_last_error().decode())
@@ -84,6 +84,10 @@ class MapServer:
 
     def close_session(self, sid: int) -> None:
         self._check(self._L.cublox_ms_close(self._h, int(sid)))
+
+    def evict_session(self, sid: int) -> None:
+        """Move the session's map to a host snapshot (restored on readmission)."""
+        self._check(self._L.cublox_ms_evict(self._h, int(sid)))
 
     def admitted(self, sid: int) -> bool:
         out = C.c_int()
@@ -220,9 +224,10 @@ class MapServer:
         return o[0], o[1], o[2]
 
     def stats(self) -> dict:
-        out = (C.c_double * 11)()
+        out = (C.c_double * 14)()
         self._check(self._L.cublox_ms_stats(self._h, out))
         keys = ("ticks", "scans", "fetches", "points", "device_ms", "resident_bytes",
-                "workspace_bytes", "admitted", "waiting", "now_ms", "launch_ms")
+                "workspace_bytes", "admitted", "waiting", "now_ms", "launch_ms",
+                "evictions", "restores", "snapshot_bytes")
         return {k: (out[i] if k in ("device_ms", "now_ms", "launch_ms") else int(out[i]))
                 for i, k in enumerate(keys)}

@@ -60,6 +60,10 @@ struct ServerLimits {
   int max_segments_per_ray{96};      // sessions above it are refused
   bool graph{true};                  // replay the tick as a HIP graph
   std::size_t point_arena_bytes{4ull << 30};  // queued scans' device points
+  // When a session with queued requests waits for room, evict the least
+  // recently used idle session (nothing queued or in flight): its map goes
+  // to a host snapshot and comes back, bit-identical, when it is readmitted.
+  bool evict_idle{false};
 };
 
 class MapServer {
@@ -82,6 +86,8 @@ public:
                                 // device to the next tick's launch
     std::size_t resident_bytes{0}, workspace_bytes{0};
     int admitted{0}, waiting{0};
+    long long evictions{0}, restores{0};
+    std::size_t snapshot_bytes{0};  // host bytes held by evicted sessions
   };
 
   explicit MapServer(const ServerLimits &limits);
@@ -99,6 +105,13 @@ public:
   // freed and waiting sessions are admitted.
   void closeSession(int sid);
   bool admitted(int sid) const;
+  // Moves the session's map to a host snapshot (cublox/snapshot.h) and
+  // frees its device memory once none of its requests is in flight. Later
+  // requests queue; the session is readmitted (its map restored exactly)
+  // when it has requests and fits again.
+  void evictSession(int sid);
+  // The snapshot an evicted session holds (empty for a resident one).
+  std::vector<uint8_t> evictedSnapshot(int sid) const;
 
   // Thread-safe; the points are copied to the device before the call
   // returns (it waits while the point arena is full). T is row-major 3x4
@@ -148,7 +161,11 @@ private:
     std::size_t bytes{0};
     int tiles{0}, seg_bound{0};
     bool admitted{false}, closing{false};
+    bool in_waiting{false};       // listed in the admission queue
+    bool evict_requested{false};
     int busy{0};  // requests taken into a tick not yet complete
+    long long last_tick{-1};      // last tick that took its requests
+    std::vector<uint8_t> snapshot;  // the evicted map (empty when resident)
     std::deque<Request> queue;
   };
   struct FetchJob {
@@ -176,6 +193,10 @@ private:
   void run();
   void admitWaiting();                       // worker thread, lock held
   void retireClosed();                       // worker thread, lock held
+  void evictRequested();                     // worker thread, lock held
+  void evict(int sid, Session &s);           // worker thread, lock held
+  bool evictIdle();                          // worker thread, lock held
+  bool idleVictim() const;                   // lock held
   bool formTick(Tick &t);                    // lock held
   void submitTick(Tick &t);
   void enqueueFetches(Tick &t);

@@ -21,9 +21,10 @@ typedef struct cublox_ms cublox_ms;
  * [2] max points per scan, [3] max points per tick, [4] max grids per
  * tick, [5] max tiles per tick, [6] max segments per ray (sessions whose
  * segment bound is larger are refused), [7] graph replay (0/1), [8] bytes
- * of the device arena holding queued scans' points (0: 4 GiB). NULL on
- * failure. */
-cublox_ms *cublox_ms_create(const long long limits[9]);
+ * of the device arena holding queued scans' points (0: 4 GiB), [9] evict
+ * idle sessions to host snapshots when a session with requests waits for
+ * room (0/1). NULL on failure. */
+cublox_ms *cublox_ms_create(const long long limits[10]);
 void cublox_ms_destroy(cublox_ms *h);
 /* Device bytes a session of this half size is charged (4 B per voxel plus
  * the fetch output reserve). */
@@ -64,13 +65,17 @@ int cublox_ms_fetch_result(cublox_ms *h, long long id, float *pos,
                            unsigned char *state, int cap, int *n);
 
 /* After cublox_ms_drain: the session grid's occupancy (hash order) and
- * window origin. */
+ * window origin (read from its snapshot while the session is evicted). */
 int cublox_ms_copy_occupancy(cublox_ms *h, int sid, float *dst);
 int cublox_ms_origin(cublox_ms *h, int sid, int origin_i[3]);
 /* out: ticks, scans, fetches, points, device ms, resident bytes,
  * workspace bytes, admitted, waiting, ms since start, host ms between a
- * tick's end on the device and the next tick's launch (summed). */
-int cublox_ms_stats(cublox_ms *h, double out[11]);
+ * tick's end on the device and the next tick's launch (summed), evictions,
+ * restores, host bytes of evicted sessions' snapshots. */
+int cublox_ms_stats(cublox_ms *h, double out[14]);
+/* MapServer::evictSession: the session's map moves to a host snapshot once
+ * nothing of it is in flight, and is restored when it is readmitted. */
+int cublox_ms_evict(cublox_ms *h, int sid);
 
 /* Open-loop load (benchmark client): `clients` threads, each owning every
  * clients-th session, each with Poisson scan arrivals at its share of

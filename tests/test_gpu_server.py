@@ -221,3 +221,70 @@ def test_server_small_point_arena(cb, gpu_ready):
     assert_occ_bitexact(srv.occupancy(sid), g.occupancy(), "small arena")
     g.close()
     srv.close()
+
+
+def _scans_of(reqs, i):
+    return [r for r in reqs[i] if r[0] == "scan"]
+
+
+def test_server_manual_eviction(cb, gpu_ready):
+    """Sessions evicted to host snapshots between scans come back exactly:
+    every session's map equals its alone run, evicted or resident."""
+    reqs = _requests(cb)
+    srv = _server(cb)
+    sids = _open_all(cb, srv)
+    for i, sid in enumerate(sids):
+        for _, x in _scans_of(reqs, i)[:2]:
+            srv.submit_scan(sid, x.xs, x.ys, x.zs, x.T, x.base)
+    srv.drain()
+    srv.evict_session(sids[0])
+    srv.evict_session(sids[2])
+    srv.drain()
+    st = srv.stats()
+    assert st["evictions"] == 2 and st["snapshot_bytes"] > 0, st
+    assert not srv.admitted(sids[0]) and not srv.admitted(sids[2])
+    # An evicted map reads back from its snapshot.
+    g, _ = _alone(cb, 2, _scans_of(reqs, 2)[:2])
+    assert_occ_bitexact(srv.occupancy(sids[2]), g.occupancy(), "evicted map")
+    assert srv.origin(sids[2]) == g.geometry.origin_i
+    g.close()
+    for i, sid in enumerate(sids):
+        for _, x in _scans_of(reqs, i)[2:]:
+            srv.submit_scan(sid, x.xs, x.ys, x.zs, x.T, x.base)
+    srv.drain()
+    st = srv.stats()
+    assert st["restores"] == 2 and st["snapshot_bytes"] == 0, st
+    for i, sid in enumerate(sids):
+        assert srv.admitted(sid)
+        g, _ = _alone(cb, i, _scans_of(reqs, i))
+        assert srv.origin(sid) == g.geometry.origin_i, i
+        assert_occ_bitexact(srv.occupancy(sid), g.occupancy(), f"session {i}")
+        g.close()
+    srv.close()
+
+
+def test_server_evicts_idle_sessions_for_waiting_ones(cb, gpu_ready):
+    """A budget for two of three sessions with idle eviction on: each
+    session's scans run in turn, the idle ones going to host snapshots and
+    back, and every map ends as its alone run."""
+    reqs = _requests(cb)
+    b = [cb.MapServer.session_bytes(s[0]) for s in SESSIONS[:3]]
+    srv = _server(cb, hbm_budget_bytes=sorted(b)[1] + sorted(b)[2] + 1, evict_idle=True)
+    sids = _open_all(cb, srv)[:3]
+    order = [0, 1, 2, 0, 2, 1, 0, 1, 2]  # rounds of one or two scans
+    pos = [0, 0, 0]
+    for i in order:
+        scans = _scans_of(reqs, i)
+        for _, x in scans[pos[i]:pos[i] + 2]:
+            srv.submit_scan(sids[i], x.xs, x.ys, x.zs, x.T, x.base)
+        pos[i] = min(pos[i] + 2, len(scans))
+        srv.drain()
+    st = srv.stats()
+    assert st["evictions"] >= 2 and st["restores"] >= 1, st
+    assert st["resident_bytes"] <= sorted(b)[1] + sorted(b)[2] + 1
+    for i in range(3):
+        g, _ = _alone(cb, i, _scans_of(reqs, i)[:pos[i]])
+        assert srv.origin(sids[i]) == g.geometry.origin_i, i
+        assert_occ_bitexact(srv.occupancy(sids[i]), g.occupancy(), f"session {i}")
+        g.close()
+    srv.close()
