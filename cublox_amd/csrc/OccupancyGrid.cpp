@@ -96,6 +96,8 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_dense_pos_ = std::exchange(o.d_fetch_dense_pos_, nullptr);
   d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
+  h_stage_ = std::exchange(o.h_stage_, nullptr);
+  h_stage_bytes_ = std::exchange(o.h_stage_bytes_, 0);
   d_viz_ = std::exchange(o.d_viz_, nullptr);
   d_viz_count_ = std::exchange(o.d_viz_count_, nullptr);
   viz_capacity_ = std::exchange(o.viz_capacity_, 0);
@@ -151,6 +153,9 @@ void OccupancyGrid::releaseAll() noexcept {
   delta_capacity_ = 0;
   freeDevice(d_raw_);
   raw_capacity_ = 0;
+  if (h_stage_) cublox_host_free(h_stage_);
+  h_stage_ = nullptr;
+  h_stage_bytes_ = 0;
   freeDevice(d_viz_);
   freeDevice(d_viz_count_);
   viz_capacity_ = 0;
@@ -931,6 +936,17 @@ size_t OccupancyGrid::buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
 
 // ---- map snapshots ----------------------------------------------------
 
+uint8_t *OccupancyGrid::pinnedStage(size_t bytes) const {
+  if (bytes > h_stage_bytes_) {
+    if (h_stage_) cublox_host_free(h_stage_);
+    h_stage_ = nullptr;
+    h_stage_bytes_ = 0;
+    CUBLOX_CHECK(cublox_host_alloc(&h_stage_, bytes));
+    h_stage_bytes_ = bytes;
+  }
+  return static_cast<uint8_t *>(h_stage_);
+}
+
 std::vector<uint8_t> OccupancyGrid::saveSnapshot() const {
   cublox_snapshot_header h{};
   std::memcpy(h.magic, "CBLXSNAP", 8);
@@ -981,8 +997,8 @@ std::vector<uint8_t> OccupancyGrid::saveSnapshot() const {
     if (static_cast<size_t>(nnz) * 8 + 8 < V * 4) {
       h.encoding = CUBLOX_SNAPSHOT_SPARSE;
       h.payload_bytes = 8 + static_cast<uint64_t>(nnz) * 8;
-      out.resize(sizeof(h) + h.payload_bytes);
-      uint8_t *p = out.data() + sizeof(h);
+      // Device -> pinned staging -> the output, one host copy.
+      uint8_t *p = pinnedStage(h.payload_bytes);
       const uint32_t head[2] = {nnz, 0u};
       std::memcpy(p, head, 8);
       if (nnz > 0) {
@@ -996,12 +1012,18 @@ std::vector<uint8_t> OccupancyGrid::saveSnapshot() const {
                                              nnz * sizeof(float), stream_));
         sync();
       }
+      out.reserve(sizeof(h) + h.payload_bytes);
+      out.resize(sizeof(h));
+      out.insert(out.end(), p, p + h.payload_bytes);
     } else {
       h.encoding = CUBLOX_SNAPSHOT_DENSE;
       h.payload_bytes = V * 4;
-      out.resize(sizeof(h) + h.payload_bytes);
-      CUBLOX_CHECK(cublox_memcpy_d2h_async(out.data() + sizeof(h), d_occ_, V * 4, stream_));
+      uint8_t *p = pinnedStage(h.payload_bytes);
+      CUBLOX_CHECK(cublox_memcpy_d2h_async(p, d_occ_, V * 4, stream_));
       sync();
+      out.reserve(sizeof(h) + h.payload_bytes);
+      out.resize(sizeof(h));
+      out.insert(out.end(), p, p + h.payload_bytes);
     }
   } catch (...) {
     freeDevice(d_ids);

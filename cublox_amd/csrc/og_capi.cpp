@@ -11,6 +11,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
 #include <string>
 
 namespace {
@@ -177,17 +180,26 @@ int cublox_og_viz_cloud(const cublox_og *h, const float robot_pos[3],
   });
 }
 
+// Snapshot buffers handed to the caller: the vectors themselves, found by
+// their data pointer when the caller frees them (no copy on the way out).
+static std::mutex g_buf_mu;
+static std::unordered_map<void *, std::vector<uint8_t> *> g_bufs;
+
 int cublox_og_save_snapshot(const cublox_og *h, void **out, size_t *bytes) {
   return guarded([&] {
     if (!out || !bytes) throw std::invalid_argument("cublox_og_save_snapshot: null output");
     *out = nullptr;
     *bytes = 0;
-    const std::vector<uint8_t> b = OGC->saveSnapshot();
-    void *p = std::malloc(b.size());
-    if (!p) throw std::bad_alloc();
-    std::memcpy(p, b.data(), b.size());
+    auto v = std::make_unique<std::vector<uint8_t>>(OGC->saveSnapshot());
+    void *p = v->data();
+    const size_t n = v->size();
+    {
+      std::lock_guard<std::mutex> lk(g_buf_mu);
+      g_bufs.emplace(p, v.get());
+    }
+    v.release();
     *out = p;
-    *bytes = b.size();
+    *bytes = n;
   });
 }
 
@@ -196,7 +208,18 @@ int cublox_og_load_snapshot(cublox_og *h, const void *data, size_t bytes) {
       [&] { OGP->loadSnapshot(static_cast<const uint8_t *>(data), bytes); });
 }
 
-void cublox_og_free_buffer(void *p) { std::free(p); }
+void cublox_og_free_buffer(void *p) {
+  if (!p) return;
+  std::vector<uint8_t> *v = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_buf_mu);
+    auto it = g_bufs.find(p);
+    if (it == g_bufs.end()) return;
+    v = it->second;
+    g_bufs.erase(it);
+  }
+  delete v;
+}
 
 int cublox_og_save_snapshot_file(const cublox_og *h, const char *path) {
   return guarded([&] { OGC->saveSnapshotFile(path ? path : ""); });

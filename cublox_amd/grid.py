@@ -8,6 +8,7 @@ reference) in snake_case.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Sequence, Tuple
 
@@ -233,14 +234,18 @@ class OccupancyGrid:
         return out[:wr.value].copy(), int(fetched.value)
 
     # ---- map snapshots (format: include/cublox/snapshot.h) ---------------
-    def save_snapshot(self) -> bytes:
+    def save_snapshot(self) -> np.ndarray:
+        """The snapshot as a uint8 array viewing the library's buffer (freed
+        with the array)."""
         out = C.c_void_p(None)
         n = C.c_size_t(0)
         self._check(self._L.cublox_og_save_snapshot(self._h, C.byref(out), C.byref(n)))
-        try:
-            return C.string_at(out, n.value)
-        finally:
+        if not n.value:
             self._L.cublox_og_free_buffer(out)
+            return np.zeros(0, np.uint8)
+        arr = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint8)), (n.value,))
+        weakref.finalize(arr, self._L.cublox_og_free_buffer, C.c_void_p(out.value))
+        return arr
 
     def load_snapshot(self, data: bytes) -> None:
         buf = np.frombuffer(data, np.uint8)

@@ -336,6 +336,9 @@ private:
   mutable float *d_fetch_dense_pos_{nullptr};
   mutable unsigned char *d_fetch_dense_state_{nullptr};
   mutable unsigned int *d_fetch_offsets_{nullptr};
+  mutable void *h_stage_{nullptr};  // pinned staging of snapshot saves
+  mutable size_t h_stage_bytes_{0};
+  uint8_t *pinnedStage(size_t bytes) const;
   mutable float *d_viz_{nullptr};
   mutable unsigned int *d_viz_count_{nullptr};
   mutable long long viz_capacity_{0};

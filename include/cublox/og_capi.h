@@ -99,8 +99,8 @@ int cublox_og_viz_cloud(const cublox_og *h, const float robot_pos[3],
                         float *out4, unsigned int out_cap, unsigned int *written,
                         unsigned long long *fetched);
 /* Map snapshots (OccupancyGrid::saveSnapshot / loadSnapshot; format in
- * cublox/snapshot.h). save: *out is malloc'ed (free it with
- * cublox_og_free_buffer), *bytes its size. */
+ * cublox/snapshot.h). save: *out points to a buffer owned by the library
+ * until passed to cublox_og_free_buffer; *bytes is its size. */
 int cublox_og_save_snapshot(const cublox_og *h, void **out, size_t *bytes);
 int cublox_og_load_snapshot(cublox_og *h, const void *data, size_t bytes);
 void cublox_og_free_buffer(void *p);
