@@ -4,7 +4,8 @@
 // Same results as the reference's raycast-then-apply pair (kernels.cu:33-210)
 // — the same per-voxel op/hit counts folded once per scan — organised for
 // CDNA4, where device-scope atomics are the scarce resource (measured on
-// MI355X: ~27 G/s scattered, ~90 M/s on one address) and LDS is plentiful:
+// MI355X: ~27 G/s scattered, tens of M/s on one address) and LDS is
+// plentiful:
 //
 //   walk        one ray per lane. Steps < K (the near field: step s visits a
 //               voxel at L1 distance s from the sensor voxel) count into an
@@ -13,15 +14,18 @@
 //               entry into a 16x16x32 tile of the hash-ordered grid: each
 //               segment (32 B) carries the exact DDA state at the tile entry
 //               and goes to the ray's own slot range, its tile key to a
-//               parallel array; the ray's constants are stored once.
-//   bin_count   per workgroup, an LDS hash table aggregates its segments by
-//               tile; one global atomic per (workgroup, tile).
+//               parallel array. The four-lane walk (launches of up to
+//               262,144 rays) also counts its workgroup's segments per tile
+//               in an LDS hash: one global atomic per (workgroup, tile).
+//   bin_count   (after the one-lane walk of large launches) the same
+//               counts, from the stored segments.
 //   tile_scan   tile cursors and work items — one per tile, or one per
-//               1024-record chunk of a hot tile — in decreasing cost order
-//               (log2 classes); one workgroup for up to 12288 tiles, three
+//               chunk of a hot tile — in decreasing cost order (log2
+//               classes); one workgroup for up to 12288 tiles, three
 //               multi-workgroup passes beyond.
-//   bin_place   the same aggregation; one returning atomic per (workgroup,
-//               tile) reserves a range; segments are copied to tile order.
+//   bin_place   per workgroup, an LDS hash aggregates its segments by tile;
+//               one returning atomic per (workgroup, tile) reserves a
+//               range; each segment's slot index is written in tile order.
 //   tile_apply  one workgroup per work item: replay every segment's voxels
 //               into packed 16+16-bit LDS counters (32 KB), then fold the
 //               tile's counts into occ with all loads in flight. Chunks of
@@ -33,7 +37,7 @@
 // own geometry, sensor pose, points and occupancy buffer): per-grid
 // descriptors live in device memory and every kernel maps its rays, tiles
 // or cells to their grid, so a batch of any mix of grids drains through
-// the same seven-to-nine launches.
+// the same six-to-eight launches.
 //
 // The walk's t_max recurrence is a running FP32 sum, so a segment resumed
 // from its carried state is bit-identical to the uninterrupted walk. Every
@@ -54,6 +58,7 @@ constexpr int kMaxNearK = 12;
 constexpr int kRayAlign = 1024;   // grids' ray ranges start at multiples of it
 constexpr int kBinThreads = 512;
 constexpr int kHash = 4096;
+constexpr int kWalkHash = 1024;  // per-walk-workgroup tile record counts
 constexpr int kHashProbes = 64;
 constexpr int kChunk = 1024;      // records per chunk of a hot tile (default)
 constexpr int kApplyThreads = 256;
@@ -641,6 +646,30 @@ __device__ __forceinline__ GridDesc grid_of_near(const Batch &b, int cell) {
   return grid_at(b, cell, offsetof(GridDesc, near_base));
 }
 
+// Finds (inserting if needed) the LDS slot of `key` in a kSize-entry
+// table; -1 when the bounded probe finds neither. A key is either in the
+// table for every caller or for none: slots only fill, so a full probe
+// window stays full.
+template <int kSize>
+__device__ __forceinline__ int lds_hash_slot_n(int *keys, int key, bool insert) {
+  unsigned h = (static_cast<unsigned>(key) * 2654435761u) >> 20;
+  for (int p = 0; p < kHashProbes; ++p, ++h) {
+    h &= kSize - 1;
+    const int k = keys[h];
+    if (k == key) return static_cast<int>(h);
+    if (k == -1) {
+      if (!insert) return -1;
+      const int prev = atomicCAS(keys + h, -1, key);
+      if (prev == -1 || prev == key) return static_cast<int>(h);
+    }
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
+  return lds_hash_slot_n<kHash>(keys, key, insert);
+}
+
 // ----------------------------------------------------------------------
 // walk kernel
 // ----------------------------------------------------------------------
@@ -675,12 +704,34 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
   extern __shared__ int s_near[];  // the grid's near cube, sized at launch
   __shared__ int s_hist[kLenBuckets];
   __shared__ unsigned short s_perm[kWalkBlock];
+  // Records per tile of this workgroup (the binning's counts: one global
+  // atomic per distinct tile of the workgroup at its end). Only the quad
+  // walk counts: its launches are small, and its workgroups' few tiles fit
+  // the table; the one-lane walk of large batches leaves the counts to a
+  // binning pass.
+  constexpr bool kCount = kLanes == 4;
+  __shared__ int s_tkey[kCount ? kWalkHash : 1];
+  __shared__ int s_tcnt[kCount ? kWalkHash : 1];
   const int ray0 = blockIdx.x * kWalkBlock;
+  if (blockIdx.x == 0) {
+    // The changed-voxel counts start from zero (the folds run later).
+    const int n_grids = batch_grids(b);
+    for (int k = threadIdx.x; k < n_grids; k += blockDim.x) {
+      unsigned *c = b.descs ? load_const(b.descs + k).delta_count : b.one.delta_count;
+      if (c) *c = 0u;
+    }
+  }
   if (ray0 >= batch_rays(b)) return;
   const GridDesc d = grid_of_ray(b, ray0);
   const IntegrateArgs &g = d.g;
   const int cells = g.cube * g.cube * g.cube;
   for (int c = threadIdx.x; c < cells; c += blockDim.x) s_near[c] = 0;
+  if (kCount) {
+    for (int e = threadIdx.x; e < kWalkHash; e += blockDim.x) {
+      s_tkey[e] = -1;
+      s_tcnt[e] = 0;
+    }
+  }
   if (threadIdx.x < kLenBuckets) s_hist[threadIdx.x] = 0;
   __syncthreads();
   {
@@ -723,8 +774,17 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
     auto put = [&](const Segment &sg, int tile) {
       if (!lead) {
       } else if (nseg < g.max_seg) {
+        const int key = d.tile_base + tile;
         mine[nseg] = sg;
-        mine_tile[nseg] = d.tile_base + tile;
+        mine_tile[nseg] = key;
+        if (kCount) {
+          const int e = lds_hash_slot_n<kWalkHash>(s_tkey, key, true);
+          if (e >= 0) {
+            atomicAdd(s_tcnt + e, 1);
+          } else {
+            atomic_add_i32(w.tile_count + key, 1);
+          }
+        }
       } else {
         w.misc[2] = 1;
       }
@@ -825,29 +885,17 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
     const int v = s_near[c];
     if (v) atomic_add_i32(w.near_op + d.near_base + c, v);
   }
+  if (kCount) {
+    for (int e = threadIdx.x; e < kWalkHash; e += blockDim.x) {
+      const int key = s_tkey[e];
+      if (key >= 0) atomic_add_i32(w.tile_count + key, s_tcnt[e]);
+    }
+  }
 }
 
 // ----------------------------------------------------------------------
 // binning: LDS hash aggregation of segments by tile
 // ----------------------------------------------------------------------
-
-// Finds (inserting if needed) the LDS slot of `key`; -1 when the bounded
-// probe finds neither. A key is either in the table for every caller or
-// for none: slots only fill, so a full probe window stays full.
-__device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
-  unsigned h = (static_cast<unsigned>(key) * 2654435761u) >> 20;
-  for (int p = 0; p < kHashProbes; ++p, ++h) {
-    h &= kHash - 1;
-    const int k = keys[h];
-    if (k == key) return static_cast<int>(h);
-    if (k == -1) {
-      if (!insert) return -1;
-      const int prev = atomicCAS(keys + h, -1, key);
-      if (prev == -1 || prev == key) return static_cast<int>(h);
-    }
-  }
-  return -1;
-}
 
 // A block bins the records of `rays` consecutive rays (a divisor of
 // kRayAlign, so all of one grid), taken in flattened order -- an LDS prefix
@@ -865,14 +913,6 @@ __global__ void __launch_bounds__(kBinThreads)
   __shared__ int s_wave[kBinThreads / 64];
   const int r0 = blockIdx.x * rays;
   const int n_rays = batch_rays(b);
-  if (!kPlace && blockIdx.x == 0) {
-    // The changed-voxel counts start from zero (the folds run later).
-    const int n_grids = batch_grids(b);
-    for (int k = threadIdx.x; k < n_grids; k += kBinThreads) {
-      unsigned *c = b.descs ? load_const(b.descs + k).delta_count : b.one.delta_count;
-      if (c) *c = 0u;
-    }
-  }
   if (r0 >= n_rays) return;
   const GridDesc d = grid_of_ray(b, r0);
   const int t = threadIdx.x;
@@ -1867,11 +1907,15 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   }
   int st = check_launch("walk_kernel");
   if (st != CUBLOX_OK) return st;
+  // The quad walk counted each tile's records; after the one-lane walk a
+  // binning pass counts them.
   const int bin_rays = bin_rays_per_block(c);
   const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + bin_rays - 1) / bin_rays);
-  bin_kernel<false><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
-  st = check_launch("bin_kernel<count>");
-  if (st != CUBLOX_OK) return st;
+  if (c.max_rays > kQuadWalkRays) {
+    bin_kernel<false><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
+    st = check_launch("bin_kernel<count>");
+    if (st != CUBLOX_OK) return st;
+  }
   if (c.max_tiles <= kScanStage) {
     tile_scan_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(b, w);
     st = check_launch("tile_scan_kernel");
