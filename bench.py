@@ -589,7 +589,52 @@ def cpu_baseline(cb, w, scans, budget_s, threads):
                 kind="port",
                 sample=f"first {len(per)} scans of the same sequence, update+recenter, "
                        f"OpenMP {threads} threads (median)",
-                one_thread_ms=statistics.median(per1), one_thread_scans=len(per1))
+                one_thread_ms=statistics.median(per1), one_thread_scans=len(per1),
+                driver_steps=cpu_driver_steps(cb, w, scans, oracle))
+
+
+def cpu_driver_steps(cb, w, scans, oracle, reps=5):
+    """The driver's own CPU steps that the device rows replace, on one host
+    thread as the driver runs them (the oracle's restatement of main.cpp):
+    the PointCloud2 build (decode, filter, transform), the depth
+    back-projection, and the viz cloud (OCCUPIED fetch + build loop) on
+    the oracle grid after the first scans."""
+    out = {}
+    n = scans[0].n
+    rec = _records(scans[:reps], n)
+    r2 = np.float32(w.max_range) ** 2
+    times = []
+    for k in range(reps):
+        buf = rec[k * n * 4:(k + 1) * n * 4].view(np.uint8)
+        a = time.perf_counter()
+        oracle.build_cloud(buf, n, 16, (0, 4, 8), scans[k].T, r2)
+        times.append((time.perf_counter() - a) * 1e3)
+    out["point_cloud2_build_ms"] = statistics.median(times)
+    frames = _depth_frames(reps)
+    K = (525.0, 525.0, 319.5, 239.5)
+    times = []
+    for img in frames:
+        a = time.perf_counter()
+        oracle.depth_to_cloud(img.view(np.uint8).reshape(-1), 640, 480, 1280, 0, K)
+        times.append((time.perf_counter() - a) * 1e3)
+    out["depth_back_projection_ms"] = statistics.median(times)
+    o = oracle.OracleGrid(w.half, w.resolution, w.origin_at_center, w.threshold, threads=1)
+    o.set_max_raycast_range(w.max_range)
+    o.set_log_odds_params(*cb.log_odds())
+    for s in scans[:reps]:
+        o.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+        o.recenter(s.base)
+    base = np.asarray(scans[reps - 1].base, np.float32)
+    a = time.perf_counter()
+    pos, _st, fetched = o.fetch(base, 25.0, 2, 1_000_000, cap=1_000_000)
+    b = time.perf_counter()
+    recs = oracle.viz_points(pos, base, 25.0, 1.0 / w.resolution, w.origin_at_center, 1,
+                             1_000_000)
+    c = time.perf_counter()
+    out["viz_cloud"] = dict(radius_m=25.0, fetch_ms=(b - a) * 1e3, build_ms=(c - b) * 1e3,
+                            points=int(len(recs)), grid=f"after {reps} scans")
+    out["threads"] = 1
+    return out
 
 
 def main():
