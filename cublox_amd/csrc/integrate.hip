@@ -58,6 +58,7 @@ constexpr int kMaxNearK = 12;
 constexpr int kRayAlign = 1024;   // grids' ray ranges start at multiples of it
 constexpr int kBinThreads = 512;
 constexpr int kHash = 4096;
+constexpr int kBinCache = 16;  // records per binning thread kept for placing
 constexpr int kWalkHash = 1024;  // per-walk-workgroup tile record counts
 constexpr int kHashProbes = 64;
 constexpr int kChunk = 1024;      // records per chunk of a hot tile (default)
@@ -956,15 +957,31 @@ __global__ void __launch_bounds__(kBinThreads)
     return base0 + static_cast<long long>(lo) * max_seg + (f - s_pre[lo]);
   };
 
-  for (int f = t; f < total; f += kBinThreads) {
-    const int tile = w.seg_tile[slot_of(f)];
+  // The first kBinCache records of each thread keep their slot and hash
+  // entry (or ~tile when the hash had no room) in registers for the
+  // placing pass; the loop is unrolled so that they stay in registers and
+  // their loads are in flight together.
+  int c_off[kBinCache], c_ent[kBinCache];
+  auto count = [&](int f, int i) {
+    const long long slot = slot_of(f);
+    const int tile = w.seg_tile[slot];
     const int e = lds_hash_slot(s_key, tile, true);
     if (e >= 0) {
       atomicAdd(s_cnt + e, 1);
     } else if (!kPlace) {
       atomic_add_i32(w.tile_count + tile, 1);
     }
+    if (i >= 0) {
+      c_off[i] = static_cast<int>(slot - base0);
+      c_ent[i] = e >= 0 ? e : ~tile;
+    }
+  };
+#pragma unroll
+  for (int i = 0; i < kBinCache; ++i) {
+    const int f = t + i * kBinThreads;
+    if (f < total) count(f, kPlace ? i : -1);
   }
+  for (int f = t + kBinCache * kBinThreads; f < total; f += kBinThreads) count(f, -1);
   __syncthreads();
   for (int e = t; e < kHash; e += kBinThreads) {
     const int key = s_key[e];
@@ -978,13 +995,23 @@ __global__ void __launch_bounds__(kBinThreads)
   }
   if (!kPlace) return;
   __syncthreads();
-  for (int f = t; f < total; f += kBinThreads) {
-    const long long slot = slot_of(f);
-    const int tile = w.seg_tile[slot];
-    const int e = lds_hash_slot(s_key, tile, false);
+  auto place = [&](long long slot, int e, int tile) {
     const long long pos = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
                                  : atomicAdd(w.tile_cursor + tile, 1);
     if (pos < w.seg_capacity) w.order[pos] = static_cast<unsigned>(slot);
+  };
+#pragma unroll
+  for (int i = 0; i < kBinCache; ++i) {
+    const int f = t + i * kBinThreads;
+    if (f < total) {
+      const int v = c_ent[i];
+      place(base0 + c_off[i], v >= 0 ? v : -1, ~v);
+    }
+  }
+  for (int f = t + kBinCache * kBinThreads; f < total; f += kBinThreads) {
+    const long long slot = slot_of(f);
+    const int tile = w.seg_tile[slot];
+    place(slot, lds_hash_slot(s_key, tile, false), tile);
   }
 }
 
