@@ -1317,9 +1317,27 @@ __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
   if (k0 < end) first = w.segs[w.order[k0]];
   lds_zero(s_cnt4);
   __syncthreads();
-  if (k0 < end) replay_record(first, s_cnt);
-  for (int k = k0 + kApplyThreads; k < end; k += kApplyThreads) {
-    replay_record(w.segs[w.order[k]], s_cnt);
+  if (end - beg <= 2 * kApplyThreads) {  // uniform: at most two records a thread
+    if (k0 < end) replay_record(first, s_cnt);
+    for (int k = k0 + kApplyThreads; k < end; k += kApplyThreads) {
+      replay_record(w.segs[w.order[k]], s_cnt);
+    }
+  } else {
+    // Longer items (hot-tile chunks) pipeline a thread's records: while
+    // record k replays, record k + kApplyThreads is in flight and the slot
+    // index after it has arrived, so no replay waits on a dependent pair
+    // of loads.
+    Segment cur = first;
+    int o_next = k0 + kApplyThreads < end ? w.order[k0 + kApplyThreads] : 0;
+    for (int k = k0; k < end; k += kApplyThreads) {
+      const int k1 = k + kApplyThreads;
+      Segment nxt{};
+      if (k1 < end) nxt = w.segs[o_next];
+      const int o_after = k1 + kApplyThreads < end ? w.order[k1 + kApplyThreads] : 0;
+      replay_record(cur, s_cnt);
+      cur = nxt;
+      o_next = o_after;
+    }
   }
   __syncthreads();
 }
