@@ -34,6 +34,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Sources of the fused update kernels: a recorded HBM traffic figure holds
+# only for the kernels it was measured on.
+UPDATE_SOURCES = ("cublox_amd/csrc/integrate.hip", "cublox_amd/csrc/raycast_common.hpp",
+                  "cublox_amd/csrc/hip_common.hpp")
+
+
+def update_source_digest() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in UPDATE_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def host_cpus() -> dict:
+    """CPU counts of the host this runs on."""
+    return dict(nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
 def pct(xs, q):
     xs = sorted(xs)
     if not xs:
@@ -423,14 +444,29 @@ def run_fetch(cb, w, g, scene, k, reps=5):
     cap = min((2 * r_vox + 1) ** 3, 16384)
     times = []
     counts = None
+    k0 = g.kernel_times()
     for _ in range(reps):
         a = time.perf_counter()
         counts, _, _ = g.fetch_occupancy_batch(qs, w.fetch_radius, cb.OCCUPIED, cap)
         times.append((time.perf_counter() - a) * 1e3)
+    k1 = g.kernel_times()
+    launches = max(k1["fetch_launches"] - k0["fetch_launches"], 1)
+    kernel_ms = (k1["fetch_ms"] - k0["fetch_ms"]) / launches
+    tested = (k1["fetch_tested_voxels"] - k0["fetch_tested_voxels"]) / launches
+    samples = (k1["fetch_samples"] - k0["fetch_samples"]) / launches
+    # Algorithmic bytes of one batched launch: 4 B per voxel tested, 13 B
+    # per sample written (position + state), 4 B per query count.
+    fbytes = 4.0 * tested + 13.0 * samples + 4.0 * len(qs)
+    gbs = fbytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(fbytes), "avg_launch_ms": round(kernel_ms, 5),
+            "voxels_tested": int(tested), "samples": int(samples),
+            "kernel": "fetch_batch_kernel"}
     return dict(queries=int(len(qs)), radius_m=w.fetch_radius, filter="occupied",
                 ms_per_batch=statistics.median(times),
                 samples=int(counts.sum()), per_query_cap=int(cap),
-                queries_at_cap=int((counts >= cap).sum()))
+                queries_at_cap=int((counts >= cap).sum()), roofline=roof)
 
 
 def _records(scans, n, step=16):
@@ -738,10 +774,19 @@ def main():
         rooflines = {"update": rl(up_bytes, up_ms), "recenter": rl(re_bytes, re_ms)}
         # HBM bytes per update launch from the FETCH_SIZE / WRITE_SIZE runs
         # recorded in profiles/ (readme workload only).
+        # The figure holds only for the kernel sources it was measured on:
+        # with any other sources the traffic is reported as unmeasured.
         tpath = os.path.join(ROOT, "profiles", "update_traffic_readme_701x701x151.json")
+        digest = update_source_digest()
+        rooflines["update"]["source_digest"] = digest
         if w.name == "readme" and os.path.exists(tpath):
             with open(tpath) as fh:
-                rooflines["update"]["traffic"] = json.load(fh)["update_traffic_bytes"]
+                rec = json.load(fh)
+            if rec.get("source_digest") == digest:
+                rooflines["update"]["traffic"] = rec["update_traffic_bytes"]
+            else:
+                log(f"update traffic in {tpath} was measured on other kernel sources "
+                    f"({rec.get('source_digest')} != {digest}): reported as null")
         rooflines["update"]["visits_per_s"] = visits / (up_ms * 1e-3) if up_ms else 0
         totals = {"update": kt["update_ms"], "recenter": kt["recenter_ms"]}
         dom = max(totals, key=totals.get)
@@ -760,7 +805,19 @@ def main():
         log(f"per-voxel atomics path: {extras['per_voxel_atomics_path']}")
         pcie = run_pcie(cb, w, scans, args.warmup, args.steps)
         extras["pcie_inclusive_ms_per_scan"] = statistics.mean(pcie)
+        # The reference's own figure (README.md:32-35) times update() on a
+        # host PointCloud, i.e. with the points' host-to-device copies.
+        extras["h2d_inclusive"] = dict(
+            ms_per_scan=round(statistics.mean(pcie), 5),
+            p50_ms=round(pct(pcie, 50), 5), p99_ms=round(pct(pcie, 99), 5),
+            form="blocking updateSensorFrame on host columns (staged through a "
+                 "grid-owned pinned buffer, then DMA) + recenter",
+            speedup_vs_3090_readme=round(README_3090_MS / statistics.mean(pcie), 2),
+            target_ms=0.5)
+        log(f"h2d-inclusive: {extras['h2d_inclusive']}")
         extras["fetch"] = run_fetch(cb, w, g, scene, total - 1)
+        if extras["fetch"]:
+            rooflines["fetch"] = extras["fetch"]["roofline"]
         extras["driver_steps"] = run_driver_steps(
             cb, w, scans, bufs, n, args.warmup, args.steps, g,
             kt["update_ms"] / max(kt["update_launches"], 1))
@@ -778,8 +835,13 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        # The GPU host allots 16 CPU threads to a one-GPU job (its
+        # OMP_NUM_THREADS); the OpenMP restatement runs on that share and on
+        # one thread, and the host's own counts are reported beside them.
+        cpus = host_cpus()
+        threads = max(1, min(16, cpus["affinity"]))
         cpu = cpu_baseline(cb, w, scans, args.cpu_budget, threads)
+        cpu["host_cpus"] = cpus
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -810,6 +872,9 @@ def main():
         "kernel_ms": {k: round(v / max(kt[k.replace('_ms', '_launches')], 1), 5)
                       for k, v in kt.items() if k.endswith("_ms")},
         "speedup_vs_3090_readme": round(README_3090_MS / ms_per_step, 2),
+        "speedup_note": "speedup_vs_3090_readme divides the 3090's H2D-inclusive "
+                        "figure by the HBM-resident value; h2d_inclusive holds the "
+                        "like-for-like ratio",
         "roofline": roofline,
         "rooflines": rooflines,
         "cpu_baseline": cpu,

@@ -151,6 +151,7 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_launch_fetch_occupancy_batch", i, vp, i, i, i, vp, vp, vp,
          vp, C.c_uint, vp)
     _sig(L, "cublox_counter_census", i, vp, vp, i, vp, vp)
+    _sig(L, "cublox_launch_index_chain", i, vp, i, vp, vp)
     _sig(L, "cublox_launch_fetch_gather", i, vp, vp, i, C.c_uint, C.c_uint, vp,
          vp, vp, vp, vp)
     _sig(L, "cublox_integrate_workspace_bytes", S, C.POINTER(RaycastCfg), i, i)
@@ -211,7 +212,7 @@ def host() -> C.CDLL:
     _sig(L, "cublox_og_load_snapshot_file", i, vp, C.c_char_p)
     _sig(L, "cublox_og_viz_cloud", i, vp, f32p, C.c_float, i, i, f32p, C.c_uint,
          u32p, C.POINTER(C.c_ulonglong))
-    _sig(L, "cublox_og_update_cloud2", i, vp, vp, i, u32p, f32p, i32p)
+    _sig(L, "cublox_og_update_cloud2", i, vp, vp, i, C.c_ulonglong, u32p, f32p, i32p)
     _sig(L, "cublox_og_update_depth", i, vp, vp, i, C.c_uint, C.c_uint,
          C.c_uint, i, f32p, f32p, i32p)
     _sig(L, "cublox_og_changed_voxels", i, vp, i32p, f32p, C.c_uint,

@@ -68,7 +68,11 @@ MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
   try {
     MS_CHECK(cublox_stream_create(&upload_stream_));
     MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_arena_), lim_.point_arena_bytes));
+    MS_CHECK(cublox_host_alloc(&h_upload_, 3 * sizeof(float) *
+                                               static_cast<std::size_t>(lim_.max_points_per_scan)));
   } catch (...) {
+    if (d_arena_) cublox_free(d_arena_);
+    d_arena_ = nullptr;
     if (upload_stream_) cublox_stream_destroy(upload_stream_);
     throw;
   }
@@ -93,6 +97,7 @@ MapServer::~MapServer() {
   batch_.reset();
   for (auto &a : farena_) growFetchArena(a, 0, 0);  // frees
   if (d_arena_) cublox_free(d_arena_);
+  if (h_upload_) cublox_host_free(h_upload_);
   if (upload_stream_) cublox_stream_destroy(upload_stream_);
   sessions_.clear();  // frees the grids
 }
@@ -219,12 +224,24 @@ long long MapServer::submitScan(int sid, const float *xs, const float *ys, const
     r.arena_off = allocPoints(r.arena_bytes);
     r.d_pts = reinterpret_cast<float *>(d_arena_ + r.arena_off);
     try {
+      // The columns go through the pinned stage in pieces: each piece's DMA
+      // runs while the next is copied in (a pageable copy runs at a few
+      // GB/s).
       std::lock_guard<std::mutex> lk(upload_mu_);
       const std::size_t b = sizeof(float) * static_cast<std::size_t>(n);
-      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts, xs, b, upload_stream_));
-      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts + n, ys, b, upload_stream_));
-      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts + 2 * static_cast<std::size_t>(n), zs, b,
-                                       upload_stream_));
+      const float *cols[3] = {xs, ys, zs};
+      auto *stage = static_cast<uint8_t *>(h_upload_);
+      auto *dst = reinterpret_cast<uint8_t *>(r.d_pts);
+      constexpr std::size_t kPiece = std::size_t{128} << 10;
+      for (int c = 0; c < 3; ++c) {
+        const auto *src = reinterpret_cast<const uint8_t *>(cols[c]);
+        for (std::size_t off = 0; off < b; off += kPiece) {
+          const std::size_t len = std::min(kPiece, b - off);
+          std::memcpy(stage + c * b + off, src + off, len);
+          MS_CHECK(cublox_memcpy_h2d_async(dst + c * b + off, stage + c * b + off, len,
+                                           upload_stream_));
+        }
+      }
       MS_CHECK(cublox_stream_synchronize(upload_stream_));
     } catch (...) {
       releasePoints(r.arena_off, r.arena_bytes);

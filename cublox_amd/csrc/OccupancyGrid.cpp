@@ -96,6 +96,8 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_dense_pos_ = std::exchange(o.d_fetch_dense_pos_, nullptr);
   d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
+  h_upload_ = std::exchange(o.h_upload_, nullptr);
+  h_upload_bytes_ = std::exchange(o.h_upload_bytes_, 0);
   h_stage_ = std::exchange(o.h_stage_, nullptr);
   h_stage_bytes_ = std::exchange(o.h_stage_bytes_, 0);
   d_viz_ = std::exchange(o.d_viz_, nullptr);
@@ -113,7 +115,7 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   fetch_capacity_ = std::exchange(o.fetch_capacity_, 0);
   fetch_cfg_capacity_ = std::exchange(o.fetch_cfg_capacity_, 0);
   timing_ = std::exchange(o.timing_, false);
-  for (int k = 0; k < 6; ++k) events_[k] = std::exchange(o.events_[k], nullptr);
+  for (int k = 0; k < 8; ++k) events_[k] = std::exchange(o.events_[k], nullptr);
   for (int k = 0; k < 3; ++k) pending_[k] = std::exchange(o.pending_[k], false);
   pending_zeroed_ = std::exchange(o.pending_zeroed_, 0);
   times_ = o.times_;
@@ -156,6 +158,9 @@ void OccupancyGrid::releaseAll() noexcept {
   if (h_stage_) cublox_host_free(h_stage_);
   h_stage_ = nullptr;
   h_stage_bytes_ = 0;
+  if (h_upload_) cublox_host_free(h_upload_);
+  h_upload_ = nullptr;
+  h_upload_bytes_ = 0;
   freeDevice(d_viz_);
   freeDevice(d_viz_count_);
   viz_capacity_ = 0;
@@ -164,8 +169,7 @@ void OccupancyGrid::releaseAll() noexcept {
   freeDevice(d_hit_cnt_);
   freeDevice(d_occ_);
   freeDevice(d_cloud_x_);
-  freeDevice(d_cloud_y_);
-  freeDevice(d_cloud_z_);
+  d_cloud_y_ = d_cloud_z_ = nullptr;
   freeDevice(d_ws_);
   ws_bytes_ = 0;
   ws_points_ = 0;
@@ -256,8 +260,10 @@ void OccupancyGrid::resetVoxel(const int hash_id) {
 }
 
 // OccupancyGrid.cpp:141-179. The plan runs on the host (integer math on the
-// config), so there is no device round trip. The counters are zero between
-// updates (apply zeroes every non-zero counter), so only occ is cleared.
+// config), so there is no device round trip. When the per-voxel counters
+// exist they are cleared with occ, as the reference's launchRecenter does:
+// the stage-wise calls (castRays* then applyCounters) can leave counts
+// pending across a recenter, and those in the cleared slabs must go too.
 void OccupancyGrid::recenter(const Eigen::Vector3f &pos) {
   if (!config_.recenter_threshold.has_value() || d_occ_ == nullptr) return;
   if ((pos - origin_f_).squaredNorm() < config_.recenter_threshold_squared) return;
@@ -276,7 +282,8 @@ void OccupancyGrid::recenter(const Eigen::Vector3f &pos) {
 
   cublox_recenter_result result{};
   mark(4);
-  CUBLOX_CHECK(cublox_launch_recenter(d_occ_, nullptr, nullptr, &cfg, stream_, &result));
+  CUBLOX_CHECK(cublox_launch_recenter(d_occ_, d_op_cnt_, d_hit_cnt_, &cfg, stream_,
+                                      &result));
   mark(5);
   if (result.action == 0) return;
   sync();
@@ -522,18 +529,66 @@ void OccupancyGrid::launchApply() {
   pending_[1] = timing_;
 }
 
+// Host bytes to device memory on the grid's stream. A pageable
+// hipMemcpyAsync of a scan's columns runs at a few GB/s; instead the bytes
+// are copied into a grid-owned pinned buffer piece by piece, and each piece
+// goes out by DMA while the next is copied. The buffer is reused by the
+// next call, so the caller syncs the stream before returning (every public
+// update blocks).
+void OccupancyGrid::uploadHost(void *d_dst, const void *const *h_srcs,
+                              const size_t *lens, int parts) {
+  size_t bytes = 0;
+  for (int k = 0; k < parts; ++k) bytes += lens[k];
+  if (bytes == 0) return;
+  if (bytes > h_upload_bytes_) {
+    sync();  // an earlier upload may still read the old buffer
+    if (h_upload_) cublox_host_free(h_upload_);
+    h_upload_ = nullptr;
+    h_upload_bytes_ = 0;
+    const size_t cap = std::max<size_t>(bytes, size_t{1} << 20);
+    CUBLOX_CHECK(cublox_host_alloc(&h_upload_, cap));
+    h_upload_bytes_ = cap;
+  }
+  constexpr size_t kPiece = size_t{128} << 10;
+  auto *stage = static_cast<uint8_t *>(h_upload_);
+  auto *dst = static_cast<uint8_t *>(d_dst);
+  size_t base = 0;
+  for (int k = 0; k < parts; ++k) {
+    const auto *src = static_cast<const uint8_t *>(h_srcs[k]);
+    for (size_t off = 0; off < lens[k]; off += kPiece) {
+      const size_t len = std::min(kPiece, lens[k] - off);
+      std::memcpy(stage + base + off, src + off, len);
+      CUBLOX_CHECK(cublox_memcpy_h2d_async(dst + base + off, stage + base + off, len,
+                                           stream_));
+    }
+    base += lens[k];
+  }
+}
+
+void OccupancyGrid::uploadColumns(const float *x, const float *y, const float *z,
+                                  int n) {
+  const size_t b = static_cast<size_t>(n) * sizeof(float);
+  // d_cloud_y_ and d_cloud_z_ follow d_cloud_x_ at n floats each.
+  const void *srcs[3] = {x, y, z};
+  const size_t lens[3] = {b, b, b};
+  uploadHost(d_cloud_x_, srcs, lens, 3);
+}
+
+// One device buffer holds the three columns of the current cloud back to
+// back (x, then y at +n, then z at +2n), so one pinned stage uploads them.
+// d_cloud_y_ / d_cloud_z_ are set per call by placeCloud.
 void OccupancyGrid::ensureCloudCapacity(const int n) {
-  if (n <= cloud_capacity_) return;
-  sync();
-  freeDevice(d_cloud_x_);
-  freeDevice(d_cloud_y_);
-  freeDevice(d_cloud_z_);
-  cloud_capacity_ = 0;
-  const size_t bytes = static_cast<size_t>(n) * sizeof(float);
-  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_x_), bytes));
-  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_y_), bytes));
-  CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_z_), bytes));
-  cloud_capacity_ = n;
+  if (n > cloud_capacity_) {
+    sync();
+    freeDevice(d_cloud_x_);
+    d_cloud_y_ = d_cloud_z_ = nullptr;
+    cloud_capacity_ = 0;
+    const size_t bytes = 3 * static_cast<size_t>(n) * sizeof(float);
+    CUBLOX_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_cloud_x_), bytes));
+    cloud_capacity_ = n;
+  }
+  d_cloud_y_ = d_cloud_x_ + n;
+  d_cloud_z_ = d_cloud_x_ + 2 * static_cast<size_t>(n);
 }
 
 // OccupancyGrid.cpp:181-238: cast, then fold, then one sync (the reference
@@ -551,10 +606,7 @@ void OccupancyGrid::updateColumns(const float *x, const float *y,
   if (config_.voxel_num <= 0 || n <= 0) return;
   pinFirstOrigin(sensor_origin);
   ensureCloudCapacity(n);
-  const size_t bytes = static_cast<size_t>(n) * sizeof(float);
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_x_, x, bytes, stream_));
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_y_, y, bytes, stream_));
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_z_, z, bytes, stream_));
+  uploadColumns(x, y, z, n);
   if (update_path_ == UpdatePath::FUSED_TILES) {
     launchIntegrate(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, nullptr,
                     sensor_origin, false);
@@ -585,10 +637,7 @@ void OccupancyGrid::updateSensorFrameColumns(const float *xs, const float *ys,
   if (config_.voxel_num <= 0 || n <= 0) return;
   pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
   ensureCloudCapacity(n);
-  const size_t bytes = static_cast<size_t>(n) * sizeof(float);
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_x_, xs, bytes, stream_));
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_y_, ys, bytes, stream_));
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_cloud_z_, zs, bytes, stream_));
+  uploadColumns(xs, ys, zs, n);
   if (update_path_ == UpdatePath::FUSED_TILES) {
     launchIntegrate(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T,
                     Eigen::Vector3f(T[3], T[7], T[11]), false);
@@ -671,11 +720,26 @@ static int cappedPoints(uint32_t width, uint32_t height) {
       n, static_cast<unsigned long long>(OccupancyGrid::kMaxCloudPoints)));
 }
 
+// The payload must hold the records read: the first min(width * height,
+// kMaxCloudPoints) records of point_step bytes each.
+static void checkPayload(const OccupancyGrid::PointCloud2Layout &l, int n,
+                         size_t data_bytes) {
+  const unsigned long long need = static_cast<unsigned long long>(n) * l.point_step;
+  if (data_bytes < need) {
+    throw std::invalid_argument(
+        "cublox::OccupancyGrid: PointCloud2 payload of " + std::to_string(data_bytes) +
+        " bytes is shorter than the " + std::to_string(need) +
+        " bytes its width, height and point step describe");
+  }
+}
+
 bool OccupancyGrid::updatePointCloud2Device(const void *d_data,
                                             const PointCloud2Layout &layout,
-                                            const float T[12]) {
+                                            const float T[12],
+                                            size_t data_bytes) {
   checkLayout(layout);
   const int n = cappedPoints(layout.width, layout.height);
+  checkPayload(layout, n, data_bytes);
   if (n == 0) return false;
   return ingest(rawFloats(d_data, layout.off_x), rawFloats(d_data, layout.off_y),
                 rawFloats(d_data, layout.off_z),
@@ -684,9 +748,10 @@ bool OccupancyGrid::updatePointCloud2Device(const void *d_data,
 
 bool OccupancyGrid::updatePointCloud2(const void *data,
                                       const PointCloud2Layout &layout,
-                                      const float T[12]) {
+                                      const float T[12], size_t data_bytes) {
   checkLayout(layout);
   const int n = cappedPoints(layout.width, layout.height);
+  checkPayload(layout, n, data_bytes);
   if (n == 0) return false;
   const size_t bytes = static_cast<size_t>(n) * layout.point_step;
   if (bytes > raw_capacity_) {
@@ -696,8 +761,9 @@ bool OccupancyGrid::updatePointCloud2(const void *data,
     CUBLOX_CHECK(cublox_malloc(&d_raw_, bytes));
     raw_capacity_ = bytes;
   }
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_raw_, data, bytes, stream_));
-  return updatePointCloud2Device(d_raw_, layout, T);
+  const void *srcs[1] = {data};
+  uploadHost(d_raw_, srcs, &bytes, 1);
+  return updatePointCloud2Device(d_raw_, layout, T, bytes);
 }
 
 bool OccupancyGrid::updateDepthImageDevice(const void *d_data, uint32_t width,
@@ -727,7 +793,8 @@ bool OccupancyGrid::updateDepthImage(const void *data, uint32_t width,
     CUBLOX_CHECK(cublox_malloc(&d_raw_, bytes));
     raw_capacity_ = bytes;
   }
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_raw_, data, bytes, stream_));
+  const void *srcs[1] = {data};
+  uploadHost(d_raw_, srcs, &bytes, 1);
   return updateDepthImageDevice(d_raw_, width, height, step, encoding, K, T);
 }
 
@@ -1190,13 +1257,29 @@ void OccupancyGrid::fetchOccupancyBatch(
                                        sizeof(cublox_fetch_cfg) * nq, stream_));
   CUBLOX_CHECK(cublox_memset_async(d_fetch_counts_, 0, sizeof(unsigned int) * nq,
                                    stream_));
+  mark(6);
   CUBLOX_CHECK(cublox_launch_fetch_occupancy_batch(
       static_cast<const cublox_fetch_cfg *>(d_fetch_cfgs_), nq, max_diam, max_nz,
       d_occ_, d_fetch_counts_, d_fetch_pos_, d_fetch_state_,
       static_cast<unsigned int>(per_query_cap), stream_));
+  mark(7);
   CUBLOX_CHECK(cublox_memcpy_d2h_async(counts.data(), d_fetch_counts_,
                                        sizeof(unsigned int) * nq, stream_));
   sync();
+  if (timing_) {
+    float ms = 0.0f;
+    CUBLOX_CHECK(cublox_event_elapsed_ms(events_[6], events_[7], &ms));
+    times_.fetch_ms += ms;
+    times_.fetch_launches += 1;
+    for (const cublox_fetch_cfg &c : cfgs) {
+      if (c.disk_radius_vox < 0) continue;
+      const long long d = 2LL * c.disk_radius_vox + 1;
+      times_.fetch_tested_voxels += d * d * (c.gz_max - c.gz_min + 1);
+    }
+    for (int q = 0; q < nq; ++q) {
+      times_.fetch_samples += std::min(counts[q], static_cast<unsigned int>(per_query_cap));
+    }
+  }
   // Compact on the device: query q's samples go to offsets[q] of a dense
   // buffer, so only the real samples cross PCIe.
   std::vector<unsigned int> offsets(static_cast<size_t>(nq) + 1, 0u);

@@ -111,6 +111,30 @@ __host__ __device__ __forceinline__ int global_to_hash(int gx, int gy, int gz,
   return lx * my * mz + ly * mz + lz;
 }
 
+// utils.cuh:82-92: the global coordinate of local index l in the window
+// centred on o, i.e. the one g in [o - h, o + h] with g = l (mod map).
+__host__ __device__ __forceinline__ int local_to_global(int l, int map, int half,
+                                                        int o) {
+  const int lo = o - half;
+  int d = (l - lo) % map;
+  if (d < 0) d += map;
+  return lo + d;
+}
+
+// utils.cuh:120-129: hash id -> local index (each axis in [-half, half]).
+__host__ __device__ __forceinline__ void hash_to_local(int hid, int my, int mz,
+                                                       int hx, int hy, int hz,
+                                                       int &lx, int &ly,
+                                                       int &lz) {
+  const int yz = my * mz;
+  const int x = hid / yz;
+  const int r = hid - x * yz;
+  const int y = r / mz;
+  lx = x - hx;
+  ly = y - hy;
+  lz = (r - y * mz) - hz;
+}
+
 // kernels.cu:23-28
 __host__ __device__ __forceinline__ bool inside_window(int gx, int gy, int gz,
                                                        int ox, int oy, int oz,

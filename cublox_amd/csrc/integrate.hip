@@ -1328,12 +1328,13 @@ __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
     // index after it has arrived, so no replay waits on a dependent pair
     // of loads.
     Segment cur = first;
-    int o_next = k0 + kApplyThreads < end ? w.order[k0 + kApplyThreads] : 0;
+    unsigned o_next = k0 + kApplyThreads < end ? w.order[k0 + kApplyThreads] : 0u;
     for (int k = k0; k < end; k += kApplyThreads) {
       const int k1 = k + kApplyThreads;
       Segment nxt{};
       if (k1 < end) nxt = w.segs[o_next];
-      const int o_after = k1 + kApplyThreads < end ? w.order[k1 + kApplyThreads] : 0;
+      const unsigned o_after =
+          k1 + kApplyThreads < end ? w.order[k1 + kApplyThreads] : 0u;
       replay_record(cur, s_cnt);
       cur = nxt;
       o_next = o_after;
@@ -1727,7 +1728,9 @@ struct Layout {
 constexpr int kMinChunk = 256;
 
 // Record slots are indexed by 32-bit unsigned integers in tile order.
-bool caps_fit(const Caps &c) { return c.seg_capacity <= 0xFFFFFFFFll; }
+// Record slots, work-item ranges and tile cursors are 32-bit signed
+// indices: the workspace holds at most INT_MAX record slots.
+bool caps_fit(const Caps &c) { return c.seg_capacity <= 0x7FFFFFFFll; }
 
 Layout make_layout(const Caps &c) {
   Layout L{};

@@ -876,6 +876,45 @@ int launch_regions(float *occ, int *op, int *hit,
 
 using namespace cublox_hip;
 
+// Each link of the index chain of one case (utils.cuh:24-166), with the
+// device functions the kernels use.
+__global__ void __launch_bounds__(256)
+    index_chain_kernel(const cublox_index_case *__restrict__ cases, int n,
+                       cublox_index_chain *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cublox_index_case c = cases[i];
+  const int l[3] = {c.local.x, c.local.y, c.local.z};
+  const int m[3] = {c.map_size_i.x, c.map_size_i.y, c.map_size_i.z};
+  const int h[3] = {c.half_map_size_i.x, c.half_map_size_i.y, c.half_map_size_i.z};
+  const int o[3] = {c.origin_i.x, c.origin_i.y, c.origin_i.z};
+  const bool ctr = c.origin_at_center != 0;
+  int g[3], lb[3], gp[3];
+  float p[3];
+  for (int a = 0; a < 3; ++a) {
+    g[a] = local_to_global(l[a], m[a], h[a], o[a]);
+    lb[a] = global_to_local(g[a], m[a], h[a]);
+    p[a] = global_to_pos(g[a], c.resolution, ctr);
+    gp[a] = pos_to_global(p[a], c.inv_resolution, ctr);
+  }
+  // utils.cuh:131-136 on the local index itself (global_to_hash of any
+  // global coordinate with these local indices gives the same id).
+  const int hid = (l[0] + h[0]) * m[1] * m[2] + (l[1] + h[1]) * m[2] + (l[2] + h[2]);
+  int lh[3];
+  hash_to_local(hid, m[1], m[2], h[0], h[1], h[2], lh[0], lh[1], lh[2]);
+  cublox_index_chain r;
+  r.global = cublox_int3{g[0], g[1], g[2]};
+  r.local_back = cublox_int3{lb[0], lb[1], lb[2]};
+  r.hash_id = hid;
+  r.local_from_hash = cublox_int3{lh[0], lh[1], lh[2]};
+  r.global_from_hash = cublox_int3{local_to_global(lh[0], m[0], h[0], o[0]),
+                                   local_to_global(lh[1], m[1], h[1], o[1]),
+                                   local_to_global(lh[2], m[2], h[2], o[2])};
+  r.pos = cublox_float3{p[0], p[1], p[2]};
+  r.global_from_pos = cublox_int3{gp[0], gp[1], gp[2]};
+  out[i] = r;
+}
+
 extern "C" {
 
 int cublox_launch_ray_cast_update(const cublox_raycast_cfg *cfg,
@@ -1274,6 +1313,14 @@ int cublox_counter_census(const int *d_op, const int *d_hit, int n,
   int blocks = std::max(1, std::min((n + 255) / 256, 2048));
   census_kernel<<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(d_op, d_hit, n, d_out);
   return check_launch("census_kernel");
+}
+
+int cublox_launch_index_chain(const cublox_index_case *d_cases, int n,
+                              cublox_index_chain *d_out, cublox_stream s) {
+  if (n <= 0 || !d_cases || !d_out) return CUBLOX_OK;
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  index_chain_kernel<<<dim3(blocks), dim3(256), 0, as_stream(s)>>>(d_cases, n, d_out);
+  return check_launch("index_chain_kernel");
 }
 
 } // extern "C"

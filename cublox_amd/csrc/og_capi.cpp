@@ -127,6 +127,7 @@ int cublox_og_set_near_steps(cublox_og *h, int k) {
 }
 
 int cublox_og_update_cloud2(cublox_og *h, const void *data, int on_device,
+                            unsigned long long data_bytes,
                             const unsigned int layout[6], const float T[12],
                             int *updated) {
   return guarded([&] {
@@ -137,8 +138,9 @@ int cublox_og_update_cloud2(cublox_og *h, const void *data, int on_device,
     l.off_x = layout[3];
     l.off_y = layout[4];
     l.off_z = layout[5];
-    const bool u = on_device ? OGP->updatePointCloud2Device(data, l, T)
-                             : OGP->updatePointCloud2(data, l, T);
+    const size_t b = static_cast<size_t>(data_bytes);
+    const bool u = on_device ? OGP->updatePointCloud2Device(data, l, T, b)
+                             : OGP->updatePointCloud2(data, l, T, b);
     if (updated) *updated = u ? 1 : 0;
   });
 }
@@ -384,17 +386,21 @@ int cublox_og_enable_timing(cublox_og *h, int on) {
 
 // out_ms[3] = ray cast, apply, recenter device time; out_n[4] = launches of
 // each, then voxels zeroed by recenters.
-int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
-                           long long out_n[4]) {
+int cublox_og_kernel_times(const cublox_og *h, double out_ms[4],
+                           long long out_n[7]) {
   return guarded([&] {
     const auto t = OGC->kernelTimes();
     out_ms[0] = t.update_ms;
     out_ms[1] = t.apply_ms;
     out_ms[2] = t.recenter_ms;
+    out_ms[3] = t.fetch_ms;
     out_n[0] = t.update_launches;
     out_n[1] = t.apply_launches;
     out_n[2] = t.recenter_launches;
     out_n[3] = t.recenter_zeroed_voxels;
+    out_n[4] = t.fetch_launches;
+    out_n[5] = t.fetch_tested_voxels;
+    out_n[6] = t.fetch_samples;
   });
 }
 

@@ -114,3 +114,27 @@ class Event:
 
 def synchronize() -> None:
     _ffi.check(_ffi.hip().cublox_device_synchronize())
+
+
+# cublox_index_case / cublox_index_chain of include/cublox/capi.h.
+INDEX_CASE = np.dtype([("local", "<i4", 3), ("map", "<i4", 3), ("half", "<i4", 3),
+                       ("origin", "<i4", 3), ("res", "<f4"), ("inv", "<f4"),
+                       ("center", "<i4")])
+INDEX_CHAIN = np.dtype([("g", "<i4", 3), ("l_back", "<i4", 3), ("hash", "<i4"),
+                        ("l_hash", "<i4", 3), ("g_hash", "<i4", 3), ("pos", "<f4", 3),
+                        ("g_pos", "<i4", 3)])
+
+
+def index_chain(cases: np.ndarray) -> np.ndarray:
+    """Runs the device index chain (cublox_launch_index_chain) over an array
+    of INDEX_CASE records; returns INDEX_CHAIN records."""
+    cases = np.ascontiguousarray(cases, INDEX_CASE)
+    n = len(cases)
+    if n == 0:
+        return np.zeros(0, INDEX_CHAIN)
+    d_in = DeviceBuffer.from_array(cases.view(np.uint8))
+    d_out = DeviceBuffer(n * INDEX_CHAIN.itemsize)
+    _ffi.check(_ffi.hip().cublox_launch_index_chain(d_in.ptr, n, d_out.ptr, None))
+    raw = d_out.download(np.uint8, n * INDEX_CHAIN.itemsize)
+    return raw.view(INDEX_CHAIN).copy()
+

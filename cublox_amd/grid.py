@@ -171,22 +171,34 @@ class OccupancyGrid:
     # ---- driver ingest (reference main.cpp:558-693, on the device) ------
     def update_point_cloud2(self, data, width: int, height: int,
                             point_step: int, offsets=(0, 4, 8), T=None,
-                            device: bool = False) -> bool:
+                            device: bool = False,
+                            nbytes: Optional[int] = None) -> bool:
         """A PointCloud2 xyz payload (host bytes, or a device pointer with
         device=True) in the sensor frame: the first 1,000,000 records,
         finite and within the max raycast range, transformed by T and
-        integrated. False (nothing updated) when no point is kept."""
+        integrated. False (nothing updated) when no point is kept.
+
+        The payload must hold min(width * height, 1,000,000) * point_step
+        bytes (ValueError otherwise). A host payload's length is its own;
+        a device payload's is `nbytes`, or taken as exactly that many bytes
+        when `nbytes` is None."""
         lay = (C.c_uint * 6)(int(width), int(height), int(point_step),
                              int(offsets[0]), int(offsets[1]), int(offsets[2]))
+        need = min(int(width) * int(height), 1_000_000) * int(point_step)
         Tm = np.ascontiguousarray(np.asarray(T, np.float32).reshape(12))
         upd = C.c_int(0)
         if device:
             ptr, keep = C.c_void_p(int(data)), None
+            size = need if nbytes is None else int(nbytes)
         else:
             keep = np.ascontiguousarray(np.frombuffer(memoryview(data).cast("B"), np.uint8))
             ptr = C.c_void_p(keep.ctypes.data)
+            size = keep.nbytes
+        if size < need:
+            raise ValueError(f"PointCloud2 payload of {size} bytes is shorter than the "
+                             f"{need} bytes its width, height and point step describe")
         self._check(self._L.cublox_og_update_cloud2(
-            self._h, ptr, 1 if device else 0, lay, _ffi.fptr(Tm), C.byref(upd)))
+            self._h, ptr, 1 if device else 0, size, lay, _ffi.fptr(Tm), C.byref(upd)))
         return bool(upd.value)
 
     def update_depth_image(self, image, K, T, *, width: int = 0, height: int = 0,
@@ -409,12 +421,13 @@ class OccupancyGrid:
 
     def kernel_times(self) -> dict:
         """Device ms per kernel (HIP events on this grid's stream)."""
-        ms = (C.c_double * 3)()
-        n = (C.c_longlong * 4)()
+        ms = (C.c_double * 4)()
+        n = (C.c_longlong * 7)()
         self._check(self._L.cublox_og_kernel_times(self._h, ms, n))
         return dict(update_ms=ms[0], apply_ms=ms[1], recenter_ms=ms[2],
-                    update_launches=n[0], apply_launches=n[1],
-                    recenter_launches=n[2], recenter_zeroed_voxels=n[3])
+                    fetch_ms=ms[3], update_launches=n[0], apply_launches=n[1],
+                    recenter_launches=n[2], recenter_zeroed_voxels=n[3],
+                    fetch_launches=n[4], fetch_tested_voxels=n[5], fetch_samples=n[6])
 
     @property
     def device_occupancy_ptr(self) -> int:

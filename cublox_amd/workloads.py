@@ -109,3 +109,20 @@ def batched_grid(w: Workload, grid_id: int):
     scene = Scene(seed=w.seed(grid_id), n_boxes=w.n_boxes, box_extent=w.box_extent,
                   speed=w.speed, yaw_rate_deg=w.yaw_rate_deg)
     return beams, az, rmax, scene
+
+
+def batched_scan_shape(w: Workload, grid_id: int, tick: int):
+    """Per (grid, tick) draw of the batched workload's scan: (beams,
+    azimuths, max range), i.e. a point count in [8,192, 131,072] and a range
+    in [10, 40] m, from a generator seeded with the grid's seed and the
+    tick."""
+    import numpy as np
+    rng = np.random.default_rng([w.seed(grid_id), int(tick)])
+    lo_b, hi_b = w.extra["beams_range"]
+    lo_a, hi_a = w.extra["az_range"]
+    lo_r, hi_r = w.extra["range_range"]
+    beams = int(rng.integers(lo_b, hi_b + 1))
+    az = int(rng.integers(lo_a, hi_a + 1))
+    rmax = float(np.float32(rng.uniform(lo_r, hi_r)))
+    return beams, az, rmax
+

@@ -219,7 +219,8 @@ private:
   std::condition_variable arena_cv_;
   bool arena_closed_{false};
   void *upload_stream_{nullptr};
-  std::mutex upload_mu_;
+  std::mutex upload_mu_;  // one upload at a time through the pinned stage
+  void *h_upload_{nullptr};  // pinned: one largest scan's three columns
 
   mutable std::mutex mu_;
   std::condition_variable cv_work_, cv_idle_;

@@ -171,12 +171,15 @@ public:
   // the max raycast range (main.cpp:672-679), transformed by T (row-major
   // 3x4 [R | t], sensor -> world) and integrated with t as the sensor
   // origin. Returns false, updating nothing, when no point is kept
-  // (main.cpp:686-690). Runs on the fused path. Blocks.
+  // (main.cpp:686-690). Runs on the fused path. Blocks. `data_bytes` is the
+  // payload's length (the message's data size): std::invalid_argument when
+  // it is shorter than the records read (min(width * height, 1 M) *
+  // point_step bytes).
   bool updatePointCloud2(const void *data, const PointCloud2Layout &layout,
-                         const float T[12]);
+                         const float T[12], size_t data_bytes);
   bool updatePointCloud2Device(const void *d_data,
                                const PointCloud2Layout &layout,
-                               const float T[12]);
+                               const float T[12], size_t data_bytes);
   // depthCallback (main.cpp:558-638) then the above: the image (row pitch
   // `step` bytes) is back-projected with K = {fx, fy, cx, cy} into the
   // sensor-frame cloud the driver publishes, invalid pixels as NaN points.
@@ -243,10 +246,14 @@ public:
   // (off by default). update_ms covers the ray cast (and, on the fused
   // path, the fold); apply_ms the separate fold of the per-voxel path.
   // Recenter also accumulates the voxels it zeroed.
+  // fetch_ms covers the batched fetch's classification kernel
+  // (fetchOccupancyBatch), with the voxels its queries tested and the
+  // samples they found.
   struct KernelTimes {
-    double update_ms{0.0}, apply_ms{0.0}, recenter_ms{0.0};
+    double update_ms{0.0}, apply_ms{0.0}, recenter_ms{0.0}, fetch_ms{0.0};
     long long update_launches{0}, apply_launches{0}, recenter_launches{0};
     long long recenter_zeroed_voxels{0};
+    long long fetch_launches{0}, fetch_tested_voxels{0}, fetch_samples{0};
   };
   void enableKernelTiming(bool on);
   KernelTimes kernelTimes() const { return times_; }
@@ -303,12 +310,13 @@ private:
 
   void *stream_{nullptr};
 
-  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter.
+  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter, [6,7] fetch.
   bool timing_{false};
-  void *events_[6]{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void *events_[8]{nullptr, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, nullptr, nullptr};
   bool pending_[3]{false, false, false};
   long long pending_zeroed_{0};
-  KernelTimes times_{};
+  mutable KernelTimes times_{};
 
   int *d_op_cnt_{nullptr};
   int *d_hit_cnt_{nullptr};
@@ -336,6 +344,13 @@ private:
   mutable float *d_fetch_dense_pos_{nullptr};
   mutable unsigned char *d_fetch_dense_state_{nullptr};
   mutable unsigned int *d_fetch_offsets_{nullptr};
+  // Pinned staging of host inputs (clouds, payloads, images): a host copy
+  // into it, then asynchronous DMA, in pieces so the two overlap.
+  void *h_upload_{nullptr};
+  size_t h_upload_bytes_{0};
+  void uploadHost(void *d_dst, const void *const *h_srcs, const size_t *lens,
+                  int parts);
+  void uploadColumns(const float *x, const float *y, const float *z, int n);
   mutable void *h_stage_{nullptr};  // pinned staging of snapshot saves
   mutable size_t h_stage_bytes_{0};
   uint8_t *pinnedStage(size_t bytes) const;

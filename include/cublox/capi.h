@@ -517,6 +517,40 @@ int cublox_integrate_counters(const void *d_ws, size_t ws_bytes,
 int cublox_counter_census(const int *d_op_cnt, const int *d_hit_cnt, int n,
                           unsigned long long *d_out, cublox_stream s);
 
+/* ------------------------------------------------------------------ */
+/* Device index chain: the reference's device round trip over the index
+ * math of utils.cuh (test/test_utils_index_roundtrip.cu:160-211, which runs
+ * localIndexToGlobalIndex, globalIndexToLocalIndex, localIndexToHashId,
+ * hashIdToLocalIndex, hashIdToGlobalIndex, localIndexToPos and
+ * posToGlobalIndex in a kernel). Here the kernel writes every link of the
+ * chain for each case, computed by the same device functions the kernels of
+ * this library use, so a caller can compare each value with a host
+ * computation rather than only count failures. d_cases and d_out are
+ * device arrays of n entries.                                          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  cublox_int3 local;      /* local index, each axis in [-half, half] */
+  cublox_int3 map_size_i;
+  cublox_int3 half_map_size_i;
+  cublox_int3 origin_i;   /* window origin, global voxel */
+  float resolution;
+  float inv_resolution;
+  int origin_at_center;
+} cublox_index_case;
+
+typedef struct {
+  cublox_int3 global;            /* local -> global */
+  cublox_int3 local_back;        /* global -> local */
+  int hash_id;                   /* local -> hash */
+  cublox_int3 local_from_hash;   /* hash -> local */
+  cublox_int3 global_from_hash;  /* hash -> global */
+  cublox_float3 pos;             /* local -> position */
+  cublox_int3 global_from_pos;   /* position -> global */
+} cublox_index_chain;
+
+int cublox_launch_index_chain(const cublox_index_case *d_cases, int n,
+                              cublox_index_chain *d_out, cublox_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,8 +83,11 @@ int cublox_og_set_near_steps(cublox_og *h, int k);
  * OccupancyGrid::updatePointCloud2 / updateDepthImage. layout = {width,
  * height, point_step, off_x, off_y, off_z}; on_device: data is device
  * memory. encoding 0: 16-bit millimetres, 1: 32-bit float metres; K = {fx,
- * fy, cx, cy}. *updated = 1 when some point was kept (the grid updated). */
+ * fy, cx, cy}. *updated = 1 when some point was kept (the grid updated).
+ * data_bytes: the payload's length; CUBLOX_ERR_INVALID when it is shorter
+ * than min(width * height, 1 M) * point_step. */
 int cublox_og_update_cloud2(cublox_og *h, const void *data, int on_device,
+                            unsigned long long data_bytes,
                             const unsigned int layout[6], const float T[12],
                             int *updated);
 int cublox_og_update_depth(cublox_og *h, const void *data, int on_device,
@@ -159,11 +162,13 @@ int cublox_og_max_horizontal_radius(const cublox_og *h, const float c[3],
                                     float *r);
 
 /* Per-kernel device timing (no reference counterpart): out_ms = ray cast
- * (or fused update), apply, recenter device-time totals; out_n = their
- * launch counts, then the voxels zeroed by recenters. */
+ * (or fused update), apply, recenter and batched-fetch kernel device-time
+ * totals; out_n = the first three launch counts, the voxels zeroed by
+ * recenters, the fetch launches, the voxels those fetches tested and the
+ * samples they returned. */
 int cublox_og_enable_timing(cublox_og *h, int on);
-int cublox_og_kernel_times(const cublox_og *h, double out_ms[3],
-                           long long out_n[4]);
+int cublox_og_kernel_times(const cublox_og *h, double out_ms[4],
+                           long long out_n[7]);
 int cublox_og_reset_kernel_times(cublox_og *h);
 
 /* Batched updates of many grids (cublox::GridBatch, no reference

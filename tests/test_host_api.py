@@ -158,3 +158,38 @@ def test_snapshot_header_layout(tmp_path):
                                           text=True).stdout.split()]
     assert out[0] == HEADER.itemsize == 120
     assert out[1:] == [HEADER.fields[f][1] for f in fields]
+
+
+def test_index_chain_struct_layout(cb, tmp_path):
+    """The numpy records of the device index chain match the C structs."""
+    from cublox_amd.device import INDEX_CASE, INDEX_CHAIN
+    src = tmp_path / "idx.c"
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include <cublox/capi.h>\n"
+        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(cublox_index_case),"
+        " sizeof(cublox_index_chain), offsetof(cublox_index_case, resolution),"
+        " offsetof(cublox_index_chain, pos), offsetof(cublox_index_chain, hash_id));"
+        "return 0;}\n")
+    exe = tmp_path / "idx"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert out == [INDEX_CASE.itemsize, INDEX_CHAIN.itemsize,
+                   INDEX_CASE.fields["res"][1], INDEX_CHAIN.fields["pos"][1],
+                   INDEX_CHAIN.fields["hash"][1]]
+
+
+def test_cpp_dropin_caller_links(cb, tmp_path):
+    """A reference-shaped C++ caller (OccupancyGrid.hpp, Eigen PointCloud,
+    config_ members, update / recenter / fetchOccupancyAround / isOccupied)
+    compiles against the drop-in headers and links against libcublox.so
+    (it runs in the GPU tests)."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_caller.cpp")
+    exe = tmp_path / "caller"
+    pkg = os.path.join(ROOT, "cublox_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-Wall", "-Werror",
+                    f"-I{ROOT}/include", f"-I{ROOT}/include/cublox/compat", src,
+                    f"-L{pkg}", "-lcublox", f"-Wl,-rpath,{pkg}", "-o", str(exe)],
+                   check=True)
+    assert os.path.exists(exe)
