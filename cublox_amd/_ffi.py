@@ -152,6 +152,9 @@ def hip() -> C.CDLL:
          vp, C.c_uint, vp)
     _sig(L, "cublox_counter_census", i, vp, vp, i, vp, vp)
     _sig(L, "cublox_launch_index_chain", i, vp, i, vp, vp)
+    _sig(L, "cublox_launch_integrate_noalloc", i, C.POINTER(RaycastCfg), f32p, vp, vp,
+         vp, i, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, i, vp,
+         vp, vp)
     _sig(L, "cublox_launch_fetch_gather", i, vp, vp, i, C.c_uint, C.c_uint, vp,
          vp, vp, vp, vp)
     _sig(L, "cublox_integrate_workspace_bytes", S, C.POINTER(RaycastCfg), i, i)

@@ -48,6 +48,9 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <mutex>
+#include <string>
+#include <vector>
 
 namespace cublox_hip {
 
@@ -2014,6 +2017,67 @@ int read_counters(const Layout &L, const void *d_ws, size_t ws_bytes, int *out,
   return CUBLOX_OK;
 }
 
+// ---------------------------------------------------------------------
+// The update without allocation: its workspace is this file-scope device
+// array. A host-side reference to `scratch` is only a stand-in for the
+// symbol, so host code uses the address hipGetSymbolAddress returns for the
+// current device (each device holds its own copy), looked up on a device's
+// first call and kept per device.
+// ---------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = CUBLOX_NOALLOC_SCRATCH_BYTES;
+constexpr int kNoAllocSlabs = 64;  // hot tiles past them run unsplit
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int noalloc_scratch(void **out) {
+  static std::mutex mu;
+  static std::vector<void *> ptr;  // per device
+  int dev = 0;
+  CUBLOX_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if (static_cast<size_t>(dev) >= ptr.size()) ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    CUBLOX_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(CUBLOX_ERR_RUNTIME, "integrate_noalloc: no scratch address");
+    ptr[dev] = p;
+  }
+  *out = ptr[dev];
+  return CUBLOX_OK;
+}
+
+// Checks a single-grid update and builds its descriptor; Caps from the
+// grid's geometry for `max_points` points.
+int prepare_grid(const cublox_integrate_grid *q, int max_points, int near_steps,
+                 Caps &caps, Batch &b) {
+  int st = check_cfg(q->cfg, near_steps);
+  if (st != CUBLOX_OK) return st;
+  if (q->n > max_points) {
+    return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
+  }
+  caps = single_caps(q->cfg, max_points, near_steps);
+  if (!caps_fit(caps)) {
+    return set_error(CUBLOX_ERR_CAPACITY,
+                     "integrate: max_points x segment bound above 2^31 slots");
+  }
+  if ((q->d_export_op == nullptr) != (q->d_export_hit == nullptr)) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate: export needs both arrays");
+  }
+  st = check_delta(q->d_delta_count, q->d_delta_idx, q->d_delta_val,
+                   q->delta_capacity, q->d_export_op != nullptr, "integrate");
+  if (st != CUBLOX_OK) return st;
+  st = check_input(*q, "integrate");
+  if (st != CUBLOX_OK) return st;
+  b = Batch{};
+  b.one = make_desc(q->cfg, q->use_T ? q->T : nullptr, q->d_xs, q->d_ys, q->d_zs,
+                    q->n, q->d_occ,
+                    ApplyArgs{q->l_hit, q->l_miss, q->l_min, q->l_max, q->l_unknown},
+                    near_steps, q->d_export_op, q->d_export_hit);
+  set_input(b.one, *q);
+  set_delta(b.one, q->d_delta_count, q->d_delta_idx, q->d_delta_val,
+            q->delta_capacity);
+  return CUBLOX_OK;
+}
+
 } // namespace cublox_hip
 
 using namespace cublox_hip;
@@ -2095,38 +2159,76 @@ int cublox_launch_integrate_grid(const cublox_integrate_grid *q, void *d_ws,
     return q->d_delta_count ? cublox_fill_zero_async(q->d_delta_count, 1, s) : CUBLOX_OK;
   }
   const int near_steps = q->near_steps > 0 ? q->near_steps : 8;
-  int st = check_cfg(q->cfg, near_steps);
+  Caps caps{};
+  Batch b{};
+  const int st = prepare_grid(q, max_points, near_steps, caps, b);
   if (st != CUBLOX_OK) return st;
-  if (q->n > max_points) {
-    return set_error(CUBLOX_ERR_CAPACITY, "integrate: n exceeds max_points");
-  }
-  const Caps caps = single_caps(q->cfg, max_points, near_steps);
-  if (!caps_fit(caps)) {
-    return set_error(CUBLOX_ERR_CAPACITY,
-                     "integrate: max_points x segment bound above 2^32 slots");
-  }
   const Layout L = make_layout(caps);
   if (!d_ws || ws_bytes < L.total) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
   }
-  if ((q->d_export_op == nullptr) != (q->d_export_hit == nullptr)) {
-    return set_error(CUBLOX_ERR_INVALID, "integrate: export needs both arrays");
-  }
-  st = check_delta(q->d_delta_count, q->d_delta_idx, q->d_delta_val,
-                   q->delta_capacity, q->d_export_op != nullptr, "integrate");
-  if (st != CUBLOX_OK) return st;
-  st = check_input(*q, "integrate");
-  if (st != CUBLOX_OK) return st;
-  Batch b{};
-  b.one = make_desc(q->cfg, q->use_T ? q->T : nullptr, q->d_xs, q->d_ys, q->d_zs,
-                    q->n, q->d_occ,
-                    ApplyArgs{q->l_hit, q->l_miss, q->l_min, q->l_max, q->l_unknown},
-                    near_steps, q->d_export_op, q->d_export_hit);
-  set_input(b.one, *q);
-  set_delta(b.one, q->d_delta_count, q->d_delta_idx, q->d_delta_val,
-            q->delta_capacity);
   return launch_sequence(b, L, bind(L, d_ws), q->d_delta_count != nullptr,
                          as_stream(s));
+}
+
+int cublox_launch_integrate_noalloc(const cublox_raycast_cfg *cfg, const float *T,
+                                    const float *d_xs, const float *d_ys,
+                                    const float *d_zs, int n, float *d_occ,
+                                    float l_hit, float l_miss, float l_min,
+                                    float l_max, float l_unknown, int near_steps,
+                                    int *d_export_op, int *d_export_hit,
+                                    cublox_stream s) {
+  if (!cfg || n <= 0 || !d_xs || !d_ys || !d_zs || !d_occ) return CUBLOX_OK;
+  if (near_steps < 1 || near_steps > kMaxNearK) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_noalloc: near_steps out of [1, 12]");
+  }
+  cublox_integrate_grid q{};
+  q.cfg = *cfg;
+  if (T) {
+    for (int k = 0; k < 12; ++k) q.T[k] = T[k];
+    q.use_T = 1;
+  }
+  q.d_xs = d_xs;
+  q.d_ys = d_ys;
+  q.d_zs = d_zs;
+  q.n = n;
+  q.d_occ = d_occ;
+  q.l_hit = l_hit;
+  q.l_miss = l_miss;
+  q.l_min = l_min;
+  q.l_max = l_max;
+  q.l_unknown = l_unknown;
+  q.near_steps = near_steps;
+  q.d_export_op = d_export_op;
+  q.d_export_hit = d_export_hit;
+  Caps caps{};
+  Batch b{};
+  int st = prepare_grid(&q, n, near_steps, caps, b);
+  if (st != CUBLOX_OK) return st;
+  caps.max_slabs = kNoAllocSlabs;
+  const Layout L = make_layout(caps);
+  if (L.total > kNoAllocScratchBytes) {
+    return set_error(CUBLOX_ERR_CAPACITY,
+                     "integrate_noalloc: " + std::to_string(n) + " points on this grid need " +
+                         std::to_string(L.total) + " bytes of workspace, above the " +
+                         std::to_string(kNoAllocScratchBytes) + "-byte scratch");
+  }
+  void *ws = nullptr;
+  st = noalloc_scratch(&ws);
+  if (st != CUBLOX_OK) return st;
+  // The array keeps the last call's contents: the counters the kernels
+  // expect at zero are cleared first, with fill kernels (no memset node
+  // when the caller captures the stream). They lie at off_misc, off_nop,
+  // off_nhit (consecutive) and off_tcount.
+  char *base = static_cast<char *>(ws);
+  st = cublox_fill_zero_async(base + L.off_tcount, static_cast<size_t>(caps.max_tiles), s);
+  if (st != CUBLOX_OK) return st;
+  st = cublox_fill_zero_async(base + L.off_misc,
+                              (L.off_nhit + static_cast<size_t>(caps.max_near) * 4 -
+                               L.off_misc) / 4,
+                              s);
+  if (st != CUBLOX_OK) return st;
+  return launch_sequence(b, L, bind(L, ws), false, as_stream(s));
 }
 
 int cublox_integrate_segment_bound(const cublox_raycast_cfg *cfg) {

@@ -344,6 +344,29 @@ int cublox_launch_integrate(const cublox_raycast_cfg *cfg, const float *T,
                             int *d_export_op, int *d_export_hit,
                             cublox_stream s);
 
+/* Fused update that allocates nothing. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It is
+ * cublox_launch_integrate with the workspace arguments removed: the
+ * workspace is a library-owned device array of CUBLOX_NOALLOC_SCRATCH_BYTES
+ * per device, held from the library's load to the end of the process, with
+ * at most 64 hot-tile slabs (hotter tiles run unsplit; results are the
+ * same). CUBLOX_ERR_CAPACITY (enqueuing nothing) when the workspace for n
+ * points on this grid exceeds the array: 256 MiB holds a 65,536-point scan
+ * of a 701x701x151 grid at 0.1 m with a 30 m range. Every call on one
+ * device shares that device's array, so calls that overlap on one device
+ * (on different streams or threads) race; calls in one stream's order do
+ * not. The counters the kernels expect at zero are cleared by fill kernels
+ * at the start of each call.                                             */
+#define CUBLOX_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int cublox_launch_integrate_noalloc(const cublox_raycast_cfg *cfg, const float *T,
+                                    const float *d_xs, const float *d_ys,
+                                    const float *d_zs, int n, float *d_occ,
+                                    float l_hit, float l_miss, float l_min,
+                                    float l_max, float l_unknown, int near_steps,
+                                    int *d_export_op, int *d_export_hit,
+                                    cublox_stream s);
+
 /* Fused update with a changed-voxel stream (the reference's modified-voxel
  * compaction in applyUpdateKernel, kernels.cu:202-209, gated off at
  * kernels.cu:251-262, here enabled for the fused update): for every voxel
