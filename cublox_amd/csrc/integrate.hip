@@ -141,7 +141,7 @@ __device__ __forceinline__ int span_width(int t0, int m, int T) {
 struct Workspace {
   int *seg_count;     // n: segments written by each ray
   int *seg_tile;      // n * max_seg: tile of each slot's segment
-  Segment *segs;      // n * max_seg, per-ray slot ranges
+  Segment *segs;      // n * max_seg: record j of ray i at j * n + i
   unsigned *order;    // n * max_seg: record slots in tile order
   int *tile_count;    // n_tiles (zero between launches)
   int *tile_cursor;   // n_tiles
@@ -567,7 +567,7 @@ struct GridDesc {
   int *export_op, *export_hit;
   int transform;       // points are sensor-frame, world = T * p
   int ray_base;        // first ray; a multiple of kRayAlign
-  long long slot_base; // first segment slot; rays own g.max_seg slots each
+  long long slot_base; // first segment slot; record j of ray i at slot_base + j * n + i
   int tile_base;       // first tile
   int near_base;       // first near-field cell
   // Changed-voxel stream (optional, delta_count non-null): the (hash, new
@@ -771,7 +771,10 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
     const unsigned signs =
         static_cast<unsigned>((r.sx + 1) | ((r.sy + 1) << 2) | ((r.sz + 1) << 4))
         << kMetaSign;
-    const long long slot0 = d.slot_base + static_cast<long long>(i) * g.max_seg;
+    // Record j of ray i sits at slot slot_base + j * n + i: the j-th records
+    // of neighbouring rays, which mostly fall in one tile, share lines.
+    const long long slot0 = d.slot_base + i;
+    const long long sstride = g.n;
     Segment *__restrict__ mine = w.segs + slot0;
     int *__restrict__ mine_tile = w.seg_tile + slot0;
     int nseg = 0;
@@ -779,8 +782,8 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
       if (!lead) {
       } else if (nseg < g.max_seg) {
         const int key = d.tile_base + tile;
-        mine[nseg] = sg;
-        mine_tile[nseg] = key;
+        mine[nseg * sstride] = sg;
+        mine_tile[nseg * sstride] = key;
         if (kCount) {
           const int e = lds_hash_slot_n<kWalkHash>(s_tkey, key, true);
           if (e >= 0) {
@@ -902,19 +905,22 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
 // ----------------------------------------------------------------------
 
 // A block bins the records of `rays` consecutive rays (a divisor of
-// kRayAlign, so all of one grid), taken in flattened order -- an LDS prefix
-// of the rays' record counts maps record f to its ray and slot -- so
-// consecutive lanes read consecutive slots. Counts aggregate per tile in an
-// LDS hash: one global atomic per distinct tile of the block (records that
-// find no hash slot go to the global counters directly).
+// kRayAlign, so all of one grid). Record j of ray i lives at slot
+// slot_base + j * n + i, so the block walks its records j-major: iteration
+// x is record x / rays of ray x % rays (skipped when that ray has fewer
+// records), and consecutive lanes read consecutive slots. Counts aggregate
+// per tile in an LDS hash: one global atomic per distinct tile of the block
+// (records that find no hash slot go to the global counters directly). The
+// placing pass writes each record's slot in the same order, so a tile's
+// records from one block sit in the tile's run in slot order.
 template <bool kPlace>
 __global__ void __launch_bounds__(kBinThreads)
     bin_kernel(const Batch b, Workspace w, int rays) {
   __shared__ int s_key[kHash];
   __shared__ int s_cnt[kHash];
   __shared__ int s_base[kHash];
-  __shared__ int s_pre[kRayAlign + 1];
-  __shared__ int s_wave[kBinThreads / 64];
+  __shared__ unsigned short s_rc[kRayAlign];  // records per ray
+  __shared__ int s_max;
   const int r0 = blockIdx.x * rays;
   const int n_rays = batch_rays(b);
   if (r0 >= n_rays) return;
@@ -924,67 +930,55 @@ __global__ void __launch_bounds__(kBinThreads)
     s_key[e] = -1;
     s_cnt[e] = 0;
   }
-  constexpr int kPerT = kRayAlign / kBinThreads;
-  int c[kPerT];
-  int sum = 0;
-#pragma unroll
-  for (int j = 0; j < kPerT; ++j) {
-    const int r = t * kPerT + j;
-    c[j] = (r < rays && r0 + r < n_rays) ? w.seg_count[r0 + r] : 0;
-    sum += c[j];
-  }
-  const int incl = block_scan_incl(s_wave, sum);
-  int run = incl - sum;
-#pragma unroll
-  for (int j = 0; j < kPerT; ++j) {
-    s_pre[t * kPerT + j] = run;
-    run += c[j];
-  }
-  if (t == kBinThreads - 1) s_pre[kRayAlign] = incl;
+  if (t == 0) s_max = 0;
   __syncthreads();
-  const int total = s_pre[kRayAlign];
-  const long long base0 =
-      d.slot_base + static_cast<long long>(r0 - d.ray_base) * d.g.max_seg;
-  const int max_seg = d.g.max_seg;
-  // Slot of record f: its ray is the last with s_pre <= f.
-  auto slot_of = [&](int f) {
-    int lo = 0, hi = rays - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_pre[mid] <= f) {
-        lo = mid;
-      } else {
-        hi = mid - 1;
-      }
-    }
-    return base0 + static_cast<long long>(lo) * max_seg + (f - s_pre[lo]);
+  int mx = 0;
+  for (int r = t; r < rays; r += kBinThreads) {
+    const int c = r0 + r < n_rays ? w.seg_count[r0 + r] : 0;
+    s_rc[r] = static_cast<unsigned short>(c);
+    mx = max(mx, c);
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+  if ((t & 63) == 0 && mx > 0) atomicMax(&s_max, mx);
+  __syncthreads();
+  const int iters = s_max * rays;
+  const int i0 = r0 - d.ray_base;  // the block's first ray within its grid
+  const int stride = d.g.n;
+  const int lg_rays = 31 - __clz(rays);  // rays is a power of two
+  // Slot of iteration x relative to the grid's first slot (below the
+  // grid's n * max_seg <= INT_MAX slots), or -1.
+  auto rel_slot = [&](int x) -> int {
+    const int j = x >> lg_rays, r = x & (rays - 1);
+    if (j >= static_cast<int>(s_rc[r])) return -1;
+    return j * stride + (i0 + r);
   };
 
-  // The first kBinCache records of each thread keep their slot and hash
+  // The first kBinCache iterations of each thread keep their slot and hash
   // entry (or ~tile when the hash had no room) in registers for the
   // placing pass; the loop is unrolled so that they stay in registers and
   // their loads are in flight together.
-  int c_off[kBinCache], c_ent[kBinCache];
-  auto count = [&](int f, int i) {
-    const long long slot = slot_of(f);
-    const int tile = w.seg_tile[slot];
+  int c_off[kBinCache];
+  int c_ent[kBinCache];
+  auto count = [&](int rel, int i) {
+    const int tile = w.seg_tile[d.slot_base + rel];
     const int e = lds_hash_slot(s_key, tile, true);
     if (e >= 0) {
       atomicAdd(s_cnt + e, 1);
     } else if (!kPlace) {
       atomic_add_i32(w.tile_count + tile, 1);
     }
-    if (i >= 0) {
-      c_off[i] = static_cast<int>(slot - base0);
-      c_ent[i] = e >= 0 ? e : ~tile;
-    }
+    if (i >= 0) c_ent[i] = e >= 0 ? e : ~tile;
   };
 #pragma unroll
   for (int i = 0; i < kBinCache; ++i) {
-    const int f = t + i * kBinThreads;
-    if (f < total) count(f, kPlace ? i : -1);
+    const int x = t + i * kBinThreads;
+    c_off[i] = x < iters ? rel_slot(x) : -1;
+    if (c_off[i] >= 0) count(c_off[i], kPlace ? i : -1);
   }
-  for (int f = t + kBinCache * kBinThreads; f < total; f += kBinThreads) count(f, -1);
+  for (int x = t + kBinCache * kBinThreads; x < iters; x += kBinThreads) {
+    const int rel = rel_slot(x);
+    if (rel >= 0) count(rel, -1);
+  }
   __syncthreads();
   for (int e = t; e < kHash; e += kBinThreads) {
     const int key = s_key[e];
@@ -1005,14 +999,15 @@ __global__ void __launch_bounds__(kBinThreads)
   };
 #pragma unroll
   for (int i = 0; i < kBinCache; ++i) {
-    const int f = t + i * kBinThreads;
-    if (f < total) {
+    if (c_off[i] >= 0) {
       const int v = c_ent[i];
-      place(base0 + c_off[i], v >= 0 ? v : -1, ~v);
+      place(d.slot_base + c_off[i], v >= 0 ? v : -1, ~v);
     }
   }
-  for (int f = t + kBinCache * kBinThreads; f < total; f += kBinThreads) {
-    const long long slot = slot_of(f);
+  for (int x = t + kBinCache * kBinThreads; x < iters; x += kBinThreads) {
+    const int rel = rel_slot(x);
+    if (rel < 0) continue;
+    const long long slot = d.slot_base + rel;
     const int tile = w.seg_tile[slot];
     place(slot, lds_hash_slot(s_key, tile, false), tile);
   }
