@@ -129,6 +129,7 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_event_record", i, vp, vp)
     _sig(L, "cublox_event_synchronize", i, vp)
     _sig(L, "cublox_event_elapsed_ms", i, vp, vp, f32p)
+    _sig(L, "cublox_stream_wait_event", i, vp, vp)
     _sig(L, "cublox_fill_zero_async", i, vp, S, vp)
     _sig(L, "cublox_launch_ray_cast_update", i, C.POINTER(RaycastCfg), vp, vp,
          vp, i, vp, vp, vp)

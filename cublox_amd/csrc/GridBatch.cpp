@@ -183,6 +183,7 @@ void GridBatch::stage(
     const GridScan &s = scans[k];
     if (!s.grid) throw std::invalid_argument("cublox::GridBatch: null grid");
     OccupancyGrid &g = *s.grid;
+    g.settle();  // a plain recenter's clear still running on the grid's side stream
     const Eigen::Vector3f sensor =
         s.sensor_frame ? Eigen::Vector3f(s.T[3], s.T[7], s.T[11]) : s.origin;
     g.pinFirstOrigin(sensor);
@@ -224,6 +225,7 @@ void GridBatch::stage(
   for (const auto &[gp, pos] : recenters) {
     if (!gp) throw std::invalid_argument("cublox::GridBatch: null grid");
     OccupancyGrid &g = *gp;
+    g.settle();
     if (!g.config_.recenter_threshold.has_value() || g.d_occ_ == nullptr) continue;
     if ((pos - g.origin_f_).squaredNorm() < g.config_.recenter_threshold_squared) continue;
     cublox_recenter_item it{};

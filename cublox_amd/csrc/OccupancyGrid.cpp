@@ -81,6 +81,10 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   static_cast<Grid &>(*this) = static_cast<const Grid &>(o);
   first_run_ = o.first_run_;
   stream_ = std::exchange(o.stream_, nullptr);
+  zstream_ = std::exchange(o.zstream_, nullptr);
+  ev_main_ = std::exchange(o.ev_main_, nullptr);
+  ev_zero_ = std::exchange(o.ev_zero_, nullptr);
+  zero_pending_ = std::exchange(o.zero_pending_, false);
   d_op_cnt_ = std::exchange(o.d_op_cnt_, nullptr);
   d_hit_cnt_ = std::exchange(o.d_hit_cnt_, nullptr);
   d_occ_ = std::exchange(o.d_occ_, nullptr);
@@ -138,7 +142,15 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
 }
 
 void OccupancyGrid::releaseAll() noexcept {
+  if (zstream_) cublox_stream_synchronize(zstream_);
   if (stream_) cublox_stream_synchronize(stream_);
+  zero_pending_ = false;
+  for (void **e : {&ev_main_, &ev_zero_}) {
+    if (*e) cublox_event_destroy(*e);
+    *e = nullptr;
+  }
+  if (zstream_) cublox_stream_destroy(zstream_);
+  zstream_ = nullptr;
   freeDevice(d_fetch_state_);
   freeDevice(d_fetch_pos_);
   freeDevice(d_fetch_count_);
@@ -200,9 +212,32 @@ void OccupancyGrid::mark(int ev) const {
   if (timing_) CUBLOX_CHECK(cublox_event_record(events_[ev], stream_));
 }
 
-// Called after a stream sync: fold the completed event pairs into times_.
-void OccupancyGrid::harvestTimes() {
+void OccupancyGrid::join() const {
+  if (!zero_pending_) return;
+  CUBLOX_CHECK(cublox_stream_wait_event(stream_, ev_zero_));
+  zero_pending_ = false;
+}
+
+void OccupancyGrid::settle() const {
+  if (!zero_pending_) return;
+  CUBLOX_CHECK(cublox_stream_synchronize(zstream_));
+  zero_pending_ = false;
+}
+
+OccupancyGrid::KernelTimes OccupancyGrid::kernelTimes() const {
+  if (timing_) {
+    if (zstream_) CUBLOX_CHECK(cublox_stream_synchronize(zstream_));
+    sync();
+    harvestTimes();
+  }
+  return times_;
+}
+
+// Called once the events' work is done (after a stream sync; a recenter's
+// events are waited for): fold the completed event pairs into times_.
+void OccupancyGrid::harvestTimes() const {
   if (!timing_) return;
+  if (pending_[2]) CUBLOX_CHECK(cublox_event_synchronize(events_[5]));
   double *acc[3] = {&times_.update_ms, &times_.apply_ms, &times_.recenter_ms};
   long long *cnt[3] = {&times_.update_launches, &times_.apply_launches,
                        &times_.recenter_launches};
@@ -240,6 +275,7 @@ void OccupancyGrid::allocateVoxelBuffers() {
 }
 
 void OccupancyGrid::reset() {
+  join();
   if (d_occ_ == nullptr) return;
   const size_t n = static_cast<size_t>(config_.voxel_num);
   CUBLOX_CHECK(cublox_fill_zero_async(d_occ_, n, stream_));
@@ -250,6 +286,7 @@ void OccupancyGrid::reset() {
 }
 
 void OccupancyGrid::resetVoxel(const int hash_id) {
+  join();
   if (d_occ_ == nullptr || hash_id < 0 || hash_id >= config_.voxel_num) return;
   CUBLOX_CHECK(cublox_memset_async(d_occ_ + hash_id, 0, sizeof(float), stream_));
   if (d_op_cnt_) {
@@ -280,17 +317,32 @@ void OccupancyGrid::recenter(const Eigen::Vector3f &pos) {
   cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
   cfg.map_vox_num = config_.voxel_num;
 
+  // The plan is host arithmetic; only an action needs device work.
+  cublox_recenter_result plan{};
+  CUBLOX_CHECK(cublox_plan_recenter(&cfg, nullptr, nullptr, nullptr, &plan));
+  if (plan.action == 0) return;
+  if (!zstream_) {
+    CUBLOX_CHECK(cublox_stream_create(&zstream_));
+    CUBLOX_CHECK(cublox_event_create(&ev_main_));
+    CUBLOX_CHECK(cublox_event_create(&ev_zero_));
+  }
+  if (timing_ && pending_[2]) {  // the previous clear's times first
+    CUBLOX_CHECK(cublox_stream_synchronize(zstream_));
+    harvestTimes();
+  }
+  // The clear follows everything enqueued on the grid so far.
+  CUBLOX_CHECK(cublox_event_record(ev_main_, stream_));
+  CUBLOX_CHECK(cublox_stream_wait_event(zstream_, ev_main_));
   cublox_recenter_result result{};
-  mark(4);
-  CUBLOX_CHECK(cublox_launch_recenter(d_occ_, d_op_cnt_, d_hit_cnt_, &cfg, stream_,
+  if (timing_) CUBLOX_CHECK(cublox_event_record(events_[4], zstream_));
+  CUBLOX_CHECK(cublox_launch_recenter(d_occ_, d_op_cnt_, d_hit_cnt_, &cfg, zstream_,
                                       &result));
-  mark(5);
-  if (result.action == 0) return;
-  sync();
+  if (timing_) CUBLOX_CHECK(cublox_event_record(events_[5], zstream_));
+  CUBLOX_CHECK(cublox_event_record(ev_zero_, zstream_));
+  zero_pending_ = true;
   if (timing_) {
     pending_[2] = true;
     pending_zeroed_ += result.zeroed_voxels;
-    harvestTimes();
   }
   updateOriginAndBound(Eigen::Vector3i(result.new_origin_i.x,
                                        result.new_origin_i.y,
@@ -465,15 +517,19 @@ void OccupancyGrid::launchIntegrate(const float *d_xs, const float *d_ys,
   }
   q.point_stride = stride;
   q.range_drop_sq = drop_sq;
+  // A pending recenter clear: the walk overlaps it, the folds wait for it.
+  q.occ_ready = zero_pending_ ? ev_zero_ : nullptr;
   mark(0);
   CUBLOX_CHECK(cublox_launch_integrate_grid(&q, d_ws_, ws_bytes_, ws_points_, stream_));
   mark(1);
+  zero_pending_ = false;
   pending_[0] = timing_;
 }
 
 void OccupancyGrid::castRaysWorld(const float *d_x, const float *d_y,
                                   const float *d_z, int n,
                                   const Eigen::Vector3f &sensor_origin) {
+  join();
   ensureCounters();
   cublox_raycast_cfg cfg{};
   cfg.origin = f3(sensor_origin);
@@ -495,6 +551,7 @@ void OccupancyGrid::castRaysWorld(const float *d_x, const float *d_y,
 void OccupancyGrid::castRaysSensor(const float *d_xs, const float *d_ys,
                                    const float *d_zs, int n,
                                    const float T[12]) {
+  join();
   ensureCounters();
   cublox_raycast_cfg cfg{};
   cfg.origin = cublox_float3{T[3], T[7], T[11]};
@@ -515,6 +572,7 @@ void OccupancyGrid::castRaysSensor(const float *d_xs, const float *d_ys,
 }
 
 void OccupancyGrid::launchApply() {
+  join();
   ensureCounters();
   checkDeltaGate(UpdatePath::PER_VOXEL_ATOMICS, delta_capacity_, config_.voxel_num);
   const bool delta = delta_capacity_ > 0;
@@ -847,6 +905,7 @@ void OccupancyGrid::castRaysFusedSensorFrameDevice(const float *d_xs,
 }
 
 void OccupancyGrid::counterCensus(unsigned long long out[4]) const {
+  join();
   for (int k = 0; k < 4; ++k) out[k] = 0;
   if (!d_occ_ || !d_op_cnt_) return;
   unsigned long long *d = nullptr;
@@ -889,6 +948,7 @@ void OccupancyGrid::ensureFetchCapacity(const long long n) const {
 std::vector<OccupancyGrid::OccupancySample> OccupancyGrid::fetchOccupancyAround(
     const Eigen::Vector3f &center, const float radius,
     const std::optional<VoxelState> state_filter, const int max_results) const {
+  join();
   std::vector<OccupancySample> out;
   if (d_occ_ == nullptr || config_.voxel_num <= 0 || radius <= 0.0f) return out;
 
@@ -940,6 +1000,7 @@ size_t OccupancyGrid::buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
                                              const int subsample,
                                              const int max_points,
                                              std::vector<VizPoint> &out) const {
+  join();
   out.clear();
   if (d_occ_ == nullptr || config_.voxel_num <= 0) return 0;
   // main.cpp:484-485; fetchOccupancyAround returns nothing for r <= 0.
@@ -1015,6 +1076,7 @@ uint8_t *OccupancyGrid::pinnedStage(size_t bytes) const {
 }
 
 std::vector<uint8_t> OccupancyGrid::saveSnapshot() const {
+  join();
   cublox_snapshot_header h{};
   std::memcpy(h.magic, "CBLXSNAP", 8);
   h.version = CUBLOX_SNAPSHOT_VERSION;
@@ -1106,6 +1168,7 @@ std::vector<uint8_t> OccupancyGrid::saveSnapshot() const {
 }
 
 void OccupancyGrid::loadSnapshot(const uint8_t *data, size_t bytes) {
+  join();
   auto bad = [](const char *why) {
     throw std::invalid_argument(std::string("cublox::OccupancyGrid::loadSnapshot: ") + why);
   };
@@ -1213,6 +1276,7 @@ void OccupancyGrid::fetchOccupancyBatch(
     const std::optional<VoxelState> state_filter, const int per_query_cap,
     std::vector<unsigned int> &counts, std::vector<float> &positions,
     std::vector<unsigned char> &states) const {
+  join();
   const int nq = static_cast<int>(centers.size());
   counts.assign(static_cast<size_t>(nq), 0u);
   positions.clear();
@@ -1323,6 +1387,7 @@ void OccupancyGrid::fetchOccupancyBatch(
 }
 
 float OccupancyGrid::storedAt(const int hash_id) const {
+  join();
   if (d_occ_ == nullptr || hash_id < 0 || hash_id >= config_.voxel_num) return 0.0f;
   float v = 0.0f;
   CUBLOX_CHECK(cublox_memcpy_d2h_async(&v, d_occ_ + hash_id, sizeof(float), stream_));
@@ -1383,6 +1448,7 @@ bool OccupancyGrid::isFree(const int hash_id) const {
 }
 
 void OccupancyGrid::copyOccupancyToHost(float *dst) const {
+  join();
   if (!d_occ_) return;
   CUBLOX_CHECK(cublox_memcpy_d2h_async(
       dst, d_occ_, static_cast<size_t>(config_.voxel_num) * sizeof(float), stream_));
@@ -1390,6 +1456,7 @@ void OccupancyGrid::copyOccupancyToHost(float *dst) const {
 }
 
 void OccupancyGrid::copyCountersToHost(int *op, int *hit) const {
+  join();
   if (!d_occ_) return;
   const size_t bytes = static_cast<size_t>(config_.voxel_num) * sizeof(int);
   if (!d_op_cnt_) {
@@ -1403,6 +1470,7 @@ void OccupancyGrid::copyCountersToHost(int *op, int *hit) const {
 }
 
 void OccupancyGrid::loadOccupancyFromHost(const float *src) {
+  join();
   if (!d_occ_) return;
   CUBLOX_CHECK(cublox_memcpy_h2d_async(
       d_occ_, src, static_cast<size_t>(config_.voxel_num) * sizeof(float), stream_));

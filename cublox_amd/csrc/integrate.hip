@@ -1942,7 +1942,7 @@ int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
 // the kernels read the actual counts from the batch. delta selects the fold
 // variants that feed changed-voxel streams (grids without one skip it).
 int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
-                    bool delta, hipStream_t hs) {
+                    bool delta, hipStream_t hs, hipEvent_t occ_ready = nullptr) {
   const Caps &c = L.cap;
   const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
@@ -1979,6 +1979,8 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   bin_kernel<true><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
   st = check_launch("bin_kernel<place>");
   if (st != CUBLOX_OK) return st;
+  // Nothing above reads or writes occ: the folds wait for it here.
+  if (occ_ready) CUBLOX_TRY(hipStreamWaitEvent(hs, occ_ready, 0));
   return delta ? launch_folds<true>(b, L, w, hs) : launch_folds<false>(b, L, w, hs);
 }
 
@@ -2163,7 +2165,7 @@ int cublox_launch_integrate_grid(const cublox_integrate_grid *q, void *d_ws,
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
   }
   return launch_sequence(b, L, bind(L, d_ws), q->d_delta_count != nullptr,
-                         as_stream(s));
+                         as_stream(s), reinterpret_cast<hipEvent_t>(q->occ_ready));
 }
 
 int cublox_launch_integrate_noalloc(const cublox_raycast_cfg *cfg, const float *T,

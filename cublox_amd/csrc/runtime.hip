@@ -187,6 +187,12 @@ int cublox_event_elapsed_ms(cublox_event a, cublox_event b, float *ms) {
   return CUBLOX_OK;
 }
 
+int cublox_stream_wait_event(cublox_stream s, cublox_event e) {
+  if (!e) return CUBLOX_OK;
+  CUBLOX_TRY(hipStreamWaitEvent(as_stream(s), reinterpret_cast<hipEvent_t>(e), 0));
+  return CUBLOX_OK;
+}
+
 int cublox_graph_begin_capture(cublox_stream s) {
   CUBLOX_TRY(hipStreamBeginCapture(as_stream(s), hipStreamCaptureModeThreadLocal));
   return CUBLOX_OK;

@@ -58,7 +58,10 @@ public:
   void update(const PointCloud &input_cloud,
               const Eigen::Vector3f &sensor_origin);
 
-  // Sliding-window recenter: host slab plan, one device launch. Blocks.
+  // Sliding-window recenter: host slab plan, one device launch. The window
+  // origin and bounds change at once; the slab clear may still run when the
+  // call returns, and every later call on this grid is ordered after it
+  // (the next update's ray walk overlaps it).
   void recenter(const Eigen::Vector3f &pos);
 
   // Voxels within `radius` [m] of `center`, optionally of one state.
@@ -256,7 +259,7 @@ public:
     long long fetch_launches{0}, fetch_tested_voxels{0}, fetch_samples{0};
   };
   void enableKernelTiming(bool on);
-  KernelTimes kernelTimes() const { return times_; }
+  KernelTimes kernelTimes() const;
   void resetKernelTimes() { times_ = KernelTimes{}; }
 
   // Device views and host copies for integration and tests.
@@ -264,7 +267,13 @@ public:
   void copyOccupancyToHost(float *dst) const;
   void copyCountersToHost(int *op, int *hit) const;
   void loadOccupancyFromHost(const float *src);
-  void *stream() const { return stream_; }
+  // The grid's stream; work enqueued on it after this call is ordered after
+  // every call made on the grid so far (a pending recenter's clear
+  // included).
+  void *stream() const {
+    join();
+    return stream_;
+  }
   float getLogOddsFree() const { return l_free_; }
   float getLogOddsOccupied() const { return l_occupied_; }
 
@@ -306,16 +315,29 @@ private:
   void ensureWorkspace(int n);
   void sync() const;
   void mark(int ev) const;
-  void harvestTimes();
+  void harvestTimes() const;
 
   void *stream_{nullptr};
 
-  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter, [6,7] fetch.
+  // A recenter's slab clear runs on zstream_ after the work enqueued on
+  // stream_ before it (ev_main_), and returns without waiting; ev_zero_
+  // marks its end. The next fused update's folds wait for ev_zero_ (its ray
+  // walk overlaps the clear); every other call that reads or writes occ or
+  // the counters makes stream_ wait for it first (join).
+  void *zstream_{nullptr};
+  void *ev_main_{nullptr};
+  void *ev_zero_{nullptr};
+  mutable bool zero_pending_{false};
+  void join() const;
+  void settle() const;  // host wait for a pending clear (for other streams)
+
+  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter (on
+  // zstream_), [6,7] fetch.
   bool timing_{false};
   void *events_[8]{nullptr, nullptr, nullptr, nullptr,
                    nullptr, nullptr, nullptr, nullptr};
-  bool pending_[3]{false, false, false};
-  long long pending_zeroed_{0};
+  mutable bool pending_[3]{false, false, false};
+  mutable long long pending_zeroed_{0};
   mutable KernelTimes times_{};
 
   int *d_op_cnt_{nullptr};

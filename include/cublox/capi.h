@@ -78,6 +78,9 @@ int cublox_event_destroy(cublox_event e);
 int cublox_event_record(cublox_event e, cublox_stream s);
 int cublox_event_synchronize(cublox_event e);
 int cublox_event_elapsed_ms(cublox_event start, cublox_event stop, float *ms);
+/* Work enqueued on s after this call waits for the work before the last
+ * cublox_event_record of e (hipStreamWaitEvent). */
+int cublox_stream_wait_event(cublox_stream s, cublox_event e);
 /* Zero `count` floats/ints with a fill kernel (safe inside graph capture,
  * unlike a memset node). */
 int cublox_fill_zero_async(void *ptr, size_t count32, cublox_stream s);
@@ -487,6 +490,11 @@ typedef struct {
   /* > 0 (sensor-frame points only): points with x*x + y*y + z*z above it
    * are dropped before the transform (main.cpp:676-679).                  */
   float range_drop_sq;
+  /* Optional (single-grid launches): an event the kernels that write occ
+   * (or the count exports) wait for; the walk, binning and tile scan, which
+   * read neither, start before it completes. A recenter's slab clear on
+   * another stream then overlaps the next update's ray walk. NULL: none. */
+  cublox_event occ_ready;
 } cublox_integrate_grid;
 
 /* Single-grid fused update from a batch item (every option above; no
