@@ -155,25 +155,32 @@ def run_pcie(cb, w, scans, warmup, steps):
     return per
 
 
-TUNE_CANDIDATES = [(4, 512), (8, 512), (8, 1024), (12, 1024), (8, 2048)]
+# Autotuner candidates of the batched workload: (near steps, hot-tile chunk,
+# lanes per ray = 64 / lanes rays per wavefront, hot-tile mode 0 chunk slabs
+# / 1 shared slabs with global atomics).
+TUNE_CANDIDATES = [(8, 1024, 1, 0), (8, 1024, 4, 0), (8, 1024, 1, 1), (8, 1024, 4, 1),
+                   (8, 2048, 1, 0), (8, 2048, 1, 1), (12, 2048, 1, 1), (4, 512, 1, 0)]
 
 
 def run_batched(cb, tune_ticks_each, ticks):
-    """The batched workload: 64 grids of 401x401x101, per-grid beams,
-    azimuths, range and scene. The same scans run through the plain form
-    (each grid's blocking update + recenter in turn) and through one
-    GridBatch tick per step (graph replay, after online tuning)."""
-    from cublox_amd.workloads import batched_grid
+    """The batched workload: 64 grids of 401x401x101, each with its own
+    scene, trajectory and max range; every tick each grid gets a scan whose
+    beams x azimuths (8,192..131,072 points) are drawn for that tick. The
+    same scans run through the plain form (each grid's blocking update +
+    recenter in turn) and through one GridBatch tick per step (graph
+    replay, after online tuning)."""
+    from cublox_amd.workloads import batched_grid, batched_scan_shape
     w = cb.WORKLOADS["batched"]
     G = w.extra["grids"]
     tune_ticks = len(TUNE_CANDIDATES) * (tune_ticks_each + 1)
-    rc_ticks = 4  # after the timed ticks, without graph: recenter timed alone
+    rc_ticks = 8  # after the timed ticks, without graph: recenter timed alone
     total = tune_ticks + ticks + rc_ticks
     timed = range(tune_ticks, tune_ticks + ticks)
     draws = [batched_grid(w, g) for g in range(G)]
     t = time.perf_counter()
-    scans = [[d[3].scan(k, d[0], d[1], w.el_lo, w.el_hi, d[2]) for k in range(total)]
-             for d in draws]
+    shapes = [[batched_scan_shape(w, g, k)[:2] for k in range(total)] for g in range(G)]
+    scans = [[draws[g][3].scan(k, *shapes[g][k], w.el_lo, w.el_hi, draws[g][2])
+              for k in range(total)] for g in range(G)]
     bufs = [[[cb.DeviceBuffer.from_array(np.ascontiguousarray(c))
               for c in (s.xs, s.ys, s.zs)] for s in gs] for gs in scans]
     pts = [sum(scans[g][k].n for g in range(G)) for k in range(total)]
@@ -189,8 +196,15 @@ def run_batched(cb, tune_ticks_each, ticks):
             out.append(x)
         return out
 
-    # Plain form.
+    # Plain form. Each grid's workspace is sized for its largest scan first
+    # (allocation is set-up, not per-scan work).
     plain = grids()
+    for g in range(G):
+        kmax = max(range(total), key=lambda k: scans[g][k].n)
+        b = bufs[g][kmax]
+        plain[g].update_sensor_frame_device(b[0].ptr, b[1].ptr, b[2].ptr, scans[g][kmax].n,
+                                            scans[g][kmax].T)
+        plain[g].reset()
     per_scan, tick_ms = [], []
     for k in range(total):
         a = time.perf_counter()
@@ -232,6 +246,10 @@ def run_batched(cb, tune_ticks_each, ticks):
     batch_sps = G * len(btick) / (sum(btick) * 1e-3)
     # Batched recenter: 4 B per zeroed voxel over the recenter sequence's
     # device time (descriptor copy + one zeroing launch for all grids).
+    # Only ticks whose recenters moved some window count.
+    rc_ev = [(ms, v) for ms, v in zip(rc_ms, rc_vox) if v > 0]
+    rc_ms = [ms for ms, _ in rc_ev] or [0.0]
+    rc_vox = [v for _, v in rc_ev] or [0]
     rc_gbs = 4.0 * sum(rc_vox) / (sum(rc_ms) * 1e-3) / 1e9 if sum(rc_ms) > 0 else 0.0
     tune = batch.tune_results(len(TUNE_CANDIDATES))
     out = dict(
@@ -247,8 +265,12 @@ def run_batched(cb, tune_ticks_each, ticks):
                      tick_p95_ms=round(pct(btick, 95), 4),
                      tick_p99_ms=round(pct(btick, 99), 4),
                      device_tick_ms=round(statistics.mean(bdev), 4),
-                     graph=True, setting=dict(zip(("near_steps", "chunk"), batch.setting)),
-                     tuning_ms={f"{a}/{b}": round(m, 4) for (a, b), m in zip(TUNE_CANDIDATES, tune)}),
+                     graph=True,
+                     setting=dict(zip(("near_steps", "chunk", "lanes_per_ray", "hot_mode"),
+                                      batch.setting)),
+                     tuning_ms={"/".join(map(str, c)): round(m, 4)
+                                for c, m in zip(TUNE_CANDIDATES, tune)},
+                     tuning_keys="near_steps/chunk/lanes_per_ray/hot_mode"),
         speedup_scans_per_s=round(batch_sps / plain_sps, 2),
         recenter=dict(zeroed_voxels_per_tick=round(statistics.mean(zeroed)),
                       batched_recenter_ms=round(statistics.mean(rc_ms), 4),

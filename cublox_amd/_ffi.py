@@ -168,6 +168,7 @@ def hip() -> C.CDLL:
     _sig(L, "cublox_integrate_batch_workspace_bytes", S, vp)
     _sig(L, "cublox_integrate_batch_staging_bytes", S, vp)
     _sig(L, "cublox_integrate_batch_stage", i, vp, i, vp, i, vp)
+    _sig(L, "cublox_integrate_batch_stage_tuned", i, vp, i, vp, vp, vp)
     _sig(L, "cublox_launch_integrate_batch_staged", i, vp, vp, S, vp, vp)
     _sig(L, "cublox_launch_integrate_batch", i, vp, i, vp, vp, vp, S, vp)
     _sig(L, "cublox_integrate_batch_counters", i, vp, S, vp, vp, vp)
@@ -251,7 +252,7 @@ def host() -> C.CDLL:
     _sig(L, "cublox_gb_set_tuning", i, vp, i, i32p, i)
     _sig(L, "cublox_gb_tuning", i, vp, i32p)
     _sig(L, "cublox_gb_setting", i, vp, i32p)
-    _sig(L, "cublox_gb_set_setting", i, vp, i, i)
+    _sig(L, "cublox_gb_set_setting", i, vp, i, i, i, i)
     _sig(L, "cublox_gb_tune_results", i, vp, C.POINTER(C.c_double), i)
     _sig(L, "cublox_gb_last", i, vp, C.POINTER(C.c_double))
     _sig(L, "cublox_gb_counters", i, vp, C.POINTER(C.c_int))

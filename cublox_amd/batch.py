@@ -95,8 +95,15 @@ class GridBatch:
     def set_graph_mode(self, on: bool) -> None:
         self._check(self._L.cublox_gb_set_graph_mode(self._h, 1 if on else 0))
 
-    def set_tuning(self, candidates: Sequence[Tuple[int, int]], ticks_each: int) -> None:
-        arr = np.array([v for c in candidates for v in c] or [0], np.int32)
+    @staticmethod
+    def _setting4(c) -> Tuple[int, int, int, int]:
+        """(near_steps, chunk[, lanes per ray[, hot-tile mode]]), the last
+        two 0 (automatic lanes, chunk slabs) when left out."""
+        c = tuple(int(v) for v in c)
+        return (c + (0, 0))[:4] if len(c) >= 2 else (8, 1024, 0, 0)
+
+    def set_tuning(self, candidates: Sequence[Tuple[int, ...]], ticks_each: int) -> None:
+        arr = np.array([v for c in candidates for v in self._setting4(c)] or [0], np.int32)
         self._check(self._L.cublox_gb_set_tuning(self._h, len(candidates),
                                                  _ffi.iptr(arr), int(ticks_each)))
 
@@ -107,13 +114,16 @@ class GridBatch:
         return bool(v.value)
 
     @property
-    def setting(self) -> Tuple[int, int]:
-        out = np.zeros(2, np.int32)
+    def setting(self) -> Tuple[int, int, int, int]:
+        """(near_steps, chunk, lanes per ray, hot-tile mode)."""
+        out = np.zeros(4, np.int32)
         self._check(self._L.cublox_gb_setting(self._h, _ffi.iptr(out)))
-        return int(out[0]), int(out[1])
+        return tuple(int(v) for v in out)
 
-    def set_setting(self, near_steps: int, chunk: int) -> None:
-        self._check(self._L.cublox_gb_set_setting(self._h, int(near_steps), int(chunk)))
+    def set_setting(self, near_steps: int, chunk: int, lanes: int = 0,
+                    hot_mode: int = 0) -> None:
+        self._check(self._L.cublox_gb_set_setting(self._h, int(near_steps), int(chunk),
+                                                  int(lanes), int(hot_mode)))
 
     def tune_results(self, n: int) -> List[float]:
         out = (C.c_double * max(n, 1))()

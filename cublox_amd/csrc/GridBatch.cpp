@@ -134,9 +134,15 @@ void GridBatch::setGraphMode(bool on) {
   graph_mode_ = on;
 }
 
+static bool settingValid(const TuneSetting &s, const BatchLimits &lim) {
+  return s.near_steps >= 1 && s.near_steps <= lim.max_near_steps &&
+         s.chunk >= lim.min_chunk && s.chunk <= 65535 &&
+         (s.lanes == 0 || s.lanes == 1 || s.lanes == 4) &&
+         (s.hot_mode == 0 || s.hot_mode == 1);
+}
+
 void GridBatch::setSetting(const TuneSetting &s) {
-  if (s.near_steps < 1 || s.near_steps > lim_.max_near_steps ||
-      s.chunk < lim_.min_chunk || s.chunk > 65535) {
+  if (!settingValid(s, lim_)) {
     throw std::invalid_argument("cublox::GridBatch: setting outside the limits");
   }
   setting_ = s;
@@ -145,8 +151,7 @@ void GridBatch::setSetting(const TuneSetting &s) {
 void GridBatch::setTuning(const std::vector<TuneSetting> &candidates,
                           int ticks_each) {
   for (const auto &s : candidates) {
-    if (s.near_steps < 1 || s.near_steps > lim_.max_near_steps ||
-        s.chunk < lim_.min_chunk || s.chunk > 65535) {
+    if (!settingValid(s, lim_)) {
       throw std::invalid_argument("cublox::GridBatch: candidate outside the limits");
     }
   }
@@ -215,8 +220,12 @@ void GridBatch::stage(
     }
   }
   const cublox_batch_limits c = abiLimits(lim_);
-  CUBLOX_CHECK(cublox_integrate_batch_stage(items.data(), static_cast<int>(items.size()),
-                                            &c, setting_.chunk, h_stage_));
+  cublox_batch_tuning tune{};
+  tune.chunk = setting_.chunk;
+  tune.lanes_per_ray = setting_.lanes;
+  tune.hot_mode = setting_.hot_mode;
+  CUBLOX_CHECK(cublox_integrate_batch_stage_tuned(
+      items.data(), static_cast<int>(items.size()), &c, &tune, h_stage_));
 
   // OccupancyGrid.cpp:141-179 per grid: threshold pre-check on the host,
   // then the plan; origins move once the tick has run.

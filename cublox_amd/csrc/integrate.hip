@@ -96,8 +96,17 @@ struct Item {  // one tile_apply work unit: records [beg, end) of a tile
 };
 
 struct HotTile {  // a tile split into chunks, folded by hot_fold
-  int tile, slab0, chunks, pad;
+  int tile, slab0, chunks;
+  int atomic;  // the chunks add into one shared op/hit slab pair
 };
+
+// Slabs a hot tile of `chunks` chunks takes: one per chunk, or in the
+// shared mode one op and one hit slab (32-bit counts) that every chunk adds
+// its counts into with global atomics.
+constexpr int kSharedSlabs = 2;
+__host__ __device__ __forceinline__ int slab_need(int chunks, bool shared) {
+  return shared ? kSharedSlabs : chunks;
+}
 
 struct IntegrateArgs {
   RayArgs a;
@@ -587,7 +596,9 @@ struct BatchHeader {  // device-resident in batch mode
   int n_tiles;  // tiles of all grids
   int n_near;   // near-field cells of all grids
   int chunk;    // records per hot-tile work item
-  int pad[3];
+  int lanes;    // lanes per ray of the walk: 1 or 4
+  int shared;   // hot-tile chunks add into shared slabs (global atomics)
+  int pad;
 };
 
 struct Batch {
@@ -621,7 +632,17 @@ __device__ __forceinline__ int batch_near(const Batch &b) {
 __device__ __forceinline__ int batch_chunk(const Batch &b, const Workspace &w) {
   return b.descs ? batch_word(b, 4) : w.chunk;
 }
-static_assert(offsetof(BatchHeader, n_rays) == 4 && offsetof(BatchHeader, chunk) == 16,
+// Batches carry the walk's lanes per ray and the hot-tile mode in the
+// header (so a captured launch sequence follows them); a single-grid launch
+// runs only the walk it was launched with and the chunk slabs.
+__device__ __forceinline__ bool batch_walk_is(const Batch &b, int lanes) {
+  return !b.descs || batch_word(b, 5) == lanes;
+}
+__device__ __forceinline__ bool batch_shared(const Batch &b) {
+  return b.descs && batch_word(b, 6) != 0;
+}
+static_assert(offsetof(BatchHeader, n_rays) == 4 && offsetof(BatchHeader, chunk) == 16 &&
+                  offsetof(BatchHeader, lanes) == 20 && offsetof(BatchHeader, shared) == 24,
               "BatchHeader words");
 
 // Last grid whose base (the int field at byte offset `off`) is <= key.
@@ -716,6 +737,7 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
   constexpr bool kCount = kLanes == 4;
   __shared__ int s_tkey[kCount ? kWalkHash : 1];
   __shared__ int s_tcnt[kCount ? kWalkHash : 1];
+  if (!batch_walk_is(b, kLanes)) return;  // a batch tuned to the other walk
   const int ray0 = blockIdx.x * kWalkBlock;
   if (blockIdx.x == 0) {
     // The changed-voxel counts start from zero (the folds run later).
@@ -921,6 +943,7 @@ __global__ void __launch_bounds__(kBinThreads)
   __shared__ int s_base[kHash];
   __shared__ unsigned short s_rc[kRayAlign];  // records per ray
   __shared__ int s_max;
+  if (!kPlace && batch_walk_is(b, 4) && b.descs) return;  // the quad walk counted
   const int r0 = blockIdx.x * rays;
   const int n_rays = batch_rays(b);
   if (r0 >= n_rays) return;
@@ -1049,12 +1072,12 @@ __device__ __forceinline__ int chunks_of(int c, int chunk) {
 __device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
                                            long long run, int sl, int sp,
                                            int chunk, bool split, int bucket,
-                                           int slots, int *s_cursor) {
+                                           int slots, int *s_cursor, bool shared) {
   w.tile_cursor[k] = static_cast<int>(run);
   if (c <= 0) return;
   const int ch = split ? chunks_of(c, chunk) : 1;
   const int per_chunk = (c + ch - 1) / ch;
-  if (split) w.hot[sp] = HotTile{k, sl, ch, 0};
+  if (split) w.hot[sp] = HotTile{k, sl, ch, shared ? 1 : 0};
   const int pos0 = atomicAdd(s_cursor + bucket, slots);  // one claim per tile
   for (int q = 0; q < slots; ++q) {
     const int pos = pos0 + q;
@@ -1063,7 +1086,7 @@ __device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
       const int ib = static_cast<int>(run) + q * per_chunk;
       const int ie = min(ib + per_chunk, static_cast<int>(run) + c);
       const bool big = !split && c > chunk;  // hot tile without slabs
-      w.items[pos] = Item{k, ib, ie, split ? sl + q : (big ? -2 : -1)};
+      w.items[pos] = Item{k, ib, ie, split ? (shared ? sl : sl + q) : (big ? -2 : -1)};
       if (big) w.big[atomicAdd(w.misc + 4, 1)] = pos;
     } else {
       w.items[pos] = Item{-1, 0, 0, -1};
@@ -1080,6 +1103,7 @@ __global__ void __launch_bounds__(kScanThreads)
   __shared__ int s_count[kScanStage];
   const int n_tiles = batch_tiles(b);
   const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
   const int t = threadIdx.x;
   if (t < kCostBuckets) s_bucket[t] = 0;
   if (t == 0) w.misc[4] = 0;
@@ -1095,7 +1119,7 @@ __global__ void __launch_bounds__(kScanThreads)
   for (int k = beg; k < end; ++k) {
     const int c = s_count[k];
     seg += c;
-    if (c > chunk) slabs += chunks_of(c, chunk);
+    if (c > chunk) slabs += slab_need(chunks_of(c, chunk), shared);
   }
   const int seg_incl = block_scan_incl(s_buf, seg);
   const int slab_incl = block_scan_incl(s_buf, slabs);
@@ -1106,13 +1130,13 @@ __global__ void __launch_bounds__(kScanThreads)
     const int c = s_count[k];
     if (c <= 0) continue;
     const int ch = chunks_of(c, chunk);
-    if (ch > 1 && sl + ch <= w.slabs_cap) {
+    if (ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap) {
       atomicAdd(s_bucket + cost_bucket((c + ch - 1) / ch), ch);
       ++hot;
     } else {
       atomicAdd(s_bucket + cost_bucket(c), 1);
     }
-    if (ch > 1) sl += ch;
+    if (ch > 1) sl += slab_need(ch, shared);
   }
   const int hot_incl = block_scan_incl(s_buf, hot);
   if (t == 0) {
@@ -1130,12 +1154,12 @@ __global__ void __launch_bounds__(kScanThreads)
   for (int k = beg; k < end; ++k) {
     const int c = s_count[k];
     const int ch = chunks_of(c, chunk);
-    const bool split = ch > 1 && sl + ch <= w.slabs_cap;
+    const bool split = ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap;
     const int bucket = cost_bucket(split ? (c + ch - 1) / ch : c);
     place_tile(w, k, c, run, sl, sp, chunk, split, bucket, split ? ch : 1,
-               s_cursor);
+               s_cursor, shared);
     if (split) ++sp;
-    if (ch > 1) sl += ch;
+    if (ch > 1) sl += slab_need(ch, shared);
     run += c;
   }
   if (t == kScanThreads - 1) {
@@ -1152,6 +1176,7 @@ __global__ void __launch_bounds__(kScanThreads)
   __shared__ int s_sum[kScanSums];
   const int n_tiles = batch_tiles(b);
   const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
   const int k0 = blockIdx.x * kScanTilesPerBlock;
   if (threadIdx.x < kScanSums) s_sum[threadIdx.x] = 0;
   __syncthreads();
@@ -1162,7 +1187,7 @@ __global__ void __launch_bounds__(kScanThreads)
     seg += c;
     const int ch = chunks_of(c, chunk);
     if (c > 0) atomicAdd(s_sum + 3 + cost_bucket((c + ch - 1) / ch), ch);
-    if (ch > 1) slabs += ch;
+    if (ch > 1) slabs += slab_need(ch, shared);
   }
   atomicAdd(s_sum + 0, seg);
   atomicAdd(s_sum + 1, slabs);
@@ -1218,6 +1243,7 @@ __global__ void __launch_bounds__(kScanThreads)
   __shared__ int s_hot;
   const int n_tiles = batch_tiles(b);
   const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
   const int k0 = blockIdx.x * kScanTilesPerBlock;
   const int k1 = min(k0 + kScanTilesPerBlock, n_tiles);
   const int t = threadIdx.x;
@@ -1232,7 +1258,7 @@ __global__ void __launch_bounds__(kScanThreads)
     for (int k = beg; k < end; ++k) {
       const int c = w.tile_count[k];
       seg += c;
-      if (c > chunk) slabs += chunks_of(c, chunk);
+      if (c > chunk) slabs += slab_need(chunks_of(c, chunk), shared);
     }
   }
   const int seg_incl = block_scan_incl(s_buf, seg);
@@ -1242,8 +1268,8 @@ __global__ void __launch_bounds__(kScanThreads)
     const int c = w.tile_count[k];
     const int ch = chunks_of(c, chunk);
     if (ch > 1) {
-      if (sl + ch <= w.slabs_cap) ++hot;
-      sl += ch;
+      if (sl + slab_need(ch, shared) <= w.slabs_cap) ++hot;
+      sl += slab_need(ch, shared);
     }
   }
   const int hot_incl = block_scan_incl(s_buf, hot);
@@ -1255,11 +1281,11 @@ __global__ void __launch_bounds__(kScanThreads)
   for (int k = beg; k < end; ++k) {
     const int c = w.tile_count[k];
     const int ch = chunks_of(c, chunk);
-    const bool split = ch > 1 && sl + ch <= w.slabs_cap;
+    const bool split = ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap;
     place_tile(w, k, c, run, sl, sp, chunk, split,
-               cost_bucket((c + ch - 1) / ch), ch, s_cursor);
+               cost_bucket((c + ch - 1) / ch), ch, s_cursor, shared);
     if (split) ++sp;
-    if (ch > 1) sl += ch;
+    if (ch > 1) sl += slab_need(ch, shared);
     run += c;
     w.tile_count[k] = 0;
   }
@@ -1494,10 +1520,23 @@ __global__ void __launch_bounds__(kApplyThreads)
         min(static_cast<long long>(itm.end), w.seg_capacity));
     if (!big && itm.slab >= 0) {
       tile_accumulate(w, itm.beg, end, s_cnt4);
-      uint4 *slab = reinterpret_cast<uint4 *>(
-          w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
-      for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
-        slab[k] = s_cnt4[k];
+      if (batch_shared(b)) {
+        // Shared mode: the chunk's nonzero counts go into the tile's op and
+        // hit slabs (zeroed before this kernel) with global atomics.
+        unsigned *op_slab = w.scratch + static_cast<size_t>(itm.slab) * kTileVox;
+        unsigned *hit_slab = op_slab + kTileVox;
+        for (int k = threadIdx.x; k < kTileVox; k += kApplyThreads) {
+          const unsigned c = s_cnt[k];
+          if (c == 0) continue;
+          atomic_add_u32(op_slab + k, c & 0xFFFFu);
+          if (c >> 16) atomic_add_u32(hit_slab + k, c >> 16);
+        }
+      } else {
+        uint4 *slab = reinterpret_cast<uint4 *>(
+            w.scratch + static_cast<size_t>(itm.slab) * kTileVox);
+        for (int k = threadIdx.x; k < kTileVox / 4; k += kApplyThreads) {
+          slab[k] = s_cnt4[k];
+        }
       }
       __syncthreads();
       continue;
@@ -1540,6 +1579,23 @@ __global__ void __launch_bounds__(kApplyThreads)
   }
 }
 
+// Shared hot-tile mode: zeroes the op and hit slabs of this launch's split
+// tiles before the chunks add into them (a batch may have run the chunk-slab
+// mode before, which leaves counts in every slab it used).
+__global__ void __launch_bounds__(256) hot_zero_kernel(const Batch b, Workspace w) {
+  if (!batch_shared(b)) return;
+  const int n_hot = w.misc[3];
+  constexpr int kWords4 = kSharedSlabs * kTileVox / 4;  // uint4 per hot tile
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < static_cast<long long>(n_hot) * kWords4;
+       e += static_cast<long long>(gridDim.x) * 256) {
+    const int h = static_cast<int>(e / kWords4);
+    const int k = static_cast<int>(e % kWords4);
+    const HotTile ht = load_const(w.hot + h);
+    reinterpret_cast<uint4 *>(w.scratch + static_cast<size_t>(ht.slab0) * kTileVox)[k] =
+        make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 // Sums the scratch slabs of each split tile and folds: kHotSlices
 // workgroups per tile, four consecutive voxels (one z-run piece) per lane.
 template <bool kDelta>
@@ -1555,7 +1611,13 @@ __global__ void __launch_bounds__(256)
                                         static_cast<size_t>(ht.slab0) * kTileVox) +
         k4;
     unsigned op[4] = {0, 0, 0, 0}, hit[4] = {0, 0, 0, 0};
-    int q = 0;
+    int q = ht.atomic ? ht.chunks : 0;  // shared mode: no chunk slabs to sum
+    if (ht.atomic) {
+      const uint4 o4 = src[0];
+      const uint4 h4 = src[kTileVox / 4];
+      op[0] = o4.x; op[1] = o4.y; op[2] = o4.z; op[3] = o4.w;
+      hit[0] = h4.x; hit[1] = h4.y; hit[2] = h4.z; hit[3] = h4.w;
+    }
     for (; q + 4 <= ht.chunks; q += 4) {
       uint4 v[4];
 #pragma unroll
@@ -1920,6 +1982,11 @@ template <bool kDelta>
 int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
                  hipStream_t hs) {
   const Caps &c = L.cap;
+  if (b.descs) {  // batches may run the shared hot-tile mode (the header says)
+    hot_zero_kernel<<<dim3(2048), dim3(256), 0, hs>>>(b, w);
+    const int st0 = check_launch("hot_zero_kernel");
+    if (st0 != CUBLOX_OK) return st0;
+  }
   const int apply_blocks = std::min(L.max_items, 1 << 16);
   tile_apply_kernel<false, kDelta><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
   int st = check_launch("tile_apply_kernel");
@@ -1946,9 +2013,14 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   const Caps &c = L.cap;
   const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
-  if (c.max_rays <= kQuadWalkRays) {
+  // A single grid takes the walk its size calls for; a batch launches both
+  // and runs the one its header names (the autotuner's lanes per ray), so a
+  // captured sequence follows the setting of each tick.
+  const bool batch = b.descs != nullptr;
+  if (batch || c.max_rays <= kQuadWalkRays) {
     walk_kernel<4><<<dim3(walk_blocks), dim3(4 * kWalkBlock), near_lds, hs>>>(b, w);
-  } else {
+  }
+  if (batch || c.max_rays > kQuadWalkRays) {
     walk_kernel<1><<<dim3(walk_blocks), dim3(kWalkBlock), near_lds, hs>>>(b, w);
   }
   int st = check_launch("walk_kernel");
@@ -1957,7 +2029,7 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   // binning pass counts them.
   const int bin_rays = bin_rays_per_block(c);
   const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + bin_rays - 1) / bin_rays);
-  if (c.max_rays > kQuadWalkRays) {
+  if (batch || c.max_rays > kQuadWalkRays) {
     bin_kernel<false><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
     st = check_launch("bin_kernel<count>");
     if (st != CUBLOX_OK) return st;
@@ -2255,6 +2327,16 @@ size_t cublox_integrate_batch_staging_bytes(const cublox_batch_limits *lim) {
 int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
                                  int n_grids, const cublox_batch_limits *lim,
                                  int chunk_override, void *h_staging) {
+  cublox_batch_tuning t{};
+  t.chunk = chunk_override;
+  return cublox_integrate_batch_stage_tuned(grids, n_grids, lim, &t, h_staging);
+}
+
+int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n_grids,
+                                       const cublox_batch_limits *lim,
+                                       const cublox_batch_tuning *tuning,
+                                       void *h_staging) {
+  const int chunk_override = tuning ? tuning->chunk : 0;
   if (!lim || !h_staging || n_grids < 0 || (n_grids > 0 && !grids)) {
     return set_error(CUBLOX_ERR_INVALID, "integrate_batch: bad arguments");
   }
@@ -2324,7 +2406,16 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
     return set_error(CUBLOX_ERR_INVALID,
                      "integrate_batch: chunk below the limits' chunk or above 65535");
   }
-  sg->hdr = BatchHeader{n_grids, static_cast<int>(ray), tile, near, chunk, {0, 0, 0}};
+  int lanes = tuning ? tuning->lanes_per_ray : 0;
+  if (lanes == 0) lanes = c.max_rays <= kQuadWalkRays ? 4 : 1;
+  if (lanes != 1 && lanes != 4) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_batch: lanes per ray must be 1 or 4");
+  }
+  const int shared = tuning && tuning->hot_mode == 1 ? 1 : 0;
+  if (tuning && (tuning->hot_mode < 0 || tuning->hot_mode > 1)) {
+    return set_error(CUBLOX_ERR_INVALID, "integrate_batch: hot-tile mode must be 0 or 1");
+  }
+  sg->hdr = BatchHeader{n_grids, static_cast<int>(ray), tile, near, chunk, lanes, shared, 0};
   return CUBLOX_OK;
 }
 

@@ -477,7 +477,8 @@ int cublox_gb_set_tuning(cublox_gb *b, int n, const int *settings,
   return guarded([&] {
     std::vector<cublox::TuneSetting> c;
     for (int k = 0; k < n; ++k) {
-      c.push_back(cublox::TuneSetting{settings[2 * k], settings[2 * k + 1]});
+      const int *v = settings + 4 * k;
+      c.push_back(cublox::TuneSetting{v[0], v[1], v[2], v[3]});
     }
     GBP->setTuning(c, ticks_each);
   });
@@ -487,16 +488,21 @@ int cublox_gb_tuning(const cublox_gb *b, int *active) {
   return guarded([&] { *active = GBC->tuning() ? 1 : 0; });
 }
 
-int cublox_gb_setting(const cublox_gb *b, int out[2]) {
+int cublox_gb_setting(const cublox_gb *b, int out[4]) {
   return guarded([&] {
     const auto s = GBC->setting();
     out[0] = s.near_steps;
     out[1] = s.chunk;
+    out[2] = s.lanes;
+    out[3] = s.hot_mode;
   });
 }
 
-int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk) {
-  return guarded([&] { GBP->setSetting(cublox::TuneSetting{near_steps, chunk}); });
+int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk, int lanes,
+                          int hot_mode) {
+  return guarded([&] {
+    GBP->setSetting(cublox::TuneSetting{near_steps, chunk, lanes, hot_mode});
+  });
 }
 
 int cublox_gb_tune_results(const cublox_gb *b, double *out_ms, int cap) {

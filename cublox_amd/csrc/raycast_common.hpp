@@ -11,6 +11,10 @@ __device__ __forceinline__ void atomic_add_i32(int *p, int v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ void atomic_add_u32(unsigned *p, unsigned v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ unsigned atomic_add_u32_ret(unsigned *p,
                                                        unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED,

@@ -56,9 +56,21 @@ struct GridScan {
   float range_drop_sq{0.0f};
 };
 
+// One autotuner setting (none changes any result):
+//   near_steps  ray steps counted in the LDS cube around the sensor voxel
+//               before rays are cut into tile segments, 1..max_near_steps;
+//   chunk       records per work item of a hot (split) tile;
+//   lanes       lanes per ray of the walk, i.e. rays per wavefront: 1 (64
+//               rays) or 4 (16 rays); 0 picks by the batch capacity;
+//   hot_mode    deduplication of a hot tile's chunks: 0 a 32 KB slab of
+//               LDS counts per chunk, summed by the fold; 1 one shared
+//               op/hit slab pair per tile that the chunks add their
+//               nonzero counts to with global atomics.
 struct TuneSetting {
   int near_steps{8};
   int chunk{1024};
+  int lanes{0};
+  int hot_mode{0};
 };
 
 class GridBatch {

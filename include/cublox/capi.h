@@ -512,6 +512,27 @@ size_t cublox_integrate_batch_staging_bytes(const cublox_batch_limits *lim);
 int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
                                  int n_grids, const cublox_batch_limits *lim,
                                  int chunk, void *h_staging);
+/* The per-tick settings of a batch (none changes any result; they travel
+ * in the staged header, so a captured launch sequence follows them):
+ *   chunk          records per hot-tile work item (0: the limits' chunk);
+ *   lanes_per_ray  1 (64 rays per wavefront, a binning pass counts the
+ *                  records per tile) or 4 (16 rays per wavefront: each
+ *                  ray's quad shares the per-axis work of a span and counts
+ *                  its workgroup's records per tile in LDS); 0: 4 up to
+ *                  262,144 rays of capacity, else 1;
+ *   hot_mode       how the chunks of a hot tile combine their LDS counts:
+ *                  0 a slab per chunk, summed by the hot-tile fold; 1 one
+ *                  shared op/hit slab pair per tile, added to with global
+ *                  atomics for the nonzero counts only.                  */
+typedef struct {
+  int chunk;
+  int lanes_per_ray;
+  int hot_mode;
+} cublox_batch_tuning;
+int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids,
+                                       int n_grids, const cublox_batch_limits *lim,
+                                       const cublox_batch_tuning *tuning,
+                                       void *h_staging);
 /* Copies h_staging to the workspace and launches the update sequence. */
 int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
                                          size_t ws_bytes,

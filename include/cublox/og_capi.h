@@ -191,12 +191,14 @@ int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
                    int n_recenter, cublox_og *const *rgrids,
                    const float *rpos);
 int cublox_gb_set_graph_mode(cublox_gb *b, int on);
-/* settings: n pairs (near_steps, chunk); ticks_each measured ticks each. */
+/* settings: n quadruples (near_steps, chunk, lanes per ray 0/1/4, hot-tile
+ * mode 0/1) as cublox::TuneSetting; ticks_each measured ticks each. */
 int cublox_gb_set_tuning(cublox_gb *b, int n, const int *settings,
                          int ticks_each);
 int cublox_gb_tuning(const cublox_gb *b, int *active);
-int cublox_gb_setting(const cublox_gb *b, int out[2]);
-int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk);
+int cublox_gb_setting(const cublox_gb *b, int out[4]);
+int cublox_gb_set_setting(cublox_gb *b, int near_steps, int chunk, int lanes,
+                          int hot_mode);
 /* out_ms[k]: mean device ms of candidate k (cap entries). */
 int cublox_gb_tune_results(const cublox_gb *b, double *out_ms, int cap);
 /* out: update ms, recenter ms (0 in graph mode: all in update), zeroed. */

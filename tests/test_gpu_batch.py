@@ -41,7 +41,8 @@ def _setup(cb, orc):
     return grids, oracles, scenes
 
 
-def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=None):
+def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=None,
+         setting=None):
     grids, oracles, scenes = _setup(cb, orc)
     total = max_points or sum(b * a for (_, _, _, _, _, b, a, _, _) in GRIDS)
     kw = {} if max_tiles is None else {"max_tiles": max_tiles}
@@ -51,6 +52,8 @@ def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=No
         batch.set_graph_mode(True)
     if tuning:
         batch.set_tuning(tuning, 1)
+    if setting:
+        batch.set_setting(*setting)
     for k in range(ticks):
         scans, recenters, keep = [], [], []
         for i, (half, res, ctr, thr, rng, beams, az, speed, seed) in enumerate(GRIDS):
@@ -90,9 +93,23 @@ def test_batch_large_bin_blocks(cb, orc, gpu_ready):
     _run(cb, orc, 3, max_points=1 << 21)
 
 
-def test_batch_autotuner_keeps_results(cb, orc, gpu_ready):
-    cands = [(4, 256), (8, 512), (12, 1024), (8, 2048)]
-    batch = _run(cb, orc, 10, tuning=cands)
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_autotuner_keeps_results(cb, orc, gpu_ready, graph):
+    """The tuner switches near steps, chunk, lanes per ray and the hot-tile
+    mode between ticks (with graph replay: the captured sequence follows the
+    staged header); every tick stays bit-exact."""
+    cands = [(4, 256, 1, 0), (8, 512, 4, 1), (12, 1024, 1, 1), (8, 2048, 4, 0),
+             (8, 256, 0, 1)]
+    batch = _run(cb, orc, 12, graph=graph, tuning=cands)
     assert not batch.tuning
     assert batch.setting in cands
     assert all(ms > 0 for ms in batch.tune_results(len(cands)))
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_batch_shared_hot_tiles(cb, orc, gpu_ready, lanes):
+    """Hot tiles split into 256-record chunks that add into one shared slab
+    pair per tile with global atomics, under both walks."""
+    batch = _run(cb, orc, 4, setting=(8, 256, lanes, 1))
+    assert batch.counters()["split_tiles"] > 0
+
