@@ -92,8 +92,10 @@ GridBatch::~GridBatch() { release(); }
 
 void GridBatch::release() noexcept {
   if (stream_) cublox_stream_synchronize(stream_);
-  if (graph_) cublox_graph_destroy(graph_);
-  graph_ = nullptr;
+  for (auto &g : graphs_) {
+    if (g) cublox_graph_destroy(g);
+    g = nullptr;
+  }
   if (d_ws_) cublox_free(d_ws_);
   if (d_rarea_) cublox_free(d_rarea_);
   if (h_stage_) cublox_host_free(h_stage_);
@@ -126,10 +128,12 @@ int GridBatch::tileCount(const OccupancyGrid &g) {
 }
 
 void GridBatch::setGraphMode(bool on) {
-  if (!on && graph_) {
+  if (!on) {
     CUBLOX_CHECK(cublox_stream_synchronize(stream_));
-    CUBLOX_CHECK(cublox_graph_destroy(graph_));
-    graph_ = nullptr;
+    for (auto &g : graphs_) {
+      if (g) CUBLOX_CHECK(cublox_graph_destroy(g));
+      g = nullptr;
+    }
   }
   graph_mode_ = on;
 }
@@ -226,6 +230,7 @@ void GridBatch::stage(
   tune.hot_mode = setting_.hot_mode;
   CUBLOX_CHECK(cublox_integrate_batch_stage_tuned(
       items.data(), static_cast<int>(items.size()), &c, &tune, h_stage_));
+  lanes_ = tune.lanes_per_ray;
 
   // OccupancyGrid.cpp:141-179 per grid: threshold pre-check on the host,
   // then the plan; origins move once the tick has run.
@@ -279,6 +284,7 @@ void GridBatch::launch() {
   };
   CUBLOX_CHECK(cublox_event_record(ev_[0], stream_));
   if (graph_mode_) {
+    void *&graph_ = graphs_[(lanes_ == 4 ? 1 : 0) + 2 * (setting_.hot_mode ? 1 : 0)];
     if (!graph_) {
       CUBLOX_CHECK(cublox_graph_begin_capture(stream_));
       try {

@@ -1980,9 +1980,9 @@ int bin_rays_per_block(const Caps &c) {
 // The fold launches; kDelta: the variants that feed changed-voxel streams.
 template <bool kDelta>
 int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
-                 hipStream_t hs) {
+                 hipStream_t hs, bool shared) {
   const Caps &c = L.cap;
-  if (b.descs) {  // batches may run the shared hot-tile mode (the header says)
+  if (shared) {  // a batch in the shared hot-tile mode
     hot_zero_kernel<<<dim3(2048), dim3(256), 0, hs>>>(b, w);
     const int st0 = check_launch("hot_zero_kernel");
     if (st0 != CUBLOX_OK) return st0;
@@ -2009,18 +2009,18 @@ int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
 // the kernels read the actual counts from the batch. delta selects the fold
 // variants that feed changed-voxel streams (grids without one skip it).
 int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
-                    bool delta, hipStream_t hs, hipEvent_t occ_ready = nullptr) {
+                    bool delta, hipStream_t hs, hipEvent_t occ_ready = nullptr,
+                    int batch_lanes = 0, bool shared = false) {
   const Caps &c = L.cap;
   const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
-  // A single grid takes the walk its size calls for; a batch launches both
-  // and runs the one its header names (the autotuner's lanes per ray), so a
-  // captured sequence follows the setting of each tick.
-  const bool batch = b.descs != nullptr;
-  if (batch || c.max_rays <= kQuadWalkRays) {
+  // A single grid takes the walk its size calls for; a batch the one its
+  // staged header names (the autotuner's lanes per ray; a captured sequence
+  // holds one of them, so a graph is per lanes value).
+  const bool quad = batch_lanes ? batch_lanes == 4 : c.max_rays <= kQuadWalkRays;
+  if (quad) {
     walk_kernel<4><<<dim3(walk_blocks), dim3(4 * kWalkBlock), near_lds, hs>>>(b, w);
-  }
-  if (batch || c.max_rays > kQuadWalkRays) {
+  } else {
     walk_kernel<1><<<dim3(walk_blocks), dim3(kWalkBlock), near_lds, hs>>>(b, w);
   }
   int st = check_launch("walk_kernel");
@@ -2029,7 +2029,7 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   // binning pass counts them.
   const int bin_rays = bin_rays_per_block(c);
   const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + bin_rays - 1) / bin_rays);
-  if (batch || c.max_rays > kQuadWalkRays) {
+  if (!quad) {
     bin_kernel<false><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
     st = check_launch("bin_kernel<count>");
     if (st != CUBLOX_OK) return st;
@@ -2053,7 +2053,8 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   if (st != CUBLOX_OK) return st;
   // Nothing above reads or writes occ: the folds wait for it here.
   if (occ_ready) CUBLOX_TRY(hipStreamWaitEvent(hs, occ_ready, 0));
-  return delta ? launch_folds<true>(b, L, w, hs) : launch_folds<false>(b, L, w, hs);
+  return delta ? launch_folds<true>(b, L, w, hs, shared)
+               : launch_folds<false>(b, L, w, hs, shared);
 }
 
 Caps batch_caps(const cublox_batch_limits &lim) {
@@ -2334,8 +2335,7 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
 
 int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n_grids,
                                        const cublox_batch_limits *lim,
-                                       const cublox_batch_tuning *tuning,
-                                       void *h_staging) {
+                                       cublox_batch_tuning *tuning, void *h_staging) {
   const int chunk_override = tuning ? tuning->chunk : 0;
   if (!lim || !h_staging || n_grids < 0 || (n_grids > 0 && !grids)) {
     return set_error(CUBLOX_ERR_INVALID, "integrate_batch: bad arguments");
@@ -2416,6 +2416,7 @@ int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n
     return set_error(CUBLOX_ERR_INVALID, "integrate_batch: hot-tile mode must be 0 or 1");
   }
   sg->hdr = BatchHeader{n_grids, static_cast<int>(ray), tile, near, chunk, lanes, shared, 0};
+  if (tuning) tuning->lanes_per_ray = lanes;  // the walk the launch will take
   return CUBLOX_OK;
 }
 
@@ -2440,7 +2441,9 @@ int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
   Batch b{};
   b.descs = reinterpret_cast<const GridDesc *>(base + L.off_desc);
   b.hdr = reinterpret_cast<const BatchHeader *>(base + L.off_hdr);
-  return launch_sequence(b, L, bind(L, d_ws), lim->delta != 0, hs);
+  const BatchHeader &h = static_cast<const Staging *>(h_staging)->hdr;
+  return launch_sequence(b, L, bind(L, d_ws), lim->delta != 0, hs, nullptr,
+                         h.lanes == 4 ? 4 : 1, h.shared != 0);
 }
 
 int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,

@@ -141,7 +141,10 @@ private:
   void *h_rstage_{nullptr};     // recenter staging (pinned)
   void *d_rarea_{nullptr};      // recenter staging, device copy
   int max_regions_{0};
-  void *graph_{nullptr};
+  // Captured sequences, one per (walk: lanes per ray 1 or 4, hot-tile
+  // mode): the staged launch holds the kernels the tick's setting names.
+  void *graphs_[4]{nullptr, nullptr, nullptr, nullptr};
+  int lanes_{1};  // the walk of the staged tick
   bool graph_mode_{false};
   bool in_flight_{false};
   void *ev_[3]{nullptr, nullptr, nullptr};

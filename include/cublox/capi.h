@@ -523,7 +523,11 @@ int cublox_integrate_batch_stage(const cublox_integrate_grid *grids,
  *   hot_mode       how the chunks of a hot tile combine their LDS counts:
  *                  0 a slab per chunk, summed by the hot-tile fold; 1 one
  *                  shared op/hit slab pair per tile, added to with global
- *                  atomics for the nonzero counts only.                  */
+ *                  atomics for the nonzero counts only.
+ * The staged launch takes the walk and hot-tile kernels the header names,
+ * so a captured graph holds one (lanes, hot_mode) pair: the call sets
+ * tuning->lanes_per_ray to the lanes the launch will use (a caller keeps
+ * one graph per pair).                                                   */
 typedef struct {
   int chunk;
   int lanes_per_ray;
@@ -531,8 +535,7 @@ typedef struct {
 } cublox_batch_tuning;
 int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids,
                                        int n_grids, const cublox_batch_limits *lim,
-                                       const cublox_batch_tuning *tuning,
-                                       void *h_staging);
+                                       cublox_batch_tuning *tuning, void *h_staging);
 /* Copies h_staging to the workspace and launches the update sequence. */
 int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
                                          size_t ws_bytes,
