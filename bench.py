@@ -98,18 +98,21 @@ def run_gpu(cb, w, scans, bufs, n, warmup, steps, path=0, timing=True):
     g = new_grid(cb, w, path)
     g.enable_kernel_timing(timing)
 
+    moved = []
+
     def step(i):
         off = 4 * n * i
         s = scans[i]
         g.update_sensor_frame_device(bufs[0].ptr + off, bufs[1].ptr + off,
                                      bufs[2].ptr + off, n, s.T)
-        g.recenter(s.base)
+        moved.append(bool(g.recenter(s.base)))
 
     for i in range(warmup):
         step(i)
     g.reset_kernel_times()
     cb.synchronize()
     per = []
+    moved.clear()
     t0 = time.perf_counter()
     for i in range(warmup, warmup + steps):
         a = time.perf_counter()
@@ -118,7 +121,24 @@ def run_gpu(cb, w, scans, bufs, n, warmup, steps, path=0, timing=True):
     cb.synchronize()
     t1 = time.perf_counter()
     kt = g.kernel_times()
+    run_gpu.moved = moved
     return g, per, (t1 - t0) * 1e3, kt
+
+
+def recenter_cost(per, moved):
+    """Critical-path time a recenter adds: the mean of a recentering step
+    plus the step after it, less the same pair when no recenter happens
+    (the clear runs beside the next update's ray walk)."""
+    pairs_r = [per[k] + per[k + 1] for k in range(len(per) - 1) if moved[k]]
+    pairs_n = [per[k] + per[k + 1] for k in range(len(per) - 1) if not moved[k]]
+    if not pairs_r or not pairs_n:
+        return None
+    return dict(added_ms=round(statistics.mean(pairs_r) - statistics.mean(pairs_n), 5),
+                events=len(pairs_r),
+                step_ms_recenter=round(statistics.mean(per[k] for k in range(len(per))
+                                                        if moved[k]), 5),
+                step_ms_no_recenter=round(statistics.mean(per[k] for k in range(len(per))
+                                                           if not moved[k]), 5))
 
 
 def census(cb, w, scans, bufs, n, warmup, steps):
@@ -769,6 +789,8 @@ def main():
     # scans again with HIP events for the per-kernel device times.
     g0, per, wall_ms, _ = run_gpu(cb, w, scans, bufs, n, args.warmup, args.steps,
                                   timing=False)
+    rc_cost = recenter_cost(per, run_gpu.moved)
+    log(f"recenter critical path: {rc_cost}")
     g0.close()
     g, per_t, wall_t, kt = run_gpu(cb, w, scans, bufs, n, args.warmup, args.steps)
     log(f"with timing events: {wall_t / args.steps:.4f} ms/scan")
@@ -898,6 +920,7 @@ def main():
             "form": "plain per-scan, blocking drop-in calls, inputs HBM-resident",
         },
         "p50_ms": round(pct(per, 50), 5),
+        "recenter_critical_path": rc_cost,
         "p99_ms": round(pct(per, 99), 5),
         "scans_per_s": round(1e3 / ms_per_step, 2),
         "kernel_ms": {k: round(v / max(kt[k.replace('_ms', '_launches')], 1), 5)
