@@ -24,6 +24,9 @@ void statusCheck(int status, const char *expr) {
 // samples at 12 B position + 1 B state, plus the count word.
 constexpr std::size_t kFetchReserve = 500000ull * 13ull + 256ull;
 
+// Pinned upload stages (each one largest scan's columns).
+constexpr int kUploadStages = 8;
+
 cublox_raycast_cfg shapeCfg(const SessionSpec &s) {
   cublox_raycast_cfg c{};
   c.map_size_i = cublox_int3{2 * s.half_map_size_i.x() + 1, 2 * s.half_map_size_i.y() + 1,
@@ -66,14 +69,24 @@ MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
   batch_ = std::make_unique<GridBatch>(bl);
   batch_->setGraphMode(lim_.graph);
   try {
-    MS_CHECK(cublox_stream_create(&upload_stream_));
     MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_arena_), lim_.point_arena_bytes));
-    MS_CHECK(cublox_host_alloc(&h_upload_, 3 * sizeof(float) *
-                                               static_cast<std::size_t>(lim_.max_points_per_scan)));
+    const std::size_t stage_bytes =
+        3 * sizeof(float) * static_cast<std::size_t>(lim_.max_points_per_scan);
+    for (int k = 0; k < kUploadStages; ++k) {
+      UploadStage st;
+      stages_.push_back(st);
+      MS_CHECK(cublox_host_alloc(&stages_.back().host, stage_bytes));
+      MS_CHECK(cublox_stream_create(&stages_.back().stream));
+      free_stages_.push_back(k);
+    }
   } catch (...) {
     if (d_arena_) cublox_free(d_arena_);
     d_arena_ = nullptr;
-    if (upload_stream_) cublox_stream_destroy(upload_stream_);
+    for (auto &st : stages_) {
+      if (st.host) cublox_host_free(st.host);
+      if (st.stream) cublox_stream_destroy(st.stream);
+    }
+    stages_.clear();
     throw;
   }
   arena_free_[0] = lim_.point_arena_bytes;
@@ -97,8 +110,10 @@ MapServer::~MapServer() {
   batch_.reset();
   for (auto &a : farena_) growFetchArena(a, 0, 0);  // frees
   if (d_arena_) cublox_free(d_arena_);
-  if (h_upload_) cublox_host_free(h_upload_);
-  if (upload_stream_) cublox_stream_destroy(upload_stream_);
+  for (auto &st : stages_) {
+    if (st.host) cublox_host_free(st.host);
+    if (st.stream) cublox_stream_destroy(st.stream);
+  }
   sessions_.clear();  // frees the grids
 }
 
@@ -224,25 +239,35 @@ long long MapServer::submitScan(int sid, const float *xs, const float *ys, const
     r.arena_off = allocPoints(r.arena_bytes);
     r.d_pts = reinterpret_cast<float *>(d_arena_ + r.arena_off);
     try {
-      // The columns go through the pinned stage in pieces: each piece's DMA
-      // runs while the next is copied in (a pageable copy runs at a few
-      // GB/s).
-      std::lock_guard<std::mutex> lk(upload_mu_);
-      const std::size_t b = sizeof(float) * static_cast<std::size_t>(n);
-      const float *cols[3] = {xs, ys, zs};
-      auto *stage = static_cast<uint8_t *>(h_upload_);
-      auto *dst = reinterpret_cast<uint8_t *>(r.d_pts);
-      constexpr std::size_t kPiece = std::size_t{128} << 10;
-      for (int c = 0; c < 3; ++c) {
-        const auto *src = reinterpret_cast<const uint8_t *>(cols[c]);
-        for (std::size_t off = 0; off < b; off += kPiece) {
-          const std::size_t len = std::min(kPiece, b - off);
-          std::memcpy(stage + c * b + off, src + off, len);
-          MS_CHECK(cublox_memcpy_h2d_async(dst + c * b + off, stage + c * b + off, len,
-                                           upload_stream_));
-        }
+      // The three columns go into a free pinned stage (one host copy each)
+      // and on to the arena in one transfer; a pageable copy runs at a few
+      // GB/s. Stages let several client threads upload at once.
+      int k = -1;
+      {
+        std::unique_lock<std::mutex> lk(stage_mu_);
+        stage_cv_.wait(lk, [&] { return !free_stages_.empty(); });
+        k = free_stages_.back();
+        free_stages_.pop_back();
       }
-      MS_CHECK(cublox_stream_synchronize(upload_stream_));
+      struct Release {
+        MapServer *s;
+        int k;
+        ~Release() {
+          {
+            std::lock_guard<std::mutex> lk(s->stage_mu_);
+            s->free_stages_.push_back(k);
+          }
+          s->stage_cv_.notify_one();
+        }
+      } release{this, k};
+      const std::size_t b = sizeof(float) * static_cast<std::size_t>(n);
+      auto *stage = static_cast<uint8_t *>(stages_[static_cast<std::size_t>(k)].host);
+      std::memcpy(stage, xs, b);
+      std::memcpy(stage + b, ys, b);
+      std::memcpy(stage + 2 * b, zs, b);
+      void *us = stages_[static_cast<std::size_t>(k)].stream;
+      MS_CHECK(cublox_memcpy_h2d_async(r.d_pts, stage, 3 * b, us));
+      MS_CHECK(cublox_stream_synchronize(us));
     } catch (...) {
       releasePoints(r.arena_off, r.arena_bytes);
       throw;

@@ -588,11 +588,10 @@ void OccupancyGrid::launchApply() {
 }
 
 // Host bytes to device memory on the grid's stream. A pageable
-// hipMemcpyAsync of a scan's columns runs at a few GB/s; instead the bytes
-// are copied into a grid-owned pinned buffer piece by piece, and each piece
-// goes out by DMA while the next is copied. The buffer is reused by the
-// next call, so the caller syncs the stream before returning (every public
-// update blocks).
+// hipMemcpyAsync of a scan's columns runs at a few GB/s; instead the parts
+// are copied into a grid-owned pinned buffer, and each goes out by DMA
+// while the next is copied. The buffer is reused by the next call, so the
+// caller syncs the stream before returning (every public update blocks).
 void OccupancyGrid::uploadHost(void *d_dst, const void *const *h_srcs,
                               const size_t *lens, int parts) {
   size_t bytes = 0;
@@ -607,18 +606,15 @@ void OccupancyGrid::uploadHost(void *d_dst, const void *const *h_srcs,
     CUBLOX_CHECK(cublox_host_alloc(&h_upload_, cap));
     h_upload_bytes_ = cap;
   }
-  constexpr size_t kPiece = size_t{128} << 10;
+  // One transfer per part: each goes out while the next part is copied in
+  // (a transfer's launch costs several microseconds, so parts are not cut
+  // finer).
   auto *stage = static_cast<uint8_t *>(h_upload_);
   auto *dst = static_cast<uint8_t *>(d_dst);
   size_t base = 0;
   for (int k = 0; k < parts; ++k) {
-    const auto *src = static_cast<const uint8_t *>(h_srcs[k]);
-    for (size_t off = 0; off < lens[k]; off += kPiece) {
-      const size_t len = std::min(kPiece, lens[k] - off);
-      std::memcpy(stage + base + off, src + off, len);
-      CUBLOX_CHECK(cublox_memcpy_h2d_async(dst + base + off, stage + base + off, len,
-                                           stream_));
-    }
+    std::memcpy(stage + base, h_srcs[k], lens[k]);
+    CUBLOX_CHECK(cublox_memcpy_h2d_async(dst + base, stage + base, lens[k], stream_));
     base += lens[k];
   }
 }

@@ -218,9 +218,17 @@ private:
   std::mutex arena_mu_;
   std::condition_variable arena_cv_;
   bool arena_closed_{false};
-  void *upload_stream_{nullptr};
-  std::mutex upload_mu_;  // one upload at a time through the pinned stage
-  void *h_upload_{nullptr};  // pinned: one largest scan's three columns
+  // Upload stages: pinned buffers of one largest scan's three columns, each
+  // with its own stream; a submitting thread takes a free one, so uploads
+  // from several client threads run side by side.
+  struct UploadStage {
+    void *host{nullptr};
+    void *stream{nullptr};
+  };
+  std::vector<UploadStage> stages_;
+  std::vector<int> free_stages_;
+  std::mutex stage_mu_;
+  std::condition_variable stage_cv_;
 
   mutable std::mutex mu_;
   std::condition_variable cv_work_, cv_idle_;
