@@ -95,6 +95,22 @@ struct Item {  // one tile_apply work unit: records [beg, end) of a tile
   int slab;  // < 0: accumulate and fold; else a hot-tile chunk's slab
 };
 
+// A record's place in its tile, as the counting pass leaves it (the quad
+// walk, or the binning count after the one-lane walk; both count records
+// per tile in an LDS hash per workgroup, a "counting group" of rays): bit
+// 31 set, the place itself, taken from the tile's global count when the
+// hash had no room; else the record's hash entry e (bits 19..30) and its
+// rank among the group's records of that tile (bits 0..18: a ray's path
+// enters one tile once, and once more per wrap seam it crosses, and its
+// endpoint adds a record, so a ray has at most five records in a tile and
+// a group of at most kRayAlign rays fewer than 2^19). The
+// group's first place in the tile is grp_base[group * hash size + e].
+constexpr unsigned kRankGlobal = 1u << 31;
+constexpr int kRankBits = 19;
+__device__ __forceinline__ unsigned rank_code(int e, unsigned rank) {
+  return (static_cast<unsigned>(e) << kRankBits) | rank;
+}
+
 struct HotTile {  // a tile split into chunks, folded by hot_fold
   int tile, slab0, chunks;
   int atomic;  // the chunks add into one shared op/hit slab pair
@@ -164,6 +180,9 @@ struct Workspace {
   int *big;           // item slots of unsplit hot tiles (passes)
   int *block_sums;    // multi-block tile scan: per-block sums
   int *block_bases;   // multi-block tile scan: per-block bases
+  unsigned *seg_rank; // n * max_seg: each record's place in its tile (rank_code)
+  int *grp_base;      // per counting group and hash entry: the group's first
+                      // place in that entry's tile
   long long seg_capacity;
   int max_items;
   int slabs_cap;      // scratch slabs
@@ -799,6 +818,7 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
     const long long sstride = g.n;
     Segment *__restrict__ mine = w.segs + slot0;
     int *__restrict__ mine_tile = w.seg_tile + slot0;
+    unsigned *__restrict__ mine_rank = w.seg_rank + slot0;
     int nseg = 0;
     auto put = [&](const Segment &sg, int tile) {
       if (!lead) {
@@ -808,11 +828,10 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
         mine_tile[nseg * sstride] = key;
         if (kCount) {
           const int e = lds_hash_slot_n<kWalkHash>(s_tkey, key, true);
-          if (e >= 0) {
-            atomicAdd(s_tcnt + e, 1);
-          } else {
-            atomic_add_i32(w.tile_count + key, 1);
-          }
+          mine_rank[nseg * sstride] =
+              e >= 0 ? rank_code(e, atomicAdd(reinterpret_cast<unsigned *>(s_tcnt) + e, 1u))
+                     : kRankGlobal | atomic_add_u32_ret(
+                                         reinterpret_cast<unsigned *>(w.tile_count) + key, 1u);
         }
       } else {
         w.misc[2] = 1;
@@ -915,35 +934,39 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
     if (v) atomic_add_i32(w.near_op + d.near_base + c, v);
   }
   if (kCount) {
+    // The group's first place in each of its tiles.
+    int *base = w.grp_base + static_cast<size_t>(blockIdx.x) * kWalkHash;
     for (int e = threadIdx.x; e < kWalkHash; e += blockDim.x) {
       const int key = s_tkey[e];
-      if (key >= 0) atomic_add_i32(w.tile_count + key, s_tcnt[e]);
+      if (key >= 0) {
+        base[e] = static_cast<int>(atomic_add_u32_ret(
+            reinterpret_cast<unsigned *>(w.tile_count) + key, static_cast<unsigned>(s_tcnt[e])));
+      }
     }
   }
 }
 
 // ----------------------------------------------------------------------
-// binning: LDS hash aggregation of segments by tile
+// binning: counts per tile after the one-lane walk, and the placing pass
 // ----------------------------------------------------------------------
 
-// A block bins the records of `rays` consecutive rays (a divisor of
-// kRayAlign, so all of one grid). Record j of ray i lives at slot
-// slot_base + j * n + i, so the block walks its records j-major: iteration
-// x is record x / rays of ray x % rays (skipped when that ray has fewer
-// records), and consecutive lanes read consecutive slots. Counts aggregate
-// per tile in an LDS hash: one global atomic per distinct tile of the block
-// (records that find no hash slot go to the global counters directly). The
-// placing pass writes each record's slot in the same order, so a tile's
-// records from one block sit in the tile's run in slot order.
-template <bool kPlace>
+// A block counts the records of `rays` consecutive rays (a divisor of
+// kRayAlign, so all of one grid; the block is the records' counting group).
+// Record j of ray i lives at slot slot_base + j * n + i, so the block walks
+// its records j-major: iteration x is record x / rays of ray x % rays
+// (skipped when that ray has fewer records), and consecutive lanes read
+// consecutive slots. Counts aggregate per tile in an LDS hash: one global
+// atomic per distinct tile of the block (records that find no hash slot
+// take their place from the global count directly). Each record's rank
+// among the block's records of its tile goes to seg_rank, the block's base
+// in each of its tiles to grp_base.
 __global__ void __launch_bounds__(kBinThreads)
     bin_kernel(const Batch b, Workspace w, int rays) {
   __shared__ int s_key[kHash];
-  __shared__ int s_cnt[kHash];
-  __shared__ int s_base[kHash];
+  __shared__ unsigned s_cnt[kHash];
   __shared__ unsigned short s_rc[kRayAlign];  // records per ray
   __shared__ int s_max;
-  if (!kPlace && batch_walk_is(b, 4) && b.descs) return;  // the quad walk counted
+  if (batch_walk_is(b, 4) && b.descs) return;  // the quad walk counted
   const int r0 = blockIdx.x * rays;
   const int n_rays = batch_rays(b);
   if (r0 >= n_rays) return;
@@ -975,66 +998,81 @@ __global__ void __launch_bounds__(kBinThreads)
     if (j >= static_cast<int>(s_rc[r])) return -1;
     return j * stride + (i0 + r);
   };
-
-  // The first kBinCache iterations of each thread keep their slot and hash
-  // entry (or ~tile when the hash had no room) in registers for the
-  // placing pass; the loop is unrolled so that they stay in registers and
+  auto count = [&](int rel) {
+    const size_t slot = d.slot_base + rel;
+    const int tile = w.seg_tile[slot];
+    const int e = lds_hash_slot(s_key, tile, true);
+    w.seg_rank[slot] =
+        e >= 0 ? rank_code(e, atomicAdd(s_cnt + e, 1u))
+               : kRankGlobal | atomic_add_u32_ret(
+                                   reinterpret_cast<unsigned *>(w.tile_count) + tile, 1u);
+  };
+  // The first kBinCache iterations of each thread are unrolled so that
   // their loads are in flight together.
   int c_off[kBinCache];
-  int c_ent[kBinCache];
-  auto count = [&](int rel, int i) {
-    const int tile = w.seg_tile[d.slot_base + rel];
-    const int e = lds_hash_slot(s_key, tile, true);
-    if (e >= 0) {
-      atomicAdd(s_cnt + e, 1);
-    } else if (!kPlace) {
-      atomic_add_i32(w.tile_count + tile, 1);
-    }
-    if (i >= 0) c_ent[i] = e >= 0 ? e : ~tile;
-  };
 #pragma unroll
   for (int i = 0; i < kBinCache; ++i) {
     const int x = t + i * kBinThreads;
     c_off[i] = x < iters ? rel_slot(x) : -1;
-    if (c_off[i] >= 0) count(c_off[i], kPlace ? i : -1);
+    if (c_off[i] >= 0) count(c_off[i]);
   }
   for (int x = t + kBinCache * kBinThreads; x < iters; x += kBinThreads) {
     const int rel = rel_slot(x);
-    if (rel >= 0) count(rel, -1);
+    if (rel >= 0) count(rel);
   }
   __syncthreads();
+  int *base = w.grp_base + static_cast<size_t>(blockIdx.x) * kHash;
   for (int e = t; e < kHash; e += kBinThreads) {
     const int key = s_key[e];
     if (key < 0) continue;
-    if (!kPlace) {
-      atomic_add_i32(w.tile_count + key, s_cnt[e]);
-    } else {
-      s_base[e] = atomicAdd(w.tile_cursor + key, s_cnt[e]);
-      s_cnt[e] = 0;
-    }
-  }
-  if (!kPlace) return;
-  __syncthreads();
-  auto place = [&](long long slot, int e, int tile) {
-    const long long pos = e >= 0 ? s_base[e] + atomicAdd(s_cnt + e, 1)
-                                 : atomicAdd(w.tile_cursor + tile, 1);
-    if (pos < w.seg_capacity) w.order[pos] = static_cast<unsigned>(slot);
-  };
-#pragma unroll
-  for (int i = 0; i < kBinCache; ++i) {
-    if (c_off[i] >= 0) {
-      const int v = c_ent[i];
-      place(d.slot_base + c_off[i], v >= 0 ? v : -1, ~v);
-    }
-  }
-  for (int x = t + kBinCache * kBinThreads; x < iters; x += kBinThreads) {
-    const int rel = rel_slot(x);
-    if (rel < 0) continue;
-    const long long slot = d.slot_base + rel;
-    const int tile = w.seg_tile[slot];
-    place(slot, lds_hash_slot(s_key, tile, false), tile);
+    base[e] = static_cast<int>(
+        atomic_add_u32_ret(reinterpret_cast<unsigned *>(w.tile_count) + key, s_cnt[e]));
   }
 }
+
+// Placing pass (after tile_scan has set each tile's run start in
+// tile_cursor): record j of every ray of this block's kWalkBlock rays, for
+// the kPlaceJ values of j this block's y index names, goes to its place in
+// tile order. The record's counting group is its ray's group of 2^lg_group
+// rays, whose hash has `hash` entries. Every record is independent, so the
+// pass is a gather and a scatter with no atomics.
+constexpr int kPlaceJ = 4;
+__global__ void __launch_bounds__(kWalkBlock)
+    place_kernel(const Batch b, Workspace w, int lg_group, int hash) {
+  const int ray = static_cast<int>(blockIdx.x) * kWalkBlock + static_cast<int>(threadIdx.x);
+  const int ray0 = static_cast<int>(blockIdx.x) * kWalkBlock;
+  const int n_rays = batch_rays(b);
+  if (ray0 >= n_rays) return;
+  const int cnt = ray < n_rays ? w.seg_count[ray] : 0;
+  const int j0 = static_cast<int>(blockIdx.y) * kPlaceJ;
+  if (j0 >= cnt) return;
+  const GridDesc d = grid_of_ray(b, ray0);
+  const long long slot0 = d.slot_base + (ray - d.ray_base);
+  const long long stride = d.g.n;
+  const int *gbase = w.grp_base + static_cast<size_t>(ray >> lg_group) * hash;
+  int tile[kPlaceJ];
+  unsigned rk[kPlaceJ];
+#pragma unroll
+  for (int q = 0; q < kPlaceJ; ++q) {
+    if (j0 + q < cnt) {
+      tile[q] = w.seg_tile[slot0 + (j0 + q) * stride];
+      rk[q] = w.seg_rank[slot0 + (j0 + q) * stride];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPlaceJ; ++q) {
+    if (j0 + q >= cnt) break;
+    long long pos = w.tile_cursor[tile[q]];
+    pos += (rk[q] & kRankGlobal)
+               ? static_cast<long long>(rk[q] & ~kRankGlobal)
+               : static_cast<long long>(gbase[rk[q] >> kRankBits]) +
+                     static_cast<long long>(rk[q] & ((1u << kRankBits) - 1u));
+    if (pos < w.seg_capacity) {
+      w.order[pos] = static_cast<unsigned>(slot0 + (j0 + q) * stride);
+    }
+  }
+}
+
 
 // ----------------------------------------------------------------------
 // tile_scan: cursors, work items, hot-tile splits
@@ -1774,6 +1812,7 @@ struct Caps {
   int max_near;            // near cells
   int chunk;               // smallest records per hot-tile chunk
   int max_slabs;           // scratch slab cap
+  int max_seg;             // record slots per ray
 };
 
 struct Layout {
@@ -1782,8 +1821,20 @@ struct Layout {
   int max_items, scan_blocks;
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
-      off_bbase, off_hdr, off_desc, total;
+      off_bbase, off_srank, off_gbase, off_hdr, off_desc, total;
 };
+
+int bin_rays_per_block(const Caps &c);
+
+// Entries of the counting groups' base table: the quad walk's groups
+// (kWalkBlock rays, kWalkHash entries) or the binning count's (bin blocks,
+// kHash entries), whichever a launch uses.
+long long group_table_entries(const Caps &c) {
+  const long long walk = (c.max_rays + kWalkBlock - 1) / kWalkBlock * kWalkHash;
+  const int br = bin_rays_per_block(c);
+  const long long bin = (c.max_rays + br - 1) / br * kHash;
+  return std::max(walk, bin);
+}
 
 constexpr int kMinChunk = 256;
 
@@ -1822,6 +1873,8 @@ Layout make_layout(const Caps &c) {
   take(L.off_nhit, static_cast<size_t>(c.max_near) * 4);
   take(L.off_bsum, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
   take(L.off_bbase, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
+  take(L.off_srank, static_cast<size_t>(c.seg_capacity) * 4);
+  take(L.off_gbase, static_cast<size_t>(group_table_entries(c)) * 4);
   // Header and descriptors, laid out as the host staging buffer so one
   // copy moves both.
   take(L.off_hdr, staging_bytes(c.max_grids));
@@ -1848,6 +1901,8 @@ Workspace bind(const Layout &L, void *base) {
   w.near_hit = reinterpret_cast<int *>(b + L.off_nhit);
   w.block_sums = reinterpret_cast<int *>(b + L.off_bsum);
   w.block_bases = reinterpret_cast<int *>(b + L.off_bbase);
+  w.seg_rank = reinterpret_cast<unsigned *>(b + L.off_srank);
+  w.grp_base = reinterpret_cast<int *>(b + L.off_gbase);
   w.seg_capacity = L.cap.seg_capacity;
   w.max_items = L.max_items;
   w.slabs_cap = static_cast<int>(L.slabs);
@@ -1867,6 +1922,7 @@ Caps single_caps(const cublox_raycast_cfg &cfg, int max_points, int near_k) {
   c.max_near = G.cube * G.cube * G.cube;
   c.chunk = kChunk;
   c.max_slabs = 4096;
+  c.max_seg = G.max_seg;
   return c;
 }
 
@@ -2030,8 +2086,8 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   const int bin_rays = bin_rays_per_block(c);
   const unsigned bin_blocks = static_cast<unsigned>((c.max_rays + bin_rays - 1) / bin_rays);
   if (!quad) {
-    bin_kernel<false><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
-    st = check_launch("bin_kernel<count>");
+    bin_kernel<<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
+    st = check_launch("bin_kernel");
     if (st != CUBLOX_OK) return st;
   }
   if (c.max_tiles <= kScanStage) {
@@ -2048,8 +2104,13 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     st = check_launch("tile_place_kernel");
   }
   if (st != CUBLOX_OK) return st;
-  bin_kernel<true><<<dim3(bin_blocks), dim3(kBinThreads), 0, hs>>>(b, w, bin_rays);
-  st = check_launch("bin_kernel<place>");
+  {
+    // Counting groups: the quad walk's workgroups, else the bin blocks.
+    const int lg_group = 31 - __builtin_clz(static_cast<unsigned>(quad ? kWalkBlock : bin_rays));
+    const dim3 grid(walk_blocks, static_cast<unsigned>((c.max_seg + kPlaceJ - 1) / kPlaceJ));
+    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash);
+    st = check_launch("place_kernel");
+  }
   if (st != CUBLOX_OK) return st;
   // Nothing above reads or writes occ: the folds wait for it here.
   if (occ_ready) CUBLOX_TRY(hipStreamWaitEvent(hs, occ_ready, 0));
@@ -2069,6 +2130,7 @@ Caps batch_caps(const cublox_batch_limits &lim) {
   c.max_near = c.max_grids * c.max_cube * c.max_cube * c.max_cube;
   c.chunk = std::max(kMinChunk, std::min(lim.chunk > 0 ? lim.chunk : kChunk, 8192));
   c.max_slabs = lim.max_slabs > 0 ? lim.max_slabs : 65536;
+  c.max_seg = std::max(lim.max_segments_per_ray, 1);
   return c;
 }
 
