@@ -68,7 +68,7 @@ constexpr int kChunk = 1024;      // records per chunk of a hot tile (default)
 constexpr int kApplyThreads = 256;
 constexpr int kPerThread = kTileVox / kApplyThreads;  // 32 voxels
 constexpr int kHotSlices = 8;     // hot_fold workgroups per split tile
-constexpr int kCostBuckets = 18;  // log2 classes of item cost
+constexpr int kCostBuckets = 4;   // classes of item cost (records per item)
 
 // A segment is the part of one ray inside one tile span: its DDA state at
 // the span entry, the increments, and in `meta` the entry voxel's index in
@@ -183,6 +183,7 @@ struct Workspace {
   unsigned *seg_rank; // n * max_seg: each record's place in its tile (rank_code)
   int *grp_base;      // per counting group and hash entry: the group's first
                       // place in that entry's tile
+  int *scan_state;    // single-workgroup tile scan: each thread's bases
   long long seg_capacity;
   int max_items;
   int slabs_cap;      // scratch slabs
@@ -1030,17 +1031,463 @@ __global__ void __launch_bounds__(kBinThreads)
   }
 }
 
+// ----------------------------------------------------------------------
+// tile_scan: cursors, work items, hot-tile splits
+// ----------------------------------------------------------------------
+// A tile with more than `chunk` records is split into chunks of about
+// `chunk` records, each its own work item with its own scratch slab, while
+// the split tiles' slabs (taken in tile order) fit the scratch area; a hot
+// tile beyond that is one item, accumulated in 16-bit passes. Items are
+// placed in decreasing cost classes (records per item) so the heaviest
+// start first.
+constexpr int kScanThreads = 1024;
+constexpr int kScanWaves = kScanThreads / 64;
+constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
+constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
+constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
+constexpr int kScanState = 8;  // bases a tile_scan thread leaves: run, slab, split, classes
+constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
+constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
+
+
+// Cost class of a work item (its record count), larger classes first:
+// above 512 records, above 128, above 32, the rest. Items are placed class
+// by class so the heaviest start first; within a class in any order.
+__device__ __forceinline__ int cost_bucket(int records) {
+  return records > 512 ? 0 : (records > 128 ? 1 : (records > 32 ? 2 : 3));
+}
+
+// Number of chunks a tile with c records is split into (1: not split).
+__device__ __forceinline__ int chunks_of(int c, int chunk) {
+  return c > chunk ? (c + chunk - 1) / chunk : 1;
+}
+
+// Split decision of a tile with c records whose slab range would start at
+// slab sl: its chunks, whether it splits (its slabs fit the cap), the item
+// slots it takes, its cost class, and the slabs it consumes in the slab
+// prefix (every hot tile does, split or not). Only hot tiles divide.
+struct TileSplit {
+  int ch, slots, cls, need;
+  bool split;
+};
+__device__ __forceinline__ TileSplit tile_split(int c, int chunk, int sl, int cap,
+                                                bool shared) {
+  TileSplit r{1, 1, cost_bucket(c), 0, false};
+  if (c > chunk) {
+    r.ch = (c + chunk - 1) / chunk;
+    r.need = slab_need(r.ch, shared);
+    r.split = sl + r.need <= cap;
+    if (r.split) {
+      r.slots = r.ch;
+      r.cls = cost_bucket((c + r.ch - 1) / r.ch);
+    }
+  }
+  return r;
+}
+
+// Inclusive scans over the block's threads of N ints per thread at once
+// (one exchange of wave totals for all N); tot receives the block totals.
+// s_wave holds N * kScanWaves ints. Every thread of the block takes part.
+template <int N>
+__device__ __forceinline__ void block_scan_incl_n(int *s_wave, int (&v)[N], int (&tot)[N]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_scan_incl(v[q]);
+  if (lane == 63) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) s_wave[q * kScanWaves + wv] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    int before = 0, all = 0;
+    for (int k = 0; k < nw; ++k) {
+      const int x = s_wave[q * kScanWaves + k];
+      before += k < wv ? x : 0;
+      all += x;
+    }
+    v[q] += before;
+    tot[q] = all;
+  }
+  __syncthreads();  // s_wave is reused by the next scan
+}
+
+// Items of tile k (c records from `run`; its slab range would start at sl
+// and its split index is sp): ch chunk items when split, else one item.
+// The tile's `slots` item positions start at pos0; those an unsplit hot
+// tile leaves unused become empty items.
+__device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
+                                           long long run, int sl, int sp,
+                                           int chunk, bool split, int slots,
+                                           int pos0, bool shared) {
+  if (c <= 0) return;
+  const int ch = split ? chunks_of(c, chunk) : 1;
+  const int per_chunk = split ? (c + ch - 1) / ch : c;
+  if (split) w.hot[sp] = HotTile{k, sl, ch, shared ? 1 : 0};
+  for (int q = 0; q < slots; ++q) {
+    const int pos = pos0 + q;
+    if (pos >= w.max_items) break;
+    if (q < ch) {
+      const int ib = static_cast<int>(run) + q * per_chunk;
+      const int ie = min(ib + per_chunk, static_cast<int>(run) + c);
+      const bool big = !split && c > chunk;  // hot tile without slabs
+      w.items[pos] = Item{k, ib, ie, split ? (shared ? sl : sl + q) : (big ? -2 : -1)};
+      if (big) w.big[atomicAdd(w.misc + 4, 1)] = pos;
+    } else {
+      w.items[pos] = Item{-1, 0, 0, -1};
+    }
+  }
+}
+
+// Takes `slots` item positions for an item of class q from the thread's
+// class cursors (registers; the loop unrolls to selects).
+__device__ __forceinline__ int take_slots(int (&cur)[kCostBuckets], int q, int slots) {
+  int pos = 0;
+#pragma unroll
+  for (int u = 0; u < kCostBuckets; ++u) {
+    if (q == u) {
+      pos = cur[u];
+      cur[u] += slots;
+    }
+  }
+  return pos;
+}
+
+// Single-workgroup scan for up to kScanStage tiles. Each thread owns a run
+// of consecutive tiles; one block scan gives the record and slab prefixes,
+// a second one (after the split decisions, which need the slab prefix)
+// each thread's first item slot in every cost class and its first split
+// index. The kernel writes the tiles' run starts and each thread's bases;
+// the work items are written from those by the placing pass (place_items),
+// spread over several workgroups: one workgroup's VALU would take longer
+// than the rest of this scan.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_scan_kernel(const Batch b, Workspace w) {
+  __shared__ int s_wave[(1 + kCostBuckets) * kScanWaves];
+  __shared__ int s_count[kScanStage];
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
+  const int t = threadIdx.x;
+  if (t == 0) w.misc[4] = 0;
+  for (int k = t; k < n_tiles; k += kScanThreads) {
+    s_count[k] = w.tile_count[k];
+    w.tile_count[k] = 0;
+  }
+  __syncthreads();
+  const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
+  const int beg = min(t * per, n_tiles);
+  const int end = min(beg + per, n_tiles);
+  int a[2] = {0, 0}, ta[2];  // records, slabs the hot tiles would take
+  for (int k = beg; k < end; ++k) {
+    const int c = s_count[k];
+    a[0] += c;
+    if (c > chunk) a[1] += slab_need(chunks_of(c, chunk), shared);
+  }
+  const int seg = a[0], slabs = a[1];
+  block_scan_incl_n<2>(s_wave, a, ta);
+  // Split decisions (slab prefix over all hot tiles in tile order): item
+  // slots per cost class, and split tiles.
+  int n[1 + kCostBuckets] = {}, tn[1 + kCostBuckets];
+  int sl = a[1] - slabs;
+  for (int k = beg; k < end; ++k) {
+    const int c = s_count[k];
+    if (c <= 0) continue;
+    const TileSplit r = tile_split(c, chunk, sl, w.slabs_cap, shared);
+#pragma unroll
+    for (int u = 0; u < kCostBuckets; ++u) n[u] += r.cls == u ? r.slots : 0;
+    n[kCostBuckets] += r.split ? 1 : 0;
+    sl += r.need;
+  }
+  int own[1 + kCostBuckets];
+#pragma unroll
+  for (int u = 0; u <= kCostBuckets; ++u) own[u] = n[u];
+  block_scan_incl_n<1 + kCostBuckets>(s_wave, n, tn);
+  int cur[kCostBuckets];
+  int start = 0;
+#pragma unroll
+  for (int u = 0; u < kCostBuckets; ++u) {
+    cur[u] = start + n[u] - own[u];
+    start += tn[u];
+  }
+  if (t == 0) {
+    w.misc[0] = min(start, w.max_items);
+    w.misc[1] = ta[0];
+    w.misc[3] = tn[kCostBuckets];
+  }
+  long long run = a[0] - seg;
+  int *st = w.scan_state + t * kScanState;
+  st[0] = static_cast<int>(run);
+  st[1] = a[1] - slabs;
+  st[2] = n[kCostBuckets] - own[kCostBuckets];
+#pragma unroll
+  for (int u = 0; u < kCostBuckets; ++u) st[3 + u] = cur[u];
+  for (int k = beg; k < end; ++k) {
+    w.tile_cursor[k] = static_cast<int>(run);
+    run += s_count[k];
+  }
+}
+
+// The work items of tiles k0 .. k0 + blockDim - 1 (one thread per tile),
+// from the bases tile_scan saved for the thread that owned each (a run of
+// `per` consecutive tiles): the run's earlier tiles' split tiles, slabs and
+// item slots per class are recounted from the tiles' run starts (a tile's
+// record count is the difference of consecutive run starts). A split
+// tile's chunk items are written by the whole workgroup. Every thread of
+// the workgroup calls this.
+struct SplitEntry {
+  int k, run, c, per_chunk, pos0, sl, ch, pad;
+};
+__device__ __forceinline__ void place_items(const Batch &b, const Workspace &w, int k0,
+                                            SplitEntry *s_split, int *s_ch, int *s_wave,
+                                            int *s_nsplit) {
+  const int k = k0 + static_cast<int>(threadIdx.x);
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
+  if (threadIdx.x == 0) *s_nsplit = 0;
+  __syncthreads();
+  if (k < n_tiles) {
+    const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
+    const int t = k / per;
+    const int beg = t * per;
+    const int *st = w.scan_state + t * kScanState;
+    int sl = st[1], sp = st[2];
+    int cur[kCostBuckets];
+#pragma unroll
+    for (int u = 0; u < kCostBuckets; ++u) cur[u] = st[3 + u];
+    // Run starts of tiles beg .. k + 1, loaded up front (at most
+    // kScanStage / kScanThreads + 1 of them).
+    constexpr int kMaxPer = kScanStage / kScanThreads;
+    const int total = w.misc[1];
+    int cv[kMaxPer + 1];
+#pragma unroll
+    for (int q = 0; q <= kMaxPer; ++q) {
+      const int j = beg + q;
+      cv[q] = j <= k + 1 ? (j < n_tiles ? w.tile_cursor[j] : total) : 0;
+    }
+    // The run's tiles before k.
+    const int mine = k - beg;
+#pragma unroll
+    for (int q = 0; q < kMaxPer; ++q) {
+      if (q >= mine) break;
+      const int c = cv[q + 1] - cv[q];
+      if (c <= 0) continue;
+      const TileSplit r = tile_split(c, chunk, sl, w.slabs_cap, shared);
+      take_slots(cur, r.cls, r.slots);
+      sp += r.split ? 1 : 0;
+      sl += r.need;
+    }
+    int c = 0, run = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxPer; ++q) {
+      if (q == mine) {
+        c = cv[q + 1] - cv[q];
+        run = cv[q];
+      }
+    }
+    if (c > 0) {
+      const TileSplit r = tile_split(c, chunk, sl, w.slabs_cap, shared);
+      const int pos0 = take_slots(cur, r.cls, r.slots);
+      if (!r.split) {
+        place_tile(w, k, c, run, sl, sp, chunk, false, 1, pos0, shared);
+      } else {
+        w.hot[sp] = HotTile{k, sl, r.ch, shared ? 1 : 0};
+        const int e = atomicAdd(s_nsplit, 1);
+        s_split[e] = SplitEntry{k, run, c, (c + r.ch - 1) / r.ch, pos0, sl, r.ch, 0};
+      }
+    }
+  }
+  __syncthreads();
+  // Chunk items of the split tiles, one per thread at a time: entry e's
+  // chunks follow the chunks of entries before it (s_ch: inclusive prefix).
+  const int ns = *s_nsplit;
+  if (ns == 0) return;  // uniform
+  int ch = static_cast<int>(threadIdx.x) < ns ? s_split[threadIdx.x].ch : 0;
+  ch = block_scan_incl(s_wave, ch);
+  if (static_cast<int>(threadIdx.x) < ns) s_ch[threadIdx.x] = ch;
+  __syncthreads();
+  const int all = s_ch[ns - 1];
+  for (int g = threadIdx.x; g < all; g += blockDim.x) {
+    int lo = 0, hi = ns - 1;  // first entry whose prefix exceeds g
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_ch[mid] > g) hi = mid; else lo = mid + 1;
+    }
+    const SplitEntry se = s_split[lo];
+    const int q = g - (s_ch[lo] - se.ch);
+    const int pos = se.pos0 + q;
+    if (pos >= w.max_items) continue;
+    const int ib = se.run + q * se.per_chunk;
+    const int ie = min(ib + se.per_chunk, se.run + se.c);
+    w.items[pos] = Item{se.k, ib, ie, shared ? se.sl : se.sl + q};
+  }
+}
+
+// Multi-workgroup scan, pass 1: per block of kScanTilesPerBlock tiles, the
+// records, the slabs its hot tiles would take, and the items per cost class
+// as if every hot tile were split.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_sums_kernel(const Batch b, Workspace w) {
+  __shared__ int s_sum[kScanSums];
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
+  const int k0 = blockIdx.x * kScanTilesPerBlock;
+  if (threadIdx.x < kScanSums) s_sum[threadIdx.x] = 0;
+  __syncthreads();
+  int v[kScanSums] = {};  // records, slabs, (split tiles: pass 3), classes
+  for (int k = k0 + threadIdx.x; k < min(k0 + kScanTilesPerBlock, n_tiles);
+       k += kScanThreads) {
+    const int c = w.tile_count[k];
+    v[0] += c;
+    const int ch = chunks_of(c, chunk);
+    if (c > 0) {
+      const int q = cost_bucket((c + ch - 1) / ch);
+#pragma unroll
+      for (int u = 0; u < kCostBuckets; ++u) v[3 + u] += q == u ? ch : 0;
+    }
+    if (ch > 1) v[1] += slab_need(ch, shared);
+  }
+#pragma unroll
+  for (int q = 0; q < kScanSums; ++q) {
+    int x = v[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x != 0) atomicAdd(s_sum + q, x);
+  }
+  __syncthreads();
+  if (threadIdx.x < kScanSums) {
+    w.block_sums[blockIdx.x * kScanSums + threadIdx.x] = s_sum[threadIdx.x];
+  }
+}
+
+// Pass 2 (one workgroup): exclusive per-block bases of records and slabs,
+// and for every cost class the block's first item slot.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_bases_kernel(const Batch b, Workspace w) {
+  __shared__ int s_wave[kScanSums * kScanWaves];
+  const int n_blocks =
+      (batch_tiles(b) + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
+  const int t = threadIdx.x;  // one thread per block (at most kScanThreads)
+  int v[kScanSums], tot[kScanSums];
+#pragma unroll
+  for (int q = 0; q < kScanSums; ++q) v[q] = t < n_blocks ? w.block_sums[t * kScanSums + q] : 0;
+  int own[kScanSums];
+#pragma unroll
+  for (int q = 0; q < kScanSums; ++q) own[q] = v[q];
+  block_scan_incl_n<kScanSums>(s_wave, v, tot);
+  int start = 0;
+#pragma unroll
+  for (int q = 0; q < kCostBuckets; ++q) {
+    v[3 + q] += start;  // first slot of class q after the earlier classes
+    start += tot[3 + q];
+  }
+  if (t < n_blocks) {
+#pragma unroll
+    for (int q = 0; q < kScanSums; ++q) w.block_bases[t * kScanSums + q] = v[q] - own[q];
+  }
+  if (t == 0) {
+    w.misc[0] = min(start, w.max_items);
+    w.misc[1] = tot[0];
+    w.misc[3] = 0;  // split and unsplit hot tiles, counted by pass 3
+    w.misc[4] = 0;
+  }
+}
+
+// Pass 3: per block, cursors, items and split tiles from the block bases.
+// A hot tile whose slab range does not fit keeps one item (in its class
+// slot) and turns its other slots into empty items.
+__global__ void __launch_bounds__(kScanThreads)
+    tile_place_kernel(const Batch b, Workspace w) {
+  __shared__ int s_wave[(1 + kCostBuckets) * kScanWaves];
+  __shared__ int s_hot;
+  const int n_tiles = batch_tiles(b);
+  const int chunk = batch_chunk(b, w);
+  const bool shared = batch_shared(b);
+  const int k0 = blockIdx.x * kScanTilesPerBlock;
+  const int k1 = min(k0 + kScanTilesPerBlock, n_tiles);
+  const int t = threadIdx.x;
+  const int *base = w.block_bases + blockIdx.x * kScanSums;
+  constexpr int kPer = kScanTilesPerBlock / kScanThreads;
+  const int beg = min(k0 + t * kPer, k1);
+  const int end = min(beg + kPer, k1);
+  int a[2] = {0, 0}, ta[2];  // records, slabs the hot tiles would take
+  for (int k = beg; k < end; ++k) {
+    const int c = w.tile_count[k];
+    a[0] += c;
+    if (c > chunk) a[1] += slab_need(chunks_of(c, chunk), shared);
+  }
+  const int seg = a[0], slabs = a[1];
+  block_scan_incl_n<2>(s_wave, a, ta);
+  // Split tiles under the slab prefix, and item slots per class (ch each).
+  int n[1 + kCostBuckets] = {}, tn[1 + kCostBuckets];
+  int sl = base[1] + a[1] - slabs;
+  for (int k = beg; k < end; ++k) {
+    const int c = w.tile_count[k];
+    const int ch = chunks_of(c, chunk);
+    if (c > 0) {
+      const int q = cost_bucket((c + ch - 1) / ch);
+#pragma unroll
+      for (int u = 0; u < kCostBuckets; ++u) n[1 + u] += q == u ? ch : 0;
+    }
+    if (ch > 1) {
+      if (sl + slab_need(ch, shared) <= w.slabs_cap) ++n[0];
+      sl += slab_need(ch, shared);
+    }
+  }
+  int own[1 + kCostBuckets];
+#pragma unroll
+  for (int u = 0; u <= kCostBuckets; ++u) own[u] = n[u];
+  block_scan_incl_n<1 + kCostBuckets>(s_wave, n, tn);
+  if (t == 0) s_hot = atomicAdd(w.misc + 3, tn[0]);
+  __syncthreads();
+  int cur[kCostBuckets];
+#pragma unroll
+  for (int u = 0; u < kCostBuckets; ++u) cur[u] = base[3 + u] + n[1 + u] - own[1 + u];
+  long long run = static_cast<long long>(base[0]) + a[0] - seg;
+  sl = base[1] + a[1] - slabs;
+  int sp = s_hot + n[0] - own[0];
+  for (int k = beg; k < end; ++k) {
+    const int c = w.tile_count[k];
+    const int ch = chunks_of(c, chunk);
+    const bool split = ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap;
+    const int pos0 = c > 0 ? take_slots(cur, cost_bucket((c + ch - 1) / ch), ch) : 0;
+    w.tile_cursor[k] = static_cast<int>(run);
+    place_tile(w, k, c, run, sl, sp, chunk, split, ch, pos0, shared);
+    if (split) ++sp;
+    if (ch > 1) sl += slab_need(ch, shared);
+    run += c;
+    w.tile_count[k] = 0;
+  }
+}
+
 // Placing pass (after tile_scan has set each tile's run start in
 // tile_cursor): record j of every ray of this block's kWalkBlock rays, for
 // the kPlaceJ values of j this block's y index names, goes to its place in
 // tile order. The record's counting group is its ray's group of 2^lg_group
 // rays, whose hash has `hash` entries. Every record is independent, so the
-// pass is a gather and a scatter with no atomics.
+// pass is a gather and a scatter with no atomics. After the single-workgroup
+// tile scan, the first item_blocks blocks of y index 0 write the work items
+// (place_items, one thread per tile) beside it; they come first in dispatch
+// order, so their loads are not queued behind the records' traffic.
 constexpr int kPlaceJ = 4;
 __global__ void __launch_bounds__(kWalkBlock)
-    place_kernel(const Batch b, Workspace w, int lg_group, int hash) {
-  const int ray = static_cast<int>(blockIdx.x) * kWalkBlock + static_cast<int>(threadIdx.x);
-  const int ray0 = static_cast<int>(blockIdx.x) * kWalkBlock;
+    place_kernel(const Batch b, Workspace w, int lg_group, int hash, int item_blocks) {
+  if (static_cast<int>(blockIdx.x) < item_blocks) {
+    if (blockIdx.y == 0) {
+      __shared__ SplitEntry s_split[kWalkBlock];
+      __shared__ int s_ch[kWalkBlock];
+      __shared__ int s_wave[kWalkBlock / 64];
+      __shared__ int s_nsplit;
+      place_items(b, w, static_cast<int>(blockIdx.x) * kWalkBlock, s_split, s_ch, s_wave,
+                  &s_nsplit);
+    }
+    return;
+  }
+  const int bx = static_cast<int>(blockIdx.x) - item_blocks;
+  const int ray = bx * kWalkBlock + static_cast<int>(threadIdx.x);
+  const int ray0 = bx * kWalkBlock;
   const int n_rays = batch_rays(b);
   if (ray0 >= n_rays) return;
   const int cnt = ray < n_rays ? w.seg_count[ray] : 0;
@@ -1073,261 +1520,6 @@ __global__ void __launch_bounds__(kWalkBlock)
   }
 }
 
-
-// ----------------------------------------------------------------------
-// tile_scan: cursors, work items, hot-tile splits
-// ----------------------------------------------------------------------
-// A tile with more than `chunk` records is split into chunks of about
-// `chunk` records, each its own work item with its own scratch slab, while
-// the split tiles' slabs (taken in tile order) fit the scratch area; a hot
-// tile beyond that is one item, accumulated in 16-bit passes. Items are
-// placed in decreasing cost classes (log2 of records per item) so the
-// heaviest start first.
-constexpr int kScanThreads = 1024;
-constexpr int kScanWaves = kScanThreads / 64;
-constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
-constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
-constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
-constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
-constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
-
-
-// Cost class of a work item (its record count), larger classes first.
-__device__ __forceinline__ int cost_bucket(int records) {
-  const int lg = 31 - __clz(max(records, 1));
-  return kCostBuckets - 1 - min(lg, kCostBuckets - 1);
-}
-
-// Number of chunks a tile with c records is split into (1: not split).
-__device__ __forceinline__ int chunks_of(int c, int chunk) {
-  return c > chunk ? (c + chunk - 1) / chunk : 1;
-}
-
-// Items of tile k (c records from `run`; its slab range would start at sl
-// and its split index is sp): ch chunk items when split, else one item.
-// `slots` item positions are taken from the class cursor `bucket`; those
-// an unsplit hot tile leaves unused become empty items.
-__device__ __forceinline__ void place_tile(const Workspace &w, int k, int c,
-                                           long long run, int sl, int sp,
-                                           int chunk, bool split, int bucket,
-                                           int slots, int *s_cursor, bool shared) {
-  w.tile_cursor[k] = static_cast<int>(run);
-  if (c <= 0) return;
-  const int ch = split ? chunks_of(c, chunk) : 1;
-  const int per_chunk = (c + ch - 1) / ch;
-  if (split) w.hot[sp] = HotTile{k, sl, ch, shared ? 1 : 0};
-  const int pos0 = atomicAdd(s_cursor + bucket, slots);  // one claim per tile
-  for (int q = 0; q < slots; ++q) {
-    const int pos = pos0 + q;
-    if (pos >= w.max_items) break;
-    if (q < ch) {
-      const int ib = static_cast<int>(run) + q * per_chunk;
-      const int ie = min(ib + per_chunk, static_cast<int>(run) + c);
-      const bool big = !split && c > chunk;  // hot tile without slabs
-      w.items[pos] = Item{k, ib, ie, split ? (shared ? sl : sl + q) : (big ? -2 : -1)};
-      if (big) w.big[atomicAdd(w.misc + 4, 1)] = pos;
-    } else {
-      w.items[pos] = Item{-1, 0, 0, -1};
-    }
-  }
-}
-
-// Single-workgroup scan for up to kScanStage tiles.
-__global__ void __launch_bounds__(kScanThreads)
-    tile_scan_kernel(const Batch b, Workspace w) {
-  __shared__ int s_buf[kScanWaves];
-  __shared__ int s_bucket[kCostBuckets];
-  __shared__ int s_cursor[kCostBuckets];
-  __shared__ int s_count[kScanStage];
-  const int n_tiles = batch_tiles(b);
-  const int chunk = batch_chunk(b, w);
-  const bool shared = batch_shared(b);
-  const int t = threadIdx.x;
-  if (t < kCostBuckets) s_bucket[t] = 0;
-  if (t == 0) w.misc[4] = 0;
-  for (int k = t; k < n_tiles; k += kScanThreads) {
-    s_count[k] = w.tile_count[k];
-    w.tile_count[k] = 0;
-  }
-  __syncthreads();
-  const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
-  const int beg = min(t * per, n_tiles);
-  const int end = min(beg + per, n_tiles);
-  int seg = 0, slabs = 0;
-  for (int k = beg; k < end; ++k) {
-    const int c = s_count[k];
-    seg += c;
-    if (c > chunk) slabs += slab_need(chunks_of(c, chunk), shared);
-  }
-  const int seg_incl = block_scan_incl(s_buf, seg);
-  const int slab_incl = block_scan_incl(s_buf, slabs);
-  // Split decisions (slab prefix over all hot tiles in tile order) and the
-  // items per cost class they give.
-  int sl = slab_incl - slabs, hot = 0;
-  for (int k = beg; k < end; ++k) {
-    const int c = s_count[k];
-    if (c <= 0) continue;
-    const int ch = chunks_of(c, chunk);
-    if (ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap) {
-      atomicAdd(s_bucket + cost_bucket((c + ch - 1) / ch), ch);
-      ++hot;
-    } else {
-      atomicAdd(s_bucket + cost_bucket(c), 1);
-    }
-    if (ch > 1) sl += slab_need(ch, shared);
-  }
-  const int hot_incl = block_scan_incl(s_buf, hot);
-  if (t == 0) {
-    int run = 0;
-    for (int q = 0; q < kCostBuckets; ++q) {
-      s_cursor[q] = run;
-      run += s_bucket[q];
-    }
-    w.misc[0] = min(run, w.max_items);
-  }
-  __syncthreads();
-  long long run = seg_incl - seg;
-  int sp = hot_incl - hot;
-  sl = slab_incl - slabs;
-  for (int k = beg; k < end; ++k) {
-    const int c = s_count[k];
-    const int ch = chunks_of(c, chunk);
-    const bool split = ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap;
-    const int bucket = cost_bucket(split ? (c + ch - 1) / ch : c);
-    place_tile(w, k, c, run, sl, sp, chunk, split, bucket, split ? ch : 1,
-               s_cursor, shared);
-    if (split) ++sp;
-    if (ch > 1) sl += slab_need(ch, shared);
-    run += c;
-  }
-  if (t == kScanThreads - 1) {
-    w.misc[1] = seg_incl;
-    w.misc[3] = hot_incl;
-  }
-}
-
-// Multi-workgroup scan, pass 1: per block of kScanTilesPerBlock tiles, the
-// records, the slabs its hot tiles would take, and the items per cost class
-// as if every hot tile were split.
-__global__ void __launch_bounds__(kScanThreads)
-    tile_sums_kernel(const Batch b, Workspace w) {
-  __shared__ int s_sum[kScanSums];
-  const int n_tiles = batch_tiles(b);
-  const int chunk = batch_chunk(b, w);
-  const bool shared = batch_shared(b);
-  const int k0 = blockIdx.x * kScanTilesPerBlock;
-  if (threadIdx.x < kScanSums) s_sum[threadIdx.x] = 0;
-  __syncthreads();
-  int seg = 0, slabs = 0;
-  for (int k = k0 + threadIdx.x; k < min(k0 + kScanTilesPerBlock, n_tiles);
-       k += kScanThreads) {
-    const int c = w.tile_count[k];
-    seg += c;
-    const int ch = chunks_of(c, chunk);
-    if (c > 0) atomicAdd(s_sum + 3 + cost_bucket((c + ch - 1) / ch), ch);
-    if (ch > 1) slabs += slab_need(ch, shared);
-  }
-  atomicAdd(s_sum + 0, seg);
-  atomicAdd(s_sum + 1, slabs);
-  __syncthreads();
-  if (threadIdx.x < kScanSums) {
-    w.block_sums[blockIdx.x * kScanSums + threadIdx.x] = s_sum[threadIdx.x];
-  }
-}
-
-// Pass 2 (one workgroup): exclusive per-block bases of records and slabs,
-// and for every cost class the block's first item slot.
-__global__ void __launch_bounds__(kScanThreads)
-    tile_bases_kernel(const Batch b, Workspace w) {
-  __shared__ int s_buf[kScanWaves];
-  __shared__ int s_total[kScanSums];
-  const int n_blocks =
-      (batch_tiles(b) + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
-  const int t = threadIdx.x;  // one thread per block (at most kScanThreads)
-  for (int q = 0; q < kScanSums; ++q) {
-    const int v = t < n_blocks ? w.block_sums[t * kScanSums + q] : 0;
-    const int incl = block_scan_incl(s_buf, v);
-    if (t < n_blocks) w.block_bases[t * kScanSums + q] = incl - v;
-    if (t == kScanThreads - 1) s_total[q] = incl;
-  }
-  __syncthreads();
-  if (t == 0) {
-    int run = 0;
-    for (int q = 0; q < kCostBuckets; ++q) {
-      const int n = s_total[3 + q];
-      s_total[3 + q] = run;  // first slot of class q
-      run += n;
-    }
-    w.misc[0] = min(run, w.max_items);
-    w.misc[1] = s_total[0];
-  }
-  __syncthreads();
-  for (int q = 0; q < kCostBuckets; ++q) {
-    if (t < n_blocks) w.block_bases[t * kScanSums + 3 + q] += s_total[3 + q];
-  }
-  if (t == 0) {  // split and unsplit hot tiles, counted by pass 3
-    w.misc[3] = 0;
-    w.misc[4] = 0;
-  }
-}
-
-// Pass 3: per block, cursors, items and split tiles from the block bases.
-// A hot tile whose slab range does not fit keeps one item (in its class
-// slot) and turns its other slots into empty items.
-__global__ void __launch_bounds__(kScanThreads)
-    tile_place_kernel(const Batch b, Workspace w) {
-  __shared__ int s_buf[kScanWaves];
-  __shared__ int s_cursor[kCostBuckets];
-  __shared__ int s_hot;
-  const int n_tiles = batch_tiles(b);
-  const int chunk = batch_chunk(b, w);
-  const bool shared = batch_shared(b);
-  const int k0 = blockIdx.x * kScanTilesPerBlock;
-  const int k1 = min(k0 + kScanTilesPerBlock, n_tiles);
-  const int t = threadIdx.x;
-  const int *base = w.block_bases + blockIdx.x * kScanSums;
-  if (t < kCostBuckets) s_cursor[t] = base[3 + t];
-  constexpr int kPer = kScanTilesPerBlock / kScanThreads;
-  const int beg = min(k0 + t * kPer, k1);
-  const int end = min(beg + kPer, k1);
-  int seg = 0, slabs = 0, hot = 0;
-  {
-    // Split decisions need the slab prefix; count split tiles under them.
-    for (int k = beg; k < end; ++k) {
-      const int c = w.tile_count[k];
-      seg += c;
-      if (c > chunk) slabs += slab_need(chunks_of(c, chunk), shared);
-    }
-  }
-  const int seg_incl = block_scan_incl(s_buf, seg);
-  const int slab_incl = block_scan_incl(s_buf, slabs);
-  int sl = base[1] + slab_incl - slabs;
-  for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
-    const int ch = chunks_of(c, chunk);
-    if (ch > 1) {
-      if (sl + slab_need(ch, shared) <= w.slabs_cap) ++hot;
-      sl += slab_need(ch, shared);
-    }
-  }
-  const int hot_incl = block_scan_incl(s_buf, hot);
-  if (t == kScanThreads - 1) s_hot = atomicAdd(w.misc + 3, hot_incl);
-  __syncthreads();
-  long long run = static_cast<long long>(base[0]) + seg_incl - seg;
-  sl = base[1] + slab_incl - slabs;
-  int sp = s_hot + hot_incl - hot;
-  for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
-    const int ch = chunks_of(c, chunk);
-    const bool split = ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap;
-    place_tile(w, k, c, run, sl, sp, chunk, split,
-               cost_bucket((c + ch - 1) / ch), ch, s_cursor, shared);
-    if (split) ++sp;
-    if (ch > 1) sl += slab_need(ch, shared);
-    run += c;
-    w.tile_count[k] = 0;
-  }
-}
 
 // ----------------------------------------------------------------------
 // tile_apply
@@ -1821,7 +2013,7 @@ struct Layout {
   int max_items, scan_blocks;
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
-      off_bbase, off_srank, off_gbase, off_hdr, off_desc, total;
+      off_bbase, off_srank, off_gbase, off_sstate, off_hdr, off_desc, total;
 };
 
 int bin_rays_per_block(const Caps &c);
@@ -1875,6 +2067,7 @@ Layout make_layout(const Caps &c) {
   take(L.off_bbase, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
   take(L.off_srank, static_cast<size_t>(c.seg_capacity) * 4);
   take(L.off_gbase, static_cast<size_t>(group_table_entries(c)) * 4);
+  take(L.off_sstate, static_cast<size_t>(kScanThreads) * kScanState * 4);
   // Header and descriptors, laid out as the host staging buffer so one
   // copy moves both.
   take(L.off_hdr, staging_bytes(c.max_grids));
@@ -1903,6 +2096,7 @@ Workspace bind(const Layout &L, void *base) {
   w.block_bases = reinterpret_cast<int *>(b + L.off_bbase);
   w.seg_rank = reinterpret_cast<unsigned *>(b + L.off_srank);
   w.grp_base = reinterpret_cast<int *>(b + L.off_gbase);
+  w.scan_state = reinterpret_cast<int *>(b + L.off_sstate);
   w.seg_capacity = L.cap.seg_capacity;
   w.max_items = L.max_items;
   w.slabs_cap = static_cast<int>(L.slabs);
@@ -2107,8 +2301,15 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   {
     // Counting groups: the quad walk's workgroups, else the bin blocks.
     const int lg_group = 31 - __builtin_clz(static_cast<unsigned>(quad ? kWalkBlock : bin_rays));
-    const dim3 grid(walk_blocks, static_cast<unsigned>((c.max_seg + kPlaceJ - 1) / kPlaceJ));
-    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash);
+    // After the single-workgroup scan, more workgroups write the work
+    // items, one thread per tile.
+    const bool single = c.max_tiles <= kScanStage;
+    const unsigned item_blocks =
+        single ? static_cast<unsigned>((c.max_tiles + kWalkBlock - 1) / kWalkBlock) : 0u;
+    const dim3 grid(walk_blocks + item_blocks,
+                    static_cast<unsigned>((c.max_seg + kPlaceJ - 1) / kPlaceJ));
+    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash,
+                                                     static_cast<int>(item_blocks));
     st = check_launch("place_kernel");
   }
   if (st != CUBLOX_OK) return st;
