@@ -71,18 +71,16 @@ constexpr int kHotSlices = 8;     // hot_fold workgroups per split tile
 constexpr int kCostBuckets = 4;   // classes of item cost (records per item)
 
 // A segment is the part of one ray inside one tile span: its DDA state at
-// the span entry, the increments, and in `meta` the entry voxel's index in
-// the tile (bits 0..12), the number of voxels it visits (13..19), the step
-// signs (20..25, s + 1 two bits each), an endpoint flag (26) and the
-// clipped flag (27). An endpoint record adds op (and hit unless clipped) at
-// its voxel.
-struct Segment {  // 32 B
+// the span entry and in `meta` the entry voxel's index in the tile (bits
+// 0..12), the number of voxels it visits (13..19), the step signs (20..25,
+// s + 1 two bits each), an endpoint flag (26) and the clipped flag (27).
+// The increments are the ray's own (ray_td). An endpoint record adds op
+// (and hit unless clipped) at its voxel.
+struct Segment {  // 16 B
   float tMx, tMy, tMz;
-  float tDx, tDy, tDz;
   unsigned meta;
-  unsigned pad;
 };
-static_assert(sizeof(Segment) == 32, "records move as two 16 B words");
+static_assert(sizeof(Segment) == 16, "a record moves as one 16 B word");
 
 constexpr unsigned kMetaLi = 0x1FFFu;
 constexpr int kMetaN = 13;
@@ -167,6 +165,7 @@ struct Workspace {
   int *seg_count;     // n: segments written by each ray
   int *seg_tile;      // n * max_seg: tile of each slot's segment
   Segment *segs;      // n * max_seg: record j of ray i at j * n + i
+  float4 *ray_td;     // per ray: the DDA increments (tDx, tDy, tDz, 0)
   unsigned *order;    // n * max_seg: record slots in tile order
   int *tile_count;    // n_tiles (zero between launches)
   int *tile_cursor;   // n_tiles
@@ -606,9 +605,10 @@ struct GridDesc {
   int *delta_idx;
   float *delta_val;
   unsigned delta_cap;
+  FastDiv n_div;       // division by g.n (a record slot's ray)
 };
 
-static_assert(sizeof(GridDesc) == 296, "descriptor size (documented)");
+static_assert(sizeof(GridDesc) == 304, "descriptor size (documented)");
 
 struct BatchHeader {  // device-resident in batch mode
   int n_grids;
@@ -844,12 +844,8 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
       sg.tMx = tmx;
       sg.tMy = tmy;
       sg.tMz = tmz;
-      sg.tDx = r.tDx;
-      sg.tDy = r.tDy;
-      sg.tDz = r.tDz;
       sg.meta = static_cast<unsigned>(li) |
                 (static_cast<unsigned>(n) << kMetaN) | signs;
-      sg.pad = 0;
       put(sg, tile);
     };
     const int s0x = r.ix, s0y = r.iy, s0z = r.iz;  // sensor voxel
@@ -918,14 +914,15 @@ __global__ void __launch_bounds__(kWalkBlock * kLanes)
         const HashCoord e = window_hash_coord(g, r.eix, r.eiy, r.eiz);
         Segment sg;
         sg.tMx = sg.tMy = sg.tMz = 0.0f;
-        sg.tDx = sg.tDy = sg.tDz = 0.0f;
         sg.meta = static_cast<unsigned>(tile_local(e.cx, e.cy, e.cz)) |
                   kMetaEndpoint | (r.clipped ? kMetaClipped : 0u);
-        sg.pad = 0;
         put(sg, tile_of(g, e.cx, e.cy, e.cz));
       }
     }
-    if (lead) w.seg_count[ray] = min(nseg, g.max_seg);
+    if (lead) {
+      w.seg_count[ray] = min(nseg, g.max_seg);
+      w.ray_td[ray] = make_float4(r.tDx, r.tDy, r.tDz, 0.0f);
+    }
   } else if (lead) {
     w.seg_count[ray] = 0;
   }
@@ -1525,8 +1522,10 @@ __global__ void __launch_bounds__(kWalkBlock)
 // tile_apply
 // ----------------------------------------------------------------------
 
-// Replays one record into the packed LDS counters.
-__device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt) {
+// Replays one record (of a ray with increments td) into the packed LDS
+// counters.
+__device__ __forceinline__ void replay_record(const Segment &sg, const float4 &td,
+                                              unsigned *s_cnt) {
   int li = static_cast<int>(sg.meta & kMetaLi);
   if (sg.meta & kMetaEndpoint) {
     atomicAdd(s_cnt + li, (sg.meta & kMetaClipped) ? 1u : 0x10001u);
@@ -1543,9 +1542,9 @@ __device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt
     const bool x_lt_y = tMx < tMy;
     const bool step_x = x_lt_y && (tMx < tMz);
     const bool step_y = !x_lt_y && (tMy < tMz);
-    const float nx = tMx + sg.tDx;
-    const float ny = tMy + sg.tDy;
-    const float nz = tMz + sg.tDz;
+    const float nx = tMx + td.x;
+    const float ny = tMy + td.y;
+    const float nz = tMz + td.z;
     tMx = step_x ? nx : tMx;
     tMy = step_y ? ny : tMy;
     tMz = (!step_x && !step_y) ? nz : tMz;
@@ -1560,21 +1559,43 @@ __device__ __forceinline__ void replay_record(const Segment &sg, unsigned *s_cnt
 // a voxel at most once, so neither field carries into the other.
 __device__ __forceinline__ void lds_zero(uint4 *s4);
 
+// The grid fields that map a record slot to its ray: slot = slot_base +
+// j * n + i for record j of ray i (global ray ray_base + i).
+struct SlotRays {
+  long long slot_base;
+  int ray_base, n;
+  FastDiv n_div;
+};
+__device__ __forceinline__ SlotRays slot_rays(const GridDesc &d) {
+  return SlotRays{d.slot_base, d.ray_base, d.g.n, d.n_div};
+}
+__device__ __forceinline__ int ray_of_slot(const SlotRays &sr, unsigned slot) {
+  const unsigned rel = static_cast<unsigned>(static_cast<long long>(slot) - sr.slot_base);
+  const unsigned j = fast_div(rel, sr.n_div);
+  return sr.ray_base + static_cast<int>(rel - j * static_cast<unsigned>(sr.n));
+}
+
 // Clears the tile's counters and accumulates records [beg, end) into them.
-// Each thread's first record is gathered before the clear, so that load's
-// latency overlaps it.
-__device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
-                                                int end, uint4 *s_cnt4) {
+// Each thread's first record (and its ray's increments) is gathered before
+// the clear, so those loads' latency overlaps it.
+__device__ __forceinline__ void tile_accumulate(const Workspace &w, const SlotRays &sr,
+                                                int beg, int end, uint4 *s_cnt4) {
   unsigned *s_cnt = reinterpret_cast<unsigned *>(s_cnt4);
   const int k0 = beg + static_cast<int>(threadIdx.x);
   Segment first{};
-  if (k0 < end) first = w.segs[w.order[k0]];
+  float4 td0{};
+  if (k0 < end) {
+    const unsigned o = w.order[k0];
+    first = w.segs[o];
+    td0 = w.ray_td[ray_of_slot(sr, o)];
+  }
   lds_zero(s_cnt4);
   __syncthreads();
   if (end - beg <= 2 * kApplyThreads) {  // uniform: at most two records a thread
-    if (k0 < end) replay_record(first, s_cnt);
+    if (k0 < end) replay_record(first, td0, s_cnt);
     for (int k = k0 + kApplyThreads; k < end; k += kApplyThreads) {
-      replay_record(w.segs[w.order[k]], s_cnt);
+      const unsigned o = w.order[k];
+      replay_record(w.segs[o], w.ray_td[ray_of_slot(sr, o)], s_cnt);
     }
   } else {
     // Longer items (hot-tile chunks) pipeline a thread's records: while
@@ -1582,15 +1603,21 @@ __device__ __forceinline__ void tile_accumulate(const Workspace &w, int beg,
     // index after it has arrived, so no replay waits on a dependent pair
     // of loads.
     Segment cur = first;
+    float4 tcur = td0;
     unsigned o_next = k0 + kApplyThreads < end ? w.order[k0 + kApplyThreads] : 0u;
     for (int k = k0; k < end; k += kApplyThreads) {
       const int k1 = k + kApplyThreads;
       Segment nxt{};
-      if (k1 < end) nxt = w.segs[o_next];
+      float4 tnxt{};
+      if (k1 < end) {
+        nxt = w.segs[o_next];
+        tnxt = w.ray_td[ray_of_slot(sr, o_next)];
+      }
       const unsigned o_after =
           k1 + kApplyThreads < end ? w.order[k1 + kApplyThreads] : 0u;
-      replay_record(cur, s_cnt);
+      replay_record(cur, tcur, s_cnt);
       cur = nxt;
+      tcur = tnxt;
       o_next = o_after;
     }
   }
@@ -1748,8 +1775,10 @@ __global__ void __launch_bounds__(kApplyThreads)
     if (itm.tile < 0 || (!kBig && itm.slab == -2)) continue;  // uniform
     const int end = static_cast<int>(
         min(static_cast<long long>(itm.end), w.seg_capacity));
+    const GridDesc d = grid_of_tile(b, itm.tile);
+    const SlotRays sr = slot_rays(d);
     if (!big && itm.slab >= 0) {
-      tile_accumulate(w, itm.beg, end, s_cnt4);
+      tile_accumulate(w, sr, itm.beg, end, s_cnt4);
       if (batch_shared(b)) {
         // Shared mode: the chunk's nonzero counts go into the tile's op and
         // hit slabs (zeroed before this kernel) with global atomics.
@@ -1771,14 +1800,13 @@ __global__ void __launch_bounds__(kApplyThreads)
       __syncthreads();
       continue;
     }
-    const GridDesc d = grid_of_tile(b, itm.tile);
     const int tile = itm.tile - d.tile_base;
     const int tz = tile % d.g.ntz;
     const int ty = (tile / d.g.ntz) % d.g.nty;
     const int tx = tile / (d.g.ntz * d.g.nty);
     if (!big) {
       // One pass: the packed 16-bit counts fold as they are.
-      tile_accumulate(w, itm.beg, end, s_cnt4);
+      tile_accumulate(w, sr, itm.beg, end, s_cnt4);
       unsigned c[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) c[j] = s_cnt[j * kApplyThreads + threadIdx.x];
@@ -1794,7 +1822,7 @@ __global__ void __launch_bounds__(kApplyThreads)
         hit[j] = 0;
       }
       for (int p0 = itm.beg; p0 < end; p0 += kPass) {
-        tile_accumulate(w, p0, min(p0 + kPass, end), s_cnt4);
+        tile_accumulate(w, sr, p0, min(p0 + kPass, end), s_cnt4);
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
           const unsigned c = s_cnt[j * kApplyThreads + threadIdx.x];
@@ -2013,7 +2041,7 @@ struct Layout {
   int max_items, scan_blocks;
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
-      off_bbase, off_srank, off_gbase, off_sstate, off_hdr, off_desc, total;
+      off_bbase, off_srank, off_gbase, off_sstate, off_rtd, off_hdr, off_desc, total;
 };
 
 int bin_rays_per_block(const Caps &c);
@@ -2053,6 +2081,7 @@ Layout make_layout(const Caps &c) {
   take(L.off_count, static_cast<size_t>(c.max_rays) * 4);
   take(L.off_stile, static_cast<size_t>(c.seg_capacity) * 4);
   take(L.off_segs, static_cast<size_t>(c.seg_capacity) * sizeof(Segment));
+  take(L.off_rtd, static_cast<size_t>(c.max_rays) * sizeof(float4));
   take(L.off_sorted, static_cast<size_t>(c.seg_capacity) * sizeof(unsigned));
   take(L.off_tcount, static_cast<size_t>(c.max_tiles) * 4);
   take(L.off_tcur, static_cast<size_t>(c.max_tiles) * 4);
@@ -2082,6 +2111,7 @@ Workspace bind(const Layout &L, void *base) {
   w.seg_count = reinterpret_cast<int *>(b + L.off_count);
   w.seg_tile = reinterpret_cast<int *>(b + L.off_stile);
   w.segs = reinterpret_cast<Segment *>(b + L.off_segs);
+  w.ray_td = reinterpret_cast<float4 *>(b + L.off_rtd);
   w.order = reinterpret_cast<unsigned *>(b + L.off_sorted);
   w.tile_count = reinterpret_cast<int *>(b + L.off_tcount);
   w.tile_cursor = reinterpret_cast<int *>(b + L.off_tcur);
@@ -2190,6 +2220,7 @@ GridDesc make_desc(const cublox_raycast_cfg &cfg, const float *T,
     g.a.oz = T[11];
   }
   g.n = n;
+  d.n_div = make_fastdiv(static_cast<unsigned>(std::max(n, 1)));
   g.ntx = G.ntx;
   g.nty = G.nty;
   g.ntz = G.ntz;
