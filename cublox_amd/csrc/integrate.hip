@@ -1460,15 +1460,17 @@ __global__ void __launch_bounds__(kScanThreads)
 }
 
 // Placing pass (after tile_scan has set each tile's run start in
-// tile_cursor): record j of every ray of this block's kWalkBlock rays, for
-// the kPlaceJ values of j this block's y index names, goes to its place in
-// tile order. The record's counting group is its ray's group of 2^lg_group
+// tile_cursor): record j of every ray of this block's kWalkBlock rays goes
+// to its place in tile order, kPlaceJ values of j at a time: the block of y
+// index y takes j = kPlaceJ * y + kPlaceJ * gridDim.y * m (m = 0, 1, ...)
+// while the ray has records there. The record's counting group is its ray's group of 2^lg_group
 // rays, whose hash has `hash` entries. Every record is independent, so the
 // pass is a gather and a scatter with no atomics. After the single-workgroup
 // tile scan, the first item_blocks blocks of y index 0 write the work items
 // (place_items, one thread per tile) beside it; they come first in dispatch
 // order, so their loads are not queued behind the records' traffic.
 constexpr int kPlaceJ = 4;
+constexpr int kPlaceRows = 4;
 __global__ void __launch_bounds__(kWalkBlock)
     place_kernel(const Batch b, Workspace w, int lg_group, int hash, int item_blocks) {
   if (static_cast<int>(blockIdx.x) < item_blocks) {
@@ -1488,31 +1490,33 @@ __global__ void __launch_bounds__(kWalkBlock)
   const int n_rays = batch_rays(b);
   if (ray0 >= n_rays) return;
   const int cnt = ray < n_rays ? w.seg_count[ray] : 0;
-  const int j0 = static_cast<int>(blockIdx.y) * kPlaceJ;
-  if (j0 >= cnt) return;
+  if (static_cast<int>(blockIdx.y) * kPlaceJ >= cnt) return;
   const GridDesc d = grid_of_ray(b, ray0);
   const long long slot0 = d.slot_base + (ray - d.ray_base);
   const long long stride = d.g.n;
   const int *gbase = w.grp_base + static_cast<size_t>(ray >> lg_group) * hash;
-  int tile[kPlaceJ];
-  unsigned rk[kPlaceJ];
+  for (int j0 = static_cast<int>(blockIdx.y) * kPlaceJ; j0 < cnt;
+       j0 += kPlaceJ * static_cast<int>(gridDim.y)) {
+    int tile[kPlaceJ];
+    unsigned rk[kPlaceJ];
 #pragma unroll
-  for (int q = 0; q < kPlaceJ; ++q) {
-    if (j0 + q < cnt) {
-      tile[q] = w.seg_tile[slot0 + (j0 + q) * stride];
-      rk[q] = w.seg_rank[slot0 + (j0 + q) * stride];
+    for (int q = 0; q < kPlaceJ; ++q) {
+      if (j0 + q < cnt) {
+        tile[q] = w.seg_tile[slot0 + (j0 + q) * stride];
+        rk[q] = w.seg_rank[slot0 + (j0 + q) * stride];
+      }
     }
-  }
 #pragma unroll
-  for (int q = 0; q < kPlaceJ; ++q) {
-    if (j0 + q >= cnt) break;
-    long long pos = w.tile_cursor[tile[q]];
-    pos += (rk[q] & kRankGlobal)
-               ? static_cast<long long>(rk[q] & ~kRankGlobal)
-               : static_cast<long long>(gbase[rk[q] >> kRankBits]) +
-                     static_cast<long long>(rk[q] & ((1u << kRankBits) - 1u));
-    if (pos < w.seg_capacity) {
-      w.order[pos] = static_cast<unsigned>(slot0 + (j0 + q) * stride);
+    for (int q = 0; q < kPlaceJ; ++q) {
+      if (j0 + q >= cnt) break;
+      long long pos = w.tile_cursor[tile[q]];
+      pos += (rk[q] & kRankGlobal)
+                 ? static_cast<long long>(rk[q] & ~kRankGlobal)
+                 : static_cast<long long>(gbase[rk[q] >> kRankBits]) +
+                       static_cast<long long>(rk[q] & ((1u << kRankBits) - 1u));
+      if (pos < w.seg_capacity) {
+        w.order[pos] = static_cast<unsigned>(slot0 + (j0 + q) * stride);
+      }
     }
   }
 }
@@ -2275,8 +2279,10 @@ int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
   tile_apply_kernel<true, kDelta><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
   st = check_launch("tile_apply_kernel<big>");
   if (st != CUBLOX_OK) return st;
+  // A grid-stride loop over the split tiles' slices: a few workgroups per
+  // CU (empty workgroups of a larger grid only cost dispatch time).
   const long long hot_blocks =
-      std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 1 << 16);
+      std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 4096);
   hot_fold_kernel<kDelta><<<dim3(static_cast<unsigned>(hot_blocks)), dim3(256), 0, hs>>>(b, w);
   st = check_launch("hot_fold_kernel");
   if (st != CUBLOX_OK) return st;
@@ -2337,8 +2343,11 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     const bool single = c.max_tiles <= kScanStage;
     const unsigned item_blocks =
         single ? static_cast<unsigned>((c.max_tiles + kWalkBlock - 1) / kWalkBlock) : 0u;
+    // At most kPlaceRows block rows (j chunks of kPlaceJ; a row loops over
+    // its further chunks): rows past a ray's records are empty workgroups,
+    // whose dispatch costs time on large launches.
     const dim3 grid(walk_blocks + item_blocks,
-                    static_cast<unsigned>((c.max_seg + kPlaceJ - 1) / kPlaceJ));
+                    static_cast<unsigned>(std::min((c.max_seg + kPlaceJ - 1) / kPlaceJ, kPlaceRows)));
     place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash,
                                                      static_cast<int>(item_blocks));
     st = check_launch("place_kernel");
