@@ -197,8 +197,9 @@ def run_batched(cb, tune_ticks_each, ticks):
     G = w.extra["grids"]
     tune_ticks = len(TUNE_CANDIDATES) * (tune_ticks_each + 1)
     rc_ticks = 8  # after the timed ticks, without graph: recenter timed alone
-    total = tune_ticks + ticks + rc_ticks
-    timed = range(tune_ticks, tune_ticks + ticks)
+    total = tune_ticks + 2 * ticks + rc_ticks
+    timed = range(tune_ticks, tune_ticks + ticks)  # one tick at a time
+    piped = range(tune_ticks + ticks, tune_ticks + 2 * ticks)  # overlapped ticks
     draws = [batched_grid(w, g) for g in range(G)]
     t = time.perf_counter()
     shapes = [[batched_scan_shape(w, g, k)[:2] for k in range(total)] for g in range(G)]
@@ -237,9 +238,9 @@ def run_batched(cb, tune_ticks_each, ticks):
             c0 = time.perf_counter()
             plain[g].update_sensor_frame_device(b[0].ptr, b[1].ptr, b[2].ptr, s.n, s.T)
             plain[g].recenter(s.base)
-            if k in timed:
+            if k in timed or k in piped:
                 per_scan.append((time.perf_counter() - c0) * 1e3)
-        if k in timed:
+        if k in timed or k in piped:
             tick_ms.append((time.perf_counter() - a) * 1e3)
     for x in plain:
         x.close()
@@ -251,11 +252,22 @@ def run_batched(cb, tune_ticks_each, ticks):
     batch.set_graph_mode(True)
     batch.set_tuning(TUNE_CANDIDATES, tune_ticks_each)
     btick, bdev, zeroed, rc_ms, rc_vox = [], [], [], [], []
+    pipe_ms = 0.0
     for k in range(total):
-        if k == tune_ticks + ticks:
-            batch.set_graph_mode(False)
         sc = [(bg[g], *bufs[g][k], scans[g][k].n, scans[g][k].T) for g in range(G)]
         rc = [(bg[g], scans[g][k].base) for g in range(G)]
+        if k in piped:
+            # Overlapped ticks: each submit waits only for the tick two back.
+            if k == piped[0]:
+                batch.set_pipelined(True)
+                a0 = time.perf_counter()
+            batch.submit(sc, rc)
+            if k == piped[-1]:
+                batch.wait()
+                pipe_ms = (time.perf_counter() - a0) * 1e3
+                batch.set_pipelined(False)
+                batch.set_graph_mode(False)
+            continue
         a = time.perf_counter()
         batch.tick(sc, rc)
         if k in timed:
@@ -267,6 +279,7 @@ def run_batched(cb, tune_ticks_each, ticks):
             rc_vox.append(batch.last[2])
             records = batch.counters()["records"]
     batch_sps = G * len(btick) / (sum(btick) * 1e-3)
+    pipe_sps = G * len(piped) / (pipe_ms * 1e-3)
     # Batched recenter: 4 B per zeroed voxel over the recenter sequence's
     # device time (descriptor copy + one zeroing launch for all grids).
     # Only ticks whose recenters moved some window count.
@@ -294,7 +307,13 @@ def run_batched(cb, tune_ticks_each, ticks):
                      tuning_ms={"/".join(map(str, c)): round(m, 4)
                                 for c, m in zip(TUNE_CANDIDATES, tune)},
                      tuning_keys="near_steps/chunk/lanes_per_ray/hot_mode"),
-        speedup_scans_per_s=round(batch_sps / plain_sps, 2),
+        pipelined=dict(scans_per_s=round(pipe_sps, 1), ticks=len(piped),
+                       ms_per_tick=round(pipe_ms / len(piped), 4), graph=True,
+                       form="consecutive ticks overlapped: each tick's walk, binning, "
+                            "tile scan and placing on one of two streams beside the "
+                            "previous tick's folds and recenters"),
+        speedup_scans_per_s=round(max(batch_sps, pipe_sps) / plain_sps, 2),
+        speedup_one_tick_at_a_time=round(batch_sps / plain_sps, 2),
         recenter=dict(zeroed_voxels_per_tick=round(statistics.mean(zeroed)),
                       batched_recenter_ms=round(statistics.mean(rc_ms), 4),
                       batched_recenter_zeroed_voxels=round(statistics.mean(rc_vox)),

@@ -248,6 +248,10 @@ def host() -> C.CDLL:
     _sig(L, "cublox_gb_tile_count", i, vp)
     _sig(L, "cublox_gb_tick", i, vp, i, pp, pp, pp, pp, i32p, f32p, i32p, f32p,
          i, pp, f32p)
+    _sig(L, "cublox_gb_submit", i, vp, i, pp, pp, pp, pp, i32p, f32p, i32p, f32p,
+         i, pp, f32p)
+    _sig(L, "cublox_gb_wait", i, vp)
+    _sig(L, "cublox_gb_set_pipelined", i, vp, i)
     _sig(L, "cublox_gb_set_graph_mode", i, vp, i)
     _sig(L, "cublox_gb_set_tuning", i, vp, i, i32p, i)
     _sig(L, "cublox_gb_tuning", i, vp, i32p)

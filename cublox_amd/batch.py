@@ -66,7 +66,29 @@ class GridBatch:
     def tick(self, scans: Sequence[Tuple], recenters: Iterable[Tuple] = ()) -> None:
         """scans: (grid, d_x, d_y, d_z, n, T 3x4 sensor->world[, point
         stride[, range drop squared]]) with device buffers (objects with
-        .ptr or ints); recenters: (grid, pos)."""
+        .ptr or ints); recenters: (grid, pos). Blocks until done."""
+        self._tick(self._L.cublox_gb_tick, scans, recenters)
+
+    def submit(self, scans: Sequence[Tuple], recenters: Iterable[Tuple] = ()) -> None:
+        """tick() without the wait: returns once the tick is launched (when
+        pipelined, after the tick submitted two before it has finished).
+        The scans' device buffers must live until wait()."""
+        self._tick(self._L.cublox_gb_submit, scans, recenters)
+
+    def wait(self) -> None:
+        """Blocks until every submitted tick has finished."""
+        self._check(self._L.cublox_gb_wait(self._h))
+
+    def set_pipelined(self, on: bool) -> None:
+        """Overlap consecutive ticks (a second stream and workspace)."""
+        self._check(self._L.cublox_gb_set_pipelined(self._h, 1 if on else 0))
+        self._pipelined = bool(on)
+
+    @property
+    def pipelined(self) -> bool:
+        return getattr(self, "_pipelined", False)
+
+    def _tick(self, fn, scans: Sequence[Tuple], recenters: Iterable[Tuple]) -> None:
         scans = list(scans)
         recenters = list(recenters)
         k = len(scans)
@@ -88,7 +110,7 @@ class GridBatch:
         rp = np.ascontiguousarray(
             np.array([np.asarray(r[1], np.float32).reshape(3) for r in recenters]
                      or [np.zeros(3, np.float32)], np.float32))
-        self._check(self._L.cublox_gb_tick(
+        self._check(fn(
             self._h, k, grids, xs, ys, zs, _ffi.iptr(n), _ffi.fptr(T),
             _ffi.iptr(stride), _ffi.fptr(drop), len(recenters), rg, _ffi.fptr(rp)))
 

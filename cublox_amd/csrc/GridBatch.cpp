@@ -71,17 +71,9 @@ GridBatch::GridBatch(const BatchLimits &limits) : lim_(limits) {
   setting_.chunk = std::max(setting_.chunk, lim_.min_chunk);
   max_regions_ = lim_.max_regions > 0 ? lim_.max_regions : 6 * lim_.max_grids;
   const cublox_batch_limits c = abiLimits(lim_);
+  ws_bytes_ = cublox_integrate_batch_workspace_bytes(&c);
   try {
-    CUBLOX_CHECK(cublox_stream_create(&stream_));
-    for (auto &e : ev_) CUBLOX_CHECK(cublox_event_create(&e));
-    ws_bytes_ = cublox_integrate_batch_workspace_bytes(&c);
-    CUBLOX_CHECK(cublox_malloc(&d_ws_, ws_bytes_));
-    CUBLOX_CHECK(cublox_integrate_workspace_init(d_ws_, ws_bytes_, stream_));
-    CUBLOX_CHECK(cublox_host_alloc(&h_stage_, cublox_integrate_batch_staging_bytes(&c)));
-    const size_t rb = cublox_recenter_batch_staging_bytes(max_regions_);
-    CUBLOX_CHECK(cublox_host_alloc(&h_rstage_, rb));
-    CUBLOX_CHECK(cublox_malloc(&d_rarea_, rb));
-    CUBLOX_CHECK(cublox_stream_synchronize(stream_));
+    allocSlot(slots_[0]);
   } catch (...) {
     release();
     throw;
@@ -90,30 +82,59 @@ GridBatch::GridBatch(const BatchLimits &limits) : lim_(limits) {
 
 GridBatch::~GridBatch() { release(); }
 
-void GridBatch::release() noexcept {
-  if (stream_) cublox_stream_synchronize(stream_);
-  for (auto &g : graphs_) {
-    if (g) cublox_graph_destroy(g);
-    g = nullptr;
+void GridBatch::allocSlot(Slot &s) {
+  const cublox_batch_limits c = abiLimits(lim_);
+  CUBLOX_CHECK(cublox_stream_create(&s.stream));
+  for (auto &e : s.ev) CUBLOX_CHECK(cublox_event_create(&e));
+  CUBLOX_CHECK(cublox_malloc(&s.d_ws, ws_bytes_));
+  CUBLOX_CHECK(cublox_integrate_workspace_init(s.d_ws, ws_bytes_, s.stream));
+  CUBLOX_CHECK(cublox_host_alloc(&s.h_stage, cublox_integrate_batch_staging_bytes(&c)));
+  const size_t rb = cublox_recenter_batch_staging_bytes(max_regions_);
+  CUBLOX_CHECK(cublox_host_alloc(&s.h_rstage, rb));
+  CUBLOX_CHECK(cublox_malloc(&s.d_rarea, rb));
+  CUBLOX_CHECK(cublox_stream_synchronize(s.stream));
+}
+
+void GridBatch::dropGraphs(Slot &s) {
+  for (void **set : {s.front, s.back}) {
+    for (int k = 0; k < 2; ++k) {
+      if (set[k]) cublox_graph_destroy(set[k]);
+      set[k] = nullptr;
+    }
   }
-  if (d_ws_) cublox_free(d_ws_);
-  if (d_rarea_) cublox_free(d_rarea_);
-  if (h_stage_) cublox_host_free(h_stage_);
-  if (h_rstage_) cublox_host_free(h_rstage_);
-  d_ws_ = d_rarea_ = h_stage_ = h_rstage_ = nullptr;
-  for (auto &e : ev_) {
+}
+
+void GridBatch::releaseSlot(Slot &s) noexcept {
+  if (s.stream) cublox_stream_synchronize(s.stream);
+  dropGraphs(s);
+  if (s.d_ws) cublox_free(s.d_ws);
+  if (s.d_rarea) cublox_free(s.d_rarea);
+  if (s.h_stage) cublox_host_free(s.h_stage);
+  if (s.h_rstage) cublox_host_free(s.h_rstage);
+  s.d_ws = s.d_rarea = s.h_stage = s.h_rstage = nullptr;
+  for (auto &e : s.ev) {
     if (e) cublox_event_destroy(e);
     e = nullptr;
   }
-  if (stream_) cublox_stream_destroy(stream_);
-  stream_ = nullptr;
+  if (s.stream) cublox_stream_destroy(s.stream);
+  s.stream = nullptr;
+  s.in_flight = false;
+}
+
+void GridBatch::release() noexcept {
+  // Both streams first: a tick's folds may wait on the other slot.
+  for (auto &s : slots_) {
+    if (s.stream) cublox_stream_synchronize(s.stream);
+  }
+  for (auto &s : slots_) releaseSlot(s);
 }
 
 std::array<int, 5> GridBatch::lastCounters() const {
   std::array<int, 5> out{};
   const cublox_batch_limits c = abiLimits(lim_);
-  CUBLOX_CHECK(cublox_integrate_batch_counters(d_ws_, ws_bytes_, &c, out.data(),
-                                               stream_));
+  const Slot &s = slots_[cur_];
+  CUBLOX_CHECK(cublox_integrate_batch_counters(s.d_ws, ws_bytes_, &c, out.data(),
+                                               s.stream));
   return out;
 }
 
@@ -129,13 +150,41 @@ int GridBatch::tileCount(const OccupancyGrid &g) {
 
 void GridBatch::setGraphMode(bool on) {
   if (!on) {
-    CUBLOX_CHECK(cublox_stream_synchronize(stream_));
-    for (auto &g : graphs_) {
-      if (g) CUBLOX_CHECK(cublox_graph_destroy(g));
-      g = nullptr;
+    wait();
+    for (auto &s : slots_) {
+      if (!s.stream) continue;
+      CUBLOX_CHECK(cublox_stream_synchronize(s.stream));
+      dropGraphs(s);
     }
   }
   graph_mode_ = on;
+}
+
+void GridBatch::setPipelined(bool on) {
+  if (on && lim_.changed_voxels) {
+    throw std::invalid_argument(
+        "cublox::GridBatch: pipelined ticks do not feed changed-voxel streams");
+  }
+  wait();
+  if (on) {
+    if (!slots_[1].stream) {
+      try {
+        allocSlot(slots_[1]);
+      } catch (...) {
+        releaseSlot(slots_[1]);
+        throw;
+      }
+    }
+    n_slots_ = 2;
+  } else {
+    releaseSlot(slots_[1]);
+    n_slots_ = 1;
+    cur_ = 0;
+  }
+}
+
+bool GridBatch::inFlight() const {
+  return slots_[0].in_flight || slots_[1].in_flight;
 }
 
 static bool settingValid(const TuneSetting &s, const BatchLimits &lim) {
@@ -175,10 +224,10 @@ std::vector<std::pair<TuneSetting, double>> GridBatch::tuneResults() const {
   return out;
 }
 
-// Host staging of one tick: the update descriptors (pinning each grid's
-// first origin, as update() does) and the recenter plans.
+// Host staging of one tick into slot s: the update descriptors (pinning
+// each grid's first origin, as update() does) and the recenter plans.
 void GridBatch::stage(
-    const std::vector<GridScan> &scans,
+    Slot &slot, const std::vector<GridScan> &scans,
     const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
   std::vector<cublox_integrate_grid> items(scans.size());
   for (const GridScan &s : scans) {
@@ -229,11 +278,12 @@ void GridBatch::stage(
   tune.lanes_per_ray = setting_.lanes;
   tune.hot_mode = setting_.hot_mode;
   CUBLOX_CHECK(cublox_integrate_batch_stage_tuned(
-      items.data(), static_cast<int>(items.size()), &c, &tune, h_stage_));
-  lanes_ = tune.lanes_per_ray;
+      items.data(), static_cast<int>(items.size()), &c, &tune, slot.h_stage));
+  slot.lanes = tune.lanes_per_ray;
+  slot.hot_mode = setting_.hot_mode;
 
   // OccupancyGrid.cpp:141-179 per grid: threshold pre-check on the host,
-  // then the plan; origins move once the tick has run.
+  // then the plan; origins move once the tick is launched.
   std::vector<cublox_recenter_item> ritems;
   std::vector<OccupancyGrid *> rgrids;
   for (const auto &[gp, pos] : recenters) {
@@ -259,82 +309,99 @@ void GridBatch::stage(
   }
   std::vector<cublox_recenter_result> res(ritems.size());
   CUBLOX_CHECK(cublox_recenter_batch_stage(ritems.data(), static_cast<int>(ritems.size()),
-                                           max_regions_, h_rstage_, res.data()));
+                                           max_regions_, slot.h_rstage, res.data()));
   rc_grids_.clear();
   rc_origins_.clear();
-  rc_zeroed_.clear();
+  slot.zeroed = 0;
   for (size_t k = 0; k < res.size(); ++k) {
     if (res[k].action == 0) continue;
     rc_grids_.push_back(rgrids[k]);
     rc_origins_.emplace_back(res[k].new_origin_i.x, res[k].new_origin_i.y,
                              res[k].new_origin_i.z);
-    rc_zeroed_.push_back(res[k].zeroed_voxels);
+    slot.zeroed += res[k].zeroed_voxels;
   }
 }
 
-void GridBatch::launch() {
+// Enqueues slot q's staged tick on its stream. prev >= 0: the slot of the
+// tick submitted before it, on the other stream; its stream is marked now
+// (after everything enqueued there so far, that tick's folds and recenter
+// and any work the caller added) and this tick's folds wait for the mark.
+// The first half does not touch occupancy, so it runs beside that work.
+void GridBatch::launch(int q, int prev) {
+  Slot &s = slots_[q];
   const cublox_batch_limits c = abiLimits(lim_);
-  auto enqueue = [&]() {
+  auto front = [&]() {
     // Both staging copies first, so the recenter sequence is one launch.
-    CUBLOX_CHECK(cublox_recenter_batch_upload(h_rstage_, d_rarea_, max_regions_, stream_));
-    CUBLOX_CHECK(cublox_launch_integrate_batch_staged(h_stage_, d_ws_, ws_bytes_, &c,
-                                                      stream_));
-    if (!graph_mode_) CUBLOX_CHECK(cublox_event_record(ev_[1], stream_));
-    CUBLOX_CHECK(cublox_launch_recenter_batch_uploaded(d_rarea_, max_regions_, stream_));
+    CUBLOX_CHECK(cublox_recenter_batch_upload(s.h_rstage, s.d_rarea, max_regions_, s.stream));
+    CUBLOX_CHECK(cublox_launch_integrate_batch_staged_part(s.h_stage, s.d_ws, ws_bytes_, &c, 1,
+                                                           s.stream));
   };
-  CUBLOX_CHECK(cublox_event_record(ev_[0], stream_));
-  if (graph_mode_) {
-    void *&graph_ = graphs_[(lanes_ == 4 ? 1 : 0) + 2 * (setting_.hot_mode ? 1 : 0)];
-    if (!graph_) {
-      CUBLOX_CHECK(cublox_graph_begin_capture(stream_));
+  auto back = [&]() {
+    CUBLOX_CHECK(cublox_launch_integrate_batch_staged_part(s.h_stage, s.d_ws, ws_bytes_, &c, 2,
+                                                           s.stream));
+    if (!graph_mode_) CUBLOX_CHECK(cublox_event_record(s.ev[1], s.stream));
+    CUBLOX_CHECK(cublox_launch_recenter_batch_uploaded(s.d_rarea, max_regions_, s.stream));
+  };
+  // Runs a part directly, or as its captured graph (captured on first use).
+  auto run = [&](auto &&part, void *&graph) {
+    if (!graph_mode_) {
+      part();
+      return;
+    }
+    if (!graph) {
+      CUBLOX_CHECK(cublox_graph_begin_capture(s.stream));
       try {
-        enqueue();
+        part();
       } catch (...) {
         cublox_graph g = nullptr;
-        cublox_graph_end_capture(stream_, &g);  // leave capture mode
+        cublox_graph_end_capture(s.stream, &g);  // leave capture mode
         if (g) cublox_graph_destroy(g);
         throw;
       }
-      CUBLOX_CHECK(cublox_graph_end_capture(stream_, &graph_));
+      CUBLOX_CHECK(cublox_graph_end_capture(s.stream, &graph));
     }
-    CUBLOX_CHECK(cublox_graph_launch(graph_, stream_));
-  } else {
-    enqueue();
-  }
-  CUBLOX_CHECK(cublox_event_record(ev_[2], stream_));
+    CUBLOX_CHECK(cublox_graph_launch(graph, s.stream));
+  };
+  if (prev >= 0) CUBLOX_CHECK(cublox_event_record(slots_[prev].ev[3], slots_[prev].stream));
+  CUBLOX_CHECK(cublox_event_record(s.ev[0], s.stream));
+  run(front, s.front[s.lanes == 4 ? 1 : 0]);
+  if (prev >= 0) CUBLOX_CHECK(cublox_stream_wait_event(s.stream, slots_[prev].ev[3]));
+  run(back, s.back[s.hot_mode ? 1 : 0]);
+  CUBLOX_CHECK(cublox_event_record(s.ev[2], s.stream));
 }
 
-void GridBatch::finish() {
-  CUBLOX_CHECK(cublox_stream_synchronize(stream_));
+void GridBatch::finish(int q) {
+  Slot &s = slots_[q];
+  s.in_flight = false;
+  CUBLOX_CHECK(cublox_stream_synchronize(s.stream));
   float total = 0.0f, upd = 0.0f;
-  CUBLOX_CHECK(cublox_event_elapsed_ms(ev_[0], ev_[2], &total));
+  CUBLOX_CHECK(cublox_event_elapsed_ms(s.ev[0], s.ev[2], &total));
   if (!graph_mode_) {
-    CUBLOX_CHECK(cublox_event_elapsed_ms(ev_[0], ev_[1], &upd));
+    CUBLOX_CHECK(cublox_event_elapsed_ms(s.ev[0], s.ev[1], &upd));
   } else {
     upd = total;
   }
   last_update_ms_ = upd;
   last_recenter_ms_ = graph_mode_ ? 0.0 : static_cast<double>(total - upd);
-  last_zeroed_ = 0;
-  for (size_t k = 0; k < rc_grids_.size(); ++k) last_zeroed_ += rc_zeroed_[k];
-  if (tuning()) {
-    // One warm-up tick per candidate, then ticks_each measured ones.
-    if (tune_count_ > 0) {
-      tune_ms_[tune_pos_] += total;
-      tune_n_[tune_pos_] += 1;
-    }
-    if (++tune_count_ > tune_ticks_) {
-      tune_count_ = 0;
-      if (++tune_pos_ == tune_.size()) {
-        size_t best = 0;
-        for (size_t k = 1; k < tune_.size(); ++k) {
-          if (tune_ms_[k] / std::max(tune_n_[k], 1) <
-              tune_ms_[best] / std::max(tune_n_[best], 1)) {
-            best = k;
-          }
+  last_zeroed_ = s.zeroed;
+  if (s.tune_idx < 0 || !tuning() || static_cast<size_t>(s.tune_idx) != tune_pos_) return;
+  // One warm-up tick per candidate, then ticks_each measured ones; a tick
+  // that shared the GPU with another tick is not measured.
+  if (tune_count_ > 0 && s.alone) {
+    tune_ms_[tune_pos_] += total;
+    tune_n_[tune_pos_] += 1;
+  }
+  if (++tune_count_ > tune_ticks_) {
+    tune_count_ = 0;
+    if (++tune_pos_ == tune_.size()) {
+      size_t best = 0;
+      for (size_t k = 1; k < tune_.size(); ++k) {
+        if (tune_ms_[k] / std::max(tune_n_[k], 1) <
+            tune_ms_[best] / std::max(tune_n_[best], 1)) {
+          best = k;
         }
-        setting_ = tune_[best];
       }
+      setting_ = tune_[best];
     }
   }
 }
@@ -342,16 +409,32 @@ void GridBatch::finish() {
 void GridBatch::submit(
     const std::vector<GridScan> &scans,
     const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {
-  if (in_flight_) {
+  if (n_slots_ == 1 && slots_[0].in_flight) {
     throw std::logic_error("cublox::GridBatch: submit() before wait() of the last tick");
   }
   if (static_cast<int>(scans.size()) > lim_.max_grids) {
     throw std::invalid_argument("cublox::GridBatch: more scans than max_grids");
   }
-  if (tuning()) setting_ = tune_[tune_pos_];
-  stage(scans, recenters);
-  launch();
-  in_flight_ = true;
+  int tune_idx = -1;
+  if (tuning()) {
+    wait();  // an autotuning tick runs alone
+    if (tuning()) {
+      setting_ = tune_[tune_pos_];
+      tune_idx = static_cast<int>(tune_pos_);
+    }
+  }
+  const int q = n_slots_ == 1 ? 0 : 1 - cur_;
+  if (slots_[q].in_flight) finish(q);  // the tick two submits back
+  Slot &s = slots_[q];
+  stage(s, scans, recenters);
+  const int prev = q != cur_ ? cur_ : -1;
+  const bool overlap = prev >= 0 && slots_[prev].in_flight;
+  if (overlap) slots_[prev].alone = false;
+  launch(q, prev);
+  s.in_flight = true;
+  s.alone = !overlap;
+  s.tune_idx = tune_idx;
+  cur_ = q;
   // The grids' host state describes them after all work enqueued so far.
   for (size_t k = 0; k < rc_grids_.size(); ++k) {
     rc_grids_[k]->updateOriginAndBound(rc_origins_[k]);
@@ -359,9 +442,9 @@ void GridBatch::submit(
 }
 
 void GridBatch::wait() {
-  if (!in_flight_) return;
-  in_flight_ = false;
-  finish();
+  // The older tick first.
+  if (n_slots_ == 2 && slots_[1 - cur_].in_flight) finish(1 - cur_);
+  if (slots_[cur_].in_flight) finish(cur_);
 }
 
 void GridBatch::tick(

@@ -2291,13 +2291,23 @@ int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
   return check_launch("near_apply_kernel");
 }
 
+// Parts of a launch sequence: the kernels that read no occupancy (walk,
+// record counting, tile scan, placing) and the folds.
+constexpr int kPartFront = 1, kPartBack = 2, kPartBoth = 3;
+
 // The launch sequence for a workspace of layout L. Launch shapes depend on
 // the capacities only (so the sequence can be captured once and replayed);
 // the kernels read the actual counts from the batch. delta selects the fold
 // variants that feed changed-voxel streams (grids without one skip it).
+// `parts` selects the front, the folds or both.
 int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
                     bool delta, hipStream_t hs, hipEvent_t occ_ready = nullptr,
-                    int batch_lanes = 0, bool shared = false) {
+                    int batch_lanes = 0, bool shared = false,
+                    int parts = kPartBoth) {
+  if (!(parts & kPartFront)) {
+    return delta ? launch_folds<true>(b, L, w, hs, shared)
+                 : launch_folds<false>(b, L, w, hs, shared);
+  }
   const Caps &c = L.cap;
   const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
@@ -2353,6 +2363,7 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     st = check_launch("place_kernel");
   }
   if (st != CUBLOX_OK) return st;
+  if (!(parts & kPartBack)) return CUBLOX_OK;
   // Nothing above reads or writes occ: the folds wait for it here.
   if (occ_ready) CUBLOX_TRY(hipStreamWaitEvent(hs, occ_ready, 0));
   return delta ? launch_folds<true>(b, L, w, hs, shared)
@@ -2723,11 +2734,12 @@ int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n
   return CUBLOX_OK;
 }
 
-int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
-                                         size_t ws_bytes,
-                                         const cublox_batch_limits *lim,
-                                         cublox_stream s) {
-  if (!lim || !h_staging || lim->max_tiles > kMaxBatchTiles || !caps_fit(batch_caps(*lim))) {
+int cublox_launch_integrate_batch_staged_part(const void *h_staging, void *d_ws,
+                                              size_t ws_bytes,
+                                              const cublox_batch_limits *lim,
+                                              int part, cublox_stream s) {
+  if (!lim || !h_staging || lim->max_tiles > kMaxBatchTiles || !caps_fit(batch_caps(*lim)) ||
+      part < kPartFront || part > kPartBoth) {
     return set_error(CUBLOX_ERR_INVALID, "integrate_batch: bad arguments");
   }
   const Layout L = make_layout(batch_caps(*lim));
@@ -2736,17 +2748,27 @@ int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
   }
   hipStream_t hs = as_stream(s);
   char *base = static_cast<char *>(d_ws);
-  const int st = check(hipMemcpyAsync(base + L.off_hdr, h_staging,
-                                      staging_bytes(L.cap.max_grids),
-                                      hipMemcpyHostToDevice, hs),
-                       "hipMemcpyAsync(batch descriptors)");
-  if (st != CUBLOX_OK) return st;
+  if (part & kPartFront) {
+    const int st = check(hipMemcpyAsync(base + L.off_hdr, h_staging,
+                                        staging_bytes(L.cap.max_grids),
+                                        hipMemcpyHostToDevice, hs),
+                         "hipMemcpyAsync(batch descriptors)");
+    if (st != CUBLOX_OK) return st;
+  }
   Batch b{};
   b.descs = reinterpret_cast<const GridDesc *>(base + L.off_desc);
   b.hdr = reinterpret_cast<const BatchHeader *>(base + L.off_hdr);
   const BatchHeader &h = static_cast<const Staging *>(h_staging)->hdr;
   return launch_sequence(b, L, bind(L, d_ws), lim->delta != 0, hs, nullptr,
-                         h.lanes == 4 ? 4 : 1, h.shared != 0);
+                         h.lanes == 4 ? 4 : 1, h.shared != 0, part);
+}
+
+int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
+                                         size_t ws_bytes,
+                                         const cublox_batch_limits *lim,
+                                         cublox_stream s) {
+  return cublox_launch_integrate_batch_staged_part(h_staging, d_ws, ws_bytes, lim,
+                                                   kPartBoth, s);
 }
 
 int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,

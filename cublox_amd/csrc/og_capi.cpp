@@ -440,6 +440,37 @@ int cublox_gb_tile_count(const cublox_og *h) {
   return cublox::GridBatch::tileCount(*OGC);
 }
 
+namespace {
+// The tick arguments of cublox_gb_tick / cublox_gb_submit as GridBatch takes them.
+struct TickArgs {
+  std::vector<cublox::GridScan> scans;
+  std::vector<std::pair<OG *, Eigen::Vector3f>> rc;
+};
+TickArgs tick_args(int n_scans, cublox_og *const *grids, const float *const *d_x,
+                   const float *const *d_y, const float *const *d_z, const int *n,
+                   const float *T, const int *point_stride, const float *range_drop_sq,
+                   int n_recenter, cublox_og *const *rgrids, const float *rpos) {
+  TickArgs a;
+  a.scans.resize(static_cast<size_t>(std::max(n_scans, 0)));
+  for (int k = 0; k < n_scans; ++k) {
+    cublox::GridScan &s = a.scans[static_cast<size_t>(k)];
+    s.grid = reinterpret_cast<OG *>(grids[k]);
+    s.d_x = d_x[k];
+    s.d_y = d_y[k];
+    s.d_z = d_z[k];
+    s.n = n[k];
+    s.sensor_frame = true;
+    for (int i = 0; i < 12; ++i) s.T[static_cast<size_t>(i)] = T[12 * k + i];
+    if (point_stride) s.point_stride = point_stride[k];
+    if (range_drop_sq) s.range_drop_sq = range_drop_sq[k];
+  }
+  for (int k = 0; k < n_recenter; ++k) {
+    a.rc.emplace_back(reinterpret_cast<OG *>(rgrids[k]), v3(rpos + 3 * k));
+  }
+  return a;
+}
+}  // namespace
+
 int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
                    const float *const *d_x, const float *const *d_y,
                    const float *const *d_z, const int *n, const float *T,
@@ -447,25 +478,31 @@ int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
                    int n_recenter, cublox_og *const *rgrids,
                    const float *rpos) {
   return guarded([&] {
-    std::vector<cublox::GridScan> scans(static_cast<size_t>(std::max(n_scans, 0)));
-    for (int k = 0; k < n_scans; ++k) {
-      cublox::GridScan &s = scans[static_cast<size_t>(k)];
-      s.grid = reinterpret_cast<OG *>(grids[k]);
-      s.d_x = d_x[k];
-      s.d_y = d_y[k];
-      s.d_z = d_z[k];
-      s.n = n[k];
-      s.sensor_frame = true;
-      for (int i = 0; i < 12; ++i) s.T[static_cast<size_t>(i)] = T[12 * k + i];
-      if (point_stride) s.point_stride = point_stride[k];
-      if (range_drop_sq) s.range_drop_sq = range_drop_sq[k];
-    }
-    std::vector<std::pair<OG *, Eigen::Vector3f>> rc;
-    for (int k = 0; k < n_recenter; ++k) {
-      rc.emplace_back(reinterpret_cast<OG *>(rgrids[k]), v3(rpos + 3 * k));
-    }
-    GBP->tick(scans, rc);
+    const TickArgs a = tick_args(n_scans, grids, d_x, d_y, d_z, n, T, point_stride,
+                                 range_drop_sq, n_recenter, rgrids, rpos);
+    GBP->tick(a.scans, a.rc);
   });
+}
+
+int cublox_gb_submit(cublox_gb *b, int n_scans, cublox_og *const *grids,
+                     const float *const *d_x, const float *const *d_y,
+                     const float *const *d_z, const int *n, const float *T,
+                     const int *point_stride, const float *range_drop_sq,
+                     int n_recenter, cublox_og *const *rgrids,
+                     const float *rpos) {
+  return guarded([&] {
+    const TickArgs a = tick_args(n_scans, grids, d_x, d_y, d_z, n, T, point_stride,
+                                 range_drop_sq, n_recenter, rgrids, rpos);
+    GBP->submit(a.scans, a.rc);
+  });
+}
+
+int cublox_gb_wait(cublox_gb *b) {
+  return guarded([&] { GBP->wait(); });
+}
+
+int cublox_gb_set_pipelined(cublox_gb *b, int on) {
+  return guarded([&] { GBP->setPipelined(on != 0); });
 }
 
 int cublox_gb_set_graph_mode(cublox_gb *b, int on) {

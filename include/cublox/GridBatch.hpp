@@ -7,8 +7,14 @@
 // through the same kernels, so a tick fills the GPU with any mix of grids.
 // Each grid ends the tick exactly as its own update() + recenter() would
 // leave it. The per-grid descriptors travel in one host->device copy at the
-// head of each sequence; with graph mode on, the two sequences are captured
+// head of each sequence; with graph mode on, the sequences are captured
 // once and each tick restages and replays them.
+//
+// Pipelined mode overlaps consecutive ticks: each tick's first half (ray
+// walk, record counting, tile scan, placing: no occupancy access) runs on
+// one of two streams and workspaces beside the folds of the tick before,
+// and every tick's folds and recenters wait for the previous tick's, so
+// every grid still sees its updates in tick order.
 //
 // The online autotuner cycles candidate settings over the first ticks
 // (results do not depend on them) and keeps the fastest by device time.
@@ -92,14 +98,23 @@ public:
   // tick() in two halves: submit() stages and launches the tick on stream()
   // and returns at once, with the recentered grids' origins already moved
   // (work the caller enqueues on stream() afterwards sees the grids after
-  // the tick); wait() blocks until the stream is idle. Every submit() needs
-  // its wait() before the next submit(), and the scans' device columns must
-  // stay valid until then.
+  // the tick); wait() blocks until every submitted tick has finished. Without
+  // pipelining every submit() needs its wait() before the next submit();
+  // pipelined, a submit() first waits for the tick submitted two before it,
+  // so at most two ticks are in flight. The scans' device columns must stay
+  // valid until their tick has finished, and the grids must not be used
+  // directly while a tick is in flight.
   void submit(const std::vector<GridScan> &scans,
               const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
                   &recenters);
   void wait();
-  bool inFlight() const { return in_flight_; }
+  bool inFlight() const;
+
+  // Pipelined ticks (a second stream and workspace, allocated when first
+  // switched on). Not with changed-voxel streams: a tick's stream restarts
+  // in its first half. Autotuning ticks run one at a time either way.
+  void setPipelined(bool on);
+  bool pipelined() const { return n_slots_ == 2; }
 
   // Capture the tick's launch sequences in a HIP graph on the next tick and
   // replay it afterwards.
@@ -115,44 +130,60 @@ public:
   void setSetting(const TuneSetting &s);
   std::vector<std::pair<TuneSetting, double>> tuneResults() const;
 
-  // Device time of the last tick's update and recenter sequences [ms],
-  // and the voxels its recenters zeroed.
+  // Device time of the last finished tick's update and recenter sequences
+  // [ms] (span on its stream, overlap included), and the voxels its
+  // recenters zeroed.
   double lastUpdateMs() const { return last_update_ms_; }
   double lastRecenterMs() const { return last_recenter_ms_; }
   long long lastZeroedVoxels() const { return last_zeroed_; }
   // Counters of the last tick's update: work items, tile records, overflow
   // flag, split hot tiles, unsplit hot tiles.
   std::array<int, 5> lastCounters() const;
-  void *stream() const { return stream_; }
+  // The stream of the last submitted tick.
+  void *stream() const { return slots_[cur_].stream; }
 
 private:
-  void stage(const std::vector<GridScan> &scans,
+  // A tick's resources: stream, workspace, staging and captured sequences.
+  struct Slot {
+    void *stream{nullptr};
+    void *d_ws{nullptr};
+    void *h_stage{nullptr};   // integrate staging (pinned)
+    void *h_rstage{nullptr};  // recenter staging (pinned)
+    void *d_rarea{nullptr};   // recenter staging, device copy
+    // Captured first halves by walk (lanes per ray 1, 4) and folds +
+    // recenter by hot-tile mode: a staged launch holds the kernels its
+    // tick's setting names.
+    void *front[2]{nullptr, nullptr};
+    void *back[2]{nullptr, nullptr};
+    void *ev[4]{nullptr, nullptr, nullptr, nullptr};  // start, folds done, end, order mark
+    bool in_flight{false};
+    bool alone{true};     // no other tick overlapped it
+    int tune_idx{-1};     // autotuner candidate it ran, or -1
+    int lanes{1};
+    int hot_mode{0};
+    long long zeroed{0};  // voxels its recenters zero
+  };
+  void allocSlot(Slot &s);
+  void releaseSlot(Slot &s) noexcept;
+  void dropGraphs(Slot &s);
+  void stage(Slot &s, const std::vector<GridScan> &scans,
              const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
                  &recenters);
-  void launch();
-  void finish();
+  void launch(int q, int prev);
+  void finish(int q);
   void release() noexcept;
 
   BatchLimits lim_;
-  void *stream_{nullptr};
-  void *d_ws_{nullptr};
   size_t ws_bytes_{0};
-  void *h_stage_{nullptr};      // integrate staging (pinned)
-  void *h_rstage_{nullptr};     // recenter staging (pinned)
-  void *d_rarea_{nullptr};      // recenter staging, device copy
   int max_regions_{0};
-  // Captured sequences, one per (walk: lanes per ray 1 or 4, hot-tile
-  // mode): the staged launch holds the kernels the tick's setting names.
-  void *graphs_[4]{nullptr, nullptr, nullptr, nullptr};
-  int lanes_{1};  // the walk of the staged tick
+  Slot slots_[2];
+  int n_slots_{1};
+  int cur_{0};  // slot of the last submitted tick
   bool graph_mode_{false};
-  bool in_flight_{false};
-  void *ev_[3]{nullptr, nullptr, nullptr};
 
   // Recenter outcomes of the staged tick, applied to the grids after it.
   std::vector<OccupancyGrid *> rc_grids_;
   std::vector<Eigen::Vector3i> rc_origins_;
-  std::vector<long long> rc_zeroed_;
 
   TuneSetting setting_{};
   std::vector<TuneSetting> tune_;

@@ -541,6 +541,18 @@ int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,
                                          size_t ws_bytes,
                                          const cublox_batch_limits *lim,
                                          cublox_stream s);
+/* The staged launch in parts (no reference counterpart): part 1 copies
+ * h_staging to the workspace and launches the kernels that read and write
+ * no occupancy (ray walk, record counting, tile scan, placing); part 2
+ * launches the folds into the grids' occupancy (h_staging must still hold
+ * the same tick); 3 is both, as cublox_launch_integrate_batch_staged. Part 1
+ * of a tick may run on another stream and workspace beside part 2 of the
+ * tick before it, which is how a caller overlaps consecutive ticks; every
+ * grid's folds must stay in tick order (an event between the streams).   */
+int cublox_launch_integrate_batch_staged_part(const void *h_staging, void *d_ws,
+                                              size_t ws_bytes,
+                                              const cublox_batch_limits *lim,
+                                              int part, cublox_stream s);
 /* stage + staged launch. */
 int cublox_launch_integrate_batch(const cublox_integrate_grid *grids,
                                   int n_grids, const cublox_batch_limits *lim,

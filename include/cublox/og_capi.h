@@ -190,6 +190,19 @@ int cublox_gb_tick(cublox_gb *b, int n_scans, cublox_og *const *grids,
                    const int *point_stride, const float *range_drop_sq,
                    int n_recenter, cublox_og *const *rgrids,
                    const float *rpos);
+/* cublox_gb_tick in two halves (GridBatch::submit / wait): submit returns
+ * once the tick is launched; with pipelining it first waits for the tick
+ * submitted two before it. wait blocks until every submitted tick ends. */
+int cublox_gb_submit(cublox_gb *b, int n_scans, cublox_og *const *grids,
+                     const float *const *d_x, const float *const *d_y,
+                     const float *const *d_z, const int *n, const float *T,
+                     const int *point_stride, const float *range_drop_sq,
+                     int n_recenter, cublox_og *const *rgrids,
+                     const float *rpos);
+int cublox_gb_wait(cublox_gb *b);
+/* Pipelined ticks (GridBatch::setPipelined): consecutive ticks overlap on
+ * two streams and workspaces, every grid's folds in tick order. */
+int cublox_gb_set_pipelined(cublox_gb *b, int on);
 int cublox_gb_set_graph_mode(cublox_gb *b, int on);
 /* settings: n quadruples (near_steps, chunk, lanes per ray 0/1/4, hot-tile
  * mode 0/1) as cublox::TuneSetting; ticks_each measured ticks each. */

@@ -42,7 +42,11 @@ def _setup(cb, orc):
 
 
 def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=None,
-         setting=None):
+         setting=None, pipelined=False, check_every=1):
+    """Ticks every grid with its own scan and recenter; checks every grid
+    against its oracle after every check_every-th tick (and the last).
+    Pipelined: ticks are submitted without waits between the checks, so
+    consecutive ticks overlap."""
     grids, oracles, scenes = _setup(cb, orc)
     total = max_points or sum(b * a for (_, _, _, _, _, b, a, _, _) in GRIDS)
     kw = {} if max_tiles is None else {"max_tiles": max_tiles}
@@ -54,6 +58,9 @@ def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=No
         batch.set_tuning(tuning, 1)
     if setting:
         batch.set_setting(*setting)
+    if pipelined:
+        batch.set_pipelined(True)
+    live = []  # device columns of the ticks not yet waited for
     for k in range(ticks):
         scans, recenters, keep = [], [], []
         for i, (half, res, ctr, thr, rng, beams, az, speed, seed) in enumerate(GRIDS):
@@ -65,8 +72,17 @@ def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=No
             recenters.append((grids[i], s.base))
             oracles[i].update_sensor_frame((s.xs, s.ys, s.zs), s.T)
             oracles[i].recenter(s.base)
-        print(f"[batch] tick {k} graph={graph} setting={batch.setting}", flush=True)
-        batch.tick(scans, recenters)
+        print(f"[batch] tick {k} graph={graph} pipelined={pipelined} "
+              f"setting={batch.setting}", flush=True)
+        if not pipelined:
+            batch.tick(scans, recenters)
+        else:
+            live.append(keep)
+            batch.submit(scans, recenters)
+            if (k + 1) % check_every and k + 1 < ticks:
+                continue
+            batch.wait()
+            live.clear()
         for i in range(len(GRIDS)):
             assert grids[i].geometry.origin_i == oracles[i].origin()[0], (k, i)
             assert_occ_bitexact(grids[i].occupancy(), oracles[i].occupancy(),
@@ -113,3 +129,29 @@ def test_batch_shared_hot_tiles(cb, orc, gpu_ready, lanes):
     batch = _run(cb, orc, 4, setting=(8, 256, lanes, 1))
     assert batch.counters()["split_tiles"] > 0
 
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_pipelined_matches_oracle(cb, orc, gpu_ready, graph):
+    """Consecutive ticks overlap (a tick's walk and binning beside the
+    previous tick's folds and recenters, on two streams and workspaces);
+    every grid stays bit-exact at the checks between runs of four ticks."""
+    batch = _run(cb, orc, 9, graph=graph, pipelined=True, check_every=4)
+    assert batch.pipelined
+
+
+def test_batch_pipelined_after_tuning(cb, orc, gpu_ready):
+    """Autotuning ticks run one at a time, then the ticks overlap; the
+    setting switches (walk and hot-tile mode) keep every grid exact."""
+    cands = [(8, 512, 4, 1), (8, 256, 1, 0)]
+    batch = _run(cb, orc, 10, graph=True, tuning=cands, pipelined=True, check_every=3)
+    assert not batch.tuning
+
+
+def test_batch_pipelined_rejects_changed_voxels(cb, gpu_ready):
+    g = cb.OccupancyGrid((20, 20, 8), 0.1, False, 1.0)
+    batch = cb.GridBatch.for_grids([g], 4096, changed_voxels=True)
+    with pytest.raises(RuntimeError, match="changed-voxel"):
+        batch.set_pipelined(True)
+    batch.close()
+    g.close()
