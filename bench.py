@@ -367,11 +367,11 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
     log(f"served: {n_sessions} sessions, {statistics.mean(pts):.0f} points/scan mean, "
         f"templates in {time.perf_counter() - t:.1f} s")
 
-    def serve(max_grids):
+    def serve(max_grids, pipelined=False):
         srv = cb.MapServer(hbm_budget_bytes=200 << 30, max_sessions=n_sessions,
                            max_points_per_scan=131072, max_points_per_tick=tick_points,
                            max_grids_per_tick=max_grids, max_tiles_per_tick=1 << 21,
-                           max_segments_per_ray=72, graph=True)
+                           max_segments_per_ray=72, graph=True, pipelined=pipelined)
         sids = [srv.open_session(h, r, c, th, mr, cb.log_odds()) for (h, r, c, th, mr) in specs]
         return srv, sids
 
@@ -390,7 +390,8 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
         c = srv.completions()
         return len(c) / ((c["done_ms"].max() - c["start_ms"].min()) * 1e-3)
 
-    srv, sids = serve(256)
+    # The server with pipelined ticks (two in flight, overlapped).
+    srv, sids = serve(256, pipelined=True)
     t = time.perf_counter()
     queued(srv, sids, 0)  # warm-up: first origins, graph capture
     st0 = srv.stats()
@@ -422,6 +423,12 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
         log(f"served: load {f}: {levels[-1]}")
     st = srv.stats()
     srv.close()
+    # The same server with one tick at a time.
+    srv0, sids0 = serve(256)
+    queued(srv0, sids0, 0)
+    serial_sat = queued(srv0, sids0, depth)
+    srv0.close()
+    log(f"served: one tick at a time, saturation {serial_sat:.0f} scans/s")
     # Unbatched: the same sessions, one scan per tick.
     srv1, sids1 = serve(1)
     queued(srv1, sids1, 0)
@@ -460,7 +467,9 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
                         for s in SERVED_SHAPES],
                 points_per_scan_mean=round(statistics.mean(pts)),
                 hbm_budget_gib=200, resident_gib=round(st["resident_bytes"] / 2**30, 2),
-                tick_points=tick_points, saturation_scans_per_s=round(sat, 1),
+                tick_points=tick_points, pipelined=True,
+                saturation_scans_per_s=round(sat, 1),
+                saturation_one_tick_at_a_time_scans_per_s=round(serial_sat, 1),
                 plain_scans_per_s=round(plain_sps, 1),
                 speedup_vs_plain=round(sat / plain_sps, 2),
                 unbatched_saturation_scans_per_s=round(plain_sat, 1),

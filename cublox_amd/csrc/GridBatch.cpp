@@ -447,6 +447,14 @@ void GridBatch::wait() {
   if (slots_[cur_].in_flight) finish(cur_);
 }
 
+void GridBatch::waitOldest() {
+  if (n_slots_ == 2 && slots_[1 - cur_].in_flight) {
+    finish(1 - cur_);
+  } else if (slots_[cur_].in_flight) {
+    finish(cur_);
+  }
+}
+
 void GridBatch::tick(
     const std::vector<GridScan> &scans,
     const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>> &recenters) {

@@ -4,6 +4,7 @@
 #include <cublox/capi.h>
 
 #include <algorithm>
+#include <deque>
 #include <cstring>
 #include <iterator>
 #include <stdexcept>
@@ -68,6 +69,7 @@ MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
   }
   batch_ = std::make_unique<GridBatch>(bl);
   batch_->setGraphMode(lim_.graph);
+  if (lim_.pipelined) batch_->setPipelined(true);
   try {
     MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_arena_), lim_.point_arena_bytes));
     const std::size_t stage_bytes =
@@ -660,13 +662,14 @@ void MapServer::enqueueFetches(Tick &t) {
 }
 
 // Blocks until the tick's device work (update, recenter, fetches) is done.
+// Ticks are waited for in submission order, so a tick with scans is the
+// batch's oldest in flight; its fetches follow it on its stream.
 void MapServer::waitTick(Tick &t) {
   if (!t.scans.empty()) {
-    batch_->wait();
+    batch_->waitOldest();
     t.device_ms = batch_->lastUpdateMs() + batch_->lastRecenterMs();
-  } else {
-    MS_CHECK(cublox_stream_synchronize(batch_->stream()));
   }
+  MS_CHECK(cublox_stream_synchronize(t.stream));
   t.done_ms = nowMs();
 }
 
@@ -727,13 +730,18 @@ void MapServer::finishTick(Tick &t) {
 // from everything queued by then; the finished tick's completions are then
 // recorded while the new tick runs.
 void MapServer::run() {
-  Tick prev;
-  bool have_prev = false;
+  // Submitted ticks not yet finished, oldest first: one at a time, or two
+  // when pipelined. A finished tick's host work (completions, fetch
+  // results, freeing its points) runs while the next ticks are on the
+  // device.
+  std::deque<Tick> flight;
+  const std::size_t depth = lim_.pipelined ? 2 : 1;
   int buf = 0;
   long long seen = -1;
   try {
     while (true) {
-      if (have_prev) waitTick(prev);
+      const bool full = flight.size() >= depth;
+      if (full) waitTick(flight.front());
       Tick cur;
       bool have_cur = false;
       {
@@ -741,7 +749,7 @@ void MapServer::run() {
         retireClosed();
         evictRequested();
         admitWaiting();
-        if (!have_prev) {
+        if (flight.empty()) {
           worker_busy_ = false;
           cv_idle_.notify_all();
           cv_work_.wait(lk, [&] { return stop_ || (gen_ != seen && !paused_); });
@@ -753,28 +761,27 @@ void MapServer::run() {
         }
         have_cur = !paused_ && formTick(cur);
         if (have_cur) cur.index = ticks_++;
-        if (!have_cur && !have_prev && stop_) break;
+        if (!have_cur && flight.empty() && stop_) break;
       }
       if (have_cur) {
         cur.buf = buf;
-        buf ^= 1;
+        buf = (buf + 1) % static_cast<int>(depth + 1);
         cur.start_ms = nowMs();
         if (!cur.scans.empty()) submitTick(cur);
+        cur.stream = batch_->stream();
         enqueueFetches(cur);
         cur.launched_ms = nowMs();
-        if (have_prev) {
+        if (full) {
           std::lock_guard<std::mutex> lk(mu_);
-          stats_.launch_ms += cur.launched_ms - prev.done_ms;
+          stats_.launch_ms += cur.launched_ms - flight.front().done_ms;
         }
       }
-      if (have_prev) {
-        finishTick(prev);
-        have_prev = false;
+      if (!full && !have_cur && !flight.empty()) waitTick(flight.front());
+      if (full || (!have_cur && !flight.empty())) {
+        finishTick(flight.front());
+        flight.pop_front();
       }
-      if (have_cur) {
-        prev = std::move(cur);
-        have_prev = true;
-      }
+      if (have_cur) flight.push_back(std::move(cur));
       {
         std::lock_guard<std::mutex> lk(mu_);
         cv_idle_.notify_all();

@@ -108,6 +108,9 @@ public:
               const std::vector<std::pair<OccupancyGrid *, Eigen::Vector3f>>
                   &recenters);
   void wait();
+  // Blocks until the oldest submitted tick still in flight has finished
+  // (the last one when not pipelined); last*() then describe it.
+  void waitOldest();
   bool inFlight() const;
 
   // Pipelined ticks (a second stream and workspace, allocated when first

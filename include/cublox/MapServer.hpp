@@ -64,6 +64,11 @@ struct ServerLimits {
   // recently used idle session (nothing queued or in flight): its map goes
   // to a host snapshot and comes back, bit-identical, when it is readmitted.
   bool evict_idle{false};
+  // Overlap consecutive ticks (GridBatch pipelining: a tick's walk and
+  // binning beside the previous tick's folds, recenters and fetches; a
+  // second workspace). Every session's requests still take effect in
+  // submission order.
+  bool pipelined{false};
 };
 
 class MapServer {
@@ -182,6 +187,7 @@ private:
     double start_ms{0.0}, done_ms{0.0}, launched_ms{0.0};
     double device_ms{0.0};
     int buf{0};                  // fetch output arena
+    void *stream{nullptr};       // the batch stream its work went to
   };
   struct FetchArena {            // fetch outputs of one tick
     float *d_pos{nullptr}, *h_pos{nullptr};
@@ -210,7 +216,7 @@ private:
 
   ServerLimits lim_;
   std::unique_ptr<GridBatch> batch_;
-  FetchArena farena_[2];
+  FetchArena farena_[3];  // one per tick alive: in flight, or being finished
   // Device arena of queued scans' points: first-fit free list (offset ->
   // size), its own lock, and the stream the submitting threads copy on.
   char *d_arena_{nullptr};

@@ -46,13 +46,17 @@ def _cb_setup(cb, orc):
     return w, grids, oracles, scenes
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_batched_shape_matches_oracle(cb, orc, gpu_ready, graph):
+@pytest.mark.parametrize("graph,pipelined", [(False, False), (True, False), (True, True)])
+def test_batched_shape_matches_oracle(cb, orc, gpu_ready, graph, pipelined):
+    """Pipelined: ticks overlap in pairs (checked after every second tick)."""
     from cublox_amd.workloads import batched_scan_shape
     w, grids, oracles, scenes = _cb_setup(cb, orc)
     batch = cb.GridBatch.for_grids(grids, CB_GRIDS * 131072, max_near_steps=12)
     batch.set_graph_mode(graph)
+    batch.set_pipelined(pipelined)
     moved = 0
+    live = []
+    before = [grids[i].geometry.origin_i for i in range(CB_GRIDS)]
     for k in range(CB_TICKS):
         scans, recenters, keep = [], [], []
         for i in range(CB_GRIDS):
@@ -70,15 +74,24 @@ def test_batched_shape_matches_oracle(cb, orc, gpu_ready, graph):
             recenters.append((grids[i], s.base))
             oracles[i].update_sensor_frame((s.xs, s.ys, s.zs), s.T)
             oracles[i].recenter(s.base)
-        before = [grids[i].geometry.origin_i for i in range(CB_GRIDS)]
-        print(f"[cb] tick {k} graph={graph} points={[x[4] for x in scans]}", flush=True)
-        batch.tick(scans, recenters)
+        print(f"[cb] tick {k} graph={graph} pipelined={pipelined} "
+              f"points={[x[4] for x in scans]}", flush=True)
+        if pipelined:
+            live.append(keep)
+            batch.submit(scans, recenters)
+            if k % 2 == 0 and k + 1 < CB_TICKS:
+                continue
+            batch.wait()
+            live.clear()
+        else:
+            batch.tick(scans, recenters)
         for i in range(CB_GRIDS):
             gi = grids[i].geometry.origin_i
             moved += int(gi != before[i])
             assert gi == oracles[i].origin()[0], (k, i)
             assert_occ_bitexact(grids[i].occupancy(), oracles[i].occupancy(),
                                 f"tick {k} grid {i}")
+        before = [grids[i].geometry.origin_i for i in range(CB_GRIDS)]
     assert moved >= CB_GRIDS, "every grid must recenter at least once"
     c = batch.counters()
     assert c["overflow"] == 0, c
@@ -136,9 +149,10 @@ def _served_alone(cb, i, reqs):
 
 
 def test_served_shapes_mix_matches_alone_and_oracle(cb, orc, gpu_ready):
+    """The served configuration: graph replay and pipelined ticks."""
     reqs = _served_requests(cb)
     srv = cb.MapServer(max_points_per_scan=131072, max_points_per_tick=1 << 20,
-                       max_segments_per_ray=96, graph=True)
+                       max_segments_per_ray=96, graph=True, pipelined=True)
     srv.keep_fetch_results(True)
     sids = [srv.open_session(half, 0.1, False, 2.0, rng_m, cb.log_odds())
             for (half, _, _, rng_m, _) in SERVED]

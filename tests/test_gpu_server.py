@@ -99,13 +99,16 @@ def _submit_interleaved(srv, sids, reqs, seed):
     return scan_ids, fetch_ids
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_server_mix_matches_alone(cb, orc, gpu_ready, graph):
+@pytest.mark.parametrize("graph,pipelined", [(False, False), (True, False),
+                                              (False, True), (True, True)])
+def test_server_mix_matches_alone(cb, orc, gpu_ready, graph, pipelined):
+    """Pipelined: two ticks in flight, each tick's walk beside the previous
+    tick's folds, recenters and fetches."""
     reqs = _requests(cb)
-    srv = _server(cb, graph=graph)
+    srv = _server(cb, graph=graph, pipelined=pipelined)
     srv.keep_fetch_results(True)
     sids = _open_all(cb, srv)
-    print(f"[server] graph={graph} sessions={sids}", flush=True)
+    print(f"[server] graph={graph} pipelined={pipelined} sessions={sids}", flush=True)
     srv.pause()  # everything queued first: ticks then batch deterministically
     scan_ids, fetch_ids = _submit_interleaved(srv, sids, reqs, seed=7)
     srv.resume()
@@ -134,11 +137,12 @@ def test_server_mix_matches_alone(cb, orc, gpu_ready, graph):
     srv.close()
 
 
-def test_server_sessions_match_oracle(cb, orc, gpu_ready):
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_server_sessions_match_oracle(cb, orc, gpu_ready, pipelined):
     """Per-session oracle agreement for two of the shapes, served in a mix
     with requests arriving while ticks run."""
     reqs = _requests(cb)
-    srv = _server(cb)
+    srv = _server(cb, pipelined=pipelined)
     sids = _open_all(cb, srv)
     _submit_interleaved(srv, sids, reqs, seed=11)
     srv.drain()
