@@ -96,12 +96,14 @@ void GridBatch::allocSlot(Slot &s) {
 }
 
 void GridBatch::dropGraphs(Slot &s) {
-  for (void **set : {s.front, s.back}) {
-    for (int k = 0; k < 2; ++k) {
-      if (set[k]) cublox_graph_destroy(set[k]);
-      set[k] = nullptr;
-    }
+  auto drop = [](void *&g) {
+    if (g) cublox_graph_destroy(g);
+    g = nullptr;
+  };
+  for (auto &by_near : s.front) {
+    for (void *&g : by_near) drop(g);
   }
+  for (void *&g : s.back) drop(g);
 }
 
 void GridBatch::releaseSlot(Slot &s) noexcept {
@@ -280,6 +282,7 @@ void GridBatch::stage(
   CUBLOX_CHECK(cublox_integrate_batch_stage_tuned(
       items.data(), static_cast<int>(items.size()), &c, &tune, slot.h_stage));
   slot.lanes = tune.lanes_per_ray;
+  slot.near_steps = setting_.near_steps;
   slot.hot_mode = setting_.hot_mode;
 
   // OccupancyGrid.cpp:141-179 per grid: threshold pre-check on the host,
@@ -364,7 +367,7 @@ void GridBatch::launch(int q, int prev) {
   };
   if (prev >= 0) CUBLOX_CHECK(cublox_event_record(slots_[prev].ev[3], slots_[prev].stream));
   CUBLOX_CHECK(cublox_event_record(s.ev[0], s.stream));
-  run(front, s.front[s.lanes == 4 ? 1 : 0]);
+  run(front, s.front[s.lanes == 4 ? 1 : 0][s.near_steps]);
   if (prev >= 0) CUBLOX_CHECK(cublox_stream_wait_event(s.stream, slots_[prev].ev[3]));
   run(back, s.back[s.hot_mode ? 1 : 0]);
   CUBLOX_CHECK(cublox_event_record(s.ev[2], s.stream));

@@ -618,7 +618,8 @@ struct BatchHeader {  // device-resident in batch mode
   int chunk;    // records per hot-tile work item
   int lanes;    // lanes per ray of the walk: 1 or 4
   int shared;   // hot-tile chunks add into shared slabs (global atomics)
-  int pad;
+  int cube;     // largest near-field cube edge of the batch's grids (the
+                // walk's LDS cube is sized by it)
 };
 
 struct Batch {
@@ -2303,13 +2304,17 @@ constexpr int kPartFront = 1, kPartBack = 2, kPartBoth = 3;
 int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
                     bool delta, hipStream_t hs, hipEvent_t occ_ready = nullptr,
                     int batch_lanes = 0, bool shared = false,
-                    int parts = kPartBoth) {
+                    int parts = kPartBoth, int cube = 0) {
   if (!(parts & kPartFront)) {
     return delta ? launch_folds<true>(b, L, w, hs, shared)
                  : launch_folds<false>(b, L, w, hs, shared);
   }
   const Caps &c = L.cap;
-  const size_t near_lds = static_cast<size_t>(c.max_cube) * c.max_cube * c.max_cube * 4;
+  // The walk's LDS near-field cube: the batch's largest (cube > 0), else the
+  // capacity's. It bounds the walk's occupancy, so it is not sized for the
+  // limits when the tick needs less.
+  const int edge = cube > 0 ? std::min(cube, c.max_cube) : c.max_cube;
+  const size_t near_lds = static_cast<size_t>(edge) * edge * edge * 4;
   const unsigned walk_blocks = static_cast<unsigned>((c.max_rays + kWalkBlock - 1) / kWalkBlock);
   // A single grid takes the walk its size calls for; a batch the one its
   // staged header names (the autotuner's lanes per ray; a captured sequence
@@ -2667,7 +2672,7 @@ int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n
   }
   Staging *sg = static_cast<Staging *>(h_staging);
   long long ray = 0, slot = 0, points = 0;
-  int tile = 0, near = 0;
+  int tile = 0, near = 0, max_cube = 1;
   for (int k = 0; k < n_grids; ++k) {
     const cublox_integrate_grid &q = grids[k];
     const int near_k = q.near_steps > 0 ? q.near_steps : 8;
@@ -2710,6 +2715,7 @@ int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n
     slot += n * d.g.max_seg;
     tile += d.g.n_tiles;
     near += d.g.cube * d.g.cube * d.g.cube;
+    max_cube = std::max(max_cube, d.g.cube);
   }
   if (points > lim->max_points || ray > c.max_rays || slot > c.seg_capacity ||
       tile > c.max_tiles || near > c.max_near) {
@@ -2729,7 +2735,8 @@ int cublox_integrate_batch_stage_tuned(const cublox_integrate_grid *grids, int n
   if (tuning && (tuning->hot_mode < 0 || tuning->hot_mode > 1)) {
     return set_error(CUBLOX_ERR_INVALID, "integrate_batch: hot-tile mode must be 0 or 1");
   }
-  sg->hdr = BatchHeader{n_grids, static_cast<int>(ray), tile, near, chunk, lanes, shared, 0};
+  sg->hdr = BatchHeader{n_grids, static_cast<int>(ray), tile, near, chunk, lanes, shared,
+                        max_cube};
   if (tuning) tuning->lanes_per_ray = lanes;  // the walk the launch will take
   return CUBLOX_OK;
 }
@@ -2760,7 +2767,7 @@ int cublox_launch_integrate_batch_staged_part(const void *h_staging, void *d_ws,
   b.hdr = reinterpret_cast<const BatchHeader *>(base + L.off_hdr);
   const BatchHeader &h = static_cast<const Staging *>(h_staging)->hdr;
   return launch_sequence(b, L, bind(L, d_ws), lim->delta != 0, hs, nullptr,
-                         h.lanes == 4 ? 4 : 1, h.shared != 0, part);
+                         h.lanes == 4 ? 4 : 1, h.shared != 0, part, h.cube);
 }
 
 int cublox_launch_integrate_batch_staged(const void *h_staging, void *d_ws,

@@ -153,16 +153,17 @@ private:
     void *h_stage{nullptr};   // integrate staging (pinned)
     void *h_rstage{nullptr};  // recenter staging (pinned)
     void *d_rarea{nullptr};   // recenter staging, device copy
-    // Captured first halves by walk (lanes per ray 1, 4) and folds +
-    // recenter by hot-tile mode: a staged launch holds the kernels its
-    // tick's setting names.
-    void *front[2]{nullptr, nullptr};
+    // Captured first halves by walk (lanes per ray 1, 4) and near-field
+    // steps (the walk's LDS cube), folds + recenter by hot-tile mode: a
+    // staged launch holds the kernels and shapes its tick's setting names.
+    void *front[2][13]{};
     void *back[2]{nullptr, nullptr};
     void *ev[4]{nullptr, nullptr, nullptr, nullptr};  // start, folds done, end, order mark
     bool in_flight{false};
     bool alone{true};     // no other tick overlapped it
     int tune_idx{-1};     // autotuner candidate it ran, or -1
     int lanes{1};
+    int near_steps{8};
     int hot_mode{0};
     long long zeroed{0};  // voxels its recenters zero
   };
