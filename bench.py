@@ -181,8 +181,8 @@ def run_pcie(cb, w, scans, warmup, steps):
 # Autotuner candidates of the batched workload: (near steps, hot-tile chunk,
 # lanes per ray = 64 / lanes rays per wavefront, hot-tile mode 0 chunk slabs
 # / 1 shared slabs with global atomics).
-TUNE_CANDIDATES = [(8, 1024, 1, 0), (8, 1024, 4, 0), (8, 1024, 1, 1), (8, 1024, 4, 1),
-                   (8, 2048, 1, 0), (8, 2048, 1, 1), (12, 2048, 1, 1), (4, 512, 1, 0)]
+TUNE_CANDIDATES = [(8, 2048, 1, 0), (8, 4096, 1, 0), (8, 8192, 1, 0), (8, 16384, 1, 0),
+                   (8, 8192, 4, 0), (8, 2048, 1, 1), (6, 8192, 1, 0), (10, 8192, 1, 0)]
 
 
 def run_batched(cb, tune_ticks_each, ticks):
@@ -338,7 +338,7 @@ def _lat(c):
 
 
 def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
-               tick_points=1 << 20):
+               tick_points=1 << 20, chunk=0):
     """The map server workload: n_sessions sessions cycling over three grid
     shapes, HBM-budget admission (200 GB), continuous batching. Saturation
     is the drain rate of `depth` queued scans per session; then open-loop
@@ -367,11 +367,12 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
     log(f"served: {n_sessions} sessions, {statistics.mean(pts):.0f} points/scan mean, "
         f"templates in {time.perf_counter() - t:.1f} s")
 
-    def serve(max_grids, pipelined=False):
+    def serve(max_grids, piped=False):
         srv = cb.MapServer(hbm_budget_bytes=200 << 30, max_sessions=n_sessions,
                            max_points_per_scan=131072, max_points_per_tick=tick_points,
                            max_grids_per_tick=max_grids, max_tiles_per_tick=1 << 21,
-                           max_segments_per_ray=72, graph=True, pipelined=pipelined)
+                           max_segments_per_ray=72, graph=True, pipelined=piped,
+                           chunk=chunk)
         sids = [srv.open_session(h, r, c, th, mr, cb.log_odds()) for (h, r, c, th, mr) in specs]
         return srv, sids
 
@@ -390,45 +391,49 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
         c = srv.completions()
         return len(c) / ((c["done_ms"].max() - c["start_ms"].min()) * 1e-3)
 
-    # The server with pipelined ticks (two in flight, overlapped).
-    srv, sids = serve(256, pipelined=True)
-    t = time.perf_counter()
-    queued(srv, sids, 0)  # warm-up: first origins, graph capture
-    st0 = srv.stats()
-    sat = queued(srv, sids, depth)
-    st1 = srv.stats()
-    log(f"served: saturation {sat:.0f} scans/s over {st1['ticks'] - st0['ticks']} ticks "
-        f"({time.perf_counter() - t:.1f} s), resident {st1['resident_bytes'] / 2**30:.1f} GiB")
-    levels = []
-    for f in loads:
-        st_a = srv.stats()
-        rate = f * sat
-        n_sub, lag = srv.run_load(sids, templates, scan_rate_hz=rate, duration_s=seconds,
-                                  fetch_ratio=0.2, fetch_radius=0.5, fetch_radius_max=3.0,
-                                  speed=0.5, seed=17, clients=8)
-        srv.drain()
-        c = srv.completions()
-        st_b = srv.stats()
-        nt = max(st_b["ticks"] - st_a["ticks"], 1)
-        scans = c[c["kind"] == 0]
-        span = (c["done_ms"].max() - c["submit_ms"].min()) * 1e-3
-        levels.append(dict(load=f, offered_scans_per_s=round(rate, 1),
-                           achieved_scans_per_s=round(len(scans) / span, 1),
-                           requests=int(n_sub), client_max_lag_ms=round(lag, 3),
-                           ticks=nt, scans_per_tick=round(len(scans) / nt, 1),
-                           device_ms_per_tick=round((st_b["device_ms"] - st_a["device_ms"]) / nt, 3),
-                           host_ms_between_ticks=round((st_b["launch_ms"] - st_a["launch_ms"]) / nt, 3),
-                           scan_latency=_lat(scans),
-                           fetch_latency=_lat(c[c["kind"] == 1])))
-        log(f"served: load {f}: {levels[-1]}")
-    st = srv.stats()
-    srv.close()
-    # The same server with one tick at a time.
-    srv0, sids0 = serve(256)
-    queued(srv0, sids0, 0)
-    serial_sat = queued(srv0, sids0, depth)
-    srv0.close()
-    log(f"served: one tick at a time, saturation {serial_sat:.0f} scans/s")
+    def measure(piped):
+        """Saturation and the open-loop load levels of one server mode."""
+        srv, sids = serve(256, piped=piped)
+        t = time.perf_counter()
+        queued(srv, sids, 0)  # warm-up: first origins, graph capture
+        st0 = srv.stats()
+        sat = queued(srv, sids, depth)
+        st1 = srv.stats()
+        log(f"served (pipelined={piped}): saturation {sat:.0f} scans/s over "
+            f"{st1['ticks'] - st0['ticks']} ticks ({time.perf_counter() - t:.1f} s), "
+            f"resident {st1['resident_bytes'] / 2**30:.1f} GiB")
+        levels = []
+        for f in loads:
+            st_a = srv.stats()
+            rate = f * sat
+            n_sub, lag = srv.run_load(sids, templates, scan_rate_hz=rate, duration_s=seconds,
+                                      fetch_ratio=0.2, fetch_radius=0.5, fetch_radius_max=3.0,
+                                      speed=0.5, seed=17, clients=8)
+            srv.drain()
+            c = srv.completions()
+            st_b = srv.stats()
+            nt = max(st_b["ticks"] - st_a["ticks"], 1)
+            scans = c[c["kind"] == 0]
+            span = (c["done_ms"].max() - c["submit_ms"].min()) * 1e-3
+            levels.append(dict(load=f, offered_scans_per_s=round(rate, 1),
+                               achieved_scans_per_s=round(len(scans) / span, 1),
+                               requests=int(n_sub), client_max_lag_ms=round(lag, 3),
+                               ticks=nt, scans_per_tick=round(len(scans) / nt, 1),
+                               device_ms_per_tick=round(
+                                   (st_b["device_ms"] - st_a["device_ms"]) / nt, 3),
+                               host_ms_between_ticks=round(
+                                   (st_b["launch_ms"] - st_a["launch_ms"]) / nt, 3),
+                               scan_latency=_lat(scans),
+                               fetch_latency=_lat(c[c["kind"] == 1])))
+            log(f"served (pipelined={piped}): load {f}: {levels[-1]}")
+        st = srv.stats()
+        srv.close()
+        return sat, levels, st
+
+    # One tick at a time, then pipelined ticks (two in flight while half a
+    # tick of points is queued): throughput against tail latency.
+    sat, levels, st = measure(False)
+    pipe_sat, pipe_levels, _ = measure(True)
     # Unbatched: the same sessions, one scan per tick.
     srv1, sids1 = serve(1)
     queued(srv1, sids1, 0)
@@ -467,13 +472,17 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
                         for s in SERVED_SHAPES],
                 points_per_scan_mean=round(statistics.mean(pts)),
                 hbm_budget_gib=200, resident_gib=round(st["resident_bytes"] / 2**30, 2),
-                tick_points=tick_points, pipelined=True,
+                tick_points=tick_points, chunk=chunk or 4096,
                 saturation_scans_per_s=round(sat, 1),
-                saturation_one_tick_at_a_time_scans_per_s=round(serial_sat, 1),
                 plain_scans_per_s=round(plain_sps, 1),
                 speedup_vs_plain=round(sat / plain_sps, 2),
                 unbatched_saturation_scans_per_s=round(plain_sat, 1),
                 batching_speedup=round(sat / plain_sat, 2), levels=levels,
+                pipelined=dict(saturation_scans_per_s=round(pipe_sat, 1),
+                               speedup_vs_plain=round(pipe_sat / plain_sps, 2),
+                               levels=pipe_levels,
+                               form="two ticks in flight while half a tick of points "
+                                    "is queued, else one"),
                 seconds_per_level=seconds)
 
 

@@ -70,6 +70,9 @@ MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
   batch_ = std::make_unique<GridBatch>(bl);
   batch_->setGraphMode(lim_.graph);
   if (lim_.pipelined) batch_->setPipelined(true);
+  TuneSetting ts = batch_->setting();
+  ts.chunk = std::max(lim_.chunk, bl.min_chunk);
+  batch_->setSetting(ts);
   try {
     MS_CHECK(cublox_malloc(reinterpret_cast<void **>(&d_arena_), lim_.point_arena_bytes));
     const std::size_t stage_bytes =
@@ -81,7 +84,14 @@ MapServer::MapServer(const ServerLimits &limits) : lim_(limits) {
       MS_CHECK(cublox_stream_create(&stages_.back().stream));
       free_stages_.push_back(k);
     }
+    // Fetch output arenas sized up front (they still grow on demand): a
+    // first growth inside a tick's path frees and allocates device and
+    // pinned memory, a stall of milliseconds.
+    for (auto &a : farena_) {
+      growFetchArena(a, static_cast<std::size_t>(lim_.fetch_samples_per_tick) / 2, 512);
+    }
   } catch (...) {
+    for (auto &a : farena_) growFetchArena(a, 0, 0);
     if (d_arena_) cublox_free(d_arena_);
     d_arena_ = nullptr;
     for (auto &st : stages_) {
@@ -510,6 +520,23 @@ void MapServer::evictRequested() {
   }
 }
 
+// Whether the admitted sessions' queued scans hold at least half a tick of
+// points (lock held): requests are backing up, not just arriving.
+bool MapServer::backlog() const {
+  long long points = 0;
+  const long long half = lim_.max_points_per_tick / 2;
+  for (const auto &kv : sessions_) {
+    const Session &s = kv.second;
+    if (!s.admitted || s.closing) continue;
+    for (const Request &r : s.queue) {
+      if (r.kind != Kind::SCAN) continue;
+      points += r.n;
+      if (points >= half) return true;
+    }
+  }
+  return false;
+}
+
 // Takes the tick's requests: per admitted session (round-robin from rr_),
 // a leading run of fetches, or one scan and the fetches right after it,
 // while the scan fits the tick's caps.
@@ -730,16 +757,24 @@ void MapServer::finishTick(Tick &t) {
 // from everything queued by then; the finished tick's completions are then
 // recorded while the new tick runs.
 void MapServer::run() {
-  // Submitted ticks not yet finished, oldest first: one at a time, or two
-  // when pipelined. A finished tick's host work (completions, fetch
-  // results, freeing its points) runs while the next ticks are on the
-  // device.
+  // Submitted ticks not yet finished, oldest first. One at a time; when
+  // pipelined, two while requests back up (a formed tick left at least
+  // half a tick of points queued), so a lighter load keeps the latency of
+  // single ticks. A finished
+  // tick's host work (completions, fetch results, freeing its points) runs
+  // while the next ticks are on the device.
   std::deque<Tick> flight;
-  const std::size_t depth = lim_.pipelined ? 2 : 1;
+  const int arenas = lim_.pipelined ? 3 : 2;
+  std::size_t depth = 1;
   int buf = 0;
   long long seen = -1;
   try {
     while (true) {
+      while (flight.size() > depth) {  // the backlog cleared: back to one
+        waitTick(flight.front());
+        finishTick(flight.front());
+        flight.pop_front();
+      }
       const bool full = flight.size() >= depth;
       if (full) waitTick(flight.front());
       Tick cur;
@@ -762,10 +797,11 @@ void MapServer::run() {
         have_cur = !paused_ && formTick(cur);
         if (have_cur) cur.index = ticks_++;
         if (!have_cur && flight.empty() && stop_) break;
+        depth = lim_.pipelined && have_cur && backlog() ? 2 : 1;
       }
       if (have_cur) {
         cur.buf = buf;
-        buf = (buf + 1) % static_cast<int>(depth + 1);
+        buf = (buf + 1) % arenas;
         cur.start_ms = nowMs();
         if (!cur.scans.empty()) submitTick(cur);
         cur.stream = batch_->stream();

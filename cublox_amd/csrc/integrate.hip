@@ -64,7 +64,11 @@ constexpr int kHash = 4096;
 constexpr int kBinCache = 16;  // records per binning thread kept for placing
 constexpr int kWalkHash = 1024;  // per-walk-workgroup tile record counts
 constexpr int kHashProbes = 64;
-constexpr int kChunk = 1024;      // records per chunk of a hot tile (default)
+// Records per chunk of a hot tile (single-grid default). Every chunk costs a
+// 32 KB slab written and read back; larger chunks mean fewer of them and
+// longer work items (measured at 701x701x151: 1024 records 218, 2048 210,
+// 4096 208, 8192 216 us per update).
+constexpr int kChunk = 4096;
 constexpr int kApplyThreads = 256;
 constexpr int kPerThread = kTileVox / kApplyThreads;  // 32 voxels
 constexpr int kHotSlices = 8;     // hot_fold workgroups per split tile
@@ -745,7 +749,7 @@ __device__ __forceinline__ int length_class(const GridDesc &d, int i) {
 // LDS), so each wave's rays walk similar distances; every ray's records are
 // its own, whichever lane walks it.
 template <int kLanes>
-__global__ void __launch_bounds__(kWalkBlock * kLanes)
+__global__ void __launch_bounds__(kWalkBlock * kLanes, kLanes == 1 ? 8 : 1)
     walk_kernel(const Batch b, Workspace w) {
   extern __shared__ int s_near[];  // the grid's near cube, sized at launch
   __shared__ int s_hist[kLenBuckets];

@@ -62,7 +62,7 @@ cublox::MapServer &server_of(cublox_ms *h) { return *H->server; }
 
 extern "C" {
 
-cublox_ms *cublox_ms_create(const long long limits[11]) {
+cublox_ms *cublox_ms_create(const long long limits[12]) {
   cublox_ms *out = nullptr;
   guarded([&] {
     cublox::ServerLimits l;
@@ -77,6 +77,7 @@ cublox_ms *cublox_ms_create(const long long limits[11]) {
     if (limits[8] > 0) l.point_arena_bytes = static_cast<std::size_t>(limits[8]);
     l.evict_idle = limits[9] != 0;
     l.pipelined = limits[10] != 0;
+    if (limits[11] > 0) l.chunk = static_cast<int>(limits[11]);
     auto *hd = new Handle;
     try {
       hd->server = std::make_unique<MS>(l);

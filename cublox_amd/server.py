@@ -31,14 +31,14 @@ class MapServer:
                  max_grids_per_tick: int = 256, max_tiles_per_tick: int = 1 << 21,
                  max_segments_per_ray: int = 96, graph: bool = True,
                  point_arena_bytes: int = 0, evict_idle: bool = False,
-                 pipelined: bool = False):
+                 pipelined: bool = False, chunk: int = 0):
         L = _ffi.host()
-        lim = (C.c_longlong * 11)(int(hbm_budget_bytes), int(max_sessions),
+        lim = (C.c_longlong * 12)(int(hbm_budget_bytes), int(max_sessions),
                                   int(max_points_per_scan), int(max_points_per_tick),
                                   int(max_grids_per_tick), int(max_tiles_per_tick),
                                   int(max_segments_per_ray), int(bool(graph)),
                                   int(point_arena_bytes), int(bool(evict_idle)),
-                                  int(bool(pipelined)))
+                                  int(bool(pipelined)), int(chunk))
         h = L.cublox_ms_create(lim)
         if not h:
             raise RuntimeError(L.cublox_og_last_error().decode())

@@ -74,7 +74,7 @@ struct GridScan {
 //               nonzero counts to with global atomics.
 struct TuneSetting {
   int near_steps{8};
-  int chunk{1024};
+  int chunk{8192};
   int lanes{0};
   int hot_mode{0};
 };

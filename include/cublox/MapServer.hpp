@@ -64,11 +64,18 @@ struct ServerLimits {
   // recently used idle session (nothing queued or in flight): its map goes
   // to a host snapshot and comes back, bit-identical, when it is readmitted.
   bool evict_idle{false};
-  // Overlap consecutive ticks (GridBatch pipelining: a tick's walk and
-  // binning beside the previous tick's folds, recenters and fetches; a
-  // second workspace). Every session's requests still take effect in
-  // submission order.
+  // Overlap consecutive ticks while requests back up (GridBatch
+  // pipelining: a tick's walk and binning beside the previous tick's folds,
+  // recenters and fetches; a second workspace); one tick at a time when a
+  // formed tick leaves less than half a tick of points queued. Every session's requests still take
+  // effect in submission order.
   bool pipelined{false};
+  // Records per hot-tile work item of the ticks (GridBatch TuneSetting
+  // chunk): ticks here are small, so smaller than the batch default.
+  int chunk{4096};
+  // Fetch samples a tick's output arena holds from the start (13 B each,
+  // device and pinned host, per arena); arenas grow past it on demand.
+  long long fetch_samples_per_tick{1 << 22};
 };
 
 class MapServer {
@@ -204,6 +211,7 @@ private:
   bool evictIdle();                          // worker thread, lock held
   bool idleVictim() const;                   // lock held
   bool formTick(Tick &t);                    // lock held
+  bool backlog() const;                      // lock held
   void submitTick(Tick &t);
   void enqueueFetches(Tick &t);
   void waitTick(Tick &t);

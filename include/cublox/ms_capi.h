@@ -24,8 +24,9 @@ typedef struct cublox_ms cublox_ms;
  * of the device arena holding queued scans' points (0: 4 GiB), [9] evict
  * idle sessions to host snapshots when a session with requests waits for
  * room (0/1), [10] pipelined ticks (0/1: consecutive ticks overlap on two
- * streams and workspaces). NULL on failure. */
-cublox_ms *cublox_ms_create(const long long limits[11]);
+ * streams and workspaces), [11] records per hot-tile work item of the
+ * ticks (0: 4096). NULL on failure. */
+cublox_ms *cublox_ms_create(const long long limits[12]);
 void cublox_ms_destroy(cublox_ms *h);
 /* Device bytes a session of this half size is charged (4 B per voxel plus
  * the fetch output reserve). */

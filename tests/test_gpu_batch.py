@@ -42,7 +42,7 @@ def _setup(cb, orc):
 
 
 def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=None,
-         setting=None, pipelined=False, check_every=1):
+         setting=None, pipelined=False, check_every=1, max_slabs=None):
     """Ticks every grid with its own scan and recenter; checks every grid
     against its oracle after every check_every-th tick (and the last).
     Pipelined: ticks are submitted without waits between the checks, so
@@ -50,6 +50,8 @@ def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=No
     grids, oracles, scenes = _setup(cb, orc)
     total = max_points or sum(b * a for (_, _, _, _, _, b, a, _, _) in GRIDS)
     kw = {} if max_tiles is None else {"max_tiles": max_tiles}
+    if max_slabs is not None:
+        kw["max_slabs"] = max_slabs
     batch = cb.GridBatch.for_grids(grids, total, max_near_steps=12, min_chunk=256,
                                    **kw)
     if graph:
@@ -155,3 +157,11 @@ def test_batch_pipelined_rejects_changed_voxels(cb, gpu_ready):
         batch.set_pipelined(True)
     batch.close()
     g.close()
+
+
+def test_batch_unsplit_hot_tiles(cb, orc, gpu_ready):
+    """Four scratch slabs for 256-record chunks: the hot tiles past them stay
+    whole and accumulate in passes (the unsplit hot-tile kernel)."""
+    batch = _run(cb, orc, 3, setting=(8, 256, 1, 0), max_slabs=4)
+    c = batch.counters()
+    assert c["unsplit_hot"] > 0 and c["split_tiles"] > 0, c
