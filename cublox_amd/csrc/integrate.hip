@@ -12,26 +12,27 @@
 //               LDS cube around the sensor voxel, flushed once per
 //               workgroup. Beyond, the walk is cut into "segments" at every
 //               entry into a 16x16x32 tile of the hash-ordered grid: each
-//               segment (32 B) carries the exact DDA state at the tile entry
-//               and goes to the ray's own slot range, its tile key to a
+//               segment (16 B) carries the exact DDA state at the tile entry
+//               and a packed word (the ray's increments are stored once per
+//               ray) and goes to the ray's own slot range, its tile key to a
 //               parallel array. The four-lane walk (launches of up to
 //               262,144 rays) also counts its workgroup's segments per tile
 //               in an LDS hash: one global atomic per (workgroup, tile).
 //   bin_count   (after the one-lane walk of large launches) the same
 //               counts, from the stored segments.
-//   tile_scan   tile cursors and work items — one per tile, or one per
-//               chunk of a hot tile — in decreasing cost order (log2
-//               classes); one workgroup for up to 12288 tiles, three
+//   tile_scan   tile run starts and the work-item plan — one item per
+//               tile, or one per chunk of a hot tile — in decreasing cost
+//               classes; one workgroup for up to 12288 tiles, three
 //               multi-workgroup passes beyond.
-//   bin_place   per workgroup, an LDS hash aggregates its segments by tile;
-//               one returning atomic per (workgroup, tile) reserves a
-//               range; each segment's slot index is written in tile order.
+//   place       each record's slot index written in tile order (its tile's
+//               run start + its counting group's base there + its rank).
 //   tile_apply  one workgroup per work item: replay every segment's voxels
 //               into packed 16+16-bit LDS counters (32 KB), then fold the
 //               tile's counts into occ with all loads in flight. Chunks of
 //               a hot tile store their counts to scratch slabs.
-//   hot_fold    sums each split tile's slabs and folds, 8 workgroups a tile.
-//   near_apply  fold the near-field cube into occ and clear it.
+//   hot_fold    sums each split tile's slabs and folds, 8 workgroups a tile;
+//               its last workgroups fold the near-field cubes into occ and
+//               clear them.
 //
 // One launch sequence serves one grid or a batch of grids (each with its
 // own geometry, sensor pose, points and occupancy buffer): per-grid
@@ -1863,14 +1864,25 @@ __global__ void __launch_bounds__(256) hot_zero_kernel(const Batch b, Workspace 
   }
 }
 
+template <bool kDelta>
+__device__ __forceinline__ void near_fold(const Batch &b, const Workspace &w, int blk,
+                                          int n_blk);
+
 // Sums the scratch slabs of each split tile and folds: kHotSlices
 // workgroups per tile, four consecutive voxels (one z-run piece) per lane.
+// Workgroups from hot_blocks on fold the near-field cubes instead (both
+// touch disjoint voxels and come after tile_apply: one launch for both).
 template <bool kDelta>
 __global__ void __launch_bounds__(256)
-    hot_fold_kernel(const Batch b, Workspace w) {
+    hot_fold_kernel(const Batch b, Workspace w, int hot_blocks) {
   __shared__ int s_tmp[8];
+  if (static_cast<int>(blockIdx.x) >= hot_blocks) {
+    near_fold<kDelta>(b, w, static_cast<int>(blockIdx.x) - hot_blocks,
+                      static_cast<int>(gridDim.x) - hot_blocks);
+    return;
+  }
   const int n_hot = w.misc[3];
-  for (int blk = blockIdx.x; blk < n_hot * kHotSlices; blk += gridDim.x) {
+  for (int blk = blockIdx.x; blk < n_hot * kHotSlices; blk += hot_blocks) {
     const HotTile ht = load_const(w.hot + blk / kHotSlices);
     const int k4 = (blk % kHotSlices) * 256 + threadIdx.x;  // < 2048
     const uint4 *src =
@@ -1943,14 +1955,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // ----------------------------------------------------------------------
-// near_apply: fold the near-field cubes, clear them
+// near fold: the near-field cubes into occ, cleared (workgroup blk of
+// n_blk, 256 threads)
 // ----------------------------------------------------------------------
 template <bool kDelta>
-__global__ void __launch_bounds__(256)
-    near_apply_kernel(const Batch b, Workspace w) {
+__device__ __forceinline__ void near_fold(const Batch &b, const Workspace &w, int blk,
+                                          int n_blk) {
   const int n_near = batch_near(b);
-  for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < n_near;
-       cell += gridDim.x * blockDim.x) {
+  for (int cell = blk * 256 + static_cast<int>(threadIdx.x); cell < n_near;
+       cell += n_blk * 256) {
     const int o = w.near_op[cell];
     if (o == 0) continue;
     const int hit = w.near_hit[cell];
@@ -2047,6 +2060,7 @@ struct Caps {
 struct Layout {
   Caps cap;
   long long slabs;
+  bool may_run_out;  // slabs below the chunk bound: hot tiles may stay unsplit
   int max_items, scan_blocks;
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
@@ -2080,6 +2094,7 @@ Layout make_layout(const Caps &c) {
   // capped; hot tiles past the cap run unsplit.
   const long long chunks = 2 * c.seg_capacity / std::max(c.chunk, kMinChunk) + 2;
   L.slabs = std::max<long long>(2, std::min<long long>(chunks, c.max_slabs));
+  L.may_run_out = chunks > c.max_slabs;
   L.max_items = static_cast<int>(std::min<long long>(c.max_tiles + chunks, 1LL << 30));
   L.scan_blocks = (c.max_tiles + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
   size_t o = 0;
@@ -2281,19 +2296,21 @@ int launch_folds(const Batch &b, const Layout &L, const Workspace &w,
   tile_apply_kernel<false, kDelta><<<dim3(apply_blocks), dim3(kApplyThreads), 0, hs>>>(b, w);
   int st = check_launch("tile_apply_kernel");
   if (st != CUBLOX_OK) return st;
-  tile_apply_kernel<true, kDelta><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
-  st = check_launch("tile_apply_kernel<big>");
-  if (st != CUBLOX_OK) return st;
+  // Hot tiles stay unsplit only when the scratch slabs run out; with
+  // slabs for every possible chunk the pass is not launched.
+  if (L.may_run_out) {
+    tile_apply_kernel<true, kDelta><<<dim3(kBigBlocks), dim3(kApplyThreads), 0, hs>>>(b, w);
+    st = check_launch("tile_apply_kernel<big>");
+    if (st != CUBLOX_OK) return st;
+  }
   // A grid-stride loop over the split tiles' slices: a few workgroups per
-  // CU (empty workgroups of a larger grid only cost dispatch time).
-  const long long hot_blocks =
-      std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 4096);
-  hot_fold_kernel<kDelta><<<dim3(static_cast<unsigned>(hot_blocks)), dim3(256), 0, hs>>>(b, w);
-  st = check_launch("hot_fold_kernel");
-  if (st != CUBLOX_OK) return st;
-  near_apply_kernel<kDelta><<<dim3(std::max(1, std::min((c.max_near + 255) / 256, 1024))),
-                              dim3(256), 0, hs>>>(b, w);
-  return check_launch("near_apply_kernel");
+  // CU (empty workgroups of a larger grid only cost dispatch time); then
+  // the near-field cubes' workgroups.
+  const int hot_blocks = static_cast<int>(std::min<long long>((L.slabs / 2 + 1) * kHotSlices, 4096));
+  const int near_blocks = std::max(1, std::min((c.max_near + 255) / 256, 1024));
+  hot_fold_kernel<kDelta><<<dim3(static_cast<unsigned>(hot_blocks + near_blocks)), dim3(256), 0,
+                            hs>>>(b, w, hot_blocks);
+  return check_launch("hot_fold_kernel");
 }
 
 // Parts of a launch sequence: the kernels that read no occupancy (walk,
