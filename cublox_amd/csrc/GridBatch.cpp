@@ -82,9 +82,14 @@ GridBatch::GridBatch(const BatchLimits &limits) : lim_(limits) {
 
 GridBatch::~GridBatch() { release(); }
 
-void GridBatch::allocSlot(Slot &s) {
+void GridBatch::allocSlot(Slot &s, bool dedicated) {
   const cublox_batch_limits c = abiLimits(lim_);
-  CUBLOX_CHECK(cublox_stream_create(&s.stream));
+  if (dedicated) {
+    CUBLOX_CHECK(cublox_stream_create_dedicated(&s.stream));
+  } else {
+    CUBLOX_CHECK(cublox_stream_create(&s.stream));
+  }
+  s.dedicated = dedicated;
   for (auto &e : s.ev) CUBLOX_CHECK(cublox_event_create(&e));
   CUBLOX_CHECK(cublox_malloc(&s.d_ws, ws_bytes_));
   CUBLOX_CHECK(cublox_integrate_workspace_init(s.d_ws, ws_bytes_, s.stream));
@@ -169,9 +174,18 @@ void GridBatch::setPipelined(bool on) {
   }
   wait();
   if (on) {
+    // Overlapped ticks need their two streams on different hardware
+    // queues: both slots take streams with queues of their own.
+    if (!slots_[0].dedicated) {
+      void *old = slots_[0].stream;
+      CUBLOX_CHECK(cublox_stream_synchronize(old));
+      CUBLOX_CHECK(cublox_stream_create_dedicated(&slots_[0].stream));
+      slots_[0].dedicated = true;
+      cublox_stream_destroy(old);
+    }
     if (!slots_[1].stream) {
       try {
-        allocSlot(slots_[1]);
+        allocSlot(slots_[1], true);
       } catch (...) {
         releaseSlot(slots_[1]);
         throw;

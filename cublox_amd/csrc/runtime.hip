@@ -141,6 +141,21 @@ int cublox_stream_create(cublox_stream *s) {
   return CUBLOX_OK;
 }
 
+int cublox_stream_create_dedicated(cublox_stream *s) {
+  if (!s) return set_error(CUBLOX_ERR_INVALID, "null stream out");
+  int dev = 0, cus = 0;
+  CUBLOX_TRY(hipGetDevice(&dev));
+  CUBLOX_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (cus <= 0) return set_error(CUBLOX_ERR_RUNTIME, "stream_create_dedicated: no CUs");
+  // Every CU enabled: the mask only gives the stream a queue of its own.
+  std::vector<uint32_t> mask(static_cast<size_t>((cus + 31) / 32), 0xFFFFFFFFu);
+  if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
+  hipStream_t st = nullptr;
+  CUBLOX_TRY(hipExtStreamCreateWithCUMask(&st, static_cast<uint32_t>(mask.size()), mask.data()));
+  *s = reinterpret_cast<cublox_stream>(st);
+  return CUBLOX_OK;
+}
+
 int cublox_stream_destroy(cublox_stream s) {
   if (!s) return CUBLOX_OK;
   CUBLOX_TRY(hipStreamDestroy(as_stream(s)));

@@ -159,6 +159,7 @@ private:
     void *front[2][13]{};
     void *back[2]{nullptr, nullptr};
     void *ev[4]{nullptr, nullptr, nullptr, nullptr};  // start, folds done, end, order mark
+    bool dedicated{false};  // its stream has a hardware queue of its own
     bool in_flight{false};
     bool alone{true};     // no other tick overlapped it
     int tune_idx{-1};     // autotuner candidate it ran, or -1
@@ -167,7 +168,7 @@ private:
     int hot_mode{0};
     long long zeroed{0};  // voxels its recenters zero
   };
-  void allocSlot(Slot &s);
+  void allocSlot(Slot &s, bool dedicated = false);
   void releaseSlot(Slot &s) noexcept;
   void dropGraphs(Slot &s);
   void stage(Slot &s, const std::vector<GridScan> &scans,

@@ -70,6 +70,12 @@ int cublox_memcpy_d2h_async(void *dst, const void *src, size_t bytes,
 int cublox_memcpy_d2d_async(void *dst, const void *src, size_t bytes,
                             cublox_stream s);
 int cublox_stream_create(cublox_stream *s);
+/* A stream on a hardware queue of its own (no reference counterpart): a
+ * CU-masked stream (hipExtStreamCreateWithCUMask) with every CU enabled.
+ * Plain streams share the runtime's pool of hardware queues, and two
+ * streams on one queue run one after the other; GridBatch's overlapped
+ * ticks take these. */
+int cublox_stream_create_dedicated(cublox_stream *s);
 int cublox_stream_destroy(cublox_stream s);
 int cublox_stream_synchronize(cublox_stream s);
 int cublox_device_synchronize(void);
