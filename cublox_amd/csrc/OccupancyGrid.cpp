@@ -81,9 +81,7 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   static_cast<Grid &>(*this) = static_cast<const Grid &>(o);
   first_run_ = o.first_run_;
   stream_ = std::exchange(o.stream_, nullptr);
-  zstream_ = std::exchange(o.zstream_, nullptr);
-  ev_main_ = std::exchange(o.ev_main_, nullptr);
-  ev_zero_ = std::exchange(o.ev_zero_, nullptr);
+  zplan_ = std::move(o.zplan_);
   zero_pending_ = std::exchange(o.zero_pending_, false);
   d_op_cnt_ = std::exchange(o.d_op_cnt_, nullptr);
   d_hit_cnt_ = std::exchange(o.d_hit_cnt_, nullptr);
@@ -142,15 +140,8 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
 }
 
 void OccupancyGrid::releaseAll() noexcept {
-  if (zstream_) cublox_stream_synchronize(zstream_);
   if (stream_) cublox_stream_synchronize(stream_);
-  zero_pending_ = false;
-  for (void **e : {&ev_main_, &ev_zero_}) {
-    if (*e) cublox_event_destroy(*e);
-    *e = nullptr;
-  }
-  if (zstream_) cublox_stream_destroy(zstream_);
-  zstream_ = nullptr;
+  zero_pending_ = false;  // the map goes: its pending clear with it
   freeDevice(d_fetch_state_);
   freeDevice(d_fetch_pos_);
   freeDevice(d_fetch_count_);
@@ -214,19 +205,19 @@ void OccupancyGrid::mark(int ev) const {
 
 void OccupancyGrid::join() const {
   if (!zero_pending_) return;
-  CUBLOX_CHECK(cublox_stream_wait_event(stream_, ev_zero_));
+  CUBLOX_CHECK(cublox_launch_recenter_plan(zplan_.data(), d_occ_, d_op_cnt_, d_hit_cnt_,
+                                           stream_));
   zero_pending_ = false;
 }
 
 void OccupancyGrid::settle() const {
   if (!zero_pending_) return;
-  CUBLOX_CHECK(cublox_stream_synchronize(zstream_));
-  zero_pending_ = false;
+  join();
+  sync();
 }
 
 OccupancyGrid::KernelTimes OccupancyGrid::kernelTimes() const {
   if (timing_) {
-    if (zstream_) CUBLOX_CHECK(cublox_stream_synchronize(zstream_));
     sync();
     harvestTimes();
   }
@@ -317,32 +308,27 @@ void OccupancyGrid::recenter(const Eigen::Vector3f &pos) {
   cfg.origin_at_center = config_.origin_at_center ? 1 : 0;
   cfg.map_vox_num = config_.voxel_num;
 
-  // The plan is host arithmetic; only an action needs device work.
-  cublox_recenter_result plan{};
-  CUBLOX_CHECK(cublox_plan_recenter(&cfg, nullptr, nullptr, nullptr, &plan));
-  if (plan.action == 0) return;
-  if (!zstream_) {
-    CUBLOX_CHECK(cublox_stream_create(&zstream_));
-    CUBLOX_CHECK(cublox_event_create(&ev_main_));
-    CUBLOX_CHECK(cublox_event_create(&ev_zero_));
-  }
-  if (timing_ && pending_[2]) {  // the previous clear's times first
-    CUBLOX_CHECK(cublox_stream_synchronize(zstream_));
-    harvestTimes();
-  }
-  // The clear follows everything enqueued on the grid so far.
-  CUBLOX_CHECK(cublox_event_record(ev_main_, stream_));
-  CUBLOX_CHECK(cublox_stream_wait_event(zstream_, ev_main_));
+  // The plan is host arithmetic; only an action needs device work. An
+  // earlier clear still pending runs first (clears follow each other).
+  join();
+  if (zplan_.empty()) zplan_.resize(cublox_recenter_plan_bytes());
   cublox_recenter_result result{};
-  if (timing_) CUBLOX_CHECK(cublox_event_record(events_[4], zstream_));
-  CUBLOX_CHECK(cublox_launch_recenter(d_occ_, d_op_cnt_, d_hit_cnt_, &cfg, zstream_,
-                                      &result));
-  if (timing_) CUBLOX_CHECK(cublox_event_record(events_[5], zstream_));
-  CUBLOX_CHECK(cublox_event_record(ev_zero_, zstream_));
-  zero_pending_ = true;
+  CUBLOX_CHECK(cublox_recenter_plan_stage(&cfg, zplan_.data(), &result));
+  if (result.action == 0) return;
   if (timing_) {
+    // Timed: the clear runs now, on its own, between the recenter events.
+    if (pending_[2]) {  // the previous clear's times first
+      CUBLOX_CHECK(cublox_event_synchronize(events_[5]));
+      harvestTimes();
+    }
+    CUBLOX_CHECK(cublox_event_record(events_[4], stream_));
+    CUBLOX_CHECK(cublox_launch_recenter_plan(zplan_.data(), d_occ_, d_op_cnt_, d_hit_cnt_,
+                                             stream_));
+    CUBLOX_CHECK(cublox_event_record(events_[5], stream_));
     pending_[2] = true;
     pending_zeroed_ += result.zeroed_voxels;
+  } else {
+    zero_pending_ = true;  // the next update's walk launch, or join()
   }
   updateOriginAndBound(Eigen::Vector3i(result.new_origin_i.x,
                                        result.new_origin_i.y,
@@ -517,8 +503,10 @@ void OccupancyGrid::launchIntegrate(const float *d_xs, const float *d_ys,
   }
   q.point_stride = stride;
   q.range_drop_sq = drop_sq;
-  // A pending recenter clear: the walk overlaps it, the folds wait for it.
-  q.occ_ready = zero_pending_ ? ev_zero_ : nullptr;
+  // A pending recenter clear rides in the walk launch (the folds follow).
+  q.zero_plan = zero_pending_ ? zplan_.data() : nullptr;
+  q.d_zero_op = d_op_cnt_;
+  q.d_zero_hit = d_hit_cnt_;
   mark(0);
   CUBLOX_CHECK(cublox_launch_integrate_grid(&q, d_ws_, ws_bytes_, ws_points_, stream_));
   mark(1);

@@ -46,6 +46,7 @@
 // near-field cube and the tiles own disjoint voxel sets and each voxel is
 // folded exactly once.
 #include "raycast_common.hpp"
+#include "zero_regions.hpp"
 
 #include <algorithm>
 #include <cstddef>
@@ -726,6 +727,28 @@ __device__ __forceinline__ int lds_hash_slot(int *keys, int key, bool insert) {
 // ----------------------------------------------------------------------
 constexpr int kLenBuckets = 64;
 
+// A pending recenter's slab clear that a single-grid walk launch carries:
+// its last `chunks` workgroups zero the regions (and the counters, when
+// the grid has them) while the others walk. Passed by value.
+constexpr int kInlineRegions = 16;
+struct InlineClear {
+  float *occ;
+  int *op, *hit;
+  int n;
+  long long chunks;
+  ChunkedRegion r[kInlineRegions];
+};
+
+// Workgroup j of an inline clear (every thread of the workgroup calls it).
+__device__ __forceinline__ void inline_clear(const InlineClear &z, long long j) {
+  int k = 0;  // the region holding chunk j (chunk0 ascending)
+  while (k + 1 < z.n && z.r[k + 1].chunk0 <= j) ++k;
+  const ZeroRegion part = chunk_part(z.r[k], j - z.r[k].chunk0);
+  zero_region(as_global(z.occ), part, 0, 1);
+  if (z.op) zero_region(as_global(z.op), part, 0, 1);
+  if (z.hit) zero_region(as_global(z.hit), part, 0, 1);
+}
+
 // Length class of point i for the in-block ordering: its range in voxels
 // (clipped to the max range), 8 voxels per class. Any value is correct;
 // similar classes in one wave only shorten the longest lane's walk.
@@ -751,7 +774,13 @@ __device__ __forceinline__ int length_class(const GridDesc &d, int i) {
 // its own, whichever lane walks it.
 template <int kLanes>
 __global__ void __launch_bounds__(kWalkBlock * kLanes, kLanes == 1 ? 8 : 1)
-    walk_kernel(const Batch b, Workspace w) {
+    walk_kernel(const Batch b, Workspace w, const InlineClear zc) {
+  // The last zc.chunks workgroups clear a pending recenter's slabs.
+  const long long clear0 = static_cast<long long>(gridDim.x) - zc.chunks;
+  if (static_cast<long long>(blockIdx.x) >= clear0) {
+    inline_clear(zc, static_cast<long long>(blockIdx.x) - clear0);
+    return;
+  }
   extern __shared__ int s_near[];  // the grid's near cube, sized at launch
   __shared__ int s_hist[kLenBuckets];
   __shared__ unsigned short s_perm[kWalkBlock];
@@ -2325,7 +2354,8 @@ constexpr int kPartFront = 1, kPartBack = 2, kPartBoth = 3;
 int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
                     bool delta, hipStream_t hs, hipEvent_t occ_ready = nullptr,
                     int batch_lanes = 0, bool shared = false,
-                    int parts = kPartBoth, int cube = 0) {
+                    int parts = kPartBoth, int cube = 0,
+                    const InlineClear *clear = nullptr) {
   if (!(parts & kPartFront)) {
     return delta ? launch_folds<true>(b, L, w, hs, shared)
                  : launch_folds<false>(b, L, w, hs, shared);
@@ -2341,10 +2371,12 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
   // staged header names (the autotuner's lanes per ray; a captured sequence
   // holds one of them, so a graph is per lanes value).
   const bool quad = batch_lanes ? batch_lanes == 4 : c.max_rays <= kQuadWalkRays;
+  const InlineClear zc = clear ? *clear : InlineClear{};
+  const unsigned grid = walk_blocks + static_cast<unsigned>(zc.chunks);
   if (quad) {
-    walk_kernel<4><<<dim3(walk_blocks), dim3(4 * kWalkBlock), near_lds, hs>>>(b, w);
+    walk_kernel<4><<<dim3(grid), dim3(4 * kWalkBlock), near_lds, hs>>>(b, w, zc);
   } else {
-    walk_kernel<1><<<dim3(walk_blocks), dim3(kWalkBlock), near_lds, hs>>>(b, w);
+    walk_kernel<1><<<dim3(grid), dim3(kWalkBlock), near_lds, hs>>>(b, w, zc);
   }
   int st = check_launch("walk_kernel");
   if (st != CUBLOX_OK) return st;
@@ -2560,25 +2592,57 @@ int cublox_launch_integrate_delta(
   return cublox_launch_integrate_grid(&q, d_ws, ws_bytes, max_points, s);
 }
 
+// Target workgroups of an inline recenter clear: about one per CU, beside
+// the walk's.
+constexpr long long kInlineClearBlocks = 256;
+
 int cublox_launch_integrate_grid(const cublox_integrate_grid *q, void *d_ws,
                                  size_t ws_bytes, int max_points,
                                  cublox_stream s) {
   if (!q) return set_error(CUBLOX_ERR_INVALID, "integrate: null grid");
+  // A staged recenter clear goes into the walk launch when it is a slab
+  // clear of few regions, else it is launched on its own first.
+  InlineClear zc{};
+  const StagedClear *plan = static_cast<const StagedClear *>(q->zero_plan);
+  bool inline_ok = plan && q->d_occ && plan->action == 1 && plan->n > 0 &&
+                   plan->n <= kInlineRegions;
   if (q->n <= 0 || !q->d_xs || !q->d_ys || !q->d_zs || !q->d_occ) {
-    // Nothing to integrate: nothing changed.
+    // Nothing to integrate: nothing changed (but a staged clear still runs).
+    if (plan) {
+      const int st = cublox_launch_recenter_plan(plan, q->d_occ, q->d_zero_op,
+                                                 q->d_zero_hit, s);
+      if (st != CUBLOX_OK) return st;
+    }
     return q->d_delta_count ? cublox_fill_zero_async(q->d_delta_count, 1, s) : CUBLOX_OK;
   }
   const int near_steps = q->near_steps > 0 ? q->near_steps : 8;
   Caps caps{};
   Batch b{};
-  const int st = prepare_grid(q, max_points, near_steps, caps, b);
+  int st = prepare_grid(q, max_points, near_steps, caps, b);
   if (st != CUBLOX_OK) return st;
   const Layout L = make_layout(caps);
   if (!d_ws || ws_bytes < L.total) {
     return set_error(CUBLOX_ERR_CAPACITY, "integrate: workspace too small");
   }
+  if (plan && !inline_ok) {
+    st = cublox_launch_recenter_plan(plan, q->d_occ, q->d_zero_op, q->d_zero_hit, s);
+    if (st != CUBLOX_OK) return st;
+  }
+  if (inline_ok) {
+    long long elems = 0;
+    for (int k = 0; k < plan->n; ++k) elems += plan->r[k].n_outer * plan->r[k].run;
+    const long long piece = chunk_piece(elems, kInlineClearBlocks);
+    zc.occ = q->d_occ;
+    zc.op = q->d_zero_op;
+    zc.hit = q->d_zero_hit;
+    zc.n = plan->n;
+    for (int k = 0; k < plan->n; ++k) {
+      zc.chunks += chunk_region(plan->r[k], piece, zc.chunks, &zc.r[k]);
+    }
+  }
   return launch_sequence(b, L, bind(L, d_ws), q->d_delta_count != nullptr,
-                         as_stream(s), reinterpret_cast<hipEvent_t>(q->occ_ready));
+                         as_stream(s), reinterpret_cast<hipEvent_t>(q->occ_ready), 0, false,
+                         kPartBoth, 0, inline_ok ? &zc : nullptr);
 }
 
 int cublox_launch_integrate_noalloc(const cublox_raycast_cfg *cfg, const float *T,

@@ -319,20 +319,19 @@ private:
 
   void *stream_{nullptr};
 
-  // A recenter's slab clear runs on zstream_ after the work enqueued on
-  // stream_ before it (ev_main_), and returns without waiting; ev_zero_
-  // marks its end. The next fused update's folds wait for ev_zero_ (its ray
-  // walk overlaps the clear); every other call that reads or writes occ or
-  // the counters makes stream_ wait for it first (join).
-  void *zstream_{nullptr};
-  void *ev_main_{nullptr};
-  void *ev_zero_{nullptr};
+  // A recenter plans its slab clear on the host into zplan_ and returns:
+  // the next fused update carries it into its walk launch, whose extra
+  // workgroups clear the slabs beside the walk (the folds come after);
+  // every other call that reads or writes occ or the counters first
+  // launches it on stream_ (join).
+  std::vector<unsigned char> zplan_;
   mutable bool zero_pending_{false};
   void join() const;
-  void settle() const;  // host wait for a pending clear (for other streams)
+  void settle() const;  // join, then a host wait (for work on other streams)
 
-  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter (on
-  // zstream_), [6,7] fetch.
+  // Timing events: [0,1] ray cast, [2,3] apply, [4,5] recenter (with
+  // timing on, a recenter launches its clear at once, between them), [6,7]
+  // fetch.
   bool timing_{false};
   void *events_[8]{nullptr, nullptr, nullptr, nullptr,
                    nullptr, nullptr, nullptr, nullptr};

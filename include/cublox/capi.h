@@ -193,6 +193,19 @@ int cublox_launch_recenter(float *d_occ, int *d_op_cnt, int *d_hit_cnt,
                            const cublox_recenter_cfg *cfg, cublox_stream s,
                            cublox_recenter_result *result);
 
+/* A recenter's clear staged for later (no reference counterpart): the
+ * plan of cublox_launch_recenter made on the host into `plan` (host memory
+ * of cublox_recenter_plan_bytes() bytes) without launching anything.
+ * cublox_launch_recenter_plan then clears occ (and the counters when
+ * non-NULL) on s, as cublox_launch_recenter would have; or the plan rides
+ * in the next fused update (cublox_integrate_grid.zero_plan), whose walk
+ * launch clears the slabs with workgroups of its own, beside the walk.   */
+size_t cublox_recenter_plan_bytes(void);
+int cublox_recenter_plan_stage(const cublox_recenter_cfg *cfg, void *plan,
+                               cublox_recenter_result *result);
+int cublox_launch_recenter_plan(const void *plan, float *d_occ, int *d_op_cnt,
+                                int *d_hit_cnt, cublox_stream s);
+
 /* launchClearRecenterSlabsForAxis (kernels.hpp:112-117): zero every voxel
  * whose local index on `axis` is listed in the DEVICE array d_slices. */
 /* Batched recenter (no reference counterpart): the plans of many grids,
@@ -501,6 +514,13 @@ typedef struct {
    * read neither, start before it completes. A recenter's slab clear on
    * another stream then overlaps the next update's ray walk. NULL: none. */
   cublox_event occ_ready;
+  /* Optional (single-grid launches): a recenter's clear staged with
+   * cublox_recenter_plan_stage for this grid and not yet launched. The
+   * update clears it first: its walk launch takes extra workgroups that
+   * zero the slabs of d_occ (and of d_zero_op / d_zero_hit when non-NULL)
+   * while the walk runs; the folds come after both. NULL: none.           */
+  const void *zero_plan;
+  int *d_zero_op, *d_zero_hit;
 } cublox_integrate_grid;
 
 /* Single-grid fused update from a batch item (every option above; no

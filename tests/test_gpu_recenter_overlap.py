@@ -1,8 +1,9 @@
-"""A recenter returns before its slab clear has run (the clear goes to the
-grid's side stream and the next update's ray walk overlaps it). Every later
-call must still see the cleared window, bit for bit as the oracle
-(reference OccupancyGrid.cpp:141-179 then the next call):
-  * update right after a recenter (the folds wait for the clear);
+"""A recenter returns before its slab clear has run (the clear is planned
+on the host and rides in the next update's walk launch, whose extra
+workgroups clear the slabs beside the walk; any other call launches it
+first). Every later call must still see the cleared window, bit for bit as
+the oracle (reference OccupancyGrid.cpp:141-179 then the next call):
+  * update right after a recenter (the folds come after the clear);
   * two recenters in a row, then reads (fetch, point query, occupancy);
   * a recenter, then a batched tick of the same grid on another stream;
   * the per-voxel path and the stage-wise calls after a recenter.
