@@ -1203,9 +1203,24 @@ __global__ void __launch_bounds__(kScanThreads)
   const bool shared = batch_shared(b);
   const int t = threadIdx.x;
   if (t == 0) w.misc[4] = 0;
-  for (int k = t; k < n_tiles; k += kScanThreads) {
-    s_count[k] = w.tile_count[k];
-    w.tile_count[k] = 0;
+  {
+    // All of a thread's counts in flight at once (one memory round trip),
+    // then staged in LDS and cleared for the next launch.
+    constexpr int kPer = kScanStage / kScanThreads;
+    int v[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int k = t + i * kScanThreads;
+      v[i] = k < n_tiles ? w.tile_count[k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int k = t + i * kScanThreads;
+      if (k < n_tiles) {
+        s_count[k] = v[i];
+        w.tile_count[k] = 0;
+      }
+    }
   }
   __syncthreads();
   const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
@@ -1541,14 +1556,24 @@ __global__ void __launch_bounds__(kWalkBlock)
         rk[q] = w.seg_rank[slot0 + (j0 + q) * stride];
       }
     }
+    // The records' run starts and group bases, all in flight together.
+    int run[kPlaceJ], base[kPlaceJ];
 #pragma unroll
     for (int q = 0; q < kPlaceJ; ++q) {
-      if (j0 + q >= cnt) break;
-      long long pos = w.tile_cursor[tile[q]];
-      pos += (rk[q] & kRankGlobal)
-                 ? static_cast<long long>(rk[q] & ~kRankGlobal)
-                 : static_cast<long long>(gbase[rk[q] >> kRankBits]) +
-                       static_cast<long long>(rk[q] & ((1u << kRankBits) - 1u));
+      if (j0 + q < cnt) {
+        run[q] = w.tile_cursor[tile[q]];
+        base[q] = (rk[q] & kRankGlobal) ? 0 : gbase[rk[q] >> kRankBits];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPlaceJ; ++q) {
+      if (j0 + q >= cnt) continue;
+      const long long pos =
+          static_cast<long long>(run[q]) +
+          ((rk[q] & kRankGlobal)
+               ? static_cast<long long>(rk[q] & ~kRankGlobal)
+               : static_cast<long long>(base[q]) +
+                     static_cast<long long>(rk[q] & ((1u << kRankBits) - 1u)));
       if (pos < w.seg_capacity) {
         w.order[pos] = static_cast<unsigned>(slot0 + (j0 + q) * stride);
       }
