@@ -91,35 +91,55 @@ def new_grid(cb, w, path=0):
     return g
 
 
-def run_gpu(cb, w, scans, bufs, n, warmup, steps, path=0, timing=True):
-    """The timed per-scan loop. With timing, HIP events around every kernel
-    launch feed kernel_times(); the headline pass runs without them, so the
-    wall clock holds only the drop-in calls."""
+def run_gpu(cb, w, scans, bufs, n, warmup, steps, path=0, timing=True, native=True):
+    """The timed per-scan loop: per scan the blocking update on the scan's
+    HBM-resident columns, then the recenter. native: the loop runs in C++
+    (cublox_seq_replay, as a C++ driver calls the drop-in API); else in
+    Python, one binding call per call. With timing, HIP events around every
+    kernel launch feed kernel_times(); the headline pass runs without them,
+    so the wall clock holds only the drop-in calls."""
+    from cublox_amd.grid import Replay
     g = new_grid(cb, w, path)
     g.enable_kernel_timing(timing)
 
-    moved = []
-
-    def step(i):
+    def spec(i):
         off = 4 * n * i
-        s = scans[i]
-        g.update_sensor_frame_device(bufs[0].ptr + off, bufs[1].ptr + off,
-                                     bufs[2].ptr + off, n, s.T)
-        moved.append(bool(g.recenter(s.base)))
+        return (g, bufs[0].ptr + off, bufs[1].ptr + off, bufs[2].ptr + off, n,
+                scans[i].T, scans[i].base)
 
-    for i in range(warmup):
-        step(i)
-    g.reset_kernel_times()
-    cb.synchronize()
-    per = []
-    moved.clear()
-    t0 = time.perf_counter()
-    for i in range(warmup, warmup + steps):
-        a = time.perf_counter()
-        step(i)
-        per.append((time.perf_counter() - a) * 1e3)
-    cb.synchronize()
-    t1 = time.perf_counter()
+    if native:
+        warm = Replay([spec(i) for i in range(warmup)])
+        timed = Replay([spec(i) for i in range(warmup, warmup + steps)])
+        warm.run()
+        g.reset_kernel_times()
+        cb.synchronize()
+        t0 = time.perf_counter()
+        per, moved = timed.run()
+        cb.synchronize()
+        t1 = time.perf_counter()
+    else:
+        moved = []
+
+        def step(i):
+            off = 4 * n * i
+            s = scans[i]
+            g.update_sensor_frame_device(bufs[0].ptr + off, bufs[1].ptr + off,
+                                         bufs[2].ptr + off, n, s.T)
+            moved.append(bool(g.recenter(s.base)))
+
+        for i in range(warmup):
+            step(i)
+        g.reset_kernel_times()
+        cb.synchronize()
+        per = []
+        moved.clear()
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            a = time.perf_counter()
+            step(i)
+            per.append((time.perf_counter() - a) * 1e3)
+        cb.synchronize()
+        t1 = time.perf_counter()
     kt = g.kernel_times()
     run_gpu.moved = moved
     return g, per, (t1 - t0) * 1e3, kt
@@ -161,19 +181,17 @@ def census(cb, w, scans, bufs, n, warmup, steps):
 
 
 def run_pcie(cb, w, scans, warmup, steps):
-    """Same steps with host-resident points (H2D inside update)."""
+    """Same steps with host-resident points (H2D inside update), in the
+    native loop."""
+    from cublox_amd.grid import Replay
     g = new_grid(cb, w)
-    for i in range(warmup):
+
+    def spec(i):
         s = scans[i]
-        g.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
-        g.recenter(s.base)
-    per = []
-    for i in range(warmup, warmup + steps):
-        s = scans[i]
-        a = time.perf_counter()
-        g.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
-        g.recenter(s.base)
-        per.append((time.perf_counter() - a) * 1e3)
+        return (g, s.xs, s.ys, s.zs, s.n, s.T, s.base)
+
+    Replay([spec(i) for i in range(warmup)], on_host=True).run()
+    per, _ = Replay([spec(i) for i in range(warmup, warmup + steps)], on_host=True).run()
     g.close()
     return per
 
@@ -229,19 +247,18 @@ def run_batched(cb, tune_ticks_each, ticks):
         plain[g].update_sensor_frame_device(b[0].ptr, b[1].ptr, b[2].ptr, scans[g][kmax].n,
                                             scans[g][kmax].T)
         plain[g].reset()
+    # Each tick's 64 blocking update + recenter calls run in the native loop
+    # (cublox_seq_replay), prepared before the clock starts.
+    from cublox_amd.grid import Replay
     per_scan, tick_ms = [], []
     for k in range(total):
+        rp = Replay([(plain[g], *(b.ptr for b in bufs[g][k]), scans[g][k].n, scans[g][k].T,
+                      scans[g][k].base) for g in range(G)])
         a = time.perf_counter()
-        for g in range(G):
-            s = scans[g][k]
-            b = bufs[g][k]
-            c0 = time.perf_counter()
-            plain[g].update_sensor_frame_device(b[0].ptr, b[1].ptr, b[2].ptr, s.n, s.T)
-            plain[g].recenter(s.base)
-            if k in timed or k in piped:
-                per_scan.append((time.perf_counter() - c0) * 1e3)
+        step_ms, _ = rp.run()
         if k in timed or k in piped:
             tick_ms.append((time.perf_counter() - a) * 1e3)
+            per_scan.extend(step_ms)
     for x in plain:
         x.close()
     plain_sps = G * len(tick_ms) / (sum(tick_ms) * 1e-3)
@@ -450,19 +467,25 @@ def run_served(cb, n_sessions=256, seconds=3.0, loads=(0.5, 0.8, 1.0), depth=4,
         x.set_log_odds_params(*cb.log_odds())
         plain.append(x)
 
+    # The rounds run in the native loop (cublox_seq_replay), prepared
+    # before the clock starts.
+    from cublox_amd.grid import Replay
+
     def plain_round(j):
+        steps = []
         for x, (xs, ys, zs, T, base) in zip(plain, templates):
             Tj = np.array(T, np.float32)
             Tj[0, 3] += 0.5 * j
             b = np.array(base, np.float32)
             b[0] += 0.5 * j
-            x.update_sensor_frame((xs, ys, zs), Tj)
-            x.recenter(b)
+            steps.append((x, xs, ys, zs, xs.size, Tj, b))
+        return Replay(steps, on_host=True)
 
-    plain_round(0)  # first origins, workspaces
+    plain_round(0).run()  # first origins, workspaces
+    rounds = [plain_round(j) for j in range(1, depth + 1)]
     t = time.perf_counter()
-    for j in range(1, depth + 1):
-        plain_round(j)
+    for rp in rounds:
+        rp.run()
     plain_sps = n_sessions * depth / (time.perf_counter() - t)
     for x in plain:
         x.close()
@@ -829,6 +852,12 @@ def main():
     rc_cost = recenter_cost(per, run_gpu.moved)
     log(f"recenter critical path: {rc_cost}")
     g0.close()
+    # The same blocking calls made one by one from Python (one binding call
+    # each): the interpreter's share of a step.
+    gp, per_p, wall_p, _ = run_gpu(cb, w, scans, bufs, n, args.warmup, args.steps,
+                                   timing=False, native=False)
+    gp.close()
+    log(f"python loop: {wall_p / args.steps:.4f} ms/scan")
     g, per_t, wall_t, kt = run_gpu(cb, w, scans, bufs, n, args.warmup, args.steps)
     log(f"with timing events: {wall_t / args.steps:.4f} ms/scan")
     ms_per_step = wall_ms / args.steps
@@ -901,7 +930,7 @@ def main():
             ms_per_scan=round(statistics.mean(pcie), 5),
             p50_ms=round(pct(pcie, 50), 5), p99_ms=round(pct(pcie, 99), 5),
             form="blocking updateSensorFrame on host columns (staged through a "
-                 "grid-owned pinned buffer, then DMA) + recenter",
+                 "grid-owned pinned buffer, then DMA) + recenter, native loop",
             speedup_vs_3090_readme=round(README_3090_MS / statistics.mean(pcie), 2),
             target_ms=0.5)
         log(f"h2d-inclusive: {extras['h2d_inclusive']}")
@@ -954,9 +983,11 @@ def main():
                         f"{w.speed} m/scan, recenter threshold {w.threshold} m",
             "voxels": w.voxels,
             "points_per_scan": w.points,
-            "form": "plain per-scan, blocking drop-in calls, inputs HBM-resident",
+            "form": "plain per-scan, blocking drop-in calls from a native loop "
+                    "(cublox_seq_replay), inputs HBM-resident",
         },
         "p50_ms": round(pct(per, 50), 5),
+        "python_loop_ms_per_scan": round(wall_p / args.steps, 5),
         "recenter_critical_path": rc_cost,
         "p99_ms": round(pct(per, 99), 5),
         "scans_per_s": round(1e3 / ms_per_step, 2),

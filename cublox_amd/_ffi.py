@@ -2,6 +2,7 @@
 
 libcublox_hip.so      HIP kernels + C ABI (include/cublox/capi.h)
 libcublox.so          drop-in C++ OccupancyGrid + flat C binding (og_capi.cpp)
+libcublox_seq.so      native replay of a driver's scan loop (seq_replay.h)
 libcublox_scangen.so  synthetic LiDAR workload generator
 
 Nothing here falls back to a CPU path: a missing library raises.
@@ -294,6 +295,30 @@ def host() -> C.CDLL:
     _sig(L, "cublox_utils_pos_to_hash", i, f32p, i32p, i32p, F, i)
     _sig(L, "cublox_logit", F, F)
     _host = L
+    return L
+
+
+class SeqStep(C.Structure):
+    """cublox_seq_step (include/cublox/seq_replay.h)."""
+    _fields_ = [
+        ("grid", vp), ("xs", vp), ("ys", vp), ("zs", vp), ("n", C.c_int),
+        ("T", C.c_float * 12), ("base", C.c_float * 3),
+    ]
+
+
+_seq = None
+
+
+def seq() -> C.CDLL:
+    """The native scan-loop replay."""
+    global _seq
+    if _seq is not None:
+        return _seq
+    host()
+    L = _load("libcublox_seq.so")
+    _sig(L, "cublox_seq_replay", C.c_int, C.POINTER(SeqStep), C.c_int, C.c_int,
+         C.POINTER(C.c_double), i32p)
+    _seq = L
     return L
 
 

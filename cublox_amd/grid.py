@@ -436,3 +436,47 @@ class OccupancyGrid:
     @property
     def stream(self):
         return self._L.cublox_og_stream(self._h)
+
+
+class Replay:
+    """A prepared sequence of driver steps for cublox_seq_replay
+    (include/cublox/seq_replay.h): per step the blocking update of `grid`
+    with n sensor-frame points (pose T) and the recenter at `base`, run in
+    native code. Host columns (on_host) must stay alive while the replay
+    runs; this object keeps the arrays it was given."""
+
+    def __init__(self, steps, on_host: bool = False):
+        self._keep = []
+        arr = (_ffi.SeqStep * max(len(steps), 1))()
+        for k, (grid, xs, ys, zs, n, T, base) in enumerate(steps):
+            st = arr[k]
+            st.grid = grid._h
+            if on_host:
+                cols = [np.ascontiguousarray(np.asarray(c, np.float32)) for c in (xs, ys, zs)]
+                self._keep.append(cols)
+                st.xs, st.ys, st.zs = (c.ctypes.data for c in cols)
+            else:
+                st.xs, st.ys, st.zs = int(xs), int(ys), int(zs)
+            st.n = int(n)
+            st.T[:] = [float(v) for v in np.asarray(T, np.float32).reshape(12)]
+            st.base[:] = [float(v) for v in _f3(base)]
+        self._arr = arr
+        self._count = len(steps)
+        self._on_host = 1 if on_host else 0
+        self.step_ms = np.zeros(max(self._count, 1), np.float64)
+        self.moved = np.zeros(max(self._count, 1), np.int32)
+
+    def run(self):
+        """Runs the steps; returns (per-step wall ms, per-step moved flags)."""
+        st = _ffi.seq().cublox_seq_replay(
+            self._arr, self._count, self._on_host,
+            self.step_ms.ctypes.data_as(C.POINTER(C.c_double)), _ffi.iptr(self.moved))
+        _ffi.check(st, _ffi.host())
+        return list(self.step_ms[:self._count]), [bool(m) for m in self.moved[:self._count]]
+
+
+def replay(steps, on_host: bool = False):
+    """Runs driver steps (grid, xs, ys, zs, n, T, base) natively; returns
+    (per-step wall ms, moved flags). Prepare a Replay once to keep the
+    argument packing out of a timed region."""
+    return Replay(steps, on_host).run()

@@ -46,6 +46,21 @@ def test_host_library_exports(cb):
     assert sorted(set(defined) - set(names)) == []
 
 
+def test_replay_library_exports(cb):
+    """seq_replay.h's entry point is exported by libcublox_seq.so, and a
+    negative count or missing steps is refused before any call is made."""
+    import ctypes as C
+    from cublox_amd import _ffi
+    L = _ffi.seq()
+    names = _declared("seq_replay.h")
+    assert names == ["cublox_seq_replay"]
+    assert hasattr(L, "cublox_seq_replay")
+    assert L.cublox_seq_replay(None, -1, 0, None, None) == 1
+    assert L.cublox_seq_replay(None, 3, 0, None, None) == 1
+    assert L.cublox_seq_replay(None, 0, 0, None, None) == 0
+    assert C.sizeof(_ffi.SeqStep) == 96  # pointer fields, n, T[12], base[3]
+
+
 def test_server_library_exports(cb):
     """ms_capi.h (the map server binding) and ms_capi.cpp agree, and every
     declared entry point is exported."""
