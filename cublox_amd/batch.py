@@ -15,6 +15,14 @@ import numpy as np
 from . import _ffi
 
 
+class PreparedTick:
+    """A tick's packed arguments (GridBatch.prepare)."""
+
+    def __init__(self, args: tuple, keep: tuple):
+        self.args = args
+        self._keep = keep
+
+
 class GridBatch:
     def __init__(self, *, max_grids: int, max_points: int, max_tiles: int,
                  max_segments_per_ray: int, max_near_steps: int = 12,
@@ -88,7 +96,26 @@ class GridBatch:
     def pipelined(self) -> bool:
         return getattr(self, "_pipelined", False)
 
+    def prepare(self, scans: Sequence[Tuple], recenters: Iterable[Tuple] = ()) -> "PreparedTick":
+        """The arguments of a tick packed once, for tick_prepared /
+        submit_prepared (a caller that replays known ticks keeps the packing
+        out of its loop). The scans' device buffers must outlive its use."""
+        return PreparedTick(*self._pack(scans, recenters))
+
+    def tick_prepared(self, p: "PreparedTick") -> None:
+        """tick() with arguments packed by prepare()."""
+        self._check(self._L.cublox_gb_tick(self._h, *p.args))
+
+    def submit_prepared(self, p: "PreparedTick") -> None:
+        """submit() with arguments packed by prepare()."""
+        self._check(self._L.cublox_gb_submit(self._h, *p.args))
+
     def _tick(self, fn, scans: Sequence[Tuple], recenters: Iterable[Tuple]) -> None:
+        args, _keep = self._pack(scans, recenters)
+        self._check(fn(self._h, *args))
+
+    @staticmethod
+    def _pack(scans: Sequence[Tuple], recenters: Iterable[Tuple]) -> tuple:
         scans = list(scans)
         recenters = list(recenters)
         k = len(scans)
@@ -110,9 +137,10 @@ class GridBatch:
         rp = np.ascontiguousarray(
             np.array([np.asarray(r[1], np.float32).reshape(3) for r in recenters]
                      or [np.zeros(3, np.float32)], np.float32))
-        self._check(fn(
-            self._h, k, grids, xs, ys, zs, _ffi.iptr(n), _ffi.fptr(T),
-            _ffi.iptr(stride), _ffi.fptr(drop), len(recenters), rg, _ffi.fptr(rp)))
+        args = (k, grids, xs, ys, zs, _ffi.iptr(n), _ffi.fptr(T), _ffi.iptr(stride),
+                _ffi.fptr(drop), len(recenters), rg, _ffi.fptr(rp))
+        # The arrays behind the pointers, kept alive beside the arguments.
+        return args, (n, T, stride, drop, rp)
 
     def set_graph_mode(self, on: bool) -> None:
         self._check(self._L.cublox_gb_set_graph_mode(self._h, 1 if on else 0))

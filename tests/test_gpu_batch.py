@@ -42,11 +42,12 @@ def _setup(cb, orc):
 
 
 def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=None,
-         setting=None, pipelined=False, check_every=1, max_slabs=None):
+         setting=None, pipelined=False, check_every=1, max_slabs=None, prepared=False):
     """Ticks every grid with its own scan and recenter; checks every grid
     against its oracle after every check_every-th tick (and the last).
     Pipelined: ticks are submitted without waits between the checks, so
-    consecutive ticks overlap."""
+    consecutive ticks overlap. Prepared: the tick's arguments are packed by
+    GridBatch.prepare and passed to tick_prepared / submit_prepared."""
     grids, oracles, scenes = _setup(cb, orc)
     total = max_points or sum(b * a for (_, _, _, _, _, b, a, _, _) in GRIDS)
     kw = {} if max_tiles is None else {"max_tiles": max_tiles}
@@ -76,11 +77,18 @@ def _run(cb, orc, ticks, graph=False, tuning=None, max_tiles=None, max_points=No
             oracles[i].recenter(s.base)
         print(f"[batch] tick {k} graph={graph} pipelined={pipelined} "
               f"setting={batch.setting}", flush=True)
+        p = batch.prepare(scans, recenters) if prepared else None
         if not pipelined:
-            batch.tick(scans, recenters)
+            if p is not None:
+                batch.tick_prepared(p)
+            else:
+                batch.tick(scans, recenters)
         else:
             live.append(keep)
-            batch.submit(scans, recenters)
+            if p is not None:
+                batch.submit_prepared(p)
+            else:
+                batch.submit(scans, recenters)
             if (k + 1) % check_every and k + 1 < ticks:
                 continue
             batch.wait()
@@ -98,6 +106,11 @@ def test_batch_matches_oracle_per_grid(cb, orc, gpu_ready):
 
 def test_batch_graph_replay_matches(cb, orc, gpu_ready):
     _run(cb, orc, 6, graph=True)
+
+
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_batch_prepared_ticks_match_oracle(cb, orc, gpu_ready, pipelined):
+    _run(cb, orc, 5, graph=True, pipelined=pipelined, check_every=2, prepared=True)
 
 
 def test_batch_multiblock_tile_scan(cb, orc, gpu_ready):
