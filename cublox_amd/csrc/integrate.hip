@@ -20,10 +20,11 @@
 //               in an LDS hash: one global atomic per (workgroup, tile).
 //   bin_count   (after the one-lane walk of large launches) the same
 //               counts, from the stored segments.
-//   tile_scan   tile run starts and the work-item plan — one item per
+//   tile scan   tile run starts and the work-item plan — one item per
 //               tile, or one per chunk of a hot tile — in decreasing cost
-//               classes; one workgroup for up to 12288 tiles, three
-//               multi-workgroup passes beyond.
+//               classes: per-block sums, then per-block cursors and items
+//               (blocks of 256 tiles for up to 12288 tiles, each summing
+//               the sums before it; beyond, a bases pass between).
 //   place       each record's slot index written in tile order (its tile's
 //               run start + its counting group's base there + its rank).
 //   tile_apply  one workgroup per work item: replay every segment's voxels
@@ -188,7 +189,6 @@ struct Workspace {
   unsigned *seg_rank; // n * max_seg: each record's place in its tile (rank_code)
   int *grp_base;      // per counting group and hash entry: the group's first
                       // place in that entry's tile
-  int *scan_state;    // single-workgroup tile scan: each thread's bases
   long long seg_capacity;
   int max_items;
   int slabs_cap;      // scratch slabs
@@ -1064,20 +1064,26 @@ __global__ void __launch_bounds__(kBinThreads)
 }
 
 // ----------------------------------------------------------------------
-// tile_scan: cursors, work items, hot-tile splits
+// tile scan: cursors, work items, hot-tile splits
 // ----------------------------------------------------------------------
 // A tile with more than `chunk` records is split into chunks of about
 // `chunk` records, each its own work item with its own scratch slab, while
 // the split tiles' slabs (taken in tile order) fit the scratch area; a hot
 // tile beyond that is one item, accumulated in 16-bit passes. Items are
 // placed in decreasing cost classes (records per item) so the heaviest
-// start first.
+// start first. The scan runs over blocks of tiles: per-block sums (pass 1),
+// per-block bases (pass 2), then each block's cursors and items (pass 3).
 constexpr int kScanThreads = 1024;
 constexpr int kScanWaves = kScanThreads / 64;
-constexpr int kScanStage = 12288;  // tile counts staged in LDS (48 KB)
-constexpr int kScanTilesPerBlock = 4096;  // multi-block scan: tiles per block
+constexpr int kScanTilesPerBlock = 4096;  // tiles per scan block (large workspaces)
+// Workspaces of at most kScanSmall tiles scan in blocks of kScanSmallTiles
+// tiles (one thread each), and each pass-3 block sums the block sums before
+// it itself (no pass-2 launch): measured at 9,680 tiles, 4.8 + 7.1 us
+// against 13.6 us for one 1,024-thread workgroup over all tiles, whose
+// instruction issue on one CU bounded it.
+constexpr int kScanSmall = 12288;
+constexpr int kScanSmallTiles = 256;
 constexpr int kScanSums = 3 + kCostBuckets;  // records, slabs, hot tiles, classes
-constexpr int kScanState = 8;  // bases a tile_scan thread leaves: run, slab, split, classes
 constexpr int kPass = 65535;  // records per LDS pass (16-bit counts)
 constexpr int kBigBlocks = 256;  // workgroups over unsplit hot tiles
 
@@ -1092,29 +1098,6 @@ __device__ __forceinline__ int cost_bucket(int records) {
 // Number of chunks a tile with c records is split into (1: not split).
 __device__ __forceinline__ int chunks_of(int c, int chunk) {
   return c > chunk ? (c + chunk - 1) / chunk : 1;
-}
-
-// Split decision of a tile with c records whose slab range would start at
-// slab sl: its chunks, whether it splits (its slabs fit the cap), the item
-// slots it takes, its cost class, and the slabs it consumes in the slab
-// prefix (every hot tile does, split or not). Only hot tiles divide.
-struct TileSplit {
-  int ch, slots, cls, need;
-  bool split;
-};
-__device__ __forceinline__ TileSplit tile_split(int c, int chunk, int sl, int cap,
-                                                bool shared) {
-  TileSplit r{1, 1, cost_bucket(c), 0, false};
-  if (c > chunk) {
-    r.ch = (c + chunk - 1) / chunk;
-    r.need = slab_need(r.ch, shared);
-    r.split = sl + r.need <= cap;
-    if (r.split) {
-      r.slots = r.ch;
-      r.cls = cost_bucket((c + r.ch - 1) / r.ch);
-    }
-  }
-  return r;
 }
 
 // Inclusive scans over the block's threads of N ints per thread at once
@@ -1186,207 +1169,26 @@ __device__ __forceinline__ int take_slots(int (&cur)[kCostBuckets], int q, int s
   return pos;
 }
 
-// Single-workgroup scan for up to kScanStage tiles. Each thread owns a run
-// of consecutive tiles; one block scan gives the record and slab prefixes,
-// a second one (after the split decisions, which need the slab prefix)
-// each thread's first item slot in every cost class and its first split
-// index. The kernel writes the tiles' run starts and each thread's bases;
-// the work items are written from those by the placing pass (place_items),
-// spread over several workgroups: one workgroup's VALU would take longer
-// than the rest of this scan.
+// Multi-workgroup scan, pass 1: per block of tpb tiles, the records, the
+// slabs its hot tiles would take, its hot tiles, and the items per cost
+// class as if every hot tile were split.
 __global__ void __launch_bounds__(kScanThreads)
-    tile_scan_kernel(const Batch b, Workspace w) {
-  __shared__ int s_wave[(1 + kCostBuckets) * kScanWaves];
-  __shared__ int s_count[kScanStage];
-  const int n_tiles = batch_tiles(b);
-  const int chunk = batch_chunk(b, w);
-  const bool shared = batch_shared(b);
-  const int t = threadIdx.x;
-  if (t == 0) w.misc[4] = 0;
-  {
-    // All of a thread's counts in flight at once (one memory round trip),
-    // then staged in LDS and cleared for the next launch.
-    constexpr int kPer = kScanStage / kScanThreads;
-    int v[kPer];
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int k = t + i * kScanThreads;
-      v[i] = k < n_tiles ? w.tile_count[k] : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int k = t + i * kScanThreads;
-      if (k < n_tiles) {
-        s_count[k] = v[i];
-        w.tile_count[k] = 0;
-      }
-    }
-  }
-  __syncthreads();
-  const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
-  const int beg = min(t * per, n_tiles);
-  const int end = min(beg + per, n_tiles);
-  int a[2] = {0, 0}, ta[2];  // records, slabs the hot tiles would take
-  for (int k = beg; k < end; ++k) {
-    const int c = s_count[k];
-    a[0] += c;
-    if (c > chunk) a[1] += slab_need(chunks_of(c, chunk), shared);
-  }
-  const int seg = a[0], slabs = a[1];
-  block_scan_incl_n<2>(s_wave, a, ta);
-  // Split decisions (slab prefix over all hot tiles in tile order): item
-  // slots per cost class, and split tiles.
-  int n[1 + kCostBuckets] = {}, tn[1 + kCostBuckets];
-  int sl = a[1] - slabs;
-  for (int k = beg; k < end; ++k) {
-    const int c = s_count[k];
-    if (c <= 0) continue;
-    const TileSplit r = tile_split(c, chunk, sl, w.slabs_cap, shared);
-#pragma unroll
-    for (int u = 0; u < kCostBuckets; ++u) n[u] += r.cls == u ? r.slots : 0;
-    n[kCostBuckets] += r.split ? 1 : 0;
-    sl += r.need;
-  }
-  int own[1 + kCostBuckets];
-#pragma unroll
-  for (int u = 0; u <= kCostBuckets; ++u) own[u] = n[u];
-  block_scan_incl_n<1 + kCostBuckets>(s_wave, n, tn);
-  int cur[kCostBuckets];
-  int start = 0;
-#pragma unroll
-  for (int u = 0; u < kCostBuckets; ++u) {
-    cur[u] = start + n[u] - own[u];
-    start += tn[u];
-  }
-  if (t == 0) {
-    w.misc[0] = min(start, w.max_items);
-    w.misc[1] = ta[0];
-    w.misc[3] = tn[kCostBuckets];
-  }
-  long long run = a[0] - seg;
-  int *st = w.scan_state + t * kScanState;
-  st[0] = static_cast<int>(run);
-  st[1] = a[1] - slabs;
-  st[2] = n[kCostBuckets] - own[kCostBuckets];
-#pragma unroll
-  for (int u = 0; u < kCostBuckets; ++u) st[3 + u] = cur[u];
-  for (int k = beg; k < end; ++k) {
-    w.tile_cursor[k] = static_cast<int>(run);
-    run += s_count[k];
-  }
-}
-
-// The work items of tiles k0 .. k0 + blockDim - 1 (one thread per tile),
-// from the bases tile_scan saved for the thread that owned each (a run of
-// `per` consecutive tiles): the run's earlier tiles' split tiles, slabs and
-// item slots per class are recounted from the tiles' run starts (a tile's
-// record count is the difference of consecutive run starts). A split
-// tile's chunk items are written by the whole workgroup. Every thread of
-// the workgroup calls this.
-struct SplitEntry {
-  int k, run, c, per_chunk, pos0, sl, ch, pad;
-};
-__device__ __forceinline__ void place_items(const Batch &b, const Workspace &w, int k0,
-                                            SplitEntry *s_split, int *s_ch, int *s_wave,
-                                            int *s_nsplit) {
-  const int k = k0 + static_cast<int>(threadIdx.x);
-  const int n_tiles = batch_tiles(b);
-  const int chunk = batch_chunk(b, w);
-  const bool shared = batch_shared(b);
-  if (threadIdx.x == 0) *s_nsplit = 0;
-  __syncthreads();
-  if (k < n_tiles) {
-    const int per = (n_tiles + kScanThreads - 1) / kScanThreads;
-    const int t = k / per;
-    const int beg = t * per;
-    const int *st = w.scan_state + t * kScanState;
-    int sl = st[1], sp = st[2];
-    int cur[kCostBuckets];
-#pragma unroll
-    for (int u = 0; u < kCostBuckets; ++u) cur[u] = st[3 + u];
-    // Run starts of tiles beg .. k + 1, loaded up front (at most
-    // kScanStage / kScanThreads + 1 of them).
-    constexpr int kMaxPer = kScanStage / kScanThreads;
-    const int total = w.misc[1];
-    int cv[kMaxPer + 1];
-#pragma unroll
-    for (int q = 0; q <= kMaxPer; ++q) {
-      const int j = beg + q;
-      cv[q] = j <= k + 1 ? (j < n_tiles ? w.tile_cursor[j] : total) : 0;
-    }
-    // The run's tiles before k.
-    const int mine = k - beg;
-#pragma unroll
-    for (int q = 0; q < kMaxPer; ++q) {
-      if (q >= mine) break;
-      const int c = cv[q + 1] - cv[q];
-      if (c <= 0) continue;
-      const TileSplit r = tile_split(c, chunk, sl, w.slabs_cap, shared);
-      take_slots(cur, r.cls, r.slots);
-      sp += r.split ? 1 : 0;
-      sl += r.need;
-    }
-    int c = 0, run = 0;
-#pragma unroll
-    for (int q = 0; q < kMaxPer; ++q) {
-      if (q == mine) {
-        c = cv[q + 1] - cv[q];
-        run = cv[q];
-      }
-    }
-    if (c > 0) {
-      const TileSplit r = tile_split(c, chunk, sl, w.slabs_cap, shared);
-      const int pos0 = take_slots(cur, r.cls, r.slots);
-      if (!r.split) {
-        place_tile(w, k, c, run, sl, sp, chunk, false, 1, pos0, shared);
-      } else {
-        w.hot[sp] = HotTile{k, sl, r.ch, shared ? 1 : 0};
-        const int e = atomicAdd(s_nsplit, 1);
-        s_split[e] = SplitEntry{k, run, c, (c + r.ch - 1) / r.ch, pos0, sl, r.ch, 0};
-      }
-    }
-  }
-  __syncthreads();
-  // Chunk items of the split tiles, one per thread at a time: entry e's
-  // chunks follow the chunks of entries before it (s_ch: inclusive prefix).
-  const int ns = *s_nsplit;
-  if (ns == 0) return;  // uniform
-  int ch = static_cast<int>(threadIdx.x) < ns ? s_split[threadIdx.x].ch : 0;
-  ch = block_scan_incl(s_wave, ch);
-  if (static_cast<int>(threadIdx.x) < ns) s_ch[threadIdx.x] = ch;
-  __syncthreads();
-  const int all = s_ch[ns - 1];
-  for (int g = threadIdx.x; g < all; g += blockDim.x) {
-    int lo = 0, hi = ns - 1;  // first entry whose prefix exceeds g
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (s_ch[mid] > g) hi = mid; else lo = mid + 1;
-    }
-    const SplitEntry se = s_split[lo];
-    const int q = g - (s_ch[lo] - se.ch);
-    const int pos = se.pos0 + q;
-    if (pos >= w.max_items) continue;
-    const int ib = se.run + q * se.per_chunk;
-    const int ie = min(ib + se.per_chunk, se.run + se.c);
-    w.items[pos] = Item{se.k, ib, ie, shared ? se.sl : se.sl + q};
-  }
-}
-
-// Multi-workgroup scan, pass 1: per block of kScanTilesPerBlock tiles, the
-// records, the slabs its hot tiles would take, and the items per cost class
-// as if every hot tile were split.
-__global__ void __launch_bounds__(kScanThreads)
-    tile_sums_kernel(const Batch b, Workspace w) {
+    tile_sums_kernel(const Batch b, Workspace w, int tpb) {
   __shared__ int s_sum[kScanSums];
   const int n_tiles = batch_tiles(b);
   const int chunk = batch_chunk(b, w);
   const bool shared = batch_shared(b);
-  const int k0 = blockIdx.x * kScanTilesPerBlock;
+  const int k0 = blockIdx.x * tpb;
   if (threadIdx.x < kScanSums) s_sum[threadIdx.x] = 0;
+  // The split and unsplit hot-tile counters that pass 3 adds to.
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.misc[3] = 0;
+    w.misc[4] = 0;
+  }
   __syncthreads();
-  int v[kScanSums] = {};  // records, slabs, (split tiles: pass 3), classes
-  for (int k = k0 + threadIdx.x; k < min(k0 + kScanTilesPerBlock, n_tiles);
-       k += kScanThreads) {
+  int v[kScanSums] = {};  // records, slabs, hot tiles, classes
+  for (int k = k0 + threadIdx.x; k < min(k0 + tpb, n_tiles);
+       k += blockDim.x) {
     const int c = w.tile_count[k];
     v[0] += c;
     const int ch = chunks_of(c, chunk);
@@ -1395,7 +1197,10 @@ __global__ void __launch_bounds__(kScanThreads)
 #pragma unroll
       for (int u = 0; u < kCostBuckets; ++u) v[3 + u] += q == u ? ch : 0;
     }
-    if (ch > 1) v[1] += slab_need(ch, shared);
+    if (ch > 1) {
+      v[1] += slab_need(ch, shared);
+      v[2] += 1;
+    }
   }
 #pragma unroll
   for (int q = 0; q < kScanSums; ++q) {
@@ -1412,10 +1217,9 @@ __global__ void __launch_bounds__(kScanThreads)
 // Pass 2 (one workgroup): exclusive per-block bases of records and slabs,
 // and for every cost class the block's first item slot.
 __global__ void __launch_bounds__(kScanThreads)
-    tile_bases_kernel(const Batch b, Workspace w) {
+    tile_bases_kernel(const Batch b, Workspace w, int tpb) {
   __shared__ int s_wave[kScanSums * kScanWaves];
-  const int n_blocks =
-      (batch_tiles(b) + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
+  const int n_blocks = (batch_tiles(b) + tpb - 1) / tpb;
   const int t = threadIdx.x;  // one thread per block (at most kScanThreads)
   int v[kScanSums], tot[kScanSums];
 #pragma unroll
@@ -1437,104 +1241,136 @@ __global__ void __launch_bounds__(kScanThreads)
   if (t == 0) {
     w.misc[0] = min(start, w.max_items);
     w.misc[1] = tot[0];
-    w.misc[3] = 0;  // split and unsplit hot tiles, counted by pass 3
-    w.misc[4] = 0;
   }
+}
+
+// Pass 2 done by a pass-3 block itself (at most 64 blocks): its bases (as
+// tile_bases_kernel gives them) from every block's sums, summed by the first
+// wave, into s_base; block 0 also writes the item and record totals. Every
+// thread of the block calls this.
+static_assert((kScanSmall + kScanSmallTiles - 1) / kScanSmallTiles <= 64,
+              "one wave sums the block sums of a small scan");
+__device__ __forceinline__ void own_block_bases(const Batch &b, const Workspace &w, int tpb,
+                                                int *s_base) {
+  const int n_blocks = (batch_tiles(b) + tpb - 1) / tpb;
+  const int t = threadIdx.x;
+  const int me = static_cast<int>(blockIdx.x);
+  if (t < 64) {
+    int pre[kScanSums], tot[kScanSums];
+#pragma unroll
+    for (int q = 0; q < kScanSums; ++q) {
+      const int x = t < n_blocks ? w.block_sums[t * kScanSums + q] : 0;
+      pre[q] = t < me ? x : 0;
+      tot[q] = x;
+    }
+#pragma unroll
+    for (int q = 0; q < kScanSums; ++q) {
+      for (int off = 32; off > 0; off >>= 1) {
+        pre[q] += __shfl_xor(pre[q], off, 64);
+        tot[q] += __shfl_xor(tot[q], off, 64);
+      }
+    }
+    if (t == 0) {
+    int start = 0;
+#pragma unroll
+    for (int q = 0; q < kCostBuckets; ++q) {
+      s_base[3 + q] = start + pre[3 + q];  // first slot of class q in this block
+      start += tot[3 + q];
+    }
+    s_base[0] = pre[0];
+    s_base[1] = pre[1];
+    s_base[2] = pre[2];
+    if (me == 0) {
+      w.misc[0] = min(start, w.max_items);
+      w.misc[1] = tot[0];
+    }
+    }
+  }
+  __syncthreads();
 }
 
 // Pass 3: per block, cursors, items and split tiles from the block bases.
 // A hot tile whose slab range does not fit keeps one item (in its class
-// slot) and turns its other slots into empty items.
+// slot) and turns its other slots into empty items. The slab prefix only
+// grows, so the tiles that fit are the first hot tiles in tile order: a
+// split tile's index in the hot list is its hot-tile prefix, and the list's
+// length the largest such index plus one.
 __global__ void __launch_bounds__(kScanThreads)
-    tile_place_kernel(const Batch b, Workspace w) {
-  __shared__ int s_wave[(1 + kCostBuckets) * kScanWaves];
-  __shared__ int s_hot;
+    tile_place_kernel(const Batch b, Workspace w, int tpb, int own_bases) {
+  __shared__ int s_wave[kScanSums * kScanWaves];
+  __shared__ int s_base[kScanSums];
   const int n_tiles = batch_tiles(b);
   const int chunk = batch_chunk(b, w);
   const bool shared = batch_shared(b);
-  const int k0 = blockIdx.x * kScanTilesPerBlock;
-  const int k1 = min(k0 + kScanTilesPerBlock, n_tiles);
+  const int k0 = blockIdx.x * tpb;
+  const int k1 = min(k0 + tpb, n_tiles);
   const int t = threadIdx.x;
-  const int *base = w.block_bases + blockIdx.x * kScanSums;
-  constexpr int kPer = kScanTilesPerBlock / kScanThreads;
+  const int kPer = tpb / static_cast<int>(blockDim.x);
   const int beg = min(k0 + t * kPer, k1);
   const int end = min(beg + kPer, k1);
-  int a[2] = {0, 0}, ta[2];  // records, slabs the hot tiles would take
-  for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
-    a[0] += c;
-    if (c > chunk) a[1] += slab_need(chunks_of(c, chunk), shared);
+  // The thread's first count, in flight while the bases are summed.
+  const int c_first = beg < end ? w.tile_count[beg] : 0;
+  const int *base = w.block_bases + blockIdx.x * kScanSums;
+  if (own_bases) {  // uniform
+    own_block_bases(b, w, tpb, s_base);
+    base = s_base;
   }
-  const int seg = a[0], slabs = a[1];
-  block_scan_incl_n<2>(s_wave, a, ta);
-  // Split tiles under the slab prefix, and item slots per class (ch each).
-  int n[1 + kCostBuckets] = {}, tn[1 + kCostBuckets];
-  int sl = base[1] + a[1] - slabs;
+  // Records, slabs, hot tiles and item slots per class (ch each) of the
+  // thread's tiles, scanned over the block at once.
+  int v[kScanSums] = {}, tv[kScanSums];
   for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
+    const int c = k == beg ? c_first : w.tile_count[k];
     const int ch = chunks_of(c, chunk);
+    v[0] += c;
     if (c > 0) {
       const int q = cost_bucket((c + ch - 1) / ch);
 #pragma unroll
-      for (int u = 0; u < kCostBuckets; ++u) n[1 + u] += q == u ? ch : 0;
+      for (int u = 0; u < kCostBuckets; ++u) v[3 + u] += q == u ? ch : 0;
     }
     if (ch > 1) {
-      if (sl + slab_need(ch, shared) <= w.slabs_cap) ++n[0];
-      sl += slab_need(ch, shared);
+      v[1] += slab_need(ch, shared);
+      v[2] += 1;
     }
   }
-  int own[1 + kCostBuckets];
+  int own[kScanSums];
 #pragma unroll
-  for (int u = 0; u <= kCostBuckets; ++u) own[u] = n[u];
-  block_scan_incl_n<1 + kCostBuckets>(s_wave, n, tn);
-  if (t == 0) s_hot = atomicAdd(w.misc + 3, tn[0]);
-  __syncthreads();
+  for (int q = 0; q < kScanSums; ++q) own[q] = v[q];
+  block_scan_incl_n<kScanSums>(s_wave, v, tv);
   int cur[kCostBuckets];
 #pragma unroll
-  for (int u = 0; u < kCostBuckets; ++u) cur[u] = base[3 + u] + n[1 + u] - own[1 + u];
-  long long run = static_cast<long long>(base[0]) + a[0] - seg;
-  sl = base[1] + a[1] - slabs;
-  int sp = s_hot + n[0] - own[0];
+  for (int u = 0; u < kCostBuckets; ++u) cur[u] = base[3 + u] + v[3 + u] - own[3 + u];
+  long long run = static_cast<long long>(base[0]) + v[0] - own[0];
+  int sl = base[1] + v[1] - own[1];
+  int sp = base[2] + v[2] - own[2];
   for (int k = beg; k < end; ++k) {
-    const int c = w.tile_count[k];
+    const int c = k == beg ? c_first : w.tile_count[k];
     const int ch = chunks_of(c, chunk);
     const bool split = ch > 1 && sl + slab_need(ch, shared) <= w.slabs_cap;
     const int pos0 = c > 0 ? take_slots(cur, cost_bucket((c + ch - 1) / ch), ch) : 0;
     w.tile_cursor[k] = static_cast<int>(run);
     place_tile(w, k, c, run, sl, sp, chunk, split, ch, pos0, shared);
-    if (split) ++sp;
-    if (ch > 1) sl += slab_need(ch, shared);
+    if (split) atomicMax(w.misc + 3, sp + 1);
+    if (ch > 1) {
+      sl += slab_need(ch, shared);
+      ++sp;
+    }
     run += c;
     w.tile_count[k] = 0;
   }
 }
 
-// Placing pass (after tile_scan has set each tile's run start in
+// Placing pass (after the tile scan has set each tile's run start in
 // tile_cursor): record j of every ray of this block's kWalkBlock rays goes
 // to its place in tile order, kPlaceJ values of j at a time: the block of y
 // index y takes j = kPlaceJ * y + kPlaceJ * gridDim.y * m (m = 0, 1, ...)
-// while the ray has records there. The record's counting group is its ray's group of 2^lg_group
-// rays, whose hash has `hash` entries. Every record is independent, so the
-// pass is a gather and a scatter with no atomics. After the single-workgroup
-// tile scan, the first item_blocks blocks of y index 0 write the work items
-// (place_items, one thread per tile) beside it; they come first in dispatch
-// order, so their loads are not queued behind the records' traffic.
+// while the ray has records there. The record's counting group is its ray's
+// group of 2^lg_group rays, whose hash has `hash` entries. Every record is
+// independent, so the pass is a gather and a scatter with no atomics.
 constexpr int kPlaceJ = 4;
 constexpr int kPlaceRows = 4;
 __global__ void __launch_bounds__(kWalkBlock)
-    place_kernel(const Batch b, Workspace w, int lg_group, int hash, int item_blocks) {
-  if (static_cast<int>(blockIdx.x) < item_blocks) {
-    if (blockIdx.y == 0) {
-      __shared__ SplitEntry s_split[kWalkBlock];
-      __shared__ int s_ch[kWalkBlock];
-      __shared__ int s_wave[kWalkBlock / 64];
-      __shared__ int s_nsplit;
-      place_items(b, w, static_cast<int>(blockIdx.x) * kWalkBlock, s_split, s_ch, s_wave,
-                  &s_nsplit);
-    }
-    return;
-  }
-  const int bx = static_cast<int>(blockIdx.x) - item_blocks;
+    place_kernel(const Batch b, Workspace w, int lg_group, int hash) {
+  const int bx = static_cast<int>(blockIdx.x);
   const int ray = bx * kWalkBlock + static_cast<int>(threadIdx.x);
   const int ray0 = bx * kWalkBlock;
   const int n_rays = batch_rays(b);
@@ -2116,9 +1952,10 @@ struct Layout {
   long long slabs;
   bool may_run_out;  // slabs below the chunk bound: hot tiles may stay unsplit
   int max_items, scan_blocks;
+  int scan_tpb;  // tiles per multi-block scan block
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
-      off_bbase, off_srank, off_gbase, off_sstate, off_rtd, off_hdr, off_desc, total;
+      off_bbase, off_srank, off_gbase, off_rtd, off_hdr, off_desc, total;
 };
 
 int bin_rays_per_block(const Caps &c);
@@ -2150,7 +1987,8 @@ Layout make_layout(const Caps &c) {
   L.slabs = std::max<long long>(2, std::min<long long>(chunks, c.max_slabs));
   L.may_run_out = chunks > c.max_slabs;
   L.max_items = static_cast<int>(std::min<long long>(c.max_tiles + chunks, 1LL << 30));
-  L.scan_blocks = (c.max_tiles + kScanTilesPerBlock - 1) / kScanTilesPerBlock;
+  L.scan_tpb = c.max_tiles <= kScanSmall ? kScanSmallTiles : kScanTilesPerBlock;
+  L.scan_blocks = (c.max_tiles + L.scan_tpb - 1) / L.scan_tpb;
   size_t o = 0;
   auto take = [&](size_t &off, size_t bytes) {
     off = o;
@@ -2174,7 +2012,6 @@ Layout make_layout(const Caps &c) {
   take(L.off_bbase, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
   take(L.off_srank, static_cast<size_t>(c.seg_capacity) * 4);
   take(L.off_gbase, static_cast<size_t>(group_table_entries(c)) * 4);
-  take(L.off_sstate, static_cast<size_t>(kScanThreads) * kScanState * 4);
   // Header and descriptors, laid out as the host staging buffer so one
   // copy moves both.
   take(L.off_hdr, staging_bytes(c.max_grids));
@@ -2204,7 +2041,6 @@ Workspace bind(const Layout &L, void *base) {
   w.block_bases = reinterpret_cast<int *>(b + L.off_bbase);
   w.seg_rank = reinterpret_cast<unsigned *>(b + L.off_srank);
   w.grp_base = reinterpret_cast<int *>(b + L.off_gbase);
-  w.scan_state = reinterpret_cast<int *>(b + L.off_sstate);
   w.seg_capacity = L.cap.seg_capacity;
   w.max_items = L.max_items;
   w.slabs_cap = static_cast<int>(L.slabs);
@@ -2414,35 +2250,34 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     st = check_launch("bin_kernel");
     if (st != CUBLOX_OK) return st;
   }
-  if (c.max_tiles <= kScanStage) {
-    tile_scan_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(b, w);
-    st = check_launch("tile_scan_kernel");
-  } else {
-    tile_sums_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w);
+  if (L.scan_tpb == kScanSmallTiles) {
+    // Few blocks: each placing block finds its own bases.
+    tile_sums_kernel<<<dim3(L.scan_blocks), dim3(kScanSmallTiles), 0, hs>>>(b, w, L.scan_tpb);
     st = check_launch("tile_sums_kernel");
     if (st != CUBLOX_OK) return st;
-    tile_bases_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(b, w);
+    tile_place_kernel<<<dim3(L.scan_blocks), dim3(kScanSmallTiles), 0, hs>>>(b, w, L.scan_tpb,
+                                                                             1);
+    st = check_launch("tile_place_kernel");
+  } else {
+    tile_sums_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w, L.scan_tpb);
+    st = check_launch("tile_sums_kernel");
+    if (st != CUBLOX_OK) return st;
+    tile_bases_kernel<<<dim3(1), dim3(kScanThreads), 0, hs>>>(b, w, L.scan_tpb);
     st = check_launch("tile_bases_kernel");
     if (st != CUBLOX_OK) return st;
-    tile_place_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w);
+    tile_place_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w, L.scan_tpb, 0);
     st = check_launch("tile_place_kernel");
   }
   if (st != CUBLOX_OK) return st;
   {
     // Counting groups: the quad walk's workgroups, else the bin blocks.
     const int lg_group = 31 - __builtin_clz(static_cast<unsigned>(quad ? kWalkBlock : bin_rays));
-    // After the single-workgroup scan, more workgroups write the work
-    // items, one thread per tile.
-    const bool single = c.max_tiles <= kScanStage;
-    const unsigned item_blocks =
-        single ? static_cast<unsigned>((c.max_tiles + kWalkBlock - 1) / kWalkBlock) : 0u;
     // At most kPlaceRows block rows (j chunks of kPlaceJ; a row loops over
     // its further chunks): rows past a ray's records are empty workgroups,
     // whose dispatch costs time on large launches.
-    const dim3 grid(walk_blocks + item_blocks,
+    const dim3 grid(walk_blocks,
                     static_cast<unsigned>(std::min((c.max_seg + kPlaceJ - 1) / kPlaceJ, kPlaceRows)));
-    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash,
-                                                     static_cast<int>(item_blocks));
+    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash);
     st = check_launch("place_kernel");
   }
   if (st != CUBLOX_OK) return st;
