@@ -1301,8 +1301,14 @@ void OccupancyGrid::fetchOccupancyBatch(
                                sizeof(unsigned int) * nq));
     fetch_cfg_capacity_ = nq;
   }
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_fetch_cfgs_, cfgs.data(),
-                                       sizeof(cublox_fetch_cfg) * nq, stream_));
+  // Transfers go through the grid's pinned stage (pageable copies run at a
+  // few GB/s): the configurations and the counts first, then (after the
+  // sync below) the offsets and the samples.
+  const size_t cfg_bytes = sizeof(cublox_fetch_cfg) * static_cast<size_t>(nq);
+  const size_t cnt_off = (cfg_bytes + 255) & ~static_cast<size_t>(255);
+  uint8_t *stage = pinnedStage(cnt_off + sizeof(unsigned int) * static_cast<size_t>(nq));
+  std::memcpy(stage, cfgs.data(), cfg_bytes);
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_fetch_cfgs_, stage, cfg_bytes, stream_));
   CUBLOX_CHECK(cublox_memset_async(d_fetch_counts_, 0, sizeof(unsigned int) * nq,
                                    stream_));
   mark(6);
@@ -1311,9 +1317,10 @@ void OccupancyGrid::fetchOccupancyBatch(
       d_occ_, d_fetch_counts_, d_fetch_pos_, d_fetch_state_,
       static_cast<unsigned int>(per_query_cap), stream_));
   mark(7);
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(counts.data(), d_fetch_counts_,
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(stage + cnt_off, d_fetch_counts_,
                                        sizeof(unsigned int) * nq, stream_));
   sync();
+  std::memcpy(counts.data(), stage + cnt_off, sizeof(unsigned int) * nq);
   if (timing_) {
     float ms = 0.0f;
     CUBLOX_CHECK(cublox_event_elapsed_ms(events_[6], events_[7], &ms));
@@ -1355,19 +1362,25 @@ void OccupancyGrid::fetchOccupancyBatch(
     fetch_dense_capacity_ = static_cast<long long>(total);
     fetch_offsets_capacity_ = nq + 1;
   }
-  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_fetch_offsets_, offsets.data(),
-                                       sizeof(unsigned int) * (nq + 1), stream_));
+  const size_t off_bytes = sizeof(unsigned int) * (static_cast<size_t>(nq) + 1);
+  const size_t pos_off = (off_bytes + 255) & ~static_cast<size_t>(255);
+  const size_t st_off = (pos_off + total * 3 * sizeof(float) + 255) & ~static_cast<size_t>(255);
+  stage = pinnedStage(st_off + total);  // nothing in flight reads the stage now
+  std::memcpy(stage, offsets.data(), off_bytes);
+  CUBLOX_CHECK(cublox_memcpy_h2d_async(d_fetch_offsets_, stage, off_bytes, stream_));
   CUBLOX_CHECK(cublox_launch_fetch_gather(
       d_fetch_counts_, d_fetch_offsets_, nq,
       static_cast<unsigned int>(per_query_cap), max_count, d_fetch_pos_,
       d_fetch_state_, d_fetch_dense_pos_, d_fetch_dense_state_, stream_));
-  positions.resize(total * 3);
-  states.resize(total);
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(positions.data(), d_fetch_dense_pos_,
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(stage + pos_off, d_fetch_dense_pos_,
                                        total * 3 * sizeof(float), stream_));
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(states.data(), d_fetch_dense_state_, total,
+  CUBLOX_CHECK(cublox_memcpy_d2h_async(stage + st_off, d_fetch_dense_state_, total,
                                        stream_));
   sync();
+  positions.resize(total * 3);
+  states.resize(total);
+  std::memcpy(positions.data(), stage + pos_off, total * 3 * sizeof(float));
+  std::memcpy(states.data(), stage + st_off, total);
 }
 
 float OccupancyGrid::storedAt(const int hash_id) const {

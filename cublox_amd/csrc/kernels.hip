@@ -352,38 +352,6 @@ __device__ __forceinline__ bool fetch_classify(const cublox_fetch_cfg &c,
   return c.state_filter < 0 || static_cast<int>(st) == c.state_filter;
 }
 
-// Enumerates a fetch and numbers the kept samples with one atomic on
-// *count per wave; sink(keep, slot, px, py, pz, state) then runs in every
-// lane of the wave (slot is meaningful where keep is set). Used where many
-// queries share a launch (each query's counter sees few waves).
-template <typename Sink>
-__device__ __forceinline__ void fetch_scan(const cublox_fetch_cfg &c,
-                                           const float *__restrict__ occ,
-                                           unsigned *__restrict__ count,
-                                           const int block, const int nblocks,
-                                           Sink sink) {
-  FetchSpace f;
-  if (!fetch_space(c, f)) return;
-  const unsigned stride = static_cast<unsigned>(nblocks) * blockDim.x;
-  const int lane = __lane_id();
-  const unsigned long long lt_mask = (1ull << lane) - 1ull;
-  // Wave-uniform trip count so that every lane reaches each ballot.
-  const unsigned first = static_cast<unsigned>(block) * blockDim.x + (threadIdx.x & ~63u);
-  for (unsigned w0 = first; w0 < f.total; w0 += stride) {
-    const unsigned e = w0 + lane;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    unsigned char st = 0;
-    const bool keep = e < f.total && fetch_classify(c, occ, f, e, px, py, pz, st);
-    const unsigned long long m = __ballot(keep);
-    if (m == 0ull) continue;
-    const int leader = __ffsll(static_cast<long long>(m)) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomic_add_u32_ret(count, __popcll(m));
-    base = __shfl(base, leader);
-    sink(keep, base + __popcll(m & lt_mask), px, py, pz, st);
-  }
-}
-
 // Workgroup-staged form for one large query: kept samples (x, y, z, state)
 // gather in LDS and each flush reserves its slots on *count with one
 // atomic (a flush every kFetchStage - 256 samples at most, usually one per
@@ -435,24 +403,119 @@ __device__ __forceinline__ void fetch_scan_staged(const cublox_fetch_cfg &c,
   if (n > 0) staged_flush(count, n, L, flush);
 }
 
-__device__ __forceinline__ void fetch_body(const cublox_fetch_cfg &c,
-                                           const float *__restrict__ occ,
-                                           unsigned *__restrict__ count,
-                                           float *__restrict__ out_pos,
-                                           unsigned char *__restrict__ out_state,
-                                           const unsigned cap,
-                                           const unsigned max_results,
-                                           const int block, const int nblocks) {
-  fetch_scan(c, occ, count, block, nblocks,
-             [&](bool keep, unsigned slot, float px, float py, float pz,
-                 unsigned char st) {
-               if (keep && !((max_results > 0u && slot >= max_results) || slot >= cap)) {
-                 out_pos[3ull * slot + 0] = px;
-                 out_pos[3ull * slot + 1] = py;
-                 out_pos[3ull * slot + 2] = pz;
-                 out_state[slot] = st;
-               }
-             });
+// Batched fetch (many queries in a launch, few samples each): one (x, y)
+// column of a query's disk per half-wave. The column's window test,
+// position, squared planar distance and hash base are formed once, with
+// the expressions of fetch_classify; the half-wave's lanes take its z
+// voxels 32 at a time, so occ reads are contiguous and no lane divides. A
+// column whose squared planar distance already exceeds radius_sq holds no
+// kept voxel (adding dz * dz >= 0 cannot round below it) and is skipped.
+// Kept samples collect in the wave's LDS stage; the wave reserves their
+// slots on the query's count with one atomic when the stage fills and once
+// at its end, so no column waits on a global round trip.
+constexpr int kWaveStage = 256;  // staged samples per wave
+
+__device__ __forceinline__ void fetch_columns(const cublox_fetch_cfg &c,
+                                              const float *__restrict__ occ,
+                                              unsigned *__restrict__ count,
+                                              float *__restrict__ out_pos,
+                                              unsigned char *__restrict__ out_state,
+                                              const unsigned cap, const int block,
+                                              const int nblocks, float4 *stage) {
+  // The enumeration's extents (fetch_space without its divisors).
+  FetchSpace f;
+  f.r = c.disk_radius_vox;
+  f.diam = 2 * f.r + 1;
+  f.nz = c.gz_max - c.gz_min + 1;
+  if (f.diam <= 0 || f.nz <= 0) return;  // uniform: the query's configuration
+  f.ctr = c.origin_at_center != 0;
+  auto pmod = [](int v, int m) {
+    const int r = v % m;
+    return r < 0 ? r + m : r;
+  };
+  f.qx = pmod(c.origin_i.x, c.map_size_i.x);
+  f.qy = pmod(c.origin_i.y, c.map_size_i.y);
+  f.qz = pmod(c.origin_i.z, c.map_size_i.z);
+  const int lane = __lane_id();
+  const int zl = lane & 31;
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const int cols = f.diam * f.diam;
+  int n = 0;  // staged samples (wave-uniform)
+  // Writes the staged samples to slots reserved with one atomic.
+  auto flush = [&]() {
+    if (n == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    unsigned base = 0;
+    if (lane == 0) base = atomic_add_u32_ret(count, static_cast<unsigned>(n));
+    base = __shfl(base, 0);
+    for (int k = lane; k < n; k += 64) {
+      const unsigned slot = base + static_cast<unsigned>(k);
+      if (slot >= cap) continue;
+      const float4 q = stage[k];
+      out_pos[3ull * slot + 0] = q.x;
+      out_pos[3ull * slot + 1] = q.y;
+      out_pos[3ull * slot + 2] = q.z;
+      out_state[slot] = static_cast<unsigned char>(__float_as_int(q.w));
+    }
+    __builtin_amdgcn_wave_barrier();  // the stage is refilled after this
+    n = 0;
+  };
+  const int waves = static_cast<int>(blockDim.x) >> 6;
+  const int step = 2 * nblocks * waves;
+  for (int col0 = 2 * (block * waves + (static_cast<int>(threadIdx.x) >> 6)); col0 < cols;
+       col0 += step) {  // wave-uniform
+    const int col = col0 + (lane >> 5);
+    const int tx = col / f.diam;
+    const int ty = col - tx * f.diam;
+    const int gx = c.center_g.x + tx - f.r;
+    const int gy = c.center_g.y + ty - f.r;
+    const float px = global_to_pos(gx, c.resolution, f.ctr);
+    const float py = global_to_pos(gy, c.resolution, f.ctr);
+    const float dx = px - c.center.x;
+    const float dy = py - c.center.y;
+    const float dxy = dx * dx + dy * dy;
+    const bool col_in = col < cols && abs(gx - c.origin_i.x) <= c.half_map_size_i.x &&
+                        abs(gy - c.origin_i.y) <= c.half_map_size_i.y &&
+                        !(dxy > c.radius_sq);
+    if (__ballot(col_in) == 0ull) continue;
+    const int hxy =
+        col_in ? (in_window_coord(gx, c.origin_i.x, c.half_map_size_i.x, c.map_size_i.x, f.qx) *
+                      c.map_size_i.y +
+                  in_window_coord(gy, c.origin_i.y, c.half_map_size_i.y, c.map_size_i.y, f.qy)) *
+                     c.map_size_i.z
+               : 0;
+    for (int z0 = 0; z0 < f.nz; z0 += 32) {  // wave-uniform
+      const int gz = c.gz_min + z0 + zl;
+      const float pz = global_to_pos(gz, c.resolution, f.ctr);
+      unsigned char st = 0;
+      bool keep = false;
+      if (col_in && z0 + zl < f.nz && abs(gz - c.origin_i.z) <= c.half_map_size_i.z) {
+        const float dz = pz - c.center.z;
+        if (!(dxy + dz * dz > c.radius_sq)) {
+          const int h = hxy + in_window_coord(gz, c.origin_i.z, c.half_map_size_i.z,
+                                              c.map_size_i.z, f.qz);
+          const float l = occ[h] + c.l_unknown;
+          if (l >= c.l_occupied) {
+            st = 2;
+          } else if (l < c.l_free) {
+            st = 1;
+          }
+          keep = c.state_filter < 0 || static_cast<int>(st) == c.state_filter;
+        }
+      }
+      const unsigned long long m = __ballot(keep);
+      if (m == 0ull) continue;
+      if (keep) {
+        stage[n + __popcll(m & lt_mask)] =
+            make_float4(px, py, pz, __int_as_float(static_cast<int>(st)));
+      }
+      n += __popcll(m);
+      if (n > kWaveStage - 64) flush();  // room for the next 64
+    }
+  }
+  flush();
 }
 
 // ===================================================================
@@ -575,11 +638,12 @@ __global__ void __launch_bounds__(256)
                        float *__restrict__ out_pos,
                        unsigned char *__restrict__ out_state,
                        const unsigned per_query_cap) {
+  __shared__ float4 s_stage[4][kWaveStage];  // 256 threads: four waves
   const int q = blockIdx.y;
   const cublox_fetch_cfg c = cfgs[q];
   const size_t off = static_cast<size_t>(q) * per_query_cap;
-  fetch_body(c, occ, counts + q, out_pos + 3 * off, out_state + off,
-             per_query_cap, 0u, blockIdx.x, gridDim.x);
+  fetch_columns(c, occ, counts + q, out_pos + 3 * off, out_state + off, per_query_cap,
+                blockIdx.x, gridDim.x, s_stage[threadIdx.x >> 6]);
 }
 
 constexpr int kMaxGridY = 65535;  // queries per batched launch
@@ -1205,12 +1269,13 @@ int cublox_launch_fetch_occupancy_batch(const cublox_fetch_cfg *d_cfgs,
   if (total > INT_MAX) {
     return set_error(CUBLOX_ERR_INVALID, "fetch batch: more than 2^31 - 1 voxels per query");
   }
-  // A few workgroups per query, each striding over the disk volume about
-  // sixteen times (a workgroup per 256 voxels made the dispatch of tens of
-  // thousands of short workgroups the cost: 0.206 -> 0.078 ms for 1,000
-  // queries of 29,791 voxels). Queries go in launches of at
+  // A few workgroups per query (eight disk columns per pass of a 256-thread
+  // workgroup), each striding over the columns about sixteen times: a
+  // workgroup per few hundred voxels made the dispatch of tens of
+  // thousands of short workgroups the cost. Queries go in launches of at
   // most 65,535 (the grid's y extent).
-  long long blocks = (total + 4095) / 4096;
+  const long long cols = static_cast<long long>(max_disk_diam) * max_disk_diam;
+  long long blocks = (cols + 127) / 128;
   blocks = std::max<long long>(1, std::min<long long>(blocks, 64));
   for (int q0 = 0; q0 < n_queries; q0 += kMaxGridY) {
     const int nq = std::min(kMaxGridY, n_queries - q0);

@@ -366,14 +366,21 @@ class OccupancyGrid:
         cs = np.ascontiguousarray(np.asarray(centers, np.float32).reshape(-1, 3))
         q = cs.shape[0]
         counts = np.zeros(q, np.uint32)
-        pos = np.zeros((q * per_query_cap, 3), np.float32)
-        st = np.zeros(q * per_query_cap, np.uint8)
+        # The binding writes at most q * per_query_cap samples: output
+        # arrays of that size are kept between calls (they only grow), and
+        # the filled part is returned as copies.
+        need = max(q * int(per_query_cap), 1)
+        buf = getattr(self, "_fetch_out", None)
+        if buf is None or buf[1].shape[0] < need:
+            buf = (np.empty((need, 3), np.float32), np.empty(need, np.uint8))
+            self._fetch_out = buf
+        pos, st = buf
         self._check(self._L.cublox_og_fetch_batch(
             self._h, _ffi.fptr(cs), q, float(radius),
             -1 if state_filter is None else int(state_filter), int(per_query_cap),
             _ffi.uptr(counts), _ffi.fptr(pos), _ffi.bptr(st)))
         total = int(counts.sum())
-        return counts, pos[:total], st[:total]
+        return counts, pos[:total].copy(), st[:total].copy()
 
     def _query(self, kind: int, by: int, q) -> bool:
         qf = np.zeros(3, np.float32)

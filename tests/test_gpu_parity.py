@@ -149,3 +149,45 @@ def test_batched_fetch_beyond_one_launch(cb, orc, gpu_ready):
         a, b = offs[j], offs[j + 1]
         assert np.array_equal(sample_multiset(pos[a:b], st[a:b]),
                               sample_multiset(op_, os_)), j
+
+
+def test_batched_fetch_dense_queries(cb, orc, gpu_ready):
+    """Queries that keep hundreds of samples per wave (no state filter,
+    1.5 m spheres, some reaching past the window): each wave's staged
+    samples flush several times; counts and samples equal the oracle's.
+    With a cap below the counts, counts stay exact and every returned
+    sample is one of the oracle's, without repeats."""
+    from helpers import sample_multiset
+    w = cb.WORKLOADS["small"]
+    scene = w.scene()
+    g, o = make_pair(cb, orc, w)
+    for k in range(3):
+        s = scene.scan(k, w.beams, w.azimuths, w.el_lo, w.el_hi, w.max_range)
+        g.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+        o.update_sensor_frame((s.xs, s.ys, s.zs), s.T)
+    rng = np.random.default_rng(21)
+    base = np.asarray(s.base, np.float32)
+    cs = (base + rng.uniform([-11, -11, -3], [11, 11, 3], (24, 3))).astype(np.float32)
+    radius = 1.5
+    full = 40_000
+    counts, pos, st = g.fetch_occupancy_batch(cs, radius, None, full)
+    offs = np.zeros(len(cs) + 1, np.int64)
+    offs[1:] = np.cumsum(counts.astype(np.int64))
+    assert counts.max() > 5_000, "queries should keep thousands of samples"
+    for j, c in enumerate(cs):
+        op_, os_, on = o.fetch(c, radius, None, full, cap=full)
+        assert counts[j] == on, j
+        a, b = offs[j], offs[j + 1]
+        assert np.array_equal(sample_multiset(pos[a:b], st[a:b]),
+                              sample_multiset(op_, os_)), j
+    cap = 1000
+    counts_c, pos_c, st_c = g.fetch_occupancy_batch(cs, radius, None, cap)
+    assert np.array_equal(counts_c, np.minimum(counts, cap))
+    offs_c = np.zeros(len(cs) + 1, np.int64)
+    offs_c[1:] = np.cumsum(counts_c.astype(np.int64))
+    for j, c in enumerate(cs):
+        op_, os_, _ = o.fetch(c, radius, None, full, cap=full)
+        every = set(map(tuple, sample_multiset(op_, os_).tolist()))
+        got = sample_multiset(pos_c[offs_c[j]:offs_c[j + 1]], st_c[offs_c[j]:offs_c[j + 1]])
+        rows = list(map(tuple, got.tolist()))
+        assert len(set(rows)) == len(rows) and set(rows) <= every, j
