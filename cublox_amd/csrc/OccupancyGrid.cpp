@@ -413,9 +413,10 @@ unsigned long long OccupancyGrid::changedVoxels(std::vector<ChangedVoxel> &out) 
   std::vector<int> ids(n);
   std::vector<float> vals(n);
   if (n > 0) {
-    CUBLOX_CHECK(cublox_memcpy_d2h_async(ids.data(), d_delta_idx_, n * sizeof(int), stream_));
-    CUBLOX_CHECK(cublox_memcpy_d2h_async(vals.data(), d_delta_val_, n * sizeof(float), stream_));
-    sync();
+    void *const dst[2] = {ids.data(), vals.data()};
+    const void *const src[2] = {d_delta_idx_, d_delta_val_};
+    const size_t len[2] = {n * sizeof(int), n * sizeof(float)};
+    downloadHost(dst, src, len, 2);
   }
   out.resize(n);
   for (unsigned int k = 0; k < n; ++k) out[k] = ChangedVoxel{ids[k], vals[k]};
@@ -967,9 +968,12 @@ std::vector<OccupancyGrid::OccupancySample> OccupancyGrid::fetchOccupancyAround(
 
   std::vector<float> pos(static_cast<size_t>(n) * 3);
   std::vector<unsigned char> st(n);
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(pos.data(), d_fetch_pos_, pos.size() * sizeof(float), stream_));
-  CUBLOX_CHECK(cublox_memcpy_d2h_async(st.data(), d_fetch_state_, st.size(), stream_));
-  sync();
+  {
+    void *const dst[2] = {pos.data(), st.data()};
+    const void *const src[2] = {d_fetch_pos_, d_fetch_state_};
+    const size_t len[2] = {pos.size() * sizeof(float), st.size()};
+    downloadHost(dst, src, len, 2);
+  }
   out.reserve(n);
   for (unsigned int i = 0; i < n; ++i) {
     out.push_back(OccupancySample{
@@ -1035,10 +1039,10 @@ size_t OccupancyGrid::buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
       std::min<unsigned int>(counts[1], static_cast<unsigned int>(out_cap));
   if (written > 0) {
     out.resize(written);
-    CUBLOX_CHECK(cublox_memcpy_d2h_async(out.data(), d_viz_,
-                                         static_cast<size_t>(written) * sizeof(VizPoint),
-                                         stream_));
-    sync();
+    void *const dst[1] = {out.data()};
+    const void *const src[1] = {d_viz_};
+    const size_t len[1] = {static_cast<size_t>(written) * sizeof(VizPoint)};
+    downloadHost(dst, src, len, 1);
   }
   // fetchOccupancyAround's sample count (OccupancyGrid.cpp:335-356).
   unsigned int n = std::min(counts[0], output_cap);
@@ -1047,6 +1051,29 @@ size_t OccupancyGrid::buildOccupancyVizCloud(const Eigen::Vector3f &robot_pos,
 }
 
 // ---- map snapshots ----------------------------------------------------
+
+// Device bytes to host memory through the pinned stage (pageable copies
+// run at a few GB/s): every part goes out by DMA into the stage, then one
+// host copy each. Syncs the stream.
+void OccupancyGrid::downloadHost(void *const *h_dsts, const void *const *d_srcs,
+                                 const size_t *lens, int parts) const {
+  if (parts < 0 || parts > 8) throw std::logic_error("downloadHost: at most 8 parts");
+  size_t off[8];
+  size_t total = 0;
+  for (int k = 0; k < parts; ++k) {
+    off[k] = total;
+    total = (total + lens[k] + 255) & ~static_cast<size_t>(255);
+  }
+  if (total == 0) return;
+  uint8_t *stage = pinnedStage(total);
+  for (int k = 0; k < parts; ++k) {
+    if (lens[k]) CUBLOX_CHECK(cublox_memcpy_d2h_async(stage + off[k], d_srcs[k], lens[k], stream_));
+  }
+  sync();
+  for (int k = 0; k < parts; ++k) {
+    if (lens[k]) std::memcpy(h_dsts[k], stage + off[k], lens[k]);
+  }
+}
 
 uint8_t *OccupancyGrid::pinnedStage(size_t bytes) const {
   if (bytes > h_stage_bytes_) {

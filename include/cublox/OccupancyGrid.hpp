@@ -372,9 +372,11 @@ private:
   void uploadHost(void *d_dst, const void *const *h_srcs, const size_t *lens,
                   int parts);
   void uploadColumns(const float *x, const float *y, const float *z, int n);
-  mutable void *h_stage_{nullptr};  // pinned staging of snapshot saves
+  mutable void *h_stage_{nullptr};  // pinned staging of downloads and snapshots
   mutable size_t h_stage_bytes_{0};
   uint8_t *pinnedStage(size_t bytes) const;
+  void downloadHost(void *const *h_dsts, const void *const *d_srcs, const size_t *lens,
+                    int parts) const;  // at most 8 parts
   mutable float *d_viz_{nullptr};
   mutable unsigned int *d_viz_count_{nullptr};
   mutable long long viz_capacity_{0};
