@@ -933,8 +933,9 @@ def main():
         extras["h2d_inclusive"] = dict(
             ms_per_scan=round(statistics.mean(pcie), 5),
             p50_ms=round(pct(pcie, 50), 5), p99_ms=round(pct(pcie, 99), 5),
-            form="blocking updateSensorFrame on host columns (staged through a "
-                 "grid-owned pinned buffer, then DMA) + recenter, native loop",
+            form="blocking updateSensorFrame on host columns (copied into a "
+                 "grid-owned pinned buffer that the ray walk reads over the bus) "
+                 "+ recenter, native loop",
             speedup_vs_3090_readme=round(README_3090_MS / statistics.mean(pcie), 2),
             target_ms=0.5)
         log(f"h2d-inclusive: {extras['h2d_inclusive']}")

@@ -99,6 +99,7 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_dense_state_ = std::exchange(o.d_fetch_dense_state_, nullptr);
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
   h_upload_ = std::exchange(o.h_upload_, nullptr);
+  h_upload_dev_ = std::exchange(o.h_upload_dev_, nullptr);
   h_upload_bytes_ = std::exchange(o.h_upload_bytes_, 0);
   h_stage_ = std::exchange(o.h_stage_, nullptr);
   h_stage_bytes_ = std::exchange(o.h_stage_bytes_, 0);
@@ -163,6 +164,7 @@ void OccupancyGrid::releaseAll() noexcept {
   h_stage_bytes_ = 0;
   if (h_upload_) cublox_host_free(h_upload_);
   h_upload_ = nullptr;
+  h_upload_dev_ = nullptr;
   h_upload_bytes_ = 0;
   freeDevice(d_viz_);
   freeDevice(d_viz_count_);
@@ -590,6 +592,7 @@ void OccupancyGrid::uploadHost(void *d_dst, const void *const *h_srcs,
     sync();  // an earlier upload may still read the old buffer
     if (h_upload_) cublox_host_free(h_upload_);
     h_upload_ = nullptr;
+    h_upload_dev_ = nullptr;
     h_upload_bytes_ = 0;
     const size_t cap = std::max<size_t>(bytes, size_t{1} << 20);
     CUBLOX_CHECK(cublox_host_alloc(&h_upload_, cap));
@@ -606,6 +609,36 @@ void OccupancyGrid::uploadHost(void *d_dst, const void *const *h_srcs,
     CUBLOX_CHECK(cublox_memcpy_h2d_async(dst + base, stage + base, lens[k], stream_));
     base += lens[k];
   }
+}
+
+// Host columns staged in the grid's pinned buffer for the fused path,
+// which reads them from there over the bus (each point is read by one
+// walk workgroup): no DMA to HBM before the walk. *dx, *dy, *dz receive
+// the columns' device addresses. The buffer is reused by the next call,
+// and every public update syncs its stream before returning.
+void OccupancyGrid::stageColumns(const float *x, const float *y, const float *z, int n,
+                                 const float **dx, const float **dy, const float **dz) {
+  const size_t b = static_cast<size_t>(n) * sizeof(float);
+  const size_t col = (b + 255) & ~static_cast<size_t>(255);
+  if (3 * col > h_upload_bytes_) {
+    sync();  // an earlier upload may still read the old buffer
+    if (h_upload_) cublox_host_free(h_upload_);
+    h_upload_ = nullptr;
+    h_upload_dev_ = nullptr;
+    h_upload_bytes_ = 0;
+    const size_t cap = std::max<size_t>(3 * col, size_t{1} << 20);
+    CUBLOX_CHECK(cublox_host_alloc(&h_upload_, cap));
+    h_upload_bytes_ = cap;
+  }
+  if (!h_upload_dev_) CUBLOX_CHECK(cublox_host_device_pointer(&h_upload_dev_, h_upload_));
+  auto *stage = static_cast<uint8_t *>(h_upload_);
+  std::memcpy(stage, x, b);
+  std::memcpy(stage + col, y, b);
+  std::memcpy(stage + 2 * col, z, b);
+  const auto *dev = static_cast<const uint8_t *>(h_upload_dev_);
+  *dx = reinterpret_cast<const float *>(dev);
+  *dy = reinterpret_cast<const float *>(dev + col);
+  *dz = reinterpret_cast<const float *>(dev + 2 * col);
 }
 
 void OccupancyGrid::uploadColumns(const float *x, const float *y, const float *z,
@@ -648,12 +681,13 @@ void OccupancyGrid::updateColumns(const float *x, const float *y,
                                   const Eigen::Vector3f &sensor_origin) {
   if (config_.voxel_num <= 0 || n <= 0) return;
   pinFirstOrigin(sensor_origin);
-  ensureCloudCapacity(n);
-  uploadColumns(x, y, z, n);
   if (update_path_ == UpdatePath::FUSED_TILES) {
-    launchIntegrate(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, nullptr,
-                    sensor_origin, false);
+    const float *dx, *dy, *dz;
+    stageColumns(x, y, z, n, &dx, &dy, &dz);
+    launchIntegrate(dx, dy, dz, n, nullptr, sensor_origin, false);
   } else {
+    ensureCloudCapacity(n);
+    uploadColumns(x, y, z, n);
     castRaysWorld(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, sensor_origin);
     launchApply();
   }
@@ -679,12 +713,13 @@ void OccupancyGrid::updateSensorFrameColumns(const float *xs, const float *ys,
                                              const float T[12]) {
   if (config_.voxel_num <= 0 || n <= 0) return;
   pinFirstOrigin(Eigen::Vector3f(T[3], T[7], T[11]));
-  ensureCloudCapacity(n);
-  uploadColumns(xs, ys, zs, n);
   if (update_path_ == UpdatePath::FUSED_TILES) {
-    launchIntegrate(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T,
-                    Eigen::Vector3f(T[3], T[7], T[11]), false);
+    const float *dx, *dy, *dz;
+    stageColumns(xs, ys, zs, n, &dx, &dy, &dz);
+    launchIntegrate(dx, dy, dz, n, T, Eigen::Vector3f(T[3], T[7], T[11]), false);
   } else {
+    ensureCloudCapacity(n);
+    uploadColumns(xs, ys, zs, n);
     castRaysSensor(d_cloud_x_, d_cloud_y_, d_cloud_z_, n, T);
     launchApply();
   }

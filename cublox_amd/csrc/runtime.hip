@@ -97,6 +97,13 @@ int cublox_host_alloc(void **ptr, size_t bytes) {
   return CUBLOX_OK;
 }
 
+int cublox_host_device_pointer(void **dptr, void *hptr) {
+  if (!dptr || !hptr) return set_error(CUBLOX_ERR_INVALID, "null ptr");
+  *dptr = nullptr;
+  CUBLOX_TRY(hipHostGetDevicePointer(dptr, hptr, 0));
+  return CUBLOX_OK;
+}
+
 int cublox_host_free(void *ptr) {
   if (!ptr) return CUBLOX_OK;
   CUBLOX_TRY(hipHostFree(ptr));

@@ -368,7 +368,10 @@ private:
   // Pinned staging of host inputs (clouds, payloads, images): a host copy
   // into it, then asynchronous DMA, in pieces so the two overlap.
   void *h_upload_{nullptr};
+  void *h_upload_dev_{nullptr};  // its device address (stageColumns)
   size_t h_upload_bytes_{0};
+  void stageColumns(const float *x, const float *y, const float *z, int n,
+                    const float **dx, const float **dy, const float **dz);
   void uploadHost(void *d_dst, const void *const *h_srcs, const size_t *lens,
                   int parts);
   void uploadColumns(const float *x, const float *y, const float *z, int n);

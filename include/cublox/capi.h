@@ -62,6 +62,9 @@ int cublox_malloc(void **ptr, size_t bytes);
 int cublox_free(void *ptr);
 int cublox_host_alloc(void **ptr, size_t bytes); /* pinned host memory */
 int cublox_host_free(void *ptr);
+/* The device address of pinned host memory from cublox_host_alloc: kernels
+ * given it read the host memory over the bus, with no copy to HBM first. */
+int cublox_host_device_pointer(void **dptr, void *hptr);
 int cublox_memset_async(void *ptr, int value, size_t bytes, cublox_stream s);
 int cublox_memcpy_h2d_async(void *dst, const void *src, size_t bytes,
                             cublox_stream s);
