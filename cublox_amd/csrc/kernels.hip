@@ -464,11 +464,41 @@ __device__ __forceinline__ void fetch_columns(const cublox_fetch_cfg &c,
   };
   const int waves = static_cast<int>(blockDim.x) >> 6;
   const int step = 2 * nblocks * waves;
-  for (int col0 = 2 * (block * waves + (static_cast<int>(threadIdx.x) >> 6)); col0 < cols;
-       col0 += step) {  // wave-uniform
+  // The lane's column col = tx * diam + ty advances by `step` per pass
+  // without a division: step = sq * diam + sr with sr < diam.
+  const int col_lane = 2 * (block * waves + (static_cast<int>(threadIdx.x) >> 6)) + (lane >> 5);
+  const int sq = step / f.diam;
+  const int sr = step - sq * f.diam;
+  int tx = col_lane / f.diam;
+  int ty = col_lane - tx * f.diam;
+  auto advance = [&]() {
+    tx += sq;
+    ty += sr;
+    if (ty >= f.diam) {
+      ty -= f.diam;
+      ++tx;
+    }
+  };
+  // The z terms of a lane are the same in every column: those of the
+  // first 32 z voxels are formed once (deeper columns form the rest).
+  struct ZTerms {
+    float pz, dz2;
+    int hz;
+    bool in;
+  };
+  auto z_terms = [&](int z0) {
+    const int gz = c.gz_min + z0 + zl;
+    ZTerms t;
+    t.pz = global_to_pos(gz, c.resolution, f.ctr);
+    const float dz = t.pz - c.center.z;
+    t.dz2 = dz * dz;
+    t.in = z0 + zl < f.nz && abs(gz - c.origin_i.z) <= c.half_map_size_i.z;
+    t.hz = in_window_coord(gz, c.origin_i.z, c.half_map_size_i.z, c.map_size_i.z, f.qz);
+    return t;
+  };
+  const ZTerms z_first = z_terms(0);
+  for (int col0 = col_lane - (lane >> 5); col0 < cols; col0 += step, advance()) {  // wave-uniform
     const int col = col0 + (lane >> 5);
-    const int tx = col / f.diam;
-    const int ty = col - tx * f.diam;
     const int gx = c.center_g.x + tx - f.r;
     const int gy = c.center_g.y + ty - f.r;
     const float px = global_to_pos(gx, c.resolution, f.ctr);
@@ -487,15 +517,13 @@ __device__ __forceinline__ void fetch_columns(const cublox_fetch_cfg &c,
                      c.map_size_i.z
                : 0;
     for (int z0 = 0; z0 < f.nz; z0 += 32) {  // wave-uniform
-      const int gz = c.gz_min + z0 + zl;
-      const float pz = global_to_pos(gz, c.resolution, f.ctr);
+      const ZTerms zt = z0 == 0 ? z_first : z_terms(z0);
+      const float pz = zt.pz;
       unsigned char st = 0;
       bool keep = false;
-      if (col_in && z0 + zl < f.nz && abs(gz - c.origin_i.z) <= c.half_map_size_i.z) {
-        const float dz = pz - c.center.z;
-        if (!(dxy + dz * dz > c.radius_sq)) {
-          const int h = hxy + in_window_coord(gz, c.origin_i.z, c.half_map_size_i.z,
-                                              c.map_size_i.z, f.qz);
+      if (col_in && zt.in) {
+        if (!(dxy + zt.dz2 > c.radius_sq)) {
+          const int h = hxy + zt.hz;
           const float l = occ[h] + c.l_unknown;
           if (l >= c.l_occupied) {
             st = 2;
