@@ -658,6 +658,25 @@ def run_driver_steps(cb, w, scans, bufs, n, warmup, steps, g, base_update_ms):
     gi.close()
     del rec
 
+    # The same payloads from host memory (as a driver receives them): each
+    # scan's records are copied into the grid's pinned stage, which the
+    # ingest kernels read over the bus.
+    hrec = _records(scans, n)
+    gh = new_grid(cb, w)
+    per = []
+    for i in range(total):
+        s = scans[i]
+        payload = hrec[4 * n * i:4 * n * (i + 1)]  # n records of 4 floats
+        a = time.perf_counter()
+        gh.update_point_cloud2(payload, n, 1, 16, (0, 4, 8), s.T)
+        gh.recenter(s.base)
+        if i >= warmup:
+            per.append((time.perf_counter() - a) * 1e3)
+    out["ingest_point_cloud2_host"] = dict(ms_per_scan=statistics.mean(per),
+                                           p99_ms=pct(per, 99), point_step=16)
+    gh.close()
+    del hrec
+
     # Depth images (640x480, 16-bit mm), device-resident.
     frames = _depth_frames(min(total, 30))
     dbuf = cb.DeviceBuffer.from_array(np.stack(frames).reshape(-1))

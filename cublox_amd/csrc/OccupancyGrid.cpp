@@ -106,8 +106,6 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_viz_ = std::exchange(o.d_viz_, nullptr);
   d_viz_count_ = std::exchange(o.d_viz_count_, nullptr);
   viz_capacity_ = std::exchange(o.viz_capacity_, 0);
-  d_raw_ = std::exchange(o.d_raw_, nullptr);
-  raw_capacity_ = std::exchange(o.raw_capacity_, 0);
   d_kept_ = std::exchange(o.d_kept_, nullptr);
   d_delta_count_ = std::exchange(o.d_delta_count_, nullptr);
   d_delta_idx_ = std::exchange(o.d_delta_idx_, nullptr);
@@ -157,8 +155,6 @@ void OccupancyGrid::releaseAll() noexcept {
   freeDevice(d_delta_idx_);
   freeDevice(d_delta_val_);
   delta_capacity_ = 0;
-  freeDevice(d_raw_);
-  raw_capacity_ = 0;
   if (h_stage_) cublox_host_free(h_stage_);
   h_stage_ = nullptr;
   h_stage_bytes_ = 0;
@@ -620,18 +616,7 @@ void OccupancyGrid::stageColumns(const float *x, const float *y, const float *z,
                                  const float **dx, const float **dy, const float **dz) {
   const size_t b = static_cast<size_t>(n) * sizeof(float);
   const size_t col = (b + 255) & ~static_cast<size_t>(255);
-  if (3 * col > h_upload_bytes_) {
-    sync();  // an earlier upload may still read the old buffer
-    if (h_upload_) cublox_host_free(h_upload_);
-    h_upload_ = nullptr;
-    h_upload_dev_ = nullptr;
-    h_upload_bytes_ = 0;
-    const size_t cap = std::max<size_t>(3 * col, size_t{1} << 20);
-    CUBLOX_CHECK(cublox_host_alloc(&h_upload_, cap));
-    h_upload_bytes_ = cap;
-  }
-  if (!h_upload_dev_) CUBLOX_CHECK(cublox_host_device_pointer(&h_upload_dev_, h_upload_));
-  auto *stage = static_cast<uint8_t *>(h_upload_);
+  uint8_t *stage = readStage(3 * col);
   std::memcpy(stage, x, b);
   std::memcpy(stage + col, y, b);
   std::memcpy(stage + 2 * col, z, b);
@@ -639,6 +624,30 @@ void OccupancyGrid::stageColumns(const float *x, const float *y, const float *z,
   *dx = reinterpret_cast<const float *>(dev);
   *dy = reinterpret_cast<const float *>(dev + col);
   *dz = reinterpret_cast<const float *>(dev + 2 * col);
+}
+
+// The pinned stage with room for `bytes`, and its device address set.
+uint8_t *OccupancyGrid::readStage(size_t bytes) {
+  if (bytes > h_upload_bytes_) {
+    sync();  // an earlier call's kernels may still read the old buffer
+    if (h_upload_) cublox_host_free(h_upload_);
+    h_upload_ = nullptr;
+    h_upload_dev_ = nullptr;
+    h_upload_bytes_ = 0;
+    const size_t cap = std::max<size_t>(bytes, size_t{1} << 20);
+    CUBLOX_CHECK(cublox_host_alloc(&h_upload_, cap));
+    h_upload_bytes_ = cap;
+  }
+  if (!h_upload_dev_) CUBLOX_CHECK(cublox_host_device_pointer(&h_upload_dev_, h_upload_));
+  return static_cast<uint8_t *>(h_upload_);
+}
+
+// A host payload (PointCloud2 records, a depth image) copied into the
+// pinned stage; returns the copy's device address, which the ingest
+// kernels read over the bus (no DMA to HBM first).
+const void *OccupancyGrid::stageHost(const void *src, size_t bytes) {
+  std::memcpy(readStage(bytes), src, bytes);
+  return h_upload_dev_;
 }
 
 void OccupancyGrid::uploadColumns(const float *x, const float *y, const float *z,
@@ -832,16 +841,7 @@ bool OccupancyGrid::updatePointCloud2(const void *data,
   checkPayload(layout, n, data_bytes);
   if (n == 0) return false;
   const size_t bytes = static_cast<size_t>(n) * layout.point_step;
-  if (bytes > raw_capacity_) {
-    sync();
-    freeDevice(d_raw_);
-    raw_capacity_ = 0;
-    CUBLOX_CHECK(cublox_malloc(&d_raw_, bytes));
-    raw_capacity_ = bytes;
-  }
-  const void *srcs[1] = {data};
-  uploadHost(d_raw_, srcs, &bytes, 1);
-  return updatePointCloud2Device(d_raw_, layout, T, bytes);
+  return updatePointCloud2Device(stageHost(data, bytes), layout, T, bytes);
 }
 
 bool OccupancyGrid::updateDepthImageDevice(const void *d_data, uint32_t width,
@@ -864,16 +864,7 @@ bool OccupancyGrid::updateDepthImage(const void *data, uint32_t width,
                                      const float T[12]) {
   if (width == 0 || height == 0) return false;
   const size_t bytes = static_cast<size_t>(step) * height;
-  if (bytes > raw_capacity_) {
-    sync();
-    freeDevice(d_raw_);
-    raw_capacity_ = 0;
-    CUBLOX_CHECK(cublox_malloc(&d_raw_, bytes));
-    raw_capacity_ = bytes;
-  }
-  const void *srcs[1] = {data};
-  uploadHost(d_raw_, srcs, &bytes, 1);
-  return updateDepthImageDevice(d_raw_, width, height, step, encoding, K, T);
+  return updateDepthImageDevice(stageHost(data, bytes), width, height, step, encoding, K, T);
 }
 
 void OccupancyGrid::updateDevice(const float *d_x, const float *d_y,

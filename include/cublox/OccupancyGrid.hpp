@@ -343,8 +343,6 @@ private:
   int *d_hit_cnt_{nullptr};
   float *d_occ_{nullptr};
 
-  void *d_raw_{nullptr};  // ingest payload staging
-  size_t raw_capacity_{0};
   unsigned int *d_kept_{nullptr};
 
   unsigned int *d_delta_count_{nullptr};
@@ -372,6 +370,8 @@ private:
   size_t h_upload_bytes_{0};
   void stageColumns(const float *x, const float *y, const float *z, int n,
                     const float **dx, const float **dy, const float **dz);
+  uint8_t *readStage(size_t bytes);
+  const void *stageHost(const void *src, size_t bytes);
   void uploadHost(void *d_dst, const void *const *h_srcs, const size_t *lens,
                   int parts);
   void uploadColumns(const float *x, const float *y, const float *z, int n);
