@@ -100,6 +100,7 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
   d_fetch_offsets_ = std::exchange(o.d_fetch_offsets_, nullptr);
   h_upload_ = std::exchange(o.h_upload_, nullptr);
   h_upload_dev_ = std::exchange(o.h_upload_dev_, nullptr);
+  stage_in_use_ = std::exchange(o.stage_in_use_, false);
   h_upload_bytes_ = std::exchange(o.h_upload_bytes_, 0);
   h_stage_ = std::exchange(o.h_stage_, nullptr);
   h_stage_bytes_ = std::exchange(o.h_stage_bytes_, 0);
@@ -140,6 +141,7 @@ OccupancyGrid &OccupancyGrid::operator=(OccupancyGrid &&o) noexcept {
 
 void OccupancyGrid::releaseAll() noexcept {
   if (stream_) cublox_stream_synchronize(stream_);
+  stage_in_use_ = false;
   zero_pending_ = false;  // the map goes: its pending clear with it
   freeDevice(d_fetch_state_);
   freeDevice(d_fetch_pos_);
@@ -242,7 +244,10 @@ void OccupancyGrid::harvestTimes() const {
   pending_zeroed_ = 0;
 }
 
-void OccupancyGrid::sync() const { CUBLOX_CHECK(cublox_stream_synchronize(stream_)); }
+void OccupancyGrid::sync() const {
+  CUBLOX_CHECK(cublox_stream_synchronize(stream_));
+  stage_in_use_ = false;
+}
 
 // OccupancyGrid.cpp:431-480: the occupancy array of V, zeroed; partial
 // allocations are rolled back before throwing. The per-voxel op/hit counters
@@ -584,8 +589,11 @@ void OccupancyGrid::uploadHost(void *d_dst, const void *const *h_srcs,
   size_t bytes = 0;
   for (int k = 0; k < parts; ++k) bytes += lens[k];
   if (bytes == 0) return;
+  if (stage_in_use_) sync();  // kernels of a call that threw may read it
+  stage_in_use_ = true;
   if (bytes > h_upload_bytes_) {
     sync();  // an earlier upload may still read the old buffer
+    stage_in_use_ = true;
     if (h_upload_) cublox_host_free(h_upload_);
     h_upload_ = nullptr;
     h_upload_dev_ = nullptr;
@@ -626,8 +634,11 @@ void OccupancyGrid::stageColumns(const float *x, const float *y, const float *z,
   *dz = reinterpret_cast<const float *>(dev + 2 * col);
 }
 
-// The pinned stage with room for `bytes`, and its device address set.
+// The pinned stage with room for `bytes`, and its device address set; the
+// caller's kernels will read it (until the next sync). A call that threw
+// after handing the stage to kernels leaves it marked: then wait for them.
 uint8_t *OccupancyGrid::readStage(size_t bytes) {
+  if (stage_in_use_) sync();
   if (bytes > h_upload_bytes_) {
     sync();  // an earlier call's kernels may still read the old buffer
     if (h_upload_) cublox_host_free(h_upload_);
@@ -639,6 +650,7 @@ uint8_t *OccupancyGrid::readStage(size_t bytes) {
     h_upload_bytes_ = cap;
   }
   if (!h_upload_dev_) CUBLOX_CHECK(cublox_host_device_pointer(&h_upload_dev_, h_upload_));
+  stage_in_use_ = true;
   return static_cast<uint8_t *>(h_upload_);
 }
 

@@ -367,6 +367,7 @@ private:
   // into it, then asynchronous DMA, in pieces so the two overlap.
   void *h_upload_{nullptr};
   void *h_upload_dev_{nullptr};  // its device address (stageColumns)
+  mutable bool stage_in_use_{false};  // kernels may read the stage until a sync
   size_t h_upload_bytes_{0};
   void stageColumns(const float *x, const float *y, const float *z, int n,
                     const float **dx, const float **dy, const float **dz);
