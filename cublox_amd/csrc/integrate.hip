@@ -185,6 +185,7 @@ struct Workspace {
   int *near_hit;      // near-field cells (zero between launches)
   int *big;           // item slots of unsplit hot tiles (passes)
   int *block_sums;    // multi-block tile scan: per-block sums
+  int *scan_flag;     // one-pass small scan: block sums published (zero between launches)
   int *block_bases;   // multi-block tile scan: per-block bases
   unsigned *seg_rank; // n * max_seg: each record's place in its tile (rank_code)
   int *grp_base;      // per counting group and hash entry: the group's first
@@ -1250,16 +1251,36 @@ __global__ void __launch_bounds__(kScanThreads)
 // thread of the block calls this.
 static_assert((kScanSmall + kScanSmallTiles - 1) / kScanSmallTiles <= 64,
               "one wave sums the block sums of a small scan");
+// With `published`, the sums come from the blocks of this same launch:
+// lane t waits (a bounded spin) for block t's flag, then reads its sums
+// with agent-scope loads.
+constexpr int kScanSpinLimit = 1 << 22;
 __device__ __forceinline__ void own_block_bases(const Batch &b, const Workspace &w, int tpb,
-                                                int *s_base) {
+                                                int *s_base, bool published) {
   const int n_blocks = (batch_tiles(b) + tpb - 1) / tpb;
   const int t = threadIdx.x;
   const int me = static_cast<int>(blockIdx.x);
   if (t < 64) {
+    if (published && t < n_blocks) {
+      int spins = 0;
+      while (__hip_atomic_load(w.scan_flag + t, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+             0) {
+        if (++spins > kScanSpinLimit) {  // never expected: flag it and go on
+          __hip_atomic_store(w.misc + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
     int pre[kScanSums], tot[kScanSums];
 #pragma unroll
     for (int q = 0; q < kScanSums; ++q) {
-      const int x = t < n_blocks ? w.block_sums[t * kScanSums + q] : 0;
+      int x = 0;
+      if (t < n_blocks) {
+        x = published ? __hip_atomic_load(w.block_sums + t * kScanSums + q, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                      : w.block_sums[t * kScanSums + q];
+      }
       pre[q] = t < me ? x : 0;
       tot[q] = x;
     }
@@ -1297,6 +1318,8 @@ __device__ __forceinline__ void own_block_bases(const Batch &b, const Workspace 
 // length the largest such index plus one.
 __global__ void __launch_bounds__(kScanThreads)
     tile_place_kernel(const Batch b, Workspace w, int tpb, int own_bases) {
+  // own_bases: 0, the bases pass's; 1, summed from the sums pass's; 2, one
+  // pass (no sums launch): every block publishes its sums and sums them all.
   __shared__ int s_wave[kScanSums * kScanWaves];
   __shared__ int s_base[kScanSums];
   const int n_tiles = batch_tiles(b);
@@ -1308,13 +1331,7 @@ __global__ void __launch_bounds__(kScanThreads)
   const int kPer = tpb / static_cast<int>(blockDim.x);
   const int beg = min(k0 + t * kPer, k1);
   const int end = min(beg + kPer, k1);
-  // The thread's first count, in flight while the bases are summed.
   const int c_first = beg < end ? w.tile_count[beg] : 0;
-  const int *base = w.block_bases + blockIdx.x * kScanSums;
-  if (own_bases) {  // uniform
-    own_block_bases(b, w, tpb, s_base);
-    base = s_base;
-  }
   // Records, slabs, hot tiles and item slots per class (ch each) of the
   // thread's tiles, scanned over the block at once.
   int v[kScanSums] = {}, tv[kScanSums];
@@ -1336,6 +1353,25 @@ __global__ void __launch_bounds__(kScanThreads)
 #pragma unroll
   for (int q = 0; q < kScanSums; ++q) own[q] = v[q];
   block_scan_incl_n<kScanSums>(s_wave, v, tv);
+  const int *base = w.block_bases + blockIdx.x * kScanSums;
+  if (own_bases) {  // uniform
+    if (own_bases == 2 && threadIdx.x == 0) {
+      // One pass: this block's sums (its scan totals) published for the
+      // others; block 0 first clears the hot-list counters they add to.
+      if (blockIdx.x == 0) {
+        __hip_atomic_store(w.misc + 3, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(w.misc + 4, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int q = 0; q < kScanSums; ++q) {
+        __hip_atomic_store(w.block_sums + blockIdx.x * kScanSums + q, tv[q], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __hip_atomic_store(w.scan_flag + blockIdx.x, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    own_block_bases(b, w, tpb, s_base, own_bases == 2);
+    base = s_base;
+  }
   int cur[kCostBuckets];
 #pragma unroll
   for (int u = 0; u < kCostBuckets; ++u) cur[u] = base[3 + u] + v[3 + u] - own[3 + u];
@@ -1369,7 +1405,12 @@ __global__ void __launch_bounds__(kScanThreads)
 constexpr int kPlaceJ = 4;
 constexpr int kPlaceRows = 4;
 __global__ void __launch_bounds__(kWalkBlock)
-    place_kernel(const Batch b, Workspace w, int lg_group, int hash) {
+    place_kernel(const Batch b, Workspace w, int lg_group, int hash, int flags) {
+  // The one-pass scan's flags, zero again for the next launch (the scan
+  // has finished).
+  if (blockIdx.x == 0 && blockIdx.y == 0 && static_cast<int>(threadIdx.x) < flags) {
+    w.scan_flag[threadIdx.x] = 0;
+  }
   const int bx = static_cast<int>(blockIdx.x);
   const int ray = bx * kWalkBlock + static_cast<int>(threadIdx.x);
   const int ray0 = bx * kWalkBlock;
@@ -1953,6 +1994,7 @@ struct Layout {
   bool may_run_out;  // slabs below the chunk bound: hot tiles may stay unsplit
   int max_items, scan_blocks;
   int scan_tpb;  // tiles per multi-block scan block
+  size_t off_sflag;
   size_t off_count, off_stile, off_segs, off_sorted, off_tcount, off_tcur,
       off_items, off_big, off_hot, off_scratch, off_misc, off_nop, off_nhit, off_bsum,
       off_bbase, off_srank, off_gbase, off_rtd, off_hdr, off_desc, total;
@@ -2006,6 +2048,7 @@ Layout make_layout(const Caps &c) {
   take(L.off_hot, static_cast<size_t>(L.slabs) * sizeof(HotTile));
   take(L.off_scratch, static_cast<size_t>(L.slabs) * kTileVox * 4);
   take(L.off_misc, 64 * 4);
+  take(L.off_sflag, static_cast<size_t>(L.scan_blocks) * 4);  // cleared with misc
   take(L.off_nop, static_cast<size_t>(c.max_near) * 4);
   take(L.off_nhit, static_cast<size_t>(c.max_near) * 4);
   take(L.off_bsum, static_cast<size_t>(L.scan_blocks) * kScanSums * 4);
@@ -2038,6 +2081,7 @@ Workspace bind(const Layout &L, void *base) {
   w.near_op = reinterpret_cast<int *>(b + L.off_nop);
   w.near_hit = reinterpret_cast<int *>(b + L.off_nhit);
   w.block_sums = reinterpret_cast<int *>(b + L.off_bsum);
+  w.scan_flag = reinterpret_cast<int *>(b + L.off_sflag);
   w.block_bases = reinterpret_cast<int *>(b + L.off_bbase);
   w.seg_rank = reinterpret_cast<unsigned *>(b + L.off_srank);
   w.grp_base = reinterpret_cast<int *>(b + L.off_gbase);
@@ -2250,13 +2294,12 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     st = check_launch("bin_kernel");
     if (st != CUBLOX_OK) return st;
   }
-  if (L.scan_tpb == kScanSmallTiles) {
-    // Few blocks: each placing block finds its own bases.
-    tile_sums_kernel<<<dim3(L.scan_blocks), dim3(kScanSmallTiles), 0, hs>>>(b, w, L.scan_tpb);
-    st = check_launch("tile_sums_kernel");
-    if (st != CUBLOX_OK) return st;
+  const bool small = L.scan_tpb == kScanSmallTiles;
+  if (small) {
+    // Few blocks, all resident at once: one pass, each block publishing its
+    // sums and waiting for the others'.
     tile_place_kernel<<<dim3(L.scan_blocks), dim3(kScanSmallTiles), 0, hs>>>(b, w, L.scan_tpb,
-                                                                             1);
+                                                                             2);
     st = check_launch("tile_place_kernel");
   } else {
     tile_sums_kernel<<<dim3(L.scan_blocks), dim3(kScanThreads), 0, hs>>>(b, w, L.scan_tpb);
@@ -2277,7 +2320,8 @@ int launch_sequence(const Batch &b, const Layout &L, const Workspace &w,
     // whose dispatch costs time on large launches.
     const dim3 grid(walk_blocks,
                     static_cast<unsigned>(std::min((c.max_seg + kPlaceJ - 1) / kPlaceJ, kPlaceRows)));
-    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash);
+    place_kernel<<<grid, dim3(kWalkBlock), 0, hs>>>(b, w, lg_group, quad ? kWalkHash : kHash,
+                                                     small ? L.scan_blocks : 0);
     st = check_launch("place_kernel");
   }
   if (st != CUBLOX_OK) return st;
