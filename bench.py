@@ -40,13 +40,25 @@ UPDATE_SOURCES = ("cublox_amd/csrc/integrate.hip", "cublox_amd/csrc/raycast_comm
                   "cublox_amd/csrc/hip_common.hpp")
 
 
-def update_source_digest() -> str:
+def code_digest(texts) -> str:
+    """Digest of C++/HIP sources with comments and whitespace removed, so
+    that only a change to the code itself makes a recorded figure stale."""
     import hashlib
+    import re
     h = hashlib.sha256()
-    for rel in UPDATE_SOURCES:
-        with open(os.path.join(ROOT, rel), "rb") as fh:
-            h.update(fh.read())
+    for t in texts:
+        t = re.sub(r"/\*.*?\*/", " ", t, flags=re.S)
+        t = re.sub(r"//[^\n]*", " ", t)
+        h.update(" ".join(t.split()).encode())
     return h.hexdigest()[:16]
+
+
+def update_source_digest() -> str:
+    texts = []
+    for rel in UPDATE_SOURCES:
+        with open(os.path.join(ROOT, rel), encoding="utf-8") as fh:
+            texts.append(fh.read())
+    return code_digest(texts)
 
 
 def host_cpus() -> dict:
